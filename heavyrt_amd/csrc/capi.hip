@@ -1,0 +1,425 @@
+// capi.hip -- implementation of the C-ABI declared in include/hrt.h.
+//
+// Scene model (mirrors BVH2CommonRT, core/BVH2CommonRT.h:70-85): a list of
+// geometries, each with its own device BLAS, and a list of instances
+// (geometry id + column-major matrix).  CommitScene freezes the instance
+// list.  This build traces scenes with one instance whose matrix is the
+// identity; other instance sets are rejected with HRT_ERR_UNSUPPORTED.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/hrt.h"
+#include "hrt_internal.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+hrt_status fail(hrt_status st, const std::string& msg) {
+  g_last_error = msg;
+  return st;
+}
+
+hrt_status fail_hip(hipError_t e, const char* what) {
+  if (e == hipErrorOutOfMemory) return fail(HRT_ERR_OUT_OF_MEMORY, std::string(what) + ": out of device memory");
+  return fail(HRT_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+struct Instance {
+  uint32_t geom;
+  float fwd[16];
+  bool identity;
+};
+
+bool is_identity(const float* m) {
+  for (int c = 0; c < 4; c++)
+    for (int r = 0; r < 4; r++)
+      if (m[4 * c + r] != ((c == r) ? 1.0f : 0.0f)) return false;
+  return true;
+}
+
+}  // namespace
+
+struct hrt_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+};
+
+struct hrt_scene {
+  hrt_ctx* ctx = nullptr;
+  std::vector<hrt::Blas> geoms;
+  std::vector<Instance> instances;
+  bool committed = false;
+  bool keep_debug = false;
+  // staging for hrt_query_nearest_one
+  std::mutex one_mu;
+  hrt::Ray* one_ray = nullptr;
+  hrt_hit* one_hit = nullptr;
+};
+
+namespace {
+
+hrt_status select_blas(hrt_scene* s, const hrt::Blas** out, uint32_t* inst) {
+  if (!s) return fail(HRT_ERR_INVALID_ARG, "scene is null");
+  if (!s->committed) return fail(HRT_ERR_NOT_COMMITTED, "CommitScene has not been called since the last change");
+  if (s->instances.size() != 1 || !s->instances[0].identity)
+    return fail(HRT_ERR_UNSUPPORTED, "this build traces scenes with exactly one identity instance");
+  *out = &s->geoms[s->instances[0].geom];
+  *inst = 0;
+  return HRT_OK;
+}
+
+hrt_status trace_common(hrt_scene* s, const float* rays, size_t n, void* out, int format, int anyhit,
+                        uint32_t* stats, void* stream) {
+  const hrt::Blas* b = nullptr;
+  uint32_t inst = 0;
+  hrt_status st = select_blas(s, &b, &inst);
+  if (st != HRT_OK) return st;
+  if (n == 0) return HRT_OK;
+  if (!rays || !out) return fail(HRT_ERR_INVALID_ARG, "rays/out is null");
+  if (!anyhit && format != HRT_HIT_FULL && format != HRT_HIT_COMPACT)
+    return fail(HRT_ERR_INVALID_ARG, "unknown hit format");
+  hrt::TraceArgs a;
+  a.rays = reinterpret_cast<const hrt::Ray*>(rays);
+  a.n = n;
+  a.out = out;
+  a.format = format;
+  a.anyhit = anyhit;
+  a.stats = stats;
+  hipStream_t hs = stream ? static_cast<hipStream_t>(stream) : s->ctx->stream;
+  hipError_t e = hipSetDevice(s->ctx->device);
+  if (e == hipSuccess) e = hrt::trace_blas(*b, a, hs);
+  if (e != hipSuccess) return fail_hip(e, "trace");
+  return HRT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* hrt_last_error(void) { return g_last_error.c_str(); }
+
+hrt_status hrt_init(int device, hrt_ctx** out) {
+  if (!out) return fail(HRT_ERR_INVALID_ARG, "out is null");
+  *out = nullptr;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess) return fail_hip(e, "hipGetDeviceCount");
+  if (device < 0 || device >= count) return fail(HRT_ERR_INVALID_ARG, "no such device");
+  if ((e = hipSetDevice(device)) != hipSuccess) return fail_hip(e, "hipSetDevice");
+  hrt_ctx* c = new (std::nothrow) hrt_ctx();
+  if (!c) return fail(HRT_ERR_OUT_OF_MEMORY, "host allocation failed");
+  c->device = device;
+  if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) {
+    delete c;
+    return fail_hip(e, "hipStreamCreate");
+  }
+  *out = c;
+  return HRT_OK;
+}
+
+void hrt_shutdown(hrt_ctx* ctx) {
+  if (!ctx) return;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+  }
+  delete ctx;
+}
+
+void* hrt_ctx_stream(hrt_ctx* ctx) { return ctx ? static_cast<void*>(ctx->stream) : nullptr; }
+
+hrt_status hrt_scene_create(hrt_ctx* ctx, hrt_scene** out) {
+  if (!ctx || !out) return fail(HRT_ERR_INVALID_ARG, "ctx/out is null");
+  hrt_scene* s = new (std::nothrow) hrt_scene();
+  if (!s) return fail(HRT_ERR_OUT_OF_MEMORY, "host allocation failed");
+  s->ctx = ctx;
+  *out = s;
+  return HRT_OK;
+}
+
+hrt_status hrt_scene_set_keep_build_arrays(hrt_scene* s, int keep) {
+  if (!s) return fail(HRT_ERR_INVALID_ARG, "scene is null");
+  s->keep_debug = keep != 0;
+  return HRT_OK;
+}
+
+hrt_status hrt_scene_clear_geom(hrt_scene* s) {
+  if (!s) return fail(HRT_ERR_INVALID_ARG, "scene is null");
+  (void)hipSetDevice(s->ctx->device);
+  (void)hipStreamSynchronize(s->ctx->stream);
+  for (auto& g : s->geoms) hrt::blas_free(g);
+  s->geoms.clear();
+  s->instances.clear();  // ClearGeom also clears the scene (BVH2CommonRT_host.cpp:42)
+  s->committed = false;
+  return HRT_OK;
+}
+
+void hrt_scene_destroy(hrt_scene* s) {
+  if (!s) return;
+  hrt_scene_clear_geom(s);
+  if (s->one_ray) (void)hipFree(s->one_ray);
+  if (s->one_hit) (void)hipFree(s->one_hit);
+  delete s;
+}
+
+hrt_status hrt_scene_clear_instances(hrt_scene* s) {
+  if (!s) return fail(HRT_ERR_INVALID_ARG, "scene is null");
+  s->instances.clear();
+  s->committed = false;
+  return HRT_OK;
+}
+
+hrt_status hrt_add_triangles(hrt_scene* s, const float* vpos, size_t nVert, size_t vStrideBytes,
+                             const uint32_t* idx, size_t nIdx, int quality, uint32_t* outGeomId) {
+  (void)quality;  // every quality level builds the LBVH (the reference's cbvh_lbvh preset)
+  if (!s || !vpos || !idx) return fail(HRT_ERR_INVALID_ARG, "scene/vpos/idx is null");
+  if (vStrideBytes == 0) vStrideBytes = 12;
+  if (vStrideBytes % 4 != 0 || vStrideBytes < 12) return fail(HRT_ERR_INVALID_ARG, "vertex stride must be >= 12 and a multiple of 4");
+  const size_t ntris = nIdx / 3;
+  if (ntris == 0) return fail(HRT_ERR_INVALID_ARG, "need at least one triangle");
+  if (ntris > 0x00FFFFFFu) return fail(HRT_ERR_INVALID_ARG, "more than 16777215 triangles in one geometry (24-bit leaf start)");
+  if (nVert == 0 || nVert > 0xFFFFFFFFull) return fail(HRT_ERR_INVALID_ARG, "bad vertex count");
+  for (size_t i = 0; i < 3 * ntris; i++)
+    if (idx[i] >= nVert) return fail(HRT_ERR_INVALID_ARG, "index out of range");
+
+  // repack to float4 with w = 1 (BVH2CommonRT_host.cpp:79-84)
+  const size_t stride = vStrideBytes / 4;
+  std::vector<float> v4(4 * nVert);
+  for (size_t i = 0; i < nVert; i++) {
+    v4[4 * i + 0] = vpos[i * stride + 0];
+    v4[4 * i + 1] = vpos[i * stride + 1];
+    v4[4 * i + 2] = vpos[i * stride + 2];
+    v4[4 * i + 3] = 1.0f;
+  }
+  hipError_t e = hipSetDevice(s->ctx->device);
+  if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
+  hrt::Blas b;
+  if ((e = hrt::blas_alloc(b, uint32_t(nVert), uint32_t(ntris), s->keep_debug)) != hipSuccess) {
+    hrt::blas_free(b);
+    return fail_hip(e, "AddGeom_Triangles3f allocation");
+  }
+  hipStream_t hs = s->ctx->stream;
+  e = hipMemcpyAsync(b.verts, v4.data(), 16 * nVert, hipMemcpyHostToDevice, hs);
+  if (e == hipSuccess) e = hipMemcpyAsync(b.idx, idx, 12 * ntris, hipMemcpyHostToDevice, hs);
+  if (e == hipSuccess) e = hrt::blas_build(b, hs);
+  if (e != hipSuccess) {
+    hrt::blas_free(b);
+    return fail_hip(e, "LBVH build");
+  }
+  if (b.max_run > 127) {
+    hrt::blas_free(b);
+    return fail(HRT_ERR_LEAF_OVERFLOW,
+                "more than 127 triangles share one Morton code; the 7-bit leaf count (raytrace_common.h:40) cannot hold them");
+  }
+  s->geoms.push_back(b);
+  s->committed = false;
+  if (outGeomId) *outGeomId = uint32_t(s->geoms.size() - 1);
+  return HRT_OK;
+}
+
+hrt_status hrt_add_instance(hrt_scene* s, uint32_t geomId, const float colMajor16[16], uint32_t* outInstId) {
+  if (!s || !colMajor16) return fail(HRT_ERR_INVALID_ARG, "scene/matrix is null");
+  if (geomId >= s->geoms.size()) return fail(HRT_ERR_INVALID_ARG, "no such geometry");
+  Instance in;
+  in.geom = geomId;
+  std::memcpy(in.fwd, colMajor16, sizeof(in.fwd));
+  in.identity = is_identity(colMajor16);
+  s->instances.push_back(in);
+  s->committed = false;
+  if (outInstId) *outInstId = uint32_t(s->instances.size() - 1);
+  return HRT_OK;
+}
+
+hrt_status hrt_commit(hrt_scene* s, int quality, int layout) {
+  (void)quality;
+  if (!s) return fail(HRT_ERR_INVALID_ARG, "scene is null");
+  if (layout != HRT_LAYOUT_BVH2) return fail(HRT_ERR_UNSUPPORTED, "unknown layout");
+  s->committed = true;
+  return HRT_OK;
+}
+
+hrt_status hrt_trace_nearest(hrt_scene* s, const float* rays, size_t n, void* hits, int fmt, void* stream) {
+  return trace_common(s, rays, n, hits, fmt, 0, nullptr, stream);
+}
+
+hrt_status hrt_trace_any(hrt_scene* s, const float* rays, size_t n, uint8_t* out, void* stream) {
+  return trace_common(s, rays, n, out, HRT_HIT_FULL, 1, nullptr, stream);
+}
+
+hrt_status hrt_trace_nearest_stats(hrt_scene* s, const float* rays, size_t n, void* hits, int fmt, uint32_t* stats,
+                                   void* stream) {
+  if (!stats) return fail(HRT_ERR_INVALID_ARG, "stats is null");
+  return trace_common(s, rays, n, hits, fmt, 0, stats, stream);
+}
+
+hrt_status hrt_trace_any_stats(hrt_scene* s, const float* rays, size_t n, uint8_t* out, uint32_t* stats,
+                               void* stream) {
+  if (!stats) return fail(HRT_ERR_INVALID_ARG, "stats is null");
+  return trace_common(s, rays, n, out, HRT_HIT_FULL, 1, stats, stream);
+}
+
+hrt_status hrt_query_nearest_one(hrt_scene* s, const float posNear[4], const float dirFar[4], hrt_hit* out) {
+  if (!s || !posNear || !dirFar || !out) return fail(HRT_ERR_INVALID_ARG, "null argument");
+  std::lock_guard<std::mutex> lock(s->one_mu);
+  hipError_t e = hipSetDevice(s->ctx->device);
+  if (e == hipSuccess && !s->one_ray) e = hipMalloc(&s->one_ray, sizeof(hrt::Ray));
+  if (e == hipSuccess && !s->one_hit) e = hipMalloc(&s->one_hit, sizeof(hrt_hit));
+  if (e != hipSuccess) return fail_hip(e, "query staging");
+  float r[8];
+  std::memcpy(r, posNear, 16);
+  std::memcpy(r + 4, dirFar, 16);
+  hipStream_t hs = s->ctx->stream;
+  if ((e = hipMemcpyAsync(s->one_ray, r, sizeof(r), hipMemcpyHostToDevice, hs)) != hipSuccess)
+    return fail_hip(e, "query upload");
+  hrt_status st = trace_common(s, reinterpret_cast<const float*>(s->one_ray), 1, s->one_hit, HRT_HIT_FULL, 0, nullptr,
+                               hs);
+  if (st != HRT_OK) return st;
+  if ((e = hipMemcpyAsync(out, s->one_hit, sizeof(hrt_hit), hipMemcpyDeviceToHost, hs)) != hipSuccess)
+    return fail_hip(e, "query download");
+  if ((e = hipStreamSynchronize(hs)) != hipSuccess) return fail_hip(e, "query sync");
+  const Instance& in = s->instances[0];
+  if (out->primId != 0xFFFFFFFFu) {
+    out->instId = 0;
+    out->geomId = in.geom;
+  }
+  return HRT_OK;
+}
+
+hrt_status hrt_rtao_make_rays(hrt_scene* s, const float* primaryRays, const void* primaryHitsCompact, size_t nPixels,
+                              uint32_t raysPerHit, float tMax, uint64_t seed, float* outRays, void* stream) {
+  const hrt::Blas* b = nullptr;
+  uint32_t inst = 0;
+  hrt_status st = select_blas(s, &b, &inst);
+  if (st != HRT_OK) return st;
+  if (nPixels == 0 || raysPerHit == 0) return HRT_OK;
+  if (!primaryRays || !primaryHitsCompact || !outRays) return fail(HRT_ERR_INVALID_ARG, "null buffer");
+  hipStream_t hs = stream ? static_cast<hipStream_t>(stream) : s->ctx->stream;
+  hipError_t e = hipSetDevice(s->ctx->device);
+  if (e == hipSuccess)
+    e = hrt::rtao_make_rays(*b, reinterpret_cast<const hrt::Ray*>(primaryRays), primaryHitsCompact, nPixels,
+                            raysPerHit, tMax, seed, reinterpret_cast<hrt::Ray*>(outRays), hs);
+  if (e != hipSuccess) return fail_hip(e, "rtao rays");
+  return HRT_OK;
+}
+
+uint32_t hrt_geom_count(const hrt_scene* s) { return s ? uint32_t(s->geoms.size()) : 0u; }
+uint32_t hrt_instance_count(const hrt_scene* s) { return s ? uint32_t(s->instances.size()) : 0u; }
+
+hrt_status hrt_debug_dump(hrt_scene* s, uint32_t geomId, int what, void* host_out, size_t bytes) {
+  if (!s || !host_out) return fail(HRT_ERR_INVALID_ARG, "scene/out is null");
+  if (geomId >= s->geoms.size()) return fail(HRT_ERR_INVALID_ARG, "no such geometry");
+  const hrt::Blas& b = s->geoms[geomId];
+  const uint32_t N = b.ntris, L = b.leaves;
+  const uint32_t npairs = std::max<uint32_t>(L > 1 ? L - 1 : 1, 1);
+  auto need = [&](size_t n) -> hrt_status {
+    if (bytes < n) return fail(HRT_ERR_INVALID_ARG, "output buffer too small");
+    return HRT_OK;
+  };
+  hipError_t e = hipSetDevice(s->ctx->device);
+  if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
+  if (what == HRT_DUMP_SIZES) {
+    if (need(16)) return HRT_ERR_INVALID_ARG;
+    uint32_t v[4] = {N, L, b.max_run, npairs};
+    std::memcpy(host_out, v, 16);
+    return HRT_OK;
+  }
+  if (what == HRT_DUMP_SCENE_BOX) {
+    if (need(48)) return HRT_ERR_INVALID_ARG;
+    std::memcpy(host_out, b.box, 32);
+    std::memcpy(static_cast<char*>(host_out) + 32, b.zscale, 16);
+    return HRT_OK;
+  }
+  if (what == HRT_DUMP_PERM) {
+    if (need(4ull * N)) return HRT_ERR_INVALID_ARG;
+    e = hipMemcpy(host_out, b.perm, 4ull * N, hipMemcpyDeviceToHost);
+    return e == hipSuccess ? HRT_OK : fail_hip(e, "dump");
+  }
+  if (!b.keep_debug) return fail(HRT_ERR_INVALID_ARG, "build arrays were not kept (hrt_scene_set_keep_build_arrays)");
+  const void* src = nullptr;
+  size_t n = 0;
+  switch (what) {
+    case HRT_DUMP_CODES_INPUT: src = b.codes_in; n = 4ull * N; break;
+    case HRT_DUMP_CODES_SORTED: src = b.codes_sorted; n = 4ull * N; break;
+    case HRT_DUMP_IDS_SORTED: src = b.ids_sorted; n = 4ull * N; break;
+    case HRT_DUMP_RUN_START: src = b.run_start; n = 4ull * L; break;
+    case HRT_DUMP_UNIQUE_CODES: src = b.ucodes; n = 4ull * L; break;
+    case HRT_DUMP_LR_NODES: break;
+    default: return fail(HRT_ERR_INVALID_ARG, "unknown dump");
+  }
+  if (src) {
+    if (need(n)) return HRT_ERR_INVALID_ARG;
+    e = hipMemcpy(host_out, src, n, hipMemcpyDeviceToHost);
+    return e == hipSuccess ? HRT_OK : fail_hip(e, "dump");
+  }
+  // Reconstruct the reference's left/right node array (2L-1 nodes of 8 words)
+  // from the node pairs: internal node k = union of pair k's boxes, children
+  // from the Karras step; leaf m = the box its parent pair holds.
+  const size_t nodes = 2ull * L - 1;
+  if (need(nodes * 32)) return HRT_ERR_INVALID_ARG;
+  std::vector<uint32_t> pw(16ull * npairs), lr(2ull * npairs), rs(L);
+  if ((e = hipMemcpy(pw.data(), b.pairs, 64ull * npairs, hipMemcpyDeviceToHost)) != hipSuccess) return fail_hip(e, "dump");
+  if ((e = hipMemcpy(rs.data(), b.run_start, 4ull * L, hipMemcpyDeviceToHost)) != hipSuccess) return fail_hip(e, "dump");
+  if (L > 1 && (e = hipMemcpy(lr.data(), b.karras_lr, 8ull * (L - 1), hipMemcpyDeviceToHost)) != hipSuccess)
+    return fail_hip(e, "dump");
+  uint32_t* o = static_cast<uint32_t*>(host_out);
+  auto fmin_ = [](uint32_t a, uint32_t c) {
+    float x, y;
+    std::memcpy(&x, &a, 4);
+    std::memcpy(&y, &c, 4);
+    float r = (y < x) ? y : x;
+    uint32_t u;
+    std::memcpy(&u, &r, 4);
+    return u;
+  };
+  auto fmax_ = [](uint32_t a, uint32_t c) {
+    float x, y;
+    std::memcpy(&x, &a, 4);
+    std::memcpy(&y, &c, 4);
+    float r = (x < y) ? y : x;
+    uint32_t u;
+    std::memcpy(&u, &r, 4);
+    return u;
+  };
+  if (L == 1) {
+    for (int k = 0; k < 8; k++) o[k] = pw[k];
+    o[3] = pw[3];
+    o[7] = 0xFFFFFFFFu;
+    return HRT_OK;
+  }
+  for (uint32_t k = 0; k + 1 < L; k++) {
+    const uint32_t* p = &pw[16ull * k];
+    uint32_t* nd = &o[8ull * k];
+    for (int a = 0; a < 3; a++) {
+      nd[a] = fmin_(p[a], p[8 + a]);
+      nd[4 + a] = fmax_(p[4 + a], p[12 + a]);
+    }
+    nd[3] = lr[2 * k];
+    nd[7] = lr[2 * k + 1];
+    for (int side = 0; side < 2; side++) {
+      const uint32_t child = lr[2 * k + side];
+      if (child >= L - 1) {  // leaf
+        const uint32_t m = child - (L - 1);
+        uint32_t* lf = &o[8ull * child];
+        for (int a = 0; a < 3; a++) {
+          lf[a] = p[8 * side + a];
+          lf[4 + a] = p[8 * side + 4 + a];
+        }
+        lf[3] = p[8 * side + 3];
+        lf[7] = 0xFFFFFFFFu;
+        (void)m;
+      }
+    }
+  }
+  return HRT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,130 @@
+// hrt_device.h -- device-side formats and arithmetic shared by the builder and
+// the traversal kernels (gfx950, wave64).
+//
+// Every floating-point expression here mirrors the operation order of the
+// reference so that results are bit-identical to the CPU path when the file
+// is compiled with -ffp-contract=off (no fused multiply-add) and with the
+// default correctly rounded fp32 division:
+//   slab test           core/raytrace_common.h:67-80
+//   safe reciprocal     core/raytrace_common.h:57-65
+//   Moller-Trumbore     core/BVH2CommonRT.cpp:26-37 (1e-6 slack, :37)
+//   leaf packing        core/raytrace_common.h:38-55
+//
+// min/max use v_min_f32/v_max_f32.  For finite rays and boxes they return
+// exactly what std::min/std::max return, except possibly the sign of a zero;
+// the slab results are only ever compared (<=, <, >=), where +0 == -0, so the
+// traversal decisions and the hits are unaffected.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hrt {
+
+constexpr uint32_t kStartMask = 0x00FFFFFFu;
+constexpr uint32_t kSizeMask = 0x7F000000u;
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr uint32_t kInvalid = 0xFFFFFFFFu;
+// Deepest possible BLAS path: the Karras tree over unique 30-bit codes has at
+// most 30 internal levels, so at most 30 deferred children are ever pending.
+constexpr int kStackDepth = 32;
+
+__host__ __device__ inline uint32_t pack_leaf(uint32_t start, uint32_t count) {
+  return kLeafBit | ((count << 24) & kSizeMask) | (start & kStartMask);
+}
+__host__ __device__ inline uint32_t leaf_start(uint32_t v) { return v & kStartMask; }
+__host__ __device__ inline uint32_t leaf_count(uint32_t v) { return (v & kSizeMask) >> 24; }
+
+// ---------------------------------------------------------------------------
+// HBM layouts
+// ---------------------------------------------------------------------------
+// BVH2 node pair: the two children of one internal Karras node, 64 bytes,
+// fetched together on every visit (four 16-byte loads).  Pair k holds the
+// children of internal node k, so the pair array is indexed by the Karras
+// internal-node id and is laid out in Morton order.  A child reference is
+// either an internal node id (pair index) or a packed leaf (kLeafBit set).
+struct __attribute__((aligned(64))) NodePair {
+  float4 lmin;  // xyz = left child box min,  w = bits of left child ref
+  float4 lmax;  // xyz = left child box max,  w = unused (0)
+  float4 rmin;  // xyz = right child box min, w = bits of right child ref
+  float4 rmax;  // xyz = right child box max, w = unused (0)
+};
+
+// Triangle in leaf order, 48 bytes: vertex A, the edges B-A and C-A as the
+// reference computes them at test time (BVH2CommonRT.cpp:26-27; the
+// subtraction is exact-once either way, so storing it is bit-identical), and
+// the original triangle id in a.w.
+struct __attribute__((aligned(16))) TriPre {
+  float4 a;   // xyz = A, w = bits(original triangle id)
+  float4 e1;  // xyz = B - A
+  float4 e2;  // xyz = C - A
+};
+
+struct Ray {  // 32-byte ray record: posAndNear, dirAndFar (CrossRT.h:143)
+  float4 pos_near;
+  float4 dir_far;
+};
+
+// ---------------------------------------------------------------------------
+// Arithmetic
+// ---------------------------------------------------------------------------
+__device__ inline float3 safe_inverse(float3 d) {
+  const float eps = 1.0e-36f;
+  float3 r;
+  r.x = 1.0f / (fabsf(d.x) > eps ? d.x : copysignf(eps, d.x));
+  r.y = 1.0f / (fabsf(d.y) > eps ? d.y : copysignf(eps, d.y));
+  r.z = 1.0f / (fabsf(d.z) > eps ? d.z : copysignf(eps, d.z));
+  return r;
+}
+
+__device__ inline float2 slab(float3 o, float3 inv, float bminx, float bminy, float bminz, float bmaxx,
+                              float bmaxy, float bmaxz) {
+  const float lo = inv.x * (bminx - o.x);
+  const float hi = inv.x * (bmaxx - o.x);
+  const float lo1 = inv.y * (bminy - o.y);
+  const float hi1 = inv.y * (bmaxy - o.y);
+  const float lo2 = inv.z * (bminz - o.z);
+  const float hi2 = inv.z * (bmaxz - o.z);
+  const float tmin = fmaxf(fminf(lo, hi), fminf(lo1, hi1));
+  const float tmax = fminf(fmaxf(lo, hi), fmaxf(lo1, hi1));
+  return make_float2(fmaxf(tmin, fminf(lo2, hi2)), fminf(tmax, fmaxf(lo2, hi2)));
+}
+
+struct HitRec {
+  float t;
+  uint32_t prim;  // original triangle id, kInvalid on miss
+  float u, v;
+};
+
+// Moller-Trumbore, operation order of BVH2CommonRT.cpp:26-37.
+__device__ inline bool mt_test(float3 o, float3 d, float tnear, const TriPre& tri, HitRec& h) {
+  const float3 A = make_float3(tri.a.x, tri.a.y, tri.a.z);
+  const float3 e1 = make_float3(tri.e1.x, tri.e1.y, tri.e1.z);
+  const float3 e2 = make_float3(tri.e2.x, tri.e2.y, tri.e2.z);
+  // pvec = cross(dir, e2)
+  const float px = d.y * e2.z - d.z * e2.y;
+  const float py = d.z * e2.x - d.x * e2.z;
+  const float pz = d.x * e2.y - d.y * e2.x;
+  // tvec = pos - A
+  const float tx = o.x - A.x;
+  const float ty = o.y - A.y;
+  const float tz = o.z - A.z;
+  // qvec = cross(tvec, e1)
+  const float qx = ty * e1.z - tz * e1.y;
+  const float qy = tz * e1.x - tx * e1.z;
+  const float qz = tx * e1.y - ty * e1.x;
+  const float invDet = 1.0f / (e1.x * px + e1.y * py + e1.z * pz);
+  const float v = (tx * px + ty * py + tz * pz) * invDet;
+  const float u = (qx * d.x + qy * d.y + qz * d.z) * invDet;
+  const float t = (e2.x * qx + e2.y * qy + e2.z * qz) * invDet;
+  if (v >= -1e-6f && u >= -1e-6f && (u + v <= 1.0f + 1e-6f) && t > tnear && t < h.t) {
+    h.t = t;
+    h.prim = __float_as_uint(tri.a.w);
+    h.u = u;
+    h.v = v;
+    return true;
+  }
+  return false;
+}
+
+}  // namespace hrt

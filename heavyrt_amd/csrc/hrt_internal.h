@@ -1,0 +1,69 @@
+// hrt_internal.h -- host-side declarations shared by the engine's translation
+// units (builder, traversal, C-ABI).  Not part of the public interface.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "hrt_device.h"
+
+namespace hrt {
+
+// Device buffers of one bottom-level structure (one geometry).
+struct Blas {
+  uint32_t ntris = 0;
+  uint32_t nverts = 0;
+  uint32_t leaves = 0;        // unique Morton codes (valid after build)
+  uint32_t max_run = 0;       // longest run of equal codes (leaf size)
+  float box[8] = {0};         // scene box over vertices: min xyzw, max xyzw
+  float zscale[4] = {0};      // Morton scale
+  float4* verts = nullptr;    // float4 positions, w = 1
+  uint32_t* idx = nullptr;    // input index buffer (3 * ntris)
+  NodePair* pairs = nullptr;  // max(leaves - 1, 1) pairs
+  TriPre* tris = nullptr;     // ntris, leaf order
+  uint32_t* perm = nullptr;   // leaf-order position -> original triangle id
+  // parity hooks (kept only when requested)
+  uint32_t* codes_in = nullptr;    // Morton code per input triangle
+  uint32_t* codes_sorted = nullptr;
+  uint32_t* ids_sorted = nullptr;
+  uint32_t* run_start = nullptr;   // leaves
+  uint32_t* ucodes = nullptr;      // leaves
+  uint32_t* karras_lr = nullptr;   // 2 * (leaves - 1): children in the reference's left/right numbering
+  bool keep_debug = false;
+};
+
+// Scratch for one build, sized for up to `cap` triangles; reused across builds.
+struct BuildScratch {
+  size_t cap = 0;
+  size_t bytes = 0;
+  void* base = nullptr;
+};
+
+hipError_t blas_alloc(Blas& b, uint32_t nverts, uint32_t ntris, bool keep_debug);
+void blas_free(Blas& b);
+// Builds the LBVH of b.verts / b.idx (already on the device) on `stream`;
+// synchronises the stream before returning (leaves/max_run/box are host
+// values afterwards).
+hipError_t blas_build(Blas& b, hipStream_t stream);
+
+// Traversal ------------------------------------------------------------------
+enum HitFormat : int { kHitFull = 0, kHitCompact = 1 };
+
+struct TraceArgs {
+  const Ray* rays;
+  uint64_t n;
+  void* out;      // CRT_Hit-compatible records, compact (t, prim) pairs or bytes
+  int format;     // HitFormat for nearest-hit
+  int anyhit;     // 1: boolean occlusion query
+  uint32_t* stats;  // optional per-ray [pairs visited, triangles tested]
+};
+hipError_t trace_blas(const Blas& b, const TraceArgs& a, hipStream_t stream);
+
+// Ambient-occlusion rays from primary hits (compact records), spp per pixel.
+hipError_t rtao_make_rays(const Blas& b, const Ray* prim_rays, const void* prim_hits_compact, uint64_t npix,
+                          uint32_t spp, float tmax, uint64_t seed, Ray* out, hipStream_t stream);
+
+}  // namespace hrt

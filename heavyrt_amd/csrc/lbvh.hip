@@ -1,0 +1,593 @@
+// lbvh.hip -- LBVH builder for gfx950.
+//
+// Stages (reference: core/builders/lbvh.cpp:732-827, BuildFromTriangles):
+//   1. scene box over all vertices            lbvh.cpp:228-239, 746
+//   2. 30-bit Morton codes per triangle       lbvh.cpp:25-51, 259-287
+//   3. stable LSD radix sort of (code, id)    lbvh.cpp:755 (order fixed to (code, id))
+//   4. unique-code runs (flag, scan, compact) lbvh.cpp:289-319, 762-767
+//   5. Karras 2012 hierarchy                  lbvh.cpp:386-441
+//   6. leaves + atomic bottom-up refit        lbvh.cpp:350-384, 443-489
+//   7. triangles re-laid out in leaf order    BVH2CommonRT_host.cpp:45-64
+//
+// Every integer result (codes, order, runs, children) is bit-identical to the
+// reference's; boxes are min/max unions, exact in any order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+
+#include "hrt_internal.h"
+
+namespace hrt {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr float kLbvhInf = 1e10f;        // lbvh.cpp:53
+constexpr float kLbvhEpsBias = 1e-4f;    // lbvh.cpp:54
+
+// Radix sort tile: 4 waves x 16 rounds x 64 lanes.
+constexpr int kSortWaves = 4;
+constexpr int kSortRounds = 16;
+constexpr int kSortTile = kSortWaves * kSortRounds * 64;  // 4096 keys
+constexpr int kDigits = 256;
+
+// Scan tile: 256 threads x 8 items.
+constexpr int kScanItems = 8;
+constexpr int kScanTile = kBlock * kScanItems;
+
+#define HRT_CHECK(x)                         \
+  do {                                       \
+    hipError_t e_ = (x);                     \
+    if (e_ != hipSuccess) return e_;         \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// 1. scene box
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_box_partial(const float4* __restrict__ v, uint32_t n,
+                                                        float4* __restrict__ part) {
+  __shared__ float4 smin[kBlock];
+  __shared__ float4 smax[kBlock];
+  float4 lo = make_float4(kLbvhInf, kLbvhInf, kLbvhInf, kLbvhInf);
+  float4 hi = make_float4(-kLbvhInf, -kLbvhInf, -kLbvhInf, -kLbvhInf);
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
+    const float4 p = v[i];
+    lo = make_float4(fminf(lo.x, p.x), fminf(lo.y, p.y), fminf(lo.z, p.z), fminf(lo.w, p.w));
+    hi = make_float4(fmaxf(hi.x, p.x), fmaxf(hi.y, p.y), fmaxf(hi.z, p.z), fmaxf(hi.w, p.w));
+  }
+  smin[threadIdx.x] = lo;
+  smax[threadIdx.x] = hi;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      const float4 a = smin[threadIdx.x + s], b = smax[threadIdx.x + s];
+      float4& l = smin[threadIdx.x];
+      float4& h = smax[threadIdx.x];
+      l = make_float4(fminf(l.x, a.x), fminf(l.y, a.y), fminf(l.z, a.z), fminf(l.w, a.w));
+      h = make_float4(fmaxf(h.x, b.x), fmaxf(h.y, b.y), fmaxf(h.z, b.z), fmaxf(h.w, b.w));
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    part[2 * blockIdx.x + 0] = smin[0];
+    part[2 * blockIdx.x + 1] = smax[0];
+  }
+}
+
+// Reduces the partial boxes and derives the Morton scale
+// (ZIndexMultCoeff(bboxMin - 1e-4, bboxMax + 1e-4), lbvh.cpp:41-51, 263).
+__global__ __launch_bounds__(kBlock) void k_box_final(const float4* __restrict__ part, int nparts,
+                                                      float* __restrict__ box8, float* __restrict__ zscale4) {
+  __shared__ float4 smin[kBlock];
+  __shared__ float4 smax[kBlock];
+  float4 lo = make_float4(kLbvhInf, kLbvhInf, kLbvhInf, kLbvhInf);
+  float4 hi = make_float4(-kLbvhInf, -kLbvhInf, -kLbvhInf, -kLbvhInf);
+  for (int i = threadIdx.x; i < nparts; i += kBlock) {
+    const float4 a = part[2 * i], b = part[2 * i + 1];
+    lo = make_float4(fminf(lo.x, a.x), fminf(lo.y, a.y), fminf(lo.z, a.z), fminf(lo.w, a.w));
+    hi = make_float4(fmaxf(hi.x, b.x), fmaxf(hi.y, b.y), fmaxf(hi.z, b.z), fmaxf(hi.w, b.w));
+  }
+  smin[threadIdx.x] = lo;
+  smax[threadIdx.x] = hi;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      const float4 a = smin[threadIdx.x + s], b = smax[threadIdx.x + s];
+      float4& l = smin[threadIdx.x];
+      float4& h = smax[threadIdx.x];
+      l = make_float4(fminf(l.x, a.x), fminf(l.y, a.y), fminf(l.z, a.z), fminf(l.w, a.w));
+      h = make_float4(fmaxf(h.x, b.x), fmaxf(h.y, b.y), fmaxf(h.z, b.z), fmaxf(h.w, b.w));
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const float4 l = smin[0], h = smax[0];
+    box8[0] = l.x; box8[1] = l.y; box8[2] = l.z; box8[3] = l.w;
+    box8[4] = h.x; box8[5] = h.y; box8[6] = h.z; box8[7] = h.w;
+    const float ex[4] = {(h.x + kLbvhEpsBias) - (l.x - kLbvhEpsBias), (h.y + kLbvhEpsBias) - (l.y - kLbvhEpsBias),
+                         (h.z + kLbvhEpsBias) - (l.z - kLbvhEpsBias), (h.w + kLbvhEpsBias) - (l.w - kLbvhEpsBias)};
+    for (int k = 0; k < 4; k++) {
+      float inv = 1.0f / ex[k];
+      if (k < 3 && fabsf(ex[k]) < 1e-20f) inv = 0.0f;
+      zscale4[k] = 1024.0f * inv;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 2. Morton codes
+// ---------------------------------------------------------------------------
+__device__ inline uint32_t spread_bits(int x, int offset) {  // lbvh.cpp:25-34
+  x = (x | (x << 10)) & 0x000F801F;
+  x = (x | (x << 4)) & 0x00E181C3;
+  x = (x | (x << 2)) & 0x03248649;
+  x = (x | (x << 2)) & 0x09249249;
+  return uint32_t(x) << offset;
+}
+
+__global__ __launch_bounds__(kBlock) void k_morton(const float4* __restrict__ v, const uint32_t* __restrict__ idx,
+                                                   uint32_t ntris, const float* __restrict__ box8,
+                                                   const float* __restrict__ zscale4, uint32_t* __restrict__ codes,
+                                                   uint32_t* __restrict__ ids) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= ntris) return;
+  const float4 v1 = v[idx[3 * i + 0]];
+  const float4 v2 = v[idx[3 * i + 1]];
+  const float4 v3 = v[idx[3 * i + 2]];
+  float cx, cy, cz;
+  if (ntris < 256) {  // lbvh.cpp:279: centroid for small meshes, box centre otherwise
+    cx = 0.33333333f * ((v1.x + v2.x) + v3.x);
+    cy = 0.33333333f * ((v1.y + v2.y) + v3.y);
+    cz = 0.33333333f * ((v1.z + v2.z) + v3.z);
+  } else {
+    cx = (fminf(v1.x, fminf(v2.x, v3.x)) + fmaxf(v1.x, fmaxf(v2.x, v3.x))) * 0.5f;
+    cy = (fminf(v1.y, fminf(v2.y, v3.y)) + fmaxf(v1.y, fmaxf(v2.y, v3.y))) * 0.5f;
+    cz = (fminf(v1.z, fminf(v2.z, v3.z)) + fmaxf(v1.z, fmaxf(v2.z, v3.z))) * 0.5f;
+  }
+  const float sx = zscale4[0] * (cx - box8[0]);
+  const float sy = zscale4[1] * (cy - box8[1]);
+  const float sz = zscale4[2] * (cz - box8[2]);
+  // float -> int truncates toward zero (v_cvt_i32_f32 saturates, NaN -> 0)
+  const int ix = int(sx);
+  const int iy = int(sy);
+  const int iz = int(sz);
+  codes[i] = spread_bits(ix, 0) | spread_bits(iy, 1) | spread_bits(iz, 2);
+  ids[i] = i;
+}
+
+// ---------------------------------------------------------------------------
+// 3. stable LSD radix sort, 8-bit digits
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n, int shift,
+                                                       uint32_t* __restrict__ hist, uint32_t ntiles) {
+  __shared__ uint32_t cnt[kDigits];
+  cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t base = uint64_t(blockIdx.x) * kSortTile;
+#pragma unroll 4
+  for (int k = 0; k < kSortTile / kBlock; k++) {
+    const uint64_t i = base + uint64_t(k) * kBlock + threadIdx.x;
+    if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 0xFF], 1u);
+  }
+  __syncthreads();
+  hist[uint64_t(threadIdx.x) * ntiles + blockIdx.x] = cnt[threadIdx.x];
+}
+
+// Scatter with stable in-tile ranks.  Wave w owns keys [w*1024, w*1024+1024)
+// of the tile and walks them 64 at a time in index order; lanes holding the
+// same digit find each other with eight ballots, so the rank of a key among
+// equal digits follows its index -- the sort is stable, and sorting
+// (code, id) pairs whose ids start ascending yields the (code, id) order.
+__global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t* __restrict__ kin,
+                                                          const uint32_t* __restrict__ vin,
+                                                          uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
+                                                          uint32_t n, int shift,
+                                                          const uint32_t* __restrict__ hist_scan, uint32_t ntiles) {
+  __shared__ uint32_t wcnt[kSortWaves][kDigits];
+  const int lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6;
+  for (int k = threadIdx.x; k < kSortWaves * kDigits; k += kBlock) (&wcnt[0][0])[k] = 0;
+  __syncthreads();
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+  const uint64_t base = uint64_t(blockIdx.x) * kSortTile + uint64_t(w) * (kSortRounds * 64);
+  uint32_t key[kSortRounds], val[kSortRounds], pos[kSortRounds];
+#pragma unroll
+  for (int r = 0; r < kSortRounds; r++) {
+    const uint64_t i = base + uint64_t(r) * 64 + lane;
+    const bool valid = i < n;
+    key[r] = valid ? kin[i] : 0u;
+    val[r] = valid ? vin[i] : 0u;
+    const uint32_t digit = (key[r] >> shift) & 0xFF;
+    uint64_t peers = __ballot(valid);
+#pragma unroll
+    for (int b = 0; b < 8; b++) {
+      const bool bit = (digit >> b) & 1u;
+      const uint64_t m = __ballot(bit);
+      peers &= bit ? m : ~m;
+    }
+    const uint32_t rank = __popcll(peers & lt_mask);
+    const uint32_t cnt = __popcll(peers);
+    const uint32_t before = wcnt[w][digit];
+    __builtin_amdgcn_wave_barrier();
+    if (valid && rank == 0) wcnt[w][digit] = before + cnt;
+    __builtin_amdgcn_wave_barrier();
+    pos[r] = before + rank;
+  }
+  __syncthreads();
+  {
+    const int d = threadIdx.x;  // kBlock == kDigits
+    uint32_t run = hist_scan[uint64_t(d) * ntiles + blockIdx.x];
+#pragma unroll
+    for (int ww = 0; ww < kSortWaves; ww++) {
+      const uint32_t c = wcnt[ww][d];
+      wcnt[ww][d] = run;
+      run += c;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kSortRounds; r++) {
+    const uint64_t i = base + uint64_t(r) * 64 + lane;
+    if (i < n) {
+      const uint32_t digit = (key[r] >> shift) & 0xFF;
+      const uint32_t p = wcnt[w][digit] + pos[r];
+      kout[p] = key[r];
+      vout[p] = val[r];
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan of u32 (three phases, recursive over block sums)
+// ---------------------------------------------------------------------------
+__device__ inline uint32_t block_exclusive_scan(uint32_t x, uint32_t* total) {
+  __shared__ uint32_t s[kBlock];
+  s[threadIdx.x] = x;
+  __syncthreads();
+  for (int off = 1; off < kBlock; off <<= 1) {
+    const uint32_t y = (threadIdx.x >= off) ? s[threadIdx.x - off] : 0u;
+    __syncthreads();
+    s[threadIdx.x] += y;
+    __syncthreads();
+  }
+  const uint32_t incl = s[threadIdx.x];
+  *total = s[kBlock - 1];
+  __syncthreads();
+  return incl - x;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_tile(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
+                                                      uint64_t n, uint32_t* __restrict__ sums) {
+  const uint64_t base = uint64_t(blockIdx.x) * kScanTile + uint64_t(threadIdx.x) * kScanItems;
+  uint32_t v[kScanItems];
+  uint32_t acc = 0;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    v[k] = (base + k < n) ? in[base + k] : 0u;
+    acc += v[k];
+  }
+  uint32_t total;
+  uint32_t run = block_exclusive_scan(acc, &total);
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+  if (threadIdx.x == 0 && sums) sums[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scan_add(uint32_t* __restrict__ out, uint64_t n,
+                                                     const uint32_t* __restrict__ sums) {
+  const uint64_t base = uint64_t(blockIdx.x) * kScanTile;
+  const uint32_t add = sums[blockIdx.x];
+  for (int k = threadIdx.x; k < kScanTile; k += kBlock)
+    if (base + k < n) out[base + k] += add;
+}
+
+size_t scan_temp_words(uint64_t n) {
+  size_t words = 0;
+  while (n > uint64_t(kScanTile)) {
+    n = (n + kScanTile - 1) / kScanTile;
+    words += n;
+  }
+  return words + 1;
+}
+
+hipError_t exclusive_scan(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* temp, hipStream_t s) {
+  const uint64_t tiles = (n + kScanTile - 1) / kScanTile;
+  if (tiles <= 1) {
+    k_scan_tile<<<1, kBlock, 0, s>>>(in, out, n, nullptr);
+    return hipGetLastError();
+  }
+  uint32_t* sums = temp;
+  k_scan_tile<<<dim3(uint32_t(tiles)), kBlock, 0, s>>>(in, out, n, sums);
+  HRT_CHECK(hipGetLastError());
+  HRT_CHECK(exclusive_scan(sums, sums, tiles, temp + tiles, s));
+  k_scan_add<<<dim3(uint32_t(tiles)), kBlock, 0, s>>>(out, n, sums);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// 4. unique-code runs
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_run_flags(const uint32_t* __restrict__ codes, uint32_t n,
+                                                      uint32_t* __restrict__ flags) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  flags[i] = (i == 0 || codes[i - 1] != codes[i]) ? 1u : 0u;
+}
+
+// meta[0] = leaves, meta[1] = max run length
+__global__ __launch_bounds__(kBlock) void k_run_compact(const uint32_t* __restrict__ codes,
+                                                        const uint32_t* __restrict__ flags,
+                                                        const uint32_t* __restrict__ pos, uint32_t n,
+                                                        uint32_t* __restrict__ run_start,
+                                                        uint32_t* __restrict__ ucodes, uint32_t* __restrict__ meta) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  if (flags[i]) {
+    run_start[pos[i]] = i;
+    ucodes[pos[i]] = codes[i];
+  }
+  if (i == n - 1) meta[0] = pos[i] + flags[i];
+}
+
+// ---------------------------------------------------------------------------
+// 5. Karras 2012
+// ---------------------------------------------------------------------------
+__device__ inline int kdelta(const uint32_t* __restrict__ c, int L, int i, int j) {  // lbvh.cpp:386-403
+  if (j < 0 || j >= L) return -1;
+  const uint32_t x = c[i] ^ c[j];
+  return x == 0 ? 32 : int(__builtin_clz(x));
+}
+
+__device__ inline uint32_t leaf_ref(const uint32_t* __restrict__ run_start, uint32_t L, uint32_t ntris, uint32_t m) {
+  const uint32_t s = run_start[m];
+  const uint32_t e = (m + 1 < L) ? run_start[m + 1] : ntris;
+  return pack_leaf(s, e - s);
+}
+
+__global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* __restrict__ ucodes,
+                                                   const uint32_t* __restrict__ run_start,
+                                                   const uint32_t* __restrict__ meta, uint32_t ntris,
+                                                   NodePair* __restrict__ pairs, uint32_t* __restrict__ parent_int,
+                                                   uint32_t* __restrict__ parent_leaf,
+                                                   uint32_t* __restrict__ karras_lr) {
+  const int L = int(meta[0]);
+  const int i = int(blockIdx.x * kBlock + threadIdx.x);
+  if (i >= L - 1) return;
+  const int dr = kdelta(ucodes, L, i, i + 1) - kdelta(ucodes, L, i, i - 1);
+  const int d = (dr > 0) - (dr < 0);
+  const int dmin = kdelta(ucodes, L, i, i - d);
+  int lmax = 2;
+  while (kdelta(ucodes, L, i, i + lmax * d) > dmin) lmax <<= 1;
+  int l = 0;
+  for (int t = lmax >> 1; t > 0; t >>= 1)
+    if (kdelta(ucodes, L, i, i + (l + t) * d) > dmin) l += t;
+  const int j = i + l * d;
+  const int dnode = kdelta(ucodes, L, i, j);
+  int s = 0;
+  for (int t = (l + 1) >> 1; t > 0; t = (t == 1 ? 0 : (t + 1) >> 1))
+    if (kdelta(ucodes, L, i, i + (s + t) * d) > dnode) s += t;
+  const int gamma = i + s * d + min(d, 0);
+  const bool left_leaf = (min(i, j) == gamma);
+  const bool right_leaf = (max(i, j) == gamma + 1);
+  uint32_t* pw = reinterpret_cast<uint32_t*>(pairs + i);
+  uint32_t lref, rref;
+  if (left_leaf) {
+    lref = leaf_ref(run_start, uint32_t(L), ntris, uint32_t(gamma));
+    parent_leaf[gamma] = uint32_t(i) << 1;
+  } else {
+    lref = uint32_t(gamma);
+    parent_int[gamma] = uint32_t(i) << 1;
+  }
+  if (right_leaf) {
+    rref = leaf_ref(run_start, uint32_t(L), ntris, uint32_t(gamma + 1));
+    parent_leaf[gamma + 1] = (uint32_t(i) << 1) | 1u;
+  } else {
+    rref = uint32_t(gamma + 1);
+    parent_int[gamma + 1] = (uint32_t(i) << 1) | 1u;
+  }
+  pw[3] = lref;
+  pw[7] = 0u;
+  pw[11] = rref;
+  pw[15] = 0u;
+  if (karras_lr) {
+    karras_lr[2 * i + 0] = left_leaf ? uint32_t(L - 1 + gamma) : uint32_t(gamma);
+    karras_lr[2 * i + 1] = right_leaf ? uint32_t(L - 1 + gamma + 1) : uint32_t(gamma + 1);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 6. leaves + bottom-up refit
+// ---------------------------------------------------------------------------
+__device__ inline void store_box(NodePair* pairs, uint32_t p, uint32_t side, float3 lo, float3 hi) {
+  uint32_t* pw = reinterpret_cast<uint32_t*>(pairs + p) + side * 8;
+  pw[0] = __float_as_uint(lo.x);
+  pw[1] = __float_as_uint(lo.y);
+  pw[2] = __float_as_uint(lo.z);
+  pw[4] = __float_as_uint(hi.x);
+  pw[5] = __float_as_uint(hi.y);
+  pw[6] = __float_as_uint(hi.z);
+}
+
+__global__ __launch_bounds__(kBlock) void k_leaves_refit(
+    const float4* __restrict__ v, const uint32_t* __restrict__ idx, const uint32_t* __restrict__ ids_sorted,
+    const uint32_t* __restrict__ run_start, uint32_t* __restrict__ meta, uint32_t ntris, NodePair* pairs,
+    const uint32_t* __restrict__ parent_int, const uint32_t* __restrict__ parent_leaf, uint32_t* arrive) {
+  const uint32_t L = meta[0];
+  const uint32_t m = blockIdx.x * kBlock + threadIdx.x;
+  if (m >= L) return;
+  const uint32_t s = run_start[m];
+  const uint32_t e = (m + 1 < L) ? run_start[m + 1] : ntris;
+  float3 lo = make_float3(kLbvhInf, kLbvhInf, kLbvhInf);
+  float3 hi = make_float3(-kLbvhInf, -kLbvhInf, -kLbvhInf);
+  for (uint32_t j = s; j < e; j++) {
+    const uint32_t t = ids_sorted[j];
+    const float4 a = v[idx[3 * t + 0]], b = v[idx[3 * t + 1]], c = v[idx[3 * t + 2]];
+    lo = make_float3(fminf(lo.x, fminf(a.x, fminf(b.x, c.x))), fminf(lo.y, fminf(a.y, fminf(b.y, c.y))),
+                     fminf(lo.z, fminf(a.z, fminf(b.z, c.z))));
+    hi = make_float3(fmaxf(hi.x, fmaxf(a.x, fmaxf(b.x, c.x))), fmaxf(hi.y, fmaxf(a.y, fmaxf(b.y, c.y))),
+                     fmaxf(hi.z, fmaxf(a.z, fmaxf(b.z, c.z))));
+  }
+  atomicMax(&meta[1], e - s);
+  if (L == 1) {
+    // Root is a leaf: pair 0 = (root leaf, empty dummy), as the reference's
+    // peach layout stores it (lbvh.cpp:497-504, cbvh.h:28-36).
+    store_box(pairs, 0, 0, lo, hi);
+    store_box(pairs, 0, 1, make_float3(0.f, 0.f, 0.f), make_float3(0.f, 0.f, 0.f));
+    uint32_t* pw = reinterpret_cast<uint32_t*>(pairs);
+    pw[3] = pack_leaf(s, e - s);
+    pw[7] = 0u;
+    pw[11] = 0x80000000u;
+    pw[15] = 0u;
+    return;
+  }
+  uint32_t pl = parent_leaf[m];
+  store_box(pairs, pl >> 1, pl & 1u, lo, hi);
+  uint32_t p = pl >> 1;
+  while (true) {
+    // release our box, then count arrivals; the second child to arrive owns the parent
+    const uint32_t old = __hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == 0) return;
+    const uint32_t* pw = reinterpret_cast<const uint32_t*>(pairs + p);
+    const float3 l0 = make_float3(__uint_as_float(pw[0]), __uint_as_float(pw[1]), __uint_as_float(pw[2]));
+    const float3 h0 = make_float3(__uint_as_float(pw[4]), __uint_as_float(pw[5]), __uint_as_float(pw[6]));
+    const float3 l1 = make_float3(__uint_as_float(pw[8]), __uint_as_float(pw[9]), __uint_as_float(pw[10]));
+    const float3 h1 = make_float3(__uint_as_float(pw[12]), __uint_as_float(pw[13]), __uint_as_float(pw[14]));
+    if (p == 0) return;  // root box is not stored: traversal starts at the root's children
+    const float3 ul = make_float3(fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z));
+    const float3 uh = make_float3(fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z));
+    const uint32_t pp = parent_int[p];
+    store_box(pairs, pp >> 1, pp & 1u, ul, uh);
+    p = pp >> 1;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 7. triangles in leaf order
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_tris(const float4* __restrict__ v, const uint32_t* __restrict__ idx,
+                                                 const uint32_t* __restrict__ ids_sorted, uint32_t ntris,
+                                                 TriPre* __restrict__ tris, uint32_t* __restrict__ perm) {
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= ntris) return;
+  const uint32_t t = ids_sorted[j];
+  const float4 a = v[idx[3 * t + 0]], b = v[idx[3 * t + 1]], c = v[idx[3 * t + 2]];
+  TriPre r;
+  r.a = make_float4(a.x, a.y, a.z, __uint_as_float(t));
+  r.e1 = make_float4(b.x - a.x, b.y - a.y, b.z - a.z, 0.0f);
+  r.e2 = make_float4(c.x - a.x, c.y - a.y, c.z - a.z, 0.0f);
+  tris[j] = r;
+  perm[j] = t;
+}
+
+inline uint32_t blocks_for(uint64_t n) { return uint32_t((n + kBlock - 1) / kBlock); }
+
+}  // namespace
+
+hipError_t blas_alloc(Blas& b, uint32_t nverts, uint32_t ntris, bool keep_debug) {
+  b.nverts = nverts;
+  b.ntris = ntris;
+  b.keep_debug = keep_debug;
+  const size_t npairs = std::max<size_t>(ntris > 1 ? ntris - 1 : 1, 1);
+  HRT_CHECK(hipMalloc(&b.verts, sizeof(float4) * std::max<size_t>(nverts, 1)));
+  HRT_CHECK(hipMalloc(&b.idx, sizeof(uint32_t) * 3 * size_t(ntris)));
+  HRT_CHECK(hipMalloc(&b.pairs, sizeof(NodePair) * npairs));
+  HRT_CHECK(hipMalloc(&b.tris, sizeof(TriPre) * size_t(ntris)));
+  HRT_CHECK(hipMalloc(&b.perm, sizeof(uint32_t) * size_t(ntris)));
+  if (keep_debug) {
+    HRT_CHECK(hipMalloc(&b.codes_in, sizeof(uint32_t) * size_t(ntris)));
+    HRT_CHECK(hipMalloc(&b.codes_sorted, sizeof(uint32_t) * size_t(ntris)));
+    HRT_CHECK(hipMalloc(&b.ids_sorted, sizeof(uint32_t) * size_t(ntris)));
+    HRT_CHECK(hipMalloc(&b.run_start, sizeof(uint32_t) * size_t(ntris)));
+    HRT_CHECK(hipMalloc(&b.ucodes, sizeof(uint32_t) * size_t(ntris)));
+    HRT_CHECK(hipMalloc(&b.karras_lr, sizeof(uint32_t) * 2 * npairs));
+  }
+  return hipSuccess;
+}
+
+void blas_free(Blas& b) {
+  void* ptrs[] = {b.verts, b.idx, b.pairs, b.tris, b.perm, b.codes_in, b.codes_sorted,
+                  b.ids_sorted, b.run_start, b.ucodes, b.karras_lr};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  b = Blas();
+}
+
+hipError_t blas_build(Blas& b, hipStream_t s) {
+  const uint64_t N = b.ntris;
+  if (N == 0) return hipErrorInvalidValue;
+  const uint32_t ntiles = uint32_t((N + kSortTile - 1) / kSortTile);
+  const uint64_t hist_words = uint64_t(kDigits) * ntiles;
+  const int box_parts = int(std::min<uint64_t>(1024, std::max<uint64_t>(1, (b.nverts + kBlock - 1) / kBlock)));
+  // scratch layout (u32 words)
+  const size_t w_codes = N, w_hist = hist_words, w_scan = scan_temp_words(std::max<uint64_t>(N, hist_words));
+  size_t words = 0;
+  auto take = [&](size_t n) { size_t o = words; words += (n + 63) & ~size_t(63); return o; };
+  const size_t o_ka = take(w_codes), o_va = take(w_codes), o_kb = take(w_codes), o_vb = take(w_codes);
+  const size_t o_hist = take(w_hist), o_hscan = take(w_hist), o_scan = take(w_scan);
+  const size_t o_flags = take(N), o_pos = take(N), o_rs = take(N), o_uc = take(N);
+  const size_t o_pint = take(N), o_pleaf = take(N), o_arr = take(N), o_meta = take(64);
+  const size_t o_part = take(size_t(box_parts) * 8), o_box = take(16);
+  uint32_t* base = nullptr;
+  HRT_CHECK(hipMallocAsync(reinterpret_cast<void**>(&base), words * 4, s));
+  uint32_t *ka = base + o_ka, *va = base + o_va, *kb = base + o_kb, *vb = base + o_vb;
+  uint32_t *hist = base + o_hist, *hscan = base + o_hscan, *stmp = base + o_scan;
+  uint32_t *flags = base + o_flags, *pos = base + o_pos, *rs = base + o_rs, *uc = base + o_uc;
+  uint32_t *pint = base + o_pint, *pleaf = base + o_pleaf, *arrive = base + o_arr, *meta = base + o_meta;
+  float4* part = reinterpret_cast<float4*>(base + o_part);
+  float* box = reinterpret_cast<float*>(base + o_box);
+
+  hipError_t err = hipSuccess;
+  do {
+    k_box_partial<<<box_parts, kBlock, 0, s>>>(b.verts, b.nverts, part);
+    k_box_final<<<1, kBlock, 0, s>>>(part, box_parts, box, box + 8);
+    k_morton<<<blocks_for(N), kBlock, 0, s>>>(b.verts, b.idx, uint32_t(N), box, box + 8, ka, va);
+    if ((err = hipGetLastError()) != hipSuccess) break;
+    if (b.keep_debug)
+      if ((err = hipMemcpyAsync(b.codes_in, ka, 4 * N, hipMemcpyDeviceToDevice, s)) != hipSuccess) break;
+    uint32_t *kin = ka, *vin = va, *kout = kb, *vout = vb;
+    for (int shift = 0; shift < 32; shift += 8) {
+      k_radix_hist<<<ntiles, kBlock, 0, s>>>(kin, uint32_t(N), shift, hist, ntiles);
+      if ((err = exclusive_scan(hist, hscan, hist_words, stmp, s)) != hipSuccess) break;
+      k_radix_scatter<<<ntiles, kBlock, 0, s>>>(kin, vin, kout, vout, uint32_t(N), shift, hscan, ntiles);
+      std::swap(kin, kout);
+      std::swap(vin, vout);
+    }
+    if (err != hipSuccess || (err = hipGetLastError()) != hipSuccess) break;
+    // after four passes the sorted keys are back in (ka, va)
+    k_run_flags<<<blocks_for(N), kBlock, 0, s>>>(kin, uint32_t(N), flags);
+    if ((err = exclusive_scan(flags, pos, N, stmp, s)) != hipSuccess) break;
+    if ((err = hipMemsetAsync(meta, 0, 64 * 4, s)) != hipSuccess) break;
+    k_run_compact<<<blocks_for(N), kBlock, 0, s>>>(kin, flags, pos, uint32_t(N), rs, uc, meta);
+    if ((err = hipMemsetAsync(arrive, 0, 4 * N, s)) != hipSuccess) break;
+    k_karras<<<blocks_for(N), kBlock, 0, s>>>(uc, rs, meta, uint32_t(N), b.pairs, pint, pleaf, b.karras_lr);
+    k_leaves_refit<<<blocks_for(N), kBlock, 0, s>>>(b.verts, b.idx, vin, rs, meta, uint32_t(N), b.pairs, pint,
+                                                    pleaf, arrive);
+    k_tris<<<blocks_for(N), kBlock, 0, s>>>(b.verts, b.idx, vin, uint32_t(N), b.tris, b.perm);
+    if ((err = hipGetLastError()) != hipSuccess) break;
+    if (b.keep_debug) {
+      if ((err = hipMemcpyAsync(b.codes_sorted, kin, 4 * N, hipMemcpyDeviceToDevice, s)) != hipSuccess) break;
+      if ((err = hipMemcpyAsync(b.ids_sorted, vin, 4 * N, hipMemcpyDeviceToDevice, s)) != hipSuccess) break;
+      if ((err = hipMemcpyAsync(b.run_start, rs, 4 * N, hipMemcpyDeviceToDevice, s)) != hipSuccess) break;
+      if ((err = hipMemcpyAsync(b.ucodes, uc, 4 * N, hipMemcpyDeviceToDevice, s)) != hipSuccess) break;
+    }
+    uint32_t hmeta[2];
+    float hbox[12];
+    if ((err = hipMemcpyAsync(hmeta, meta, sizeof(hmeta), hipMemcpyDeviceToHost, s)) != hipSuccess) break;
+    if ((err = hipMemcpyAsync(hbox, box, sizeof(hbox), hipMemcpyDeviceToHost, s)) != hipSuccess) break;
+    if ((err = hipStreamSynchronize(s)) != hipSuccess) break;
+    b.leaves = hmeta[0];
+    b.max_run = hmeta[1];
+    std::memcpy(b.box, hbox, sizeof(b.box));
+    std::memcpy(b.zscale, hbox + 8, sizeof(b.zscale));
+  } while (false);
+  hipError_t ferr = hipFreeAsync(base, s);
+  if (err == hipSuccess) err = ferr;
+  if (err == hipSuccess) err = hipStreamSynchronize(s);
+  return err;
+}
+
+}  // namespace hrt

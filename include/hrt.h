@@ -1,0 +1,165 @@
+/* hrt.h -- C-ABI of the MI355X ray-query engine (libhrt.so).
+ *
+ * This is the drop-in boundary for HeavyRT's ray-query path.  In the
+ * reference the boundary is the C++ interface ISceneObject plus the factory
+ * CreateSceneRT (core/CrossRT.h:64-196, core/FactoryRT.cpp:29-80); every
+ * entry point below names the reference member it replaces.  The C++ adapter
+ * in heavyrt_amd/host/HipSceneRT.cpp implements ISceneObject on top of these
+ * calls, so callers written against the reference interface run unchanged.
+ *
+ * Conventions
+ *   - Plain C types only; every call returns hrt_status (HRT_OK == 0) and
+ *     leaves a thread-local message for hrt_last_error() on failure.  No C++
+ *     exception crosses this boundary.
+ *   - Host input buffers (vertices, indices, matrices) are copied during the
+ *     call; the caller keeps ownership (BVH2CommonRT_host.cpp:76-83).
+ *   - Ray and hit buffers of the batched trace calls are DEVICE pointers
+ *     (HBM), and the calls are asynchronous on the given HIP stream
+ *     (a hipStream_t passed as void*; NULL = the context's stream).
+ *   - Scene mutation (clear/add/commit) is exclusive; concurrent trace calls
+ *     on a committed scene are allowed.
+ *   - Ray record: 8 floats = posAndNear (x, y, z, tNear), dirAndFar
+ *     (x, y, z, tFar), the argument pair of RayQuery_NearestHit
+ *     (CrossRT.h:143).  tFar <= 0 requests an any-hit query with
+ *     tFar' = -tFar that stops at the first hit (BVH4HalfRT.cpp:260-262).
+ *   - primId in every hit is the ORIGINAL triangle index of the geometry
+ *     (the REMAP_PRIM_ID rule, BVH2CommonRT.cpp:270-276, and BruteForceRT.cpp:255).
+ */
+#ifndef HRT_H_
+#define HRT_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum hrt_status {
+  HRT_OK = 0,
+  HRT_ERR_INVALID_ARG = 1,   /* null pointer, bad id, bad size */
+  HRT_ERR_HIP = 2,           /* a HIP runtime call failed */
+  HRT_ERR_OUT_OF_MEMORY = 3, /* device allocation failed */
+  HRT_ERR_NOT_COMMITTED = 4, /* trace before CommitScene */
+  HRT_ERR_UNSUPPORTED = 5,   /* valid request this build does not implement */
+  HRT_ERR_LEAF_OVERFLOW = 6  /* > 127 triangles share one Morton code */
+} hrt_status;
+
+typedef struct hrt_ctx hrt_ctx;
+typedef struct hrt_scene hrt_scene;
+
+/* Hit record, layout-identical to CRT_Hit (CrossRT.h:20-27): t, primId,
+ * instId, geomId, coords[4] with coords[0] = u, coords[1] = v.  On a miss the
+ * ids are 0xFFFFFFFF and t = tFar. */
+typedef struct hrt_hit {
+  float t;
+  uint32_t primId;
+  uint32_t instId;
+  uint32_t geomId;
+  float coords[4];
+} hrt_hit;
+
+/* Compact nearest-hit record (8 bytes): t and primId only. */
+typedef struct hrt_hit_compact {
+  float t;
+  uint32_t primId;
+} hrt_hit_compact;
+
+enum { HRT_HIT_FULL = 0, HRT_HIT_COMPACT = 1 };
+enum { HRT_LAYOUT_BVH2 = 0 };
+enum { HRT_BUILD_LOW = 0, HRT_BUILD_MEDIUM = 1, HRT_BUILD_HIGH = 2, HRT_BUILD_REFIT = 3 }; /* BuildQuality, CrossRT.h:9-15 */
+
+/* Debug / parity dumps (hrt_debug_dump `what`).  Available for geometries
+ * added after hrt_scene_set_keep_build_arrays(scene, 1). */
+enum {
+  HRT_DUMP_SIZES = 0,        /* uint32[4]: triangles, leaves, max leaf size, pairs */
+  HRT_DUMP_CODES_INPUT = 1,  /* uint32[ntris]: Morton code per input triangle */
+  HRT_DUMP_CODES_SORTED = 2, /* uint32[ntris]: codes after the (code, id) sort */
+  HRT_DUMP_IDS_SORTED = 3,   /* uint32[ntris]: triangle ids after the sort */
+  HRT_DUMP_RUN_START = 4,    /* uint32[leaves]: first sorted position of each unique code */
+  HRT_DUMP_UNIQUE_CODES = 5, /* uint32[leaves] */
+  HRT_DUMP_LR_NODES = 6,     /* uint32[(2*leaves-1)*8]: nodes in the reference's Karras
+                                left/right numbering (cbvh.h:20-26; lbvh.cpp:438-439) */
+  HRT_DUMP_PERM = 7,         /* uint32[ntris]: leaf-order position -> triangle id */
+  HRT_DUMP_SCENE_BOX = 8     /* float[12]: box min xyzw, box max xyzw, Morton scale xyzw */
+};
+
+/* Context: one device, one HIP stream.  No counterpart in the reference
+ * (its engine is CPU-only); it owns what every scene shares. */
+hrt_status hrt_init(int device, hrt_ctx** out);
+void hrt_shutdown(hrt_ctx* ctx);
+/* The context's stream as a hipStream_t. */
+void* hrt_ctx_stream(hrt_ctx* ctx);
+
+/* Replaces `new X` in CreateSceneRT (FactoryRT.cpp:29-78) / DeleteSceneRT (:80). */
+hrt_status hrt_scene_create(hrt_ctx* ctx, hrt_scene** out);
+void hrt_scene_destroy(hrt_scene* scene);
+/* Keep builder intermediates for hrt_debug_dump (parity hooks). */
+hrt_status hrt_scene_set_keep_build_arrays(hrt_scene* scene, int keep);
+
+/* ISceneObject::ClearGeom (CrossRT.h:82; BVH2CommonRT_host.cpp:20-43). */
+hrt_status hrt_scene_clear_geom(hrt_scene* scene);
+/* ISceneObject::ClearScene (CrossRT.h:110; BVH2CommonRT_host.cpp:105-116). */
+hrt_status hrt_scene_clear_instances(hrt_scene* scene);
+
+/* ISceneObject::AddGeom_Triangles3f (CrossRT.h:94; BVH2CommonRT_host.cpp:66-98):
+ * copies nVert vertices (stride vStrideBytes, 0 = packed float3) and nIdx
+ * indices, builds the LBVH on the device and returns the geometry id. */
+hrt_status hrt_add_triangles(hrt_scene* scene, const float* vpos, size_t nVert, size_t vStrideBytes,
+                             const uint32_t* idx, size_t nIdx, int quality, uint32_t* outGeomId);
+
+/* ISceneObject::AddInstance (CrossRT.h:123; BVH2CommonRT_host.cpp:131-164);
+ * colMajor16 is a column-major float4x4. */
+hrt_status hrt_add_instance(hrt_scene* scene, uint32_t geomId, const float colMajor16[16], uint32_t* outInstId);
+
+/* ISceneObject::CommitScene (CrossRT.h:115; BVH2CommonRT_host.cpp:118-129). */
+hrt_status hrt_commit(hrt_scene* scene, int quality, int layout);
+
+/* Batched ISceneObject::RayQuery_NearestHit (CrossRT.h:143; BVH2CommonRT.cpp:228-279):
+ * n rays (32 B each, device) -> n hits (device, hrt_hit or hrt_hit_compact). */
+hrt_status hrt_trace_nearest(hrt_scene* scene, const float* rays, size_t n, void* hits, int hitFormat,
+                             void* stream);
+
+/* Batched ISceneObject::RayQuery_AnyHit (CrossRT.h:151; BVH4HalfRT.cpp:327-331):
+ * out[i] = 1 if any triangle is hit in (tNear, |tFar|). */
+hrt_status hrt_trace_any(hrt_scene* scene, const float* rays, size_t n, uint8_t* out, void* stream);
+
+/* Same as hrt_trace_nearest, and also writes per-ray traversal counters
+ * (uint32 pairs visited, uint32 triangles tested) -- the reference's NC/2 and
+ * TC metrics (CrossRT.h:172-175, BVH2CommonRT.cpp:69,124). */
+hrt_status hrt_trace_nearest_stats(hrt_scene* scene, const float* rays, size_t n, void* hits, int hitFormat,
+                                   uint32_t* stats, void* stream);
+hrt_status hrt_trace_any_stats(hrt_scene* scene, const float* rays, size_t n, uint8_t* out, uint32_t* stats,
+                               void* stream);
+
+/* One synchronous host query: the per-ray path behind
+ * ISceneObject::RayQuery_NearestHit for callers that cannot batch. */
+hrt_status hrt_query_nearest_one(hrt_scene* scene, const float posNear[4], const float dirFar[4], hrt_hit* out);
+
+/* Device-side AO ray generation for the RTAO renderer (RTAO::kernel_InitAORay,
+ * examples/02_rtao/rtao.cpp:147-165; MapSampleToCosineDistribution and
+ * OffsRayPos, examples/render_common.h:56-112): for each of nPixels primary
+ * rays and its compact hit, writes raysPerHit cosine-weighted rays about the
+ * hit triangle's viewer-facing geometric normal with tNear = 0, tFar = tMax,
+ * PCG32-seeded by `seed` and the ray index.  Missed pixels get empty rays.
+ * All pointers are device pointers; asynchronous on `stream`. */
+hrt_status hrt_rtao_make_rays(hrt_scene* scene, const float* primaryRays, const void* primaryHitsCompact,
+                              size_t nPixels, uint32_t raysPerHit, float tMax, uint64_t seed, float* outRays,
+                              void* stream);
+
+/* ISceneObject::GetGeomNum / GetInstNum (CrossRT.h:163-164). */
+uint32_t hrt_geom_count(const hrt_scene* scene);
+uint32_t hrt_instance_count(const hrt_scene* scene);
+
+/* Parity hooks: copies one of the HRT_DUMP_* arrays of geometry geomId to
+ * host memory; `bytes` is the capacity of host_out. */
+hrt_status hrt_debug_dump(hrt_scene* scene, uint32_t geomId, int what, void* host_out, size_t bytes);
+
+/* Thread-local description of the last failure in this thread. */
+const char* hrt_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* HRT_H_ */

@@ -1,0 +1,816 @@
+// hrt_oracle.cpp -- TEST INFRASTRUCTURE ONLY.
+//
+// CPU restatement of the reference's ray-query path (HeavyRT), used as the
+// parity checker for the HIP engine and as the CPU baseline in bench.py.
+// Nothing in the shipped library links, loads or calls this file.
+//
+// What is restated, and where it comes from (reference paths are relative to
+// the HeavyRT source tree):
+//   Morton codes          core/builders/lbvh.cpp:25-51, 259-287
+//   scene box             core/builders/lbvh.cpp:228-239 (all vertices, +-1e10 init)
+//   sort                  core/builders/lbvh.cpp:755 -- the reference uses an
+//                         unstable std::sort on the code; this restatement
+//                         fixes the order to (code, index), i.e. a stable sort
+//   unique-code runs      core/builders/lbvh.cpp:289-319, 765
+//   leaves                core/builders/lbvh.cpp:350-384
+//   Karras 2012           core/builders/lbvh.cpp:386-441
+//   refit                 core/builders/lbvh.cpp:443-489 (32 in-place sweeps)
+//   peach layout          core/builders/lbvh.cpp:84-188, 492-729, 805-824
+//   BVH2 traversal        core/BVH2CommonRT.cpp:9-141 (+ raytrace_common.h:38-80)
+//   any-hit rule          core/BVH4HalfRT.cpp:258-331 (tFar <= 0 => stop at first hit)
+//   brute force           core/BruteForceRT.cpp:184-265
+//
+// Floating point: every float3 operation is scalar, left to right
+// (dot = x*x' + y*y' + z*z'), and the file is compiled with
+// -ffp-contract=off and without fast-math, so each operation rounds once.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+struct f3 { float x, y, z; };
+struct f4 { float x, y, z, w; };
+
+static inline f3 sub3(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+static inline float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+static inline f3 cross3(f3 a, f3 b) {
+  return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+// std::min / std::max semantics (raytrace_common.h:76-79 uses them).
+static inline float mn(float a, float b) { return (b < a) ? b : a; }
+static inline float mx(float a, float b) { return (a < b) ? b : a; }
+
+constexpr uint32_t START_MASK = 0x00FFFFFFu;   // raytrace_common.h:38
+constexpr uint32_t SIZE_MASK = 0x7F000000u;    // raytrace_common.h:40
+constexpr uint32_t LEAF_BIT = 0x80000000u;     // raytrace_common.h:41
+constexpr float LBVH_INF = 1e10f;              // lbvh.cpp:53
+constexpr float LBVH_EPS_BIAS = 1e-4f;         // lbvh.cpp:54
+constexpr uint32_t PEACHES_MIN = 256;          // lbvh.h:25
+constexpr uint32_t PEACHES_MAX = 2048;         // lbvh.h:26
+
+static inline uint32_t pack_leaf(uint32_t start, uint32_t size) {  // raytrace_common.h:48-51
+  return LEAF_BIT | ((size << 24) & SIZE_MASK) | (start & START_MASK);
+}
+static inline uint32_t leaf_start(uint32_t v) { return v & START_MASK; }
+static inline uint32_t leaf_count(uint32_t v) { return (v & SIZE_MASK) >> 24; }
+
+// 10-bit spread with the magic masks of lbvh.cpp:25-34.
+static inline uint32_t spread_bits(int x, int offset) {
+  x = (x | (x << 10)) & 0x000F801F;
+  x = (x | (x << 4)) & 0x00E181C3;
+  x = (x | (x << 2)) & 0x03248649;
+  x = (x | (x << 2)) & 0x09249249;
+  return uint32_t(x) << offset;
+}
+static inline uint32_t zindex3d(int x, int y, int z) {  // lbvh.cpp:36-39
+  return spread_bits(x, 0) | spread_bits(y, 1) | spread_bits(z, 2);
+}
+
+// float -> int as the reference's implicit conversion at lbvh.cpp:282 does
+// (truncation toward zero).  Values are in [0, 1024] for every input whose
+// scene box is finite; out-of-range values are clamped the way the GPU's
+// v_cvt_i32_f32 saturates, so both sides agree on pathological inputs.
+static inline int f2i(float f) {
+  if (!(f == f)) return 0;
+  if (f >= 2147483647.0f) return 2147483647;
+  if (f <= -2147483648.0f) return int(0x80000000u);
+  return int(f);
+}
+
+struct Node {          // cbvh.h:20-26 (32 bytes)
+  f3 bmin;
+  uint32_t left;
+  f3 bmax;
+  uint32_t escape;
+};
+
+static inline Node dummy_node() {  // cbvh.h:28-36
+  Node n;
+  n.bmin = {0, 0, 0};
+  n.bmax = {0, 0, 0};
+  n.left = 0x80000000u;
+  n.escape = 0xFFFFFFFFu;
+  return n;
+}
+
+struct Build {
+  // inputs (float4 positions with w = 1, triangle indices)
+  std::vector<f4> verts;
+  std::vector<uint32_t> idx;
+  uint32_t ntris = 0;
+
+  // builder state, named after lbvh.h:120-145
+  f4 bbmin, bbmax, zscale;
+  std::vector<uint32_t> codes_unsorted;  // code per triangle (input order)
+  std::vector<uint32_t> sorted_codes;    // (code, tri) after the sort, first ntris entries
+  std::vector<uint32_t> sorted_ids;
+  std::vector<uint32_t> run_start;       // newIndices
+  std::vector<uint32_t> ucodes;          // compressedCodes
+  uint32_t L = 0;                        // leaves
+  std::vector<Node> lr;                  // Karras left/right layout, 2L-1 nodes
+  std::vector<uint32_t> perm;            // a_outIndices: leaf-order position -> triangle id
+  std::vector<uint32_t> temp_count;      // m_tempCount
+  uint32_t max_run = 0;
+
+  // peach layout (BVH2_LEFT_OFFSET), as BVH2CommonRT traverses it
+  std::vector<Node> peach;
+  // leaf-order index buffer (AppendTreeData, BVH2CommonRT_host.cpp:45-64)
+  std::vector<uint32_t> idx_leaf_order;
+};
+
+static size_t power_of_two(size_t n) {  // lbvh.cpp:76-82
+  size_t s = 2;
+  while (s < n) s *= 2;
+  return s;
+}
+static size_t round_up_to(size_t a, size_t n) {  // lbvh.cpp:69-74
+  while (a % n != 0) a++;
+  return a;
+}
+
+static int delta(const std::vector<uint32_t>& c, int i, int j) {  // lbvh.cpp:386-403
+  if (j < 0 || j >= int(c.size())) return -1;
+  uint32_t x = c[i] ^ c[j];
+  if (x == 0) return 32;
+  return __builtin_clz(x);
+}
+
+static inline int isign(int v) { return (v > 0) - (v < 0); }
+
+static void karras(Build& b) {  // lbvh.cpp:405-441
+  const std::vector<uint32_t>& c = b.ucodes;
+  const int Lm1 = int(b.L) - 1;
+  for (int i = 0; i < Lm1; ++i) {
+    int d = isign(delta(c, i, i + 1) - delta(c, i, i - 1));
+    int dmin = delta(c, i, i - d);
+    int lmax = 2;
+    while (delta(c, i, i + lmax * d) > dmin) lmax <<= 1;
+    int l = 0;
+    for (int t = lmax >> 1; t > 0; t >>= 1)
+      if (delta(c, i, i + (l + t) * d) > dmin) l += t;
+    int j = i + l * d;
+    int dnode = delta(c, i, j);
+    int s = 0;
+    for (int t = (l + 1) >> 1; t > 0; t = (t == 1 ? 0 : (t + 1) >> 1))
+      if (delta(c, i, i + (s + t) * d) > dnode) s += t;
+    int gamma = i + s * d + std::min(d, 0);
+    bool leftLeaf = (std::min(i, j) == gamma);
+    bool rightLeaf = (std::max(i, j) == gamma + 1);
+    b.lr[i].left = leftLeaf ? uint32_t(Lm1 + gamma) : uint32_t(gamma);
+    b.lr[i].escape = rightLeaf ? uint32_t(Lm1 + gamma + 1) : uint32_t(gamma + 1);
+  }
+}
+
+static void refit(Build& b) {  // lbvh.cpp:443-489, driver 781-785
+  const int Lm1 = int(b.L) - 1;
+  std::vector<uint32_t> lrx(Lm1 > 0 ? Lm1 : 0), lry(Lm1 > 0 ? Lm1 : 0);
+  for (int i = 0; i < Lm1; i++) {
+    lrx[i] = b.lr[i].left;
+    lry[i] = b.lr[i].escape;
+    b.temp_count[i] = 2;
+  }
+  int limit = int(b.L) - 1;  // int(newIndices.size()) - 1; set to -1 once the root is done
+  for (int pass = 0; pass < 32; pass++) {
+    for (int i = 0; i < Lm1; i++) {
+      if (lrx[i] == 0xFFFFFFFFu) continue;
+      uint32_t llx = 0xFFFFFFFFu, rlx = 0xFFFFFFFFu;
+      if (int(lrx[i]) < limit) llx = lrx[lrx[i]];
+      if (int(lry[i]) < limit) rlx = lrx[lry[i]];
+      if (llx == 0xFFFFFFFFu && rlx == 0xFFFFFFFFu) {
+        const Node& ln = b.lr[lrx[i]];
+        const Node& rn = b.lr[lry[i]];
+        Node cur;
+        cur.bmin = {mn(ln.bmin.x, rn.bmin.x), mn(ln.bmin.y, rn.bmin.y), mn(ln.bmin.z, rn.bmin.z)};
+        cur.bmax = {mx(ln.bmax.x, rn.bmax.x), mx(ln.bmax.y, rn.bmax.y), mx(ln.bmax.z, rn.bmax.z)};
+        cur.left = lrx[i];
+        cur.escape = lry[i];
+        uint32_t lc = b.temp_count[lrx[i]], rc = b.temp_count[lry[i]];
+        b.lr[i] = cur;
+        lrx[i] = 0xFFFFFFFFu;
+        lry[i] = 0xFFFFFFFFu;
+        b.temp_count[i] = lc + rc + 2;
+        if (i == 0) limit = -1;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Peach layout (lbvh.cpp:84-188 Reserve/BuildPeachTree, 492-729 conversion)
+// ---------------------------------------------------------------------------
+struct U2 { uint32_t x, y; };
+
+static void push_down_escape(std::vector<U2>& le, uint32_t off, uint32_t parentEsc) {  // lbvh.cpp:141-154
+  if (off >= le.size()) return;
+  const U2 l = le[off + 0];
+  const U2 r = le[off + 1];
+  le[off + 0].y = off + 1;
+  le[off + 1].y = parentEsc;
+  push_down_escape(le, l.x, off + 1);
+  push_down_escape(le, r.x, parentEsc);
+}
+
+static void build_peach(Build& b) {
+  const size_t nprims = b.ntris;
+  const size_t childs = 2;
+  size_t currOffs = 0, currSize = 2;
+  size_t peachSize = (nprims <= PEACHES_MAX * 4) ? PEACHES_MIN : PEACHES_MAX;
+  std::vector<U2> intervals;
+  std::vector<uint32_t> countPrefix;
+  if (nprims == 1) {
+    intervals.push_back({0, uint32_t(childs)});
+    currOffs = childs;
+  } else {
+    do {
+      intervals.push_back({uint32_t(currOffs), uint32_t(currSize)});
+      currOffs += currSize;
+      currSize *= childs;
+    } while (currSize < peachSize);
+    countPrefix.assign(currSize, 0);
+    intervals.push_back({uint32_t(currOffs), uint32_t(currSize)});
+    currOffs += currSize;
+    currSize *= childs;
+    currOffs += currSize;
+    currOffs += round_up_to(2 * nprims - 1, childs);
+  }
+  const size_t reserved = currOffs;
+  std::vector<U2> le(reserved, U2{0xFFFFFFFFu, 0});
+  for (size_t lev = 0; lev + 1 < intervals.size(); lev++) {
+    const U2 cur = intervals[lev], nxt = intervals[lev + 1];
+    for (uint32_t i = cur.x; i < cur.x + cur.y; i++) le[i] = U2{nxt.x + 2 * (i - cur.x), 0};
+  }
+  push_down_escape(le, 0, uint32_t(-2));
+
+  std::vector<U2> addr(reserved, U2{0, 0});
+  std::vector<Node> out(reserved);
+  for (auto& n : out) { n.bmin = {0, 0, 0}; n.bmax = {0, 0, 0}; n.left = 0; n.escape = 0; }
+  const std::vector<Node>& in = b.lr;
+
+  // PeachConvertInit, lbvh.cpp:492-543
+  {
+    Node root = in[0];
+    if (int(root.left) < 0) {
+      root.escape = uint32_t(-2);
+      addr[0] = U2{0xFFFFFFFFu, 0xFFFFFFFFu};
+      addr[1] = U2{0xFFFFFFFFu, 0xFFFFFFFFu};
+      out[0] = root;
+      out[1] = dummy_node();
+    } else {
+      const Node left = in[root.left];
+      const Node right = in[root.escape];
+      addr[0] = U2{left.left, left.escape};
+      addr[1] = U2{right.left, right.escape};
+      Node ln = out[0], rn = out[1];
+      ln.bmin = left.bmin; ln.bmax = left.bmax;
+      rn.bmin = right.bmin; rn.bmax = right.bmax;
+      ln.left = le[0].x; ln.escape = le[0].y;
+      rn.left = le[1].x; rn.escape = le[1].y;
+      if (int(left.left) < 0) { ln.left = left.left; addr[0] = U2{0xFFFFFFFFu, 0xFFFFFFFFu}; }
+      if (int(right.left) < 0) { rn.left = right.left; addr[1] = U2{0xFFFFFFFFu, 0xFFFFFFFFu}; }
+      out[0] = ln;
+      out[1] = rn;
+    }
+  }
+  // PeachConvertPass, lbvh.cpp:545-612
+  for (size_t lev = 0; lev + 1 < intervals.size(); lev++) {
+    const U2 cur = intervals[lev], nxt = intervals[lev + 1];
+    const bool last = (lev == intervals.size() - 2);
+    for (uint32_t i = 0; i < cur.y; i++) {
+      const U2 lr = addr[cur.x + i];
+      bool leftLeaf = true, rightLeaf = true;
+      if (lr.x == 0xFFFFFFFFu) {
+        out[nxt.x + 2 * i + 0] = dummy_node();
+        addr[nxt.x + 2 * i + 0] = U2{0xFFFFFFFFu, 0xFFFFFFFFu};
+      } else {
+        const Node left = in[lr.x];
+        const U2 lle = le[nxt.x + 2 * i + 0];
+        leftLeaf = (int(left.left) < 0);
+        Node n;
+        n.bmin = left.bmin; n.bmax = left.bmax;
+        n.left = leftLeaf ? left.left : lle.x;
+        n.escape = lle.y;
+        out[nxt.x + 2 * i + 0] = n;
+        addr[nxt.x + 2 * i + 0] = leftLeaf ? U2{0xFFFFFFFFu, 0xFFFFFFFFu} : U2{left.left, left.escape};
+      }
+      if (lr.y == 0xFFFFFFFFu) {
+        out[nxt.x + 2 * i + 1] = dummy_node();
+        addr[nxt.x + 2 * i + 1] = U2{0xFFFFFFFFu, 0xFFFFFFFFu};
+      } else {
+        const Node right = in[lr.y];
+        const U2 rle = le[nxt.x + 2 * i + 1];
+        rightLeaf = (int(right.left) < 0);
+        Node n;
+        n.bmin = right.bmin; n.bmax = right.bmax;
+        n.left = rightLeaf ? right.left : rle.x;
+        n.escape = rle.y;
+        out[nxt.x + 2 * i + 1] = n;
+        addr[nxt.x + 2 * i + 1] = rightLeaf ? U2{0xFFFFFFFFu, 0xFFFFFFFFu} : U2{right.left, right.escape};
+      }
+      if (last) {
+        countPrefix[2 * i + 0] = leftLeaf ? 0 : b.temp_count[lr.x];
+        countPrefix[2 * i + 1] = rightLeaf ? 0 : b.temp_count[lr.y];
+      }
+    }
+  }
+  for (size_t i = 1; i < countPrefix.size(); i++) countPrefix[i] += countPrefix[i - 1];  // inclusive scan, lbvh.cpp:816
+
+  // RecursiveConvert, lbvh.cpp:644-729
+  if (intervals.size() != 1) {
+    const U2 lastLev = intervals.back();
+    for (uint32_t i = 0; i < lastLev.y; i++) {
+      const U2 lr = addr[lastLev.x + i];
+      uint32_t sub = lastLev.x + lastLev.y;
+      if (i > 0) sub += countPrefix[i - 1];
+      if (lr.x == 0xFFFFFFFFu) continue;
+      out[lastLev.x + i].left = sub;
+      struct U4 { uint32_t x, y, z, w; } stack[32];
+      int top = 0;
+      uint32_t lo = lr.x, ro = lr.y, pesc = out[lastLev.x + i].escape;
+      while (top >= 0) {
+        bool pop = (int(lo) < 0);
+        if (int(lo) > 0) {
+          const Node ln = in[lo];
+          const Node rn = in[ro];
+          const uint32_t cs = sub;
+          sub += 2;
+          out[cs + 0] = ln;
+          out[cs + 1] = rn;
+          out[cs + 0].escape = cs + 1;
+          out[cs + 1].escape = pesc;
+          pop = (int(ln.left) < 0) && (int(rn.left) < 0);
+          if (int(ln.left) > 0 && int(rn.left) > 0) {
+            stack[top] = U4{rn.left, rn.escape, pesc, cs + 1};
+            top++;
+          }
+          if (int(ln.left) > 0) {
+            lo = ln.left; ro = ln.escape; pesc = cs + 1;
+            out[cs + 0].left = sub;
+          } else if (int(rn.left) > 0) {
+            lo = rn.left; ro = rn.escape;
+            out[cs + 1].left = sub;
+          }
+        }
+        if (pop) {
+          top--;
+          U4 od = stack[std::max(top, 0)];
+          lo = od.x; ro = od.y; pesc = od.z;
+          if (int(lo) > 0 && top >= 0) out[od.w].left = sub;
+        }
+      }
+    }
+  }
+  if (out.size() % 2 != 0) out.push_back(dummy_node());  // cbvh_hlbvh.cpp:232-236
+  b.peach.swap(out);
+}
+
+static void build_all(Build& b) {
+  const uint32_t N = b.ntris;
+  // scene box over all vertices, lbvh.cpp:228-239
+  f4 lo = {LBVH_INF, LBVH_INF, LBVH_INF, LBVH_INF}, hi = {-LBVH_INF, -LBVH_INF, -LBVH_INF, -LBVH_INF};
+  for (const f4& v : b.verts) {
+    lo = {mn(lo.x, v.x), mn(lo.y, v.y), mn(lo.z, v.z), mn(lo.w, v.w)};
+    hi = {mx(hi.x, v.x), mx(hi.y, v.y), mx(hi.z, v.z), mx(hi.w, v.w)};
+  }
+  b.bbmin = lo;
+  b.bbmax = hi;
+  // ZIndexMultCoeff(bboxMin - 1e-4, bboxMax + 1e-4), lbvh.cpp:41-51, 263
+  {
+    const float ex[4] = {(hi.x + LBVH_EPS_BIAS) - (lo.x - LBVH_EPS_BIAS), (hi.y + LBVH_EPS_BIAS) - (lo.y - LBVH_EPS_BIAS),
+                         (hi.z + LBVH_EPS_BIAS) - (lo.z - LBVH_EPS_BIAS), (hi.w + LBVH_EPS_BIAS) - (lo.w - LBVH_EPS_BIAS)};
+    float inv[4];
+    for (int k = 0; k < 4; k++) inv[k] = 1.0f / ex[k];
+    for (int k = 0; k < 3; k++)
+      if (std::fabs(ex[k]) < 1e-20f) inv[k] = 0.0f;
+    b.zscale = {1024.0f * inv[0], 1024.0f * inv[1], 1024.0f * inv[2], 1024.0f * inv[3]};
+  }
+  // Morton codes, lbvh.cpp:259-287
+  b.codes_unsorted.resize(N);
+  for (uint32_t i = 0; i < N; i++) {
+    const f4 v1 = b.verts[b.idx[3 * i + 0]], v2 = b.verts[b.idx[3 * i + 1]], v3 = b.verts[b.idx[3 * i + 2]];
+    float cx, cy, cz;
+    if (N < 256) {
+      cx = 0.33333333f * ((v1.x + v2.x) + v3.x);
+      cy = 0.33333333f * ((v1.y + v2.y) + v3.y);
+      cz = 0.33333333f * ((v1.z + v2.z) + v3.z);
+    } else {
+      cx = (mn(v1.x, mn(v2.x, v3.x)) + mx(v1.x, mx(v2.x, v3.x))) * 0.5f;
+      cy = (mn(v1.y, mn(v2.y, v3.y)) + mx(v1.y, mx(v2.y, v3.y))) * 0.5f;
+      cz = (mn(v1.z, mn(v2.z, v3.z)) + mx(v1.z, mx(v2.z, v3.z))) * 0.5f;
+    }
+    const float sx = b.zscale.x * (cx - lo.x);
+    const float sy = b.zscale.y * (cy - lo.y);
+    const float sz = b.zscale.z * (cz - lo.z);
+    b.codes_unsorted[i] = zindex3d(f2i(sx), f2i(sy), f2i(sz));
+  }
+  // sort by (code, index): stable
+  std::vector<uint32_t> order(N);
+  for (uint32_t i = 0; i < N; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](uint32_t a, uint32_t c) { return b.codes_unsorted[a] < b.codes_unsorted[c]; });
+  b.sorted_codes.resize(N);
+  b.sorted_ids.resize(N);
+  for (uint32_t i = 0; i < N; i++) {
+    b.sorted_ids[i] = order[i];
+    b.sorted_codes[i] = b.codes_unsorted[order[i]];
+  }
+  // unique-code runs, lbvh.cpp:289-319
+  b.run_start.clear();
+  b.ucodes.clear();
+  for (uint32_t i = 0; i < N; i++) {
+    if (i == 0 || b.sorted_codes[i - 1] != b.sorted_codes[i]) {
+      b.run_start.push_back(i);
+      b.ucodes.push_back(b.sorted_codes[i]);
+    }
+  }
+  b.L = uint32_t(b.run_start.size());
+  const uint32_t L = b.L;
+  b.lr.assign(2 * size_t(L) - 1, Node{{0, 0, 0}, 0, {0, 0, 0}, 0});
+  b.perm.assign(N, 0);
+  b.temp_count.assign(2 * size_t(N) - 1, 0);
+  b.max_run = 0;
+  // leaves, lbvh.cpp:350-384
+  for (uint32_t i = 0; i < L; i++) {
+    const uint32_t start = b.run_start[i];
+    const uint32_t count = (i < L - 1) ? b.run_start[i + 1] - start : N - b.run_start[L - 1];
+    b.max_run = std::max(b.max_run, count);
+    f4 bl = {LBVH_INF, LBVH_INF, LBVH_INF, LBVH_INF}, bh = {-LBVH_INF, -LBVH_INF, -LBVH_INF, -LBVH_INF};
+    for (uint32_t j = start; j < start + count; j++) {
+      const uint32_t t = b.sorted_ids[j];
+      b.perm[j] = t;
+      const f4 v1 = b.verts[b.idx[3 * t + 0]], v2 = b.verts[b.idx[3 * t + 1]], v3 = b.verts[b.idx[3 * t + 2]];
+      bl = {mn(bl.x, mn(v1.x, mn(v2.x, v3.x))), mn(bl.y, mn(v1.y, mn(v2.y, v3.y))), mn(bl.z, mn(v1.z, mn(v2.z, v3.z))), 0};
+      bh = {mx(bh.x, mx(v1.x, mx(v2.x, v3.x))), mx(bh.y, mx(v1.y, mx(v2.y, v3.y))), mx(bh.z, mx(v1.z, mx(v2.z, v3.z))), 0};
+    }
+    Node& n = b.lr[L - 1 + i];
+    n.bmin = {bl.x, bl.y, bl.z};
+    n.bmax = {bh.x, bh.y, bh.z};
+    n.left = pack_leaf(start, count);
+    n.escape = 0xFFFFFFFFu;
+  }
+  karras(b);
+  refit(b);
+  build_peach(b);
+  // AppendTreeData: index buffer rewritten in leaf order (BVH2CommonRT_host.cpp:54-61)
+  b.idx_leaf_order.resize(3 * size_t(N));
+  for (uint32_t i = 0; i < N; i++)
+    for (int k = 0; k < 3; k++) b.idx_leaf_order[3 * i + k] = b.idx[3 * b.perm[i] + k];
+}
+
+// ---------------------------------------------------------------------------
+// Traversal
+// ---------------------------------------------------------------------------
+struct Hit {  // CRT_Hit (CrossRT.h:20-27), primId/instId/geomId as u32
+  float t;
+  uint32_t primId, instId, geomId;
+  float u, v;
+};
+
+static inline f3 safe_inverse(f3 d) {  // raytrace_common.h:57-65
+  const float eps = 1.0e-36f;
+  return {1.0f / (std::fabs(d.x) > eps ? d.x : std::copysign(eps, d.x)),
+          1.0f / (std::fabs(d.y) > eps ? d.y : std::copysign(eps, d.y)),
+          1.0f / (std::fabs(d.z) > eps ? d.z : std::copysign(eps, d.z))};
+}
+
+static inline void slab(f3 o, f3 inv, f3 bmin, f3 bmax, float& tmin_out, float& tmax_out) {  // raytrace_common.h:67-80
+  const float lo = inv.x * (bmin.x - o.x);
+  const float hi = inv.x * (bmax.x - o.x);
+  const float lo1 = inv.y * (bmin.y - o.y);
+  const float hi1 = inv.y * (bmax.y - o.y);
+  const float lo2 = inv.z * (bmin.z - o.z);
+  const float hi2 = inv.z * (bmax.z - o.z);
+  const float tmin = mx(mn(lo, hi), mn(lo1, hi1));
+  const float tmax = mn(mx(lo, hi), mx(lo1, hi1));
+  tmin_out = mx(tmin, mn(lo2, hi2));
+  tmax_out = mn(tmax, mx(lo2, hi2));
+}
+
+// Moller-Trumbore with the reference's 1e-6 slack, BVH2CommonRT.cpp:26-45.
+// Returns true and updates the hit when accepted.
+static inline bool mt_test(f3 o, f3 d, float tNear, f3 A, f3 B, f3 C, Hit& h, uint32_t prim) {
+  const f3 e1 = sub3(B, A);
+  const f3 e2 = sub3(C, A);
+  const f3 pv = cross3(d, e2);
+  const f3 tv = sub3(o, A);
+  const f3 qv = cross3(tv, e1);
+  const float invDet = 1.0f / dot3(e1, pv);
+  const float v = dot3(tv, pv) * invDet;
+  const float u = dot3(qv, d) * invDet;
+  const float t = dot3(e2, qv) * invDet;
+  if (v >= -1e-6f && u >= -1e-6f && (u + v <= 1.0f + 1e-6f) && t > tNear && t < h.t) {
+    h.t = t;
+    h.primId = prim;
+    h.instId = 0;
+    h.geomId = 0;
+    h.u = u;
+    h.v = v;
+    return true;
+  }
+  return false;
+}
+
+static inline f3 vpos(const Build& b, uint32_t vi) {
+  const f4 v = b.verts[vi];
+  return {v.x, v.y, v.z};
+}
+
+struct Counters { uint64_t pairs = 0, tris = 0, leaves = 0; };
+
+// BVH2CommonRT::BVH2TraverseC32 over the peach layout (BVH2CommonRT.cpp:49-141).
+// primId is reported as the ORIGINAL triangle id (the REMAP_PRIM_ID rule of
+// BVH2CommonRT.cpp:270-276); the leaf-order position is perm^-1 of it.
+static Hit trace_bvh2common(const Build& b, f4 posNear, f4 dirFar, Counters* cnt) {
+  bool anyhit = (dirFar.w <= 0.0f);
+  float tFar = anyhit ? -dirFar.w : dirFar.w;
+  Hit h;
+  h.t = tFar;
+  h.primId = h.instId = h.geomId = 0xFFFFFFFFu;
+  h.u = h.v = 0.0f;
+  const f3 o = {posNear.x, posNear.y, posNear.z};
+  const f3 d = {dirFar.x, dirFar.y, dirFar.z};
+  const float tNear = posNear.w;
+  const f3 inv = safe_inverse(d);
+  uint32_t stack[80];
+  int top = 0;
+  uint32_t off = 0;
+  const Node* nodes = b.peach.data();
+  while (top >= 0) {
+    bool pop = false;
+    if ((off & LEAF_BIT) == 0) {
+      const Node n0 = nodes[off + 0];
+      const Node n1 = nodes[off + 1];
+      if (cnt) cnt->pairs++;
+      float a0, b0, a1, b1;
+      slab(o, inv, n0.bmin, n0.bmax, a0, b0);
+      slab(o, inv, n1.bmin, n1.bmax, a1, b1);
+      const bool h0 = (a0 <= b0) && (b0 >= tNear) && (a0 <= h.t);
+      const bool h1 = (a1 <= b1) && (b1 >= tNear) && (a1 <= h.t);
+      pop = (!h0 && !h1);
+      off = h0 ? n0.left : n1.left;
+      if (h0 && h1) {
+        off = (a0 <= a1) ? n0.left : n1.left;
+        stack[top] = (a0 <= a1) ? n1.left : n0.left;
+        top++;
+      }
+    } else if (off != 0xFFFFFFFFu) {
+      const uint32_t start = leaf_start(off), count = leaf_count(off);
+      if (cnt) { cnt->leaves++; cnt->tris += count; }
+      for (uint32_t t = start; t < start + count; t++) {
+        const uint32_t* ix = &b.idx_leaf_order[3 * size_t(t)];
+        mt_test(o, d, tNear, vpos(b, ix[0]), vpos(b, ix[1]), vpos(b, ix[2]), h, b.perm[t]);
+      }
+      pop = true;
+      if (anyhit && h.primId != 0xFFFFFFFFu) break;
+    }
+    if (pop) {
+      top--;
+      off = stack[std::max(top, 0)];
+    }
+  }
+  return h;
+}
+
+// BVH2_LRFT-style traversal directly over the Karras left/right layout
+// (BVH2CommonLRFT.cpp:81-221 with a single BLAS and no TLAS).  Visits the
+// same (left, right) children in the same near-first order as the peach
+// traversal, so it must give bit-identical hits; the tests use that to show
+// the hits do not depend on the node layout.
+static Hit trace_lr(const Build& b, f4 posNear, f4 dirFar) {
+  bool anyhit = (dirFar.w <= 0.0f);
+  float tFar = anyhit ? -dirFar.w : dirFar.w;
+  Hit h;
+  h.t = tFar;
+  h.primId = h.instId = h.geomId = 0xFFFFFFFFu;
+  h.u = h.v = 0.0f;
+  const f3 o = {posNear.x, posNear.y, posNear.z};
+  const f3 d = {dirFar.x, dirFar.y, dirFar.z};
+  const float tNear = posNear.w;
+  const f3 inv = safe_inverse(d);
+  const Node* nodes = b.lr.data();
+  if (b.L == 1) {  // root is a leaf: its box is tested first, as in the peach layout
+    const uint32_t leaf = nodes[0].left;
+    float a0, b0;
+    slab(o, inv, nodes[0].bmin, nodes[0].bmax, a0, b0);
+    if (!((a0 <= b0) && (b0 >= tNear) && (a0 <= h.t))) return h;
+    for (uint32_t t = leaf_start(leaf); t < leaf_start(leaf) + leaf_count(leaf); t++) {
+      const uint32_t* ix = &b.idx_leaf_order[3 * size_t(t)];
+      mt_test(o, d, tNear, vpos(b, ix[0]), vpos(b, ix[1]), vpos(b, ix[2]), h, b.perm[t]);
+    }
+    return h;
+  }
+  uint32_t stack[80];
+  int top = 0;
+  uint32_t cur = 0;  // internal node whose two children are tested next
+  while (true) {
+    const Node& p = nodes[cur];
+    const Node n0 = nodes[p.left];
+    const Node n1 = nodes[p.escape];
+    float a0, b0, a1, b1;
+    slab(o, inv, n0.bmin, n0.bmax, a0, b0);
+    slab(o, inv, n1.bmin, n1.bmax, a1, b1);
+    const bool h0 = (a0 <= b0) && (b0 >= tNear) && (a0 <= h.t);
+    const bool h1 = (a1 <= b1) && (b1 >= tNear) && (a1 <= h.t);
+    uint32_t next = 0xFFFFFFFFu;
+    if (h0 && h1) {
+      next = (a0 <= a1) ? p.left : p.escape;
+      stack[top++] = (a0 <= a1) ? p.escape : p.left;
+    } else if (h0) {
+      next = p.left;
+    } else if (h1) {
+      next = p.escape;
+    }
+    // descend until an internal node is found or the stack runs out
+    while (true) {
+      if (next == 0xFFFFFFFFu) {
+        if (top == 0) return h;
+        next = stack[--top];
+      }
+      const uint32_t lo = nodes[next].left;
+      if (lo & LEAF_BIT) {
+        for (uint32_t t = leaf_start(lo); t < leaf_start(lo) + leaf_count(lo); t++) {
+          const uint32_t* ix = &b.idx_leaf_order[3 * size_t(t)];
+          mt_test(o, d, tNear, vpos(b, ix[0]), vpos(b, ix[1]), vpos(b, ix[2]), h, b.perm[t]);
+        }
+        if (anyhit && h.primId != 0xFFFFFFFFu) return h;
+        next = 0xFFFFFFFFu;
+        continue;
+      }
+      cur = next;
+      break;
+    }
+  }
+}
+
+// BruteForceRT::RayQuery_NearestHit for one geometry and one identity
+// instance (BruteForceRT.cpp:184-265): every triangle in input order, strict
+// t < hit.t, original triangle id.
+static Hit trace_brute(const Build& b, f4 posNear, f4 dirFar) {
+  bool anyhit = (dirFar.w <= 0.0f);
+  float tFar = anyhit ? -dirFar.w : dirFar.w;
+  Hit h;
+  h.t = tFar;
+  h.primId = h.instId = h.geomId = 0xFFFFFFFFu;
+  h.u = h.v = 0.0f;
+  const f3 o = {posNear.x, posNear.y, posNear.z};
+  const f3 d = {dirFar.x, dirFar.y, dirFar.z};
+  const float tNear = posNear.w;
+  // instance box test (BruteForceRT.cpp:196-201) against the geometry box
+  {
+    f3 lo = {b.bbmin.x, b.bbmin.y, b.bbmin.z}, hi = {b.bbmax.x, b.bbmax.y, b.bbmax.z};
+    float a, c;
+    slab(o, safe_inverse(d), lo, hi, a, c);
+    if (!((a <= c) && (c >= tNear) && (a <= tFar))) return h;
+  }
+  for (uint32_t t = 0; t < b.ntris; t++) {
+    mt_test(o, d, tNear, vpos(b, b.idx[3 * t]), vpos(b, b.idx[3 * t + 1]), vpos(b, b.idx[3 * t + 2]), h, t);
+    if (anyhit && h.primId != 0xFFFFFFFFu) break;
+  }
+  return h;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C entry points (loaded by tests/ and bench.py through ctypes)
+// ===========================================================================
+extern "C" {
+
+void* hrto_build(const float* vert4, uint32_t nVert, const uint32_t* idx, uint32_t nIdx) {
+  if (nIdx < 3) return nullptr;
+  Build* b = new Build();
+  b->verts.resize(nVert);
+  std::memcpy(b->verts.data(), vert4, sizeof(f4) * nVert);
+  for (auto& v : b->verts) v.w = 1.0f;  // BVH2CommonRT_host.cpp:81 stores w = 1
+  b->ntris = nIdx / 3;
+  b->idx.assign(idx, idx + 3 * size_t(b->ntris));
+  build_all(*b);
+  return b;
+}
+
+void hrto_free(void* h) { delete static_cast<Build*>(h); }
+
+// sizes: [ntris, leaves, lr_nodes, peach_nodes, max_run]
+void hrto_sizes(void* h, uint64_t* out5) {
+  const Build* b = static_cast<Build*>(h);
+  out5[0] = b->ntris;
+  out5[1] = b->L;
+  out5[2] = b->lr.size();
+  out5[3] = b->peach.size();
+  out5[4] = b->max_run;
+}
+
+// scene box (8 floats: min xyzw, max xyzw) and Morton scale (4 floats)
+void hrto_scene_box(void* h, float* out12) {
+  const Build* b = static_cast<Build*>(h);
+  std::memcpy(out12 + 0, &b->bbmin, 16);
+  std::memcpy(out12 + 4, &b->bbmax, 16);
+  std::memcpy(out12 + 8, &b->zscale, 16);
+}
+
+void hrto_codes(void* h, uint32_t* codes_unsorted, uint32_t* sorted_codes, uint32_t* sorted_ids) {
+  const Build* b = static_cast<Build*>(h);
+  std::memcpy(codes_unsorted, b->codes_unsorted.data(), 4 * size_t(b->ntris));
+  std::memcpy(sorted_codes, b->sorted_codes.data(), 4 * size_t(b->ntris));
+  std::memcpy(sorted_ids, b->sorted_ids.data(), 4 * size_t(b->ntris));
+}
+
+void hrto_runs(void* h, uint32_t* run_start, uint32_t* ucodes) {
+  const Build* b = static_cast<Build*>(h);
+  std::memcpy(run_start, b->run_start.data(), 4 * size_t(b->L));
+  std::memcpy(ucodes, b->ucodes.data(), 4 * size_t(b->L));
+}
+
+// Karras left/right layout: 8 words per node (bmin xyz, left, bmax xyz, right)
+void hrto_lr_nodes(void* h, uint32_t* out) {
+  const Build* b = static_cast<Build*>(h);
+  std::memcpy(out, b->lr.data(), sizeof(Node) * b->lr.size());
+}
+
+void hrto_peach_nodes(void* h, uint32_t* out) {
+  const Build* b = static_cast<Build*>(h);
+  std::memcpy(out, b->peach.data(), sizeof(Node) * b->peach.size());
+}
+
+void hrto_perm(void* h, uint32_t* out) {
+  const Build* b = static_cast<Build*>(h);
+  std::memcpy(out, b->perm.data(), 4 * size_t(b->ntris));
+}
+
+void hrto_temp_count(void* h, uint32_t* out) {
+  const Build* b = static_cast<Build*>(h);
+  std::memcpy(out, b->temp_count.data(), 4 * b->temp_count.size());
+}
+
+// mode: 0 = BVH2Common over the peach layout, 1 = left/right layout,
+//       2 = brute force.  rays: 8 floats each (posNear, dirFar).
+// hits: 6 words each (t, primId, instId, geomId, u, v).
+// counters (mode 0 only, may be null): per ray [pairs, tris].
+void hrto_trace(void* h, int mode, const float* rays, uint64_t n, uint32_t* hits, uint32_t* counters, int threads) {
+  const Build* b = static_cast<Build*>(h);
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 256) if (threads != 1)
+#endif
+  for (int64_t i = 0; i < int64_t(n); i++) {
+    f4 p, d;
+    std::memcpy(&p, rays + 8 * i, 16);
+    std::memcpy(&d, rays + 8 * i + 4, 16);
+    Hit r;
+    Counters c;
+    if (mode == 0) r = trace_bvh2common(*b, p, d, counters ? &c : nullptr);
+    else if (mode == 1) r = trace_lr(*b, p, d);
+    else r = trace_brute(*b, p, d);
+    std::memcpy(hits + 6 * i, &r, sizeof(Hit));
+    if (counters && mode == 0) {
+      counters[2 * i + 0] = uint32_t(c.pairs);
+      counters[2 * i + 1] = uint32_t(c.tris);
+    }
+  }
+}
+
+// FP64 Moller-Trumbore t for a given triangle (t-error reporting).
+void hrto_t_fp64(void* h, const float* rays, const uint32_t* prim, uint64_t n, double* out_t) {
+  const Build* b = static_cast<Build*>(h);
+  for (uint64_t i = 0; i < n; i++) {
+    if (prim[i] == 0xFFFFFFFFu) { out_t[i] = NAN; continue; }
+    const uint32_t* ix = &b->idx[3 * size_t(prim[i])];
+    double A[3], B[3], C[3], o[3], d[3];
+    for (int k = 0; k < 3; k++) {
+      A[k] = (&b->verts[ix[0]].x)[k];
+      B[k] = (&b->verts[ix[1]].x)[k];
+      C[k] = (&b->verts[ix[2]].x)[k];
+      o[k] = rays[8 * i + k];
+      d[k] = rays[8 * i + 4 + k];
+    }
+    double e1[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
+    double e2[3] = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
+    double tv[3] = {o[0] - A[0], o[1] - A[1], o[2] - A[2]};
+    double pv[3] = {d[1] * e2[2] - d[2] * e2[1], d[2] * e2[0] - d[0] * e2[2], d[0] * e2[1] - d[1] * e2[0]};
+    double qv[3] = {tv[1] * e1[2] - tv[2] * e1[1], tv[2] * e1[0] - tv[0] * e1[2], tv[0] * e1[1] - tv[1] * e1[0]};
+    double det = e1[0] * pv[0] + e1[1] * pv[1] + e1[2] * pv[2];
+    out_t[i] = (e2[0] * qv[0] + e2[1] * qv[1] + e2[2] * qv[2]) / det;
+  }
+}
+
+// Reference building blocks exposed for known-answer tests.
+uint32_t hrto_zindex3d(int x, int y, int z) { return zindex3d(x, y, z); }
+uint32_t hrto_pack_leaf(uint32_t start, uint32_t size) { return pack_leaf(start, size); }
+void hrto_safe_inverse(const float* d3, float* out3) {
+  f3 r = safe_inverse(f3{d3[0], d3[1], d3[2]});
+  out3[0] = r.x; out3[1] = r.y; out3[2] = r.z;
+}
+void hrto_slab(const float* o3, const float* inv3, const float* bmin3, const float* bmax3, float* out2) {
+  slab(f3{o3[0], o3[1], o3[2]}, f3{inv3[0], inv3[1], inv3[2]}, f3{bmin3[0], bmin3[1], bmin3[2]},
+       f3{bmax3[0], bmax3[1], bmax3[2]}, out2[0], out2[1]);
+}
+
+}  // extern "C"

@@ -1,0 +1,174 @@
+"""Python view of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg may
+import this module, and only as the checker.  The engine never calls it.
+
+The oracle is a C++ restatement of the reference's LBVH builder and
+BVH2CommonRT / BruteForceRT traversal (see hrt_oracle.cpp for the file:line
+map).  This wrapper loads ``oracle/_build/libhrt_oracle.so`` (built by
+``make -C oracle``) and returns numpy arrays.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "libhrt_oracle.so")
+_lib = None
+
+MODE_BVH2COMMON = 0
+MODE_LR = 1
+MODE_BRUTE = 2
+
+
+def build_library() -> str:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {_LIB_PATH} (run `make -C oracle`)")
+        L = C.CDLL(_LIB_PATH)
+        vp, u32, u64 = C.c_void_p, C.c_uint32, C.c_uint64
+        L.hrto_build.restype = vp
+        L.hrto_build.argtypes = [vp, u32, vp, u32]
+        L.hrto_free.argtypes = [vp]
+        L.hrto_sizes.argtypes = [vp, vp]
+        L.hrto_scene_box.argtypes = [vp, vp]
+        L.hrto_codes.argtypes = [vp, vp, vp, vp]
+        L.hrto_runs.argtypes = [vp, vp, vp]
+        L.hrto_lr_nodes.argtypes = [vp, vp]
+        L.hrto_peach_nodes.argtypes = [vp, vp]
+        L.hrto_perm.argtypes = [vp, vp]
+        L.hrto_temp_count.argtypes = [vp, vp]
+        L.hrto_trace.argtypes = [vp, C.c_int, vp, u64, vp, vp, C.c_int]
+        L.hrto_t_fp64.argtypes = [vp, vp, vp, u64, vp]
+        L.hrto_zindex3d.restype = u32
+        L.hrto_zindex3d.argtypes = [C.c_int, C.c_int, C.c_int]
+        L.hrto_pack_leaf.restype = u32
+        L.hrto_pack_leaf.argtypes = [u32, u32]
+        L.hrto_safe_inverse.argtypes = [vp, vp]
+        L.hrto_slab.argtypes = [vp, vp, vp, vp, vp]
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def as_vert4(verts: np.ndarray) -> np.ndarray:
+    """float32 (n,3) or (n,4) -> contiguous (n,4) with w = 1."""
+    v = np.asarray(verts, dtype=np.float32)
+    out = np.ones((v.shape[0], 4), dtype=np.float32)
+    out[:, :3] = v[:, :3]
+    return out
+
+
+class OracleBVH:
+    """One LBVH build of one triangle mesh, with the reference's traversals."""
+
+    def __init__(self, verts: np.ndarray, idx: np.ndarray):
+        self.v4 = as_vert4(verts)
+        self.idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1)
+        L = lib()
+        self._h = L.hrto_build(_ptr(self.v4), self.v4.shape[0], _ptr(self.idx), self.idx.size)
+        if not self._h:
+            raise ValueError("oracle build needs at least one triangle")
+        s = np.zeros(5, dtype=np.uint64)
+        L.hrto_sizes(self._h, _ptr(s))
+        self.ntris, self.leaves, self.n_lr, self.n_peach, self.max_run = (int(x) for x in s)
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.hrto_free(self._h)
+            self._h = None
+
+    def scene_box(self):
+        out = np.zeros(12, dtype=np.float32)
+        lib().hrto_scene_box(self._h, _ptr(out))
+        return out[0:4].copy(), out[4:8].copy(), out[8:12].copy()
+
+    def codes(self):
+        n = self.ntris
+        a, b, c = (np.zeros(n, dtype=np.uint32) for _ in range(3))
+        lib().hrto_codes(self._h, _ptr(a), _ptr(b), _ptr(c))
+        return a, b, c
+
+    def runs(self):
+        a = np.zeros(self.leaves, dtype=np.uint32)
+        b = np.zeros(self.leaves, dtype=np.uint32)
+        lib().hrto_runs(self._h, _ptr(a), _ptr(b))
+        return a, b
+
+    def lr_nodes(self) -> np.ndarray:
+        """(2L-1, 8) uint32 words: bmin xyz, left, bmax xyz, right."""
+        out = np.zeros((self.n_lr, 8), dtype=np.uint32)
+        lib().hrto_lr_nodes(self._h, _ptr(out))
+        return out
+
+    def peach_nodes(self) -> np.ndarray:
+        out = np.zeros((self.n_peach, 8), dtype=np.uint32)
+        lib().hrto_peach_nodes(self._h, _ptr(out))
+        return out
+
+    def perm(self) -> np.ndarray:
+        out = np.zeros(self.ntris, dtype=np.uint32)
+        lib().hrto_perm(self._h, _ptr(out))
+        return out
+
+    def temp_count(self) -> np.ndarray:
+        out = np.zeros(2 * self.ntris - 1, dtype=np.uint32)
+        lib().hrto_temp_count(self._h, _ptr(out))
+        return out
+
+    def trace(self, rays: np.ndarray, mode: int = MODE_BVH2COMMON, counters: bool = False, threads: int = 0):
+        """rays: (n,8) float32 = posNear, dirFar.  Returns (t, primId, u, v[, pairs, tris])."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        hits = np.zeros((n, 6), dtype=np.uint32)
+        cnt = np.zeros((n, 2), dtype=np.uint32) if counters else None
+        lib().hrto_trace(self._h, mode, _ptr(rays), n, _ptr(hits), _ptr(cnt) if counters else None, threads)
+        t = hits[:, 0].view(np.float32).copy()
+        prim = hits[:, 1].copy()
+        u = hits[:, 4].view(np.float32).copy()
+        v = hits[:, 5].view(np.float32).copy()
+        if counters:
+            return t, prim, u, v, cnt[:, 0].copy(), cnt[:, 1].copy()
+        return t, prim, u, v
+
+    def t_fp64(self, rays: np.ndarray, prim: np.ndarray) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        prim = np.ascontiguousarray(prim, dtype=np.uint32)
+        out = np.zeros(rays.shape[0], dtype=np.float64)
+        lib().hrto_t_fp64(self._h, _ptr(rays), _ptr(prim), rays.shape[0], _ptr(out))
+        return out
+
+
+def zindex3d(x: int, y: int, z: int) -> int:
+    return int(lib().hrto_zindex3d(x, y, z))
+
+
+def pack_leaf(start: int, size: int) -> int:
+    return int(lib().hrto_pack_leaf(start, size))
+
+
+def safe_inverse(d) -> np.ndarray:
+    d = np.asarray(d, dtype=np.float32)
+    out = np.zeros(3, dtype=np.float32)
+    lib().hrto_safe_inverse(_ptr(d), _ptr(out))
+    return out
+
+
+def slab(o, inv, bmin, bmax) -> np.ndarray:
+    args = [np.asarray(a, dtype=np.float32) for a in (o, inv, bmin, bmax)]
+    out = np.zeros(2, dtype=np.float32)
+    lib().hrto_slab(*(_ptr(a) for a in args), _ptr(out))
+    return out
