@@ -1,21 +1,25 @@
 """Ray-query throughput on one MI355X: the RTAO workload.
 
 Workload (one step): a 1024x512-quad heightfield (value noise, amp 0.2) plus
-2,000 randomly rotated boxes = 1,072,576 triangles in one LBVH2; 1024x1024
-primary rays traced nearest-hit; 16 cosine-weighted AO rays per primary hit
-made on the device (tMax = 0.1 x scene diagonal); all AO rays traced any-hit.
-Rays and scene are resident in HBM before the timed region starts.
+2,000 randomly rotated boxes = 1,072,576 triangles in one LBVH; 1024x1024
+primary rays traced nearest-hit (BVH2 node pairs, the reference's visiting
+order, bit-exact with BVH2CommonRT); 16 cosine-weighted AO rays per primary
+hit made on the device (tMax = 0.1 x scene diagonal); all AO rays traced
+any-hit on the BVH4 collapse of the same tree (any-hit answers do not depend
+on the node layout; the tests check them against the oracle).  Scene and rays
+are resident in HBM before the timed region starts.
 
 Prints ONE JSON line.  `value` is total rays (primary + AO) per second over
 the timed steps.  `roofline` is for the dominant kernel (the any-hit
-traversal): algorithmic bytes per launch = sum over rays of
-32 (ray) + 1 (result) + 64 x node pairs visited + 60 x triangles tested,
-the per-ray figure the reference's bus-load metric defines
-(core/CrossRT.h:52, core/BVH2CommonRT.cpp:69-70,125), with the visit and test
-counts taken from the engine's counting variant of the same kernel; divided
-by the kernel's mean duration, measured with HIP events on the stream the
-kernel runs on.  `cpu_baseline` is the CPU oracle (a C++ restatement of the
-reference's BVH2CommonRT traversal) timed on this host on the same rays.
+traversal): algorithmic bytes per launch = sum over rays of 32 (ray) + 1
+(result) + node bytes per visit x nodes visited + 60 x triangles tested --
+the per-ray figure of the reference's bus-load metric (core/CrossRT.h:52,
+core/BVH2CommonRT.cpp:69-70,125), with 64 B per BVH2 pair, 128 B per FP32 BVH4
+node, 64 B per FP16 BVH4 node; visit and test counts come from the engine's
+counting variant of the same traversal -- divided by the kernel's mean
+duration, measured with HIP events on the stream the kernel runs on.
+`cpu_baseline` is the CPU oracle (a C++ restatement of the reference's
+BVH2CommonRT traversal) timed on this host on the same rays.
 """
 from __future__ import annotations
 
@@ -34,6 +38,7 @@ if ROOT not in sys.path:
 HBM_PEAK_GBPS = 8000.0
 SPP = 16
 SEED = 0x2545F4914F6CDD1D
+NODE_BYTES = {"bvh2": 64, "bvh4": 128, "bvh4_fp16": 64}
 
 
 def log(msg: str) -> None:
@@ -45,7 +50,10 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--any-layout", choices=["bvh2", "bvh4", "bvh4_fp16"], default="bvh4",
+                    help="node layout for the AO any-hit rays")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-variants", action="store_true", help="skip timing the other any-hit layouts")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, os.cpu_count())")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -63,6 +71,7 @@ def main() -> int:
         return 1
     torch.cuda.init()
     dev = torch.device("cuda:0")
+    layouts = {"bvh2": H.LAYOUT_BVH2, "bvh4": H.LAYOUT_BVH4, "bvh4_fp16": H.LAYOUT_BVH4_FP16}
 
     log("generating scene")
     verts, faces = W.rtao_scene()
@@ -78,10 +87,14 @@ def main() -> int:
     gid = scene.add_triangles(verts, faces)
     build_ms = (time.perf_counter() - t0) * 1e3
     scene.add_instance(gid)
-    scene.commit()
-    log(f"build {faces.shape[0]} tris: {build_ms:.2f} ms (host->device copy + LBVH)")
+    t0 = time.perf_counter()
+    scene.commit(layout=H.LAYOUT_BVH4)  # BVH4 collapse (FP32 + FP16 nodes)
+    collapse_ms = (time.perf_counter() - t0) * 1e3
+    log(f"build {faces.shape[0]} tris: {build_ms:.2f} ms (host copy + LBVH), BVH4 collapse {collapse_ms:.2f} ms")
 
-    stream = torch.cuda.current_stream()
+    # one explicit stream (default priority) for every copy, kernel and event
+    stream = torch.cuda.Stream(device=dev)
+    torch.cuda.set_stream(stream)
     sptr = stream.cuda_stream
     d_prim = torch.from_numpy(prim_rays).to(dev)
     d_hits = torch.zeros((npix, 2), dtype=torch.int32, device=dev)
@@ -89,10 +102,18 @@ def main() -> int:
     d_vis = torch.zeros(n_ao, dtype=torch.uint8, device=dev)
     torch.cuda.synchronize()
 
+    def set_any_layout(name: str) -> None:
+        # nearest hits always run on BVH2 in the reference's order; AUTO adds BVH4 any-hit
+        scene.commit(layout=H.LAYOUT_BVH2 if name == "bvh2" else
+                     (H.LAYOUT_AUTO if name == "bvh4" else layouts[name]))
+
+    def nearest():
+        scene.trace_nearest(d_prim, npix, d_hits, H.HIT_COMPACT, stream=sptr)
+
     def step(ev=None):
         if ev is not None:
             ev[0].record(stream)
-        scene.trace_nearest(d_prim, npix, d_hits, H.HIT_COMPACT, stream=sptr)
+        nearest()
         if ev is not None:
             ev[1].record(stream)
         scene.rtao_make_rays(d_prim, d_hits, npix, SPP, tmax, SEED, d_ao, stream=sptr)
@@ -102,6 +123,24 @@ def main() -> int:
         if ev is not None:
             ev[3].record(stream)
 
+    # the FP16 layout routes nearest hits through BVH4 too; keep them on BVH2
+    if args.any_layout == "bvh4_fp16":
+        def step(ev=None):  # noqa: F811
+            if ev is not None:
+                ev[0].record(stream)
+            scene.commit(layout=H.LAYOUT_BVH2)
+            nearest()
+            if ev is not None:
+                ev[1].record(stream)
+            scene.rtao_make_rays(d_prim, d_hits, npix, SPP, tmax, SEED, d_ao, stream=sptr)
+            if ev is not None:
+                ev[2].record(stream)
+            scene.commit(layout=H.LAYOUT_BVH4_FP16)
+            scene.trace_any(d_ao, n_ao, d_vis, stream=sptr)
+            if ev is not None:
+                ev[3].record(stream)
+
+    set_any_layout(args.any_layout)
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -121,23 +160,49 @@ def main() -> int:
     rays_per_step = npix + n_ao
     value = rays_per_step / (ms_per_step * 1e-3) / 1e6
     log(f"step {ms_per_step:.3f} ms wall; nearest {near_ms.mean():.3f} ms, raygen {gen_ms.mean():.3f} ms, "
-        f"any-hit {any_ms.mean():.3f} ms")
+        f"any-hit[{args.any_layout}] {any_ms.mean():.3f} ms")
 
-    # traversal counters (counting variant of the same kernels, untimed)
-    st_prim = torch.zeros((npix, 2), dtype=torch.int32, device=dev)
-    st_ao = torch.zeros((n_ao, 2), dtype=torch.int32, device=dev)
-    hits_chk = torch.zeros_like(d_hits)
-    vis_chk = torch.zeros_like(d_vis)
-    scene.trace_nearest(d_prim, npix, hits_chk, H.HIT_COMPACT, stream=sptr, stats=st_prim)
-    scene.trace_any(d_ao, n_ao, vis_chk, stream=sptr, stats=st_ao)
-    torch.cuda.synchronize()
-    sp = st_prim.cpu().numpy().view(np.uint32).astype(np.int64)
-    sa = st_ao.cpu().numpy().view(np.uint32).astype(np.int64)
-    same = bool(torch.equal(hits_chk, d_hits) and torch.equal(vis_chk, d_vis))
-    bytes_any = float(np.sum(32 + 1 + 64 * sa[:, 0] + 60 * sa[:, 1]))
-    bytes_near = float(np.sum(32 + 8 + 64 * sp[:, 0] + 60 * sp[:, 1]))
+    # traversal counters (counting variant, untimed)
+    def counters(layout_name: str, any_hit: bool):
+        scene.commit(layout=layouts[layout_name])
+        st = torch.zeros(((n_ao if any_hit else npix), 2), dtype=torch.int32, device=dev)
+        if any_hit:
+            chk = torch.zeros_like(d_vis)
+            scene.trace_any(d_ao, n_ao, chk, stream=sptr, stats=st)
+        else:
+            chk = torch.zeros_like(d_hits)
+            scene.trace_nearest(d_prim, npix, chk, H.HIT_COMPACT, stream=sptr, stats=st)
+        torch.cuda.synchronize()
+        same = bool(torch.equal(chk, d_vis if any_hit else d_hits))
+        return st.cpu().numpy().view(np.uint32).astype(np.int64), same
+
+    sp, same_p = counters("bvh2", False)
+    sa, same_a = counters(args.any_layout, True)
+    bytes_any = float(np.sum(33 + NODE_BYTES[args.any_layout] * sa[:, 0] + 60 * sa[:, 1]))
+    bytes_near = float(np.sum(40 + 64 * sp[:, 0] + 60 * sp[:, 1]))
     achieved_any = bytes_any / (any_ms.mean() * 1e-3) / 1e9
     achieved_near = bytes_near / (near_ms.mean() * 1e-3) / 1e9
+
+    # the other any-hit layouts, same rays, same answers
+    variants = {}
+    if not args.no_variants:
+        ref_vis = d_vis.clone()
+        for name in ("bvh2", "bvh4", "bvh4_fp16"):
+            scene.commit(layout=layouts[name])
+            out = torch.zeros_like(d_vis)
+            for _ in range(2):
+                scene.trace_any(d_ao, n_ao, out, stream=sptr)
+            evs = [torch.cuda.Event(enable_timing=True) for _ in range(6)]
+            for k in range(5):
+                evs[k].record(stream)
+                scene.trace_any(d_ao, n_ao, out, stream=sptr)
+            evs[5].record(stream)
+            torch.cuda.synchronize()
+            ms = np.array([evs[k].elapsed_time(evs[k + 1]) for k in range(5)])
+            variants[name] = {"any_hit_ms": round(float(ms.mean()), 4),
+                              "any_hit_mrays_s": round(n_ao / (ms.mean() * 1e-3) / 1e6, 1),
+                              "identical_to_timed": bool(torch.equal(out, ref_vis))}
+            log(f"any-hit variant {name}: {ms.mean():.3f} ms")
 
     result = {
         "metric": "Mrays/s closest/any-hit on 1 MI355X, % HBM roofline; p50/p99 query latency",
@@ -153,16 +218,17 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "rtao: 1024x1024 primary nearest-hit + 16 AO any-hit rays per hit, "
-                        "1,072,576-tri heightfield+boxes, LBVH2",
+            "workload": "rtao: 1024x1024 primary nearest-hit (LBVH2, reference order) + 16 AO any-hit rays per "
+                        f"hit ({args.any_layout} of the same LBVH), 1,072,576-tri heightfield+boxes",
             "triangles": int(faces.shape[0]),
             "primary_rays": int(npix),
             "ao_rays": int(n_ao),
             "ao_tmax": round(tmax, 6),
+            "any_hit_layout": args.any_layout,
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_trace_bvh2<any-hit>",
+            "kernel": f"any-hit traversal ({args.any_layout})",
             "achieved": round(achieved_any, 1),
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
@@ -176,10 +242,13 @@ def main() -> int:
         "nearest_roofline_frac": round(achieved_near / HBM_PEAK_GBPS, 4),
         "latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 4),
                        "p99": round(float(np.percentile(step_ms, 99)), 4)},
-        "avg_pairs_per_ray": {"primary": round(float(sp[:, 0].mean()), 3), "ao": round(float(sa[:, 0].mean()), 3)},
-        "avg_tris_per_ray": {"primary": round(float(sp[:, 1].mean()), 3), "ao": round(float(sa[:, 1].mean()), 3)},
+        "visits_per_ray": {"primary_pairs": round(float(sp[:, 0].mean()), 3),
+                           f"ao_{args.any_layout}_nodes": round(float(sa[:, 0].mean()), 3)},
+        "tris_per_ray": {"primary": round(float(sp[:, 1].mean()), 3), "ao": round(float(sa[:, 1].mean()), 3)},
         "build_ms": round(build_ms, 3),
-        "repeat_identical": same,
+        "bvh4_collapse_ms": round(collapse_ms, 3),
+        "counting_variant_identical": same_p and same_a,
+        "any_hit_layouts": variants,
         "cpu_baseline": None,
     }
 

@@ -12,8 +12,10 @@ calls raise.
 """
 from __future__ import annotations
 
+import atexit
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -23,6 +25,9 @@ LIB_PATH = os.path.join(_HERE, "libhrt.so")
 HIT_FULL = 0
 HIT_COMPACT = 1
 LAYOUT_BVH2 = 0
+LAYOUT_BVH4 = 1
+LAYOUT_BVH4_FP16 = 2
+LAYOUT_AUTO = 3
 
 DUMP_SIZES = 0
 DUMP_CODES_INPUT = 1
@@ -33,6 +38,8 @@ DUMP_UNIQUE_CODES = 5
 DUMP_LR_NODES = 6
 DUMP_PERM = 7
 DUMP_SCENE_BOX = 8
+DUMP_QUADS32 = 9
+DUMP_QUADS16 = 10
 
 STATUS_NAMES = {
     0: "HRT_OK",
@@ -61,6 +68,18 @@ class HrtError(RuntimeError):
 
 
 _lib = None
+# Native objects still alive; closed (scenes first) before the interpreter
+# exits, while the HIP runtime is still loaded.
+_live_scenes: "weakref.WeakSet" = weakref.WeakSet()
+_live_contexts: "weakref.WeakSet" = weakref.WeakSet()
+
+
+@atexit.register
+def _close_all() -> None:
+    for s in list(_live_scenes):
+        s.close()
+    for c in list(_live_contexts):
+        c.close()
 
 
 def lib():
@@ -124,6 +143,7 @@ class Context:
         self._h = C.c_void_p()
         _check(lib().hrt_init(device, C.byref(self._h)))
         self.device = device
+        _live_contexts.add(self)
 
     @property
     def stream(self) -> int:
@@ -148,6 +168,7 @@ class Scene:
         self.ctx = ctx
         self._h = C.c_void_p()
         _check(lib().hrt_scene_create(ctx._h, C.byref(self._h)))
+        _live_scenes.add(self)
         if keep_build_arrays:
             _check(lib().hrt_scene_set_keep_build_arrays(self._h, 1))
 
@@ -226,12 +247,16 @@ class Scene:
 
     # -- parity hooks -------------------------------------------------------
     def dump(self, geom_id: int, what: int) -> np.ndarray:
-        sizes = np.zeros(4, dtype=np.uint32)
+        sizes = np.zeros(5, dtype=np.uint32)
         _check(lib().hrt_debug_dump(self._h, geom_id, DUMP_SIZES, _ptr(sizes), sizes.nbytes))
         if what == DUMP_SIZES:
             return sizes
-        n, L = int(sizes[0]), int(sizes[1])
-        if what == DUMP_SCENE_BOX:
+        n, L, nq = int(sizes[0]), int(sizes[1]), int(sizes[4])
+        if what == DUMP_QUADS32:
+            out = np.zeros((nq, 32), dtype=np.uint32)
+        elif what == DUMP_QUADS16:
+            out = np.zeros((nq, 16), dtype=np.uint32)
+        elif what == DUMP_SCENE_BOX:
             out = np.zeros(12, dtype=np.float32)
         elif what in (DUMP_RUN_START, DUMP_UNIQUE_CODES):
             out = np.zeros(L, dtype=np.uint32)

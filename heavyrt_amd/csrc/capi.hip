@@ -58,6 +58,7 @@ struct hrt_scene {
   std::vector<Instance> instances;
   bool committed = false;
   bool keep_debug = false;
+  int layout = HRT_LAYOUT_BVH2;
   // staging for hrt_query_nearest_one
   std::mutex one_mu;
   hrt::Ray* one_ray = nullptr;
@@ -84,6 +85,7 @@ hrt_status trace_common(hrt_scene* s, const float* rays, size_t n, void* out, in
   if (st != HRT_OK) return st;
   if (n == 0) return HRT_OK;
   if (!rays || !out) return fail(HRT_ERR_INVALID_ARG, "rays/out is null");
+  if (n > 0x7FFFFFFFull) return fail(HRT_ERR_INVALID_ARG, "at most 2^31-1 rays per call");
   if (!anyhit && format != HRT_HIT_FULL && format != HRT_HIT_COMPACT)
     return fail(HRT_ERR_INVALID_ARG, "unknown hit format");
   hrt::TraceArgs a;
@@ -95,7 +97,13 @@ hrt_status trace_common(hrt_scene* s, const float* rays, size_t n, void* out, in
   a.stats = stats;
   hipStream_t hs = stream ? static_cast<hipStream_t>(stream) : s->ctx->stream;
   hipError_t e = hipSetDevice(s->ctx->device);
-  if (e == hipSuccess) e = hrt::trace_blas(*b, a, hs);
+  if (e == hipSuccess) {
+    const int lay = s->layout;
+    if (lay == HRT_LAYOUT_BVH2 || (lay == HRT_LAYOUT_AUTO && !anyhit))
+      e = hrt::trace_blas(*b, a, hs);
+    else
+      e = hrt::trace_blas4(*b, a, lay == HRT_LAYOUT_BVH4_FP16, hs);
+  }
   if (e != hipSuccess) return fail_hip(e, "trace");
   return HRT_OK;
 }
@@ -242,7 +250,24 @@ hrt_status hrt_add_instance(hrt_scene* s, uint32_t geomId, const float colMajor1
 hrt_status hrt_commit(hrt_scene* s, int quality, int layout) {
   (void)quality;
   if (!s) return fail(HRT_ERR_INVALID_ARG, "scene is null");
-  if (layout != HRT_LAYOUT_BVH2) return fail(HRT_ERR_UNSUPPORTED, "unknown layout");
+  if (layout < HRT_LAYOUT_BVH2 || layout > HRT_LAYOUT_AUTO) return fail(HRT_ERR_INVALID_ARG, "unknown layout");
+  s->committed = false;
+  if (layout != HRT_LAYOUT_BVH2) {
+    hipError_t e = hipSetDevice(s->ctx->device);
+    if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
+    for (const Instance& in : s->instances) {
+      hrt::Blas& b = s->geoms[in.geom];
+      if (layout == HRT_LAYOUT_BVH4_FP16) {
+        // FP16 boxes are only conservative inside the half range (cbvh_fat.cpp:447-448
+        // maps every value beyond +-65504 to +-65504)
+        for (int k = 0; k < 3; k++)
+          if (!(b.box[k] >= -65504.0f && b.box[4 + k] <= 65504.0f))
+            return fail(HRT_ERR_UNSUPPORTED, "FP16 nodes need every coordinate within +-65504");
+      }
+      if ((e = hrt::blas_build_bvh4(b, s->ctx->stream)) != hipSuccess) return fail_hip(e, "BVH4 collapse");
+    }
+  }
+  s->layout = layout;
   s->committed = true;
   return HRT_OK;
 }
@@ -327,9 +352,9 @@ hrt_status hrt_debug_dump(hrt_scene* s, uint32_t geomId, int what, void* host_ou
   hipError_t e = hipSetDevice(s->ctx->device);
   if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
   if (what == HRT_DUMP_SIZES) {
-    if (need(16)) return HRT_ERR_INVALID_ARG;
-    uint32_t v[4] = {N, L, b.max_run, npairs};
-    std::memcpy(host_out, v, 16);
+    if (need(20)) return HRT_ERR_INVALID_ARG;
+    uint32_t v[5] = {N, L, b.max_run, npairs, b.nquads};
+    std::memcpy(host_out, v, 20);
     return HRT_OK;
   }
   if (what == HRT_DUMP_SCENE_BOX) {
@@ -337,6 +362,14 @@ hrt_status hrt_debug_dump(hrt_scene* s, uint32_t geomId, int what, void* host_ou
     std::memcpy(host_out, b.box, 32);
     std::memcpy(static_cast<char*>(host_out) + 32, b.zscale, 16);
     return HRT_OK;
+  }
+  if (what == HRT_DUMP_QUADS32 || what == HRT_DUMP_QUADS16) {
+    if (!b.quads32) return fail(HRT_ERR_INVALID_ARG, "no BVH4 nodes (commit with a BVH4 layout first)");
+    const size_t n = (what == HRT_DUMP_QUADS32 ? sizeof(hrt::Quad32) : sizeof(hrt::Quad16)) * size_t(b.nquads);
+    if (need(n)) return HRT_ERR_INVALID_ARG;
+    e = hipMemcpy(host_out, what == HRT_DUMP_QUADS32 ? static_cast<const void*>(b.quads32) : b.quads16, n,
+                  hipMemcpyDeviceToHost);
+    return e == hipSuccess ? HRT_OK : fail_hip(e, "dump");
   }
   if (what == HRT_DUMP_PERM) {
     if (need(4ull * N)) return HRT_ERR_INVALID_ARG;

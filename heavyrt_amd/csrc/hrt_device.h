@@ -60,6 +60,30 @@ struct __attribute__((aligned(16))) TriPre {
   float4 e2;  // xyz = C - A
 };
 
+// BVH4 node with FP32 boxes, 128 bytes, coordinates of the four children
+// grouped per axis so each 16-byte load feeds four slab tests.  Children are
+// the grandchildren of one even-depth Karras node in left-to-right order;
+// an unused slot has the inverted box (+1e9, -1e9) and the empty leaf
+// 0x80000000 (cbvh_core.cpp:402-409).  An internal child reference is a quad
+// index.
+struct __attribute__((aligned(128))) Quad32 {
+  float4 lox, hix, loy, hiy, loz, hiz;
+  uint4 ref;
+  uint4 pad;
+};
+
+// BVH4 node with FP16 boxes, 64 bytes: per child one uint4 as the reference's
+// BVH4HalfRT stores it (half2 {minx, miny}, {minz, maxx}, {maxy, maxz}, ref;
+// cbvh_fat.cpp:477-497), minima rounded to nearest then one ulp down, maxima
+// one ulp up (cbvh_fat.cpp:447-448).
+struct __attribute__((aligned(64))) Quad16 {
+  uint4 c[4];
+};
+
+// Deepest BVH4 path: at most 15 node levels (the even depths of a BVH2 of at
+// most 30 internal levels), at most 3 deferred children per level.
+constexpr int kStackDepth4 = 48;
+
 struct Ray {  // 32-byte ray record: posAndNear, dirAndFar (CrossRT.h:143)
   float4 pos_near;
   float4 dir_far;
@@ -126,5 +150,62 @@ __device__ inline bool mt_test(float3 o, float3 d, float tnear, const TriPre& tr
   }
   return false;
 }
+
+// Same test with the (t, primId) rule of the BVH4 path: an equal-distance
+// triangle wins when its original id is smaller, so the nearest hit does not
+// depend on the order nodes are visited in.  t == tFar with no hit yet stays
+// outside the open interval.
+__device__ inline bool mt_test_lex(float3 o, float3 d, float tnear, const TriPre& tri, HitRec& h) {
+  const float3 A = make_float3(tri.a.x, tri.a.y, tri.a.z);
+  const float3 e1 = make_float3(tri.e1.x, tri.e1.y, tri.e1.z);
+  const float3 e2 = make_float3(tri.e2.x, tri.e2.y, tri.e2.z);
+  const float px = d.y * e2.z - d.z * e2.y;
+  const float py = d.z * e2.x - d.x * e2.z;
+  const float pz = d.x * e2.y - d.y * e2.x;
+  const float tx = o.x - A.x;
+  const float ty = o.y - A.y;
+  const float tz = o.z - A.z;
+  const float qx = ty * e1.z - tz * e1.y;
+  const float qy = tz * e1.x - tx * e1.z;
+  const float qz = tx * e1.y - ty * e1.x;
+  const float invDet = 1.0f / (e1.x * px + e1.y * py + e1.z * pz);
+  const float v = (tx * px + ty * py + tz * pz) * invDet;
+  const float u = (qx * d.x + qy * d.y + qz * d.z) * invDet;
+  const float t = (e2.x * qx + e2.y * qy + e2.z * qz) * invDet;
+  const uint32_t prim = __float_as_uint(tri.a.w);
+  if (v >= -1e-6f && u >= -1e-6f && (u + v <= 1.0f + 1e-6f) && t > tnear &&
+      (t < h.t || (t == h.t && h.prim != kInvalid && prim < h.prim))) {
+    h.t = t;
+    h.prim = prim;
+    h.u = u;
+    h.v = v;
+    return true;
+  }
+  return false;
+}
+
+// FP16 box coordinate -> FP32 (exact).
+__device__ inline float h2f_lo(uint32_t w) {
+  return float(__builtin_bit_cast(_Float16, uint16_t(w & 0xFFFFu)));
+}
+__device__ inline float h2f_hi(uint32_t w) { return float(__builtin_bit_cast(_Float16, uint16_t(w >> 16))); }
+
+// FP32 -> FP16 bits, round to nearest even (v_cvt_f16_f32 in the default mode).
+__device__ inline uint32_t f2h_rne(float f) { return __builtin_bit_cast(uint16_t, _Float16(f)); }
+
+// half.hpp nextafter (half.hpp:4291-4308) for non-NaN halves.
+__device__ inline uint32_t h_nextafter(uint32_t from, uint32_t to) {
+  const uint32_t fa = from & 0x7FFFu, ta = to & 0x7FFFu;
+  if (fa > 0x7C00u || ta > 0x7C00u) return from;
+  if (from == to || !(fa | ta)) return to;
+  if (!fa) return (to & 0x8000u) + 1u;
+  const uint32_t of = from ^ (0x8000u | (0x8000u - (from >> 15)));
+  const uint32_t ot = to ^ (0x8000u | (0x8000u - (to >> 15)));
+  const uint32_t up = ((from >> 15) ^ uint32_t(of < ot)) << 1;
+  return (from + up - 1u) & 0xFFFFu;
+}
+// cbvh_fat.cpp:447-448: round to nearest, then one step toward -/+ max half.
+__device__ inline uint32_t round_to_neg(float v) { return h_nextafter(f2h_rne(v), 0xFBFFu); }
+__device__ inline uint32_t round_to_pos(float v) { return h_nextafter(f2h_rne(v), 0x7BFFu); }
 
 }  // namespace hrt

@@ -25,6 +25,11 @@ struct Blas {
   NodePair* pairs = nullptr;  // max(leaves - 1, 1) pairs
   TriPre* tris = nullptr;     // ntris, leaf order
   uint32_t* perm = nullptr;   // leaf-order position -> original triangle id
+  uint32_t* parent = nullptr; // internal node -> (parent << 1) | side (Karras step)
+  // BVH4 collapse of the same tree (made on demand by blas_build_bvh4)
+  uint32_t nquads = 0;
+  Quad32* quads32 = nullptr;
+  Quad16* quads16 = nullptr;
   // parity hooks (kept only when requested)
   uint32_t* codes_in = nullptr;    // Morton code per input triangle
   uint32_t* codes_sorted = nullptr;
@@ -48,6 +53,9 @@ void blas_free(Blas& b);
 // synchronises the stream before returning (leaves/max_run/box are host
 // values afterwards).
 hipError_t blas_build(Blas& b, hipStream_t stream);
+// Collapses the built BVH2 into FP32 and FP16 BVH4 nodes (idempotent);
+// synchronises the stream.
+hipError_t blas_build_bvh4(Blas& b, hipStream_t stream);
 
 // Traversal ------------------------------------------------------------------
 enum HitFormat : int { kHitFull = 0, kHitCompact = 1 };
@@ -61,6 +69,9 @@ struct TraceArgs {
   uint32_t* stats;  // optional per-ray [pairs visited, triangles tested]
 };
 hipError_t trace_blas(const Blas& b, const TraceArgs& a, hipStream_t stream);
+
+// BVH4 traversal; fp16 selects the FP16 nodes.  Nearest hit by (t, primId).
+hipError_t trace_blas4(const Blas& b, const TraceArgs& a, bool fp16, hipStream_t stream);
 
 // Ambient-occlusion rays from primary hits (compact records), spp per pixel.
 hipError_t rtao_make_rays(const Blas& b, const Ray* prim_rays, const void* prim_hits_compact, uint64_t npix,

@@ -352,7 +352,7 @@ __global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* __restrict__ 
                                                    const uint32_t* __restrict__ run_start,
                                                    const uint32_t* __restrict__ meta, uint32_t ntris,
                                                    NodePair* __restrict__ pairs, uint32_t* __restrict__ parent_int,
-                                                   uint32_t* __restrict__ parent_leaf,
+                                                   uint32_t* __restrict__ parent_leaf, uint2* __restrict__ range,
                                                    uint32_t* __restrict__ karras_lr) {
   const int L = int(meta[0]);
   const int i = int(blockIdx.x * kBlock + threadIdx.x);
@@ -393,6 +393,7 @@ __global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* __restrict__ 
   pw[7] = 0u;
   pw[11] = rref;
   pw[15] = 0u;
+  range[i] = make_uint2(uint32_t(min(i, j)), uint32_t(max(i, j)));
   if (karras_lr) {
     karras_lr[2 * i + 0] = left_leaf ? uint32_t(L - 1 + gamma) : uint32_t(gamma);
     karras_lr[2 * i + 1] = right_leaf ? uint32_t(L - 1 + gamma + 1) : uint32_t(gamma + 1);
@@ -412,57 +413,155 @@ __device__ inline void store_box(NodePair* pairs, uint32_t p, uint32_t side, flo
   pw[6] = __float_as_uint(hi.z);
 }
 
+// Leaf boxes and the bottom-up refit.  Block b owns the leaves
+// [b * kRefitChunk, (b + 1) * kRefitChunk).  Every thread computes its leaf's
+// box, writes it into the parent's pair and climbs: at each internal node an
+// arrival counter tells the first child to stop and the second to merge the
+// two boxes and continue.  A node whose leaf range lies inside the block's
+// chunk only ever sees arrivals from this block, so its counter uses
+// workgroup-scope acquire/release; only nodes whose range crosses a chunk
+// boundary (the few ancestors of chunk seams) pay for agent-scope ordering,
+// which on gfx950 writes back and invalidates the L2 of the issuing XCD.
+constexpr int kRefitPerThread = 4;
+constexpr uint32_t kRefitChunk = kBlock * kRefitPerThread;
+
 __global__ __launch_bounds__(kBlock) void k_leaves_refit(
     const float4* __restrict__ v, const uint32_t* __restrict__ idx, const uint32_t* __restrict__ ids_sorted,
     const uint32_t* __restrict__ run_start, uint32_t* __restrict__ meta, uint32_t ntris, NodePair* pairs,
-    const uint32_t* __restrict__ parent_int, const uint32_t* __restrict__ parent_leaf, uint32_t* arrive) {
+    const uint32_t* __restrict__ parent_int, const uint32_t* __restrict__ parent_leaf,
+    const uint2* __restrict__ range, uint32_t* arrive) {
   const uint32_t L = meta[0];
-  const uint32_t m = blockIdx.x * kBlock + threadIdx.x;
-  if (m >= L) return;
-  const uint32_t s = run_start[m];
-  const uint32_t e = (m + 1 < L) ? run_start[m + 1] : ntris;
-  float3 lo = make_float3(kLbvhInf, kLbvhInf, kLbvhInf);
-  float3 hi = make_float3(-kLbvhInf, -kLbvhInf, -kLbvhInf);
-  for (uint32_t j = s; j < e; j++) {
-    const uint32_t t = ids_sorted[j];
-    const float4 a = v[idx[3 * t + 0]], b = v[idx[3 * t + 1]], c = v[idx[3 * t + 2]];
-    lo = make_float3(fminf(lo.x, fminf(a.x, fminf(b.x, c.x))), fminf(lo.y, fminf(a.y, fminf(b.y, c.y))),
-                     fminf(lo.z, fminf(a.z, fminf(b.z, c.z))));
-    hi = make_float3(fmaxf(hi.x, fmaxf(a.x, fmaxf(b.x, c.x))), fmaxf(hi.y, fmaxf(a.y, fmaxf(b.y, c.y))),
-                     fmaxf(hi.z, fmaxf(a.z, fmaxf(b.z, c.z))));
+  const uint32_t chunk_lo = blockIdx.x * kRefitChunk;
+  if (chunk_lo >= L) return;
+  const uint32_t chunk_hi = min(chunk_lo + kRefitChunk, L) - 1;
+  for (uint32_t m = chunk_lo + threadIdx.x; m <= chunk_hi; m += kBlock) {
+    const uint32_t s = run_start[m];
+    const uint32_t e = (m + 1 < L) ? run_start[m + 1] : ntris;
+    float3 lo = make_float3(kLbvhInf, kLbvhInf, kLbvhInf);
+    float3 hi = make_float3(-kLbvhInf, -kLbvhInf, -kLbvhInf);
+    for (uint32_t j = s; j < e; j++) {
+      const uint32_t t = ids_sorted[j];
+      const float4 a = v[idx[3 * t + 0]], b = v[idx[3 * t + 1]], c = v[idx[3 * t + 2]];
+      lo = make_float3(fminf(lo.x, fminf(a.x, fminf(b.x, c.x))), fminf(lo.y, fminf(a.y, fminf(b.y, c.y))),
+                       fminf(lo.z, fminf(a.z, fminf(b.z, c.z))));
+      hi = make_float3(fmaxf(hi.x, fmaxf(a.x, fmaxf(b.x, c.x))), fmaxf(hi.y, fmaxf(a.y, fmaxf(b.y, c.y))),
+                       fmaxf(hi.z, fmaxf(a.z, fmaxf(b.z, c.z))));
+    }
+    atomicMax(&meta[1], e - s);
+    if (L == 1) {
+      // Root is a leaf: pair 0 = (root leaf, empty dummy), as the reference's
+      // peach layout stores it (lbvh.cpp:497-504, cbvh.h:28-36).
+      store_box(pairs, 0, 0, lo, hi);
+      store_box(pairs, 0, 1, make_float3(0.f, 0.f, 0.f), make_float3(0.f, 0.f, 0.f));
+      uint32_t* pw = reinterpret_cast<uint32_t*>(pairs);
+      pw[3] = pack_leaf(s, e - s);
+      pw[7] = 0u;
+      pw[11] = 0x80000000u;
+      pw[15] = 0u;
+      return;
+    }
+    const uint32_t pl = parent_leaf[m];
+    store_box(pairs, pl >> 1, pl & 1u, lo, hi);
+    uint32_t p = pl >> 1;
+    while (true) {
+      const uint2 r = range[p];
+      const bool crossing = r.x < chunk_lo || r.y > chunk_hi;
+      // release our box, then count arrivals; the second child to arrive owns the parent
+      const uint32_t old = crossing
+          ? __hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT)
+          : __hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (old == 0) break;
+      if (p == 0) break;  // root box is not stored: traversal starts at the root's children
+      const uint32_t* pw = reinterpret_cast<const uint32_t*>(pairs + p);
+      const float3 l0 = make_float3(__uint_as_float(pw[0]), __uint_as_float(pw[1]), __uint_as_float(pw[2]));
+      const float3 h0 = make_float3(__uint_as_float(pw[4]), __uint_as_float(pw[5]), __uint_as_float(pw[6]));
+      const float3 l1 = make_float3(__uint_as_float(pw[8]), __uint_as_float(pw[9]), __uint_as_float(pw[10]));
+      const float3 h1 = make_float3(__uint_as_float(pw[12]), __uint_as_float(pw[13]), __uint_as_float(pw[14]));
+      const float3 ul = make_float3(fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z));
+      const float3 uh = make_float3(fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z));
+      const uint32_t pp = parent_int[p];
+      store_box(pairs, pp >> 1, pp & 1u, ul, uh);
+      p = pp >> 1;
+    }
   }
-  atomicMax(&meta[1], e - s);
-  if (L == 1) {
-    // Root is a leaf: pair 0 = (root leaf, empty dummy), as the reference's
-    // peach layout stores it (lbvh.cpp:497-504, cbvh.h:28-36).
-    store_box(pairs, 0, 0, lo, hi);
-    store_box(pairs, 0, 1, make_float3(0.f, 0.f, 0.f), make_float3(0.f, 0.f, 0.f));
-    uint32_t* pw = reinterpret_cast<uint32_t*>(pairs);
-    pw[3] = pack_leaf(s, e - s);
-    pw[7] = 0u;
-    pw[11] = 0x80000000u;
-    pw[15] = 0u;
-    return;
+}
+
+// ---------------------------------------------------------------------------
+// BVH4 collapse
+// ---------------------------------------------------------------------------
+// flag[k] = 1 when internal node k has even depth (it becomes a BVH4 node)
+__global__ __launch_bounds__(kBlock) void k_even_depth(const uint32_t* __restrict__ parent_int, uint32_t ninternal,
+                                                       uint32_t* __restrict__ flag) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  if (k >= ninternal) return;
+  uint32_t d = 0, p = k;
+  while (p != 0) {  // at most 30 steps: the tree has at most 30 internal levels
+    p = parent_int[p] >> 1;
+    d++;
   }
-  uint32_t pl = parent_leaf[m];
-  store_box(pairs, pl >> 1, pl & 1u, lo, hi);
-  uint32_t p = pl >> 1;
-  while (true) {
-    // release our box, then count arrivals; the second child to arrive owns the parent
-    const uint32_t old = __hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (old == 0) return;
-    const uint32_t* pw = reinterpret_cast<const uint32_t*>(pairs + p);
-    const float3 l0 = make_float3(__uint_as_float(pw[0]), __uint_as_float(pw[1]), __uint_as_float(pw[2]));
-    const float3 h0 = make_float3(__uint_as_float(pw[4]), __uint_as_float(pw[5]), __uint_as_float(pw[6]));
-    const float3 l1 = make_float3(__uint_as_float(pw[8]), __uint_as_float(pw[9]), __uint_as_float(pw[10]));
-    const float3 h1 = make_float3(__uint_as_float(pw[12]), __uint_as_float(pw[13]), __uint_as_float(pw[14]));
-    if (p == 0) return;  // root box is not stored: traversal starts at the root's children
-    const float3 ul = make_float3(fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z));
-    const float3 uh = make_float3(fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z));
-    const uint32_t pp = parent_int[p];
-    store_box(pairs, pp >> 1, pp & 1u, ul, uh);
-    p = pp >> 1;
+  flag[k] = (d & 1u) ? 0u : 1u;
+}
+
+struct Child {
+  float3 lo, hi;
+  uint32_t ref;
+};
+
+__device__ inline Child pair_child(const NodePair* pairs, uint32_t k, uint32_t side) {
+  const uint32_t* pw = reinterpret_cast<const uint32_t*>(pairs + k) + 8 * side;
+  Child c;
+  c.lo = make_float3(__uint_as_float(pw[0]), __uint_as_float(pw[1]), __uint_as_float(pw[2]));
+  c.hi = make_float3(__uint_as_float(pw[4]), __uint_as_float(pw[5]), __uint_as_float(pw[6]));
+  c.ref = pw[3];
+  return c;
+}
+
+// One BVH4 node per even-depth internal node k: k's children, each internal
+// child replaced by its two children, left to right
+// (cbvh_core.cpp:263-352); internal references become quad indices.
+__global__ __launch_bounds__(kBlock) void k_collapse4(const NodePair* __restrict__ pairs,
+                                                      const uint32_t* __restrict__ flag,
+                                                      const uint32_t* __restrict__ qidx, uint32_t ninternal,
+                                                      Quad32* __restrict__ q32, Quad16* __restrict__ q16) {
+  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
+  const bool single = (ninternal == 0);  // the root is a leaf: one node from pair 0
+  if (single ? k != 0 : (k >= ninternal || !flag[k])) return;
+  Child c[4];
+  int n = 0;
+  for (uint32_t side = 0; side < 2; side++) {
+    const Child ch = pair_child(pairs, k, side);
+    if (!single && (ch.ref & kLeafBit) == 0) {
+      c[n++] = pair_child(pairs, ch.ref, 0);
+      c[n++] = pair_child(pairs, ch.ref, 1);
+    } else {
+      c[n++] = ch;
+    }
   }
+  for (; n < 4; n++) {
+    c[n].lo = make_float3(1e9f, 1e9f, 1e9f);
+    c[n].hi = make_float3(-1e9f, -1e9f, -1e9f);
+    c[n].ref = pack_leaf(0, 0);
+  }
+  uint32_t ref[4];
+  for (int i = 0; i < 4; i++) ref[i] = (c[i].ref & kLeafBit) ? c[i].ref : qidx[c[i].ref];
+  const uint32_t q = single ? 0u : qidx[k];
+  Quad32 o;
+  o.lox = make_float4(c[0].lo.x, c[1].lo.x, c[2].lo.x, c[3].lo.x);
+  o.hix = make_float4(c[0].hi.x, c[1].hi.x, c[2].hi.x, c[3].hi.x);
+  o.loy = make_float4(c[0].lo.y, c[1].lo.y, c[2].lo.y, c[3].lo.y);
+  o.hiy = make_float4(c[0].hi.y, c[1].hi.y, c[2].hi.y, c[3].hi.y);
+  o.loz = make_float4(c[0].lo.z, c[1].lo.z, c[2].lo.z, c[3].lo.z);
+  o.hiz = make_float4(c[0].hi.z, c[1].hi.z, c[2].hi.z, c[3].hi.z);
+  o.ref = make_uint4(ref[0], ref[1], ref[2], ref[3]);
+  o.pad = make_uint4(0u, 0u, 0u, 0u);
+  q32[q] = o;
+  Quad16 h;
+  for (int i = 0; i < 4; i++) {  // PackNode, cbvh_fat.cpp:477-497
+    h.c[i] = make_uint4(round_to_neg(c[i].lo.x) | (round_to_neg(c[i].lo.y) << 16),
+                        round_to_neg(c[i].lo.z) | (round_to_pos(c[i].hi.x) << 16),
+                        round_to_pos(c[i].hi.y) | (round_to_pos(c[i].hi.z) << 16), ref[i]);
+  }
+  q16[q] = h;
 }
 
 // ---------------------------------------------------------------------------
@@ -497,6 +596,7 @@ hipError_t blas_alloc(Blas& b, uint32_t nverts, uint32_t ntris, bool keep_debug)
   HRT_CHECK(hipMalloc(&b.pairs, sizeof(NodePair) * npairs));
   HRT_CHECK(hipMalloc(&b.tris, sizeof(TriPre) * size_t(ntris)));
   HRT_CHECK(hipMalloc(&b.perm, sizeof(uint32_t) * size_t(ntris)));
+  HRT_CHECK(hipMalloc(&b.parent, sizeof(uint32_t) * npairs));
   if (keep_debug) {
     HRT_CHECK(hipMalloc(&b.codes_in, sizeof(uint32_t) * size_t(ntris)));
     HRT_CHECK(hipMalloc(&b.codes_sorted, sizeof(uint32_t) * size_t(ntris)));
@@ -509,8 +609,9 @@ hipError_t blas_alloc(Blas& b, uint32_t nverts, uint32_t ntris, bool keep_debug)
 }
 
 void blas_free(Blas& b) {
-  void* ptrs[] = {b.verts, b.idx, b.pairs, b.tris, b.perm, b.codes_in, b.codes_sorted,
-                  b.ids_sorted, b.run_start, b.ucodes, b.karras_lr};
+  void* ptrs[] = {b.verts,      b.idx,        b.pairs,      b.tris,      b.perm,   b.parent,
+                  b.quads32,    b.quads16,    b.codes_in,   b.codes_sorted, b.ids_sorted,
+                  b.run_start,  b.ucodes,     b.karras_lr};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   b = Blas();
@@ -529,14 +630,15 @@ hipError_t blas_build(Blas& b, hipStream_t s) {
   const size_t o_ka = take(w_codes), o_va = take(w_codes), o_kb = take(w_codes), o_vb = take(w_codes);
   const size_t o_hist = take(w_hist), o_hscan = take(w_hist), o_scan = take(w_scan);
   const size_t o_flags = take(N), o_pos = take(N), o_rs = take(N), o_uc = take(N);
-  const size_t o_pint = take(N), o_pleaf = take(N), o_arr = take(N), o_meta = take(64);
+  const size_t o_range = take(2 * N), o_pleaf = take(N), o_arr = take(N), o_meta = take(64);
   const size_t o_part = take(size_t(box_parts) * 8), o_box = take(16);
   uint32_t* base = nullptr;
   HRT_CHECK(hipMallocAsync(reinterpret_cast<void**>(&base), words * 4, s));
   uint32_t *ka = base + o_ka, *va = base + o_va, *kb = base + o_kb, *vb = base + o_vb;
   uint32_t *hist = base + o_hist, *hscan = base + o_hscan, *stmp = base + o_scan;
   uint32_t *flags = base + o_flags, *pos = base + o_pos, *rs = base + o_rs, *uc = base + o_uc;
-  uint32_t *pint = base + o_pint, *pleaf = base + o_pleaf, *arrive = base + o_arr, *meta = base + o_meta;
+  uint32_t *pleaf = base + o_pleaf, *arrive = base + o_arr, *meta = base + o_meta;
+  uint2* range = reinterpret_cast<uint2*>(base + o_range);
   float4* part = reinterpret_cast<float4*>(base + o_part);
   float* box = reinterpret_cast<float*>(base + o_box);
 
@@ -563,9 +665,10 @@ hipError_t blas_build(Blas& b, hipStream_t s) {
     if ((err = hipMemsetAsync(meta, 0, 64 * 4, s)) != hipSuccess) break;
     k_run_compact<<<blocks_for(N), kBlock, 0, s>>>(kin, flags, pos, uint32_t(N), rs, uc, meta);
     if ((err = hipMemsetAsync(arrive, 0, 4 * N, s)) != hipSuccess) break;
-    k_karras<<<blocks_for(N), kBlock, 0, s>>>(uc, rs, meta, uint32_t(N), b.pairs, pint, pleaf, b.karras_lr);
-    k_leaves_refit<<<blocks_for(N), kBlock, 0, s>>>(b.verts, b.idx, vin, rs, meta, uint32_t(N), b.pairs, pint,
-                                                    pleaf, arrive);
+    k_karras<<<blocks_for(N), kBlock, 0, s>>>(uc, rs, meta, uint32_t(N), b.pairs, b.parent, pleaf, range,
+                                              b.karras_lr);
+    k_leaves_refit<<<uint32_t((N + kRefitChunk - 1) / kRefitChunk), kBlock, 0, s>>>(
+        b.verts, b.idx, vin, rs, meta, uint32_t(N), b.pairs, b.parent, pleaf, range, arrive);
     k_tris<<<blocks_for(N), kBlock, 0, s>>>(b.verts, b.idx, vin, uint32_t(N), b.tris, b.perm);
     if ((err = hipGetLastError()) != hipSuccess) break;
     if (b.keep_debug) {
@@ -587,6 +690,46 @@ hipError_t blas_build(Blas& b, hipStream_t s) {
   hipError_t ferr = hipFreeAsync(base, s);
   if (err == hipSuccess) err = ferr;
   if (err == hipSuccess) err = hipStreamSynchronize(s);
+  return err;
+}
+
+hipError_t blas_build_bvh4(Blas& b, hipStream_t s) {
+  if (b.quads32) return hipSuccess;
+  const uint32_t ninternal = b.leaves > 1 ? b.leaves - 1 : 0;
+  const size_t nflag = std::max<uint32_t>(ninternal, 1);
+  const size_t w_scan = scan_temp_words(nflag);
+  uint32_t* base = nullptr;
+  HRT_CHECK(hipMallocAsync(reinterpret_cast<void**>(&base), 4 * (2 * nflag + w_scan + 64), s));
+  uint32_t *flag = base, *qidx = base + nflag, *stmp = base + 2 * nflag;
+  hipError_t err = hipSuccess;
+  do {
+    uint32_t nq = 1;
+    if (ninternal > 0) {
+      k_even_depth<<<blocks_for(ninternal), kBlock, 0, s>>>(b.parent, ninternal, flag);
+      if ((err = exclusive_scan(flag, qidx, ninternal, stmp, s)) != hipSuccess) break;
+      uint32_t last[2];
+      if ((err = hipMemcpyAsync(&last[0], qidx + ninternal - 1, 4, hipMemcpyDeviceToHost, s)) != hipSuccess) break;
+      if ((err = hipMemcpyAsync(&last[1], flag + ninternal - 1, 4, hipMemcpyDeviceToHost, s)) != hipSuccess) break;
+      if ((err = hipStreamSynchronize(s)) != hipSuccess) break;
+      nq = last[0] + last[1];
+    }
+    if ((err = hipMalloc(&b.quads32, sizeof(Quad32) * nq)) != hipSuccess) break;
+    if ((err = hipMalloc(&b.quads16, sizeof(Quad16) * nq)) != hipSuccess) break;
+    k_collapse4<<<blocks_for(std::max<uint32_t>(ninternal, 1)), kBlock, 0, s>>>(b.pairs, flag, qidx, ninternal,
+                                                                               b.quads32, b.quads16);
+    if ((err = hipGetLastError()) != hipSuccess) break;
+    b.nquads = nq;
+  } while (false);
+  hipError_t ferr = hipFreeAsync(base, s);
+  if (err == hipSuccess) err = ferr;
+  if (err == hipSuccess) err = hipStreamSynchronize(s);
+  if (err != hipSuccess) {
+    if (b.quads32) (void)hipFree(b.quads32);
+    if (b.quads16) (void)hipFree(b.quads16);
+    b.quads32 = nullptr;
+    b.quads16 = nullptr;
+    b.nquads = 0;
+  }
   return err;
 }
 

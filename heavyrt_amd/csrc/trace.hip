@@ -12,14 +12,33 @@
 // Layout: NodePair (64 B, both children of one internal node) and TriPre
 // (48 B, A / B-A / C-A / original id), both in HBM; the per-lane stack of
 // deferred children lives in LDS, 32 entries, interleaved by lane.
+//
+// Execution: a persistent wave64 kernel.  Each wave keeps 64 rays in flight;
+// when enough lanes have finished, the wave counts its idle lanes with one
+// ballot, takes that many consecutive ray indices from a global queue with a
+// single atomic and refills exactly those lanes (ballot-based compaction), so
+// long rays never hold a wave of finished lanes.
+//
+// Nearest-hit keeps the reference's exact order: a lane that reaches a leaf
+// intersects it before it tests any further node.  Any-hit only has to decide
+// whether some triangle is hit, which does not depend on the order, so it
+// traverses speculatively (Aila & Laine, "Understanding the Efficiency of Ray
+// Traversal on GPUs", HPG 2009): a lane that has found a leaf parks it and
+// keeps descending until every lane of the wave holds a leaf, then the whole
+// wave intersects leaves together.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "hrt_internal.h"
 
 namespace hrt {
 namespace {
 
-constexpr int kTraceBlock = 256;
+constexpr int kWave = 64;
+constexpr int kWavesPerBlock = 1;
+constexpr int kTraceBlock = kWave * kWavesPerBlock;
+constexpr int kRefillAt = 16;  // refill when at least this many lanes are idle
 
 struct HitOutFull {  // CRT_Hit (CrossRT.h:20-27)
   float t;
@@ -27,16 +46,19 @@ struct HitOutFull {  // CRT_Hit (CrossRT.h:20-27)
   float coords[4];
 };
 
-template <bool kForceAny, bool kStats>
-__global__ __launch_bounds__(kTraceBlock) void k_trace_bvh2(const NodePair* __restrict__ pairs,
-                                                            const TriPre* __restrict__ tris,
-                                                            const Ray* __restrict__ rays, uint64_t n, void* out,
-                                                            int format, uint32_t* __restrict__ stats) {
-  __shared__ uint32_t stack_mem[kStackDepth * kTraceBlock];
-  const uint64_t i = uint64_t(blockIdx.x) * kTraceBlock + threadIdx.x;
-  if (i >= n) return;
-  uint32_t* stack = stack_mem + threadIdx.x;  // entry k at stack[k * kTraceBlock]
+__global__ void k_reset_counter(uint32_t* c) { *c = 0u; }
 
+// Reference-order reference kernel (one ray per lane, no refill).  Kept for
+// the counting variant used to report the reference's traversal counters.
+template <bool kForceAny, bool kStats>
+__global__ __launch_bounds__(256) void k_trace_simple(const NodePair* __restrict__ pairs,
+                                                      const TriPre* __restrict__ tris,
+                                                      const Ray* __restrict__ rays, uint64_t n, void* out,
+                                                      int format, uint32_t* __restrict__ stats) {
+  __shared__ uint32_t stack_mem[kStackDepth * 256];
+  const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
+  if (i >= n) return;
+  uint32_t* stack = stack_mem + threadIdx.x;
   const Ray r = rays[i];
   const float3 o = make_float3(r.pos_near.x, r.pos_near.y, r.pos_near.z);
   const float3 d = make_float3(r.dir_far.x, r.dir_far.y, r.dir_far.z);
@@ -44,16 +66,14 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_bvh2(const NodePair* __re
   const bool any = kForceAny || (r.dir_far.w <= 0.0f);
   const float tfar = kForceAny ? fabsf(r.dir_far.w) : (any ? -r.dir_far.w : r.dir_far.w);
   const float3 inv = safe_inverse(d);
-
   HitRec h;
   h.t = tfar;
   h.prim = kInvalid;
   h.u = 0.0f;
   h.v = 0.0f;
   uint32_t npairs = 0, ntests = 0;
-
   int top = 0;
-  uint32_t cur = 0;  // pair index of the root's children
+  uint32_t cur = 0;
   while (top >= 0) {
     bool pop;
     if ((cur & kLeafBit) == 0) {
@@ -70,7 +90,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_bvh2(const NodePair* __re
       if (h0 && h1) {
         const bool left_first = t0.x <= t1.x;
         cur = left_first ? c0 : c1;
-        stack[top * kTraceBlock] = left_first ? c1 : c0;
+        stack[top * 256] = left_first ? c1 : c0;
         top++;
       }
     } else {
@@ -84,10 +104,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_bvh2(const NodePair* __re
     }
     if (pop) {
       top--;
-      cur = stack[max(top, 0) * kTraceBlock];
+      cur = stack[max(top, 0) * 256];
     }
   }
-
   if (kStats) {
     stats[2 * i + 0] = npairs;
     stats[2 * i + 1] = ntests;
@@ -105,27 +124,200 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_bvh2(const NodePair* __re
   }
 }
 
+// Persistent wave64 traversal with ballot-based refill.
+//   kForceAny: every ray is an any-hit query (hrt_trace_any)
+//   kSpec:     speculative while-while traversal (any-hit only)
+template <bool kForceAny, bool kSpec>
+__global__ __launch_bounds__(kTraceBlock) void k_trace_persistent(const NodePair* __restrict__ pairs,
+                                                                  const TriPre* __restrict__ tris,
+                                                                  const Ray* __restrict__ rays, uint64_t n,
+                                                                  void* out, int format,
+                                                                  uint32_t* __restrict__ queue) {
+  __shared__ uint32_t stack_mem[kStackDepth * kTraceBlock];
+  const int lane = threadIdx.x & (kWave - 1);
+  uint32_t* stack = stack_mem + threadIdx.x;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
+
+  // per-lane ray state
+  bool active = false;
+  uint64_t rid = 0;
+  float3 o = make_float3(0.f, 0.f, 0.f), d = o, inv = o;
+  float tnear = 0.f;
+  bool any = kForceAny;
+  HitRec h;
+  h.t = 0.f;
+  h.prim = kInvalid;
+  h.u = h.v = 0.f;
+  int top = 0;          // entries on the stack
+  uint32_t cur = kInvalid;   // node pair to test next, a leaf, or kInvalid
+  uint32_t leaf = 0;    // parked leaf (speculative any-hit), 0 = none
+  bool drained = false; // the queue is empty
+
+  while (true) {
+    // ---- refill idle lanes from the global queue -------------------------
+    const uint64_t idle = __ballot(!active);
+    const uint32_t nidle = __popcll(idle);
+    if (!drained && (nidle >= kRefillAt || nidle == kWave)) {
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(queue, nidle);
+      base = __shfl(base, 0);
+      if (uint64_t(base) + nidle >= n) drained = true;
+      if (!active) {
+        const uint64_t idx = uint64_t(base) + __popcll(idle & lt_mask);
+        if (idx < n) {
+          rid = idx;
+          const Ray r = rays[idx];
+          o = make_float3(r.pos_near.x, r.pos_near.y, r.pos_near.z);
+          d = make_float3(r.dir_far.x, r.dir_far.y, r.dir_far.z);
+          tnear = r.pos_near.w;
+          any = kForceAny || (r.dir_far.w <= 0.0f);
+          h.t = kForceAny ? fabsf(r.dir_far.w) : (any ? -r.dir_far.w : r.dir_far.w);
+          h.prim = kInvalid;
+          h.u = h.v = 0.f;
+          inv = safe_inverse(d);
+          top = 0;
+          cur = 0;
+          leaf = 0;
+          active = true;
+        }
+      }
+    }
+    if (__ballot(active) == 0) break;
+
+    // ---- traverse until enough lanes are idle -----------------------------
+    while (true) {
+      bool done = false;
+      if (kSpec) {
+        // inner phase: lanes without a parked leaf search for one; lanes
+        // holding one keep descending while their next node is internal
+        while (true) {
+          // park a leaf reached in `cur` (at most one per lane)
+          if (active && leaf == 0 && (cur & kLeafBit) != 0 && cur != kInvalid) {
+            leaf = cur;
+            cur = (top > 0) ? stack[--top * kTraceBlock] : kInvalid;
+          }
+          const bool searching = active && leaf == 0 && (cur & kLeafBit) == 0;
+          if (!__any(searching)) break;
+          if (active && (cur & kLeafBit) == 0) {
+            const float4* pp = reinterpret_cast<const float4*>(pairs + cur);
+            const float4 lmin = pp[0], lmax = pp[1], rmin = pp[2], rmax = pp[3];
+            const float2 t0 = slab(o, inv, lmin.x, lmin.y, lmin.z, lmax.x, lmax.y, lmax.z);
+            const float2 t1 = slab(o, inv, rmin.x, rmin.y, rmin.z, rmax.x, rmax.y, rmax.z);
+            const bool h0 = (t0.x <= t0.y) && (t0.y >= tnear) && (t0.x <= h.t);
+            const bool h1 = (t1.x <= t1.y) && (t1.y >= tnear) && (t1.x <= h.t);
+            const uint32_t c0 = __float_as_uint(lmin.w), c1 = __float_as_uint(rmin.w);
+            if (h0 && h1) {
+              const bool left_first = t0.x <= t1.x;
+              cur = left_first ? c0 : c1;
+              stack[top * kTraceBlock] = left_first ? c1 : c0;
+              top++;
+            } else if (h0 || h1) {
+              cur = h0 ? c0 : c1;
+            } else {
+              cur = (top > 0) ? stack[--top * kTraceBlock] : kInvalid;
+            }
+          }
+        }
+        // leaf phase
+        if (active && leaf != 0) {
+          const uint32_t s = leaf_start(leaf), c = leaf_count(leaf);
+          for (uint32_t k = s; k < s + c; k++) {
+            if (mt_test(o, d, tnear, tris[k], h)) break;  // any-hit: first hit decides
+          }
+          leaf = 0;
+        }
+        done = active && (h.prim != kInvalid || (leaf == 0 && cur == kInvalid));
+      } else {
+        // one step in the reference's order: a pair or a leaf
+        if (active) {
+          if ((cur & kLeafBit) == 0) {
+            const float4* pp = reinterpret_cast<const float4*>(pairs + cur);
+            const float4 lmin = pp[0], lmax = pp[1], rmin = pp[2], rmax = pp[3];
+            const float2 t0 = slab(o, inv, lmin.x, lmin.y, lmin.z, lmax.x, lmax.y, lmax.z);
+            const float2 t1 = slab(o, inv, rmin.x, rmin.y, rmin.z, rmax.x, rmax.y, rmax.z);
+            const bool h0 = (t0.x <= t0.y) && (t0.y >= tnear) && (t0.x <= h.t);
+            const bool h1 = (t1.x <= t1.y) && (t1.y >= tnear) && (t1.x <= h.t);
+            const uint32_t c0 = __float_as_uint(lmin.w), c1 = __float_as_uint(rmin.w);
+            if (h0 && h1) {
+              const bool left_first = t0.x <= t1.x;
+              cur = left_first ? c0 : c1;
+              stack[top * kTraceBlock] = left_first ? c1 : c0;
+              top++;
+            } else if (h0 || h1) {
+              cur = h0 ? c0 : c1;
+            } else {
+              cur = (top > 0) ? stack[--top * kTraceBlock] : kInvalid;
+            }
+          } else if (cur != kInvalid) {
+            const uint32_t s = leaf_start(cur), c = leaf_count(cur);
+            for (uint32_t k = s; k < s + c; k++) mt_test(o, d, tnear, tris[k], h);
+            cur = (top > 0) ? stack[--top * kTraceBlock] : kInvalid;
+          }
+          done = (cur == kInvalid) || (any && h.prim != kInvalid);
+        }
+      }
+      if (done) {
+        if (kForceAny) {
+          static_cast<uint8_t*>(out)[rid] = (h.prim != kInvalid) ? 1 : 0;
+        } else if (format == kHitCompact) {
+          static_cast<uint2*>(out)[rid] = make_uint2(__float_as_uint(h.t), h.prim);
+        } else {
+          const bool hit = h.prim != kInvalid;
+          float4* o4 = reinterpret_cast<float4*>(static_cast<HitOutFull*>(out) + rid);
+          o4[0] = make_float4(h.t, __uint_as_float(h.prim), __uint_as_float(hit ? 0u : kInvalid),
+                              __uint_as_float(hit ? 0u : kInvalid));
+          o4[1] = make_float4(h.u, h.v, 0.0f, 0.0f);
+        }
+        active = false;
+      }
+      const uint32_t nact = __popcll(__ballot(active));
+      if (nact == 0 || (!drained && kWave - nact >= kRefillAt)) break;
+    }
+  }
+}
+
+// Resident-wave count of the persistent kernel on the current device.
+int persistent_grid(const void* kernel) {
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 1024;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kTraceBlock, 0) != hipSuccess || per_cu <= 0)
+    per_cu = 8;
+  return cus * per_cu;
+}
+
+template <bool kAny, bool kSpec>
+hipError_t launch_persistent(const Blas& b, const TraceArgs& a, hipStream_t stream) {
+  uint32_t* queue = nullptr;
+  hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&queue), sizeof(uint32_t), stream);
+  if (e != hipSuccess) return e;
+  k_reset_counter<<<1, 1, 0, stream>>>(queue);
+  const void* kfn = reinterpret_cast<const void*>(&k_trace_persistent<kAny, kSpec>);
+  uint64_t want = (a.n + kTraceBlock - 1) / kTraceBlock;
+  const int grid = int(std::min<uint64_t>(want, uint64_t(persistent_grid(kfn))));
+  k_trace_persistent<kAny, kSpec><<<grid, kTraceBlock, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format,
+                                                                   queue);
+  e = hipGetLastError();
+  hipError_t f = hipFreeAsync(queue, stream);
+  return e != hipSuccess ? e : f;
+}
+
 }  // namespace
 
 hipError_t trace_blas(const Blas& b, const TraceArgs& a, hipStream_t stream) {
   if (a.n == 0) return hipSuccess;
-  const uint32_t blocks = uint32_t((a.n + kTraceBlock - 1) / kTraceBlock);
-  if (a.anyhit) {
-    if (a.stats)
-      k_trace_bvh2<true, true><<<blocks, kTraceBlock, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format,
-                                                                    a.stats);
+  if (a.stats) {
+    // counting variant: one ray per lane in the reference's order
+    const uint32_t blocks = uint32_t((a.n + 255) / 256);
+    if (a.anyhit)
+      k_trace_simple<true, true><<<blocks, 256, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format, a.stats);
     else
-      k_trace_bvh2<true, false><<<blocks, kTraceBlock, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format,
-                                                                     nullptr);
-  } else {
-    if (a.stats)
-      k_trace_bvh2<false, true><<<blocks, kTraceBlock, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format,
-                                                                     a.stats);
-    else
-      k_trace_bvh2<false, false><<<blocks, kTraceBlock, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format,
-                                                                      nullptr);
+      k_trace_simple<false, true><<<blocks, 256, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format,
+                                                               a.stats);
+    return hipGetLastError();
   }
-  return hipGetLastError();
+  if (a.anyhit) return launch_persistent<true, true>(b, a, stream);
+  return launch_persistent<false, false>(b, a, stream);
 }
 
 }  // namespace hrt

@@ -1,0 +1,305 @@
+// trace4.hip -- BVH4 nearest-hit / any-hit traversal (FP32 or FP16 boxes).
+//
+// Visiting order follows BVH4HalfRT::BVH4Traverse (core/BVH4HalfRT.cpp:73-256):
+// the four child boxes are slab-tested in FP32 (FP16 boxes are widened to FP32
+// first, UnpackBVHNode :56-71), a missed child gets MAXFLOAT, a five-comparator
+// network sorts the (distance, child) pairs (:119-180), the hit children w, z,
+// y are pushed and the walk continues with x (:184-222).  Leaves are tested
+// with the reference's Moller-Trumbore; the nearest hit is chosen by
+// (t, primId), so FP16 and FP32 nodes -- which visit in different orders --
+// return identical hits.  tFar <= 0 or hrt_trace_any: stop at the first hit.
+//
+// The stack holds at most 45 entries: BVH4 nodes are the even-depth nodes of
+// a BVH2 with at most 30 internal levels (15 node levels, 3 deferred children
+// each); 48 LDS entries per lane, interleaved by lane.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+
+#include "hrt_internal.h"
+
+namespace hrt {
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kBlock4 = 64;
+constexpr int kRefillAt4 = 16;
+constexpr float kMaxFloat = 3.402823466e+38f;
+
+struct HitOutFull {
+  float t;
+  uint32_t prim, inst, geom;
+  float coords[4];
+};
+
+__global__ void k_reset_counter4(uint32_t* c) { *c = 0u; }
+
+// Loads one BVH4 node, tests its four children, and returns the entry
+// distances (kMaxFloat for a miss) and references in reference order.
+template <bool kFp16>
+__device__ inline void visit_quad(const void* __restrict__ quads, uint32_t q, float3 o, float3 inv, float tnear,
+                                  float tcull, float tm[4], uint32_t ch[4]) {
+  float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+  if (kFp16) {
+    const uint4* p = reinterpret_cast<const uint4*>(static_cast<const Quad16*>(quads) + q);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+      const uint4 w = p[k];
+      lox[k] = h2f_lo(w.x);
+      loy[k] = h2f_hi(w.x);
+      loz[k] = h2f_lo(w.y);
+      hix[k] = h2f_hi(w.y);
+      hiy[k] = h2f_lo(w.z);
+      hiz[k] = h2f_hi(w.z);
+      ch[k] = w.w;
+    }
+  } else {
+    const float4* p = reinterpret_cast<const float4*>(static_cast<const Quad32*>(quads) + q);
+    const float4 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4], f = p[5];
+    const uint4 r = reinterpret_cast<const uint4*>(p)[6];
+    lox[0] = a.x; lox[1] = a.y; lox[2] = a.z; lox[3] = a.w;
+    hix[0] = b.x; hix[1] = b.y; hix[2] = b.z; hix[3] = b.w;
+    loy[0] = c.x; loy[1] = c.y; loy[2] = c.z; loy[3] = c.w;
+    hiy[0] = d.x; hiy[1] = d.y; hiy[2] = d.z; hiy[3] = d.w;
+    loz[0] = e.x; loz[1] = e.y; loz[2] = e.z; loz[3] = e.w;
+    hiz[0] = f.x; hiz[1] = f.y; hiz[2] = f.z; hiz[3] = f.w;
+    ch[0] = r.x; ch[1] = r.y; ch[2] = r.z; ch[3] = r.w;
+  }
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    const float2 t = slab(o, inv, lox[k], loy[k], loz[k], hix[k], hiy[k], hiz[k]);
+    const bool hit = (t.x <= t.y) && (t.y >= tnear) && (t.x <= tcull);
+    tm[k] = hit ? t.x : kMaxFloat;
+  }
+  // five-comparator network, BVH4HalfRT.cpp:121-180 (swap when strictly less)
+  auto cswap = [&](int i, int j) {
+    const bool s = tm[j] < tm[i];
+    const float ti = tm[i], tj = tm[j];
+    const uint32_t ci = ch[i], cj = ch[j];
+    tm[i] = s ? tj : ti;
+    tm[j] = s ? ti : tj;
+    ch[i] = s ? cj : ci;
+    ch[j] = s ? ci : cj;
+  };
+  cswap(0, 1);
+  cswap(2, 3);
+  cswap(0, 2);
+  cswap(1, 3);
+  cswap(1, 2);
+}
+
+template <bool kForceAny>
+__device__ inline void write_hit(void* out, uint64_t i, int format, const HitRec& h) {
+  if (kForceAny) {
+    static_cast<uint8_t*>(out)[i] = (h.prim != kInvalid) ? 1 : 0;
+  } else if (format == kHitCompact) {
+    static_cast<uint2*>(out)[i] = make_uint2(__float_as_uint(h.t), h.prim);
+  } else {
+    const bool hit = h.prim != kInvalid;
+    float4* o4 = reinterpret_cast<float4*>(static_cast<HitOutFull*>(out) + i);
+    o4[0] = make_float4(h.t, __uint_as_float(h.prim), __uint_as_float(hit ? 0u : kInvalid),
+                        __uint_as_float(hit ? 0u : kInvalid));
+    o4[1] = make_float4(h.u, h.v, 0.0f, 0.0f);
+  }
+}
+
+// One ray per lane, the reference's order step by step (also the counting
+// variant: quads visited, triangles in visited leaves).
+template <bool kForceAny, bool kFp16, bool kStats>
+__global__ __launch_bounds__(kBlock4) void k_trace4_simple(const void* __restrict__ quads,
+                                                           const TriPre* __restrict__ tris,
+                                                           const Ray* __restrict__ rays, uint64_t n, void* out,
+                                                           int format, uint32_t* __restrict__ stats) {
+  __shared__ uint32_t stack_mem[kStackDepth4 * kBlock4];
+  const uint64_t i = uint64_t(blockIdx.x) * kBlock4 + threadIdx.x;
+  if (i >= n) return;
+  uint32_t* stack = stack_mem + threadIdx.x;
+  const Ray r = rays[i];
+  const float3 o = make_float3(r.pos_near.x, r.pos_near.y, r.pos_near.z);
+  const float3 d = make_float3(r.dir_far.x, r.dir_far.y, r.dir_far.z);
+  const float tnear = r.pos_near.w;
+  const bool any = kForceAny || (r.dir_far.w <= 0.0f);
+  const float3 inv = safe_inverse(d);
+  HitRec h;
+  h.t = kForceAny ? fabsf(r.dir_far.w) : (any ? -r.dir_far.w : r.dir_far.w);
+  h.prim = kInvalid;
+  h.u = h.v = 0.0f;
+  uint32_t nvisits = 0, ntests = 0;
+  int top = 0;
+  uint32_t cur = 0;
+  while (true) {
+    bool pop = false;
+    if ((cur & kLeafBit) == 0) {
+      float tm[4];
+      uint32_t ch[4];
+      visit_quad<kFp16>(quads, cur, o, inv, tnear, h.t, tm, ch);
+      if (kStats) nvisits++;
+      if (tm[3] < kMaxFloat) stack[(top++) * kBlock4] = ch[3];
+      if (tm[2] < kMaxFloat) stack[(top++) * kBlock4] = ch[2];
+      if (tm[1] < kMaxFloat) stack[(top++) * kBlock4] = ch[1];
+      if (tm[0] < kMaxFloat) cur = ch[0];
+      else pop = true;
+    }
+    if (!pop && (cur & kLeafBit) != 0 && cur != kInvalid) {
+      const uint32_t s = leaf_start(cur), c = leaf_count(cur);
+      if (kStats) ntests += c;
+      for (uint32_t k = s; k < s + c; k++)
+        if (mt_test_lex(o, d, tnear, tris[k], h) && any) break;
+      pop = true;
+      if (any && h.prim != kInvalid) break;
+    }
+    if (pop) {
+      if (top == 0) break;
+      cur = stack[(--top) * kBlock4];
+    }
+  }
+  if (kStats) {
+    stats[2 * i + 0] = nvisits;
+    stats[2 * i + 1] = ntests;
+  }
+  write_hit<kForceAny>(out, i, format, h);
+}
+
+// Persistent speculative any-hit over BVH4 nodes with ballot-based refill
+// (the BVH2 kernel in trace.hip describes the scheme).
+template <bool kFp16>
+__global__ __launch_bounds__(kBlock4) void k_trace4_any_persistent(const void* __restrict__ quads,
+                                                                   const TriPre* __restrict__ tris,
+                                                                   const Ray* __restrict__ rays, uint64_t n,
+                                                                   uint8_t* __restrict__ out,
+                                                                   uint32_t* __restrict__ queue) {
+  __shared__ uint32_t stack_mem[kStackDepth4 * kBlock4];
+  const int lane = threadIdx.x & (kWave - 1);
+  uint32_t* stack = stack_mem + threadIdx.x;
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
+  bool active = false;
+  uint64_t rid = 0;
+  float3 o = make_float3(0.f, 0.f, 0.f), d = o, inv = o;
+  float tnear = 0.f;
+  HitRec h;
+  h.t = 0.f;
+  h.prim = kInvalid;
+  h.u = h.v = 0.f;
+  int top = 0;
+  uint32_t cur = kInvalid;
+  uint32_t leaf = 0;
+  bool drained = false;
+
+  while (true) {
+    const uint64_t idle = __ballot(!active);
+    const uint32_t nidle = __popcll(idle);
+    if (!drained && (nidle >= kRefillAt4 || nidle == kWave)) {
+      uint32_t base = 0;
+      if (lane == 0) base = atomicAdd(queue, nidle);
+      base = __shfl(base, 0);
+      if (uint64_t(base) + nidle >= n) drained = true;
+      if (!active) {
+        const uint64_t idx = uint64_t(base) + __popcll(idle & lt_mask);
+        if (idx < n) {
+          rid = idx;
+          const Ray r = rays[idx];
+          o = make_float3(r.pos_near.x, r.pos_near.y, r.pos_near.z);
+          d = make_float3(r.dir_far.x, r.dir_far.y, r.dir_far.z);
+          tnear = r.pos_near.w;
+          h.t = fabsf(r.dir_far.w);
+          h.prim = kInvalid;
+          inv = safe_inverse(d);
+          top = 0;
+          cur = 0;
+          leaf = 0;
+          active = true;
+        }
+      }
+    }
+    if (__ballot(active) == 0) break;
+
+    while (true) {
+      // inner phase: park the first leaf found, keep descending speculatively
+      while (true) {
+        if (active && leaf == 0 && (cur & kLeafBit) != 0 && cur != kInvalid) {
+          leaf = cur;
+          cur = (top > 0) ? stack[(--top) * kBlock4] : kInvalid;
+        }
+        const bool searching = active && leaf == 0 && (cur & kLeafBit) == 0;
+        if (!__any(searching)) break;
+        if (active && (cur & kLeafBit) == 0) {
+          float tm[4];
+          uint32_t ch[4];
+          visit_quad<kFp16>(quads, cur, o, inv, tnear, h.t, tm, ch);
+          if (tm[3] < kMaxFloat) stack[(top++) * kBlock4] = ch[3];
+          if (tm[2] < kMaxFloat) stack[(top++) * kBlock4] = ch[2];
+          if (tm[1] < kMaxFloat) stack[(top++) * kBlock4] = ch[1];
+          if (tm[0] < kMaxFloat) cur = ch[0];
+          else cur = (top > 0) ? stack[(--top) * kBlock4] : kInvalid;
+        }
+      }
+      // leaf phase
+      if (active && leaf != 0) {
+        const uint32_t s = leaf_start(leaf), c = leaf_count(leaf);
+        for (uint32_t k = s; k < s + c; k++)
+          if (mt_test_lex(o, d, tnear, tris[k], h)) break;
+        leaf = 0;
+      }
+      if (active && (h.prim != kInvalid || cur == kInvalid)) {
+        out[rid] = (h.prim != kInvalid) ? 1 : 0;
+        active = false;
+      }
+      const uint32_t nact = __popcll(__ballot(active));
+      if (nact == 0 || (!drained && kWave - nact >= kRefillAt4)) break;
+    }
+  }
+}
+
+template <bool kFp16>
+hipError_t launch_any4(const Blas& b, const TraceArgs& a, hipStream_t stream) {
+  uint32_t* queue = nullptr;
+  hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&queue), sizeof(uint32_t), stream);
+  if (e != hipSuccess) return e;
+  k_reset_counter4<<<1, 1, 0, stream>>>(queue);
+  int dev = 0, cus = 256, per_cu = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  const void* kfn = reinterpret_cast<const void*>(&k_trace4_any_persistent<kFp16>);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kBlock4, 0) != hipSuccess || per_cu <= 0) per_cu = 8;
+  const uint64_t want = (a.n + kBlock4 - 1) / kBlock4;
+  const int grid = int(std::min<uint64_t>(want, uint64_t(std::max(cus, 1)) * uint64_t(per_cu)));
+  const void* q = kFp16 ? static_cast<const void*>(b.quads16) : static_cast<const void*>(b.quads32);
+  k_trace4_any_persistent<kFp16><<<grid, kBlock4, 0, stream>>>(q, b.tris, a.rays, a.n, static_cast<uint8_t*>(a.out),
+                                                               queue);
+  e = hipGetLastError();
+  hipError_t f = hipFreeAsync(queue, stream);
+  return e != hipSuccess ? e : f;
+}
+
+template <bool kFp16>
+hipError_t launch_simple4(const Blas& b, const TraceArgs& a, hipStream_t stream) {
+  const uint32_t blocks = uint32_t((a.n + kBlock4 - 1) / kBlock4);
+  const void* q = kFp16 ? static_cast<const void*>(b.quads16) : static_cast<const void*>(b.quads32);
+  if (a.anyhit) {
+    if (a.stats)
+      k_trace4_simple<true, kFp16, true><<<blocks, kBlock4, 0, stream>>>(q, b.tris, a.rays, a.n, a.out, a.format,
+                                                                        a.stats);
+    else
+      k_trace4_simple<true, kFp16, false><<<blocks, kBlock4, 0, stream>>>(q, b.tris, a.rays, a.n, a.out, a.format,
+                                                                         nullptr);
+  } else {
+    if (a.stats)
+      k_trace4_simple<false, kFp16, true><<<blocks, kBlock4, 0, stream>>>(q, b.tris, a.rays, a.n, a.out, a.format,
+                                                                         a.stats);
+    else
+      k_trace4_simple<false, kFp16, false><<<blocks, kBlock4, 0, stream>>>(q, b.tris, a.rays, a.n, a.out,
+                                                                          a.format, nullptr);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t trace_blas4(const Blas& b, const TraceArgs& a, bool fp16, hipStream_t stream) {
+  if (a.n == 0) return hipSuccess;
+  if (!b.quads32 || !b.quads16) return hipErrorInvalidValue;
+  if (a.anyhit && !a.stats) return fp16 ? launch_any4<true>(b, a, stream) : launch_any4<false>(b, a, stream);
+  return fp16 ? launch_simple4<true>(b, a, stream) : launch_simple4<false>(b, a, stream);
+}
+
+}  // namespace hrt

@@ -66,13 +66,22 @@ typedef struct hrt_hit_compact {
 } hrt_hit_compact;
 
 enum { HRT_HIT_FULL = 0, HRT_HIT_COMPACT = 1 };
-enum { HRT_LAYOUT_BVH2 = 0 };
+/* Acceleration-structure layout chosen at CommitScene:
+ *   BVH2       node pairs of the LBVH; nearest hit in BVH2CommonRT's order
+ *              (BVH2CommonRT.cpp:49-141), first strictly closer hit wins.
+ *   BVH4       the same tree collapsed to 4-wide nodes with FP32 boxes,
+ *              BVH4HalfRT's visiting order (BVH4HalfRT.cpp:73-256); nearest
+ *              hit by (t, primId) so the answer does not depend on the order.
+ *   BVH4_FP16  as BVH4 with FP16 boxes rounded outward (cbvh_fat.cpp:447-497);
+ *              requires every coordinate within +-65504.
+ *   AUTO       nearest hits on BVH2 (reference order), any-hit on BVH4. */
+enum { HRT_LAYOUT_BVH2 = 0, HRT_LAYOUT_BVH4 = 1, HRT_LAYOUT_BVH4_FP16 = 2, HRT_LAYOUT_AUTO = 3 };
 enum { HRT_BUILD_LOW = 0, HRT_BUILD_MEDIUM = 1, HRT_BUILD_HIGH = 2, HRT_BUILD_REFIT = 3 }; /* BuildQuality, CrossRT.h:9-15 */
 
 /* Debug / parity dumps (hrt_debug_dump `what`).  Available for geometries
  * added after hrt_scene_set_keep_build_arrays(scene, 1). */
 enum {
-  HRT_DUMP_SIZES = 0,        /* uint32[4]: triangles, leaves, max leaf size, pairs */
+  HRT_DUMP_SIZES = 0,        /* uint32[5]: triangles, leaves, max leaf size, pairs, BVH4 nodes */
   HRT_DUMP_CODES_INPUT = 1,  /* uint32[ntris]: Morton code per input triangle */
   HRT_DUMP_CODES_SORTED = 2, /* uint32[ntris]: codes after the (code, id) sort */
   HRT_DUMP_IDS_SORTED = 3,   /* uint32[ntris]: triangle ids after the sort */
@@ -81,7 +90,9 @@ enum {
   HRT_DUMP_LR_NODES = 6,     /* uint32[(2*leaves-1)*8]: nodes in the reference's Karras
                                 left/right numbering (cbvh.h:20-26; lbvh.cpp:438-439) */
   HRT_DUMP_PERM = 7,         /* uint32[ntris]: leaf-order position -> triangle id */
-  HRT_DUMP_SCENE_BOX = 8     /* float[12]: box min xyzw, box max xyzw, Morton scale xyzw */
+  HRT_DUMP_SCENE_BOX = 8,    /* float[12]: box min xyzw, box max xyzw, Morton scale xyzw */
+  HRT_DUMP_QUADS32 = 9,      /* uint32[nquads*32]: BVH4 nodes, FP32 boxes (after a BVH4 commit) */
+  HRT_DUMP_QUADS16 = 10      /* uint32[nquads*16]: BVH4 nodes, FP16 boxes */
 };
 
 /* Context: one device, one HIP stream.  No counterpart in the reference

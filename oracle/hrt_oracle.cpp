@@ -102,6 +102,10 @@ static inline Node dummy_node() {  // cbvh.h:28-36
   return n;
 }
 
+// BVH4 child (box + reference) and node, see collapse4 below
+struct Child4 { f3 bmin, bmax; uint32_t ref; };
+struct Quad { Child4 c[4]; };
+
 struct Build {
   // inputs (float4 positions with w = 1, triangle indices)
   std::vector<f4> verts;
@@ -125,6 +129,8 @@ struct Build {
   std::vector<Node> peach;
   // leaf-order index buffer (AppendTreeData, BVH2CommonRT_host.cpp:45-64)
   std::vector<uint32_t> idx_leaf_order;
+  // BVH4 collapse (FP32 boxes, FP16-rounded boxes), made on first use
+  std::vector<Quad> q32, q16;
 };
 
 static size_t power_of_two(size_t n) {  // lbvh.cpp:76-82
@@ -677,11 +683,269 @@ static Hit trace_brute(const Build& b, f4 posNear, f4 dirFar) {
   return h;
 }
 
+// ---------------------------------------------------------------------------
+// BVH4: collapse of the Karras BVH2, FP16 boxes, traversal
+// ---------------------------------------------------------------------------
+// Collapse rule (core/builders/cbvh_core.cpp:263-352,
+// ForceGet4ChildernForDynamicLRFormat): a BVH4 node replaces a BVH2 internal
+// node X; its children are X's children with every internal child replaced by
+// its own two children, in left-to-right order (the reference sorts the
+// gathered children by the first leaf they cover, which is that order).
+// Unused slots get box (+1e9, -1e9) and the empty leaf PackOffsetAndSize(0,0)
+// (cbvh_core.cpp:402-409 gives the box).  BVH4 nodes are therefore the BVH2
+// internal nodes at even depth; they are numbered in Karras index order.
+
+static void collapse4(const Build& b, std::vector<Quad>& quads) {
+  quads.clear();
+  const uint32_t L = b.L;
+  const Node* lr = b.lr.data();
+  auto empty_child = []() {
+    Child4 e;
+    e.bmin = {1e9f, 1e9f, 1e9f};
+    e.bmax = {-1e9f, -1e9f, -1e9f};
+    e.ref = pack_leaf(0, 0);
+    return e;
+  };
+  if (L == 1) {  // root is a leaf: one node holding it and the peach dummy
+    Quad q;
+    q.c[0] = Child4{lr[0].bmin, lr[0].bmax, lr[0].left};
+    q.c[1] = Child4{{0, 0, 0}, {0, 0, 0}, 0x80000000u};
+    q.c[2] = empty_child();
+    q.c[3] = empty_child();
+    quads.push_back(q);
+    return;
+  }
+  // depth of every internal node
+  std::vector<int> depth(L - 1, -1);
+  depth[0] = 0;
+  std::vector<uint32_t> st = {0};
+  while (!st.empty()) {
+    uint32_t k = st.back();
+    st.pop_back();
+    for (uint32_t c : {lr[k].left, lr[k].escape})
+      if (c < L - 1) { depth[c] = depth[k] + 1; st.push_back(c); }
+  }
+  std::vector<uint32_t> qidx(L - 1, 0xFFFFFFFFu);
+  uint32_t nq = 0;
+  for (uint32_t k = 0; k + 1 < L; k++)
+    if (depth[k] % 2 == 0) qidx[k] = nq++;
+  quads.resize(nq);
+  // LR index -> child record (leaf refs are the packed leaves, internal refs the quad index)
+  auto child_of = [&](uint32_t idx) {
+    Child4 c;
+    c.bmin = lr[idx].bmin;
+    c.bmax = lr[idx].bmax;
+    c.ref = (idx >= L - 1) ? lr[idx].left : qidx[idx];
+    return c;
+  };
+  for (uint32_t k = 0; k + 1 < L; k++) {
+    if (qidx[k] == 0xFFFFFFFFu) continue;
+    Quad& q = quads[qidx[k]];
+    int n = 0;
+    for (uint32_t c : {lr[k].left, lr[k].escape}) {
+      if (c < L - 1) {
+        q.c[n++] = child_of(lr[c].left);
+        q.c[n++] = child_of(lr[c].escape);
+      } else {
+        q.c[n++] = child_of(c);
+      }
+    }
+    while (n < 4) q.c[n++] = empty_child();
+  }
+}
+
+// float -> half, round to nearest even (IEEE 754 binary16), as half.hpp's
+// default conversion (round_to_nearest, half.hpp:820-834) does.
+static uint16_t f2h_rne(float f) {
+  uint32_t x;
+  std::memcpy(&x, &f, 4);
+  const uint32_t sign = (x >> 16) & 0x8000u;
+  const uint32_t ax = x & 0x7FFFFFFFu;
+  if (ax > 0x7F800000u) return uint16_t(sign | 0x7E00u | ((ax >> 13) & 0x3FFu));  // NaN
+  if (ax >= 0x477FF000u) return uint16_t(sign | 0x7C00u);  // >= 65520 rounds to infinity
+  const uint32_t e = ax >> 23;
+  if (e >= 113) {  // normal half
+    const uint32_t r = ax - 0x38000000u;
+    uint32_t h = r >> 13;
+    const uint32_t rem = r & 0x1FFFu;
+    if (rem > 0x1000u || (rem == 0x1000u && (h & 1u))) h++;
+    return uint16_t(sign | h);
+  }
+  // subnormal half (or zero): value / 2^-24 rounded to nearest even
+  const uint32_t m = (ax & 0x7FFFFFu) | (e ? 0x800000u : 0u);
+  const uint32_t shift = (e ? 126u - e : 125u);
+  if (shift > 31) return uint16_t(sign);
+  uint32_t q = m >> shift;
+  const uint32_t rem = m & ((1u << shift) - 1u);
+  const uint32_t half = 1u << (shift - 1u);
+  if (rem > half || (rem == half && (q & 1u))) q++;
+  return uint16_t(sign | q);
+}
+
+// half.hpp nextafter (half.hpp:4291-4308) restricted to non-NaN inputs.
+static uint16_t h_nextafter(uint16_t from, uint16_t to) {
+  const uint32_t fa = from & 0x7FFFu, ta = to & 0x7FFFu;
+  if (fa > 0x7C00u || ta > 0x7C00u) return from;
+  if (from == to || !(fa | ta)) return to;
+  if (!fa) return uint16_t((to & 0x8000u) + 1u);
+  auto ord = [](uint32_t v) { return v ^ (0x8000u | (0x8000u - (v >> 15))); };
+  const uint32_t up = ((from >> 15) ^ uint32_t(ord(from) < ord(to))) << 1;
+  return uint16_t(from + up - 1u);
+}
+static uint16_t round_to_neg(float v) { return h_nextafter(f2h_rne(v), 0xFBFFu); }  // cbvh_fat.cpp:447
+static uint16_t round_to_pos(float v) { return h_nextafter(f2h_rne(v), 0x7BFFu); }  // cbvh_fat.cpp:448
+
+static float h2f(uint16_t h) {
+  const uint32_t sign = uint32_t(h & 0x8000u) << 16;
+  const uint32_t e = (h >> 10) & 0x1Fu, m = h & 0x3FFu;
+  uint32_t x;
+  if (e == 0) {
+    if (m == 0) x = sign;
+    else {  // subnormal: m * 2^-24
+      float f = float(m) * (1.0f / 16777216.0f);
+      std::memcpy(&x, &f, 4);
+      x |= sign;
+    }
+  } else if (e == 31) x = sign | 0x7F800000u | (m << 13);
+  else x = sign | ((e + 112u) << 23) | (m << 13);
+  float f;
+  std::memcpy(&f, &x, 4);
+  return f;
+}
+
+// FP16 child: PackNode (cbvh_fat.cpp:477-497) then UnpackBVHNode
+// (BVH4HalfRT.cpp:56-71) -- the FP32 box the traversal actually tests.
+static Child4 fp16_child(const Child4& c) {
+  Child4 r;
+  r.bmin = {h2f(round_to_neg(c.bmin.x)), h2f(round_to_neg(c.bmin.y)), h2f(round_to_neg(c.bmin.z))};
+  r.bmax = {h2f(round_to_pos(c.bmax.x)), h2f(round_to_pos(c.bmax.y)), h2f(round_to_pos(c.bmax.z))};
+  r.ref = c.ref;
+  return r;
+}
+
+// Nearest-hit rule of the BVH4 path: (t, original primId) lexicographic, so
+// the answer does not depend on the visiting order (FP16 and FP32 nodes
+// visit in different orders).  Any-hit stops at the first accepted triangle.
+static inline bool mt_test_lex(f3 o, f3 d, float tNear, f3 A, f3 B, f3 C, Hit& h, uint32_t prim) {
+  const f3 e1 = sub3(B, A);
+  const f3 e2 = sub3(C, A);
+  const f3 pv = cross3(d, e2);
+  const f3 tv = sub3(o, A);
+  const f3 qv = cross3(tv, e1);
+  const float invDet = 1.0f / dot3(e1, pv);
+  const float v = dot3(tv, pv) * invDet;
+  const float u = dot3(qv, d) * invDet;
+  const float t = dot3(e2, qv) * invDet;
+  // t == tFar with no hit yet is outside the open interval (tNear, tFar)
+  if (v >= -1e-6f && u >= -1e-6f && (u + v <= 1.0f + 1e-6f) && t > tNear &&
+      (t < h.t || (t == h.t && h.primId != 0xFFFFFFFFu && prim < h.primId))) {
+    h.t = t;
+    h.primId = prim;
+    h.instId = 0;
+    h.geomId = 0;
+    h.u = u;
+    h.v = v;
+    return true;
+  }
+  return false;
+}
+
+// BVH4HalfRT::BVH4Traverse order (BVH4HalfRT.cpp:73-256): four slab tests,
+// misses get MAXFLOAT, five-comparator network on the entry distances, push
+// w, z, y (if hit), continue with x.
+static Hit trace_bvh4(const Build& b, const std::vector<Quad>& quads, f4 posNear, f4 dirFar, Counters* cnt) {
+  const bool anyhit = (dirFar.w <= 0.0f);
+  const float tFar = anyhit ? -dirFar.w : dirFar.w;
+  Hit h;
+  h.t = tFar;
+  h.primId = h.instId = h.geomId = 0xFFFFFFFFu;
+  h.u = h.v = 0.0f;
+  const f3 o = {posNear.x, posNear.y, posNear.z};
+  const f3 d = {dirFar.x, dirFar.y, dirFar.z};
+  const float tNear = posNear.w;
+  const f3 inv = safe_inverse(d);
+  uint32_t stack[64];
+  int top = 0;
+  uint32_t cur = 0;
+  const float MAXF = 3.402823466e+38f;
+  while (top >= 0) {
+    bool pop = false;
+    if ((cur & LEAF_BIT) == 0) {
+      const Quad& q = quads[cur];
+      if (cnt) cnt->pairs++;
+      float tm[4];
+      uint32_t ch[4];
+      for (int k = 0; k < 4; k++) {
+        float a, c;
+        slab(o, inv, q.c[k].bmin, q.c[k].bmax, a, c);
+        const bool hit = (a <= c) && (c >= tNear) && (a <= h.t);
+        tm[k] = hit ? a : MAXF;
+        ch[k] = q.c[k].ref;
+      }
+      auto cswap = [&](int i, int j) {  // swap when tm[j] < tm[i]
+        if (tm[j] < tm[i]) { std::swap(tm[i], tm[j]); std::swap(ch[i], ch[j]); }
+      };
+      cswap(0, 1);
+      cswap(2, 3);
+      cswap(0, 2);
+      cswap(1, 3);
+      cswap(1, 2);
+      if (tm[3] < MAXF) stack[top++] = ch[3];
+      if (tm[2] < MAXF) stack[top++] = ch[2];
+      if (tm[1] < MAXF) stack[top++] = ch[1];
+      if (tm[0] < MAXF) cur = ch[0];
+      pop = !(tm[0] < MAXF);
+    }
+    if (!pop && (cur & LEAF_BIT) != 0 && cur != 0xFFFFFFFFu) {
+      const uint32_t start = leaf_start(cur), count = leaf_count(cur);
+      if (cnt) { cnt->leaves++; cnt->tris += count; }
+      for (uint32_t t = start; t < start + count; t++) {
+        const uint32_t* ix = &b.idx_leaf_order[3 * size_t(t)];
+        if (mt_test_lex(o, d, tNear, vpos(b, ix[0]), vpos(b, ix[1]), vpos(b, ix[2]), h, b.perm[t]) && anyhit) break;
+      }
+      pop = true;
+      if (anyhit && h.primId != 0xFFFFFFFFu) break;
+    }
+    if (pop) {
+      top--;
+      cur = stack[std::max(top, 0)];
+    }
+  }
+  return h;
+}
+
+// Brute force with the (t, primId) rule.
+static Hit trace_brute_lex(const Build& b, f4 posNear, f4 dirFar) {
+  const bool anyhit = (dirFar.w <= 0.0f);
+  Hit h;
+  h.t = anyhit ? -dirFar.w : dirFar.w;
+  h.primId = h.instId = h.geomId = 0xFFFFFFFFu;
+  h.u = h.v = 0.0f;
+  const f3 o = {posNear.x, posNear.y, posNear.z};
+  const f3 d = {dirFar.x, dirFar.y, dirFar.z};
+  for (uint32_t t = 0; t < b.ntris; t++)
+    mt_test_lex(o, d, posNear.w, vpos(b, b.idx[3 * t]), vpos(b, b.idx[3 * t + 1]), vpos(b, b.idx[3 * t + 2]), h, t);
+  return h;
+}
+
 }  // namespace
 
 // ===========================================================================
 // C entry points (loaded by tests/ and bench.py through ctypes)
 // ===========================================================================
+namespace {
+std::vector<Quad>& quads_for(const Build* cb, int fp16) {
+  Build* b = const_cast<Build*>(cb);
+  if (b->q32.empty()) {
+    collapse4(*b, b->q32);
+    b->q16 = b->q32;
+    for (auto& q : b->q16)
+      for (auto& c : q.c) c = fp16_child(c);
+  }
+  return fp16 ? b->q16 : b->q32;
+}
+}  // namespace
+
 extern "C" {
 
 void* hrto_build(const float* vert4, uint32_t nVert, const uint32_t* idx, uint32_t nIdx) {
@@ -800,6 +1064,49 @@ void hrto_t_fp64(void* h, const float* rays, const uint32_t* prim, uint64_t n, d
     out_t[i] = (e2[0] * qv[0] + e2[1] * qv[1] + e2[2] * qv[2]) / det;
   }
 }
+
+// BVH4 nodes: quads[i] = 4 children x 8 words (bmin xyz, ref, bmax xyz, 0);
+// fp16 = 1 gives the boxes after FP16 packing and unpacking.
+uint64_t hrto_quad_count(void* h) { return quads_for(static_cast<Build*>(h), 0).size(); }
+void hrto_quads(void* h, int fp16, uint32_t* out) {
+  const std::vector<Quad>& q = quads_for(static_cast<Build*>(h), fp16);
+  for (size_t i = 0; i < q.size(); i++)
+    for (int k = 0; k < 4; k++) {
+      uint32_t* w = out + 32 * i + 8 * k;
+      std::memcpy(w + 0, &q[i].c[k].bmin, 12);
+      w[3] = q[i].c[k].ref;
+      std::memcpy(w + 4, &q[i].c[k].bmax, 12);
+      w[7] = 0;
+    }
+}
+
+// BVH4 traversal (mode 0: FP32 boxes, 1: FP16 boxes) and brute force with
+// the (t, primId) rule (mode 2).
+void hrto_trace4(void* h, int mode, const float* rays, uint64_t n, uint32_t* hits, uint32_t* counters, int threads) {
+  const Build* b = static_cast<Build*>(h);
+  const std::vector<Quad>* q = (mode == 2) ? nullptr : &quads_for(b, mode);
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 256) if (threads != 1)
+#endif
+  for (int64_t i = 0; i < int64_t(n); i++) {
+    f4 p, d;
+    std::memcpy(&p, rays + 8 * i, 16);
+    std::memcpy(&d, rays + 8 * i + 4, 16);
+    Counters c;
+    Hit r = q ? trace_bvh4(*b, *q, p, d, counters ? &c : nullptr) : trace_brute_lex(*b, p, d);
+    std::memcpy(hits + 6 * i, &r, sizeof(Hit));
+    if (counters && q) {
+      counters[2 * i + 0] = uint32_t(c.pairs);
+      counters[2 * i + 1] = uint32_t(c.tris);
+    }
+  }
+}
+
+uint32_t hrto_f2h(float v) { return f2h_rne(v); }
+uint32_t hrto_round_neg(float v) { return round_to_neg(v); }
+uint32_t hrto_round_pos(float v) { return round_to_pos(v); }
+float hrto_h2f(uint32_t h) { return h2f(uint16_t(h)); }
 
 // Reference building blocks exposed for known-answer tests.
 uint32_t hrto_zindex3d(int x, int y, int z) { return zindex3d(x, y, z); }

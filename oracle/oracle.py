@@ -24,6 +24,26 @@ MODE_BVH2COMMON = 0
 MODE_LR = 1
 MODE_BRUTE = 2
 
+MODE4_FP32 = 0
+MODE4_FP16 = 1
+MODE4_BRUTE = 2
+
+
+def f2h(v: float) -> int:
+    return int(lib().hrto_f2h(v))
+
+
+def round_to_neg(v: float) -> int:
+    return int(lib().hrto_round_neg(v))
+
+
+def round_to_pos(v: float) -> int:
+    return int(lib().hrto_round_pos(v))
+
+
+def h2f(h: int) -> float:
+    return float(lib().hrto_h2f(h))
+
 
 def build_library() -> str:
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
@@ -56,6 +76,15 @@ def lib():
         L.hrto_pack_leaf.argtypes = [u32, u32]
         L.hrto_safe_inverse.argtypes = [vp, vp]
         L.hrto_slab.argtypes = [vp, vp, vp, vp, vp]
+        L.hrto_quad_count.restype = u64
+        L.hrto_quad_count.argtypes = [vp]
+        L.hrto_quads.argtypes = [vp, C.c_int, vp]
+        L.hrto_trace4.argtypes = [vp, C.c_int, vp, u64, vp, vp, C.c_int]
+        for name in ("hrto_f2h", "hrto_round_neg", "hrto_round_pos"):
+            getattr(L, name).restype = u32
+            getattr(L, name).argtypes = [C.c_float]
+        L.hrto_h2f.restype = C.c_float
+        L.hrto_h2f.argtypes = [u32]
         _lib = L
     return _lib
 
@@ -136,6 +165,30 @@ class OracleBVH:
         hits = np.zeros((n, 6), dtype=np.uint32)
         cnt = np.zeros((n, 2), dtype=np.uint32) if counters else None
         lib().hrto_trace(self._h, mode, _ptr(rays), n, _ptr(hits), _ptr(cnt) if counters else None, threads)
+        t = hits[:, 0].view(np.float32).copy()
+        prim = hits[:, 1].copy()
+        u = hits[:, 4].view(np.float32).copy()
+        v = hits[:, 5].view(np.float32).copy()
+        if counters:
+            return t, prim, u, v, cnt[:, 0].copy(), cnt[:, 1].copy()
+        return t, prim, u, v
+
+    # -- BVH4 (collapse of the same LBVH) ---------------------------------
+    def quads(self, fp16: bool = False) -> np.ndarray:
+        """(nquads, 4, 8) uint32: per child bmin xyz, ref, bmax xyz, 0."""
+        n = int(lib().hrto_quad_count(self._h))
+        out = np.zeros((n, 4, 8), dtype=np.uint32)
+        lib().hrto_quads(self._h, 1 if fp16 else 0, _ptr(out))
+        return out
+
+    def trace4(self, rays: np.ndarray, mode: int = MODE4_FP32, counters: bool = False, threads: int = 0):
+        """BVH4 traversal (MODE4_FP32 / MODE4_FP16) or brute force (MODE4_BRUTE),
+        nearest hit by (t, primId).  Returns (t, primId, u, v[, quads, tris])."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        hits = np.zeros((n, 6), dtype=np.uint32)
+        cnt = np.zeros((n, 2), dtype=np.uint32) if counters else None
+        lib().hrto_trace4(self._h, mode, _ptr(rays), n, _ptr(hits), _ptr(cnt) if counters else None, threads)
         t = hits[:, 0].view(np.float32).copy()
         prim = hits[:, 1].copy()
         u = hits[:, 4].view(np.float32).copy()

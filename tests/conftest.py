@@ -34,6 +34,10 @@ def gpu_ctx():
     if not torch.cuda.is_available():
         pytest.fail("GPU test scheduled but no GPU is visible")
     torch.cuda.init()
+    # torch's default stream is the legacy null stream (handle 0), which the
+    # engine would read as "use the context stream"; run every torch copy and
+    # every engine launch on one explicit stream instead.
+    torch.cuda.set_stream(torch.cuda.Stream())
     import heavyrt_amd as H
 
     ctx = H.Context(0)

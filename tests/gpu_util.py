@@ -28,6 +28,7 @@ def trace(scene, rays: np.ndarray, mode: str = "compact", stats: bool = False):
     dev = torch.from_numpy(rays).cuda()
     st = torch.zeros((max(n, 1), 2), dtype=torch.int32, device="cuda") if stats else None
     stream = torch.cuda.current_stream().cuda_stream
+    assert stream, "tests run on an explicit torch stream (see conftest.gpu_ctx)"
     if mode == "any":
         out = torch.zeros(max(n, 1), dtype=torch.uint8, device="cuda")
         scene.trace_any(dev, n, out, stream=stream, stats=st)

@@ -94,6 +94,12 @@ def test_nearest_bit_exact(gpu_ctx, name, gen):
     assert (g["inst"][hit] == 0).all() and (g["inst"][~hit] == INVALID).all()
     np.testing.assert_array_equal(g["pairs"], pairs)
     np.testing.assert_array_equal(g["tris"], tris)
+    # the persistent kernel (no counters) gives the same records
+    g2 = trace(scene, rays, "full")
+    for k in ("t", "u", "v"):
+        np.testing.assert_array_equal(g2[k].view(np.uint32), g[k].view(np.uint32))
+    for k in ("prim", "inst", "geom"):
+        np.testing.assert_array_equal(g2[k], g[k])
     # compact format carries the same (t, primId)
     c = trace(scene, rays, "compact")
     np.testing.assert_array_equal(c["t"].view(np.uint32), t.view(np.uint32))
@@ -116,6 +122,9 @@ def test_any_hit_exact(gpu_ctx, name, gen):
     np.testing.assert_array_equal(g["hit"], p != INVALID)
     np.testing.assert_array_equal(g["pairs"], pairs)
     np.testing.assert_array_equal(g["tris"], tris)
+    # persistent speculative any-hit kernel (no counters): same booleans
+    g2 = trace(scene, rays, "any")
+    np.testing.assert_array_equal(g2["hit"], p != INVALID)
     # negative tFar through the nearest-hit entry follows the same stop-at-first-hit rule
     n = trace(scene, anyr, "compact")
     np.testing.assert_array_equal(n["prim"] != INVALID, p != INVALID)
