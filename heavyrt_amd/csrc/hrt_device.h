@@ -25,6 +25,7 @@ constexpr uint32_t kStartMask = 0x00FFFFFFu;
 constexpr uint32_t kSizeMask = 0x7F000000u;
 constexpr uint32_t kLeafBit = 0x80000000u;
 constexpr uint32_t kInvalid = 0xFFFFFFFFu;
+constexpr uint32_t kEmptyLeaf = 0x80000000u;  // PackOffsetAndSize(0, 0): a leaf without triangles
 // Deepest possible BLAS path: the Karras tree over unique 30-bit codes has at
 // most 30 internal levels, so at most 30 deferred children are ever pending.
 constexpr int kStackDepth = 32;
@@ -59,6 +60,29 @@ struct __attribute__((aligned(16))) TriPre {
   float4 e1;  // xyz = B - A
   float4 e2;  // xyz = C - A
 };
+
+// Per-lane traversal stack of a persistent kernel: the first kLds entries in
+// LDS (entry k of the lane at lds[k * kStride]), deeper entries in a global
+// spill area that the lane owns for the whole launch (entry k at
+// spill[(k - kLds) * spill_stride]).  Typical traversals stay well inside the
+// LDS part; the spill area makes the proven worst-case depth safe while the
+// LDS footprint stays small enough for full occupancy.
+template <int kLds, int kStride>
+struct ShortStack {
+  uint32_t* lds;
+  uint32_t* spill;
+  uint32_t spill_stride;
+  __device__ inline void push(int& top, uint32_t v) const {
+    if (top < kLds) lds[top * kStride] = v;
+    else spill[size_t(top - kLds) * spill_stride] = v;
+    top++;
+  }
+  __device__ inline uint32_t pop(int& top) const {
+    --top;
+    return (top < kLds) ? lds[top * kStride] : spill[size_t(top - kLds) * spill_stride];
+  }
+};
+constexpr int kShortStack = 16;  // LDS entries per lane in the persistent kernels
 
 // BVH4 node with FP32 boxes, 128 bytes, coordinates of the four children
 // grouped per axis so each 16-byte load feeds four slab tests.  Children are

@@ -132,10 +132,13 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_persistent(const NodePair
                                                                   const TriPre* __restrict__ tris,
                                                                   const Ray* __restrict__ rays, uint64_t n,
                                                                   void* out, int format,
-                                                                  uint32_t* __restrict__ queue) {
-  __shared__ uint32_t stack_mem[kStackDepth * kTraceBlock];
+                                                                  uint32_t* __restrict__ queue,
+                                                                  uint32_t* __restrict__ spill) {
+  __shared__ uint32_t stack_mem[kShortStack * kTraceBlock];
   const int lane = threadIdx.x & (kWave - 1);
-  uint32_t* stack = stack_mem + threadIdx.x;
+  const uint32_t lanes = gridDim.x * kTraceBlock;
+  const ShortStack<kShortStack, kTraceBlock> stk{stack_mem + threadIdx.x,
+                                                 spill + blockIdx.x * kTraceBlock + threadIdx.x, lanes};
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
 
   // per-lane ray state
@@ -194,7 +197,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_persistent(const NodePair
           // park a leaf reached in `cur` (at most one per lane)
           if (active && leaf == 0 && (cur & kLeafBit) != 0 && cur != kInvalid) {
             leaf = cur;
-            cur = (top > 0) ? stack[--top * kTraceBlock] : kInvalid;
+            cur = (top > 0) ? stk.pop(top) : kInvalid;
           }
           const bool searching = active && leaf == 0 && (cur & kLeafBit) == 0;
           if (!__any(searching)) break;
@@ -209,12 +212,11 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_persistent(const NodePair
             if (h0 && h1) {
               const bool left_first = t0.x <= t1.x;
               cur = left_first ? c0 : c1;
-              stack[top * kTraceBlock] = left_first ? c1 : c0;
-              top++;
+              stk.push(top, left_first ? c1 : c0);
             } else if (h0 || h1) {
               cur = h0 ? c0 : c1;
             } else {
-              cur = (top > 0) ? stack[--top * kTraceBlock] : kInvalid;
+              cur = (top > 0) ? stk.pop(top) : kInvalid;
             }
           }
         }
@@ -241,17 +243,16 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_persistent(const NodePair
             if (h0 && h1) {
               const bool left_first = t0.x <= t1.x;
               cur = left_first ? c0 : c1;
-              stack[top * kTraceBlock] = left_first ? c1 : c0;
-              top++;
+              stk.push(top, left_first ? c1 : c0);
             } else if (h0 || h1) {
               cur = h0 ? c0 : c1;
             } else {
-              cur = (top > 0) ? stack[--top * kTraceBlock] : kInvalid;
+              cur = (top > 0) ? stk.pop(top) : kInvalid;
             }
           } else if (cur != kInvalid) {
             const uint32_t s = leaf_start(cur), c = leaf_count(cur);
             for (uint32_t k = s; k < s + c; k++) mt_test(o, d, tnear, tris[k], h);
-            cur = (top > 0) ? stack[--top * kTraceBlock] : kInvalid;
+            cur = (top > 0) ? stk.pop(top) : kInvalid;
           }
           done = (cur == kInvalid) || (any && h.prim != kInvalid);
         }
@@ -288,17 +289,19 @@ int persistent_grid(const void* kernel) {
 
 template <bool kAny, bool kSpec>
 hipError_t launch_persistent(const Blas& b, const TraceArgs& a, hipStream_t stream) {
-  uint32_t* queue = nullptr;
-  hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&queue), sizeof(uint32_t), stream);
-  if (e != hipSuccess) return e;
-  k_reset_counter<<<1, 1, 0, stream>>>(queue);
   const void* kfn = reinterpret_cast<const void*>(&k_trace_persistent<kAny, kSpec>);
   uint64_t want = (a.n + kTraceBlock - 1) / kTraceBlock;
   const int grid = int(std::min<uint64_t>(want, uint64_t(persistent_grid(kfn))));
+  // queue counter + spill area for stack entries beyond the LDS part
+  const size_t spill_words = size_t(grid) * kTraceBlock * (kStackDepth - kShortStack);
+  uint32_t* buf = nullptr;
+  hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&buf), sizeof(uint32_t) * (64 + spill_words), stream);
+  if (e != hipSuccess) return e;
+  k_reset_counter<<<1, 1, 0, stream>>>(buf);
   k_trace_persistent<kAny, kSpec><<<grid, kTraceBlock, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format,
-                                                                   queue);
+                                                                   buf, buf + 64);
   e = hipGetLastError();
-  hipError_t f = hipFreeAsync(queue, stream);
+  hipError_t f = hipFreeAsync(buf, stream);
   return e != hipSuccess ? e : f;
 }
 
@@ -317,7 +320,11 @@ hipError_t trace_blas(const Blas& b, const TraceArgs& a, hipStream_t stream) {
     return hipGetLastError();
   }
   if (a.anyhit) return launch_persistent<true, true>(b, a, stream);
-  return launch_persistent<false, false>(b, a, stream);
+  // Nearest hits: one ray per lane.  Rays that arrive in screen order stay
+  // coherent within a wave, which the refilling kernel gives up.
+  const uint32_t blocks = uint32_t((a.n + 255) / 256);
+  k_trace_simple<false, false><<<blocks, 256, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format, nullptr);
+  return hipGetLastError();
 }
 
 }  // namespace hrt

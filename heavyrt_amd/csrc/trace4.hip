@@ -68,7 +68,10 @@ __device__ inline void visit_quad(const void* __restrict__ quads, uint32_t q, fl
 #pragma unroll
   for (int k = 0; k < 4; k++) {
     const float2 t = slab(o, inv, lox[k], loy[k], loz[k], hix[k], hiy[k], hiz[k]);
-    const bool hit = (t.x <= t.y) && (t.y >= tnear) && (t.x <= tcull);
+    // An unused slot (box +1e9..-1e9, empty leaf 0x80000000) would pass the
+    // min/max slab test as the box [-1e9, 1e9]; it holds no triangle, so it
+    // is never entered.
+    const bool hit = (t.x <= t.y) && (t.y >= tnear) && (t.x <= tcull) && ch[k] != kEmptyLeaf;
     tm[k] = hit ? t.x : kMaxFloat;
   }
   // five-comparator network, BVH4HalfRT.cpp:121-180 (swap when strictly less)
@@ -167,10 +170,12 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_any_persistent(const void* _
                                                                    const TriPre* __restrict__ tris,
                                                                    const Ray* __restrict__ rays, uint64_t n,
                                                                    uint8_t* __restrict__ out,
-                                                                   uint32_t* __restrict__ queue) {
-  __shared__ uint32_t stack_mem[kStackDepth4 * kBlock4];
+                                                                   uint32_t* __restrict__ queue,
+                                                                   uint32_t* __restrict__ spill) {
+  __shared__ uint32_t stack_mem[kShortStack * kBlock4];
   const int lane = threadIdx.x & (kWave - 1);
-  uint32_t* stack = stack_mem + threadIdx.x;
+  const ShortStack<kShortStack, kBlock4> stk{stack_mem + threadIdx.x, spill + blockIdx.x * kBlock4 + threadIdx.x,
+                                             gridDim.x * kBlock4};
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
   bool active = false;
   uint64_t rid = 0;
@@ -218,7 +223,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_any_persistent(const void* _
       while (true) {
         if (active && leaf == 0 && (cur & kLeafBit) != 0 && cur != kInvalid) {
           leaf = cur;
-          cur = (top > 0) ? stack[(--top) * kBlock4] : kInvalid;
+          cur = (top > 0) ? stk.pop(top) : kInvalid;
         }
         const bool searching = active && leaf == 0 && (cur & kLeafBit) == 0;
         if (!__any(searching)) break;
@@ -226,11 +231,11 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_any_persistent(const void* _
           float tm[4];
           uint32_t ch[4];
           visit_quad<kFp16>(quads, cur, o, inv, tnear, h.t, tm, ch);
-          if (tm[3] < kMaxFloat) stack[(top++) * kBlock4] = ch[3];
-          if (tm[2] < kMaxFloat) stack[(top++) * kBlock4] = ch[2];
-          if (tm[1] < kMaxFloat) stack[(top++) * kBlock4] = ch[1];
+          if (tm[3] < kMaxFloat) stk.push(top, ch[3]);
+          if (tm[2] < kMaxFloat) stk.push(top, ch[2]);
+          if (tm[1] < kMaxFloat) stk.push(top, ch[1]);
           if (tm[0] < kMaxFloat) cur = ch[0];
-          else cur = (top > 0) ? stack[(--top) * kBlock4] : kInvalid;
+          else cur = (top > 0) ? stk.pop(top) : kInvalid;
         }
       }
       // leaf phase
@@ -252,10 +257,6 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_any_persistent(const void* _
 
 template <bool kFp16>
 hipError_t launch_any4(const Blas& b, const TraceArgs& a, hipStream_t stream) {
-  uint32_t* queue = nullptr;
-  hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&queue), sizeof(uint32_t), stream);
-  if (e != hipSuccess) return e;
-  k_reset_counter4<<<1, 1, 0, stream>>>(queue);
   int dev = 0, cus = 256, per_cu = 0;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
@@ -264,10 +265,16 @@ hipError_t launch_any4(const Blas& b, const TraceArgs& a, hipStream_t stream) {
   const uint64_t want = (a.n + kBlock4 - 1) / kBlock4;
   const int grid = int(std::min<uint64_t>(want, uint64_t(std::max(cus, 1)) * uint64_t(per_cu)));
   const void* q = kFp16 ? static_cast<const void*>(b.quads16) : static_cast<const void*>(b.quads32);
+  // queue counter + spill area for stack entries beyond the LDS part
+  const size_t spill_words = size_t(grid) * kBlock4 * (kStackDepth4 - kShortStack);
+  uint32_t* buf = nullptr;
+  hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&buf), sizeof(uint32_t) * (64 + spill_words), stream);
+  if (e != hipSuccess) return e;
+  k_reset_counter4<<<1, 1, 0, stream>>>(buf);
   k_trace4_any_persistent<kFp16><<<grid, kBlock4, 0, stream>>>(q, b.tris, a.rays, a.n, static_cast<uint8_t*>(a.out),
-                                                               queue);
+                                                               buf, buf + 64);
   e = hipGetLastError();
-  hipError_t f = hipFreeAsync(queue, stream);
+  hipError_t f = hipFreeAsync(buf, stream);
   return e != hipSuccess ? e : f;
 }
 

@@ -529,7 +529,7 @@ static inline f3 vpos(const Build& b, uint32_t vi) {
   return {v.x, v.y, v.z};
 }
 
-struct Counters { uint64_t pairs = 0, tris = 0, leaves = 0; };
+struct Counters { uint64_t pairs = 0, tris = 0, leaves = 0, max_stack = 0; };
 
 // BVH2CommonRT::BVH2TraverseC32 over the peach layout (BVH2CommonRT.cpp:49-141).
 // primId is reported as the ORIGINAL triangle id (the REMAP_PRIM_ID rule of
@@ -566,6 +566,7 @@ static Hit trace_bvh2common(const Build& b, f4 posNear, f4 dirFar, Counters* cnt
         off = (a0 <= a1) ? n0.left : n1.left;
         stack[top] = (a0 <= a1) ? n1.left : n0.left;
         top++;
+        if (cnt && uint64_t(top) > cnt->max_stack) cnt->max_stack = uint64_t(top);
       }
     } else if (off != 0xFFFFFFFFu) {
       const uint32_t start = leaf_start(off), count = leaf_count(off);
@@ -878,7 +879,11 @@ static Hit trace_bvh4(const Build& b, const std::vector<Quad>& quads, f4 posNear
       for (int k = 0; k < 4; k++) {
         float a, c;
         slab(o, inv, q.c[k].bmin, q.c[k].bmax, a, c);
-        const bool hit = (a <= c) && (c >= tNear) && (a <= h.t);
+        // An unused slot (+1e9, -1e9) is inside out; the min/max slab test
+        // turns it into the box [-1e9, 1e9], so the reference enters it and
+        // finds its empty leaf.  Skipping slots whose reference is the empty
+        // leaf PackOffsetAndSize(0, 0) gives the same hits without the visit.
+        const bool hit = (a <= c) && (c >= tNear) && (a <= h.t) && q.c[k].ref != 0x80000000u;
         tm[k] = hit ? a : MAXF;
         ch[k] = q.c[k].ref;
       }
@@ -895,6 +900,7 @@ static Hit trace_bvh4(const Build& b, const std::vector<Quad>& quads, f4 posNear
       if (tm[1] < MAXF) stack[top++] = ch[1];
       if (tm[0] < MAXF) cur = ch[0];
       pop = !(tm[0] < MAXF);
+      if (cnt && uint64_t(top) > cnt->max_stack) cnt->max_stack = uint64_t(top);
     }
     if (!pop && (cur & LEAF_BIT) != 0 && cur != 0xFFFFFFFFu) {
       const uint32_t start = leaf_start(cur), count = leaf_count(cur);
@@ -1042,6 +1048,25 @@ void hrto_trace(void* h, int mode, const float* rays, uint64_t n, uint32_t* hits
 }
 
 // FP64 Moller-Trumbore t for a given triangle (t-error reporting).
+// Deepest traversal stack per ray (mode 0: BVH2 peach order, 1: BVH4 FP32).
+void hrto_max_stack(void* h, int mode, const float* rays, uint64_t n, uint32_t* out, int threads) {
+  const Build* b = static_cast<Build*>(h);
+  const std::vector<Quad>* q = (mode == 1) ? &quads_for(b, 0) : nullptr;
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 256) if (threads != 1)
+#endif
+  for (int64_t i = 0; i < int64_t(n); i++) {
+    f4 p, d;
+    std::memcpy(&p, rays + 8 * i, 16);
+    std::memcpy(&d, rays + 8 * i + 4, 16);
+    Counters c;
+    if (q) (void)trace_bvh4(*b, *q, p, d, &c);
+    else (void)trace_bvh2common(*b, p, d, &c);
+    out[i] = uint32_t(c.max_stack);
+  }
+}
+
 void hrto_t_fp64(void* h, const float* rays, const uint32_t* prim, uint64_t n, double* out_t) {
   const Build* b = static_cast<Build*>(h);
   for (uint64_t i = 0; i < n; i++) {
