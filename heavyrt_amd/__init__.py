@@ -58,6 +58,7 @@ EXPORTS = [
     "hrt_add_triangles", "hrt_add_instance", "hrt_commit", "hrt_trace_nearest", "hrt_trace_any",
     "hrt_trace_nearest_stats", "hrt_trace_any_stats", "hrt_query_nearest_one", "hrt_geom_count",
     "hrt_instance_count", "hrt_debug_dump", "hrt_last_error", "hrt_rtao_make_rays",
+    "hrt_trace_nearest_host", "hrt_trace_any_host",
 ]
 
 
@@ -116,6 +117,8 @@ def lib():
         L.hrt_instance_count.restype = u32
         L.hrt_debug_dump.argtypes = [vp, u32, i32, vp, sz]
         L.hrt_rtao_make_rays.argtypes = [vp, vp, vp, sz, u32, C.c_float, C.c_uint64, vp, vp]
+        L.hrt_trace_nearest_host.argtypes = [vp, vp, sz, vp, vp]
+        L.hrt_trace_any_host.argtypes = [vp, vp, sz, vp]
         _lib = L
     return _lib
 

@@ -63,18 +63,29 @@ def build_hrt(verbose: bool = True) -> str:
 def build_scene_adapter() -> str:
     """C++ ISceneObject adapter (host compiler only) over libhrt.so."""
     src = os.path.join(HOST, "HipSceneRT.cpp")
-    if not os.path.exists(src):
-        return ""
     tmp = LIB_SCENE + ".tmp"
-    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-I", INCLUDE, "-o", tmp, src,
+    _run(["g++", "-O2", "-std=c++17", "-fPIC", "-shared", "-Wall", "-I", INCLUDE, "-o", tmp, src,
           "-L", HERE, "-lhrt", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmp, LIB_SCENE)
     return LIB_SCENE
 
 
+HARNESS_SRC = os.path.join(ROOT, "tests", "cpp", "scene_harness.cpp")
+HARNESS_BIN = os.path.join(ROOT, "tests", "cpp", "_build", "scene_harness")
+
+
+def build_harness() -> str:
+    """Renderer-style C++ caller of ISceneObject used by the GPU tests."""
+    os.makedirs(os.path.dirname(HARNESS_BIN), exist_ok=True)
+    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", INCLUDE, "-o", HARNESS_BIN, HARNESS_SRC,
+          "-L", HERE, "-lhrt_scene", "-lhrt", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../../../heavyrt_amd"])
+    return HARNESS_BIN
+
+
 def build_all() -> None:
     build_hrt()
     build_scene_adapter()
+    build_harness()
 
 
 if __name__ == "__main__":

@@ -61,6 +61,9 @@ struct hrt_scene {
   int layout = HRT_LAYOUT_BVH2;
   // staging for hrt_query_nearest_one
   std::mutex one_mu;
+  // staging for the host-memory batch calls (grown on demand, guarded by one_mu)
+  void* host_dev = nullptr;
+  size_t host_dev_bytes = 0;
   hrt::Ray* one_ray = nullptr;
   hrt_hit* one_hit = nullptr;
 };
@@ -95,6 +98,8 @@ hrt_status trace_common(hrt_scene* s, const float* rays, size_t n, void* out, in
   a.format = format;
   a.anyhit = anyhit;
   a.stats = stats;
+  a.inst_id = inst;
+  a.geom_id = s->instances[inst].geom;
   hipStream_t hs = stream ? static_cast<hipStream_t>(stream) : s->ctx->stream;
   hipError_t e = hipSetDevice(s->ctx->device);
   if (e == hipSuccess) {
@@ -175,6 +180,7 @@ void hrt_scene_destroy(hrt_scene* s) {
   if (!s) return;
   hrt_scene_clear_geom(s);
   if (s->one_ray) (void)hipFree(s->one_ray);
+  if (s->host_dev) (void)hipFree(s->host_dev);
   if (s->one_hit) (void)hipFree(s->one_hit);
   delete s;
 }
@@ -292,6 +298,51 @@ hrt_status hrt_trace_any_stats(hrt_scene* s, const float* rays, size_t n, uint8_
   return trace_common(s, rays, n, out, HRT_HIT_FULL, 1, stats, stream);
 }
 
+namespace {
+// Host batch: upload rays, trace, download results (and counters).
+hrt_status host_batch(hrt_scene* s, const float* rays, size_t n, void* out, size_t out_bytes, int anyhit,
+                      uint32_t* stats) {
+  if (!s || (!rays && n) || (!out && n)) return fail(HRT_ERR_INVALID_ARG, "null argument");
+  if (n == 0) return HRT_OK;
+  std::lock_guard<std::mutex> lock(s->one_mu);
+  const size_t ray_bytes = 32 * n, res_bytes = out_bytes * n, st_bytes = stats ? 8 * n : 0;
+  const size_t need = ((ray_bytes + 255) & ~size_t(255)) + ((res_bytes + 255) & ~size_t(255)) + st_bytes;
+  hipError_t e = hipSetDevice(s->ctx->device);
+  if (e == hipSuccess && need > s->host_dev_bytes) {
+    if (s->host_dev) (void)hipFree(s->host_dev);
+    s->host_dev = nullptr;
+    s->host_dev_bytes = 0;
+    e = hipMalloc(&s->host_dev, need);
+    if (e == hipSuccess) s->host_dev_bytes = need;
+  }
+  if (e != hipSuccess) return fail_hip(e, "host batch staging");
+  char* base = static_cast<char*>(s->host_dev);
+  float* d_rays = reinterpret_cast<float*>(base);
+  void* d_out = base + ((ray_bytes + 255) & ~size_t(255));
+  uint32_t* d_st = stats ? reinterpret_cast<uint32_t*>(static_cast<char*>(d_out) + ((res_bytes + 255) & ~size_t(255)))
+                         : nullptr;
+  hipStream_t hs = s->ctx->stream;
+  if ((e = hipMemcpyAsync(d_rays, rays, ray_bytes, hipMemcpyHostToDevice, hs)) != hipSuccess)
+    return fail_hip(e, "host batch upload");
+  hrt_status st = trace_common(s, d_rays, n, d_out, HRT_HIT_FULL, anyhit, d_st, hs);
+  if (st != HRT_OK) return st;
+  if ((e = hipMemcpyAsync(out, d_out, res_bytes, hipMemcpyDeviceToHost, hs)) != hipSuccess)
+    return fail_hip(e, "host batch download");
+  if (stats && (e = hipMemcpyAsync(stats, d_st, st_bytes, hipMemcpyDeviceToHost, hs)) != hipSuccess)
+    return fail_hip(e, "host batch download");
+  if ((e = hipStreamSynchronize(hs)) != hipSuccess) return fail_hip(e, "host batch sync");
+  return HRT_OK;
+}
+}  // namespace
+
+hrt_status hrt_trace_nearest_host(hrt_scene* s, const float* rays, size_t n, hrt_hit* hits, uint32_t* stats) {
+  return host_batch(s, rays, n, hits, sizeof(hrt_hit), 0, stats);
+}
+
+hrt_status hrt_trace_any_host(hrt_scene* s, const float* rays, size_t n, uint8_t* out) {
+  return host_batch(s, rays, n, out, 1, 1, nullptr);
+}
+
 hrt_status hrt_query_nearest_one(hrt_scene* s, const float posNear[4], const float dirFar[4], hrt_hit* out) {
   if (!s || !posNear || !dirFar || !out) return fail(HRT_ERR_INVALID_ARG, "null argument");
   std::lock_guard<std::mutex> lock(s->one_mu);
@@ -311,11 +362,6 @@ hrt_status hrt_query_nearest_one(hrt_scene* s, const float posNear[4], const flo
   if ((e = hipMemcpyAsync(out, s->one_hit, sizeof(hrt_hit), hipMemcpyDeviceToHost, hs)) != hipSuccess)
     return fail_hip(e, "query download");
   if ((e = hipStreamSynchronize(hs)) != hipSuccess) return fail_hip(e, "query sync");
-  const Instance& in = s->instances[0];
-  if (out->primId != 0xFFFFFFFFu) {
-    out->instId = 0;
-    out->geomId = in.geom;
-  }
   return HRT_OK;
 }
 

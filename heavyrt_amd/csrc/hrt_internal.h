@@ -67,6 +67,8 @@ struct TraceArgs {
   int format;     // HitFormat for nearest-hit
   int anyhit;     // 1: boolean occlusion query
   uint32_t* stats;  // optional per-ray [pairs visited, triangles tested]
+  uint32_t inst_id = 0;  // written into full hit records on a hit
+  uint32_t geom_id = 0;
 };
 hipError_t trace_blas(const Blas& b, const TraceArgs& a, hipStream_t stream);
 

@@ -54,7 +54,7 @@ template <bool kForceAny, bool kStats>
 __global__ __launch_bounds__(256) void k_trace_simple(const NodePair* __restrict__ pairs,
                                                       const TriPre* __restrict__ tris,
                                                       const Ray* __restrict__ rays, uint64_t n, void* out,
-                                                      int format, uint32_t* __restrict__ stats) {
+                                                      int format, uint2 ids, uint32_t* __restrict__ stats) {
   __shared__ uint32_t stack_mem[kStackDepth * 256];
   const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
   if (i >= n) return;
@@ -118,8 +118,8 @@ __global__ __launch_bounds__(256) void k_trace_simple(const NodePair* __restrict
   } else {
     const bool hit = h.prim != kInvalid;
     float4* o4 = reinterpret_cast<float4*>(static_cast<HitOutFull*>(out) + i);
-    o4[0] = make_float4(h.t, __uint_as_float(h.prim), __uint_as_float(hit ? 0u : kInvalid),
-                        __uint_as_float(hit ? 0u : kInvalid));
+    o4[0] = make_float4(h.t, __uint_as_float(h.prim), __uint_as_float(hit ? ids.x : kInvalid),
+                        __uint_as_float(hit ? ids.y : kInvalid));
     o4[1] = make_float4(h.u, h.v, 0.0f, 0.0f);
   }
 }
@@ -131,7 +131,7 @@ template <bool kForceAny, bool kSpec>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_persistent(const NodePair* __restrict__ pairs,
                                                                   const TriPre* __restrict__ tris,
                                                                   const Ray* __restrict__ rays, uint64_t n,
-                                                                  void* out, int format,
+                                                                  void* out, int format, uint2 ids,
                                                                   uint32_t* __restrict__ queue,
                                                                   uint32_t* __restrict__ spill) {
   __shared__ uint32_t stack_mem[kShortStack * kTraceBlock];
@@ -265,8 +265,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_persistent(const NodePair
         } else {
           const bool hit = h.prim != kInvalid;
           float4* o4 = reinterpret_cast<float4*>(static_cast<HitOutFull*>(out) + rid);
-          o4[0] = make_float4(h.t, __uint_as_float(h.prim), __uint_as_float(hit ? 0u : kInvalid),
-                              __uint_as_float(hit ? 0u : kInvalid));
+          o4[0] = make_float4(h.t, __uint_as_float(h.prim), __uint_as_float(hit ? ids.x : kInvalid),
+                              __uint_as_float(hit ? ids.y : kInvalid));
           o4[1] = make_float4(h.u, h.v, 0.0f, 0.0f);
         }
         active = false;
@@ -298,7 +298,7 @@ hipError_t launch_persistent(const Blas& b, const TraceArgs& a, hipStream_t stre
   hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&buf), sizeof(uint32_t) * (64 + spill_words), stream);
   if (e != hipSuccess) return e;
   k_reset_counter<<<1, 1, 0, stream>>>(buf);
-  k_trace_persistent<kAny, kSpec><<<grid, kTraceBlock, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format,
+  k_trace_persistent<kAny, kSpec><<<grid, kTraceBlock, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format, make_uint2(a.inst_id, a.geom_id),
                                                                    buf, buf + 64);
   e = hipGetLastError();
   hipError_t f = hipFreeAsync(buf, stream);
@@ -313,9 +313,9 @@ hipError_t trace_blas(const Blas& b, const TraceArgs& a, hipStream_t stream) {
     // counting variant: one ray per lane in the reference's order
     const uint32_t blocks = uint32_t((a.n + 255) / 256);
     if (a.anyhit)
-      k_trace_simple<true, true><<<blocks, 256, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format, a.stats);
+      k_trace_simple<true, true><<<blocks, 256, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format, make_uint2(a.inst_id, a.geom_id), a.stats);
     else
-      k_trace_simple<false, true><<<blocks, 256, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format,
+      k_trace_simple<false, true><<<blocks, 256, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format, make_uint2(a.inst_id, a.geom_id),
                                                                a.stats);
     return hipGetLastError();
   }
@@ -323,7 +323,7 @@ hipError_t trace_blas(const Blas& b, const TraceArgs& a, hipStream_t stream) {
   // Nearest hits: one ray per lane.  Rays that arrive in screen order stay
   // coherent within a wave, which the refilling kernel gives up.
   const uint32_t blocks = uint32_t((a.n + 255) / 256);
-  k_trace_simple<false, false><<<blocks, 256, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format, nullptr);
+  k_trace_simple<false, false><<<blocks, 256, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format, make_uint2(a.inst_id, a.geom_id), nullptr);
   return hipGetLastError();
 }
 

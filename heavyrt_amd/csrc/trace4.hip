@@ -92,7 +92,7 @@ __device__ inline void visit_quad(const void* __restrict__ quads, uint32_t q, fl
 }
 
 template <bool kForceAny>
-__device__ inline void write_hit(void* out, uint64_t i, int format, const HitRec& h) {
+__device__ inline void write_hit(void* out, uint64_t i, int format, uint2 ids, const HitRec& h) {
   if (kForceAny) {
     static_cast<uint8_t*>(out)[i] = (h.prim != kInvalid) ? 1 : 0;
   } else if (format == kHitCompact) {
@@ -100,8 +100,8 @@ __device__ inline void write_hit(void* out, uint64_t i, int format, const HitRec
   } else {
     const bool hit = h.prim != kInvalid;
     float4* o4 = reinterpret_cast<float4*>(static_cast<HitOutFull*>(out) + i);
-    o4[0] = make_float4(h.t, __uint_as_float(h.prim), __uint_as_float(hit ? 0u : kInvalid),
-                        __uint_as_float(hit ? 0u : kInvalid));
+    o4[0] = make_float4(h.t, __uint_as_float(h.prim), __uint_as_float(hit ? ids.x : kInvalid),
+                        __uint_as_float(hit ? ids.y : kInvalid));
     o4[1] = make_float4(h.u, h.v, 0.0f, 0.0f);
   }
 }
@@ -112,7 +112,7 @@ template <bool kForceAny, bool kFp16, bool kStats>
 __global__ __launch_bounds__(kBlock4) void k_trace4_simple(const void* __restrict__ quads,
                                                            const TriPre* __restrict__ tris,
                                                            const Ray* __restrict__ rays, uint64_t n, void* out,
-                                                           int format, uint32_t* __restrict__ stats) {
+                                                           int format, uint2 ids, uint32_t* __restrict__ stats) {
   __shared__ uint32_t stack_mem[kStackDepth4 * kBlock4];
   const uint64_t i = uint64_t(blockIdx.x) * kBlock4 + threadIdx.x;
   if (i >= n) return;
@@ -160,7 +160,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_simple(const void* __restric
     stats[2 * i + 0] = nvisits;
     stats[2 * i + 1] = ntests;
   }
-  write_hit<kForceAny>(out, i, format, h);
+  write_hit<kForceAny>(out, i, format, ids, h);
 }
 
 // Persistent speculative any-hit over BVH4 nodes with ballot-based refill
@@ -284,18 +284,18 @@ hipError_t launch_simple4(const Blas& b, const TraceArgs& a, hipStream_t stream)
   const void* q = kFp16 ? static_cast<const void*>(b.quads16) : static_cast<const void*>(b.quads32);
   if (a.anyhit) {
     if (a.stats)
-      k_trace4_simple<true, kFp16, true><<<blocks, kBlock4, 0, stream>>>(q, b.tris, a.rays, a.n, a.out, a.format,
+      k_trace4_simple<true, kFp16, true><<<blocks, kBlock4, 0, stream>>>(q, b.tris, a.rays, a.n, a.out, a.format, make_uint2(a.inst_id, a.geom_id),
                                                                         a.stats);
     else
-      k_trace4_simple<true, kFp16, false><<<blocks, kBlock4, 0, stream>>>(q, b.tris, a.rays, a.n, a.out, a.format,
+      k_trace4_simple<true, kFp16, false><<<blocks, kBlock4, 0, stream>>>(q, b.tris, a.rays, a.n, a.out, a.format, make_uint2(a.inst_id, a.geom_id),
                                                                          nullptr);
   } else {
     if (a.stats)
-      k_trace4_simple<false, kFp16, true><<<blocks, kBlock4, 0, stream>>>(q, b.tris, a.rays, a.n, a.out, a.format,
+      k_trace4_simple<false, kFp16, true><<<blocks, kBlock4, 0, stream>>>(q, b.tris, a.rays, a.n, a.out, a.format, make_uint2(a.inst_id, a.geom_id),
                                                                          a.stats);
     else
       k_trace4_simple<false, kFp16, false><<<blocks, kBlock4, 0, stream>>>(q, b.tris, a.rays, a.n, a.out,
-                                                                          a.format, nullptr);
+                                                                          a.format, make_uint2(a.inst_id, a.geom_id), nullptr);
   }
   return hipGetLastError();
 }

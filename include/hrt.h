@@ -143,6 +143,14 @@ hrt_status hrt_trace_nearest_stats(hrt_scene* scene, const float* rays, size_t n
 hrt_status hrt_trace_any_stats(hrt_scene* scene, const float* rays, size_t n, uint8_t* out, uint32_t* stats,
                                void* stream);
 
+/* Host-memory batches (synchronous): rays in host memory, results written to
+ * host memory; staged through device buffers owned by the scene.  stats
+ * (host, nullable) receives the counters of hrt_trace_nearest_stats.  For
+ * callers that hold rays on the host, e.g. a renderer loop of
+ * RayQuery_NearestHit / RayQuery_AnyHit calls turned into one batch. */
+hrt_status hrt_trace_nearest_host(hrt_scene* scene, const float* rays, size_t n, hrt_hit* hits, uint32_t* stats);
+hrt_status hrt_trace_any_host(hrt_scene* scene, const float* rays, size_t n, uint8_t* out);
+
 /* One synchronous host query: the per-ray path behind
  * ISceneObject::RayQuery_NearestHit for callers that cannot batch. */
 hrt_status hrt_query_nearest_one(hrt_scene* scene, const float posNear[4], const float dirFar[4], hrt_hit* out);
