@@ -55,7 +55,22 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true", help="skip timing the other any-hit layouts")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, os.cpu_count())")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="only run 3 device-to-device copies of this many MiB (scales rocprofv3 "
+                         "FETCH_SIZE/WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.copy_calibration_mb:
+        import torch
+
+        n = args.copy_calibration_mb << 20
+        src = torch.ones(n, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n, "copies": 3}), flush=True)
+        return 0
     if args.gpus != 1:
         print("bench.py measures one GPU: every GPU box in this setup has one GPU, and the traversal "
               "shards by rays with no exchange (see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
