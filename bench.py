@@ -39,6 +39,21 @@ HBM_PEAK_GBPS = 8000.0
 SPP = 16
 SEED = 0x2545F4914F6CDD1D
 NODE_BYTES = {"bvh2": 64, "bvh4": 128, "bvh4_fp16": 64}
+# HBM bytes per any-hit launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
+# this workload, scaled by the same counters on a device copy of known size.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "rtao_anyhit_traffic.json")
+
+
+def measured_traffic(layout: str):
+    """Profiled HBM bytes per launch for `layout`, or None if not profiled."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            rec = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    if rec.get("any_hit_layout") != layout:
+        return None
+    return float(rec["hbm_bytes_per_launch"])
 
 
 def log(msg: str) -> None:
@@ -248,7 +263,8 @@ def main() -> int:
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved_any / HBM_PEAK_GBPS, 4),
-            "traffic": None,
+            "traffic": measured_traffic(args.any_layout),
+            "traffic_source": "profiles/rtao_anyhit_traffic.json (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
             "bytes_per_launch": bytes_any,
             "mean_launch_ms": round(float(any_ms.mean()), 4),
         },
