@@ -40,6 +40,8 @@ DUMP_PERM = 7
 DUMP_SCENE_BOX = 8
 DUMP_QUADS32 = 9
 DUMP_QUADS16 = 10
+# hrt_debug_dump geometry id that selects the top-level tree of an instanced scene
+TLAS_ID = 0xFFFFFFFF
 
 STATUS_NAMES = {
     0: "HRT_OK",
@@ -153,7 +155,12 @@ class Context:
         return int(lib().hrt_ctx_stream(self._h) or 0)
 
     def close(self):
+        """Destroys the context, and first every scene still open on it (a scene
+        must not outlive the context it was created on)."""
         if self._h:
+            for s in list(_live_scenes):
+                if s.ctx is self:
+                    s.close()
             lib().hrt_shutdown(self._h)
             self._h = C.c_void_p()
 

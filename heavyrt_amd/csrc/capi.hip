@@ -3,11 +3,14 @@
 // Scene model (mirrors BVH2CommonRT, core/BVH2CommonRT.h:70-85): a list of
 // geometries, each with its own device BLAS, and a list of instances
 // (geometry id + column-major matrix).  CommitScene freezes the instance
-// list.  This build traces scenes with one instance whose matrix is the
-// identity; other instance sets are rejected with HRT_ERR_UNSUPPORTED.
+// list.  A scene whose only instance has the identity matrix is traced
+// directly on its geometry's tree (BVH2 or BVH4 by layout); every other
+// instance set gets a top-level LBVH over the instance boxes at commit and
+// is traced with the two-level BVH2 kernel.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -34,9 +37,67 @@ hrt_status fail_hip(hipError_t e, const char* what) {
 
 struct Instance {
   uint32_t geom;
-  float fwd[16];
+  float fwd[16];    // column-major
+  float inv[12];    // rows 0..2 of the inverse
   bool identity;
 };
+
+// 4x4 inverse by the Laplace expansion in 2x2 minors (D. Eberly, "The Laplace
+// Expansion Theorem: Computing the Determinants and Inverses of Matrices",
+// Geometric Tools), in double, each entry rounded to float once.  The
+// reference inverts in float with LiteMath's inverse4x4; evaluating in double
+// gives the correctly rounded inverse for the matrices that matter here
+// (rotations, scales, translations).  Writes rows 0..2; false when singular.
+bool inverse_rows(const float* m, float* out12) {
+  double a[4][4];
+  for (int r = 0; r < 4; r++)
+    for (int c = 0; c < 4; c++) a[r][c] = double(m[4 * c + r]);
+  const double s0 = a[0][0] * a[1][1] - a[1][0] * a[0][1];
+  const double s1 = a[0][0] * a[1][2] - a[1][0] * a[0][2];
+  const double s2 = a[0][0] * a[1][3] - a[1][0] * a[0][3];
+  const double s3 = a[0][1] * a[1][2] - a[1][1] * a[0][2];
+  const double s4 = a[0][1] * a[1][3] - a[1][1] * a[0][3];
+  const double s5 = a[0][2] * a[1][3] - a[1][2] * a[0][3];
+  const double c5 = a[2][2] * a[3][3] - a[3][2] * a[2][3];
+  const double c4 = a[2][1] * a[3][3] - a[3][1] * a[2][3];
+  const double c3 = a[2][1] * a[3][2] - a[3][1] * a[2][2];
+  const double c2 = a[2][0] * a[3][3] - a[3][0] * a[2][3];
+  const double c1 = a[2][0] * a[3][2] - a[3][0] * a[2][2];
+  const double c0 = a[2][0] * a[3][1] - a[3][0] * a[2][1];
+  const double det = s0 * c5 - s1 * c4 + s2 * c3 + s3 * c2 - s4 * c1 + s5 * c0;
+  if (det == 0.0 || !std::isfinite(det)) return false;
+  const double cof[12] = {
+      a[1][1] * c5 - a[1][2] * c4 + a[1][3] * c3,   -a[0][1] * c5 + a[0][2] * c4 - a[0][3] * c3,
+      a[3][1] * s5 - a[3][2] * s4 + a[3][3] * s3,   -a[2][1] * s5 + a[2][2] * s4 - a[2][3] * s3,
+      -a[1][0] * c5 + a[1][2] * c2 - a[1][3] * c1,  a[0][0] * c5 - a[0][2] * c2 + a[0][3] * c1,
+      -a[3][0] * s5 + a[3][2] * s2 - a[3][3] * s1,  a[2][0] * s5 - a[2][2] * s2 + a[2][3] * s1,
+      a[1][0] * c4 - a[1][1] * c2 + a[1][3] * c0,   -a[0][0] * c4 + a[0][1] * c2 - a[0][3] * c0,
+      a[3][0] * s4 - a[3][1] * s2 + a[3][3] * s0,   -a[2][0] * s4 + a[2][1] * s2 - a[2][3] * s0};
+  for (int k = 0; k < 12; k++) {
+    out12[k] = float(cof[k] / det);
+    if (!std::isfinite(out12[k])) return false;
+  }
+  return true;
+}
+
+// World box of an instance: the box of the eight corners of the geometry's
+// vertex box moved by the forward matrix (BVH2CommonRT_host.cpp:133-157),
+// each product summed left to right as LiteMath's float4x4 * float4.
+void instance_box(const float* gbox8, const float* m, float* out8) {
+  const float* a = gbox8;
+  const float* b = gbox8 + 4;
+  const float corners[8][3] = {{a[0], a[1], a[2]}, {b[0], a[1], a[2]}, {a[0], b[1], a[2]}, {a[0], a[1], b[2]},
+                               {b[0], b[1], a[2]}, {b[0], a[1], b[2]}, {a[0], b[1], b[2]}, {b[0], b[1], b[2]}};
+  for (int k = 0; k < 8; k++) {
+    const float* c = corners[k];
+    float p[4];
+    for (int r = 0; r < 4; r++) p[r] = ((m[r] * c[0] + m[4 + r] * c[1]) + m[8 + r] * c[2]) + m[12 + r] * 1.0f;
+    for (int r = 0; r < 4; r++) {
+      if (k == 0 || p[r] < out8[r]) out8[r] = p[r];
+      if (k == 0 || out8[4 + r] < p[r]) out8[4 + r] = p[r];
+    }
+  }
+}
 
 bool is_identity(const float* m) {
   for (int c = 0; c < 4; c++)
@@ -59,6 +120,11 @@ struct hrt_scene {
   bool committed = false;
   bool keep_debug = false;
   int layout = HRT_LAYOUT_BVH2;
+  // instanced scenes: top-level tree over the instance boxes and the
+  // instance records (by instance id, then in the tree's leaf order)
+  bool two_level = false;
+  hrt::Blas tlas;
+  hrt::InstDev* inst_dev = nullptr;
   // staging for hrt_query_nearest_one
   std::mutex one_mu;
   // staging for the host-memory batch calls (grown on demand, guarded by one_mu)
@@ -69,6 +135,53 @@ struct hrt_scene {
 };
 
 namespace {
+
+void free_tlas(hrt_scene* s) {
+  if (s->tlas.pairs || s->inst_dev) {
+    (void)hipSetDevice(s->ctx->device);
+    (void)hipStreamSynchronize(s->ctx->stream);
+  }
+  hrt::blas_free(s->tlas);
+  if (s->inst_dev) (void)hipFree(s->inst_dev);
+  s->inst_dev = nullptr;
+  s->two_level = false;
+}
+
+hrt_status build_tlas(hrt_scene* s) {
+  const size_t n = s->instances.size();
+  std::vector<float> boxes(8 * n);
+  std::vector<hrt::InstDev> recs(n);
+  for (size_t i = 0; i < n; i++) {
+    const Instance& in = s->instances[i];
+    const hrt::Blas& g = s->geoms[in.geom];
+    instance_box(g.box, in.fwd, &boxes[8 * i]);
+    hrt::InstDev& r = recs[i];
+    std::memset(&r, 0, sizeof(r));
+    for (int k = 0; k < 3; k++)
+      r.inv_row[k] = make_float4(in.inv[4 * k + 0], in.inv[4 * k + 1], in.inv[4 * k + 2], in.inv[4 * k + 3]);
+    r.pairs = g.pairs;
+    r.tris = g.tris;
+    r.geom = in.geom;
+    r.inst = uint32_t(i);
+  }
+  hipStream_t hs = s->ctx->stream;
+  hipError_t e = hrt::blas_alloc(s->tlas, uint32_t(2 * n), uint32_t(n), s->keep_debug);
+  if (e == hipSuccess) e = hipMalloc(&s->inst_dev, 2 * n * sizeof(hrt::InstDev));
+  if (e == hipSuccess) e = hipMemcpyAsync(s->tlas.verts, boxes.data(), 4 * boxes.size(), hipMemcpyHostToDevice, hs);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(s->inst_dev, recs.data(), n * sizeof(hrt::InstDev), hipMemcpyHostToDevice, hs);
+  if (e == hipSuccess) e = hrt::tlas_build(s->tlas, s->inst_dev, s->inst_dev + n, hs);
+  if (e != hipSuccess) {
+    free_tlas(s);
+    return fail_hip(e, "top-level build");
+  }
+  if (s->tlas.max_run > 127) {
+    free_tlas(s);
+    return fail(HRT_ERR_LEAF_OVERFLOW, "more than 127 instances share one Morton cell of the top-level tree");
+  }
+  s->two_level = true;
+  return HRT_OK;
+}
 
 hrt_status select_blas(hrt_scene* s, const hrt::Blas** out, uint32_t* inst) {
   if (!s) return fail(HRT_ERR_INVALID_ARG, "scene is null");
@@ -82,10 +195,14 @@ hrt_status select_blas(hrt_scene* s, const hrt::Blas** out, uint32_t* inst) {
 
 hrt_status trace_common(hrt_scene* s, const float* rays, size_t n, void* out, int format, int anyhit,
                         uint32_t* stats, void* stream) {
+  if (!s) return fail(HRT_ERR_INVALID_ARG, "scene is null");
+  if (!s->committed) return fail(HRT_ERR_NOT_COMMITTED, "CommitScene has not been called since the last change");
   const hrt::Blas* b = nullptr;
   uint32_t inst = 0;
-  hrt_status st = select_blas(s, &b, &inst);
-  if (st != HRT_OK) return st;
+  if (!s->two_level) {
+    hrt_status st = select_blas(s, &b, &inst);
+    if (st != HRT_OK) return st;
+  }
   if (n == 0) return HRT_OK;
   if (!rays || !out) return fail(HRT_ERR_INVALID_ARG, "rays/out is null");
   if (n > 0x7FFFFFFFull) return fail(HRT_ERR_INVALID_ARG, "at most 2^31-1 rays per call");
@@ -99,12 +216,14 @@ hrt_status trace_common(hrt_scene* s, const float* rays, size_t n, void* out, in
   a.anyhit = anyhit;
   a.stats = stats;
   a.inst_id = inst;
-  a.geom_id = s->instances[inst].geom;
+  a.geom_id = s->two_level ? 0u : s->instances[inst].geom;
   hipStream_t hs = stream ? static_cast<hipStream_t>(stream) : s->ctx->stream;
   hipError_t e = hipSetDevice(s->ctx->device);
   if (e == hipSuccess) {
     const int lay = s->layout;
-    if (lay == HRT_LAYOUT_BVH2 || (lay == HRT_LAYOUT_AUTO && !anyhit))
+    if (s->two_level)
+      e = hrt::trace_two_level(s->tlas, s->inst_dev ? s->inst_dev + s->instances.size() : nullptr, a, hs);
+    else if (lay == HRT_LAYOUT_BVH2 || (lay == HRT_LAYOUT_AUTO && !anyhit))
       e = hrt::trace_blas(*b, a, hs);
     else
       e = hrt::trace_blas4(*b, a, lay == HRT_LAYOUT_BVH4_FP16, hs);
@@ -169,6 +288,7 @@ hrt_status hrt_scene_clear_geom(hrt_scene* s) {
   if (!s) return fail(HRT_ERR_INVALID_ARG, "scene is null");
   (void)hipSetDevice(s->ctx->device);
   (void)hipStreamSynchronize(s->ctx->stream);
+  free_tlas(s);
   for (auto& g : s->geoms) hrt::blas_free(g);
   s->geoms.clear();
   s->instances.clear();  // ClearGeom also clears the scene (BVH2CommonRT_host.cpp:42)
@@ -187,6 +307,7 @@ void hrt_scene_destroy(hrt_scene* s) {
 
 hrt_status hrt_scene_clear_instances(hrt_scene* s) {
   if (!s) return fail(HRT_ERR_INVALID_ARG, "scene is null");
+  free_tlas(s);
   s->instances.clear();
   s->committed = false;
   return HRT_OK;
@@ -247,6 +368,7 @@ hrt_status hrt_add_instance(hrt_scene* s, uint32_t geomId, const float colMajor1
   in.geom = geomId;
   std::memcpy(in.fwd, colMajor16, sizeof(in.fwd));
   in.identity = is_identity(colMajor16);
+  if (!inverse_rows(colMajor16, in.inv)) return fail(HRT_ERR_INVALID_ARG, "instance matrix is singular or not finite");
   s->instances.push_back(in);
   s->committed = false;
   if (outInstId) *outInstId = uint32_t(s->instances.size() - 1);
@@ -258,9 +380,18 @@ hrt_status hrt_commit(hrt_scene* s, int quality, int layout) {
   if (!s) return fail(HRT_ERR_INVALID_ARG, "scene is null");
   if (layout < HRT_LAYOUT_BVH2 || layout > HRT_LAYOUT_AUTO) return fail(HRT_ERR_INVALID_ARG, "unknown layout");
   s->committed = false;
-  if (layout != HRT_LAYOUT_BVH2) {
-    hipError_t e = hipSetDevice(s->ctx->device);
-    if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
+  free_tlas(s);
+  const bool single = s->instances.size() == 1 && s->instances[0].identity;
+  hipError_t e = hipSetDevice(s->ctx->device);
+  if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
+  if (!single) {
+    // instanced scenes are traced on the BVH2 trees through the top level
+    if (!s->instances.empty()) {
+      hrt_status st = build_tlas(s);
+      if (st != HRT_OK) return st;
+    }
+    s->two_level = true;
+  } else if (layout != HRT_LAYOUT_BVH2) {
     for (const Instance& in : s->instances) {
       hrt::Blas& b = s->geoms[in.geom];
       if (layout == HRT_LAYOUT_BVH4_FP16) {
@@ -387,8 +518,12 @@ uint32_t hrt_instance_count(const hrt_scene* s) { return s ? uint32_t(s->instanc
 
 hrt_status hrt_debug_dump(hrt_scene* s, uint32_t geomId, int what, void* host_out, size_t bytes) {
   if (!s || !host_out) return fail(HRT_ERR_INVALID_ARG, "scene/out is null");
-  if (geomId >= s->geoms.size()) return fail(HRT_ERR_INVALID_ARG, "no such geometry");
-  const hrt::Blas& b = s->geoms[geomId];
+  if (geomId == HRT_TLAS_ID) {
+    if (!s->committed || !s->tlas.pairs) return fail(HRT_ERR_INVALID_ARG, "no top-level tree (commit an instanced scene)");
+  } else if (geomId >= s->geoms.size()) {
+    return fail(HRT_ERR_INVALID_ARG, "no such geometry");
+  }
+  const hrt::Blas& b = (geomId == HRT_TLAS_ID) ? s->tlas : s->geoms[geomId];
   const uint32_t N = b.ntris, L = b.leaves;
   const uint32_t npairs = std::max<uint32_t>(L > 1 ? L - 1 : 1, 1);
   auto need = [&](size_t n) -> hrt_status {

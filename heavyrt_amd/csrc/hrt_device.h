@@ -144,8 +144,10 @@ struct HitRec {
   float u, v;
 };
 
-// Moller-Trumbore, operation order of BVH2CommonRT.cpp:26-37.
-__device__ inline bool mt_test(float3 o, float3 d, float tnear, const TriPre& tri, HitRec& h) {
+// Moller-Trumbore arithmetic in the operation order of BVH2CommonRT.cpp:26-37:
+// leaves t, u, v and says whether the barycentric slack (1e-6) and the
+// t > tNear condition hold.  Callers add their own t-acceptance rule.
+__device__ inline bool mt_eval(float3 o, float3 d, float tnear, const TriPre& tri, float& t, float& u, float& v) {
   const float3 A = make_float3(tri.a.x, tri.a.y, tri.a.z);
   const float3 e1 = make_float3(tri.e1.x, tri.e1.y, tri.e1.z);
   const float3 e2 = make_float3(tri.e2.x, tri.e2.y, tri.e2.z);
@@ -162,10 +164,16 @@ __device__ inline bool mt_test(float3 o, float3 d, float tnear, const TriPre& tr
   const float qy = tz * e1.x - tx * e1.z;
   const float qz = tx * e1.y - ty * e1.x;
   const float invDet = 1.0f / (e1.x * px + e1.y * py + e1.z * pz);
-  const float v = (tx * px + ty * py + tz * pz) * invDet;
-  const float u = (qx * d.x + qy * d.y + qz * d.z) * invDet;
-  const float t = (e2.x * qx + e2.y * qy + e2.z * qz) * invDet;
-  if (v >= -1e-6f && u >= -1e-6f && (u + v <= 1.0f + 1e-6f) && t > tnear && t < h.t) {
+  v = (tx * px + ty * py + tz * pz) * invDet;
+  u = (qx * d.x + qy * d.y + qz * d.z) * invDet;
+  t = (e2.x * qx + e2.y * qy + e2.z * qz) * invDet;
+  return v >= -1e-6f && u >= -1e-6f && (u + v <= 1.0f + 1e-6f) && t > tnear;
+}
+
+// The reference's rule: the first strictly closer hit wins (BVH2CommonRT.cpp:37).
+__device__ inline bool mt_test(float3 o, float3 d, float tnear, const TriPre& tri, HitRec& h) {
+  float t, u, v;
+  if (mt_eval(o, d, tnear, tri, t, u, v) && t < h.t) {
     h.t = t;
     h.prim = __float_as_uint(tri.a.w);
     h.u = u;
@@ -180,24 +188,9 @@ __device__ inline bool mt_test(float3 o, float3 d, float tnear, const TriPre& tr
 // depend on the order nodes are visited in.  t == tFar with no hit yet stays
 // outside the open interval.
 __device__ inline bool mt_test_lex(float3 o, float3 d, float tnear, const TriPre& tri, HitRec& h) {
-  const float3 A = make_float3(tri.a.x, tri.a.y, tri.a.z);
-  const float3 e1 = make_float3(tri.e1.x, tri.e1.y, tri.e1.z);
-  const float3 e2 = make_float3(tri.e2.x, tri.e2.y, tri.e2.z);
-  const float px = d.y * e2.z - d.z * e2.y;
-  const float py = d.z * e2.x - d.x * e2.z;
-  const float pz = d.x * e2.y - d.y * e2.x;
-  const float tx = o.x - A.x;
-  const float ty = o.y - A.y;
-  const float tz = o.z - A.z;
-  const float qx = ty * e1.z - tz * e1.y;
-  const float qy = tz * e1.x - tx * e1.z;
-  const float qz = tx * e1.y - ty * e1.x;
-  const float invDet = 1.0f / (e1.x * px + e1.y * py + e1.z * pz);
-  const float v = (tx * px + ty * py + tz * pz) * invDet;
-  const float u = (qx * d.x + qy * d.y + qz * d.z) * invDet;
-  const float t = (e2.x * qx + e2.y * qy + e2.z * qz) * invDet;
+  float t, u, v;
   const uint32_t prim = __float_as_uint(tri.a.w);
-  if (v >= -1e-6f && u >= -1e-6f && (u + v <= 1.0f + 1e-6f) && t > tnear &&
+  if (mt_eval(o, d, tnear, tri, t, u, v) &&
       (t < h.t || (t == h.t && h.prim != kInvalid && prim < h.prim))) {
     h.t = t;
     h.prim = prim;
@@ -206,6 +199,52 @@ __device__ inline bool mt_test_lex(float3 o, float3 d, float tnear, const TriPre
     return true;
   }
   return false;
+}
+
+// Two-level rule: inside one instance the first strictly closer hit wins, as
+// in mt_test; a hit at the same distance in an instance with a smaller id
+// replaces one from a larger id, so across instances the answer does not
+// depend on the order the top-level tree is walked in.
+__device__ inline bool mt_test_inst(float3 o, float3 d, float tnear, const TriPre& tri, uint32_t inst, HitRec& h,
+                                    uint32_t& h_inst) {
+  float t, u, v;
+  if (mt_eval(o, d, tnear, tri, t, u, v) &&
+      (t < h.t || (t == h.t && h.prim != kInvalid && inst < h_inst))) {
+    h.t = t;
+    h.prim = __float_as_uint(tri.a.w);
+    h.u = u;
+    h.v = v;
+    h_inst = inst;
+    return true;
+  }
+  return false;
+}
+
+// One instance as the two-level traversal reads it, 80 bytes.  inv_row[r] is
+// row r of the inverse (world -> object) matrix; a point p maps to
+//   p'.r = ((inv_row[r].x * p.x + inv_row[r].y * p.y) + inv_row[r].z * p.z) + inv_row[r].w * w
+// with w = 1 for points and w = 0 for directions (matmul4x3 / matmul3x3,
+// raytrace_common.h:82-90; the direction is not renormalised, so t keeps its
+// world-space meaning).
+struct __attribute__((aligned(16))) InstDev {
+  float4 inv_row[3];
+  const NodePair* pairs;
+  const TriPre* tris;
+  uint32_t geom;
+  uint32_t inst;
+  uint32_t pad[2];
+};
+static_assert(sizeof(InstDev) == 80, "InstDev is 80 bytes");
+
+__device__ inline float3 xform_point(const float4 r0, const float4 r1, const float4 r2, float3 p) {
+  return make_float3(((r0.x * p.x + r0.y * p.y) + r0.z * p.z) + r0.w * 1.0f,
+                     ((r1.x * p.x + r1.y * p.y) + r1.z * p.z) + r1.w * 1.0f,
+                     ((r2.x * p.x + r2.y * p.y) + r2.z * p.z) + r2.w * 1.0f);
+}
+__device__ inline float3 xform_dir(const float4 r0, const float4 r1, const float4 r2, float3 d) {
+  return make_float3(((r0.x * d.x + r0.y * d.y) + r0.z * d.z) + r0.w * 0.0f,
+                     ((r1.x * d.x + r1.y * d.y) + r1.z * d.z) + r1.w * 0.0f,
+                     ((r2.x * d.x + r2.y * d.y) + r2.z * d.z) + r2.w * 0.0f);
 }
 
 // FP16 box coordinate -> FP32 (exact).

@@ -57,6 +57,13 @@ hipError_t blas_build(Blas& b, hipStream_t stream);
 // synchronises the stream.
 hipError_t blas_build_bvh4(Blas& b, hipStream_t stream);
 
+// Top-level LBVH over instance boxes.  `t` is allocated with blas_alloc(t,
+// 2 * ninst, ninst, ...) and t.verts holds the world boxes (min, max per
+// instance).  by_id[i] is instance i's record; the records are written to
+// leaf_order in the tree's leaf order (t.perm maps back to instance ids).
+// Synchronises the stream.
+hipError_t tlas_build(Blas& t, const InstDev* by_id, InstDev* leaf_order, hipStream_t stream);
+
 // Traversal ------------------------------------------------------------------
 enum HitFormat : int { kHitFull = 0, kHitCompact = 1 };
 
@@ -74,6 +81,11 @@ hipError_t trace_blas(const Blas& b, const TraceArgs& a, hipStream_t stream);
 
 // BVH4 traversal; fp16 selects the FP16 nodes.  Nearest hit by (t, primId).
 hipError_t trace_blas4(const Blas& b, const TraceArgs& a, bool fp16, hipStream_t stream);
+
+// Two-level traversal: the TLAS (`tlas`, built by tlas_build) with instance
+// records in leaf order, each instance's BVH2 in object space.  Full hit
+// records carry the instance and geometry ids of the hit.
+hipError_t trace_two_level(const Blas& tlas, const InstDev* inst_leaf_order, const TraceArgs& a, hipStream_t stream);
 
 // Ambient-occlusion rays from primary hits (compact records), spp per pixel.
 hipError_t rtao_make_rays(const Blas& b, const Ray* prim_rays, const void* prim_hits_compact, uint64_t npix,

@@ -43,18 +43,73 @@ constexpr int kScanTile = kBlock * kScanItems;
   } while (0)
 
 // ---------------------------------------------------------------------------
+// Primitive sources.  The same pipeline builds a BLAS over triangles
+// (BuildFromTriangles, lbvh.cpp:732-827) and a TLAS over instance boxes
+// (BuildFromBoxes, lbvh.cpp:829-917); a source says what the scene box is
+// reduced over, where a primitive's Morton point lies and what its box is.
+// ---------------------------------------------------------------------------
+struct TriSource {
+  const float4* v;
+  const uint32_t* idx;
+  uint32_t nverts;
+  // scene box over every vertex (lbvh.cpp:228-239)
+  __device__ uint32_t box_items() const { return nverts; }
+  __device__ void box_item(uint32_t i, float4& lo, float4& hi) const { lo = hi = v[i]; }
+  // centroid for small meshes, box centre otherwise (lbvh.cpp:266-283)
+  __device__ float3 centre(uint32_t i, uint32_t n) const {
+    const float4 v1 = v[idx[3 * i + 0]];
+    const float4 v2 = v[idx[3 * i + 1]];
+    const float4 v3 = v[idx[3 * i + 2]];
+    if (n < 256)
+      return make_float3(0.33333333f * ((v1.x + v2.x) + v3.x), 0.33333333f * ((v1.y + v2.y) + v3.y),
+                         0.33333333f * ((v1.z + v2.z) + v3.z));
+    return make_float3((fminf(v1.x, fminf(v2.x, v3.x)) + fmaxf(v1.x, fmaxf(v2.x, v3.x))) * 0.5f,
+                       (fminf(v1.y, fminf(v2.y, v3.y)) + fmaxf(v1.y, fmaxf(v2.y, v3.y))) * 0.5f,
+                       (fminf(v1.z, fminf(v2.z, v3.z)) + fmaxf(v1.z, fmaxf(v2.z, v3.z))) * 0.5f);
+  }
+  __device__ void prim_box(uint32_t t, float3& lo, float3& hi) const {
+    const float4 a = v[idx[3 * t + 0]], b = v[idx[3 * t + 1]], c = v[idx[3 * t + 2]];
+    lo = make_float3(fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)), fminf(a.z, fminf(b.z, c.z)));
+    hi = make_float3(fmaxf(a.x, fmaxf(b.x, c.x)), fmaxf(a.y, fmaxf(b.y, c.y)), fmaxf(a.z, fmaxf(b.z, c.z)));
+  }
+};
+
+struct BoxSource {
+  const float4* boxes;  // 2 per primitive: min, max
+  uint32_t n;
+  // scene box over the primitive boxes (lbvh.cpp:190-203)
+  __device__ uint32_t box_items() const { return n; }
+  __device__ void box_item(uint32_t i, float4& lo, float4& hi) const {
+    lo = boxes[2 * i];
+    hi = boxes[2 * i + 1];
+  }
+  // box centre (lbvh.cpp:251)
+  __device__ float3 centre(uint32_t i, uint32_t) const {
+    const float4 lo = boxes[2 * i], hi = boxes[2 * i + 1];
+    return make_float3((lo.x + hi.x) * 0.5f, (lo.y + hi.y) * 0.5f, (lo.z + hi.z) * 0.5f);
+  }
+  __device__ void prim_box(uint32_t t, float3& lo, float3& hi) const {
+    const float4 a = boxes[2 * t], b = boxes[2 * t + 1];
+    lo = make_float3(a.x, a.y, a.z);
+    hi = make_float3(b.x, b.y, b.z);
+  }
+};
+
+// ---------------------------------------------------------------------------
 // 1. scene box
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_box_partial(const float4* __restrict__ v, uint32_t n,
-                                                        float4* __restrict__ part) {
+template <class Src>
+__global__ __launch_bounds__(kBlock) void k_box_partial(Src src, float4* __restrict__ part) {
   __shared__ float4 smin[kBlock];
   __shared__ float4 smax[kBlock];
   float4 lo = make_float4(kLbvhInf, kLbvhInf, kLbvhInf, kLbvhInf);
   float4 hi = make_float4(-kLbvhInf, -kLbvhInf, -kLbvhInf, -kLbvhInf);
+  const uint32_t n = src.box_items();
   for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) {
-    const float4 p = v[i];
-    lo = make_float4(fminf(lo.x, p.x), fminf(lo.y, p.y), fminf(lo.z, p.z), fminf(lo.w, p.w));
-    hi = make_float4(fmaxf(hi.x, p.x), fmaxf(hi.y, p.y), fmaxf(hi.z, p.z), fmaxf(hi.w, p.w));
+    float4 a, b;
+    src.box_item(i, a, b);
+    lo = make_float4(fminf(lo.x, a.x), fminf(lo.y, a.y), fminf(lo.z, a.z), fminf(lo.w, a.w));
+    hi = make_float4(fmaxf(hi.x, b.x), fmaxf(hi.y, b.y), fmaxf(hi.z, b.z), fmaxf(hi.w, b.w));
   }
   smin[threadIdx.x] = lo;
   smax[threadIdx.x] = hi;
@@ -126,28 +181,16 @@ __device__ inline uint32_t spread_bits(int x, int offset) {  // lbvh.cpp:25-34
   return uint32_t(x) << offset;
 }
 
-__global__ __launch_bounds__(kBlock) void k_morton(const float4* __restrict__ v, const uint32_t* __restrict__ idx,
-                                                   uint32_t ntris, const float* __restrict__ box8,
+template <class Src>
+__global__ __launch_bounds__(kBlock) void k_morton(Src src, uint32_t ntris, const float* __restrict__ box8,
                                                    const float* __restrict__ zscale4, uint32_t* __restrict__ codes,
                                                    uint32_t* __restrict__ ids) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= ntris) return;
-  const float4 v1 = v[idx[3 * i + 0]];
-  const float4 v2 = v[idx[3 * i + 1]];
-  const float4 v3 = v[idx[3 * i + 2]];
-  float cx, cy, cz;
-  if (ntris < 256) {  // lbvh.cpp:279: centroid for small meshes, box centre otherwise
-    cx = 0.33333333f * ((v1.x + v2.x) + v3.x);
-    cy = 0.33333333f * ((v1.y + v2.y) + v3.y);
-    cz = 0.33333333f * ((v1.z + v2.z) + v3.z);
-  } else {
-    cx = (fminf(v1.x, fminf(v2.x, v3.x)) + fmaxf(v1.x, fmaxf(v2.x, v3.x))) * 0.5f;
-    cy = (fminf(v1.y, fminf(v2.y, v3.y)) + fmaxf(v1.y, fmaxf(v2.y, v3.y))) * 0.5f;
-    cz = (fminf(v1.z, fminf(v2.z, v3.z)) + fmaxf(v1.z, fmaxf(v2.z, v3.z))) * 0.5f;
-  }
-  const float sx = zscale4[0] * (cx - box8[0]);
-  const float sy = zscale4[1] * (cy - box8[1]);
-  const float sz = zscale4[2] * (cz - box8[2]);
+  const float3 c = src.centre(i, ntris);
+  const float sx = zscale4[0] * (c.x - box8[0]);
+  const float sy = zscale4[1] * (c.y - box8[1]);
+  const float sz = zscale4[2] * (c.z - box8[2]);
   // float -> int truncates toward zero (v_cvt_i32_f32 saturates, NaN -> 0)
   const int ix = int(sx);
   const int iy = int(sy);
@@ -425,11 +468,11 @@ __device__ inline void store_box(NodePair* pairs, uint32_t p, uint32_t side, flo
 constexpr int kRefitPerThread = 4;
 constexpr uint32_t kRefitChunk = kBlock * kRefitPerThread;
 
+template <class Src>
 __global__ __launch_bounds__(kBlock) void k_leaves_refit(
-    const float4* __restrict__ v, const uint32_t* __restrict__ idx, const uint32_t* __restrict__ ids_sorted,
-    const uint32_t* __restrict__ run_start, uint32_t* __restrict__ meta, uint32_t ntris, NodePair* pairs,
-    const uint32_t* __restrict__ parent_int, const uint32_t* __restrict__ parent_leaf,
-    const uint2* __restrict__ range, uint32_t* arrive) {
+    Src src, const uint32_t* __restrict__ ids_sorted, const uint32_t* __restrict__ run_start,
+    uint32_t* __restrict__ meta, uint32_t ntris, NodePair* pairs, const uint32_t* __restrict__ parent_int,
+    const uint32_t* __restrict__ parent_leaf, const uint2* __restrict__ range, uint32_t* arrive) {
   const uint32_t L = meta[0];
   const uint32_t chunk_lo = blockIdx.x * kRefitChunk;
   if (chunk_lo >= L) return;
@@ -440,12 +483,10 @@ __global__ __launch_bounds__(kBlock) void k_leaves_refit(
     float3 lo = make_float3(kLbvhInf, kLbvhInf, kLbvhInf);
     float3 hi = make_float3(-kLbvhInf, -kLbvhInf, -kLbvhInf);
     for (uint32_t j = s; j < e; j++) {
-      const uint32_t t = ids_sorted[j];
-      const float4 a = v[idx[3 * t + 0]], b = v[idx[3 * t + 1]], c = v[idx[3 * t + 2]];
-      lo = make_float3(fminf(lo.x, fminf(a.x, fminf(b.x, c.x))), fminf(lo.y, fminf(a.y, fminf(b.y, c.y))),
-                       fminf(lo.z, fminf(a.z, fminf(b.z, c.z))));
-      hi = make_float3(fmaxf(hi.x, fmaxf(a.x, fmaxf(b.x, c.x))), fmaxf(hi.y, fmaxf(a.y, fmaxf(b.y, c.y))),
-                       fmaxf(hi.z, fmaxf(a.z, fmaxf(b.z, c.z))));
+      float3 a, b;
+      src.prim_box(ids_sorted[j], a, b);
+      lo = make_float3(fminf(lo.x, a.x), fminf(lo.y, a.y), fminf(lo.z, a.z));
+      hi = make_float3(fmaxf(hi.x, b.x), fmaxf(hi.y, b.y), fmaxf(hi.z, b.z));
     }
     atomicMax(&meta[1], e - s);
     if (L == 1) {
@@ -617,12 +658,26 @@ void blas_free(Blas& b) {
   b = Blas();
 }
 
-hipError_t blas_build(Blas& b, hipStream_t s) {
+namespace {
+
+// Fills n words with v (used instead of hipMemsetAsync so the build can be
+// recorded into a HIP graph).
+__global__ __launch_bounds__(kBlock) void k_fill(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+  for (uint64_t i = uint64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += uint64_t(gridDim.x) * kBlock) p[i] = v;
+}
+
+inline uint32_t fill_blocks(uint64_t n) { return uint32_t(std::min<uint64_t>(1024, std::max<uint64_t>(1, (n + kBlock - 1) / kBlock))); }
+
+// The LBVH pipeline over any primitive source.  `finish(ids_sorted)` enqueues
+// the leaf-order pass; the stream is synchronised before returning and the
+// leaf count, longest run and scene box are host values afterwards.
+template <class Src, class Finish>
+hipError_t lbvh_build(Blas& b, const Src& src, uint32_t box_items, hipStream_t s, Finish&& finish) {
   const uint64_t N = b.ntris;
   if (N == 0) return hipErrorInvalidValue;
   const uint32_t ntiles = uint32_t((N + kSortTile - 1) / kSortTile);
   const uint64_t hist_words = uint64_t(kDigits) * ntiles;
-  const int box_parts = int(std::min<uint64_t>(1024, std::max<uint64_t>(1, (b.nverts + kBlock - 1) / kBlock)));
+  const int box_parts = int(std::min<uint64_t>(1024, std::max<uint64_t>(1, (box_items + kBlock - 1) / kBlock)));
   // scratch layout (u32 words)
   const size_t w_codes = N, w_hist = hist_words, w_scan = scan_temp_words(std::max<uint64_t>(N, hist_words));
   size_t words = 0;
@@ -644,9 +699,9 @@ hipError_t blas_build(Blas& b, hipStream_t s) {
 
   hipError_t err = hipSuccess;
   do {
-    k_box_partial<<<box_parts, kBlock, 0, s>>>(b.verts, b.nverts, part);
+    k_box_partial<Src><<<box_parts, kBlock, 0, s>>>(src, part);
     k_box_final<<<1, kBlock, 0, s>>>(part, box_parts, box, box + 8);
-    k_morton<<<blocks_for(N), kBlock, 0, s>>>(b.verts, b.idx, uint32_t(N), box, box + 8, ka, va);
+    k_morton<Src><<<blocks_for(N), kBlock, 0, s>>>(src, uint32_t(N), box, box + 8, ka, va);
     if ((err = hipGetLastError()) != hipSuccess) break;
     if (b.keep_debug)
       if ((err = hipMemcpyAsync(b.codes_in, ka, 4 * N, hipMemcpyDeviceToDevice, s)) != hipSuccess) break;
@@ -662,14 +717,14 @@ hipError_t blas_build(Blas& b, hipStream_t s) {
     // after four passes the sorted keys are back in (ka, va)
     k_run_flags<<<blocks_for(N), kBlock, 0, s>>>(kin, uint32_t(N), flags);
     if ((err = exclusive_scan(flags, pos, N, stmp, s)) != hipSuccess) break;
-    if ((err = hipMemsetAsync(meta, 0, 64 * 4, s)) != hipSuccess) break;
+    k_fill<<<1, kBlock, 0, s>>>(meta, 64, 0u);
     k_run_compact<<<blocks_for(N), kBlock, 0, s>>>(kin, flags, pos, uint32_t(N), rs, uc, meta);
-    if ((err = hipMemsetAsync(arrive, 0, 4 * N, s)) != hipSuccess) break;
+    k_fill<<<fill_blocks(N), kBlock, 0, s>>>(arrive, N, 0u);
     k_karras<<<blocks_for(N), kBlock, 0, s>>>(uc, rs, meta, uint32_t(N), b.pairs, b.parent, pleaf, range,
                                               b.karras_lr);
-    k_leaves_refit<<<uint32_t((N + kRefitChunk - 1) / kRefitChunk), kBlock, 0, s>>>(
-        b.verts, b.idx, vin, rs, meta, uint32_t(N), b.pairs, b.parent, pleaf, range, arrive);
-    k_tris<<<blocks_for(N), kBlock, 0, s>>>(b.verts, b.idx, vin, uint32_t(N), b.tris, b.perm);
+    k_leaves_refit<Src><<<uint32_t((N + kRefitChunk - 1) / kRefitChunk), kBlock, 0, s>>>(
+        src, vin, rs, meta, uint32_t(N), b.pairs, b.parent, pleaf, range, arrive);
+    finish(vin);
     if ((err = hipGetLastError()) != hipSuccess) break;
     if (b.keep_debug) {
       if ((err = hipMemcpyAsync(b.codes_sorted, kin, 4 * N, hipMemcpyDeviceToDevice, s)) != hipSuccess) break;
@@ -691,6 +746,35 @@ hipError_t blas_build(Blas& b, hipStream_t s) {
   if (err == hipSuccess) err = ferr;
   if (err == hipSuccess) err = hipStreamSynchronize(s);
   return err;
+}
+
+// Instance records in leaf order.
+__global__ __launch_bounds__(kBlock) void k_inst_order(const InstDev* __restrict__ by_id,
+                                                       const uint32_t* __restrict__ ids_sorted, uint32_t n,
+                                                       InstDev* __restrict__ leaf_order, uint32_t* __restrict__ perm) {
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t t = ids_sorted[j];
+  leaf_order[j] = by_id[t];
+  perm[j] = t;
+}
+
+}  // namespace
+
+hipError_t blas_build(Blas& b, hipStream_t s) {
+  const TriSource src{b.verts, b.idx, b.nverts};
+  const uint32_t N = b.ntris;
+  return lbvh_build(b, src, b.nverts, s, [&](const uint32_t* ids_sorted) {
+    k_tris<<<blocks_for(N), kBlock, 0, s>>>(b.verts, b.idx, ids_sorted, N, b.tris, b.perm);
+  });
+}
+
+hipError_t tlas_build(Blas& t, const InstDev* by_id, InstDev* leaf_order, hipStream_t s) {
+  const BoxSource src{t.verts, t.ntris};
+  const uint32_t N = t.ntris;
+  return lbvh_build(t, src, N, s, [&](const uint32_t* ids_sorted) {
+    k_inst_order<<<blocks_for(N), kBlock, 0, s>>>(by_id, ids_sorted, N, leaf_order, t.perm);
+  });
 }
 
 hipError_t blas_build_bvh4(Blas& b, hipStream_t s) {

@@ -167,7 +167,9 @@ _CUBE_F = np.array([[0, 2, 1], [0, 3, 2], [4, 5, 6], [4, 6, 7], [0, 1, 5], [0, 5
                     [0, 4, 7], [0, 7, 3], [1, 2, 6], [1, 6, 5]], dtype=np.int64)
 
 
-def boxes(count: int, hfn, size_x=2.0, size_z=1.0, stream: int = STREAM_BOXES):
+def _box_frames(count: int, hfn, size_x=2.0, size_z=1.0, stream: int = STREAM_BOXES):
+    """Edge length, rotation and centre of each box (same draws for the merged
+    mesh and for the instanced form)."""
     r = uniform01(pcg32(count, 6, stream))
     edge = 0.01 + 0.04 * r[:, 0]
     # uniform random rotation (Shoemake's method)
@@ -182,11 +184,42 @@ def boxes(count: int, hfn, size_x=2.0, size_z=1.0, stream: int = STREAM_BOXES):
     cz = size_z * r[:, 5]
     cy = hfn(cx, cz)
     c = np.stack([cx, cy, cz], axis=1)
+    return edge, R, c
+
+
+def boxes(count: int, hfn, size_x=2.0, size_z=1.0, stream: int = STREAM_BOXES):
+    edge, R, c = _box_frames(count, hfn, size_x, size_z, stream)
     local = _CUBE_V[None, :, :] * edge[:, None, None]
     world = np.einsum("bij,bkj->bki", R, local) + c[:, None, :]
     verts = world.reshape(-1, 3).astype(np.float32)
     faces = (_CUBE_F[None, :, :] + 8 * np.arange(count)[:, None, None]).reshape(-1, 3).astype(np.uint32)
     return verts, faces
+
+
+def unit_cube():
+    """The 12-triangle cube of edge 1 centred at the origin."""
+    return _CUBE_V.astype(np.float32), _CUBE_F.astype(np.uint32)
+
+
+def box_instance_matrices(count: int, hfn, size_x=2.0, size_z=1.0, stream: int = STREAM_BOXES) -> np.ndarray:
+    """The boxes of `boxes` as instances of `unit_cube`: (count, 16) float32
+    column-major matrices (rotation x edge, then the centre)."""
+    edge, R, c = _box_frames(count, hfn, size_x, size_z, stream)
+    M = np.zeros((count, 4, 4), dtype=np.float64)
+    M[:, :3, :3] = R * edge[:, None, None]
+    M[:, :3, 3] = c
+    M[:, 3, 3] = 1.0
+    return np.ascontiguousarray(M.transpose(0, 2, 1).reshape(count, 16)).astype(np.float32)
+
+
+def rtao_instanced_scene(nx: int = 1024, nz: int = 512, nboxes: int = 2000):
+    """The RTAO scene with the boxes as instances: geometries [heightfield,
+    unit cube], instances [(0, identity)] + [(1, box matrix)] * nboxes."""
+    hv, hf, hfn = heightfield(nx, nz)
+    cv, cf = unit_cube()
+    ident = np.eye(4, dtype=np.float32).reshape(16)
+    inst = [(0, ident)] + [(1, m) for m in box_instance_matrices(nboxes, hfn)]
+    return [(hv, hf), (cv, cf)], inst
 
 
 def rtao_scene(nx: int = 1024, nz: int = 512, nboxes: int = 2000):

@@ -98,6 +98,7 @@ enum {
 /* Context: one device, one HIP stream.  No counterpart in the reference
  * (its engine is CPU-only); it owns what every scene shares. */
 hrt_status hrt_init(int device, hrt_ctx** out);
+/* Destroy every scene created on ctx before shutting it down. */
 void hrt_shutdown(hrt_ctx* ctx);
 /* The context's stream as a hipStream_t. */
 void* hrt_ctx_stream(hrt_ctx* ctx);
@@ -120,10 +121,17 @@ hrt_status hrt_add_triangles(hrt_scene* scene, const float* vpos, size_t nVert, 
                              const uint32_t* idx, size_t nIdx, int quality, uint32_t* outGeomId);
 
 /* ISceneObject::AddInstance (CrossRT.h:123; BVH2CommonRT_host.cpp:131-164);
- * colMajor16 is a column-major float4x4. */
+ * colMajor16 is a column-major float4x4.  Fails with HRT_ERR_INVALID_ARG for
+ * a singular matrix (the reference would trace with a non-finite inverse). */
 hrt_status hrt_add_instance(hrt_scene* scene, uint32_t geomId, const float colMajor16[16], uint32_t* outInstId);
 
-/* ISceneObject::CommitScene (CrossRT.h:115; BVH2CommonRT_host.cpp:118-129). */
+/* ISceneObject::CommitScene (CrossRT.h:115; BVH2CommonRT_host.cpp:118-129).
+ * A scene with exactly one instance whose matrix is the identity is traced on
+ * its geometry's tree in `layout`.  Any other instance set gets a top-level
+ * LBVH over the instance boxes (lbvh.cpp:829-917, every instance kept in its
+ * leaf) and is traced two-level on the BVH2 trees for every layout
+ * (BVH2CommonRT.cpp:143-279): the ray is moved into each instance's object
+ * space with the inverse matrix, and on equal t the smaller instance id wins. */
 hrt_status hrt_commit(hrt_scene* scene, int quality, int layout);
 
 /* Batched ISceneObject::RayQuery_NearestHit (CrossRT.h:143; BVH2CommonRT.cpp:228-279):
@@ -171,7 +179,10 @@ uint32_t hrt_geom_count(const hrt_scene* scene);
 uint32_t hrt_instance_count(const hrt_scene* scene);
 
 /* Parity hooks: copies one of the HRT_DUMP_* arrays of geometry geomId to
- * host memory; `bytes` is the capacity of host_out. */
+ * host memory; `bytes` is the capacity of host_out.  geomId == HRT_TLAS_ID
+ * selects the top-level tree of a committed instanced scene (its "triangles"
+ * are the instances). */
+#define HRT_TLAS_ID 0xFFFFFFFFu
 hrt_status hrt_debug_dump(hrt_scene* scene, uint32_t geomId, int what, void* host_out, size_t bytes);
 
 /* Thread-local description of the last failure in this thread. */

@@ -111,6 +111,9 @@ struct Build {
   std::vector<f4> verts;
   std::vector<uint32_t> idx;
   uint32_t ntris = 0;
+  // box mode (a top-level tree over instance boxes, BuildFromBoxes,
+  // lbvh.cpp:829-917): 2 entries per primitive, min and max; verts/idx unused
+  std::vector<f4> boxes;
 
   // builder state, named after lbvh.h:120-145
   f4 bbmin, bbmax, zscale;
@@ -381,10 +384,19 @@ static void build_peach(Build& b) {
 static void build_all(Build& b) {
   const uint32_t N = b.ntris;
   // scene box over all vertices, lbvh.cpp:228-239
+  const bool box_mode = !b.boxes.empty();
   f4 lo = {LBVH_INF, LBVH_INF, LBVH_INF, LBVH_INF}, hi = {-LBVH_INF, -LBVH_INF, -LBVH_INF, -LBVH_INF};
-  for (const f4& v : b.verts) {
-    lo = {mn(lo.x, v.x), mn(lo.y, v.y), mn(lo.z, v.z), mn(lo.w, v.w)};
-    hi = {mx(hi.x, v.x), mx(hi.y, v.y), mx(hi.z, v.z), mx(hi.w, v.w)};
+  if (box_mode) {  // over the primitive boxes, lbvh.cpp:190-203
+    for (uint32_t i = 0; i < N; i++) {
+      const f4 a = b.boxes[2 * i], c = b.boxes[2 * i + 1];
+      lo = {mn(lo.x, a.x), mn(lo.y, a.y), mn(lo.z, a.z), mn(lo.w, a.w)};
+      hi = {mx(hi.x, c.x), mx(hi.y, c.y), mx(hi.z, c.z), mx(hi.w, c.w)};
+    }
+  } else {
+    for (const f4& v : b.verts) {
+      lo = {mn(lo.x, v.x), mn(lo.y, v.y), mn(lo.z, v.z), mn(lo.w, v.w)};
+      hi = {mx(hi.x, v.x), mx(hi.y, v.y), mx(hi.z, v.z), mx(hi.w, v.w)};
+    }
   }
   b.bbmin = lo;
   b.bbmax = hi;
@@ -401,8 +413,19 @@ static void build_all(Build& b) {
   // Morton codes, lbvh.cpp:259-287
   b.codes_unsorted.resize(N);
   for (uint32_t i = 0; i < N; i++) {
-    const f4 v1 = b.verts[b.idx[3 * i + 0]], v2 = b.verts[b.idx[3 * i + 1]], v3 = b.verts[b.idx[3 * i + 2]];
     float cx, cy, cz;
+    if (box_mode) {  // box centre, lbvh.cpp:251
+      const f4 a = b.boxes[2 * i], c = b.boxes[2 * i + 1];
+      cx = (a.x + c.x) * 0.5f;
+      cy = (a.y + c.y) * 0.5f;
+      cz = (a.z + c.z) * 0.5f;
+      const float sx = b.zscale.x * (cx - lo.x);
+      const float sy = b.zscale.y * (cy - lo.y);
+      const float sz = b.zscale.z * (cz - lo.z);
+      b.codes_unsorted[i] = zindex3d(f2i(sx), f2i(sy), f2i(sz));
+      continue;
+    }
+    const f4 v1 = b.verts[b.idx[3 * i + 0]], v2 = b.verts[b.idx[3 * i + 1]], v3 = b.verts[b.idx[3 * i + 2]];
     if (N < 256) {
       cx = 0.33333333f * ((v1.x + v2.x) + v3.x);
       cy = 0.33333333f * ((v1.y + v2.y) + v3.y);
@@ -452,6 +475,12 @@ static void build_all(Build& b) {
     for (uint32_t j = start; j < start + count; j++) {
       const uint32_t t = b.sorted_ids[j];
       b.perm[j] = t;
+      if (box_mode) {  // union of the primitive boxes (lbvh.cpp:321-348, every primitive of the run kept)
+        const f4 a = b.boxes[2 * t], c = b.boxes[2 * t + 1];
+        bl = {mn(bl.x, a.x), mn(bl.y, a.y), mn(bl.z, a.z), 0};
+        bh = {mx(bh.x, c.x), mx(bh.y, c.y), mx(bh.z, c.z), 0};
+        continue;
+      }
       const f4 v1 = b.verts[b.idx[3 * t + 0]], v2 = b.verts[b.idx[3 * t + 1]], v3 = b.verts[b.idx[3 * t + 2]];
       bl = {mn(bl.x, mn(v1.x, mn(v2.x, v3.x))), mn(bl.y, mn(v1.y, mn(v2.y, v3.y))), mn(bl.z, mn(v1.z, mn(v2.z, v3.z))), 0};
       bh = {mx(bh.x, mx(v1.x, mx(v2.x, v3.x))), mx(bh.y, mx(v1.y, mx(v2.y, v3.y))), mx(bh.z, mx(v1.z, mx(v2.z, v3.z))), 0};
@@ -464,6 +493,7 @@ static void build_all(Build& b) {
   }
   karras(b);
   refit(b);
+  if (box_mode) return;
   build_peach(b);
   // AppendTreeData: index buffer rewritten in leaf order (BVH2CommonRT_host.cpp:54-61)
   b.idx_leaf_order.resize(3 * size_t(N));
@@ -500,19 +530,26 @@ static inline void slab(f3 o, f3 inv, f3 bmin, f3 bmax, float& tmin_out, float& 
   tmax_out = mn(tmax, mx(lo2, hi2));
 }
 
-// Moller-Trumbore with the reference's 1e-6 slack, BVH2CommonRT.cpp:26-45.
-// Returns true and updates the hit when accepted.
-static inline bool mt_test(f3 o, f3 d, float tNear, f3 A, f3 B, f3 C, Hit& h, uint32_t prim) {
+// Moller-Trumbore with the reference's 1e-6 slack, BVH2CommonRT.cpp:26-45:
+// t, u, v and whether the barycentric and t > tNear conditions hold.
+static inline bool mt_eval(f3 o, f3 d, float tNear, f3 A, f3 B, f3 C, float& t, float& u, float& v) {
   const f3 e1 = sub3(B, A);
   const f3 e2 = sub3(C, A);
   const f3 pv = cross3(d, e2);
   const f3 tv = sub3(o, A);
   const f3 qv = cross3(tv, e1);
   const float invDet = 1.0f / dot3(e1, pv);
-  const float v = dot3(tv, pv) * invDet;
-  const float u = dot3(qv, d) * invDet;
-  const float t = dot3(e2, qv) * invDet;
-  if (v >= -1e-6f && u >= -1e-6f && (u + v <= 1.0f + 1e-6f) && t > tNear && t < h.t) {
+  v = dot3(tv, pv) * invDet;
+  u = dot3(qv, d) * invDet;
+  t = dot3(e2, qv) * invDet;
+  return v >= -1e-6f && u >= -1e-6f && (u + v <= 1.0f + 1e-6f) && t > tNear;
+}
+
+// The reference's acceptance: first strictly closer hit (BVH2CommonRT.cpp:37).
+// Returns true and updates the hit when accepted.
+static inline bool mt_test(f3 o, f3 d, float tNear, f3 A, f3 B, f3 C, Hit& h, uint32_t prim) {
+  float t, u, v;
+  if (mt_eval(o, d, tNear, A, B, C, t, u, v) && t < h.t) {
     h.t = t;
     h.primId = prim;
     h.instId = 0;
@@ -531,19 +568,13 @@ static inline f3 vpos(const Build& b, uint32_t vi) {
 
 struct Counters { uint64_t pairs = 0, tris = 0, leaves = 0, max_stack = 0; };
 
-// BVH2CommonRT::BVH2TraverseC32 over the peach layout (BVH2CommonRT.cpp:49-141).
-// primId is reported as the ORIGINAL triangle id (the REMAP_PRIM_ID rule of
-// BVH2CommonRT.cpp:270-276); the leaf-order position is perm^-1 of it.
-static Hit trace_bvh2common(const Build& b, f4 posNear, f4 dirFar, Counters* cnt) {
-  bool anyhit = (dirFar.w <= 0.0f);
-  float tFar = anyhit ? -dirFar.w : dirFar.w;
-  Hit h;
-  h.t = tFar;
-  h.primId = h.instId = h.geomId = 0xFFFFFFFFu;
-  h.u = h.v = 0.0f;
-  const f3 o = {posNear.x, posNear.y, posNear.z};
-  const f3 d = {dirFar.x, dirFar.y, dirFar.z};
-  const float tNear = posNear.w;
+// The traversal loop of BVH2CommonRT::BVH2TraverseC32 in object space, with
+// an incoming hit (culling uses its distance) and a triangle acceptance rule
+// accept(A, B, C, original id) that updates the hit.  Returns true when an
+// any-hit query has found its hit.
+template <class Accept>
+static bool visit_bvh2(const Build& b, f3 o, f3 d, float tNear, bool anyhit, Hit& h, Counters* cnt,
+                       Accept&& accept) {
   const f3 inv = safe_inverse(d);
   uint32_t stack[80];
   int top = 0;
@@ -573,16 +604,34 @@ static Hit trace_bvh2common(const Build& b, f4 posNear, f4 dirFar, Counters* cnt
       if (cnt) { cnt->leaves++; cnt->tris += count; }
       for (uint32_t t = start; t < start + count; t++) {
         const uint32_t* ix = &b.idx_leaf_order[3 * size_t(t)];
-        mt_test(o, d, tNear, vpos(b, ix[0]), vpos(b, ix[1]), vpos(b, ix[2]), h, b.perm[t]);
+        accept(vpos(b, ix[0]), vpos(b, ix[1]), vpos(b, ix[2]), b.perm[t]);
       }
       pop = true;
-      if (anyhit && h.primId != 0xFFFFFFFFu) break;
+      if (anyhit && h.primId != 0xFFFFFFFFu) return true;
     }
     if (pop) {
       top--;
       off = stack[std::max(top, 0)];
     }
   }
+  return false;
+}
+
+// BVH2CommonRT::BVH2TraverseC32 over the peach layout (BVH2CommonRT.cpp:49-141).
+// primId is reported as the ORIGINAL triangle id (the REMAP_PRIM_ID rule of
+// BVH2CommonRT.cpp:270-276); the leaf-order position is perm^-1 of it.
+static Hit trace_bvh2common(const Build& b, f4 posNear, f4 dirFar, Counters* cnt) {
+  bool anyhit = (dirFar.w <= 0.0f);
+  float tFar = anyhit ? -dirFar.w : dirFar.w;
+  Hit h;
+  h.t = tFar;
+  h.primId = h.instId = h.geomId = 0xFFFFFFFFu;
+  h.u = h.v = 0.0f;
+  const f3 o = {posNear.x, posNear.y, posNear.z};
+  const f3 d = {dirFar.x, dirFar.y, dirFar.z};
+  const float tNear = posNear.w;
+  visit_bvh2(b, o, d, tNear, anyhit, h, cnt,
+             [&](f3 A, f3 B, f3 C, uint32_t prim) { mt_test(o, d, tNear, A, B, C, h, prim); });
   return h;
 }
 
@@ -828,17 +877,9 @@ static Child4 fp16_child(const Child4& c) {
 // the answer does not depend on the visiting order (FP16 and FP32 nodes
 // visit in different orders).  Any-hit stops at the first accepted triangle.
 static inline bool mt_test_lex(f3 o, f3 d, float tNear, f3 A, f3 B, f3 C, Hit& h, uint32_t prim) {
-  const f3 e1 = sub3(B, A);
-  const f3 e2 = sub3(C, A);
-  const f3 pv = cross3(d, e2);
-  const f3 tv = sub3(o, A);
-  const f3 qv = cross3(tv, e1);
-  const float invDet = 1.0f / dot3(e1, pv);
-  const float v = dot3(tv, pv) * invDet;
-  const float u = dot3(qv, d) * invDet;
-  const float t = dot3(e2, qv) * invDet;
+  float t, u, v;
   // t == tFar with no hit yet is outside the open interval (tNear, tFar)
-  if (v >= -1e-6f && u >= -1e-6f && (u + v <= 1.0f + 1e-6f) && t > tNear &&
+  if (mt_eval(o, d, tNear, A, B, C, t, u, v) &&
       (t < h.t || (t == h.t && h.primId != 0xFFFFFFFFu && prim < h.primId))) {
     h.t = t;
     h.primId = prim;
@@ -931,6 +972,256 @@ static Hit trace_brute_lex(const Build& b, f4 posNear, f4 dirFar) {
   const f3 d = {dirFar.x, dirFar.y, dirFar.z};
   for (uint32_t t = 0; t < b.ntris; t++)
     mt_test_lex(o, d, posNear.w, vpos(b, b.idx[3 * t]), vpos(b, b.idx[3 * t + 1]), vpos(b, b.idx[3 * t + 2]), h, t);
+  return h;
+}
+
+// ---------------------------------------------------------------------------
+// Instances and the two-level traversal
+// (BVH2CommonRT::AddInstance / CommitScene / RayQuery_NearestHit,
+//  BVH2CommonRT_host.cpp:118-164, BVH2CommonRT.cpp:143-279)
+// ---------------------------------------------------------------------------
+struct Inst {
+  uint32_t geom;
+  float fwd[16];   // column-major, as AddInstance receives it
+  float inv[12];   // rows 0..2 of the inverse (world -> object)
+  f4 bmin, bmax;   // world box
+};
+
+struct Scene {
+  std::vector<Build*> geoms;
+  std::vector<Inst> inst;
+  Build tlas;
+  ~Scene() {
+    for (Build* g : geoms) delete g;
+  }
+};
+
+// LiteMath float4x4 * float4 for a column-major matrix, summed left to right:
+// r.k = ((m[0+k]*x + m[4+k]*y) + m[8+k]*z) + m[12+k]*w.
+static inline f4 mat_vec(const float* m, f4 v) {
+  return {((m[0] * v.x + m[4] * v.y) + m[8] * v.z) + m[12] * v.w,
+          ((m[1] * v.x + m[5] * v.y) + m[9] * v.z) + m[13] * v.w,
+          ((m[2] * v.x + m[6] * v.y) + m[10] * v.z) + m[14] * v.w,
+          ((m[3] * v.x + m[7] * v.y) + m[11] * v.z) + m[15] * v.w};
+}
+
+// 4x4 inverse by the Laplace expansion in 2x2 minors (D. Eberly, "The Laplace
+// Expansion Theorem: Computing the Determinants and Inverses of Matrices",
+// Geometric Tools), evaluated in double and rounded to float once per entry.
+// Writes rows 0..2; returns false for a singular or non-finite matrix.
+static bool inverse_rows(const float* m, float* out12) {
+  double a[4][4];
+  for (int r = 0; r < 4; r++)
+    for (int c = 0; c < 4; c++) a[r][c] = double(m[4 * c + r]);
+  const double s0 = a[0][0] * a[1][1] - a[1][0] * a[0][1];
+  const double s1 = a[0][0] * a[1][2] - a[1][0] * a[0][2];
+  const double s2 = a[0][0] * a[1][3] - a[1][0] * a[0][3];
+  const double s3 = a[0][1] * a[1][2] - a[1][1] * a[0][2];
+  const double s4 = a[0][1] * a[1][3] - a[1][1] * a[0][3];
+  const double s5 = a[0][2] * a[1][3] - a[1][2] * a[0][3];
+  const double c5 = a[2][2] * a[3][3] - a[3][2] * a[2][3];
+  const double c4 = a[2][1] * a[3][3] - a[3][1] * a[2][3];
+  const double c3 = a[2][1] * a[3][2] - a[3][1] * a[2][2];
+  const double c2 = a[2][0] * a[3][3] - a[3][0] * a[2][3];
+  const double c1 = a[2][0] * a[3][2] - a[3][0] * a[2][2];
+  const double c0 = a[2][0] * a[3][1] - a[3][0] * a[2][1];
+  const double det = s0 * c5 - s1 * c4 + s2 * c3 + s3 * c2 - s4 * c1 + s5 * c0;
+  if (det == 0.0 || !std::isfinite(det)) return false;
+  const double b[12] = {
+      a[1][1] * c5 - a[1][2] * c4 + a[1][3] * c3,   -a[0][1] * c5 + a[0][2] * c4 - a[0][3] * c3,
+      a[3][1] * s5 - a[3][2] * s4 + a[3][3] * s3,   -a[2][1] * s5 + a[2][2] * s4 - a[2][3] * s3,
+      -a[1][0] * c5 + a[1][2] * c2 - a[1][3] * c1,  a[0][0] * c5 - a[0][2] * c2 + a[0][3] * c1,
+      -a[3][0] * s5 + a[3][2] * s2 - a[3][3] * s1,  a[2][0] * s5 - a[2][2] * s2 + a[2][3] * s1,
+      a[1][0] * c4 - a[1][1] * c2 + a[1][3] * c0,   -a[0][0] * c4 + a[0][1] * c2 - a[0][3] * c0,
+      a[3][0] * s4 - a[3][1] * s2 + a[3][3] * s0,   -a[2][0] * s4 + a[2][1] * s2 - a[2][3] * s0};
+  for (int k = 0; k < 12; k++) {
+    out12[k] = float(b[k] / det);
+    if (!std::isfinite(out12[k])) return false;
+  }
+  return true;
+}
+
+// matmul4x3 / matmul3x3 (raytrace_common.h:82-90) with the inverse rows.
+static inline f3 xform_point(const float* r, f3 p) {
+  return {((r[0] * p.x + r[1] * p.y) + r[2] * p.z) + r[3] * 1.0f,
+          ((r[4] * p.x + r[5] * p.y) + r[6] * p.z) + r[7] * 1.0f,
+          ((r[8] * p.x + r[9] * p.y) + r[10] * p.z) + r[11] * 1.0f};
+}
+static inline f3 xform_dir(const float* r, f3 d) {
+  return {((r[0] * d.x + r[1] * d.y) + r[2] * d.z) + r[3] * 0.0f,
+          ((r[4] * d.x + r[5] * d.y) + r[6] * d.z) + r[7] * 0.0f,
+          ((r[8] * d.x + r[9] * d.y) + r[10] * d.z) + r[11] * 0.0f};
+}
+
+// Instance box: the box of the eight transformed corners of the geometry's
+// vertex box (BVH2CommonRT_host.cpp:133-157).
+static void instance_box(const Build& g, const float* fwd, f4& lo, f4& hi) {
+  const f4 a = g.bbmin, b = g.bbmax;
+  const f4 corners[8] = {{a.x, a.y, a.z, 1.0f}, {b.x, a.y, a.z, 1.0f}, {a.x, b.y, a.z, 1.0f},
+                         {a.x, a.y, b.z, 1.0f}, {b.x, b.y, a.z, 1.0f}, {b.x, a.y, b.z, 1.0f},
+                         {a.x, b.y, b.z, 1.0f}, {b.x, b.y, b.z, 1.0f}};
+  for (int k = 0; k < 8; k++) {
+    const f4 p = mat_vec(fwd, corners[k]);
+    if (k == 0) {
+      lo = p;
+      hi = p;
+    } else {
+      lo = {mn(lo.x, p.x), mn(lo.y, p.y), mn(lo.z, p.z), mn(lo.w, p.w)};
+      hi = {mx(hi.x, p.x), mx(hi.y, p.y), mx(hi.z, p.z), mx(hi.w, p.w)};
+    }
+  }
+}
+
+// Two-level acceptance: inside one instance the first strictly closer hit
+// wins; at equal distance a smaller instance id replaces a larger one.
+static inline bool mt_test_inst(f3 o, f3 d, float tNear, f3 A, f3 B, f3 C, Hit& h, uint32_t prim, uint32_t inst,
+                                uint32_t geom) {
+  float t, u, v;
+  if (mt_eval(o, d, tNear, A, B, C, t, u, v) &&
+      (t < h.t || (t == h.t && h.primId != 0xFFFFFFFFu && inst < h.instId))) {
+    h.t = t;
+    h.primId = prim;
+    h.instId = inst;
+    h.geomId = geom;
+    h.u = u;
+    h.v = v;
+    return true;
+  }
+  return false;
+}
+
+static inline uint32_t lr_ref(const Build& t, uint32_t node) {  // child reference as the GPU pairs store it
+  return (node >= t.L - 1) ? t.lr[node].left : node;
+}
+
+// Nearest / any hit over all instances.  The top-level tree is the LBVH over
+// instance boxes, walked with the same near-first rule as the BLAS (left child
+// first on equal entry distance, culling with the current hit distance); at a
+// top-level leaf every instance of the leaf is entered in leaf order with the
+// ray moved into object space, and the BLAS is walked with the running hit.
+static Hit trace_scene(const Scene& s, f4 posNear, f4 dirFar, Counters* cnt) {
+  const bool anyhit = (dirFar.w <= 0.0f);
+  Hit h;
+  h.t = anyhit ? -dirFar.w : dirFar.w;
+  h.primId = h.instId = h.geomId = 0xFFFFFFFFu;
+  h.u = h.v = 0.0f;
+  if (s.inst.empty()) return h;
+  const f3 wo = {posNear.x, posNear.y, posNear.z};
+  const f3 wd = {dirFar.x, dirFar.y, dirFar.z};
+  const float tNear = posNear.w;
+  const f3 winv = safe_inverse(wd);
+  const Build& T = s.tlas;
+  auto visit_leaf = [&](uint32_t ref) -> bool {
+    const uint32_t start = leaf_start(ref), count = leaf_count(ref);
+    for (uint32_t k = start; k < start + count; k++) {
+      const uint32_t id = T.perm[k];
+      const Inst& I = s.inst[id];
+      const f3 o = xform_point(I.inv, wo);
+      const f3 d = xform_dir(I.inv, wd);
+      const bool stop = visit_bvh2(*s.geoms[I.geom], o, d, tNear, anyhit, h, cnt,
+                                   [&](f3 A, f3 B, f3 C, uint32_t prim) {
+                                     mt_test_inst(o, d, tNear, A, B, C, h, prim, id, I.geom);
+                                   });
+      if (stop) return true;
+    }
+    return false;
+  };
+  auto box_hit = [&](const Node& n, float& tmin) {
+    float a, b;
+    slab(wo, winv, n.bmin, n.bmax, a, b);
+    tmin = a;
+    return (a <= b) && (b >= tNear) && (a <= h.t);
+  };
+  if (T.L == 1) {
+    // the root is a leaf: one pair of (leaf, empty box)
+    if (cnt) cnt->pairs++;
+    float a;
+    if (box_hit(T.lr[0], a)) visit_leaf(T.lr[0].left);
+    return h;
+  }
+  uint32_t stack[80];
+  int top = 0;
+  uint32_t cur = 0;
+  while (true) {
+    bool pop = false;
+    if ((cur & LEAF_BIT) == 0) {
+      if (cnt) cnt->pairs++;
+      const Node& n = T.lr[cur];
+      float a0, a1;
+      const bool h0 = box_hit(T.lr[n.left], a0);
+      const bool h1 = box_hit(T.lr[n.escape], a1);
+      const uint32_t c0 = lr_ref(T, n.left), c1 = lr_ref(T, n.escape);
+      if (h0 && h1) {
+        cur = (a0 <= a1) ? c0 : c1;
+        stack[top++] = (a0 <= a1) ? c1 : c0;
+      } else if (h0 || h1) {
+        cur = h0 ? c0 : c1;
+      } else {
+        pop = true;
+      }
+    } else {
+      if (visit_leaf(cur)) break;
+      pop = true;
+    }
+    if (pop) {
+      if (top == 0) break;
+      cur = stack[--top];
+    }
+  }
+  return h;
+}
+
+// The closed form the two-level walk must agree with: every instance traced on
+// its own (the single-instance traversal with the ray in object space), then
+// the smallest t, the smallest instance id on equal t.
+static Hit trace_scene_per_instance(const Scene& s, f4 posNear, f4 dirFar) {
+  const bool anyhit = (dirFar.w <= 0.0f);
+  Hit best;
+  best.t = anyhit ? -dirFar.w : dirFar.w;
+  best.primId = best.instId = best.geomId = 0xFFFFFFFFu;
+  best.u = best.v = 0.0f;
+  for (uint32_t id = 0; id < s.inst.size(); id++) {
+    const Inst& I = s.inst[id];
+    const f3 o = xform_point(I.inv, f3{posNear.x, posNear.y, posNear.z});
+    const f3 d = xform_dir(I.inv, f3{dirFar.x, dirFar.y, dirFar.z});
+    Hit r = trace_bvh2common(*s.geoms[I.geom], f4{o.x, o.y, o.z, posNear.w}, f4{d.x, d.y, d.z, dirFar.w}, nullptr);
+    if (r.primId != 0xFFFFFFFFu && r.t < best.t) {
+      best = r;
+      best.instId = id;
+      best.geomId = I.geom;
+    }
+  }
+  return best;
+}
+
+// Brute force over every triangle of every instance (ascending instance id,
+// then ascending triangle id, first strictly closer hit wins).
+static Hit trace_scene_brute(const Scene& s, f4 posNear, f4 dirFar) {
+  const bool anyhit = (dirFar.w <= 0.0f);
+  Hit h;
+  h.t = anyhit ? -dirFar.w : dirFar.w;
+  h.primId = h.instId = h.geomId = 0xFFFFFFFFu;
+  h.u = h.v = 0.0f;
+  const float tNear = posNear.w;
+  for (uint32_t id = 0; id < s.inst.size(); id++) {
+    const Inst& I = s.inst[id];
+    const Build& g = *s.geoms[I.geom];
+    const f3 o = xform_point(I.inv, f3{posNear.x, posNear.y, posNear.z});
+    const f3 d = xform_dir(I.inv, f3{dirFar.x, dirFar.y, dirFar.z});
+    for (uint32_t t = 0; t < g.ntris; t++) {
+      float tt, u, v;
+      if (mt_eval(o, d, tNear, vpos(g, g.idx[3 * t]), vpos(g, g.idx[3 * t + 1]), vpos(g, g.idx[3 * t + 2]), tt, u,
+                  v) &&
+          tt < h.t) {
+        h.t = tt;
+        h.primId = t;
+        h.instId = id;
+        h.geomId = I.geom;
+        h.u = u;
+        h.v = v;
+      }
+    }
+  }
   return h;
 }
 
@@ -1143,6 +1434,82 @@ void hrto_safe_inverse(const float* d3, float* out3) {
 void hrto_slab(const float* o3, const float* inv3, const float* bmin3, const float* bmax3, float* out2) {
   slab(f3{o3[0], o3[1], o3[2]}, f3{inv3[0], inv3[1], inv3[2]}, f3{bmin3[0], bmin3[1], bmin3[2]},
        f3{bmax3[0], bmax3[1], bmax3[2]}, out2[0], out2[1]);
+}
+
+// ---- scenes with instances --------------------------------------------------
+void* hrto_scene_new(void) { return new Scene(); }
+void hrto_scene_free(void* h) { delete static_cast<Scene*>(h); }
+
+// Returns the geometry id, or -1 when the index buffer holds no triangle.
+int hrto_scene_add_geom(void* h, const float* vert4, uint32_t nVert, const uint32_t* idx, uint32_t nIdx) {
+  Scene* s = static_cast<Scene*>(h);
+  Build* b = static_cast<Build*>(hrto_build(vert4, nVert, idx, nIdx));
+  if (!b) return -1;
+  s->geoms.push_back(b);
+  return int(s->geoms.size() - 1);
+}
+
+// Returns the instance id, or -1 for an unknown geometry or a singular matrix.
+int hrto_scene_add_instance(void* h, uint32_t geom, const float* m16) {
+  Scene* s = static_cast<Scene*>(h);
+  if (geom >= s->geoms.size()) return -1;
+  Inst in;
+  in.geom = geom;
+  std::memcpy(in.fwd, m16, sizeof(in.fwd));
+  if (!inverse_rows(m16, in.inv)) return -1;
+  instance_box(*s->geoms[geom], in.fwd, in.bmin, in.bmax);
+  s->inst.push_back(in);
+  return int(s->inst.size() - 1);
+}
+
+// Builds the top-level tree; returns its handle (valid for the hrto_* build
+// accessors until the scene is freed) or null when there are no instances.
+void* hrto_scene_commit(void* h) {
+  Scene* s = static_cast<Scene*>(h);
+  s->tlas = Build();
+  if (s->inst.empty()) return nullptr;
+  s->tlas.ntris = uint32_t(s->inst.size());
+  for (const Inst& in : s->inst) {
+    s->tlas.boxes.push_back(in.bmin);
+    s->tlas.boxes.push_back(in.bmax);
+  }
+  build_all(s->tlas);
+  return &s->tlas;
+}
+
+// Per instance: 12 floats of inverse rows and 8 floats of world box.
+void hrto_scene_inst_info(void* h, float* inv12, float* box8) {
+  const Scene* s = static_cast<Scene*>(h);
+  for (size_t i = 0; i < s->inst.size(); i++) {
+    std::memcpy(inv12 + 12 * i, s->inst[i].inv, 48);
+    std::memcpy(box8 + 8 * i, &s->inst[i].bmin, 16);
+    std::memcpy(box8 + 8 * i + 4, &s->inst[i].bmax, 16);
+  }
+}
+
+// mode 0: two-level traversal; 1: per-instance closed form; 2: brute force.
+// hits: 6 words per ray; counters (mode 0, may be null): [pairs, tris].
+void hrto_scene_trace(void* h, int mode, const float* rays, uint64_t n, uint32_t* hits, uint32_t* counters,
+                      int threads) {
+  const Scene* s = static_cast<Scene*>(h);
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 256) if (threads != 1)
+#endif
+  for (int64_t i = 0; i < int64_t(n); i++) {
+    f4 p, d;
+    std::memcpy(&p, rays + 8 * i, 16);
+    std::memcpy(&d, rays + 8 * i + 4, 16);
+    Counters c;
+    Hit r = (mode == 0) ? trace_scene(*s, p, d, counters ? &c : nullptr)
+          : (mode == 1) ? trace_scene_per_instance(*s, p, d)
+                        : trace_scene_brute(*s, p, d);
+    std::memcpy(hits + 6 * i, &r, sizeof(Hit));
+    if (counters && mode == 0) {
+      counters[2 * i + 0] = uint32_t(c.pairs);
+      counters[2 * i + 1] = uint32_t(c.tris);
+    }
+  }
 }
 
 }  // extern "C"

@@ -85,6 +85,16 @@ def lib():
             getattr(L, name).argtypes = [C.c_float]
         L.hrto_h2f.restype = C.c_float
         L.hrto_h2f.argtypes = [u32]
+        L.hrto_scene_new.restype = vp
+        L.hrto_scene_free.argtypes = [vp]
+        L.hrto_scene_add_geom.restype = C.c_int
+        L.hrto_scene_add_geom.argtypes = [vp, vp, u32, vp, u32]
+        L.hrto_scene_add_instance.restype = C.c_int
+        L.hrto_scene_add_instance.argtypes = [vp, u32, vp]
+        L.hrto_scene_commit.restype = vp
+        L.hrto_scene_commit.argtypes = [vp]
+        L.hrto_scene_inst_info.argtypes = [vp, vp, vp]
+        L.hrto_scene_trace.argtypes = [vp, C.c_int, vp, u64, vp, vp, C.c_int]
         _lib = L
     return _lib
 
@@ -108,15 +118,29 @@ class OracleBVH:
         self.v4 = as_vert4(verts)
         self.idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1)
         L = lib()
+        self._owned = True
         self._h = L.hrto_build(_ptr(self.v4), self.v4.shape[0], _ptr(self.idx), self.idx.size)
         if not self._h:
             raise ValueError("oracle build needs at least one triangle")
+        self._read_sizes()
+
+    @classmethod
+    def _view(cls, handle, owner) -> "OracleBVH":
+        """A build owned by another oracle object (the top-level tree of a scene)."""
+        self = cls.__new__(cls)
+        self._owned = False
+        self._owner = owner
+        self._h = handle
+        self._read_sizes()
+        return self
+
+    def _read_sizes(self):
         s = np.zeros(5, dtype=np.uint64)
-        L.hrto_sizes(self._h, _ptr(s))
+        lib().hrto_sizes(self._h, _ptr(s))
         self.ntris, self.leaves, self.n_lr, self.n_peach, self.max_run = (int(x) for x in s)
 
     def __del__(self):
-        if getattr(self, "_h", None) and _lib is not None:
+        if getattr(self, "_owned", False) and getattr(self, "_h", None) and _lib is not None:
             _lib.hrto_free(self._h)
             self._h = None
 
@@ -202,6 +226,68 @@ class OracleBVH:
         prim = np.ascontiguousarray(prim, dtype=np.uint32)
         out = np.zeros(rays.shape[0], dtype=np.float64)
         lib().hrto_t_fp64(self._h, _ptr(rays), _ptr(prim), rays.shape[0], _ptr(out))
+        return out
+
+
+SCENE_TWO_LEVEL = 0
+SCENE_PER_INSTANCE = 1
+SCENE_BRUTE = 2
+
+
+class OracleScene:
+    """Geometries, instances and the top-level tree over instance boxes, with
+    the two-level traversal (BVH2CommonRT.cpp:143-279), its per-instance closed
+    form and brute force."""
+
+    def __init__(self):
+        self._h = lib().hrto_scene_new()
+        self._keep = []
+        self.tlas = None
+        self.ninst = 0
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.hrto_scene_free(self._h)
+            self._h = None
+
+    def add_geom(self, verts: np.ndarray, idx: np.ndarray) -> int:
+        v4 = as_vert4(verts)
+        i = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1)
+        g = lib().hrto_scene_add_geom(self._h, _ptr(v4), v4.shape[0], _ptr(i), i.size)
+        if g < 0:
+            raise ValueError("geometry needs at least one triangle")
+        return g
+
+    def add_instance(self, geom: int, matrix_col_major: np.ndarray) -> int:
+        m = np.ascontiguousarray(np.asarray(matrix_col_major, dtype=np.float32).reshape(16))
+        r = lib().hrto_scene_add_instance(self._h, geom, _ptr(m))
+        if r < 0:
+            raise ValueError("unknown geometry or singular matrix")
+        self.ninst = r + 1
+        return r
+
+    def commit(self):
+        h = lib().hrto_scene_commit(self._h)
+        self.tlas = OracleBVH._view(h, self) if h else None
+
+    def inst_info(self):
+        """(inverse rows (n,3,4) float32, world boxes (n,2,4) float32)."""
+        inv = np.zeros((self.ninst, 3, 4), dtype=np.float32)
+        box = np.zeros((self.ninst, 2, 4), dtype=np.float32)
+        lib().hrto_scene_inst_info(self._h, _ptr(inv), _ptr(box))
+        return inv, box
+
+    def trace(self, rays: np.ndarray, mode: int = SCENE_TWO_LEVEL, counters: bool = False, threads: int = 0):
+        """Returns (t, primId, instId, geomId, u, v[, pairs, tris])."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        n = rays.shape[0]
+        hits = np.zeros((n, 6), dtype=np.uint32)
+        cnt = np.zeros((n, 2), dtype=np.uint32) if counters else None
+        lib().hrto_scene_trace(self._h, mode, _ptr(rays), n, _ptr(hits), _ptr(cnt) if counters else None, threads)
+        out = (hits[:, 0].view(np.float32).copy(), hits[:, 1].copy(), hits[:, 2].copy(), hits[:, 3].copy(),
+               hits[:, 4].view(np.float32).copy(), hits[:, 5].view(np.float32).copy())
+        if counters:
+            return out + (cnt[:, 0].copy(), cnt[:, 1].copy())
         return out
 
 
