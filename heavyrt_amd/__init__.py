@@ -40,6 +40,7 @@ DUMP_PERM = 7
 DUMP_SCENE_BOX = 8
 DUMP_QUADS32 = 9
 DUMP_QUADS16 = 10
+DUMP_PAIRS = 11
 # hrt_debug_dump geometry id that selects the top-level tree of an instanced scene
 TLAS_ID = 0xFFFFFFFF
 
@@ -61,6 +62,9 @@ EXPORTS = [
     "hrt_trace_nearest_stats", "hrt_trace_any_stats", "hrt_query_nearest_one", "hrt_geom_count",
     "hrt_instance_count", "hrt_debug_dump", "hrt_last_error", "hrt_rtao_make_rays",
     "hrt_trace_nearest_host", "hrt_trace_any_host",
+    "hrt_batch_create", "hrt_batch_destroy", "hrt_batch_buffers", "hrt_batch_run", "hrt_batch_build",
+    "hrt_batch_trace", "hrt_batch_candidate_count", "hrt_batch_candidate_info", "hrt_batch_set_candidate",
+    "hrt_batch_get_candidate", "hrt_batch_autotune", "hrt_batch_dump",
 ]
 
 
@@ -121,6 +125,19 @@ def lib():
         L.hrt_rtao_make_rays.argtypes = [vp, vp, vp, sz, u32, C.c_float, C.c_uint64, vp, vp]
         L.hrt_trace_nearest_host.argtypes = [vp, vp, sz, vp, vp]
         L.hrt_trace_any_host.argtypes = [vp, vp, sz, vp]
+        L.hrt_batch_create.argtypes = [vp, u32, vp, vp, vp, C.POINTER(vp)]
+        L.hrt_batch_destroy.argtypes = [vp]
+        L.hrt_batch_destroy.restype = None
+        L.hrt_batch_buffers.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp)]
+        L.hrt_batch_run.argtypes = [vp, vp]
+        L.hrt_batch_build.argtypes = [vp, vp]
+        L.hrt_batch_trace.argtypes = [vp, vp]
+        L.hrt_batch_candidate_count.argtypes = []
+        L.hrt_batch_candidate_info.argtypes = [i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]
+        L.hrt_batch_set_candidate.argtypes = [vp, i32]
+        L.hrt_batch_get_candidate.argtypes = [vp]
+        L.hrt_batch_autotune.argtypes = [vp, i32, C.POINTER(i32), vp]
+        L.hrt_batch_dump.argtypes = [vp, u32, i32, vp, sz]
         _lib = L
     return _lib
 
@@ -268,6 +285,8 @@ class Scene:
             out = np.zeros((nq, 16), dtype=np.uint32)
         elif what == DUMP_SCENE_BOX:
             out = np.zeros(12, dtype=np.float32)
+        elif what == DUMP_PAIRS:
+            out = np.zeros((int(sizes[3]), 16), dtype=np.uint32)
         elif what in (DUMP_RUN_START, DUMP_UNIQUE_CODES):
             out = np.zeros(L, dtype=np.uint32)
         elif what == DUMP_LR_NODES:
@@ -275,4 +294,134 @@ class Scene:
         else:
             out = np.zeros(n, dtype=np.uint32)
         _check(lib().hrt_debug_dump(self._h, geom_id, what, _ptr(out), out.nbytes))
+        return out
+
+
+def batch_candidates():
+    """[(waves per workgroup, LDS stack entries, refill threshold)] of the
+    batched trace kernel (LDS 0 = one ray per lane without refill)."""
+    L = lib()
+    out = []
+    for i in range(L.hrt_batch_candidate_count()):
+        w, s, r = C.c_int(), C.c_int(), C.c_int()
+        _check(L.hrt_batch_candidate_info(i, C.byref(w), C.byref(s), C.byref(r)))
+        out.append((w.value, s.value, r.value))
+    return out
+
+
+_hip_rt = None
+
+
+def _hip():
+    """The HIP runtime (already loaded by libhrt.so) for plain host<->device copies."""
+    global _hip_rt
+    if _hip_rt is None:
+        lib()
+        h = C.CDLL("libamdhip64.so")
+        h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        h.hipDeviceSynchronize.argtypes = []
+        _hip_rt = h
+    return _hip_rt
+
+
+def _copy(dst, src, nbytes: int, kind: int) -> None:
+    if nbytes:
+        rc = _hip().hipMemcpy(dst, src, nbytes, kind)
+        if rc != 0:
+            raise HrtError(2, f"hipMemcpy failed ({rc})")
+
+
+class Batch:
+    """Many one-geometry scenes built and traced together (hrt_batch_*)."""
+
+    def __init__(self, ctx: Context, n_verts, n_tris, n_rays):
+        nv = np.ascontiguousarray(n_verts, dtype=np.uint32)
+        nt = np.ascontiguousarray(n_tris, dtype=np.uint32)
+        nr = np.ascontiguousarray(n_rays, dtype=np.uint32)
+        assert nv.size == nt.size == nr.size
+        self.ctx = ctx
+        self.n_scenes = int(nv.size)
+        self.n_verts, self.n_tris, self.n_rays = nv, nt, nr
+        self.vert_off = np.concatenate([[0], np.cumsum(nv, dtype=np.int64)])
+        self.tri_off = np.concatenate([[0], np.cumsum(nt, dtype=np.int64)])
+        self.ray_off = np.concatenate([[0], np.cumsum(nr, dtype=np.int64)])
+        self._h = C.c_void_p()
+        _check(lib().hrt_batch_create(ctx._h, self.n_scenes, _ptr(nv), _ptr(nt), _ptr(nr), C.byref(self._h)))
+        _live_scenes.add(self)
+        v, i, r, h = C.c_void_p(), C.c_void_p(), C.c_void_p(), C.c_void_p()
+        _check(lib().hrt_batch_buffers(self._h, C.byref(v), C.byref(i), C.byref(r), C.byref(h)))
+        self.d_verts, self.d_idx, self.d_rays, self.d_hits = v.value, i.value, r.value, h.value
+
+    def close(self):
+        if self._h:
+            lib().hrt_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, scenes) -> None:
+        """scenes: [(verts (n,3|4), faces (m,3), rays (k,8))] with the sizes given
+        at creation.  Waits for the device first (a running batch may still be
+        reading its inputs)."""
+        assert len(scenes) == self.n_scenes
+        v = np.concatenate([np.hstack([np.asarray(x[0], np.float32)[:, :3], np.ones((len(x[0]), 1), np.float32)])
+                            for x in scenes])
+        f = np.concatenate([np.asarray(x[1], np.uint32).reshape(-1) for x in scenes])
+        r = np.concatenate([np.asarray(x[2], np.float32).reshape(-1, 8) for x in scenes])
+        assert v.shape[0] == self.vert_off[-1] and f.size == 3 * self.tri_off[-1] and r.shape[0] == self.ray_off[-1]
+        for s, x in enumerate(scenes):
+            fi = np.asarray(x[1])
+            if fi.size and int(fi.max()) >= int(self.n_verts[s]):
+                raise ValueError(f"scene {s}: vertex index out of range")
+        _hip().hipDeviceSynchronize()
+        for host, dev in ((v, self.d_verts), (f, self.d_idx), (r, self.d_rays)):
+            host = np.ascontiguousarray(host)
+            _copy(dev, host.ctypes.data, host.nbytes, 1)
+
+    def hits(self) -> np.ndarray:
+        """(total rays, 2) uint32: t bits and scene-local primId per ray (waits for the device)."""
+        n = int(self.ray_off[-1])
+        out = np.zeros((n, 2), dtype=np.uint32)
+        _hip().hipDeviceSynchronize()
+        _copy(out.ctypes.data, self.d_hits, out.nbytes, 2)
+        return out
+
+    def run(self, stream: int = 0):
+        _check(lib().hrt_batch_run(self._h, stream or None))
+
+    def build(self, stream: int = 0):
+        _check(lib().hrt_batch_build(self._h, stream or None))
+
+    def trace(self, stream: int = 0):
+        _check(lib().hrt_batch_trace(self._h, stream or None))
+
+    def set_candidate(self, i: int):
+        _check(lib().hrt_batch_set_candidate(self._h, i))
+
+    @property
+    def candidate(self) -> int:
+        return int(lib().hrt_batch_get_candidate(self._h))
+
+    def autotune(self, reps: int = 3):
+        """Returns (chosen candidate, ms per candidate)."""
+        n = lib().hrt_batch_candidate_count()
+        ms = np.zeros(n, dtype=np.float32)
+        ch = C.c_int()
+        _check(lib().hrt_batch_autotune(self._h, reps, C.byref(ch), _ptr(ms)))
+        return ch.value, ms
+
+    def dump(self, scene: int, what: int) -> np.ndarray:
+        sizes = np.zeros(5, dtype=np.uint32)
+        _check(lib().hrt_batch_dump(self._h, scene, DUMP_SIZES, _ptr(sizes), sizes.nbytes))
+        if what == DUMP_SIZES:
+            return sizes
+        if what == DUMP_PAIRS:
+            out = np.zeros((int(sizes[3]), 16), dtype=np.uint32)
+        else:
+            out = np.zeros(int(sizes[0]), dtype=np.uint32)
+        _check(lib().hrt_batch_dump(self._h, scene, what, _ptr(out), out.nbytes))
         return out

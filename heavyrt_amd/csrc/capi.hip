@@ -21,7 +21,7 @@
 #include "../../include/hrt.h"
 #include "hrt_internal.h"
 
-namespace {
+namespace hrt {
 
 thread_local std::string g_last_error;
 
@@ -34,6 +34,13 @@ hrt_status fail_hip(hipError_t e, const char* what) {
   if (e == hipErrorOutOfMemory) return fail(HRT_ERR_OUT_OF_MEMORY, std::string(what) + ": out of device memory");
   return fail(HRT_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
+
+}  // namespace hrt
+
+namespace {
+
+using hrt::fail;
+using hrt::fail_hip;
 
 struct Instance {
   uint32_t geom;
@@ -107,11 +114,6 @@ bool is_identity(const float* m) {
 }
 
 }  // namespace
-
-struct hrt_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-};
 
 struct hrt_scene {
   hrt_ctx* ctx = nullptr;
@@ -236,7 +238,7 @@ hrt_status trace_common(hrt_scene* s, const float* rays, size_t n, void* out, in
 
 extern "C" {
 
-const char* hrt_last_error(void) { return g_last_error.c_str(); }
+const char* hrt_last_error(void) { return hrt::g_last_error.c_str(); }
 
 hrt_status hrt_init(int device, hrt_ctx** out) {
   if (!out) return fail(HRT_ERR_INVALID_ARG, "out is null");
@@ -550,6 +552,11 @@ hrt_status hrt_debug_dump(hrt_scene* s, uint32_t geomId, int what, void* host_ou
     if (need(n)) return HRT_ERR_INVALID_ARG;
     e = hipMemcpy(host_out, what == HRT_DUMP_QUADS32 ? static_cast<const void*>(b.quads32) : b.quads16, n,
                   hipMemcpyDeviceToHost);
+    return e == hipSuccess ? HRT_OK : fail_hip(e, "dump");
+  }
+  if (what == HRT_DUMP_PAIRS) {
+    if (need(64ull * npairs)) return HRT_ERR_INVALID_ARG;
+    e = hipMemcpy(host_out, b.pairs, 64ull * npairs, hipMemcpyDeviceToHost);
     return e == hipSuccess ? HRT_OK : fail_hip(e, "dump");
   }
   if (what == HRT_DUMP_PERM) {

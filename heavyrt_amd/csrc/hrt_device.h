@@ -271,4 +271,38 @@ __device__ inline uint32_t h_nextafter(uint32_t from, uint32_t to) {
 __device__ inline uint32_t round_to_neg(float v) { return h_nextafter(f2h_rne(v), 0xFBFFu); }
 __device__ inline uint32_t round_to_pos(float v) { return h_nextafter(f2h_rne(v), 0x7BFFu); }
 
+// ---------------------------------------------------------------------------
+// Builder helpers shared by the single-scene and batched LBVH builders
+// ---------------------------------------------------------------------------
+constexpr float kLbvhInf = 1e10f;      // lbvh.cpp:53
+constexpr float kLbvhEpsBias = 1e-4f;  // lbvh.cpp:54
+
+// 10-bit spread with the magic masks of lbvh.cpp:25-34.
+__device__ inline uint32_t spread_bits(int x, int offset) {
+  x = (x | (x << 10)) & 0x000F801F;
+  x = (x | (x << 4)) & 0x00E181C3;
+  x = (x | (x << 2)) & 0x03248649;
+  x = (x | (x << 2)) & 0x09249249;
+  return uint32_t(x) << offset;
+}
+
+// Karras delta: common-prefix length of codes i and j, -1 outside [0, L)
+// (lbvh.cpp:386-403).
+__device__ inline int kdelta(const uint32_t* __restrict__ c, int L, int i, int j) {
+  if (j < 0 || j >= L) return -1;
+  const uint32_t x = c[i] ^ c[j];
+  return x == 0 ? 32 : int(__builtin_clz(x));
+}
+
+// Writes one child box of node pair p (side 0 = left, 1 = right).
+__device__ inline void store_box(NodePair* pairs, uint32_t p, uint32_t side, float3 lo, float3 hi) {
+  uint32_t* pw = reinterpret_cast<uint32_t*>(pairs + p) + side * 8;
+  pw[0] = __float_as_uint(lo.x);
+  pw[1] = __float_as_uint(lo.y);
+  pw[2] = __float_as_uint(lo.z);
+  pw[4] = __float_as_uint(hi.x);
+  pw[5] = __float_as_uint(hi.y);
+  pw[6] = __float_as_uint(hi.z);
+}
+
 }  // namespace hrt

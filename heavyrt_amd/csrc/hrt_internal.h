@@ -8,9 +8,21 @@
 #include <string>
 #include <vector>
 
+#include "../../include/hrt.h"
 #include "hrt_device.h"
 
+// One device and one default-priority stream (hrt_init).
+struct hrt_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+};
+
 namespace hrt {
+
+// Error reporting shared by the C-ABI units: records the thread-local message
+// returned by hrt_last_error and passes the status through.
+hrt_status fail(hrt_status st, const std::string& msg);
+hrt_status fail_hip(hipError_t e, const char* what);
 
 // Device buffers of one bottom-level structure (one geometry).
 struct Blas {
@@ -63,6 +75,15 @@ hipError_t blas_build_bvh4(Blas& b, hipStream_t stream);
 // leaf_order in the tree's leaf order (t.perm maps back to instance ids).
 // Synchronises the stream.
 hipError_t tlas_build(Blas& t, const InstDev* by_id, InstDev* leaf_order, hipStream_t stream);
+
+// Building blocks (graph-capturable: no allocation, no memset, no sync) -----
+// One stable 8-bit LSD radix pass of (key, value) pairs on bits [shift, shift+8).
+size_t radix_pass_scratch_words(uint64_t n);
+hipError_t radix_pass(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout, uint64_t n, int shift,
+                      uint32_t* scratch, hipStream_t stream);
+size_t scan_scratch_words(uint64_t n);
+hipError_t scan_exclusive(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* temp, hipStream_t stream);
+hipError_t fill_words(uint32_t* p, uint64_t n, uint32_t v, hipStream_t stream);
 
 // Traversal ------------------------------------------------------------------
 enum HitFormat : int { kHitFull = 0, kHitCompact = 1 };

@@ -23,8 +23,6 @@ namespace hrt {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr float kLbvhInf = 1e10f;        // lbvh.cpp:53
-constexpr float kLbvhEpsBias = 1e-4f;    // lbvh.cpp:54
 
 // Radix sort tile: 4 waves x 16 rounds x 64 lanes.
 constexpr int kSortWaves = 4;
@@ -173,13 +171,6 @@ __global__ __launch_bounds__(kBlock) void k_box_final(const float4* __restrict__
 // ---------------------------------------------------------------------------
 // 2. Morton codes
 // ---------------------------------------------------------------------------
-__device__ inline uint32_t spread_bits(int x, int offset) {  // lbvh.cpp:25-34
-  x = (x | (x << 10)) & 0x000F801F;
-  x = (x | (x << 4)) & 0x00E181C3;
-  x = (x | (x << 2)) & 0x03248649;
-  x = (x | (x << 2)) & 0x09249249;
-  return uint32_t(x) << offset;
-}
 
 template <class Src>
 __global__ __launch_bounds__(kBlock) void k_morton(Src src, uint32_t ntris, const float* __restrict__ box8,
@@ -379,11 +370,6 @@ __global__ __launch_bounds__(kBlock) void k_run_compact(const uint32_t* __restri
 // ---------------------------------------------------------------------------
 // 5. Karras 2012
 // ---------------------------------------------------------------------------
-__device__ inline int kdelta(const uint32_t* __restrict__ c, int L, int i, int j) {  // lbvh.cpp:386-403
-  if (j < 0 || j >= L) return -1;
-  const uint32_t x = c[i] ^ c[j];
-  return x == 0 ? 32 : int(__builtin_clz(x));
-}
 
 __device__ inline uint32_t leaf_ref(const uint32_t* __restrict__ run_start, uint32_t L, uint32_t ntris, uint32_t m) {
   const uint32_t s = run_start[m];
@@ -446,15 +432,6 @@ __global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* __restrict__ 
 // ---------------------------------------------------------------------------
 // 6. leaves + bottom-up refit
 // ---------------------------------------------------------------------------
-__device__ inline void store_box(NodePair* pairs, uint32_t p, uint32_t side, float3 lo, float3 hi) {
-  uint32_t* pw = reinterpret_cast<uint32_t*>(pairs + p) + side * 8;
-  pw[0] = __float_as_uint(lo.x);
-  pw[1] = __float_as_uint(lo.y);
-  pw[2] = __float_as_uint(lo.z);
-  pw[4] = __float_as_uint(hi.x);
-  pw[5] = __float_as_uint(hi.y);
-  pw[6] = __float_as_uint(hi.z);
-}
 
 // Leaf boxes and the bottom-up refit.  Block b owns the leaves
 // [b * kRefitChunk, (b + 1) * kRefitChunk).  Every thread computes its leaf's
@@ -815,6 +792,46 @@ hipError_t blas_build_bvh4(Blas& b, hipStream_t s) {
     b.nquads = 0;
   }
   return err;
+}
+
+}  // namespace hrt
+
+// ---------------------------------------------------------------------------
+// Building blocks shared with the batched builder (all graph-capturable: no
+// allocation, no memset, no synchronisation).
+// ---------------------------------------------------------------------------
+namespace hrt {
+
+size_t radix_pass_scratch_words(uint64_t n) {
+  const uint64_t ntiles = (n + kSortTile - 1) / kSortTile;
+  const uint64_t hist_words = uint64_t(kDigits) * ntiles;
+  return size_t(2 * hist_words + scan_temp_words(hist_words) + 128);
+}
+
+hipError_t radix_pass(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout, uint64_t n, int shift,
+                      uint32_t* scratch, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const uint32_t ntiles = uint32_t((n + kSortTile - 1) / kSortTile);
+  const uint64_t hist_words = uint64_t(kDigits) * ntiles;
+  uint32_t* hist = scratch;
+  uint32_t* hscan = scratch + ((hist_words + 63) & ~uint64_t(63));
+  uint32_t* stmp = hscan + ((hist_words + 63) & ~uint64_t(63));
+  k_radix_hist<<<ntiles, kBlock, 0, s>>>(kin, uint32_t(n), shift, hist, ntiles);
+  HRT_CHECK(exclusive_scan(hist, hscan, hist_words, stmp, s));
+  k_radix_scatter<<<ntiles, kBlock, 0, s>>>(kin, vin, kout, vout, uint32_t(n), shift, hscan, ntiles);
+  return hipGetLastError();
+}
+
+size_t scan_scratch_words(uint64_t n) { return scan_temp_words(n); }
+
+hipError_t scan_exclusive(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* temp, hipStream_t s) {
+  return exclusive_scan(in, out, n, temp, s);
+}
+
+hipError_t fill_words(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_fill<<<fill_blocks(n), kBlock, 0, s>>>(p, n, v);
+  return hipGetLastError();
 }
 
 }  // namespace hrt

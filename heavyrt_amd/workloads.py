@@ -31,6 +31,7 @@ STREAM_BOXES = 3
 STREAM_SOUP = 4
 STREAM_RAYS = 5
 STREAM_TEST = 9
+STREAM_BATCH = 10        # batched build+trace: sizes, then soups and rays at 1000 + 2 * scene
 
 
 def pcg32(n: int, draws: int, stream: int, seed: int = PCG_SEED) -> np.ndarray:
@@ -317,3 +318,23 @@ def load_vsgf(path: str):
     off += 8 * nvert  # texcoords
     idx = np.frombuffer(data[off:off + 4 * nidx], dtype=np.uint32)
     return pos[:, :3].copy(), idx.reshape(-1, 3).copy()
+
+
+def log_uniform_int(n: int, lo: float, hi: float, stream: int) -> np.ndarray:
+    u = uniform01(pcg32(n, 1, stream))[:, 0]
+    return np.floor(np.exp(np.log(lo) + u * (np.log(hi) - np.log(lo)))).astype(np.int64)
+
+
+def batched_workload(n_scenes: int = 256, tris=(1e3, 2e5), rays=(4096, 262144), stream: int = STREAM_BATCH):
+    """Batched build+trace workload: n_scenes random soups with triangle counts
+    log-uniform in `tris` and ray counts log-uniform in `rays` (incoherent
+    rays: origins uniform in the unit cube, directions uniform on the sphere,
+    tFar = FLT_MAX).  Returns [(verts, faces, rays)] per scene."""
+    nt = log_uniform_int(n_scenes, tris[0], tris[1], stream)
+    nr = log_uniform_int(n_scenes, rays[0], rays[1], stream + 1)
+    out = []
+    for s in range(n_scenes):
+        v, f = soup(int(nt[s]), stream=1000 + 2 * s)
+        out.append((v, f, incoherent_rays(int(nr[s]), stream=1001 + 2 * s)))
+    return out
+

@@ -92,7 +92,8 @@ enum {
   HRT_DUMP_PERM = 7,         /* uint32[ntris]: leaf-order position -> triangle id */
   HRT_DUMP_SCENE_BOX = 8,    /* float[12]: box min xyzw, box max xyzw, Morton scale xyzw */
   HRT_DUMP_QUADS32 = 9,      /* uint32[nquads*32]: BVH4 nodes, FP32 boxes (after a BVH4 commit) */
-  HRT_DUMP_QUADS16 = 10      /* uint32[nquads*16]: BVH4 nodes, FP16 boxes */
+  HRT_DUMP_QUADS16 = 10,     /* uint32[nquads*16]: BVH4 nodes, FP16 boxes */
+  HRT_DUMP_PAIRS = 11        /* uint32[pairs*16]: the BVH2 node pairs as traversed */
 };
 
 /* Context: one device, one HIP stream.  No counterpart in the reference
@@ -184,6 +185,48 @@ uint32_t hrt_instance_count(const hrt_scene* scene);
  * are the instances). */
 #define HRT_TLAS_ID 0xFFFFFFFFu
 hrt_status hrt_debug_dump(hrt_scene* scene, uint32_t geomId, int what, void* host_out, size_t bytes);
+
+/* ---- Batches: many scenes built and traced in one HIP graph launch --------
+ * No single counterpart in the reference, whose scenes are built and queried
+ * one at a time (CrossRT.h:64-193).  A batch is S independent one-geometry
+ * scenes, each the equivalent of AddGeom_Triangles3f (BVH2CommonRT_host.cpp:
+ * 66-98) + AddInstance(identity) + CommitScene, each with its own rays traced
+ * as RayQuery_NearestHit (BVH2CommonRT.cpp:228-279).  Per scene the node
+ * pairs, the leaf permutation and the hits are bit-identical to a solo
+ * hrt_add_triangles + hrt_trace_nearest.  Sizes are fixed at creation; the
+ * caller fills the device input buffers (hrt_batch_buffers) and may refill
+ * them between runs. */
+typedef struct hrt_batch hrt_batch;
+/* nVerts/nTris/nRays: per-scene counts (every scene >= 1 triangle). */
+hrt_status hrt_batch_create(hrt_ctx* ctx, uint32_t nScenes, const uint32_t* nVerts, const uint32_t* nTris,
+                            const uint32_t* nRays, hrt_batch** out);
+void hrt_batch_destroy(hrt_batch* batch);
+/* Device buffers, scenes concatenated in order: float4 vertices (w ignored),
+ * scene-local triangle indices (3 per triangle), rays (8 floats each) and
+ * compact hits (t, primId with scene-local original triangle ids). */
+hrt_status hrt_batch_buffers(hrt_batch* batch, float** verts4, uint32_t** idx, float** rays, uint32_t** hits);
+/* Calls that launch work (run, build, trace) are asynchronous on `stream`;
+ * work on different streams must be ordered by the caller.  An index beyond
+ * its scene's vertex count is clamped to the last vertex. */
+/* The whole batch: every build, then every trace, as one graph launch
+ * (recorded on first use and after the trace candidate changes). */
+hrt_status hrt_batch_run(hrt_batch* batch, void* stream);
+/* The two halves on their own, as plain launches. */
+hrt_status hrt_batch_build(hrt_batch* batch, void* stream);
+hrt_status hrt_batch_trace(hrt_batch* batch, void* stream);
+/* Trace-kernel candidates: waves per workgroup, LDS stack entries per lane
+ * (0 = one ray per lane without refill) and the refill threshold. */
+int hrt_batch_candidate_count(void);
+hrt_status hrt_batch_candidate_info(int i, int* wavesPerGroup, int* ldsStackEntries, int* refillAt);
+hrt_status hrt_batch_set_candidate(hrt_batch* batch, int i);
+int hrt_batch_get_candidate(const hrt_batch* batch);
+/* Builds if needed, times every candidate `reps` times on the batch's own
+ * rays (best of reps), selects the fastest; msPerCandidate (nullable) gets
+ * hrt_batch_candidate_count() timings.  Synchronous. */
+hrt_status hrt_batch_autotune(hrt_batch* batch, int reps, int* chosen, float* msPerCandidate);
+/* Per-scene parity hooks after a build: HRT_DUMP_SIZES (5 words: triangles,
+ * leaves, longest run, pairs, first pair), HRT_DUMP_PAIRS, HRT_DUMP_PERM. */
+hrt_status hrt_batch_dump(hrt_batch* batch, uint32_t scene, int what, void* host_out, size_t bytes);
 
 /* Thread-local description of the last failure in this thread. */
 const char* hrt_last_error(void);
