@@ -5,8 +5,10 @@ Workload (one step): a 1024x512-quad heightfield (value noise, amp 0.2) plus
 primary rays traced nearest-hit (BVH2 node pairs, the reference's visiting
 order, bit-exact with BVH2CommonRT); 16 cosine-weighted AO rays per primary
 hit made on the device (tMax = 0.1 x scene diagonal); all AO rays traced
-any-hit on the BVH4 collapse of the same tree (any-hit answers do not depend
-on the node layout; the tests check them against the oracle).  Scene and rays
+any-hit on the BVH4 collapse of the same tree with FP16 boxes rounded outward
+(BVH4HalfRT's node format; any-hit answers do not depend on the node layout,
+the tests check them against the oracle and the run checks them against the
+FP32-node and BVH2 layouts).  Scene and rays
 are resident in HBM before the timed region starts.
 
 Prints ONE JSON line.  `value` is total rays (primary + AO) per second over
@@ -65,11 +67,14 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--any-layout", choices=["bvh2", "bvh4", "bvh4_fp16"], default="bvh4",
+    ap.add_argument("--any-layout", choices=["bvh2", "bvh4", "bvh4_fp16"], default="bvh4_fp16",
                     help="node layout for the AO any-hit rays")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true", help="skip timing the other any-hit layouts")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, os.cpu_count())")
+    ap.add_argument("--workload", choices=["rtao", "batched"], default="rtao",
+                    help="rtao (default): the headline RTAO step; batched: 256 scenes built and traced "
+                         "in one graph launch")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only run 3 device-to-device copies of this many MiB (scales rocprofv3 "
                          "FETCH_SIZE/WRITE_SIZE) and exit")
@@ -100,6 +105,8 @@ def main() -> int:
         log("no GPU visible")
         return 1
     torch.cuda.init()
+    if args.workload == "batched":
+        return bench_batched(args)
     dev = torch.device("cuda:0")
     layouts = {"bvh2": H.LAYOUT_BVH2, "bvh4": H.LAYOUT_BVH4, "bvh4_fp16": H.LAYOUT_BVH4_FP16}
 
@@ -133,9 +140,9 @@ def main() -> int:
     torch.cuda.synchronize()
 
     def set_any_layout(name: str) -> None:
-        # nearest hits always run on BVH2 in the reference's order; AUTO adds BVH4 any-hit
-        scene.commit(layout=H.LAYOUT_BVH2 if name == "bvh2" else
-                     (H.LAYOUT_AUTO if name == "bvh4" else layouts[name]))
+        # nearest hits always run on BVH2 in the reference's order; the AUTO
+        # layouts add BVH4 (FP32 or FP16 boxes) for any-hit queries
+        scene.commit(layout={"bvh2": H.LAYOUT_BVH2, "bvh4": H.LAYOUT_AUTO, "bvh4_fp16": H.LAYOUT_AUTO_FP16}[name])
 
     def nearest():
         scene.trace_nearest(d_prim, npix, d_hits, H.HIT_COMPACT, stream=sptr)
@@ -152,23 +159,6 @@ def main() -> int:
         scene.trace_any(d_ao, n_ao, d_vis, stream=sptr)
         if ev is not None:
             ev[3].record(stream)
-
-    # the FP16 layout routes nearest hits through BVH4 too; keep them on BVH2
-    if args.any_layout == "bvh4_fp16":
-        def step(ev=None):  # noqa: F811
-            if ev is not None:
-                ev[0].record(stream)
-            scene.commit(layout=H.LAYOUT_BVH2)
-            nearest()
-            if ev is not None:
-                ev[1].record(stream)
-            scene.rtao_make_rays(d_prim, d_hits, npix, SPP, tmax, SEED, d_ao, stream=sptr)
-            if ev is not None:
-                ev[2].record(stream)
-            scene.commit(layout=H.LAYOUT_BVH4_FP16)
-            scene.trace_any(d_ao, n_ao, d_vis, stream=sptr)
-            if ev is not None:
-                ev[3].record(stream)
 
     set_any_layout(args.any_layout)
     for _ in range(args.warmup):
@@ -311,6 +301,143 @@ def main() -> int:
         log(f"cpu {cpu_s:.2f} s, mismatches {mism}")
 
     scene.close()
+    ctx.close()
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def bench_batched(args) -> int:
+    """256 random-soup scenes (1k-200k triangles, log-uniform) with 4k-256k
+    incoherent rays each: every LBVH build and every nearest-hit trace in one
+    hipGraph launch per step, trace kernel picked by the autotuner."""
+    import torch
+
+    import heavyrt_amd as H
+    from heavyrt_amd import workloads as W
+
+    log("generating 256 scenes")
+    scenes = W.batched_workload()
+    nv = [x[0].shape[0] for x in scenes]
+    nt = [x[1].shape[0] for x in scenes]
+    nr = [x[2].shape[0] for x in scenes]
+    T, R = int(sum(nt)), int(sum(nr))
+    ctx = H.Context(0)
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    sptr = stream.cuda_stream
+    b = H.Batch(ctx, nv, nt, nr)
+    b.upload(scenes)
+    log(f"{len(scenes)} scenes, {T} triangles, {R} rays; autotuning the trace kernel")
+    chosen, cand_ms = b.autotune(reps=3)
+    cands = H.batch_candidates()
+    log(f"chosen candidate {chosen} {cands[chosen]}: {cand_ms[chosen]:.3f} ms")
+
+    def timed(fn, reps):
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+        torch.cuda.synchronize()
+        for k in range(reps):
+            ev[k].record(stream)
+            fn()
+        ev[reps].record(stream)
+        torch.cuda.synchronize()
+        return np.array([ev[k].elapsed_time(ev[k + 1]) for k in range(reps)])
+
+    for _ in range(args.warmup):
+        b.run(sptr)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    step_ms = timed(lambda: b.run(sptr), args.steps)
+    wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    build_ms = timed(lambda: b.build(sptr), 5)
+    trace_ms = timed(lambda: b.trace(sptr), 5)
+    hits = b.hits()
+    ms = float(np.median(step_ms))
+    log(f"graph step {ms:.3f} ms (build {build_ms.mean():.3f} + trace {trace_ms.mean():.3f} as plain launches)")
+
+    # the plain path: one scene at a time through the single-scene API, rays
+    # already on the device; also the hit-parity reference and the counters
+    log("plain per-scene path")
+    d_rays = [torch.from_numpy(x[2]).cuda() for x in scenes]
+    d_out = [torch.empty((n, 2), dtype=torch.int32, device="cuda") for n in nr]
+    d_st = [torch.empty((n, 2), dtype=torch.int32, device="cuda") for n in nr]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    solo = []
+    for s, (v, f, _) in enumerate(scenes):
+        sc = H.Scene(ctx)
+        sc.add_instance(sc.add_triangles(v, f))
+        sc.commit()
+        sc.trace_nearest(d_rays[s], nr[s], d_out[s], H.HIT_COMPACT, stream=sptr)
+        solo.append(sc)
+    torch.cuda.synchronize()
+    plain_ms = (time.perf_counter() - t0) * 1e3
+    mism = 0
+    bytes_trace = 0.0
+    for s, sc in enumerate(solo):
+        ref = d_out[s].cpu().numpy().view(np.uint32)
+        mism += int(np.sum(ref != hits[b.ray_off[s]:b.ray_off[s + 1]]))
+        sc.trace_nearest(d_rays[s], nr[s], d_out[s], H.HIT_COMPACT, stream=sptr, stats=d_st[s])
+    torch.cuda.synchronize()
+    for s in range(len(scenes)):
+        st = d_st[s].cpu().numpy().view(np.uint32).astype(np.int64)
+        bytes_trace += float(np.sum(40 + 64 * st[:, 0] + 60 * st[:, 1]))
+    for sc in solo:
+        sc.close()
+    achieved = bytes_trace / (trace_ms.mean() * 1e-3) / 1e9
+    log(f"plain path {plain_ms:.1f} ms, hit mismatches {mism}")
+
+    result = {
+        "metric": "Mrays/s closest/any-hit on 1 MI355X, % HBM roofline; p50/p99 query latency",
+        "value": round(R / (ms * 1e-3) / 1e6, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "batched: 256 random soups (1k-200k tris, log-uniform) with 4k-256k incoherent "
+                               "rays each; all LBVH builds + nearest-hit traces in one hipGraph launch",
+                   "scenes": len(scenes), "triangles": T, "rays": R},
+        "roofline": {"bound": "hbm", "kernel": "batched nearest-hit traversal",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                     "bytes_per_launch": bytes_trace, "mean_launch_ms": round(float(trace_ms.mean()), 4)},
+        "latency_ms": {"p50": round(float(np.percentile(step_ms, 50)), 4),
+                       "p99": round(float(np.percentile(step_ms, 99)), 4)},
+        "wall_ms_per_step": round(wall_ms, 4),
+        "build_ms": round(float(build_ms.mean()), 4),
+        "build_mtris_s": round(T / (build_ms.mean() * 1e-3) / 1e6, 1),
+        "trace_ms": round(float(trace_ms.mean()), 4),
+        "trace_mrays_s": round(R / (trace_ms.mean() * 1e-3) / 1e6, 1),
+        "autotune": {"chosen": {"waves": cands[chosen][0], "lds_stack": cands[chosen][1],
+                                "refill": cands[chosen][2], "claim": cands[chosen][3]},
+                     "candidates_ms": {f"w{c[0]}_lds{c[1]}_r{c[2]}_c{c[3]}": round(float(m), 4)
+                                       for c, m in zip(cands, cand_ms)}},
+        "plain_per_scene_path": {"ms": round(plain_ms, 2), "mrays_s": round(R / (plain_ms * 1e-3) / 1e6, 2),
+                                 "note": "256 x (hrt_add_triangles from host memory + commit + "
+                                         "hrt_trace_nearest), rays resident"},
+        "parity_mismatches_vs_plain": mism,
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        sample = scenes[:24]
+        rs = sum(x[2].shape[0] for x in sample)
+        t0 = time.perf_counter()
+        for v, f, rays in sample:
+            O.OracleBVH(v, f).trace(rays, threads=threads)
+        cpu_s = time.perf_counter() - t0
+        result["cpu_baseline"] = {"value": round(rs / cpu_s / 1e6, 3), "unit": "Mrays/s", "cores": threads,
+                                  "kind": "port",
+                                  "sample": f"first 24 of the 256 scenes: oracle LBVH build + nearest-hit trace of "
+                                            f"{rs} rays, {cpu_s:.2f} s"}
+    b.close()
     ctx.close()
     print(json.dumps(result), flush=True)
     return 0

@@ -28,6 +28,7 @@ LAYOUT_BVH2 = 0
 LAYOUT_BVH4 = 1
 LAYOUT_BVH4_FP16 = 2
 LAYOUT_AUTO = 3
+LAYOUT_AUTO_FP16 = 4
 
 DUMP_SIZES = 0
 DUMP_CODES_INPUT = 1
@@ -133,7 +134,7 @@ def lib():
         L.hrt_batch_build.argtypes = [vp, vp]
         L.hrt_batch_trace.argtypes = [vp, vp]
         L.hrt_batch_candidate_count.argtypes = []
-        L.hrt_batch_candidate_info.argtypes = [i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]
+        L.hrt_batch_candidate_info.argtypes = [i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32), C.POINTER(i32)]
         L.hrt_batch_set_candidate.argtypes = [vp, i32]
         L.hrt_batch_get_candidate.argtypes = [vp]
         L.hrt_batch_autotune.argtypes = [vp, i32, C.POINTER(i32), vp]
@@ -298,14 +299,14 @@ class Scene:
 
 
 def batch_candidates():
-    """[(waves per workgroup, LDS stack entries, refill threshold)] of the
-    batched trace kernel (LDS 0 = one ray per lane without refill)."""
+    """[(waves per workgroup, LDS stack entries, refill threshold, claim)] of
+    the batched trace kernel (LDS 0 = one ray per lane without refill)."""
     L = lib()
     out = []
     for i in range(L.hrt_batch_candidate_count()):
-        w, s, r = C.c_int(), C.c_int(), C.c_int()
-        _check(L.hrt_batch_candidate_info(i, C.byref(w), C.byref(s), C.byref(r)))
-        out.append((w.value, s.value, r.value))
+        w, s, r, c = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        _check(L.hrt_batch_candidate_info(i, C.byref(w), C.byref(s), C.byref(r), C.byref(c)))
+        out.append((w.value, s.value, r.value, c.value))
     return out
 
 

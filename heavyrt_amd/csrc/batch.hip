@@ -299,8 +299,7 @@ __global__ __launch_bounds__(kB) void k_b_leaves_refit(const float4* __restrict_
                                                        const uint32_t* __restrict__ leaf_off, SceneOffsets off,
                                                        NodePair* pairs_all, const uint32_t* __restrict__ parent_int_all,
                                                        const uint32_t* __restrict__ parent_leaf_all,
-                                                       const uint2* __restrict__ range_all, uint32_t* arrive,
-                                                       uint32_t* __restrict__ max_run) {
+                                                       const uint2* __restrict__ range_all, uint32_t* arrive) {
   const uint32_t total = leaf_off[off.S];
   const uint32_t chunk_lo = blockIdx.x * kRefitChunkB;
   if (chunk_lo >= total) return;
@@ -327,7 +326,6 @@ __global__ __launch_bounds__(kB) void k_b_leaves_refit(const float4* __restrict_
       hi = make_float3(fmaxf(hi.x, fmaxf(a.x, fmaxf(b.x, c.x))), fmaxf(hi.y, fmaxf(a.y, fmaxf(b.y, c.y))),
                        fmaxf(hi.z, fmaxf(a.z, fmaxf(b.z, c.z))));
     }
-    atomicMax(&max_run[s], e - st);
     NodePair* pairs = pairs_all + pb;
     if (L == 1) {
       // root is a leaf: pair 0 = (root leaf, empty dummy), as in lbvh.hip
@@ -365,6 +363,30 @@ __global__ __launch_bounds__(kB) void k_b_leaves_refit(const float4* __restrict_
   }
 }
 
+// Longest run (leaf size) per scene: one workgroup per scene, a block
+// reduction, no atomics (a per-leaf atomic on the scene's counter would put
+// millions of contended operations on a few addresses).
+__global__ __launch_bounds__(kB) void k_b_max_run(const uint32_t* __restrict__ run_start_all,
+                                                  const uint32_t* __restrict__ leaf_off, SceneOffsets off,
+                                                  uint32_t* __restrict__ max_run) {
+  __shared__ uint32_t red[kB];
+  const uint32_t s = blockIdx.x;
+  const uint32_t lb = leaf_off[s], le = leaf_off[s + 1];
+  const uint32_t n = off.tri[s + 1] - off.tri[s];
+  uint32_t mx = 0;
+  for (uint32_t m = lb + threadIdx.x; m < le; m += kB) {
+    const uint32_t e = (m + 1 < le) ? run_start_all[m + 1] : n;
+    mx = max(mx, e - run_start_all[m]);
+  }
+  red[threadIdx.x] = mx;
+  __syncthreads();
+  for (int k = kB / 2; k > 0; k >>= 1) {
+    if (threadIdx.x < k) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + k]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) max_run[s] = red[0];
+}
+
 __global__ __launch_bounds__(kB) void k_b_tris(const float4* __restrict__ v, const uint32_t* __restrict__ idx,
                                                const uint32_t* __restrict__ ids_sorted, SceneOffsets off, uint32_t T,
                                                TriPre* __restrict__ tris, uint32_t* __restrict__ perm) {
@@ -392,7 +414,7 @@ __global__ void k_b_reset(uint32_t* c) { *c = 0u; }
 // Persistent nearest-hit over all scenes' rays.  kWaves waves per workgroup,
 // kLds stack entries per lane in LDS (the rest of the 32-entry worst case in
 // a per-lane global spill area), refill when kRefill lanes are idle.
-template <int kWaves, int kLds, int kRefill>
+template <int kWaves, int kLds, int kRefill, uint32_t kClaim>
 __global__ __launch_bounds__(64 * kWaves) void k_b_trace(const NodePair* __restrict__ pairs_all,
                                                          const TriPre* __restrict__ tris_all, SceneOffsets off,
                                                          const Ray* __restrict__ rays, uint32_t R,
@@ -418,18 +440,14 @@ __global__ __launch_bounds__(64 * kWaves) void k_b_trace(const NodePair* __restr
   h.u = h.v = 0.f;
   int top = 0;
   uint32_t cur = kInvalid;
-  bool drained = false;
+  RayPool pool;  // ray indices claimed by this wave
 
   while (true) {
     const uint64_t idle = __ballot(!active);
     const uint32_t nidle = __popcll(idle);
-    if (!drained && (nidle >= uint32_t(kRefill) || nidle == 64)) {
-      uint32_t base = 0;
-      if (lane == 0) base = atomicAdd(queue, nidle);
-      base = __shfl(base, 0);
-      if (uint64_t(base) + nidle >= R) drained = true;
+    if (!pool.drained() && (nidle >= uint32_t(kRefill) || nidle == 64)) {
+      const uint64_t idx = pool.take<kClaim>(nidle, __popcll(idle & lt_mask), queue, R);
       if (!active) {
-        const uint64_t idx = uint64_t(base) + __popcll(idle & lt_mask);
         if (idx < R) {
           rid = uint32_t(idx);
           const uint32_t s = seg_of(off.ray, off.S, rid);
@@ -483,7 +501,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_b_trace(const NodePair* __restr
         active = false;
       }
       const uint32_t nact = __popcll(__ballot(active));
-      if (nact == 0 || (!drained && 64 - nact >= uint32_t(kRefill))) break;
+      if (nact == 0 || (!pool.drained() && 64 - nact >= uint32_t(kRefill))) break;
     }
   }
 }
@@ -549,21 +567,22 @@ __global__ __launch_bounds__(kBlk) void k_b_trace_simple(const NodePair* __restr
 // trace-kernel candidates
 // ---------------------------------------------------------------------------
 struct Candidate {
-  int waves, lds, refill;  // lds == 0: one ray per lane, `waves` * 64 threads
+  int waves, lds, refill, claim;  // lds == 0: one ray per lane, `waves` * 64 threads
   const void* fn;
 };
 
-#define HRT_P(W, S, F) {W, S, F, reinterpret_cast<const void*>(&k_b_trace<W, S, F>)}
-#define HRT_S(W) {W, 0, 0, reinterpret_cast<const void*>(&k_b_trace_simple<64 * W>)}
+#define HRT_P(W, S, F, C) {W, S, F, C, reinterpret_cast<const void*>(&k_b_trace<W, S, F, C>)}
+#define HRT_S(W) {W, 0, 0, 0, reinterpret_cast<const void*>(&k_b_trace_simple<64 * W>)}
 const Candidate kCandidates[] = {
-    HRT_P(1, 8, 16),  HRT_P(1, 16, 8),  HRT_P(1, 16, 16), HRT_P(1, 16, 32), HRT_P(1, 24, 16),
-    HRT_P(1, 32, 16), HRT_P(2, 16, 16), HRT_P(4, 8, 16),  HRT_P(4, 16, 16), HRT_S(1),
+    HRT_P(1, 16, 16, 64),  HRT_P(1, 16, 16, 128), HRT_P(1, 16, 16, 256), HRT_P(1, 16, 32, 64),
+    HRT_P(1, 16, 32, 128), HRT_P(1, 16, 8, 128),  HRT_P(1, 8, 16, 128),  HRT_P(1, 24, 16, 128),
+    HRT_P(1, 32, 16, 128), HRT_P(2, 16, 16, 128), HRT_P(4, 16, 16, 128), HRT_S(1),
     HRT_S(4),
 };
 #undef HRT_P
 #undef HRT_S
 constexpr int kNumCandidates = int(sizeof(kCandidates) / sizeof(kCandidates[0]));
-constexpr int kDefaultCandidate = 2;
+constexpr int kDefaultCandidate = 1;
 
 int resident_blocks(const void* fn, int threads) {
   int dev = 0, cus = 0, per_cu = 0;
@@ -645,7 +664,6 @@ hipError_t enqueue_build(hrt_batch* b, hipStream_t s) {
   const uint32_t T = uint32_t(b->T);
   hipError_t e;
   if ((e = fill_words(b->arrive, b->P, 0u, s)) != hipSuccess) return e;
-  if ((e = fill_words(b->max_run, b->S, 0u, s)) != hipSuccess) return e;
   k_b_box_partial<<<b->nchunks, kB, 0, s>>>(b->verts, b->d_chunks, b->part);
   k_b_box_final<<<b->S, kB, 0, s>>>(b->part, b->d_chunk_off, b->box12);
   k_b_morton<<<blocks(T), kB, 0, s>>>(b->verts, b->idx, off, T, b->box12, b->codes_in, b->ka, b->va);
@@ -675,7 +693,8 @@ hipError_t enqueue_build(hrt_batch* b, hipStream_t s) {
                                       b->parent_leaf, b->range);
   k_b_leaves_refit<<<uint32_t((T + kRefitChunkB - 1) / kRefitChunkB), kB, 0, s>>>(
       b->verts, b->idx, vin, b->run_start, b->leaf_off, off, b->pairs, b->parent_int, b->parent_leaf, b->range,
-      b->arrive, b->max_run);
+      b->arrive);
+  k_b_max_run<<<b->S, kB, 0, s>>>(b->run_start, b->leaf_off, off, b->max_run);
   k_b_tris<<<blocks(T), kB, 0, s>>>(b->verts, b->idx, vin, off, T, b->tris, b->perm);
   return hipGetLastError();
 }
@@ -919,11 +938,12 @@ hrt_status hrt_batch_run(hrt_batch* b, void* stream) {
 
 int hrt_batch_candidate_count(void) { return hrt::kNumCandidates; }
 
-hrt_status hrt_batch_candidate_info(int i, int* wavesPerGroup, int* ldsStackEntries, int* refillAt) {
+hrt_status hrt_batch_candidate_info(int i, int* wavesPerGroup, int* ldsStackEntries, int* refillAt, int* claim) {
   if (i < 0 || i >= hrt::kNumCandidates) return fail(HRT_ERR_INVALID_ARG, "no such candidate");
   if (wavesPerGroup) *wavesPerGroup = hrt::kCandidates[i].waves;
   if (ldsStackEntries) *ldsStackEntries = hrt::kCandidates[i].lds;
   if (refillAt) *refillAt = hrt::kCandidates[i].refill;
+  if (claim) *claim = hrt::kCandidates[i].claim;
   return HRT_OK;
 }
 

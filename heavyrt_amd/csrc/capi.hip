@@ -225,10 +225,10 @@ hrt_status trace_common(hrt_scene* s, const float* rays, size_t n, void* out, in
     const int lay = s->layout;
     if (s->two_level)
       e = hrt::trace_two_level(s->tlas, s->inst_dev ? s->inst_dev + s->instances.size() : nullptr, a, hs);
-    else if (lay == HRT_LAYOUT_BVH2 || (lay == HRT_LAYOUT_AUTO && !anyhit))
+    else if (lay == HRT_LAYOUT_BVH2 || ((lay == HRT_LAYOUT_AUTO || lay == HRT_LAYOUT_AUTO_FP16) && !anyhit))
       e = hrt::trace_blas(*b, a, hs);
     else
-      e = hrt::trace_blas4(*b, a, lay == HRT_LAYOUT_BVH4_FP16, hs);
+      e = hrt::trace_blas4(*b, a, lay == HRT_LAYOUT_BVH4_FP16 || lay == HRT_LAYOUT_AUTO_FP16, hs);
   }
   if (e != hipSuccess) return fail_hip(e, "trace");
   return HRT_OK;
@@ -380,7 +380,7 @@ hrt_status hrt_add_instance(hrt_scene* s, uint32_t geomId, const float colMajor1
 hrt_status hrt_commit(hrt_scene* s, int quality, int layout) {
   (void)quality;
   if (!s) return fail(HRT_ERR_INVALID_ARG, "scene is null");
-  if (layout < HRT_LAYOUT_BVH2 || layout > HRT_LAYOUT_AUTO) return fail(HRT_ERR_INVALID_ARG, "unknown layout");
+  if (layout < HRT_LAYOUT_BVH2 || layout > HRT_LAYOUT_AUTO_FP16) return fail(HRT_ERR_INVALID_ARG, "unknown layout");
   s->committed = false;
   free_tlas(s);
   const bool single = s->instances.size() == 1 && s->instances[0].identity;
@@ -396,7 +396,7 @@ hrt_status hrt_commit(hrt_scene* s, int quality, int layout) {
   } else if (layout != HRT_LAYOUT_BVH2) {
     for (const Instance& in : s->instances) {
       hrt::Blas& b = s->geoms[in.geom];
-      if (layout == HRT_LAYOUT_BVH4_FP16) {
+      if (layout == HRT_LAYOUT_BVH4_FP16 || layout == HRT_LAYOUT_AUTO_FP16) {
         // FP16 boxes are only conservative inside the half range (cbvh_fat.cpp:447-448
         // maps every value beyond +-65504 to +-65504)
         for (int k = 0; k < 3; k++)

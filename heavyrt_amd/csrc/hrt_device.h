@@ -84,6 +84,45 @@ struct ShortStack {
 };
 constexpr int kShortStack = 16;  // LDS entries per lane in the persistent kernels
 
+// Ray indices for the idle lanes of a persistent wave.  The wave claims
+// kClaim consecutive indices from the global queue with one atomic and hands
+// them out to its own idle lanes at later refills, so the single global
+// counter is hit once per kClaim rays instead of once per refill (a counter
+// that every wave of the chip hits at each refill limits the whole kernel).
+// Every member is wave-uniform; take() must be called by all lanes of the
+// wave together.
+struct RayPool {
+  uint64_t next = 0, end = 0;  // indices claimed by the wave, not yet given out
+  bool exhausted = false;      // the global queue has been run past n
+
+  // Returns this lane's ray index, or n when it gets none.  nidle = number of
+  // idle lanes, rank = this lane's position among them.
+  template <uint32_t kClaim>
+  __device__ inline uint64_t take(uint32_t nidle, uint32_t rank, uint32_t* queue, uint64_t n) {
+    const uint64_t avail = end - next;
+    const bool claim = avail < nidle && !exhausted;
+    uint64_t nb = 0;
+    if (claim) {
+      uint32_t b = 0;
+      if ((threadIdx.x & 63) == 0) b = atomicAdd(queue, kClaim);
+      nb = uint64_t(__shfl(b, 0));
+    }
+    uint64_t idx = n;
+    if (rank < avail) idx = next + rank;
+    else if (claim) idx = nb + (rank - avail);
+    if (claim) {
+      end = (nb + kClaim < n) ? nb + kClaim : n;
+      next = nb + (nidle - avail);
+      if (next > end) next = end;
+      if (nb + kClaim >= n) exhausted = true;
+    } else {
+      next += (avail < nidle) ? avail : nidle;
+    }
+    return idx < n ? idx : n;
+  }
+  __device__ inline bool drained() const { return exhausted && next >= end; }
+};
+
 // BVH4 node with FP32 boxes, 128 bytes, coordinates of the four children
 // grouped per axis so each 16-byte load feeds four slab tests.  Children are
 // the grandchildren of one even-depth Karras node in left-to-right order;

@@ -39,6 +39,7 @@ constexpr int kWave = 64;
 constexpr int kWavesPerBlock = 1;
 constexpr int kTraceBlock = kWave * kWavesPerBlock;
 constexpr int kRefillAt = 16;  // refill when at least this many lanes are idle
+constexpr uint32_t kClaim = 128;  // ray indices a wave claims from the queue at once
 
 struct HitOutFull {  // CRT_Hit (CrossRT.h:20-27)
   float t;
@@ -154,19 +155,15 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_persistent(const NodePair
   int top = 0;          // entries on the stack
   uint32_t cur = kInvalid;   // node pair to test next, a leaf, or kInvalid
   uint32_t leaf = 0;    // parked leaf (speculative any-hit), 0 = none
-  bool drained = false; // the queue is empty
+  RayPool pool;  // ray indices claimed by this wave
 
   while (true) {
     // ---- refill idle lanes from the global queue -------------------------
     const uint64_t idle = __ballot(!active);
     const uint32_t nidle = __popcll(idle);
-    if (!drained && (nidle >= kRefillAt || nidle == kWave)) {
-      uint32_t base = 0;
-      if (lane == 0) base = atomicAdd(queue, nidle);
-      base = __shfl(base, 0);
-      if (uint64_t(base) + nidle >= n) drained = true;
+    if (!pool.drained() && (nidle >= kRefillAt || nidle == kWave)) {
+      const uint64_t idx = pool.take<kClaim>(nidle, __popcll(idle & lt_mask), queue, n);
       if (!active) {
-        const uint64_t idx = uint64_t(base) + __popcll(idle & lt_mask);
         if (idx < n) {
           rid = idx;
           const Ray r = rays[idx];
@@ -272,7 +269,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_persistent(const NodePair
         active = false;
       }
       const uint32_t nact = __popcll(__ballot(active));
-      if (nact == 0 || (!drained && kWave - nact >= kRefillAt)) break;
+      if (nact == 0 || (!pool.drained() && kWave - nact >= kRefillAt)) break;
     }
   }
 }

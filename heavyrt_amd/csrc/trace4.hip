@@ -24,6 +24,7 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kBlock4 = 64;
 constexpr int kRefillAt4 = 16;
+constexpr uint32_t kClaim = 128;  // ray indices a wave claims from the queue at once
 constexpr float kMaxFloat = 3.402823466e+38f;
 
 struct HitOutFull {
@@ -188,18 +189,14 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_any_persistent(const void* _
   int top = 0;
   uint32_t cur = kInvalid;
   uint32_t leaf = 0;
-  bool drained = false;
+  RayPool pool;  // ray indices claimed by this wave
 
   while (true) {
     const uint64_t idle = __ballot(!active);
     const uint32_t nidle = __popcll(idle);
-    if (!drained && (nidle >= kRefillAt4 || nidle == kWave)) {
-      uint32_t base = 0;
-      if (lane == 0) base = atomicAdd(queue, nidle);
-      base = __shfl(base, 0);
-      if (uint64_t(base) + nidle >= n) drained = true;
+    if (!pool.drained() && (nidle >= kRefillAt4 || nidle == kWave)) {
+      const uint64_t idx = pool.take<kClaim>(nidle, __popcll(idle & lt_mask), queue, n);
       if (!active) {
-        const uint64_t idx = uint64_t(base) + __popcll(idle & lt_mask);
         if (idx < n) {
           rid = idx;
           const Ray r = rays[idx];
@@ -250,7 +247,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_any_persistent(const void* _
         active = false;
       }
       const uint32_t nact = __popcll(__ballot(active));
-      if (nact == 0 || (!drained && kWave - nact >= kRefillAt4)) break;
+      if (nact == 0 || (!pool.drained() && kWave - nact >= kRefillAt4)) break;
     }
   }
 }

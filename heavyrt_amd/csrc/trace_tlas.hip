@@ -37,6 +37,7 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kBlock = kWave;
 constexpr int kRefillAt = 16;
+constexpr uint32_t kClaim = 128;  // ray indices a wave claims from the queue at once
 constexpr int kTwoLevelDepth = 64;
 
 __global__ void k_reset_counter(uint32_t* c) { *c = 0u; }
@@ -98,7 +99,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_two_level(const NodePair* __re
   uint32_t cur_inst = 0, cur_geom = 0;
   uint32_t cur = kInvalid;
   uint32_t npairs = 0, ntests = 0;
-  bool drained = false;
+  RayPool pool;  // ray indices claimed by this wave
 
   auto enter = [&](uint32_t slot) {
     const InstDev* I = inst + slot;
@@ -133,13 +134,9 @@ __global__ __launch_bounds__(kBlock) void k_trace_two_level(const NodePair* __re
   while (true) {
     const uint64_t idle = __ballot(!active);
     const uint32_t nidle = __popcll(idle);
-    if (!drained && (nidle >= kRefillAt || nidle == kWave)) {
-      uint32_t base = 0;
-      if (lane == 0) base = atomicAdd(queue, nidle);
-      base = __shfl(base, 0);
-      if (uint64_t(base) + nidle >= n) drained = true;
+    if (!pool.drained() && (nidle >= kRefillAt || nidle == kWave)) {
+      const uint64_t idx = pool.take<kClaim>(nidle, __popcll(idle & lt_mask), queue, n);
       if (!active) {
-        const uint64_t idx = uint64_t(base) + __popcll(idle & lt_mask);
         if (idx < n) {
           rid = idx;
           const Ray r = rays[idx];
@@ -211,7 +208,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_two_level(const NodePair* __re
         active = false;
       }
       const uint32_t nact = __popcll(__ballot(active));
-      if (nact == 0 || (!drained && kWave - nact >= kRefillAt)) break;
+      if (nact == 0 || (!pool.drained() && kWave - nact >= kRefillAt)) break;
     }
   }
 }

@@ -206,6 +206,8 @@ ISceneObject* CreateSceneRT(const char* a_implName, const char* a_buildName, con
     layout = HRT_LAYOUT_BVH4_FP16;
   else if (name.find("HipBVH4") != std::string::npos)
     layout = HRT_LAYOUT_BVH4;
+  else if (name.find("HipAutoHalf") != std::string::npos)
+    layout = HRT_LAYOUT_AUTO_FP16;
   else if (name.find("HipAuto") != std::string::npos)
     layout = HRT_LAYOUT_AUTO;
   else if (name.find("HipBVH2") == std::string::npos && name.find("HipLBVH") == std::string::npos) {

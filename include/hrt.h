@@ -74,8 +74,16 @@ enum { HRT_HIT_FULL = 0, HRT_HIT_COMPACT = 1 };
  *              hit by (t, primId) so the answer does not depend on the order.
  *   BVH4_FP16  as BVH4 with FP16 boxes rounded outward (cbvh_fat.cpp:447-497);
  *              requires every coordinate within +-65504.
- *   AUTO       nearest hits on BVH2 (reference order), any-hit on BVH4. */
-enum { HRT_LAYOUT_BVH2 = 0, HRT_LAYOUT_BVH4 = 1, HRT_LAYOUT_BVH4_FP16 = 2, HRT_LAYOUT_AUTO = 3 };
+ *   AUTO       nearest hits on BVH2 (reference order), any-hit on BVH4.
+ *   AUTO_FP16  nearest hits on BVH2 (reference order), any-hit on BVH4_FP16
+ *              (half the node bytes per visit; the same any-hit answers). */
+enum {
+  HRT_LAYOUT_BVH2 = 0,
+  HRT_LAYOUT_BVH4 = 1,
+  HRT_LAYOUT_BVH4_FP16 = 2,
+  HRT_LAYOUT_AUTO = 3,
+  HRT_LAYOUT_AUTO_FP16 = 4
+};
 enum { HRT_BUILD_LOW = 0, HRT_BUILD_MEDIUM = 1, HRT_BUILD_HIGH = 2, HRT_BUILD_REFIT = 3 }; /* BuildQuality, CrossRT.h:9-15 */
 
 /* Debug / parity dumps (hrt_debug_dump `what`).  Available for geometries
@@ -215,9 +223,10 @@ hrt_status hrt_batch_run(hrt_batch* batch, void* stream);
 hrt_status hrt_batch_build(hrt_batch* batch, void* stream);
 hrt_status hrt_batch_trace(hrt_batch* batch, void* stream);
 /* Trace-kernel candidates: waves per workgroup, LDS stack entries per lane
- * (0 = one ray per lane without refill) and the refill threshold. */
+ * (0 = one ray per lane without refill), the refill threshold (idle lanes)
+ * and the number of ray indices a wave claims from the queue at once. */
 int hrt_batch_candidate_count(void);
-hrt_status hrt_batch_candidate_info(int i, int* wavesPerGroup, int* ldsStackEntries, int* refillAt);
+hrt_status hrt_batch_candidate_info(int i, int* wavesPerGroup, int* ldsStackEntries, int* refillAt, int* claim);
 hrt_status hrt_batch_set_candidate(hrt_batch* batch, int i);
 int hrt_batch_get_candidate(const hrt_batch* batch);
 /* Builds if needed, times every candidate `reps` times on the batch's own

@@ -104,8 +104,8 @@ def test_bvh4_nearest_bit_exact(gpu_ctx, name, gen, layout):
 
 
 @pytest.mark.parametrize("name,gen", CASES, ids=[c[0] for c in CASES])
-@pytest.mark.parametrize("layout", [H.LAYOUT_BVH4, H.LAYOUT_BVH4_FP16, H.LAYOUT_AUTO],
-                         ids=["fp32", "fp16", "auto"])
+@pytest.mark.parametrize("layout", [H.LAYOUT_BVH4, H.LAYOUT_BVH4_FP16, H.LAYOUT_AUTO, H.LAYOUT_AUTO_FP16],
+                         ids=["fp32", "fp16", "auto", "auto_fp16"])
 def test_bvh4_any_hit(gpu_ctx, name, gen, layout):
     v, f = gen()
     rays = rays_for(v, 20000, 1700)
@@ -119,7 +119,7 @@ def test_bvh4_any_hit(gpu_ctx, name, gen, layout):
     g = trace(s, rays, "any")
     np.testing.assert_array_equal(g["hit"], p2 != INVALID)
     if layout != H.LAYOUT_AUTO:
-        mode = O.MODE4_FP16 if layout == H.LAYOUT_BVH4_FP16 else O.MODE4_FP32
+        mode = O.MODE4_FP16 if layout in (H.LAYOUT_BVH4_FP16, H.LAYOUT_AUTO_FP16) else O.MODE4_FP32
         _, p4, _, _, nq, nt = ob.trace4(anyr, mode, counters=True)
         gs = trace(s, rays, "any", stats=True)
         np.testing.assert_array_equal(gs["hit"], p4 != INVALID)
@@ -128,11 +128,12 @@ def test_bvh4_any_hit(gpu_ctx, name, gen, layout):
     s.close()
 
 
-def test_auto_layout_nearest_is_reference_order(gpu_ctx):
+@pytest.mark.parametrize("layout", [H.LAYOUT_AUTO, H.LAYOUT_AUTO_FP16], ids=["auto", "auto_fp16"])
+def test_auto_layout_nearest_is_reference_order(gpu_ctx, layout):
     v, f = W.eye_scene()
     rays = W.eye_rays(256, 256)
     ob = O.OracleBVH(v, f)
-    s = scene_with(gpu_ctx, v, f, H.LAYOUT_AUTO)
+    s = scene_with(gpu_ctx, v, f, layout)
     g = trace(s, rays, "full")
     t, p, u, vv = ob.trace(rays)
     np.testing.assert_array_equal(g["t"].view(np.uint32), t.view(np.uint32))

@@ -54,7 +54,7 @@ def run_harness(tmp_path, impl, v, f, rays):
     return per_ray, batch, any_per_ray, any_batch, counts, r.stdout
 
 
-@pytest.mark.parametrize("impl", ["HipBVH2", "HipBVH4", "HipBVH4Half", "HipAuto"])
+@pytest.mark.parametrize("impl", ["HipBVH2", "HipBVH4", "HipBVH4Half", "HipAuto", "HipAutoHalf"])
 def test_renderer_call_pattern(tmp_path, gpu_ctx, impl):
     assert os.path.exists(B.HARNESS_BIN), "build the harness first (python -m heavyrt_amd.build)"
     v, f = W.merge(W.icosphere(3), W.ground_quad())
@@ -62,7 +62,7 @@ def test_renderer_call_pattern(tmp_path, gpu_ctx, impl):
     say(f"harness {impl}: {f.shape[0]} tris, {rays.shape[0]} rays")
     per_ray, batch, any_pr, any_b, counts, _ = run_harness(tmp_path, impl, v, f, rays)
     ob = O.OracleBVH(v, f)
-    if impl in ("HipBVH2", "HipAuto"):
+    if impl in ("HipBVH2", "HipAuto", "HipAutoHalf"):
         t, p, u, vv = ob.trace(rays)
     else:
         t, p, u, vv = ob.trace4(rays, O.MODE4_FP16 if impl == "HipBVH4Half" else O.MODE4_FP32)
