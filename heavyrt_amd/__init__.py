@@ -66,6 +66,8 @@ EXPORTS = [
     "hrt_batch_create", "hrt_batch_destroy", "hrt_batch_buffers", "hrt_batch_run", "hrt_batch_build",
     "hrt_batch_trace", "hrt_batch_candidate_count", "hrt_batch_candidate_info", "hrt_batch_set_candidate",
     "hrt_batch_get_candidate", "hrt_batch_autotune", "hrt_batch_dump",
+    "hrt_host_alloc", "hrt_host_free", "hrt_server_create", "hrt_server_destroy", "hrt_server_add_scene",
+    "hrt_server_submit", "hrt_server_poll", "hrt_server_wait", "hrt_server_stats",
 ]
 
 
@@ -82,9 +84,14 @@ _live_scenes: "weakref.WeakSet" = weakref.WeakSet()
 _live_contexts: "weakref.WeakSet" = weakref.WeakSet()
 
 
+def _close_order(objs):
+    # servers first: they trace scenes they do not own
+    return sorted(objs, key=lambda o: 0 if type(o).__name__ == "Server" else 1)
+
+
 @atexit.register
 def _close_all() -> None:
-    for s in list(_live_scenes):
+    for s in _close_order(list(_live_scenes)):
         s.close()
     for c in list(_live_contexts):
         c.close()
@@ -139,6 +146,17 @@ def lib():
         L.hrt_batch_get_candidate.argtypes = [vp]
         L.hrt_batch_autotune.argtypes = [vp, i32, C.POINTER(i32), vp]
         L.hrt_batch_dump.argtypes = [vp, u32, i32, vp, sz]
+        L.hrt_host_alloc.argtypes = [sz, C.POINTER(vp)]
+        L.hrt_host_free.argtypes = [vp]
+        L.hrt_host_free.restype = None
+        L.hrt_server_create.argtypes = [vp, C.c_uint64, u32, C.POINTER(vp)]
+        L.hrt_server_destroy.argtypes = [vp]
+        L.hrt_server_destroy.restype = None
+        L.hrt_server_add_scene.argtypes = [vp, vp, C.POINTER(u32)]
+        L.hrt_server_submit.argtypes = [vp, u32, i32, vp, sz, vp, C.POINTER(C.c_uint64)]
+        L.hrt_server_poll.argtypes = [vp, C.c_uint64]
+        L.hrt_server_wait.argtypes = [vp, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double)]
+        L.hrt_server_stats.argtypes = [vp, vp]
         _lib = L
     return _lib
 
@@ -176,7 +194,7 @@ class Context:
         """Destroys the context, and first every scene still open on it (a scene
         must not outlive the context it was created on)."""
         if self._h:
-            for s in list(_live_scenes):
+            for s in _close_order(list(_live_scenes)):
                 if s.ctx is self:
                     s.close()
             lib().hrt_shutdown(self._h)
@@ -426,3 +444,87 @@ class Batch:
             out = np.zeros(int(sizes[0]), dtype=np.uint32)
         _check(lib().hrt_batch_dump(self._h, scene, what, _ptr(out), out.nbytes))
         return out
+
+
+class _PinnedHolder:
+    def __init__(self, ptr: int):
+        self.ptr = ptr
+
+    def __del__(self):
+        if self.ptr and _lib is not None:
+            _lib.hrt_host_free(self.ptr)
+            self.ptr = 0
+
+
+class PinnedArray(np.ndarray):
+    """numpy array in pinned host memory (hrt_host_alloc); freed with its last view."""
+
+
+def pinned_array(shape, dtype=np.float32) -> np.ndarray:
+    dtype = np.dtype(dtype)
+    count = int(np.prod(shape))
+    nbytes = max(count * dtype.itemsize, 16)
+    p = C.c_void_p()
+    _check(lib().hrt_host_alloc(nbytes, C.byref(p)))
+    holder = _PinnedHolder(p.value)
+    buf = (C.c_byte * nbytes).from_address(p.value)
+    arr = np.frombuffer(buf, dtype=dtype, count=count).reshape(shape).view(PinnedArray)
+    arr._holder = holder
+    return arr
+
+
+class Server:
+    """Resident scenes and a continuous-batching request loop (hrt_server_*)."""
+
+    def __init__(self, ctx: Context, budget_bytes: int, steps_in_flight: int = 2):
+        self.ctx = ctx
+        self._h = C.c_void_p()
+        _check(lib().hrt_server_create(ctx._h, int(budget_bytes), steps_in_flight, C.byref(self._h)))
+        _live_scenes.add(self)
+        self._keep = {}
+
+    def close(self):
+        if self._h:
+            lib().hrt_server_destroy(self._h)
+            self._h = C.c_void_p()
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add_scene(self, scene: "Scene") -> int:
+        sid = C.c_uint32()
+        _check(lib().hrt_server_add_scene(self._h, scene._h, C.byref(sid)))
+        return sid.value
+
+    def submit(self, scene_id: int, rays: np.ndarray, out: np.ndarray, any_hit: bool = False) -> int:
+        """rays: pinned (n, 8) float32; out: pinned (n, 2) uint32 (nearest) or (n,) uint8 (any-hit)."""
+        n = rays.shape[0]
+        rid = C.c_uint64()
+        _check(lib().hrt_server_submit(self._h, scene_id, 1 if any_hit else 0, rays.ctypes.data, n, out.ctypes.data,
+                                       C.byref(rid)))
+        self._keep[rid.value] = (rays, out)
+        return rid.value
+
+    def poll(self, rid: int) -> int:
+        return int(lib().hrt_server_poll(self._h, rid))
+
+    def wait(self, rid: int):
+        """Returns (latency ms: submit -> results on the host, queue ms: submit -> admission)."""
+        lat, q = C.c_double(), C.c_double()
+        try:
+            _check(lib().hrt_server_wait(self._h, rid, C.byref(lat), C.byref(q)))
+        finally:
+            self._keep.pop(rid, None)
+        return lat.value, q.value
+
+    def stats(self) -> dict:
+        o = np.zeros(6, dtype=np.uint64)
+        _check(lib().hrt_server_stats(self._h, _ptr(o)))
+        return {"completed": int(o[0]), "rays": int(o[1]), "steps": int(o[2]),
+                "mean_requests_per_step": int(o[3]) / 1000.0, "peak_bytes_in_flight": int(o[4]),
+                "budget_bytes": int(o[5])}
+

@@ -189,7 +189,7 @@ hrt_status select_blas(hrt_scene* s, const hrt::Blas** out, uint32_t* inst) {
   if (!s) return fail(HRT_ERR_INVALID_ARG, "scene is null");
   if (!s->committed) return fail(HRT_ERR_NOT_COMMITTED, "CommitScene has not been called since the last change");
   if (s->instances.size() != 1 || !s->instances[0].identity)
-    return fail(HRT_ERR_UNSUPPORTED, "this build traces scenes with exactly one identity instance");
+    return fail(HRT_ERR_UNSUPPORTED, "needs a committed scene with exactly one identity instance");
   *out = &s->geoms[s->instances[0].geom];
   *inst = 0;
   return HRT_OK;
@@ -235,6 +235,13 @@ hrt_status trace_common(hrt_scene* s, const float* rays, size_t n, void* out, in
 }
 
 }  // namespace
+
+namespace hrt {
+hrt_status scene_single_blas(hrt_scene* s, const Blas** out) {
+  uint32_t inst = 0;
+  return select_blas(s, out, &inst);
+}
+}  // namespace hrt
 
 extern "C" {
 

@@ -24,6 +24,10 @@ namespace hrt {
 hrt_status fail(hrt_status st, const std::string& msg);
 hrt_status fail_hip(hipError_t e, const char* what);
 
+struct Blas;
+// The geometry tree of a committed scene with exactly one identity instance.
+hrt_status scene_single_blas(hrt_scene* s, const Blas** out);
+
 // Device buffers of one bottom-level structure (one geometry).
 struct Blas {
   uint32_t ntris = 0;

@@ -237,6 +237,41 @@ hrt_status hrt_batch_autotune(hrt_batch* batch, int reps, int* chosen, float* ms
  * leaves, longest run, pairs, first pair), HRT_DUMP_PAIRS, HRT_DUMP_PERM. */
 hrt_status hrt_batch_dump(hrt_batch* batch, uint32_t scene, int what, void* host_out, size_t bytes);
 
+/* ---- Serving: resident scenes, continuous batching ------------------------
+ * No counterpart in the reference, which answers one query per virtual call
+ * on the caller's thread (ISceneObject::RayQuery_NearestHit / _AnyHit,
+ * CrossRT.h:143-151).  A server keeps committed scenes resident and takes
+ * requests (a ray batch for one scene, nearest or any) from any thread.  Its
+ * scheduler thread admits queued requests in arrival order while their
+ * device buffers (32 B per ray + the results) fit in the HBM budget, runs all
+ * admitted requests of a step as one trace launch, keeps up to stepsInFlight
+ * steps on the device at once, and admits new arrivals into the next step
+ * while earlier ones run.  Results equal hrt_trace_nearest (compact hits,
+ * reference order) / hrt_trace_any on the same rays, alone or in any mix. */
+typedef struct hrt_server hrt_server;
+/* Pinned host memory for request rays and results (hipHostMalloc). */
+hrt_status hrt_host_alloc(size_t bytes, void** out);
+void hrt_host_free(void* p);
+hrt_status hrt_server_create(hrt_ctx* ctx, uint64_t budgetBytes, uint32_t stepsInFlight, hrt_server** out);
+/* Waits for every submitted request, then frees the server. */
+void hrt_server_destroy(hrt_server* server);
+/* A committed scene with one identity instance; it must outlive the server. */
+hrt_status hrt_server_add_scene(hrt_server* server, hrt_scene* scene, uint32_t* sceneId);
+/* rays: n x 8 floats, out: n x hrt_hit_compact (nearest) or n bytes (any-hit);
+ * both pinned (hrt_host_alloc) and untouched by the caller until the request
+ * is done.  Fails with HRT_ERR_OUT_OF_MEMORY if the request alone exceeds
+ * the budget. */
+hrt_status hrt_server_submit(hrt_server* server, uint32_t sceneId, int anyHit, const float* rays, size_t n,
+                             void* out, uint64_t* requestId);
+/* 1 = done, 0 = pending, negative = failed or unknown. */
+int hrt_server_poll(hrt_server* server, uint64_t requestId);
+/* Blocks until the request is done and forgets it; latencyMs = submit ->
+ * results in host memory, queueMs = submit -> admission (both nullable). */
+hrt_status hrt_server_wait(hrt_server* server, uint64_t requestId, double* latencyMs, double* queueMs);
+/* out: completed requests, rays, steps, mean requests per step x 1000, peak
+ * device bytes in flight, budget bytes. */
+hrt_status hrt_server_stats(hrt_server* server, uint64_t out[6]);
+
 /* Thread-local description of the last failure in this thread. */
 const char* hrt_last_error(void);
 
