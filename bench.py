@@ -27,6 +27,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -72,9 +73,12 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true", help="skip timing the other any-hit layouts")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, os.cpu_count())")
-    ap.add_argument("--workload", choices=["rtao", "batched"], default="rtao",
+    ap.add_argument("--workload", choices=["rtao", "batched", "serving"], default="rtao",
                     help="rtao (default): the headline RTAO step; batched: 256 scenes built and traced "
-                         "in one graph launch")
+                         "in one graph launch; serving: Poisson request stream over 16 resident scenes")
+    ap.add_argument("--qps", type=str, default="1000,4000,12000",
+                    help="serving: comma-separated request rates to run")
+    ap.add_argument("--requests", type=int, default=1500, help="serving: requests per rate")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="only run 3 device-to-device copies of this many MiB (scales rocprofv3 "
                          "FETCH_SIZE/WRITE_SIZE) and exit")
@@ -107,6 +111,8 @@ def main() -> int:
     torch.cuda.init()
     if args.workload == "batched":
         return bench_batched(args)
+    if args.workload == "serving":
+        return bench_serving(args)
     dev = torch.device("cuda:0")
     layouts = {"bvh2": H.LAYOUT_BVH2, "bvh4": H.LAYOUT_BVH4, "bvh4_fp16": H.LAYOUT_BVH4_FP16}
 
@@ -438,6 +444,138 @@ def bench_batched(args) -> int:
                                   "sample": f"first 24 of the 256 scenes: oracle LBVH build + nearest-hit trace of "
                                             f"{rs} rays, {cpu_s:.2f} s"}
     b.close()
+    ctx.close()
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def bench_serving(args) -> int:
+    """16 resident scenes (50k-2M triangles, log-uniform; even ids random
+    soups, odd ids heightfields); requests of 1k-1M rays (log-uniform) arrive
+    as a Poisson stream, half nearest-hit eye rays and half any-hit AO rays,
+    each for a uniformly chosen scene.  Requests point into per-scene pinned
+    ray pools (1024x1024 eye rays, and one AO ray per eye hit made on the
+    device), so nothing is copied on the host inside the measurement."""
+    import torch
+
+    import heavyrt_amd as H
+    from heavyrt_amd import workloads as W
+
+    ctx = H.Context(0)
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    sptr = stream.cuda_stream
+    nt = W.log_uniform_int(16, 5e4, 2e6, W.STREAM_SERVE)
+    scenes, pools = [], []
+    log("building 16 resident scenes")
+    for k in range(16):
+        if k % 2 == 0:
+            v, f = W.soup(int(nt[k]), stream=W.STREAM_SERVE + 10 + k)
+        else:
+            nz = max(8, int(round(math.sqrt(nt[k] / 4.0))))
+            v, f, _ = W.heightfield(2 * nz, nz)
+        sc = H.Scene(ctx)
+        sc.add_instance(sc.add_triangles(v, f))
+        sc.commit()
+        lo, hi = v.min(0), v.max(0)
+        c = 0.5 * (lo + hi)
+        ext = float(np.max(hi - lo))
+        eye = c + np.array([0.35, 0.8, 1.6]) * ext
+        eye_rays = W.pinhole_rays(1024, 1024, eye=tuple(eye), target=tuple(c), fov_y_deg=50.0)
+        d_eye = torch.from_numpy(eye_rays).cuda()
+        d_hit = torch.empty((eye_rays.shape[0], 2), dtype=torch.int32, device="cuda")
+        d_ao = torch.empty((eye_rays.shape[0], 8), dtype=torch.float32, device="cuda")
+        sc.trace_nearest(d_eye, eye_rays.shape[0], d_hit, H.HIT_COMPACT, stream=sptr)
+        sc.rtao_make_rays(d_eye, d_hit, eye_rays.shape[0], 1, 0.1 * ext, SEED + k, d_ao, stream=sptr)
+        torch.cuda.synchronize()
+        n = eye_rays.shape[0]
+        pool = {"eye": H.pinned_array((n, 8), np.float32), "ao": H.pinned_array((n, 8), np.float32),
+                "eye_out": H.pinned_array((n, 2), np.uint32), "ao_out": H.pinned_array((n,), np.uint8)}
+        pool["eye"][:] = eye_rays
+        pool["ao"][:] = d_ao.cpu().numpy()
+        scenes.append(sc)
+        pools.append(pool)
+    tris = int(sum(int(x) for x in nt))
+    srv = H.Server(ctx, 4 << 30, steps_in_flight=3)
+    ids = [srv.add_scene(sc) for sc in scenes]
+    npool = 1024 * 1024
+    levels = []
+    for li, qps in enumerate(float(q) for q in args.qps.split(",")):
+        nreq = args.requests
+        sizes = W.log_uniform_int(nreq, 1e3, 1e6, W.STREAM_SERVE + 100 + li)
+        u = W.uniform01(W.pcg32(nreq, 4, W.STREAM_SERVE + 200 + li))
+        gaps = -np.log(np.maximum(u[:, 0], 1e-12)) / qps
+        arrivals = np.cumsum(gaps)
+        sc_id = np.minimum((u[:, 1] * 16).astype(np.int64), 15)
+        kind_any = u[:, 2] < 0.5
+        starts = (u[:, 3] * (npool - sizes)).astype(np.int64)
+        log(f"qps {qps:.0f}: {nreq} requests, {int(sizes.sum())} rays")
+        rids, t_sub = [], []
+        t0 = time.perf_counter()
+        for i in range(nreq):
+            while time.perf_counter() - t0 < arrivals[i]:
+                pass
+            p = pools[sc_id[i]]
+            a, n = int(starts[i]), int(sizes[i])
+            if kind_any[i]:
+                rid = srv.submit(ids[sc_id[i]], p["ao"][a:a + n], p["ao_out"][a:a + n], any_hit=True)
+            else:
+                rid = srv.submit(ids[sc_id[i]], p["eye"][a:a + n], p["eye_out"][a:a + n], any_hit=False)
+            t_sub.append(time.perf_counter() - t0)
+            rids.append(rid)
+        lat = np.array([srv.wait(r)[0] for r in rids])
+        t_end = max(ts + l * 1e-3 for ts, l in zip(t_sub, lat))
+        st = srv.stats()
+        levels.append({"qps_offered": qps, "qps_achieved": round(nreq / t_end, 1),
+                       "mrays_s": round(float(sizes.sum()) / t_end / 1e6, 1),
+                       "p50_ms": round(float(np.percentile(lat, 50)), 3),
+                       "p99_ms": round(float(np.percentile(lat, 99)), 3),
+                       "mean_requests_per_step": st["mean_requests_per_step"],
+                       "peak_bytes_in_flight": st["peak_bytes_in_flight"]})
+        log(f"  -> {levels[-1]}")
+    # parity: every pool entry written by some request equals a solo trace
+    mism = 0
+    for k, (sc, p) in enumerate(zip(scenes, pools)):
+        d = torch.from_numpy(np.asarray(p["eye"])).cuda()
+        o = torch.empty((npool, 2), dtype=torch.int32, device="cuda")
+        sc.trace_nearest(d, npool, o, H.HIT_COMPACT, stream=sptr)
+        da = torch.from_numpy(np.asarray(p["ao"])).cuda()
+        oa = torch.empty(npool, dtype=torch.uint8, device="cuda")
+        sc.trace_any(da, npool, oa, stream=sptr)
+        torch.cuda.synchronize()
+        ref = o.cpu().numpy().view(np.uint32)
+        got = np.asarray(p["eye_out"])
+        written = np.any(got != 0, axis=1)
+        mism += int(np.sum(np.any(got[written] != ref[written], axis=1)))
+        refa = oa.cpu().numpy()
+        gota = np.asarray(p["ao_out"])
+        mism += int(np.sum(gota[gota != 0] != refa[gota != 0]))
+    best = max(levels, key=lambda x: x["mrays_s"])
+    result = {
+        "metric": "Mrays/s closest/any-hit on 1 MI355X, % HBM roofline; p50/p99 query latency",
+        "value": best["mrays_s"],
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.requests,
+        "warmup": 0,
+        "ms_per_step": None,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": "serving: 16 resident scenes (50k-2M tris, soups and heightfields), Poisson "
+                               "requests of 1k-1M rays (log-uniform), 50/50 nearest-hit eye rays / any-hit AO "
+                               "rays, 4 GiB HBM budget, 3 steps in flight",
+                   "triangles": tris, "requests_per_rate": args.requests},
+        "latency_ms": {"p50": best["p50_ms"], "p99": best["p99_ms"]},
+        "rates": levels,
+        "parity_mismatches_vs_solo": mism,
+        "cpu_baseline": None,
+    }
+    srv.close()
+    for sc in scenes:
+        sc.close()
     ctx.close()
     print(json.dumps(result), flush=True)
     return 0

@@ -40,7 +40,10 @@ namespace {
 
 constexpr int kB = 256;
 constexpr uint32_t kBoxChunk = 4096;   // vertices per box-reduction workgroup
-constexpr uint32_t kRefitChunkB = 1024;
+#ifndef HRT_REFIT_CHUNK
+#define HRT_REFIT_CHUNK 4096
+#endif
+constexpr uint32_t kRefitChunkB = HRT_REFIT_CHUNK;  // leaves per refit workgroup
 
 inline uint32_t blocks(uint64_t n) { return uint32_t(std::max<uint64_t>(1, (n + kB - 1) / kB)); }
 

@@ -442,8 +442,10 @@ __global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* __restrict__ 
 // workgroup-scope acquire/release; only nodes whose range crosses a chunk
 // boundary (the few ancestors of chunk seams) pay for agent-scope ordering,
 // which on gfx950 writes back and invalidates the L2 of the issuing XCD.
-constexpr int kRefitPerThread = 4;
-constexpr uint32_t kRefitChunk = kBlock * kRefitPerThread;
+#ifndef HRT_REFIT_CHUNK
+#define HRT_REFIT_CHUNK 4096
+#endif
+constexpr uint32_t kRefitChunk = HRT_REFIT_CHUNK;  // leaves per workgroup
 
 template <class Src>
 __global__ __launch_bounds__(kBlock) void k_leaves_refit(

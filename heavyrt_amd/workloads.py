@@ -32,6 +32,7 @@ STREAM_SOUP = 4
 STREAM_RAYS = 5
 STREAM_TEST = 9
 STREAM_BATCH = 10        # batched build+trace: sizes, then soups and rays at 1000 + 2 * scene
+STREAM_SERVE = 20        # serving: scene sizes; soups at +10+k; request draws at +100/+200 per rate
 
 
 def pcg32(n: int, draws: int, stream: int, seed: int = PCG_SEED) -> np.ndarray:
