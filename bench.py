@@ -5,10 +5,11 @@ Workload (one step): a 1024x512-quad heightfield (value noise, amp 0.2) plus
 primary rays traced nearest-hit (BVH2 node pairs, the reference's visiting
 order, bit-exact with BVH2CommonRT); 16 cosine-weighted AO rays per primary
 hit made on the device (tMax = 0.1 x scene diagonal); all AO rays traced
-any-hit on the BVH4 collapse of the same tree with FP16 boxes rounded outward
-(BVH4HalfRT's node format; any-hit answers do not depend on the node layout,
-the tests check them against the oracle and the run checks them against the
-FP32-node and BVH2 layouts).  Scene and rays
+any-hit on FP16 BVH4 nodes (BVH4HalfRT's node format, boxes rounded outward)
+of a second LBVH over the same triangles built on a cubic Morton grid
+(any-hit answers do not depend on the tree; the tests check them against the
+oracle and every run checks them against the reference tree's BVH2, FP32 BVH4
+and FP16 BVH4).  Scene and rays
 are resident in HBM before the timed region starts.
 
 Prints ONE JSON line.  `value` is total rays (primary + AO) per second over
@@ -41,7 +42,7 @@ if ROOT not in sys.path:
 HBM_PEAK_GBPS = 8000.0
 SPP = 16
 SEED = 0x2545F4914F6CDD1D
-NODE_BYTES = {"bvh2": 64, "bvh4": 128, "bvh4_fp16": 64}
+NODE_BYTES = {"bvh2": 64, "bvh4": 128, "bvh4_fp16": 64, "bvh4_fp16_cubic": 64}
 # HBM bytes per any-hit launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
 # this workload, scaled by the same counters on a device copy of known size.
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "rtao_anyhit_traffic.json")
@@ -68,8 +69,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--any-layout", choices=["bvh2", "bvh4", "bvh4_fp16"], default="bvh4_fp16",
-                    help="node layout for the AO any-hit rays")
+    ap.add_argument("--any-layout", choices=["bvh2", "bvh4", "bvh4_fp16_cubic"], default="bvh4_fp16_cubic",
+                    help="tree for the AO any-hit rays: bvh2 (the reference tree's pairs), bvh4 (its FP32 "
+                         "BVH4), bvh4_fp16_cubic (FP16 BVH4 of the cubic-grid any-hit tree)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true", help="skip timing the other any-hit layouts")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, os.cpu_count())")
@@ -114,7 +116,11 @@ def main() -> int:
     if args.workload == "serving":
         return bench_serving(args)
     dev = torch.device("cuda:0")
-    layouts = {"bvh2": H.LAYOUT_BVH2, "bvh4": H.LAYOUT_BVH4, "bvh4_fp16": H.LAYOUT_BVH4_FP16}
+    layouts = {"bvh2": H.LAYOUT_BVH2, "bvh4": H.LAYOUT_BVH4, "bvh4_fp16": H.LAYOUT_BVH4_FP16,
+               "bvh4_fp16_cubic": H.LAYOUT_AUTO_FP16}
+    # the committed layout of a step: nearest hits on BVH2 in the reference
+    # order, any-hit on the chosen tree
+    step_layout = {"bvh2": H.LAYOUT_BVH2, "bvh4": H.LAYOUT_AUTO, "bvh4_fp16_cubic": H.LAYOUT_AUTO_FP16}
 
     log("generating scene")
     verts, faces = W.rtao_scene()
@@ -133,7 +139,11 @@ def main() -> int:
     t0 = time.perf_counter()
     scene.commit(layout=H.LAYOUT_BVH4)  # BVH4 collapse (FP32 + FP16 nodes)
     collapse_ms = (time.perf_counter() - t0) * 1e3
-    log(f"build {faces.shape[0]} tris: {build_ms:.2f} ms (host copy + LBVH), BVH4 collapse {collapse_ms:.2f} ms")
+    t0 = time.perf_counter()
+    scene.commit(layout=H.LAYOUT_AUTO_FP16)  # cubic-grid any-hit tree + its BVH4
+    any_tree_ms = (time.perf_counter() - t0) * 1e3
+    log(f"build {faces.shape[0]} tris: {build_ms:.2f} ms (host copy + LBVH), BVH4 collapse {collapse_ms:.2f} ms, "
+        f"any-hit tree {any_tree_ms:.2f} ms")
 
     # one explicit stream (default priority) for every copy, kernel and event
     stream = torch.cuda.Stream(device=dev)
@@ -146,9 +156,7 @@ def main() -> int:
     torch.cuda.synchronize()
 
     def set_any_layout(name: str) -> None:
-        # nearest hits always run on BVH2 in the reference's order; the AUTO
-        # layouts add BVH4 (FP32 or FP16 boxes) for any-hit queries
-        scene.commit(layout={"bvh2": H.LAYOUT_BVH2, "bvh4": H.LAYOUT_AUTO, "bvh4_fp16": H.LAYOUT_AUTO_FP16}[name])
+        scene.commit(layout=step_layout[name])
 
     def nearest():
         scene.trace_nearest(d_prim, npix, d_hits, H.HIT_COMPACT, stream=sptr)
@@ -190,7 +198,7 @@ def main() -> int:
 
     # traversal counters (counting variant, untimed)
     def counters(layout_name: str, any_hit: bool):
-        scene.commit(layout=layouts[layout_name])
+        scene.commit(layout=step_layout[layout_name])
         st = torch.zeros(((n_ao if any_hit else npix), 2), dtype=torch.int32, device=dev)
         if any_hit:
             chk = torch.zeros_like(d_vis)
@@ -213,7 +221,7 @@ def main() -> int:
     variants = {}
     if not args.no_variants:
         ref_vis = d_vis.clone()
-        for name in ("bvh2", "bvh4", "bvh4_fp16"):
+        for name in ("bvh2", "bvh4", "bvh4_fp16", "bvh4_fp16_cubic"):
             scene.commit(layout=layouts[name])
             out = torch.zeros_like(d_vis)
             for _ in range(2):
@@ -274,6 +282,7 @@ def main() -> int:
         "tris_per_ray": {"primary": round(float(sp[:, 1].mean()), 3), "ao": round(float(sa[:, 1].mean()), 3)},
         "build_ms": round(build_ms, 3),
         "bvh4_collapse_ms": round(collapse_ms, 3),
+        "any_hit_tree_build_ms": round(any_tree_ms, 3),
         "counting_variant_identical": same_p and same_a,
         "any_hit_layouts": variants,
         "cpu_baseline": None,

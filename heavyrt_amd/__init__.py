@@ -42,6 +42,7 @@ DUMP_SCENE_BOX = 8
 DUMP_QUADS32 = 9
 DUMP_QUADS16 = 10
 DUMP_PAIRS = 11
+DUMP_ANY_TREE = 0x100  # OR'ed into a dump kind: the any-hit tree of HRT_LAYOUT_AUTO_FP16
 # hrt_debug_dump geometry id that selects the top-level tree of an instanced scene
 TLAS_ID = 0xFFFFFFFF
 
@@ -293,8 +294,10 @@ class Scene:
 
     # -- parity hooks -------------------------------------------------------
     def dump(self, geom_id: int, what: int) -> np.ndarray:
+        flag = what & DUMP_ANY_TREE
+        what &= ~DUMP_ANY_TREE
         sizes = np.zeros(5, dtype=np.uint32)
-        _check(lib().hrt_debug_dump(self._h, geom_id, DUMP_SIZES, _ptr(sizes), sizes.nbytes))
+        _check(lib().hrt_debug_dump(self._h, geom_id, DUMP_SIZES | flag, _ptr(sizes), sizes.nbytes))
         if what == DUMP_SIZES:
             return sizes
         n, L, nq = int(sizes[0]), int(sizes[1]), int(sizes[4])
@@ -312,7 +315,7 @@ class Scene:
             out = np.zeros((2 * L - 1, 8), dtype=np.uint32)
         else:
             out = np.zeros(n, dtype=np.uint32)
-        _check(lib().hrt_debug_dump(self._h, geom_id, what, _ptr(out), out.nbytes))
+        _check(lib().hrt_debug_dump(self._h, geom_id, what | flag, _ptr(out), out.nbytes))
         return out
 
 

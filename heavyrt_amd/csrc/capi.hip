@@ -227,6 +227,8 @@ hrt_status trace_common(hrt_scene* s, const float* rays, size_t n, void* out, in
       e = hrt::trace_two_level(s->tlas, s->inst_dev ? s->inst_dev + s->instances.size() : nullptr, a, hs);
     else if (lay == HRT_LAYOUT_BVH2 || ((lay == HRT_LAYOUT_AUTO || lay == HRT_LAYOUT_AUTO_FP16) && !anyhit))
       e = hrt::trace_blas(*b, a, hs);
+    else if (lay == HRT_LAYOUT_AUTO_FP16 && b->any_tree)
+      e = hrt::trace_blas4(*b->any_tree, a, true, hs);  // any-hit on the cubic-grid tree
     else
       e = hrt::trace_blas4(*b, a, lay == HRT_LAYOUT_BVH4_FP16 || lay == HRT_LAYOUT_AUTO_FP16, hs);
   }
@@ -410,7 +412,15 @@ hrt_status hrt_commit(hrt_scene* s, int quality, int layout) {
           if (!(b.box[k] >= -65504.0f && b.box[4 + k] <= 65504.0f))
             return fail(HRT_ERR_UNSUPPORTED, "FP16 nodes need every coordinate within +-65504");
       }
-      if ((e = hrt::blas_build_bvh4(b, s->ctx->stream)) != hipSuccess) return fail_hip(e, "BVH4 collapse");
+      if (layout == HRT_LAYOUT_AUTO_FP16) {
+        // any-hit tree on a cubic Morton grid; the reference tree's BVH4 only
+        // when the cubic one cannot be stored
+        if ((e = hrt::blas_build_any_tree(b, s->ctx->stream)) != hipSuccess) return fail_hip(e, "any-hit tree");
+        if (!b.any_tree && (e = hrt::blas_build_bvh4(b, s->ctx->stream)) != hipSuccess)
+          return fail_hip(e, "BVH4 collapse");
+      } else if ((e = hrt::blas_build_bvh4(b, s->ctx->stream)) != hipSuccess) {
+        return fail_hip(e, "BVH4 collapse");
+      }
     }
   }
   s->layout = layout;
@@ -532,7 +542,11 @@ hrt_status hrt_debug_dump(hrt_scene* s, uint32_t geomId, int what, void* host_ou
   } else if (geomId >= s->geoms.size()) {
     return fail(HRT_ERR_INVALID_ARG, "no such geometry");
   }
-  const hrt::Blas& b = (geomId == HRT_TLAS_ID) ? s->tlas : s->geoms[geomId];
+  const bool any_tree = (what & HRT_DUMP_ANY_TREE) != 0;
+  what &= ~HRT_DUMP_ANY_TREE;
+  if (any_tree && (geomId == HRT_TLAS_ID || !s->geoms[geomId].any_tree))
+    return fail(HRT_ERR_INVALID_ARG, "no any-hit tree (commit with HRT_LAYOUT_AUTO_FP16)");
+  const hrt::Blas& b = (geomId == HRT_TLAS_ID) ? s->tlas : (any_tree ? *s->geoms[geomId].any_tree : s->geoms[geomId]);
   const uint32_t N = b.ntris, L = b.leaves;
   const uint32_t npairs = std::max<uint32_t>(L > 1 ? L - 1 : 1, 1);
   auto need = [&](size_t n) -> hrt_status {

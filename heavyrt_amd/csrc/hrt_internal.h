@@ -54,6 +54,9 @@ struct Blas {
   uint32_t* ucodes = nullptr;      // leaves
   uint32_t* karras_lr = nullptr;   // 2 * (leaves - 1): children in the reference's left/right numbering
   bool keep_debug = false;
+  // Second tree over the same triangles for any-hit queries, built on a cubic
+  // Morton grid (made on demand by blas_build_any_tree; owned).
+  Blas* any_tree = nullptr;
 };
 
 // Scratch for one build, sized for up to `cap` triangles; reused across builds.
@@ -68,7 +71,11 @@ void blas_free(Blas& b);
 // Builds the LBVH of b.verts / b.idx (already on the device) on `stream`;
 // synchronises the stream before returning (leaves/max_run/box are host
 // values afterwards).
-hipError_t blas_build(Blas& b, hipStream_t stream);
+hipError_t blas_build(Blas& b, hipStream_t stream, bool cubic = false);
+// Builds b.any_tree (cubic grid, with its BVH4 nodes) unless it exists.
+// Leaves b.any_tree null when some Morton cell of the cubic grid holds more
+// than 127 triangles.  Synchronises the stream.
+hipError_t blas_build_any_tree(Blas& b, hipStream_t stream);
 // Collapses the built BVH2 into FP32 and FP16 BVH4 nodes (idempotent);
 // synchronises the stream.
 hipError_t blas_build_bvh4(Blas& b, hipStream_t stream);

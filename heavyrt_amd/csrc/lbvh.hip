@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <new>
 
 #include "hrt_internal.h"
 
@@ -130,8 +131,14 @@ __global__ __launch_bounds__(kBlock) void k_box_partial(Src src, float4* __restr
 
 // Reduces the partial boxes and derives the Morton scale
 // (ZIndexMultCoeff(bboxMin - 1e-4, bboxMax + 1e-4), lbvh.cpp:41-51, 263).
+// cubic = 1: the same 1024-cell scale on every axis, set by the largest
+// extent (cells are cubes).  The reference scales each axis to its own
+// extent; on flat scenes that spends as many code bits on the thin axis as
+// on the wide ones.  Used only for the any-hit tree, whose answers do not
+// depend on the tree.
 __global__ __launch_bounds__(kBlock) void k_box_final(const float4* __restrict__ part, int nparts,
-                                                      float* __restrict__ box8, float* __restrict__ zscale4) {
+                                                      float* __restrict__ box8, float* __restrict__ zscale4,
+                                                      int cubic) {
   __shared__ float4 smin[kBlock];
   __shared__ float4 smax[kBlock];
   float4 lo = make_float4(kLbvhInf, kLbvhInf, kLbvhInf, kLbvhInf);
@@ -160,6 +167,13 @@ __global__ __launch_bounds__(kBlock) void k_box_final(const float4* __restrict__
     box8[4] = h.x; box8[5] = h.y; box8[6] = h.z; box8[7] = h.w;
     const float ex[4] = {(h.x + kLbvhEpsBias) - (l.x - kLbvhEpsBias), (h.y + kLbvhEpsBias) - (l.y - kLbvhEpsBias),
                          (h.z + kLbvhEpsBias) - (l.z - kLbvhEpsBias), (h.w + kLbvhEpsBias) - (l.w - kLbvhEpsBias)};
+    if (cubic) {
+      const float m = fmaxf(ex[0], fmaxf(ex[1], ex[2]));
+      const float inv = (fabsf(m) < 1e-20f) ? 0.0f : 1.0f / m;
+      zscale4[0] = zscale4[1] = zscale4[2] = 1024.0f * inv;
+      zscale4[3] = 1024.0f * (1.0f / ex[3]);
+      return;
+    }
     for (int k = 0; k < 4; k++) {
       float inv = 1.0f / ex[k];
       if (k < 3 && fabsf(ex[k]) < 1e-20f) inv = 0.0f;
@@ -629,6 +643,11 @@ hipError_t blas_alloc(Blas& b, uint32_t nverts, uint32_t ntris, bool keep_debug)
 }
 
 void blas_free(Blas& b) {
+  if (b.any_tree) {
+    blas_free(*b.any_tree);
+    delete b.any_tree;
+    b.any_tree = nullptr;
+  }
   void* ptrs[] = {b.verts,      b.idx,        b.pairs,      b.tris,      b.perm,   b.parent,
                   b.quads32,    b.quads16,    b.codes_in,   b.codes_sorted, b.ids_sorted,
                   b.run_start,  b.ucodes,     b.karras_lr};
@@ -651,7 +670,7 @@ inline uint32_t fill_blocks(uint64_t n) { return uint32_t(std::min<uint64_t>(102
 // the leaf-order pass; the stream is synchronised before returning and the
 // leaf count, longest run and scene box are host values afterwards.
 template <class Src, class Finish>
-hipError_t lbvh_build(Blas& b, const Src& src, uint32_t box_items, hipStream_t s, Finish&& finish) {
+hipError_t lbvh_build(Blas& b, const Src& src, uint32_t box_items, hipStream_t s, Finish&& finish, int cubic = 0) {
   const uint64_t N = b.ntris;
   if (N == 0) return hipErrorInvalidValue;
   const uint32_t ntiles = uint32_t((N + kSortTile - 1) / kSortTile);
@@ -679,7 +698,7 @@ hipError_t lbvh_build(Blas& b, const Src& src, uint32_t box_items, hipStream_t s
   hipError_t err = hipSuccess;
   do {
     k_box_partial<Src><<<box_parts, kBlock, 0, s>>>(src, part);
-    k_box_final<<<1, kBlock, 0, s>>>(part, box_parts, box, box + 8);
+    k_box_final<<<1, kBlock, 0, s>>>(part, box_parts, box, box + 8, cubic);
     k_morton<Src><<<blocks_for(N), kBlock, 0, s>>>(src, uint32_t(N), box, box + 8, ka, va);
     if ((err = hipGetLastError()) != hipSuccess) break;
     if (b.keep_debug)
@@ -740,12 +759,35 @@ __global__ __launch_bounds__(kBlock) void k_inst_order(const InstDev* __restrict
 
 }  // namespace
 
-hipError_t blas_build(Blas& b, hipStream_t s) {
+hipError_t blas_build(Blas& b, hipStream_t s, bool cubic) {
   const TriSource src{b.verts, b.idx, b.nverts};
   const uint32_t N = b.ntris;
-  return lbvh_build(b, src, b.nverts, s, [&](const uint32_t* ids_sorted) {
-    k_tris<<<blocks_for(N), kBlock, 0, s>>>(b.verts, b.idx, ids_sorted, N, b.tris, b.perm);
-  });
+  return lbvh_build(
+      b, src, b.nverts, s,
+      [&](const uint32_t* ids_sorted) {
+        k_tris<<<blocks_for(N), kBlock, 0, s>>>(b.verts, b.idx, ids_sorted, N, b.tris, b.perm);
+      },
+      cubic ? 1 : 0);
+}
+
+hipError_t blas_build_any_tree(Blas& b, hipStream_t s) {
+  if (b.any_tree) return hipSuccess;
+  Blas* a = new (std::nothrow) Blas();
+  if (!a) return hipErrorOutOfMemory;
+  hipError_t e = blas_alloc(*a, b.nverts, b.ntris, b.keep_debug);
+  if (e == hipSuccess) e = hipMemcpyAsync(a->verts, b.verts, 16ull * b.nverts, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = hipMemcpyAsync(a->idx, b.idx, 12ull * b.ntris, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess) e = blas_build(*a, s, true);
+  // a run longer than the 7-bit leaf count cannot be stored: keep the
+  // reference tree for any-hit queries then
+  if (e == hipSuccess && a->max_run <= 127) e = blas_build_bvh4(*a, s);
+  if (e != hipSuccess || a->max_run > 127) {
+    blas_free(*a);
+    delete a;
+    return e;
+  }
+  b.any_tree = a;
+  return hipSuccess;
 }
 
 hipError_t tlas_build(Blas& t, const InstDev* by_id, InstDev* leaf_order, hipStream_t s) {

@@ -75,8 +75,12 @@ enum { HRT_HIT_FULL = 0, HRT_HIT_COMPACT = 1 };
  *   BVH4_FP16  as BVH4 with FP16 boxes rounded outward (cbvh_fat.cpp:447-497);
  *              requires every coordinate within +-65504.
  *   AUTO       nearest hits on BVH2 (reference order), any-hit on BVH4.
- *   AUTO_FP16  nearest hits on BVH2 (reference order), any-hit on BVH4_FP16
- *              (half the node bytes per visit; the same any-hit answers). */
+ *   AUTO_FP16  nearest hits on BVH2 (reference order); any-hit on FP16 BVH4
+ *              nodes of a second tree built over a cubic Morton grid (every
+ *              axis scaled by the largest extent), which visits fewer nodes
+ *              on flat scenes.  Any-hit answers do not depend on the tree.
+ *              If a cubic-grid cell holds more than 127 triangles, any-hit
+ *              uses the reference tree's FP16 BVH4 instead. */
 enum {
   HRT_LAYOUT_BVH2 = 0,
   HRT_LAYOUT_BVH4 = 1,
@@ -101,7 +105,8 @@ enum {
   HRT_DUMP_SCENE_BOX = 8,    /* float[12]: box min xyzw, box max xyzw, Morton scale xyzw */
   HRT_DUMP_QUADS32 = 9,      /* uint32[nquads*32]: BVH4 nodes, FP32 boxes (after a BVH4 commit) */
   HRT_DUMP_QUADS16 = 10,     /* uint32[nquads*16]: BVH4 nodes, FP16 boxes */
-  HRT_DUMP_PAIRS = 11        /* uint32[pairs*16]: the BVH2 node pairs as traversed */
+  HRT_DUMP_PAIRS = 11,       /* uint32[pairs*16]: the BVH2 node pairs as traversed */
+  HRT_DUMP_ANY_TREE = 0x100  /* OR'ed into `what`: the same array of the any-hit tree (HRT_LAYOUT_AUTO_FP16) */
 };
 
 /* Context: one device, one HIP stream.  No counterpart in the reference

@@ -114,6 +114,9 @@ struct Build {
   // box mode (a top-level tree over instance boxes, BuildFromBoxes,
   // lbvh.cpp:829-917): 2 entries per primitive, min and max; verts/idx unused
   std::vector<f4> boxes;
+  // cubic Morton grid: one scale for x, y and z, set by the largest extent
+  // (the engine's any-hit tree); false = the reference's per-axis scale
+  bool cubic = false;
 
   // builder state, named after lbvh.h:120-145
   f4 bbmin, bbmax, zscale;
@@ -408,6 +411,11 @@ static void build_all(Build& b) {
     for (int k = 0; k < 4; k++) inv[k] = 1.0f / ex[k];
     for (int k = 0; k < 3; k++)
       if (std::fabs(ex[k]) < 1e-20f) inv[k] = 0.0f;
+    if (b.cubic) {
+      const float m = mx(ex[0], mx(ex[1], ex[2]));
+      const float im = (std::fabs(m) < 1e-20f) ? 0.0f : 1.0f / m;
+      inv[0] = inv[1] = inv[2] = im;
+    }
     b.zscale = {1024.0f * inv[0], 1024.0f * inv[1], 1024.0f * inv[2], 1024.0f * inv[3]};
   }
   // Morton codes, lbvh.cpp:259-287
@@ -1245,9 +1253,10 @@ std::vector<Quad>& quads_for(const Build* cb, int fp16) {
 
 extern "C" {
 
-void* hrto_build(const float* vert4, uint32_t nVert, const uint32_t* idx, uint32_t nIdx) {
+void* hrto_build_ex(const float* vert4, uint32_t nVert, const uint32_t* idx, uint32_t nIdx, int cubic) {
   if (nIdx < 3) return nullptr;
   Build* b = new Build();
+  b->cubic = cubic != 0;
   b->verts.resize(nVert);
   std::memcpy(b->verts.data(), vert4, sizeof(f4) * nVert);
   for (auto& v : b->verts) v.w = 1.0f;  // BVH2CommonRT_host.cpp:81 stores w = 1
@@ -1255,6 +1264,10 @@ void* hrto_build(const float* vert4, uint32_t nVert, const uint32_t* idx, uint32
   b->idx.assign(idx, idx + 3 * size_t(b->ntris));
   build_all(*b);
   return b;
+}
+
+void* hrto_build(const float* vert4, uint32_t nVert, const uint32_t* idx, uint32_t nIdx) {
+  return hrto_build_ex(vert4, nVert, idx, nIdx, 0);
 }
 
 void hrto_free(void* h) { delete static_cast<Build*>(h); }

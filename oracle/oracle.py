@@ -59,6 +59,8 @@ def lib():
         vp, u32, u64 = C.c_void_p, C.c_uint32, C.c_uint64
         L.hrto_build.restype = vp
         L.hrto_build.argtypes = [vp, u32, vp, u32]
+        L.hrto_build_ex.restype = vp
+        L.hrto_build_ex.argtypes = [vp, u32, vp, u32, C.c_int]
         L.hrto_free.argtypes = [vp]
         L.hrto_sizes.argtypes = [vp, vp]
         L.hrto_scene_box.argtypes = [vp, vp]
@@ -114,12 +116,13 @@ def as_vert4(verts: np.ndarray) -> np.ndarray:
 class OracleBVH:
     """One LBVH build of one triangle mesh, with the reference's traversals."""
 
-    def __init__(self, verts: np.ndarray, idx: np.ndarray):
+    def __init__(self, verts: np.ndarray, idx: np.ndarray, cubic: bool = False):
+        """cubic=True: the engine's any-hit tree (one Morton scale for all axes)."""
         self.v4 = as_vert4(verts)
         self.idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1)
         L = lib()
         self._owned = True
-        self._h = L.hrto_build(_ptr(self.v4), self.v4.shape[0], _ptr(self.idx), self.idx.size)
+        self._h = L.hrto_build_ex(_ptr(self.v4), self.v4.shape[0], _ptr(self.idx), self.idx.size, 1 if cubic else 0)
         if not self._h:
             raise ValueError("oracle build needs at least one triangle")
         self._read_sizes()

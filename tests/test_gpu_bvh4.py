@@ -14,7 +14,7 @@ from conftest import vsgf_path
 from gpu_util import say, trace
 from heavyrt_amd import workloads as W
 from oracle import oracle as O
-from test_gpu_parity import rays_for
+from test_gpu_parity import check_build, rays_for
 
 pytestmark = pytest.mark.gpu
 
@@ -120,11 +120,46 @@ def test_bvh4_any_hit(gpu_ctx, name, gen, layout):
     np.testing.assert_array_equal(g["hit"], p2 != INVALID)
     if layout != H.LAYOUT_AUTO:
         mode = O.MODE4_FP16 if layout in (H.LAYOUT_BVH4_FP16, H.LAYOUT_AUTO_FP16) else O.MODE4_FP32
-        _, p4, _, _, nq, nt = ob.trace4(anyr, mode, counters=True)
+        ob4 = ob
+        if layout == H.LAYOUT_AUTO_FP16 and has_any_tree(s):
+            ob4 = O.OracleBVH(v, f, cubic=True)  # AUTO_FP16 answers any-hit on the cubic-grid tree
+        _, p4, _, _, nq, nt = ob4.trace4(anyr, mode, counters=True)
         gs = trace(s, rays, "any", stats=True)
         np.testing.assert_array_equal(gs["hit"], p4 != INVALID)
         np.testing.assert_array_equal(gs["pairs"], nq)
         np.testing.assert_array_equal(gs["tris"], nt)
+    s.close()
+
+
+@pytest.mark.parametrize("layout", [H.LAYOUT_AUTO, H.LAYOUT_AUTO_FP16], ids=["auto", "auto_fp16"])
+def has_any_tree(scene) -> bool:
+    try:
+        scene.dump(0, H.DUMP_SIZES | H.DUMP_ANY_TREE)
+        return True
+    except H.HrtError:
+        return False
+
+
+@pytest.mark.parametrize("name,gen", CASES, ids=[c[0] for c in CASES])
+def test_any_tree_bit_exact(gpu_ctx, name, gen):
+    """HRT_LAYOUT_AUTO_FP16 builds a second tree on a cubic Morton grid for
+    any-hit queries: its build arrays and FP16 BVH4 nodes equal the oracle's
+    cubic-grid build bit for bit."""
+    v, f = gen()
+    s = H.Scene(gpu_ctx, keep_build_arrays=True)
+    s.add_instance(s.add_triangles(v, f))
+    s.commit(layout=H.LAYOUT_AUTO_FP16)
+    ob = O.OracleBVH(v, f, cubic=True)
+    if ob.max_run > 127:
+        assert not has_any_tree(s)
+        s.close()
+        return
+    check_build(s, ob, flag=H.DUMP_ANY_TREE)
+    boxes16, refs16 = gpu_quads16_as_floats(s.dump(0, H.DUMP_QUADS16 | H.DUMP_ANY_TREE))
+    oq16 = ob.quads(True)
+    np.testing.assert_array_equal(refs16, oq16[:, :, 3])
+    want = np.concatenate([oq16[:, :, 0:3], oq16[:, :, 4:7]], axis=2).view(np.float32)
+    np.testing.assert_array_equal(boxes16.view(np.uint32), want.view(np.uint32))
     s.close()
 
 

@@ -45,25 +45,25 @@ def rays_for(v, n, stream):
     return rays
 
 
-def check_build(scene, ob: O.OracleBVH, geom: int = 0):
-    sizes = scene.dump(geom, H.DUMP_SIZES)
+def check_build(scene, ob: O.OracleBVH, geom: int = 0, flag: int = 0):
+    sizes = scene.dump(geom, flag | H.DUMP_SIZES)
     assert int(sizes[0]) == ob.ntris
     assert int(sizes[1]) == ob.leaves
     assert int(sizes[2]) == ob.max_run
     bmin, bmax, zs = ob.scene_box()
-    box = scene.dump(geom, H.DUMP_SCENE_BOX)
+    box = scene.dump(geom, flag | H.DUMP_SCENE_BOX)
     np.testing.assert_array_equal(box[0:4].view(np.uint32), bmin.view(np.uint32))
     np.testing.assert_array_equal(box[4:8].view(np.uint32), bmax.view(np.uint32))
     np.testing.assert_array_equal(box[8:12].view(np.uint32), zs.view(np.uint32))
     c_in, c_sorted, ids = ob.codes()
-    np.testing.assert_array_equal(scene.dump(geom, H.DUMP_CODES_INPUT), c_in)
-    np.testing.assert_array_equal(scene.dump(geom, H.DUMP_CODES_SORTED), c_sorted)
-    np.testing.assert_array_equal(scene.dump(geom, H.DUMP_IDS_SORTED), ids)
+    np.testing.assert_array_equal(scene.dump(geom, flag | H.DUMP_CODES_INPUT), c_in)
+    np.testing.assert_array_equal(scene.dump(geom, flag | H.DUMP_CODES_SORTED), c_sorted)
+    np.testing.assert_array_equal(scene.dump(geom, flag | H.DUMP_IDS_SORTED), ids)
     rs, uc = ob.runs()
-    np.testing.assert_array_equal(scene.dump(geom, H.DUMP_RUN_START), rs)
-    np.testing.assert_array_equal(scene.dump(geom, H.DUMP_UNIQUE_CODES), uc)
-    np.testing.assert_array_equal(scene.dump(geom, H.DUMP_PERM), ob.perm())
-    lr = scene.dump(geom, H.DUMP_LR_NODES)
+    np.testing.assert_array_equal(scene.dump(geom, flag | H.DUMP_RUN_START), rs)
+    np.testing.assert_array_equal(scene.dump(geom, flag | H.DUMP_UNIQUE_CODES), uc)
+    np.testing.assert_array_equal(scene.dump(geom, flag | H.DUMP_PERM), ob.perm())
+    lr = scene.dump(geom, flag | H.DUMP_LR_NODES)
     np.testing.assert_array_equal(lr, ob.lr_nodes())
 
 
