@@ -75,9 +75,10 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true", help="skip timing the other any-hit layouts")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, os.cpu_count())")
-    ap.add_argument("--workload", choices=["rtao", "batched", "serving"], default="rtao",
-                    help="rtao (default): the headline RTAO step; batched: 256 scenes built and traced "
-                         "in one graph launch; serving: Poisson request stream over 16 resident scenes")
+    ap.add_argument("--workload", choices=["rtao", "eye", "batched", "precision", "serving"], default="rtao",
+                    help="rtao (default): the headline RTAO step; eye: 512x512 eye rays on the icosphere "
+                         "scene; batched: 256 scenes built and traced in one graph launch; precision: 4M-tri "
+                         "soup, FP16 vs FP32 BVH4 nodes; serving: Poisson request stream over 16 resident scenes")
     ap.add_argument("--qps", type=str, default="1000,4000,12000",
                     help="serving: comma-separated request rates to run")
     ap.add_argument("--requests", type=int, default=1500, help="serving: requests per rate")
@@ -113,6 +114,10 @@ def main() -> int:
     torch.cuda.init()
     if args.workload == "batched":
         return bench_batched(args)
+    if args.workload == "eye":
+        return bench_eye(args)
+    if args.workload == "precision":
+        return bench_precision(args)
     if args.workload == "serving":
         return bench_serving(args)
     dev = torch.device("cuda:0")
@@ -453,6 +458,174 @@ def bench_batched(args) -> int:
                                   "sample": f"first 24 of the 256 scenes: oracle LBVH build + nearest-hit trace of "
                                             f"{rs} rays, {cpu_s:.2f} s"}
     b.close()
+    ctx.close()
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def _timed(stream, fn, reps):
+    import torch
+
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(reps + 1)]
+    torch.cuda.synchronize()
+    for k in range(reps):
+        ev[k].record(stream)
+        fn()
+    ev[reps].record(stream)
+    torch.cuda.synchronize()
+    return np.array([ev[k].elapsed_time(ev[k + 1]) for k in range(reps)])
+
+
+def _line(value, unit, ms, steps, warmup, workload, extra):
+    out = {"metric": "Mrays/s closest/any-hit on 1 MI355X, % HBM roofline; p50/p99 query latency",
+           "value": value, "unit": unit, "n_gpus": 1, "steps": steps, "warmup": warmup,
+           "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+           "dtype": "f32", "data": "synthetic", "config": {"workload": workload}}
+    out.update(extra)
+    return out
+
+
+def bench_eye(args) -> int:
+    """The CPU-runnable eye-ray configuration: 512x512 pinhole rays, nearest
+    hit on the LBVH2 of an icosphere (6 subdivisions, 81,920 tris) on a
+    2-triangle ground quad; primId and t per pixel, bit-exact with the oracle."""
+    import torch
+
+    import heavyrt_amd as H
+    from heavyrt_amd import workloads as W
+
+    v, f = W.eye_scene()
+    rays = W.eye_rays()
+    n = rays.shape[0]
+    ctx = H.Context(0)
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    sptr = stream.cuda_stream
+    sc = H.Scene(ctx)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    sc.add_instance(sc.add_triangles(v, f))
+    sc.commit()
+    build_ms = (time.perf_counter() - t0) * 1e3
+    d_rays = torch.from_numpy(rays).cuda()
+    d_hits = torch.empty((n, 2), dtype=torch.int32, device="cuda")
+    d_st = torch.empty((n, 2), dtype=torch.int32, device="cuda")
+    for _ in range(args.warmup):
+        sc.trace_nearest(d_rays, n, d_hits, H.HIT_COMPACT, stream=sptr)
+    ms = _timed(stream, lambda: sc.trace_nearest(d_rays, n, d_hits, H.HIT_COMPACT, stream=sptr), args.steps)
+    hits = d_hits.cpu().numpy().view(np.uint32)
+    sc.trace_nearest(d_rays, n, torch.empty_like(d_hits), H.HIT_COMPACT, stream=sptr, stats=d_st)
+    torch.cuda.synchronize()
+    st = d_st.cpu().numpy().view(np.uint32).astype(np.int64)
+    nbytes = float(np.sum(40 + 64 * st[:, 0] + 60 * st[:, 1]))
+    mean = float(ms.mean())
+    result = _line(round(n / (mean * 1e-3) / 1e6, 2), "Mrays/s", round(mean, 4), args.steps, args.warmup,
+                   "eye: 512x512 pinhole rays, nearest hit, LBVH2 of an icosphere (81,920 tris) + ground quad",
+                   {"roofline": {"bound": "hbm", "kernel": "nearest-hit traversal (BVH2, reference order)",
+                                 "achieved": round(nbytes / (mean * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBPS,
+                                 "unit": "GB/s", "frac": round(nbytes / (mean * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+                                 "traffic": None, "bytes_per_launch": nbytes},
+                    "latency_ms": {"p50": round(float(np.percentile(ms, 50)), 4),
+                                   "p99": round(float(np.percentile(ms, 99)), 4)},
+                    "build_ms": round(build_ms, 3), "triangles": int(f.shape[0]), "rays": n,
+                    "visits_per_ray": round(float(st[:, 0].mean()), 3), "cpu_baseline": None})
+    from oracle import oracle as O
+
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    ob = O.OracleBVH(v, f)
+    t0 = time.perf_counter()
+    t, p, _, _ = ob.trace(rays, threads=threads)
+    cpu_s = time.perf_counter() - t0
+    result["cpu_baseline"] = {"value": round(n / cpu_s / 1e6, 3), "unit": "Mrays/s", "cores": threads,
+                              "kind": "port", "sample": f"the full 512x512 frame, {cpu_s * 1e3:.1f} ms"}
+    result["parity_mismatches_vs_oracle"] = int(np.sum(t.view(np.uint32) != hits[:, 0]) + np.sum(p != hits[:, 1]))
+    sc.close()
+    ctx.close()
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+def _t_fp64(v, f, rays, prim):
+    """Float64 Moller-Trumbore t of each ray against its hit triangle (NaN on a miss)."""
+    out = np.full(rays.shape[0], np.nan)
+    hit = prim != 0xFFFFFFFF
+    tri = f[prim[hit].astype(np.int64)].astype(np.int64)
+    A, B, C = (v[tri[:, k]].astype(np.float64) for k in range(3))
+    o = rays[hit, 0:3].astype(np.float64)
+    d = rays[hit, 4:7].astype(np.float64)
+    e1, e2 = B - A, C - A
+    pv = np.cross(d, e2)
+    tv = o - A
+    qv = np.cross(tv, e1)
+    out[hit] = np.sum(e2 * qv, axis=1) / np.sum(e1 * pv, axis=1)
+    return out
+
+
+def bench_precision(args) -> int:
+    """4M-triangle random soup, 4M incoherent rays, nearest hit on BVH4 with
+    FP32 boxes and with FP16 boxes rounded outward: hits must be identical;
+    t is compared with a float64 Moller-Trumbore on the hit triangle."""
+    import torch
+
+    import heavyrt_amd as H
+    from heavyrt_amd import workloads as W
+
+    n = 4 * 1024 * 1024
+    log("generating the 4M-triangle soup and 4M rays")
+    v, f = W.soup(n, stream=W.STREAM_SOUP)
+    rays = W.incoherent_rays(n, stream=W.STREAM_RAYS)
+    ctx = H.Context(0)
+    stream = torch.cuda.Stream()
+    torch.cuda.set_stream(stream)
+    sptr = stream.cuda_stream
+    sc = H.Scene(ctx)
+    sc.add_instance(sc.add_triangles(v, f))
+    d_rays = torch.from_numpy(rays).cuda()
+    res, timing = {}, {}
+    for name, lay in (("fp32", H.LAYOUT_BVH4), ("fp16", H.LAYOUT_BVH4_FP16)):
+        sc.commit(layout=lay)
+        out = torch.empty((n, 2), dtype=torch.int32, device="cuda")
+        for _ in range(args.warmup):
+            sc.trace_nearest(d_rays, n, out, H.HIT_COMPACT, stream=sptr)
+        timing[name] = _timed(stream, lambda: sc.trace_nearest(d_rays, n, out, H.HIT_COMPACT, stream=sptr), args.steps)
+        res[name] = out.cpu().numpy().view(np.uint32).copy()
+        log(f"{name} nodes: {timing[name].mean():.3f} ms")
+    same = bool(np.array_equal(res["fp32"], res["fp16"]))
+    mism = int(np.sum(np.any(res["fp32"] != res["fp16"], axis=1)))
+    prim = res["fp16"][:, 1]
+    t = res["fp16"][:, 0].view(np.float32).astype(np.float64)
+    t64 = _t_fp64(v, f, rays, prim)
+    hit = prim != 0xFFFFFFFF
+    rel = np.abs(t[hit] - t64[hit]) / np.abs(t64[hit])
+    ms16 = float(timing["fp16"].mean())
+    result = _line(round(n / (ms16 * 1e-3) / 1e6, 2), "Mrays/s", round(ms16, 4), args.steps, args.warmup,
+                   "precision: 4,194,304-tri random soup, 4,194,304 incoherent nearest-hit rays, BVH4 with FP16 "
+                   "boxes rounded outward (value) vs FP32 boxes",
+                   {"fp32_nodes": {"ms": round(float(timing["fp32"].mean()), 4),
+                                   "mrays_s": round(n / (timing["fp32"].mean() * 1e-3) / 1e6, 2)},
+                    "fp16_nodes": {"ms": round(ms16, 4), "mrays_s": round(n / (ms16 * 1e-3) / 1e6, 2)},
+                    "hits_bit_identical": same, "mismatching_rays": mism, "hit_fraction": round(float(hit.mean()), 4),
+                    "t_rel_error_vs_fp64": {"max": float(rel.max()), "mean": float(rel.mean()),
+                                            "median": float(np.median(rel)), "p99": float(np.percentile(rel, 99))},
+                    "latency_ms": {"p50": round(float(np.percentile(timing["fp16"], 50)), 4),
+                                   "p99": round(float(np.percentile(timing["fp16"], 99)), 4)},
+                    "cpu_baseline": None})
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        log("cpu baseline: oracle build of the 4M-tri soup, FP16 BVH4 trace of a 262,144-ray sample")
+        ob = O.OracleBVH(v, f)
+        ns = 262144
+        t0 = time.perf_counter()
+        tc, pc, _, _ = ob.trace4(rays[:ns], O.MODE4_FP16, threads=threads)
+        cpu_s = time.perf_counter() - t0
+        result["cpu_baseline"] = {"value": round(ns / cpu_s / 1e6, 3), "unit": "Mrays/s", "cores": threads,
+                                  "kind": "port", "sample": f"first {ns} rays, FP16 BVH4 traversal (BVH4HalfRT "
+                                                            f"restatement), {cpu_s:.2f} s"}
+        result["parity_mismatches_vs_oracle"] = int(np.sum(tc.view(np.uint32) != res["fp16"][:ns, 0]) +
+                                                    np.sum(pc != res["fp16"][:ns, 1]))
+    sc.close()
     ctx.close()
     print(json.dumps(result), flush=True)
     return 0
