@@ -590,6 +590,14 @@ def bench_precision(args) -> int:
         timing[name] = _timed(stream, lambda: sc.trace_nearest(d_rays, n, out, H.HIT_COMPACT, stream=sptr), args.steps)
         res[name] = out.cpu().numpy().view(np.uint32).copy()
         log(f"{name} nodes: {timing[name].mean():.3f} ms")
+    # counting variant on the FP16 nodes: algorithmic bytes of the timed kernel
+    st = torch.empty((n, 2), dtype=torch.int32, device="cuda")
+    sc.trace_nearest(d_rays, n, torch.empty((n, 2), dtype=torch.int32, device="cuda"), H.HIT_COMPACT, stream=sptr,
+                     stats=st)
+    torch.cuda.synchronize()
+    stc = st.cpu().numpy().view(np.uint32).astype(np.int64)
+    nbytes = float(np.sum(40 + 64 * stc[:, 0] + 60 * stc[:, 1]))
+    del st
     same = bool(np.array_equal(res["fp32"], res["fp16"]))
     mism = int(np.sum(np.any(res["fp32"] != res["fp16"], axis=1)))
     prim = res["fp16"][:, 1]
@@ -604,6 +612,12 @@ def bench_precision(args) -> int:
                    {"fp32_nodes": {"ms": round(float(timing["fp32"].mean()), 4),
                                    "mrays_s": round(n / (timing["fp32"].mean() * 1e-3) / 1e6, 2)},
                     "fp16_nodes": {"ms": round(ms16, 4), "mrays_s": round(n / (ms16 * 1e-3) / 1e6, 2)},
+                    "roofline": {"bound": "hbm", "kernel": "nearest-hit traversal (BVH4, FP16 nodes)",
+                                 "achieved": round(nbytes / (ms16 * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBPS,
+                                 "unit": "GB/s", "frac": round(nbytes / (ms16 * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
+                                 "traffic": None, "bytes_per_launch": nbytes},
+                    "visits_per_ray": round(float(stc[:, 0].mean()), 2),
+                    "tris_per_ray": round(float(stc[:, 1].mean()), 2),
                     "hits_bit_identical": same, "mismatching_rays": mism, "hit_fraction": round(float(hit.mean()), 4),
                     "t_rel_error_vs_fp64": {"max": float(rel.max()), "mean": float(rel.mean()),
                                             "median": float(np.median(rel)), "p99": float(np.percentile(rel, 99))},

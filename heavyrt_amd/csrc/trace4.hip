@@ -165,7 +165,11 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_simple(const void* __restric
 }
 
 // Persistent speculative any-hit over BVH4 nodes with ballot-based refill
-// (the BVH2 kernel in trace.hip describes the scheme).
+// from a wave-local ray pool (the BVH2 kernel in trace.hip describes the
+// scheme): a lane parks the first leaf it finds and keeps descending until
+// the wave tests leaves together; the answer does not depend on the order.
+// (Nearest hits stay on k_trace4_simple: on deep, heavily overlapping trees
+// its 48-entry LDS stack beats a short LDS stack with global spill.)
 template <bool kFp16>
 __global__ __launch_bounds__(kBlock4) void k_trace4_any_persistent(const void* __restrict__ quads,
                                                                    const TriPre* __restrict__ tris,
