@@ -63,7 +63,7 @@ EXPORTS = [
     "hrt_add_triangles", "hrt_add_instance", "hrt_commit", "hrt_trace_nearest", "hrt_trace_any",
     "hrt_trace_nearest_stats", "hrt_trace_any_stats", "hrt_query_nearest_one", "hrt_geom_count",
     "hrt_instance_count", "hrt_debug_dump", "hrt_last_error", "hrt_rtao_make_rays",
-    "hrt_trace_nearest_host", "hrt_trace_any_host",
+    "hrt_trace_nearest_host", "hrt_trace_any_host", "hrt_update_triangles", "hrt_update_instance",
     "hrt_batch_create", "hrt_batch_destroy", "hrt_batch_buffers", "hrt_batch_run", "hrt_batch_build",
     "hrt_batch_trace", "hrt_batch_candidate_count", "hrt_batch_candidate_info", "hrt_batch_set_candidate",
     "hrt_batch_get_candidate", "hrt_batch_autotune", "hrt_batch_dump",
@@ -134,6 +134,8 @@ def lib():
         L.hrt_rtao_make_rays.argtypes = [vp, vp, vp, sz, u32, C.c_float, C.c_uint64, vp, vp]
         L.hrt_trace_nearest_host.argtypes = [vp, vp, sz, vp, vp]
         L.hrt_trace_any_host.argtypes = [vp, vp, sz, vp]
+        L.hrt_update_triangles.argtypes = [vp, u32, vp, sz, sz, vp, sz, i32]
+        L.hrt_update_instance.argtypes = [vp, u32, vp]
         L.hrt_batch_create.argtypes = [vp, u32, vp, vp, vp, C.POINTER(vp)]
         L.hrt_batch_destroy.argtypes = [vp]
         L.hrt_batch_destroy.restype = None
@@ -252,6 +254,19 @@ class Scene:
         iid = C.c_uint32()
         _check(lib().hrt_add_instance(self._h, geom_id, _ptr(m), C.byref(iid)))
         return iid.value
+
+    def update_triangles(self, geom_id: int, verts: np.ndarray, idx: np.ndarray | None = None,
+                         quality: int = 3):
+        """UpdateGeom_Triangles3f: quality 3 (BUILD_REFIT) keeps the hierarchy,
+        other qualities rebuild; commit again before tracing."""
+        v = np.ascontiguousarray(verts, dtype=np.float32)
+        i = None if idx is None else np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1)
+        _check(lib().hrt_update_triangles(self._h, geom_id, _ptr(v), v.shape[0], 4 * v.shape[1],
+                                          None if i is None else _ptr(i), 0 if i is None else i.size, quality))
+
+    def update_instance(self, inst_id: int, matrix_col_major: np.ndarray):
+        m = np.ascontiguousarray(np.asarray(matrix_col_major, np.float32).reshape(16))
+        _check(lib().hrt_update_instance(self._h, inst_id, _ptr(m)))
 
     def commit(self, quality: int = 1, layout: int = LAYOUT_BVH2):
         _check(lib().hrt_commit(self._h, quality, layout))

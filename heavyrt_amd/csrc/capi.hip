@@ -106,6 +106,19 @@ void instance_box(const float* gbox8, const float* m, float* out8) {
   }
 }
 
+// Float4 copy of caller vertices with w = 1 (BVH2CommonRT_host.cpp:79-84).
+std::vector<float> repack_vertices(const float* vpos, size_t nVert, size_t vStrideBytes) {
+  const size_t stride = vStrideBytes / 4;
+  std::vector<float> v4(4 * nVert);
+  for (size_t i = 0; i < nVert; i++) {
+    v4[4 * i + 0] = vpos[i * stride + 0];
+    v4[4 * i + 1] = vpos[i * stride + 1];
+    v4[4 * i + 2] = vpos[i * stride + 2];
+    v4[4 * i + 3] = 1.0f;
+  }
+  return v4;
+}
+
 bool is_identity(const float* m) {
   for (int c = 0; c < 4; c++)
     for (int r = 0; r < 4; r++)
@@ -337,15 +350,7 @@ hrt_status hrt_add_triangles(hrt_scene* s, const float* vpos, size_t nVert, size
   for (size_t i = 0; i < 3 * ntris; i++)
     if (idx[i] >= nVert) return fail(HRT_ERR_INVALID_ARG, "index out of range");
 
-  // repack to float4 with w = 1 (BVH2CommonRT_host.cpp:79-84)
-  const size_t stride = vStrideBytes / 4;
-  std::vector<float> v4(4 * nVert);
-  for (size_t i = 0; i < nVert; i++) {
-    v4[4 * i + 0] = vpos[i * stride + 0];
-    v4[4 * i + 1] = vpos[i * stride + 1];
-    v4[4 * i + 2] = vpos[i * stride + 2];
-    v4[4 * i + 3] = 1.0f;
-  }
+  const std::vector<float> v4 = repack_vertices(vpos, nVert, vStrideBytes);
   hipError_t e = hipSetDevice(s->ctx->device);
   if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
   hrt::Blas b;
@@ -383,6 +388,85 @@ hrt_status hrt_add_instance(hrt_scene* s, uint32_t geomId, const float colMajor1
   s->instances.push_back(in);
   s->committed = false;
   if (outInstId) *outInstId = uint32_t(s->instances.size() - 1);
+  return HRT_OK;
+}
+
+hrt_status hrt_update_triangles(hrt_scene* s, uint32_t geomId, const float* vpos, size_t nVert, size_t vStrideBytes,
+                                const uint32_t* idx, size_t nIdx, int quality) {
+  if (!s || !vpos) return fail(HRT_ERR_INVALID_ARG, "scene/vpos is null");
+  if (geomId >= s->geoms.size()) return fail(HRT_ERR_INVALID_ARG, "no such geometry");
+  if (vStrideBytes == 0) vStrideBytes = 12;
+  if (vStrideBytes % 4 != 0 || vStrideBytes < 12) return fail(HRT_ERR_INVALID_ARG, "vertex stride must be >= 12 and a multiple of 4");
+  hrt::Blas& b = s->geoms[geomId];
+  const size_t ntris = idx ? nIdx / 3 : b.ntris;
+  if (nVert == 0 || nVert > b.nverts || ntris == 0 || ntris > b.ntris)
+    return fail(HRT_ERR_INVALID_ARG, "an updated geometry may not grow (CrossRT.h:101-104) and needs a triangle");
+  if (idx) {
+    for (size_t i = 0; i < 3 * ntris; i++)
+      if (idx[i] >= nVert) return fail(HRT_ERR_INVALID_ARG, "index out of range");
+  }
+  const std::vector<float> v4 = repack_vertices(vpos, nVert, vStrideBytes);
+  hipError_t e = hipSetDevice(s->ctx->device);
+  if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
+  hipStream_t hs = s->ctx->stream;
+  s->committed = false;
+  free_tlas(s);
+  if (quality == HRT_BUILD_REFIT) {
+    if (ntris != b.ntris) return fail(HRT_ERR_INVALID_ARG, "a refit keeps the triangle count");
+    if (!idx && nVert < b.nverts) {
+      // the kept indices must stay inside the new vertex range: checked on the host copy
+      std::vector<uint32_t> old(3 * size_t(b.ntris));
+      if ((e = hipMemcpy(old.data(), b.idx, 4 * old.size(), hipMemcpyDeviceToHost)) != hipSuccess)
+        return fail_hip(e, "refit");
+      for (uint32_t x : old)
+        if (x >= nVert) return fail(HRT_ERR_INVALID_ARG, "kept indices reach past the new vertex count");
+    }
+    b.nverts = uint32_t(nVert);
+    e = hipMemcpyAsync(b.verts, v4.data(), 16 * nVert, hipMemcpyHostToDevice, hs);
+    if (e == hipSuccess && idx) e = hipMemcpyAsync(b.idx, idx, 12 * ntris, hipMemcpyHostToDevice, hs);
+    if (e == hipSuccess) e = hrt::blas_refit(b, hs);
+    if (e != hipSuccess) return fail_hip(e, "refit");
+    return HRT_OK;
+  }
+  // rebuild in place of the old tree (same geometry id)
+  std::vector<uint32_t> kept;
+  if (!idx) {
+    kept.resize(3 * size_t(b.ntris));
+    if ((e = hipMemcpy(kept.data(), b.idx, 4 * kept.size(), hipMemcpyDeviceToHost)) != hipSuccess)
+      return fail_hip(e, "rebuild");
+    for (uint32_t x : kept)
+      if (x >= nVert) return fail(HRT_ERR_INVALID_ARG, "kept indices reach past the new vertex count");
+    idx = kept.data();
+  }
+  hrt::Blas nb;
+  if ((e = hrt::blas_alloc(nb, uint32_t(nVert), uint32_t(ntris), s->keep_debug)) == hipSuccess) {
+    e = hipMemcpyAsync(nb.verts, v4.data(), 16 * nVert, hipMemcpyHostToDevice, hs);
+    if (e == hipSuccess) e = hipMemcpyAsync(nb.idx, idx, 12 * ntris, hipMemcpyHostToDevice, hs);
+    if (e == hipSuccess) e = hrt::blas_build(nb, hs);
+  }
+  if (e != hipSuccess) {
+    hrt::blas_free(nb);
+    return fail_hip(e, "rebuild");
+  }
+  if (nb.max_run > 127) {
+    hrt::blas_free(nb);
+    return fail(HRT_ERR_LEAF_OVERFLOW, "more than 127 triangles share one Morton code");
+  }
+  (void)hipStreamSynchronize(hs);
+  hrt::blas_free(b);
+  b = nb;
+  return HRT_OK;
+}
+
+hrt_status hrt_update_instance(hrt_scene* s, uint32_t instId, const float colMajor16[16]) {
+  if (!s || !colMajor16) return fail(HRT_ERR_INVALID_ARG, "scene/matrix is null");
+  if (instId >= s->instances.size()) return fail(HRT_ERR_INVALID_ARG, "no such instance");
+  Instance in = s->instances[instId];
+  std::memcpy(in.fwd, colMajor16, sizeof(in.fwd));
+  in.identity = is_identity(colMajor16);
+  if (!inverse_rows(colMajor16, in.inv)) return fail(HRT_ERR_INVALID_ARG, "instance matrix is singular or not finite");
+  s->instances[instId] = in;
+  s->committed = false;
   return HRT_OK;
 }
 

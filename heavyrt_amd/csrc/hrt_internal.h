@@ -42,6 +42,11 @@ struct Blas {
   TriPre* tris = nullptr;     // ntris, leaf order
   uint32_t* perm = nullptr;   // leaf-order position -> original triangle id
   uint32_t* parent = nullptr; // internal node -> (parent << 1) | side (Karras step)
+  // kept from the build for refits (hierarchy unchanged, boxes recomputed)
+  uint32_t* leaf_run = nullptr;     // first sorted position of each leaf's run (leaves)
+  uint32_t* parent_leaf = nullptr;  // leaf -> (parent << 1) | side (leaves)
+  uint2* node_range = nullptr;      // internal node -> leaf range (pairs)
+  uint32_t* meta = nullptr;         // [0] leaves, [1] longest run (64 words)
   // BVH4 collapse of the same tree (made on demand by blas_build_bvh4)
   uint32_t nquads = 0;
   Quad32* quads32 = nullptr;
@@ -72,6 +77,12 @@ void blas_free(Blas& b);
 // synchronises the stream before returning (leaves/max_run/box are host
 // values afterwards).
 hipError_t blas_build(Blas& b, hipStream_t stream, bool cubic = false);
+// Refit after new vertex positions (and/or indices with the same triangle
+// count) were written to b.verts / b.idx: keeps the hierarchy and the leaf
+// order, recomputes every box bottom-up, the leaf-order triangles and the
+// vertex box; drops the BVH4 nodes and the any-hit tree (made again on the
+// next commit).  Synchronises the stream.
+hipError_t blas_refit(Blas& b, hipStream_t stream);
 // Builds b.any_tree (cubic grid, with its BVH4 nodes) unless it exists.
 // Leaves b.any_tree null when some Morton cell of the cubic grid holds more
 // than 127 triangles.  Synchronises the stream.

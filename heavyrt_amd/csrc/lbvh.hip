@@ -613,7 +613,7 @@ __global__ __launch_bounds__(kBlock) void k_tris(const float4* __restrict__ v, c
   r.e1 = make_float4(b.x - a.x, b.y - a.y, b.z - a.z, 0.0f);
   r.e2 = make_float4(c.x - a.x, c.y - a.y, c.z - a.z, 0.0f);
   tris[j] = r;
-  perm[j] = t;
+  if (perm) perm[j] = t;
 }
 
 inline uint32_t blocks_for(uint64_t n) { return uint32_t((n + kBlock - 1) / kBlock); }
@@ -631,6 +631,10 @@ hipError_t blas_alloc(Blas& b, uint32_t nverts, uint32_t ntris, bool keep_debug)
   HRT_CHECK(hipMalloc(&b.tris, sizeof(TriPre) * size_t(ntris)));
   HRT_CHECK(hipMalloc(&b.perm, sizeof(uint32_t) * size_t(ntris)));
   HRT_CHECK(hipMalloc(&b.parent, sizeof(uint32_t) * npairs));
+  HRT_CHECK(hipMalloc(&b.leaf_run, sizeof(uint32_t) * size_t(ntris)));
+  HRT_CHECK(hipMalloc(&b.parent_leaf, sizeof(uint32_t) * size_t(ntris)));
+  HRT_CHECK(hipMalloc(&b.node_range, sizeof(uint2) * npairs));
+  HRT_CHECK(hipMalloc(&b.meta, sizeof(uint32_t) * 64));
   if (keep_debug) {
     HRT_CHECK(hipMalloc(&b.codes_in, sizeof(uint32_t) * size_t(ntris)));
     HRT_CHECK(hipMalloc(&b.codes_sorted, sizeof(uint32_t) * size_t(ntris)));
@@ -648,7 +652,8 @@ void blas_free(Blas& b) {
     delete b.any_tree;
     b.any_tree = nullptr;
   }
-  void* ptrs[] = {b.verts,      b.idx,        b.pairs,      b.tris,      b.perm,   b.parent,
+  void* ptrs[] = {b.leaf_run,   b.parent_leaf, b.node_range, b.meta,
+                  b.verts,      b.idx,        b.pairs,      b.tris,      b.perm,   b.parent,
                   b.quads32,    b.quads16,    b.codes_in,   b.codes_sorted, b.ids_sorted,
                   b.run_start,  b.ucodes,     b.karras_lr};
   for (void* p : ptrs)
@@ -682,16 +687,16 @@ hipError_t lbvh_build(Blas& b, const Src& src, uint32_t box_items, hipStream_t s
   auto take = [&](size_t n) { size_t o = words; words += (n + 63) & ~size_t(63); return o; };
   const size_t o_ka = take(w_codes), o_va = take(w_codes), o_kb = take(w_codes), o_vb = take(w_codes);
   const size_t o_hist = take(w_hist), o_hscan = take(w_hist), o_scan = take(w_scan);
-  const size_t o_flags = take(N), o_pos = take(N), o_rs = take(N), o_uc = take(N);
-  const size_t o_range = take(2 * N), o_pleaf = take(N), o_arr = take(N), o_meta = take(64);
+  const size_t o_flags = take(N), o_pos = take(N), o_uc = take(N), o_arr = take(N);
   const size_t o_part = take(size_t(box_parts) * 8), o_box = take(16);
   uint32_t* base = nullptr;
   HRT_CHECK(hipMallocAsync(reinterpret_cast<void**>(&base), words * 4, s));
   uint32_t *ka = base + o_ka, *va = base + o_va, *kb = base + o_kb, *vb = base + o_vb;
   uint32_t *hist = base + o_hist, *hscan = base + o_hscan, *stmp = base + o_scan;
-  uint32_t *flags = base + o_flags, *pos = base + o_pos, *rs = base + o_rs, *uc = base + o_uc;
-  uint32_t *pleaf = base + o_pleaf, *arrive = base + o_arr, *meta = base + o_meta;
-  uint2* range = reinterpret_cast<uint2*>(base + o_range);
+  uint32_t *flags = base + o_flags, *pos = base + o_pos, *uc = base + o_uc, *arrive = base + o_arr;
+  // runs, leaf parents, node ranges and the leaf count stay with the tree (refit)
+  uint32_t *rs = b.leaf_run, *pleaf = b.parent_leaf, *meta = b.meta;
+  uint2* range = b.node_range;
   float4* part = reinterpret_cast<float4*>(base + o_part);
   float* box = reinterpret_cast<float*>(base + o_box);
 
@@ -768,6 +773,45 @@ hipError_t blas_build(Blas& b, hipStream_t s, bool cubic) {
         k_tris<<<blocks_for(N), kBlock, 0, s>>>(b.verts, b.idx, ids_sorted, N, b.tris, b.perm);
       },
       cubic ? 1 : 0);
+}
+
+hipError_t blas_refit(Blas& b, hipStream_t s) {
+  const uint64_t N = b.ntris;
+  if (N == 0 || !b.leaf_run) return hipErrorInvalidValue;
+  const TriSource src{b.verts, b.idx, b.nverts};
+  const int box_parts = int(std::min<uint64_t>(1024, std::max<uint64_t>(1, (b.nverts + kBlock - 1) / kBlock)));
+  uint32_t* base = nullptr;
+  const size_t o_part = (N + 63) & ~size_t(63);
+  HRT_CHECK(hipMallocAsync(reinterpret_cast<void**>(&base), 4 * (o_part + 8 * size_t(box_parts) + 16), s));
+  uint32_t* arrive = base;
+  float4* part = reinterpret_cast<float4*>(base + o_part);
+  float* box = reinterpret_cast<float*>(base + o_part + 8 * size_t(box_parts));
+  k_box_partial<TriSource><<<box_parts, kBlock, 0, s>>>(src, part);
+  k_box_final<<<1, kBlock, 0, s>>>(part, box_parts, box, box + 8, 0);
+  k_fill<<<fill_blocks(N), kBlock, 0, s>>>(arrive, N, 0u);
+  k_leaves_refit<TriSource><<<uint32_t((N + kRefitChunk - 1) / kRefitChunk), kBlock, 0, s>>>(
+      src, b.perm, b.leaf_run, b.meta, uint32_t(N), b.pairs, b.parent, b.parent_leaf, b.node_range, arrive);
+  k_tris<<<blocks_for(N), kBlock, 0, s>>>(b.verts, b.idx, b.perm, uint32_t(N), b.tris, nullptr);
+  hipError_t err = hipGetLastError();
+  float hbox[8];
+  if (err == hipSuccess) err = hipMemcpyAsync(hbox, box, sizeof(hbox), hipMemcpyDeviceToHost, s);
+  hipError_t ferr = hipFreeAsync(base, s);
+  if (err == hipSuccess) err = ferr;
+  if (err == hipSuccess) err = hipStreamSynchronize(s);
+  if (err != hipSuccess) return err;
+  std::memcpy(b.box, hbox, sizeof(b.box));
+  // boxes changed: the BVH4 nodes and the any-hit tree are made again on the next commit
+  if (b.quads32) (void)hipFree(b.quads32);
+  if (b.quads16) (void)hipFree(b.quads16);
+  b.quads32 = nullptr;
+  b.quads16 = nullptr;
+  b.nquads = 0;
+  if (b.any_tree) {
+    blas_free(*b.any_tree);
+    delete b.any_tree;
+    b.any_tree = nullptr;
+  }
+  return hipSuccess;
 }
 
 hipError_t blas_build_any_tree(Blas& b, hipStream_t s) {

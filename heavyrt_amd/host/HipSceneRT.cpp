@@ -77,9 +77,23 @@ struct HipSceneRT : public ISceneObject {
     return id;
   }
 
-  void UpdateGeom_Triangles3f(uint32_t, const float*, size_t, const uint32_t*, size_t, BuildQuality,
-                              size_t) override {
-    std::printf("[HipSceneRT::UpdateGeom_Triangles3f]: not implemented\n");
+  // CrossRT.h:97-105: the geometry keeps its id and may not grow; BUILD_REFIT
+  // keeps the hierarchy and recomputes the boxes, other qualities rebuild.
+  // The scene is traced again after the next CommitScene.
+  void UpdateGeom_Triangles3f(uint32_t a_geomId, const float* a_vpos3f, size_t a_vertNumber,
+                              const uint32_t* a_triIndices, size_t a_indNumber, BuildQuality a_qualityLevel,
+                              size_t vByteStride) override {
+    if (!m_scene) return;
+    hrt_status st = hrt_update_triangles(m_scene, a_geomId, a_vpos3f, a_vertNumber, vByteStride, a_triIndices,
+                                         a_indNumber, int(a_qualityLevel));
+    report("UpdateGeom_Triangles3f", st);
+    if (st != HRT_OK || 2 * size_t(a_geomId) + 1 >= m_geomBoxes.size()) return;
+    LiteMath::Box4f box;
+    const size_t stride = (vByteStride ? vByteStride : 12) / 4;
+    for (size_t i = 0; i < a_vertNumber; i++)
+      box.include(LiteMath::float4(a_vpos3f[i * stride], a_vpos3f[i * stride + 1], a_vpos3f[i * stride + 2], 1.0f));
+    m_geomBoxes[2 * a_geomId] = box.boxMin;
+    m_geomBoxes[2 * a_geomId + 1] = box.boxMax;
   }
 
   void ClearScene() override {
@@ -101,8 +115,10 @@ struct HipSceneRT : public ISceneObject {
     return st == HRT_OK ? id : 0xFFFFFFFFu;
   }
 
-  void UpdateInstance(uint32_t, const LiteMath::float4x4&) override {
-    std::printf("[HipSceneRT::UpdateInstance]: not implemented\n");
+  // CrossRT.h:125-130: a new matrix for an existing instance; traced after
+  // the next CommitScene.
+  void UpdateInstance(uint32_t a_instanceId, const LiteMath::float4x4& a_matrix) override {
+    if (m_scene) report("UpdateInstance", hrt_update_instance(m_scene, a_instanceId, a_matrix.data()));
   }
 
   CRT_Hit RayQuery_NearestHit(LiteMath::float4 posAndNear, LiteMath::float4 dirAndFar) override {

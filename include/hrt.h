@@ -139,6 +139,21 @@ hrt_status hrt_add_triangles(hrt_scene* scene, const float* vpos, size_t nVert, 
  * a singular matrix (the reference would trace with a non-finite inverse). */
 hrt_status hrt_add_instance(hrt_scene* scene, uint32_t geomId, const float colMajor16[16], uint32_t* outInstId);
 
+/* ISceneObject::UpdateGeom_Triangles3f (CrossRT.h:97-105; a stub in every
+ * reference engine, e.g. BVH2CommonRT_host.cpp:100-103).  The geometry keeps
+ * its id and may not grow (at most the vertex and triangle counts it was
+ * added with).  quality == HRT_BUILD_REFIT keeps the hierarchy and the leaf
+ * order and recomputes every box (the triangle count must not change; idx
+ * may be null to keep the indices); any other quality rebuilds the LBVH.
+ * The scene must be committed again before it is traced. */
+hrt_status hrt_update_triangles(hrt_scene* scene, uint32_t geomId, const float* vpos, size_t nVert,
+                                size_t vStrideBytes, const uint32_t* idx, size_t nIdx, int quality);
+
+/* ISceneObject::UpdateInstance (CrossRT.h:125-130; a stub in the reference,
+ * BVH2CommonRT_host.cpp:166-170): a new column-major matrix for an existing
+ * instance; the scene must be committed again. */
+hrt_status hrt_update_instance(hrt_scene* scene, uint32_t instId, const float colMajor16[16]);
+
 /* ISceneObject::CommitScene (CrossRT.h:115; BVH2CommonRT_host.cpp:118-129).
  * A scene with exactly one instance whose matrix is the identity is traced on
  * its geometry's tree in `layout`.  Any other instance set gets a top-level

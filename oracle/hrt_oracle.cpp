@@ -1272,6 +1272,52 @@ void* hrto_build(const float* vert4, uint32_t nVert, const uint32_t* idx, uint32
 
 void hrto_free(void* h) { delete static_cast<Build*>(h); }
 
+// Refit (BuildQuality BUILD_REFIT, CrossRT.h:14): new vertex positions (and
+// optionally indices, same triangle count), hierarchy and leaf order kept;
+// leaf boxes are recomputed from the leaf-order triangles and the internal
+// boxes by the same 32-sweep refit as a build (lbvh.cpp:443-489).  The
+// vertex box is recomputed; the Morton scale, codes and leaf order stay those
+// of the original build.  Returns 0, or -1 if the triangle count differs.
+int hrto_refit(void* h, const float* vert4, uint32_t nVert, const uint32_t* idx, uint32_t nIdx) {
+  Build& b = *static_cast<Build*>(h);
+  if (idx && nIdx / 3 != b.ntris) return -1;
+  b.verts.resize(nVert);
+  std::memcpy(b.verts.data(), vert4, sizeof(f4) * nVert);
+  for (auto& v : b.verts) v.w = 1.0f;
+  if (idx) b.idx.assign(idx, idx + 3 * size_t(b.ntris));
+  f4 lo = {LBVH_INF, LBVH_INF, LBVH_INF, LBVH_INF}, hi = {-LBVH_INF, -LBVH_INF, -LBVH_INF, -LBVH_INF};
+  for (const f4& v : b.verts) {
+    lo = {mn(lo.x, v.x), mn(lo.y, v.y), mn(lo.z, v.z), mn(lo.w, v.w)};
+    hi = {mx(hi.x, v.x), mx(hi.y, v.y), mx(hi.z, v.z), mx(hi.w, v.w)};
+  }
+  b.bbmin = lo;
+  b.bbmax = hi;
+  const uint32_t L = b.L, N = b.ntris;
+  for (uint32_t i = 0; i < L; i++) {
+    const uint32_t start = b.run_start[i];
+    const uint32_t count = (i < L - 1) ? b.run_start[i + 1] - start : N - b.run_start[L - 1];
+    f4 bl = {LBVH_INF, LBVH_INF, LBVH_INF, 0}, bh = {-LBVH_INF, -LBVH_INF, -LBVH_INF, 0};
+    for (uint32_t j = start; j < start + count; j++) {
+      const uint32_t t = b.perm[j];
+      const f4 v1 = b.verts[b.idx[3 * t + 0]], v2 = b.verts[b.idx[3 * t + 1]], v3 = b.verts[b.idx[3 * t + 2]];
+      bl = {mn(bl.x, mn(v1.x, mn(v2.x, v3.x))), mn(bl.y, mn(v1.y, mn(v2.y, v3.y))), mn(bl.z, mn(v1.z, mn(v2.z, v3.z))), 0};
+      bh = {mx(bh.x, mx(v1.x, mx(v2.x, v3.x))), mx(bh.y, mx(v1.y, mx(v2.y, v3.y))), mx(bh.z, mx(v1.z, mx(v2.z, v3.z))), 0};
+    }
+    Node& nd = b.lr[L - 1 + i];
+    nd.bmin = {bl.x, bl.y, bl.z};
+    nd.bmax = {bh.x, bh.y, bh.z};
+  }
+  refit(b);
+  b.peach.clear();
+  build_peach(b);
+  b.idx_leaf_order.resize(3 * size_t(N));
+  for (uint32_t i = 0; i < N; i++)
+    for (int k = 0; k < 3; k++) b.idx_leaf_order[3 * i + k] = b.idx[3 * b.perm[i] + k];
+  b.q32.clear();
+  b.q16.clear();
+  return 0;
+}
+
 // sizes: [ntris, leaves, lr_nodes, peach_nodes, max_run]
 void hrto_sizes(void* h, uint64_t* out5) {
   const Build* b = static_cast<Build*>(h);

@@ -62,6 +62,8 @@ def lib():
         L.hrto_build_ex.restype = vp
         L.hrto_build_ex.argtypes = [vp, u32, vp, u32, C.c_int]
         L.hrto_free.argtypes = [vp]
+        L.hrto_refit.restype = C.c_int
+        L.hrto_refit.argtypes = [vp, vp, u32, vp, u32]
         L.hrto_sizes.argtypes = [vp, vp]
         L.hrto_scene_box.argtypes = [vp, vp]
         L.hrto_codes.argtypes = [vp, vp, vp, vp]
@@ -146,6 +148,16 @@ class OracleBVH:
         if getattr(self, "_owned", False) and getattr(self, "_h", None) and _lib is not None:
             _lib.hrto_free(self._h)
             self._h = None
+
+    def refit(self, verts: np.ndarray, idx: np.ndarray | None = None):
+        """BUILD_REFIT: new positions (and optionally indices), same hierarchy."""
+        self.v4 = as_vert4(verts)
+        if idx is not None:
+            self.idx = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1)
+        r = lib().hrto_refit(self._h, _ptr(self.v4), self.v4.shape[0], None if idx is None else _ptr(self.idx),
+                             0 if idx is None else self.idx.size)
+        if r != 0:
+            raise ValueError("a refit keeps the triangle count")
 
     def scene_box(self):
         out = np.zeros(12, dtype=np.float32)
