@@ -34,16 +34,13 @@
 #include <vector>
 
 #include "hrt_internal.h"
+#include "refit.h"
 
 namespace hrt {
 namespace {
 
 constexpr int kB = 256;
 constexpr uint32_t kBoxChunk = 4096;   // vertices per box-reduction workgroup
-#ifndef HRT_REFIT_CHUNK
-#define HRT_REFIT_CHUNK 4096
-#endif
-constexpr uint32_t kRefitChunkB = HRT_REFIT_CHUNK;  // leaves per refit workgroup
 
 inline uint32_t blocks(uint64_t n) { return uint32_t(std::max<uint64_t>(1, (n + kB - 1) / kB)); }
 
@@ -54,17 +51,6 @@ __device__ inline uint32_t corner(const uint32_t* __restrict__ idx, uint32_t t, 
   return min(idx[3 * t + k], nv - 1);
 }
 
-// Scene of element i: the last s with off[s] <= i (off holds S + 1 prefix
-// sums; empty scenes are skipped because their offset equals the next one).
-__device__ inline uint32_t seg_of(const uint32_t* __restrict__ off, uint32_t S, uint32_t i) {
-  uint32_t lo = 0, hi = S;  // answer in [lo, hi)
-  while (hi - lo > 1) {
-    const uint32_t mid = (lo + hi) >> 1;
-    if (off[mid] <= i) lo = mid;
-    else hi = mid;
-  }
-  return lo;
-}
 
 // ---------------------------------------------------------------------------
 // build
@@ -149,6 +135,7 @@ struct SceneOffsets {
   const uint32_t* tri;   // S + 1
   const uint32_t* pair;  // S + 1 (max(ntris - 1, 1) pairs reserved per scene)
   const uint32_t* ray;   // S + 1
+  const uint32_t* tops;  // S + 1 (each scene's share of the refit's top-entry lists)
   uint32_t S;
 };
 
@@ -158,7 +145,7 @@ __global__ __launch_bounds__(kB) void k_b_morton(const float4* __restrict__ v, c
                                                  uint32_t* __restrict__ vals) {
   const uint32_t i = blockIdx.x * kB + threadIdx.x;
   if (i >= T) return;
-  const uint32_t s = seg_of(off.tri, off.S, i);
+  const uint32_t s = seg_of_wave(off.tri, off.S, i);
   const uint32_t n = off.tri[s + 1] - off.tri[s];
   const uint32_t nv = off.vert[s + 1] - off.vert[s];
   const float4* vs = v + off.vert[s];
@@ -205,7 +192,7 @@ __global__ __launch_bounds__(kB) void k_b_run_flags(const uint32_t* __restrict__
                                                     uint32_t T, uint32_t* __restrict__ flags) {
   const uint32_t i = blockIdx.x * kB + threadIdx.x;
   if (i >= T) return;
-  const uint32_t s = seg_of(off.tri, off.S, i);
+  const uint32_t s = seg_of_wave(off.tri, off.S, i);
   flags[i] = (i == off.tri[s] || codes[i - 1] != codes[i]) ? 1u : 0u;
 }
 
@@ -218,7 +205,7 @@ __global__ __launch_bounds__(kB) void k_b_run_compact(const uint32_t* __restrict
                                                       uint32_t* __restrict__ leaf_off) {
   const uint32_t i = blockIdx.x * kB + threadIdx.x;
   if (i >= T) return;
-  const uint32_t s = seg_of(off.tri, off.S, i);
+  const uint32_t s = seg_of_wave(off.tri, off.S, i);
   if (flags[i]) {
     run_start[pos[i]] = i - off.tri[s];
     ucodes[pos[i]] = codes[i];
@@ -245,7 +232,7 @@ __global__ __launch_bounds__(kB) void k_b_karras(const uint32_t* __restrict__ uc
                                                  uint2* __restrict__ range_all) {
   const uint32_t jg = blockIdx.x * kB + threadIdx.x;
   if (jg >= T || jg >= leaf_off[off.S]) return;
-  const uint32_t s = seg_of(leaf_off, off.S, jg);
+  const uint32_t s = seg_of_wave(leaf_off, off.S, jg);
   const uint32_t lb = leaf_off[s];
   const int L = int(leaf_off[s + 1] - lb);
   const int i = int(jg - lb);
@@ -296,40 +283,49 @@ __global__ __launch_bounds__(kB) void k_b_karras(const uint32_t* __restrict__ uc
 // Leaf boxes and the bottom-up refit, chunked over global leaves as in
 // lbvh.hip: a node whose global leaf range lies inside the workgroup's chunk
 // synchronises at workgroup scope, the rest at agent scope.
-__global__ __launch_bounds__(kB) void k_b_leaves_refit(const float4* __restrict__ v, const uint32_t* __restrict__ idx,
-                                                       const uint32_t* __restrict__ ids_sorted,
-                                                       const uint32_t* __restrict__ run_start_all,
-                                                       const uint32_t* __restrict__ leaf_off, SceneOffsets off,
-                                                       NodePair* pairs_all, const uint32_t* __restrict__ parent_int_all,
-                                                       const uint32_t* __restrict__ parent_leaf_all,
-                                                       const uint2* __restrict__ range_all, uint32_t* arrive) {
+// One thread per global leaf m: its box goes into its parent's pair (where
+// the refit starts), its triangles into leaf order with scene-local ids, and
+// its run length into the scene's longest run.
+__global__ __launch_bounds__(kB) void k_b_leaves(const float4* __restrict__ v, const uint32_t* __restrict__ idx,
+                                                 const uint32_t* __restrict__ ids_sorted,
+                                                 const uint32_t* __restrict__ run_start_all,
+                                                 const uint32_t* __restrict__ leaf_off, SceneOffsets off,
+                                                 NodePair* __restrict__ pairs_all,
+                                                 const uint32_t* __restrict__ parent_leaf_all,
+                                                 TriPre* __restrict__ tris, uint32_t* __restrict__ perm,
+                                                 uint32_t* __restrict__ max_run) {
   const uint32_t total = leaf_off[off.S];
-  const uint32_t chunk_lo = blockIdx.x * kRefitChunkB;
-  if (chunk_lo >= total) return;
-  const uint32_t chunk_hi = min(chunk_lo + kRefitChunkB, total) - 1;
-  for (uint32_t m = chunk_lo + threadIdx.x; m <= chunk_hi; m += kB) {
-    const uint32_t s = seg_of(leaf_off, off.S, m);
+  const uint32_t m = blockIdx.x * kB + threadIdx.x;
+  const bool valid = m < total;
+  uint32_t s = 0, run = 0;
+  if (valid) {
+    s = seg_of_wave(leaf_off, off.S, m);
     const uint32_t lb = leaf_off[s];
     const uint32_t L = leaf_off[s + 1] - lb;
-    const uint32_t ml = m - lb;
     const uint32_t tb = off.tri[s];
     const uint32_t n = off.tri[s + 1] - tb;
-    const uint32_t pb = off.pair[s];
     const float4* vs = v + off.vert[s];
     const uint32_t nv = off.vert[s + 1] - off.vert[s];
     const uint32_t st = run_start_all[m];
-    const uint32_t e = (ml + 1 < L) ? run_start_all[m + 1] : n;
+    const uint32_t e = (m - lb + 1 < L) ? run_start_all[m + 1] : n;
+    run = e - st;
     float3 lo = make_float3(kLbvhInf, kLbvhInf, kLbvhInf);
     float3 hi = make_float3(-kLbvhInf, -kLbvhInf, -kLbvhInf);
     for (uint32_t jj = st; jj < e; jj++) {
       const uint32_t t = ids_sorted[tb + jj];
       const float4 a = vs[corner(idx, t, 0, nv)], b = vs[corner(idx, t, 1, nv)], c = vs[corner(idx, t, 2, nv)];
+      TriPre r;
+      r.a = make_float4(a.x, a.y, a.z, __uint_as_float(t - tb));
+      r.e1 = make_float4(b.x - a.x, b.y - a.y, b.z - a.z, 0.0f);
+      r.e2 = make_float4(c.x - a.x, c.y - a.y, c.z - a.z, 0.0f);
+      tris[tb + jj] = r;
+      perm[tb + jj] = t - tb;
       lo = make_float3(fminf(lo.x, fminf(a.x, fminf(b.x, c.x))), fminf(lo.y, fminf(a.y, fminf(b.y, c.y))),
                        fminf(lo.z, fminf(a.z, fminf(b.z, c.z))));
       hi = make_float3(fmaxf(hi.x, fmaxf(a.x, fmaxf(b.x, c.x))), fmaxf(hi.y, fmaxf(a.y, fmaxf(b.y, c.y))),
                        fmaxf(hi.z, fmaxf(a.z, fmaxf(b.z, c.z))));
     }
-    NodePair* pairs = pairs_all + pb;
+    NodePair* pairs = pairs_all + off.pair[s];
     if (L == 1) {
       // root is a leaf: pair 0 = (root leaf, empty dummy), as in lbvh.hip
       store_box(pairs, 0, 0, lo, hi);
@@ -339,74 +335,24 @@ __global__ __launch_bounds__(kB) void k_b_leaves_refit(const float4* __restrict_
       pw[7] = 0u;
       pw[11] = 0x80000000u;
       pw[15] = 0u;
-      continue;
-    }
-    const uint32_t pl = parent_leaf_all[m];
-    store_box(pairs, pl >> 1, pl & 1u, lo, hi);
-    uint32_t p = pl >> 1;
-    while (true) {
-      const uint2 r = range_all[pb + p];
-      const bool crossing = r.x < chunk_lo || r.y > chunk_hi;
-      const uint32_t old = crossing
-          ? __hip_atomic_fetch_add(&arrive[pb + p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT)
-          : __hip_atomic_fetch_add(&arrive[pb + p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-      if (old == 0) break;
-      if (p == 0) break;  // the root box is not stored
-      const uint32_t* pw = reinterpret_cast<const uint32_t*>(pairs + p);
-      const float3 l0 = make_float3(__uint_as_float(pw[0]), __uint_as_float(pw[1]), __uint_as_float(pw[2]));
-      const float3 h0 = make_float3(__uint_as_float(pw[4]), __uint_as_float(pw[5]), __uint_as_float(pw[6]));
-      const float3 l1 = make_float3(__uint_as_float(pw[8]), __uint_as_float(pw[9]), __uint_as_float(pw[10]));
-      const float3 h1 = make_float3(__uint_as_float(pw[12]), __uint_as_float(pw[13]), __uint_as_float(pw[14]));
-      const float3 ul = make_float3(fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z));
-      const float3 uh = make_float3(fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z));
-      const uint32_t pp = parent_int_all[pb + p];
-      store_box(pairs, pp >> 1, pp & 1u, ul, uh);
-      p = pp >> 1;
+    } else {
+      const uint32_t pl = parent_leaf_all[m];
+      store_box(pairs, pl >> 1, pl & 1u, lo, hi);
     }
   }
-}
-
-// Longest run (leaf size) per scene: one workgroup per scene, a block
-// reduction, no atomics (a per-leaf atomic on the scene's counter would put
-// millions of contended operations on a few addresses).
-__global__ __launch_bounds__(kB) void k_b_max_run(const uint32_t* __restrict__ run_start_all,
-                                                  const uint32_t* __restrict__ leaf_off, SceneOffsets off,
-                                                  uint32_t* __restrict__ max_run) {
-  __shared__ uint32_t red[kB];
-  const uint32_t s = blockIdx.x;
-  const uint32_t lb = leaf_off[s], le = leaf_off[s + 1];
-  const uint32_t n = off.tri[s + 1] - off.tri[s];
-  uint32_t mx = 0;
-  for (uint32_t m = lb + threadIdx.x; m < le; m += kB) {
-    const uint32_t e = (m + 1 < le) ? run_start_all[m + 1] : n;
-    mx = max(mx, e - run_start_all[m]);
+  // Longest run per scene.  A wave's leaves span one or a few scenes: the
+  // lanes of each scene reduce first, then one atomic per scene and wave
+  // (a per-leaf atomic would put millions of operations on a few addresses).
+  uint64_t todo = __ballot(valid);
+  while (todo) {
+    const int leader = __builtin_ctzll(todo);
+    const uint32_t ls = __shfl(s, leader);
+    const bool mine = valid && s == ls;
+    uint32_t mv = mine ? run : 0u;
+    for (int o = 32; o > 0; o >>= 1) mv = max(mv, __shfl_xor(mv, o));
+    if (int(threadIdx.x & 63) == leader) atomicMax(&max_run[ls], mv);
+    todo &= ~__ballot(mine);
   }
-  red[threadIdx.x] = mx;
-  __syncthreads();
-  for (int k = kB / 2; k > 0; k >>= 1) {
-    if (threadIdx.x < k) red[threadIdx.x] = max(red[threadIdx.x], red[threadIdx.x + k]);
-    __syncthreads();
-  }
-  if (threadIdx.x == 0) max_run[s] = red[0];
-}
-
-__global__ __launch_bounds__(kB) void k_b_tris(const float4* __restrict__ v, const uint32_t* __restrict__ idx,
-                                               const uint32_t* __restrict__ ids_sorted, SceneOffsets off, uint32_t T,
-                                               TriPre* __restrict__ tris, uint32_t* __restrict__ perm) {
-  const uint32_t j = blockIdx.x * kB + threadIdx.x;
-  if (j >= T) return;
-  const uint32_t s = seg_of(off.tri, off.S, j);
-  const uint32_t t = ids_sorted[j];
-  const uint32_t tl = t - off.tri[s];
-  const uint32_t nv = off.vert[s + 1] - off.vert[s];
-  const float4* vs = v + off.vert[s];
-  const float4 a = vs[corner(idx, t, 0, nv)], b = vs[corner(idx, t, 1, nv)], c = vs[corner(idx, t, 2, nv)];
-  TriPre r;
-  r.a = make_float4(a.x, a.y, a.z, __uint_as_float(tl));
-  r.e1 = make_float4(b.x - a.x, b.y - a.y, b.z - a.z, 0.0f);
-  r.e2 = make_float4(c.x - a.x, c.y - a.y, c.z - a.z, 0.0f);
-  tris[j] = r;
-  perm[j] = tl;
 }
 
 // ---------------------------------------------------------------------------
@@ -519,7 +465,7 @@ __global__ __launch_bounds__(kBlk) void k_b_trace_simple(const NodePair* __restr
   const uint32_t i = blockIdx.x * kBlk + threadIdx.x;
   if (i >= R) return;
   uint32_t* stack = stack_mem + threadIdx.x;
-  const uint32_t sc = seg_of(off.ray, off.S, i);
+  const uint32_t sc = seg_of_wave(off.ray, off.S, i);
   const NodePair* P = pairs_all + off.pair[sc];
   const TriPre* Tr = tris_all + off.tri[sc];
   const Ray r = rays[i];
@@ -605,9 +551,9 @@ struct hrt_batch {
   hrt_ctx* ctx = nullptr;
   uint32_t S = 0;
   uint64_t V = 0, T = 0, R = 0, P = 0;
-  std::vector<uint32_t> vert_off, tri_off, ray_off, pair_off;
+  std::vector<uint32_t> vert_off, tri_off, ray_off, pair_off, tops_off;
   uint32_t nchunks = 0;
-  // device: offsets (4 x (S + 1)), chunk table, chunk offsets
+  // device: offsets (5 x (S + 1)), chunk table, chunk offsets
   uint32_t* d_off = nullptr;
   uint4* d_chunks = nullptr;
   uint32_t* d_chunk_off = nullptr;
@@ -623,7 +569,7 @@ struct hrt_batch {
   // scratch: one allocation, carved on the host
   void* scratch = nullptr;
   uint32_t *codes_in, *ka, *va, *kb, *vb, *sort_tmp, *flags, *pos, *scan_tmp, *run_start, *ucodes, *parent_int,
-      *parent_leaf, *arrive, *leaf_off, *max_run, *queue, *spill;
+      *parent_leaf, *arrive, *leaf_off, *max_run, *ntops, *refit_err, *tops, *queue, *spill;
   uint2* range;
   float4* part;
   float* box12;
@@ -649,6 +595,7 @@ hrt::SceneOffsets offsets(const hrt_batch* b) {
   o.tri = b->d_off + (b->S + 1);
   o.pair = b->d_off + 2 * (b->S + 1);
   o.ray = b->d_off + 3 * (b->S + 1);
+  o.tops = b->d_off + 4 * (b->S + 1);
   o.S = b->S;
   return o;
 }
@@ -667,24 +614,25 @@ hipError_t enqueue_build(hrt_batch* b, hipStream_t s) {
   const uint32_t T = uint32_t(b->T);
   hipError_t e;
   if ((e = fill_words(b->arrive, b->P, 0u, s)) != hipSuccess) return e;
+  // longest runs, top-entry counters and the overflow flag (one region)
+  if ((e = fill_words(b->max_run, 2 * uint64_t(b->S) + 1, 0u, s)) != hipSuccess) return e;
   k_b_box_partial<<<b->nchunks, kB, 0, s>>>(b->verts, b->d_chunks, b->part);
   k_b_box_final<<<b->S, kB, 0, s>>>(b->part, b->d_chunk_off, b->box12);
   k_b_morton<<<blocks(T), kB, 0, s>>>(b->verts, b->idx, off, T, b->box12, b->codes_in, b->ka, b->va);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  // four passes on the code: (ka, va) -> (kb, vb) -> (ka, va) -> ...
-  uint32_t *kin = b->ka, *vin = b->va, *kout = b->kb, *vout = b->vb;
-  for (int shift = 0; shift < 32; shift += 8) {
-    if ((e = radix_pass(kin, vin, kout, vout, T, shift, b->sort_tmp, s)) != hipSuccess) return e;
-    std::swap(kin, kout);
-    std::swap(vin, vout);
-  }
-  // stable passes on the scene id (most significant)
+  // four passes on the code: the sorted pairs end in (ka, va)
+  if ((e = radix_sort(b->ka, b->va, b->kb, b->vb, T, 0, 4, b->sort_tmp, b->refit_err, s)) != hipSuccess) return e;
+  uint32_t *kin = b->ka, *vin = b->va;
+  // stable passes on the scene id (most significant): the (scene, code, id) order
   if (b->S > 1) {
-    for (int shift = 0; (b->S - 1) >> shift; shift += 8) {
-      k_b_scene_keys<<<blocks(T), kB, 0, s>>>(vin, off, T, kin);
-      if ((e = radix_pass(kin, vin, kout, vout, T, shift, b->sort_tmp, s)) != hipSuccess) return e;
-      std::swap(kin, kout);
-      std::swap(vin, vout);
+    int np = 0;
+    while ((b->S - 1) >> (8 * np)) np++;
+    k_b_scene_keys<<<blocks(T), kB, 0, s>>>(vin, off, T, kin);
+    if ((e = radix_sort(b->ka, b->va, b->kb, b->vb, T, 0, np, b->sort_tmp, b->refit_err, s)) != hipSuccess)
+      return e;
+    if (np & 1) {
+      kin = b->kb;
+      vin = b->vb;
     }
   }
   // sorted ids in vin; sorted codes gathered into kin
@@ -694,11 +642,13 @@ hipError_t enqueue_build(hrt_batch* b, hipStream_t s) {
   k_b_run_compact<<<blocks(T), kB, 0, s>>>(kin, b->flags, b->pos, off, T, b->run_start, b->ucodes, b->leaf_off);
   k_b_karras<<<blocks(T), kB, 0, s>>>(b->ucodes, b->run_start, b->leaf_off, off, T, b->pairs, b->parent_int,
                                       b->parent_leaf, b->range);
-  k_b_leaves_refit<<<uint32_t((T + kRefitChunkB - 1) / kRefitChunkB), kB, 0, s>>>(
-      b->verts, b->idx, vin, b->run_start, b->leaf_off, off, b->pairs, b->parent_int, b->parent_leaf, b->range,
-      b->arrive);
-  k_b_max_run<<<b->S, kB, 0, s>>>(b->run_start, b->leaf_off, off, b->max_run);
-  k_b_tris<<<blocks(T), kB, 0, s>>>(b->verts, b->idx, vin, off, T, b->tris, b->perm);
+  k_b_leaves<<<blocks(T), kB, 0, s>>>(b->verts, b->idx, vin, b->run_start, b->leaf_off, off, b->pairs,
+                                      b->parent_leaf, b->tris, b->perm, b->max_run);
+  const ManyTrees trees{b->leaf_off, off.pair, off.tops, b->S};
+  k_refit_chunks<ManyTrees><<<uint32_t((T + kRefitChunk - 1) / kRefitChunk), kRefitThreads, 0, s>>>(
+      trees, b->pairs, b->parent_int, b->parent_leaf, b->range, b->tops, b->ntops, b->refit_err);
+  k_refit_tops<ManyTrees><<<b->S, kRefitTopsThreads, 0, s>>>(trees, b->pairs, b->parent_int, b->tops, b->ntops,
+                                                            b->arrive);
   return hipGetLastError();
 }
 
@@ -787,7 +737,8 @@ hrt_status hrt_batch_create(hrt_ctx* ctx, uint32_t nScenes, const uint32_t* nVer
   b->tri_off.assign(S + 1, 0);
   b->ray_off.assign(S + 1, 0);
   b->pair_off.assign(S + 1, 0);
-  uint64_t V = 0, T = 0, R = 0, P = 0;
+  b->tops_off.assign(S + 1, 0);
+  uint64_t V = 0, T = 0, R = 0, P = 0, TOPS = 0;
   std::vector<uint4> chunks;
   std::vector<uint32_t> chunk_off(S + 1, 0);
   for (uint32_t s = 0; s < S; s++) {
@@ -803,6 +754,7 @@ hrt_status hrt_batch_create(hrt_ctx* ctx, uint32_t nScenes, const uint32_t* nVer
     b->tri_off[s] = uint32_t(T);
     b->ray_off[s] = uint32_t(R);
     b->pair_off[s] = uint32_t(P);
+    b->tops_off[s] = uint32_t(TOPS);
     chunk_off[s] = uint32_t(chunks.size());
     for (uint64_t c = 0; c < nVerts[s]; c += kBoxChunk)
       chunks.push_back(make_uint4(s, uint32_t(V + c), uint32_t(V + std::min<uint64_t>(nVerts[s], c + kBoxChunk)), 0));
@@ -810,15 +762,17 @@ hrt_status hrt_batch_create(hrt_ctx* ctx, uint32_t nScenes, const uint32_t* nVer
     T += nTris[s];
     R += nRays[s];
     P += std::max<uint32_t>(nTris[s] > 1 ? nTris[s] - 1 : 1, 1);
+    TOPS += refit_tops_capacity(nTris[s]);
   }
-  if (V > 0xFFFFFFFFull || T > 0x7FFFFFFFull || R > 0x7FFFFFFFull || P > 0x7FFFFFFFull) {
+  if (V > 0xFFFFFFFFull || T > 0x3FFFFFFFull || R > 0x7FFFFFFFull || P > 0x7FFFFFFFull || TOPS > 0x7FFFFFFFull) {
     delete b;
-    return fail(HRT_ERR_INVALID_ARG, "batch too large (at most 2^31-1 triangles and rays, 2^32-1 vertices)");
+    return fail(HRT_ERR_INVALID_ARG, "batch too large (at most 2^30-1 triangles, 2^31-1 rays, 2^32-1 vertices)");
   }
   b->vert_off[S] = uint32_t(V);
   b->tri_off[S] = uint32_t(T);
   b->ray_off[S] = uint32_t(R);
   b->pair_off[S] = uint32_t(P);
+  b->tops_off[S] = uint32_t(TOPS);
   chunk_off[S] = uint32_t(chunks.size());
   b->V = V;
   b->T = T;
@@ -830,7 +784,7 @@ hrt_status hrt_batch_create(hrt_ctx* ctx, uint32_t nScenes, const uint32_t* nVer
   auto al = [&](auto** p, size_t bytes) {
     if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(p), std::max<size_t>(bytes, 16));
   };
-  al(&b->d_off, 4 * 4 * size_t(S + 1));
+  al(&b->d_off, 4 * 5 * size_t(S + 1));
   al(&b->d_chunks, sizeof(uint4) * chunks.size());
   al(&b->d_chunk_off, 4 * size_t(S + 1));
   al(&b->verts, 16 * V);
@@ -849,9 +803,10 @@ hrt_status hrt_batch_create(hrt_ctx* ctx, uint32_t nScenes, const uint32_t* nVer
   size_t words = 0;
   auto take = [&](size_t n) { size_t o = words; words += (n + 63) & ~size_t(63); return o; };
   const size_t o_codes = take(T), o_ka = take(T), o_va = take(T), o_kb = take(T), o_vb = take(T);
-  const size_t o_sort = take(radix_pass_scratch_words(T)), o_flags = take(T), o_pos = take(T);
+  const size_t o_sort = take(radix_sort_scratch_words(T, 4)), o_flags = take(T), o_pos = take(T);
   const size_t o_scan = take(scan_scratch_words(T)), o_rs = take(T), o_uc = take(T);
-  const size_t o_pint = take(P), o_pleaf = take(T), o_arr = take(P), o_loff = take(S + 1), o_mrun = take(S);
+  const size_t o_pint = take(P), o_pleaf = take(T), o_arr = take(P), o_loff = take(S + 1), o_mrun = take(2 * size_t(S) + 1);
+  const size_t o_tops = take(TOPS);
   const size_t o_q = take(64), o_range = take(2 * P), o_part = take(8 * size_t(b->nchunks)), o_box = take(12 * size_t(S));
   const size_t o_spill = take(b->spill_words);
   al(&b->scratch, 4 * words);
@@ -866,6 +821,7 @@ hrt_status hrt_batch_create(hrt_ctx* ctx, uint32_t nScenes, const uint32_t* nVer
   b->sort_tmp = w + o_sort; b->flags = w + o_flags; b->pos = w + o_pos; b->scan_tmp = w + o_scan;
   b->run_start = w + o_rs; b->ucodes = w + o_uc; b->parent_int = w + o_pint; b->parent_leaf = w + o_pleaf;
   b->arrive = w + o_arr; b->leaf_off = w + o_loff; b->max_run = w + o_mrun; b->queue = w + o_q;
+  b->ntops = b->max_run + S; b->refit_err = b->ntops + S; b->tops = w + o_tops;
   b->range = reinterpret_cast<uint2*>(w + o_range);
   b->part = reinterpret_cast<float4*>(w + o_part);
   b->box12 = reinterpret_cast<float*>(w + o_box);
@@ -876,6 +832,7 @@ hrt_status hrt_batch_create(hrt_ctx* ctx, uint32_t nScenes, const uint32_t* nVer
   offs.insert(offs.end(), b->tri_off.begin(), b->tri_off.end());
   offs.insert(offs.end(), b->pair_off.begin(), b->pair_off.end());
   offs.insert(offs.end(), b->ray_off.begin(), b->ray_off.end());
+  offs.insert(offs.end(), b->tops_off.begin(), b->tops_off.end());
   e = hipMemcpy(b->d_off, offs.data(), 4 * offs.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(b->d_chunks, chunks.data(), sizeof(uint4) * chunks.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(b->d_chunk_off, chunk_off.data(), 4 * chunk_off.size(), hipMemcpyHostToDevice);
@@ -1015,7 +972,9 @@ hrt_status hrt_batch_dump(hrt_batch* b, uint32_t scene, int what, void* host_out
   hipError_t e = hipSetDevice(b->ctx->device);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) return fail_hip(e, "dump");
-  uint32_t lo[2], mr = 0;
+  uint32_t lo[2], mr = 0, rerr = 0;
+  if ((e = hipMemcpy(&rerr, b->refit_err, 4, hipMemcpyDeviceToHost)) != hipSuccess) return fail_hip(e, "dump");
+  if (rerr) return fail(HRT_ERR_UNSUPPORTED, "a device-side check of the build failed (sort look-back or refit list)");
   if ((e = hipMemcpy(lo, b->leaf_off + scene, 8, hipMemcpyDeviceToHost)) != hipSuccess) return fail_hip(e, "dump");
   if ((e = hipMemcpy(&mr, b->max_run + scene, 4, hipMemcpyDeviceToHost)) != hipSuccess) return fail_hip(e, "dump");
   const uint32_t n = b->tri_off[scene + 1] - b->tri_off[scene];

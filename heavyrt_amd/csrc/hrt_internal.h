@@ -99,10 +99,14 @@ hipError_t blas_build_bvh4(Blas& b, hipStream_t stream);
 hipError_t tlas_build(Blas& t, const InstDev* by_id, InstDev* leaf_order, hipStream_t stream);
 
 // Building blocks (graph-capturable: no allocation, no memset, no sync) -----
-// One stable 8-bit LSD radix pass of (key, value) pairs on bits [shift, shift+8).
-size_t radix_pass_scratch_words(uint64_t n);
-hipError_t radix_pass(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout, uint64_t n, int shift,
-                      uint32_t* scratch, hipStream_t stream);
+// Stable LSD radix sort (onesweep, sort.hip) of (key, value) pairs on the
+// npasses (1-4) 8-bit digits starting at bit shift0, n < 2^30.  The sorted
+// pairs end in (ka, va) after an even number of passes, in (kb, vb) after an
+// odd one.  *err (zeroed by the caller) is set if a tile's look-back wait
+// gives up, which would leave the order incomplete.
+size_t radix_sort_scratch_words(uint64_t n, int npasses);
+hipError_t radix_sort(uint32_t* ka, uint32_t* va, uint32_t* kb, uint32_t* vb, uint64_t n, int shift0, int npasses,
+                      uint32_t* scratch, uint32_t* err, hipStream_t stream);
 size_t scan_scratch_words(uint64_t n);
 hipError_t scan_exclusive(const uint32_t* in, uint32_t* out, uint64_t n, uint32_t* temp, hipStream_t stream);
 hipError_t fill_words(uint32_t* p, uint64_t n, uint32_t v, hipStream_t stream);

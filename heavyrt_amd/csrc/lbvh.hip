@@ -3,11 +3,12 @@
 // Stages (reference: core/builders/lbvh.cpp:732-827, BuildFromTriangles):
 //   1. scene box over all vertices            lbvh.cpp:228-239, 746
 //   2. 30-bit Morton codes per triangle       lbvh.cpp:25-51, 259-287
-//   3. stable LSD radix sort of (code, id)    lbvh.cpp:755 (order fixed to (code, id))
+//   3. stable LSD radix sort of (code, id)    lbvh.cpp:755 (order fixed to (code, id)); sort.hip
 //   4. unique-code runs (flag, scan, compact) lbvh.cpp:289-319, 762-767
 //   5. Karras 2012 hierarchy                  lbvh.cpp:386-441
-//   6. leaves + atomic bottom-up refit        lbvh.cpp:350-384, 443-489
-//   7. triangles re-laid out in leaf order    BVH2CommonRT_host.cpp:45-64
+//   6. leaves, with the triangles re-laid out lbvh.cpp:350-384,
+//      in leaf order                          BVH2CommonRT_host.cpp:45-64
+//   7. atomic bottom-up refit (refit.h)       lbvh.cpp:443-489
 //
 // Every integer result (codes, order, runs, children) is bit-identical to the
 // reference's; boxes are min/max unions, exact in any order.
@@ -19,17 +20,12 @@
 #include <new>
 
 #include "hrt_internal.h"
+#include "refit.h"
 
 namespace hrt {
 namespace {
 
 constexpr int kBlock = 256;
-
-// Radix sort tile: 4 waves x 16 rounds x 64 lanes.
-constexpr int kSortWaves = 4;
-constexpr int kSortRounds = 16;
-constexpr int kSortTile = kSortWaves * kSortRounds * 64;  // 4096 keys
-constexpr int kDigits = 256;
 
 // Scan tile: 256 threads x 8 items.
 constexpr int kScanItems = 8;
@@ -48,6 +44,7 @@ constexpr int kScanTile = kBlock * kScanItems;
 // reduced over, where a primitive's Morton point lies and what its box is.
 // ---------------------------------------------------------------------------
 struct TriSource {
+  static constexpr bool kTris = true;  // the leaf pass also writes leaf-order triangles
   const float4* v;
   const uint32_t* idx;
   uint32_t nverts;
@@ -66,6 +63,11 @@ struct TriSource {
                        (fminf(v1.y, fminf(v2.y, v3.y)) + fmaxf(v1.y, fmaxf(v2.y, v3.y))) * 0.5f,
                        (fminf(v1.z, fminf(v2.z, v3.z)) + fmaxf(v1.z, fmaxf(v2.z, v3.z))) * 0.5f);
   }
+  __device__ void corners(uint32_t t, float4& a, float4& b, float4& c) const {
+    a = v[idx[3 * t + 0]];
+    b = v[idx[3 * t + 1]];
+    c = v[idx[3 * t + 2]];
+  }
   __device__ void prim_box(uint32_t t, float3& lo, float3& hi) const {
     const float4 a = v[idx[3 * t + 0]], b = v[idx[3 * t + 1]], c = v[idx[3 * t + 2]];
     lo = make_float3(fminf(a.x, fminf(b.x, c.x)), fminf(a.y, fminf(b.y, c.y)), fminf(a.z, fminf(b.z, c.z)));
@@ -74,6 +76,7 @@ struct TriSource {
 };
 
 struct BoxSource {
+  static constexpr bool kTris = false;
   const float4* boxes;  // 2 per primitive: min, max
   uint32_t n;
   // scene box over the primitive boxes (lbvh.cpp:190-203)
@@ -205,86 +208,8 @@ __global__ __launch_bounds__(kBlock) void k_morton(Src src, uint32_t ntris, cons
 }
 
 // ---------------------------------------------------------------------------
-// 3. stable LSD radix sort, 8-bit digits
+// 3. stable LSD radix sort: sort.hip
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_radix_hist(const uint32_t* __restrict__ keys, uint32_t n, int shift,
-                                                       uint32_t* __restrict__ hist, uint32_t ntiles) {
-  __shared__ uint32_t cnt[kDigits];
-  cnt[threadIdx.x] = 0;
-  __syncthreads();
-  const uint64_t base = uint64_t(blockIdx.x) * kSortTile;
-#pragma unroll 4
-  for (int k = 0; k < kSortTile / kBlock; k++) {
-    const uint64_t i = base + uint64_t(k) * kBlock + threadIdx.x;
-    if (i < n) atomicAdd(&cnt[(keys[i] >> shift) & 0xFF], 1u);
-  }
-  __syncthreads();
-  hist[uint64_t(threadIdx.x) * ntiles + blockIdx.x] = cnt[threadIdx.x];
-}
-
-// Scatter with stable in-tile ranks.  Wave w owns keys [w*1024, w*1024+1024)
-// of the tile and walks them 64 at a time in index order; lanes holding the
-// same digit find each other with eight ballots, so the rank of a key among
-// equal digits follows its index -- the sort is stable, and sorting
-// (code, id) pairs whose ids start ascending yields the (code, id) order.
-__global__ __launch_bounds__(kBlock) void k_radix_scatter(const uint32_t* __restrict__ kin,
-                                                          const uint32_t* __restrict__ vin,
-                                                          uint32_t* __restrict__ kout, uint32_t* __restrict__ vout,
-                                                          uint32_t n, int shift,
-                                                          const uint32_t* __restrict__ hist_scan, uint32_t ntiles) {
-  __shared__ uint32_t wcnt[kSortWaves][kDigits];
-  const int lane = threadIdx.x & 63;
-  const int w = threadIdx.x >> 6;
-  for (int k = threadIdx.x; k < kSortWaves * kDigits; k += kBlock) (&wcnt[0][0])[k] = 0;
-  __syncthreads();
-  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
-  const uint64_t base = uint64_t(blockIdx.x) * kSortTile + uint64_t(w) * (kSortRounds * 64);
-  uint32_t key[kSortRounds], val[kSortRounds], pos[kSortRounds];
-#pragma unroll
-  for (int r = 0; r < kSortRounds; r++) {
-    const uint64_t i = base + uint64_t(r) * 64 + lane;
-    const bool valid = i < n;
-    key[r] = valid ? kin[i] : 0u;
-    val[r] = valid ? vin[i] : 0u;
-    const uint32_t digit = (key[r] >> shift) & 0xFF;
-    uint64_t peers = __ballot(valid);
-#pragma unroll
-    for (int b = 0; b < 8; b++) {
-      const bool bit = (digit >> b) & 1u;
-      const uint64_t m = __ballot(bit);
-      peers &= bit ? m : ~m;
-    }
-    const uint32_t rank = __popcll(peers & lt_mask);
-    const uint32_t cnt = __popcll(peers);
-    const uint32_t before = wcnt[w][digit];
-    __builtin_amdgcn_wave_barrier();
-    if (valid && rank == 0) wcnt[w][digit] = before + cnt;
-    __builtin_amdgcn_wave_barrier();
-    pos[r] = before + rank;
-  }
-  __syncthreads();
-  {
-    const int d = threadIdx.x;  // kBlock == kDigits
-    uint32_t run = hist_scan[uint64_t(d) * ntiles + blockIdx.x];
-#pragma unroll
-    for (int ww = 0; ww < kSortWaves; ww++) {
-      const uint32_t c = wcnt[ww][d];
-      wcnt[ww][d] = run;
-      run += c;
-    }
-  }
-  __syncthreads();
-#pragma unroll
-  for (int r = 0; r < kSortRounds; r++) {
-    const uint64_t i = base + uint64_t(r) * 64 + lane;
-    if (i < n) {
-      const uint32_t digit = (key[r] >> shift) & 0xFF;
-      const uint32_t p = wcnt[w][digit] + pos[r];
-      kout[p] = key[r];
-      vout[p] = val[r];
-    }
-  }
-}
 
 // ---------------------------------------------------------------------------
 // exclusive scan of u32 (three phases, recursive over block sums)
@@ -444,80 +369,64 @@ __global__ __launch_bounds__(kBlock) void k_karras(const uint32_t* __restrict__ 
 }
 
 // ---------------------------------------------------------------------------
-// 6. leaves + bottom-up refit
+// 6. leaves (and leaf-order triangles)
 // ---------------------------------------------------------------------------
 
-// Leaf boxes and the bottom-up refit.  Block b owns the leaves
-// [b * kRefitChunk, (b + 1) * kRefitChunk).  Every thread computes its leaf's
-// box, writes it into the parent's pair and climbs: at each internal node an
-// arrival counter tells the first child to stop and the second to merge the
-// two boxes and continue.  A node whose leaf range lies inside the block's
-// chunk only ever sees arrivals from this block, so its counter uses
-// workgroup-scope acquire/release; only nodes whose range crosses a chunk
-// boundary (the few ancestors of chunk seams) pay for agent-scope ordering,
-// which on gfx950 writes back and invalidates the L2 of the issuing XCD.
-#ifndef HRT_REFIT_CHUNK
-#define HRT_REFIT_CHUNK 4096
-#endif
-constexpr uint32_t kRefitChunk = HRT_REFIT_CHUNK;  // leaves per workgroup
-
+// One thread per leaf m: its box, the union of its run's primitive boxes
+// (lbvh.cpp:350-384), goes into its parent's pair, where the refit starts.
+// For triangles the same gathered vertices also give the leaf-order records
+// {A, origId}, {B - A}, {C - A} (BVH2CommonRT_host.cpp:45-64,
+// BVH2CommonRT.cpp:26-27) and the permutation (perm may be null: a refit
+// keeps it).  meta[1] gets the longest run.
 template <class Src>
-__global__ __launch_bounds__(kBlock) void k_leaves_refit(
-    Src src, const uint32_t* __restrict__ ids_sorted, const uint32_t* __restrict__ run_start,
-    uint32_t* __restrict__ meta, uint32_t ntris, NodePair* pairs, const uint32_t* __restrict__ parent_int,
-    const uint32_t* __restrict__ parent_leaf, const uint2* __restrict__ range, uint32_t* arrive) {
+__global__ __launch_bounds__(kBlock) void k_leaves(Src src, const uint32_t* __restrict__ ids_sorted,
+                                                   const uint32_t* __restrict__ run_start,
+                                                   uint32_t* __restrict__ meta, uint32_t ntris,
+                                                   NodePair* __restrict__ pairs,
+                                                   const uint32_t* __restrict__ parent_leaf,
+                                                   TriPre* __restrict__ tris, uint32_t* __restrict__ perm) {
   const uint32_t L = meta[0];
-  const uint32_t chunk_lo = blockIdx.x * kRefitChunk;
-  if (chunk_lo >= L) return;
-  const uint32_t chunk_hi = min(chunk_lo + kRefitChunk, L) - 1;
-  for (uint32_t m = chunk_lo + threadIdx.x; m <= chunk_hi; m += kBlock) {
-    const uint32_t s = run_start[m];
-    const uint32_t e = (m + 1 < L) ? run_start[m + 1] : ntris;
-    float3 lo = make_float3(kLbvhInf, kLbvhInf, kLbvhInf);
-    float3 hi = make_float3(-kLbvhInf, -kLbvhInf, -kLbvhInf);
-    for (uint32_t j = s; j < e; j++) {
-      float3 a, b;
-      src.prim_box(ids_sorted[j], a, b);
-      lo = make_float3(fminf(lo.x, a.x), fminf(lo.y, a.y), fminf(lo.z, a.z));
-      hi = make_float3(fmaxf(hi.x, b.x), fmaxf(hi.y, b.y), fmaxf(hi.z, b.z));
+  const uint32_t m = blockIdx.x * kBlock + threadIdx.x;
+  if (m >= L) return;
+  const uint32_t s = run_start[m];
+  const uint32_t e = (m + 1 < L) ? run_start[m + 1] : ntris;
+  float3 lo = make_float3(kLbvhInf, kLbvhInf, kLbvhInf);
+  float3 hi = make_float3(-kLbvhInf, -kLbvhInf, -kLbvhInf);
+  for (uint32_t j = s; j < e; j++) {
+    const uint32_t t = ids_sorted[j];
+    float3 a, b;
+    if constexpr (Src::kTris) {
+      float4 va, vb, vc;
+      src.corners(t, va, vb, vc);
+      TriPre r;
+      r.a = make_float4(va.x, va.y, va.z, __uint_as_float(t));
+      r.e1 = make_float4(vb.x - va.x, vb.y - va.y, vb.z - va.z, 0.0f);
+      r.e2 = make_float4(vc.x - va.x, vc.y - va.y, vc.z - va.z, 0.0f);
+      tris[j] = r;
+      if (perm) perm[j] = t;
+      a = make_float3(fminf(va.x, fminf(vb.x, vc.x)), fminf(va.y, fminf(vb.y, vc.y)), fminf(va.z, fminf(vb.z, vc.z)));
+      b = make_float3(fmaxf(va.x, fmaxf(vb.x, vc.x)), fmaxf(va.y, fmaxf(vb.y, vc.y)), fmaxf(va.z, fmaxf(vb.z, vc.z)));
+    } else {
+      src.prim_box(t, a, b);
     }
-    atomicMax(&meta[1], e - s);
-    if (L == 1) {
-      // Root is a leaf: pair 0 = (root leaf, empty dummy), as the reference's
-      // peach layout stores it (lbvh.cpp:497-504, cbvh.h:28-36).
-      store_box(pairs, 0, 0, lo, hi);
-      store_box(pairs, 0, 1, make_float3(0.f, 0.f, 0.f), make_float3(0.f, 0.f, 0.f));
-      uint32_t* pw = reinterpret_cast<uint32_t*>(pairs);
-      pw[3] = pack_leaf(s, e - s);
-      pw[7] = 0u;
-      pw[11] = 0x80000000u;
-      pw[15] = 0u;
-      return;
-    }
-    const uint32_t pl = parent_leaf[m];
-    store_box(pairs, pl >> 1, pl & 1u, lo, hi);
-    uint32_t p = pl >> 1;
-    while (true) {
-      const uint2 r = range[p];
-      const bool crossing = r.x < chunk_lo || r.y > chunk_hi;
-      // release our box, then count arrivals; the second child to arrive owns the parent
-      const uint32_t old = crossing
-          ? __hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT)
-          : __hip_atomic_fetch_add(&arrive[p], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
-      if (old == 0) break;
-      if (p == 0) break;  // root box is not stored: traversal starts at the root's children
-      const uint32_t* pw = reinterpret_cast<const uint32_t*>(pairs + p);
-      const float3 l0 = make_float3(__uint_as_float(pw[0]), __uint_as_float(pw[1]), __uint_as_float(pw[2]));
-      const float3 h0 = make_float3(__uint_as_float(pw[4]), __uint_as_float(pw[5]), __uint_as_float(pw[6]));
-      const float3 l1 = make_float3(__uint_as_float(pw[8]), __uint_as_float(pw[9]), __uint_as_float(pw[10]));
-      const float3 h1 = make_float3(__uint_as_float(pw[12]), __uint_as_float(pw[13]), __uint_as_float(pw[14]));
-      const float3 ul = make_float3(fminf(l0.x, l1.x), fminf(l0.y, l1.y), fminf(l0.z, l1.z));
-      const float3 uh = make_float3(fmaxf(h0.x, h1.x), fmaxf(h0.y, h1.y), fmaxf(h0.z, h1.z));
-      const uint32_t pp = parent_int[p];
-      store_box(pairs, pp >> 1, pp & 1u, ul, uh);
-      p = pp >> 1;
-    }
+    lo = make_float3(fminf(lo.x, a.x), fminf(lo.y, a.y), fminf(lo.z, a.z));
+    hi = make_float3(fmaxf(hi.x, b.x), fmaxf(hi.y, b.y), fmaxf(hi.z, b.z));
   }
+  atomicMax(&meta[1], e - s);  // one address: combined across the wave
+  if (L == 1) {
+    // Root is a leaf: pair 0 = (root leaf, empty dummy), as the reference's
+    // peach layout stores it (lbvh.cpp:497-504, cbvh.h:28-36).
+    store_box(pairs, 0, 0, lo, hi);
+    store_box(pairs, 0, 1, make_float3(0.f, 0.f, 0.f), make_float3(0.f, 0.f, 0.f));
+    uint32_t* pw = reinterpret_cast<uint32_t*>(pairs);
+    pw[3] = pack_leaf(s, e - s);
+    pw[7] = 0u;
+    pw[11] = 0x80000000u;
+    pw[15] = 0u;
+    return;
+  }
+  const uint32_t pl = parent_leaf[m];
+  store_box(pairs, pl >> 1, pl & 1u, lo, hi);
 }
 
 // ---------------------------------------------------------------------------
@@ -598,24 +507,6 @@ __global__ __launch_bounds__(kBlock) void k_collapse4(const NodePair* __restrict
   q16[q] = h;
 }
 
-// ---------------------------------------------------------------------------
-// 7. triangles in leaf order
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_tris(const float4* __restrict__ v, const uint32_t* __restrict__ idx,
-                                                 const uint32_t* __restrict__ ids_sorted, uint32_t ntris,
-                                                 TriPre* __restrict__ tris, uint32_t* __restrict__ perm) {
-  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-  if (j >= ntris) return;
-  const uint32_t t = ids_sorted[j];
-  const float4 a = v[idx[3 * t + 0]], b = v[idx[3 * t + 1]], c = v[idx[3 * t + 2]];
-  TriPre r;
-  r.a = make_float4(a.x, a.y, a.z, __uint_as_float(t));
-  r.e1 = make_float4(b.x - a.x, b.y - a.y, b.z - a.z, 0.0f);
-  r.e2 = make_float4(c.x - a.x, c.y - a.y, c.z - a.z, 0.0f);
-  tris[j] = r;
-  if (perm) perm[j] = t;
-}
-
 inline uint32_t blocks_for(uint64_t n) { return uint32_t((n + kBlock - 1) / kBlock); }
 
 }  // namespace
@@ -678,22 +569,23 @@ template <class Src, class Finish>
 hipError_t lbvh_build(Blas& b, const Src& src, uint32_t box_items, hipStream_t s, Finish&& finish, int cubic = 0) {
   const uint64_t N = b.ntris;
   if (N == 0) return hipErrorInvalidValue;
-  const uint32_t ntiles = uint32_t((N + kSortTile - 1) / kSortTile);
-  const uint64_t hist_words = uint64_t(kDigits) * ntiles;
   const int box_parts = int(std::min<uint64_t>(1024, std::max<uint64_t>(1, (box_items + kBlock - 1) / kBlock)));
   // scratch layout (u32 words)
-  const size_t w_codes = N, w_hist = hist_words, w_scan = scan_temp_words(std::max<uint64_t>(N, hist_words));
+  const size_t w_codes = N, w_sort = radix_sort_scratch_words(N, 4), w_scan = scan_temp_words(N);
   size_t words = 0;
   auto take = [&](size_t n) { size_t o = words; words += (n + 63) & ~size_t(63); return o; };
   const size_t o_ka = take(w_codes), o_va = take(w_codes), o_kb = take(w_codes), o_vb = take(w_codes);
-  const size_t o_hist = take(w_hist), o_hscan = take(w_hist), o_scan = take(w_scan);
+  const size_t o_sort = take(w_sort), o_scan = take(w_scan);
   const size_t o_flags = take(N), o_pos = take(N), o_uc = take(N), o_arr = take(N);
+  const uint32_t tops_cap = refit_tops_capacity(N);
+  const size_t o_tops = take(tops_cap);
   const size_t o_part = take(size_t(box_parts) * 8), o_box = take(16);
   uint32_t* base = nullptr;
   HRT_CHECK(hipMallocAsync(reinterpret_cast<void**>(&base), words * 4, s));
   uint32_t *ka = base + o_ka, *va = base + o_va, *kb = base + o_kb, *vb = base + o_vb;
-  uint32_t *hist = base + o_hist, *hscan = base + o_hscan, *stmp = base + o_scan;
+  uint32_t *sort_tmp = base + o_sort, *stmp = base + o_scan;
   uint32_t *flags = base + o_flags, *pos = base + o_pos, *uc = base + o_uc, *arrive = base + o_arr;
+  uint32_t* tops = base + o_tops;
   // runs, leaf parents, node ranges and the leaf count stay with the tree (refit)
   uint32_t *rs = b.leaf_run, *pleaf = b.parent_leaf, *meta = b.meta;
   uint2* range = b.node_range;
@@ -702,31 +594,31 @@ hipError_t lbvh_build(Blas& b, const Src& src, uint32_t box_items, hipStream_t s
 
   hipError_t err = hipSuccess;
   do {
+    // meta: [0] leaves, [1] longest run, [2] refit top entries, [3] refit
+    // overflow, [4] sort look-back timeout; flags stay zero unless a check fails
+    k_fill<<<1, kBlock, 0, s>>>(meta, 64, 0u);
     k_box_partial<Src><<<box_parts, kBlock, 0, s>>>(src, part);
     k_box_final<<<1, kBlock, 0, s>>>(part, box_parts, box, box + 8, cubic);
     k_morton<Src><<<blocks_for(N), kBlock, 0, s>>>(src, uint32_t(N), box, box + 8, ka, va);
     if ((err = hipGetLastError()) != hipSuccess) break;
     if (b.keep_debug)
       if ((err = hipMemcpyAsync(b.codes_in, ka, 4 * N, hipMemcpyDeviceToDevice, s)) != hipSuccess) break;
-    uint32_t *kin = ka, *vin = va, *kout = kb, *vout = vb;
-    for (int shift = 0; shift < 32; shift += 8) {
-      k_radix_hist<<<ntiles, kBlock, 0, s>>>(kin, uint32_t(N), shift, hist, ntiles);
-      if ((err = exclusive_scan(hist, hscan, hist_words, stmp, s)) != hipSuccess) break;
-      k_radix_scatter<<<ntiles, kBlock, 0, s>>>(kin, vin, kout, vout, uint32_t(N), shift, hscan, ntiles);
-      std::swap(kin, kout);
-      std::swap(vin, vout);
-    }
-    if (err != hipSuccess || (err = hipGetLastError()) != hipSuccess) break;
-    // after four passes the sorted keys are back in (ka, va)
+    // four 8-bit passes: the sorted pairs end in (ka, va)
+    if ((err = radix_sort(ka, va, kb, vb, N, 0, 4, sort_tmp, meta + 4, s)) != hipSuccess) break;
+    uint32_t *kin = ka, *vin = va;
     k_run_flags<<<blocks_for(N), kBlock, 0, s>>>(kin, uint32_t(N), flags);
     if ((err = exclusive_scan(flags, pos, N, stmp, s)) != hipSuccess) break;
-    k_fill<<<1, kBlock, 0, s>>>(meta, 64, 0u);
     k_run_compact<<<blocks_for(N), kBlock, 0, s>>>(kin, flags, pos, uint32_t(N), rs, uc, meta);
     k_fill<<<fill_blocks(N), kBlock, 0, s>>>(arrive, N, 0u);
     k_karras<<<blocks_for(N), kBlock, 0, s>>>(uc, rs, meta, uint32_t(N), b.pairs, b.parent, pleaf, range,
                                               b.karras_lr);
-    k_leaves_refit<Src><<<uint32_t((N + kRefitChunk - 1) / kRefitChunk), kBlock, 0, s>>>(
-        src, vin, rs, meta, uint32_t(N), b.pairs, b.parent, pleaf, range, arrive);
+    k_leaves<Src><<<blocks_for(N), kBlock, 0, s>>>(src, vin, rs, meta, uint32_t(N), b.pairs, pleaf, b.tris,
+                                                   b.perm);
+    // meta[2] counts the refit's top entries, meta[3] flags an overflow (both zeroed above)
+    const OneTree tree{meta, tops_cap};
+    k_refit_chunks<OneTree><<<uint32_t((N + kRefitChunk - 1) / kRefitChunk), kRefitThreads, 0, s>>>(
+        tree, b.pairs, b.parent, pleaf, range, tops, meta + 2, meta + 3);
+    k_refit_tops<OneTree><<<1, kRefitTopsThreads, 0, s>>>(tree, b.pairs, b.parent, tops, meta + 2, arrive);
     finish(vin);
     if ((err = hipGetLastError()) != hipSuccess) break;
     if (b.keep_debug) {
@@ -735,11 +627,15 @@ hipError_t lbvh_build(Blas& b, const Src& src, uint32_t box_items, hipStream_t s
       if ((err = hipMemcpyAsync(b.run_start, rs, 4 * N, hipMemcpyDeviceToDevice, s)) != hipSuccess) break;
       if ((err = hipMemcpyAsync(b.ucodes, uc, 4 * N, hipMemcpyDeviceToDevice, s)) != hipSuccess) break;
     }
-    uint32_t hmeta[2];
+    uint32_t hmeta[5];
     float hbox[12];
     if ((err = hipMemcpyAsync(hmeta, meta, sizeof(hmeta), hipMemcpyDeviceToHost, s)) != hipSuccess) break;
     if ((err = hipMemcpyAsync(hbox, box, sizeof(hbox), hipMemcpyDeviceToHost, s)) != hipSuccess) break;
     if ((err = hipStreamSynchronize(s)) != hipSuccess) break;
+    if (hmeta[3] != 0u || hmeta[4] != 0u) {  // a device-side check failed: the tree is incomplete
+      err = hipErrorIllegalState;
+      break;
+    }
     b.leaves = hmeta[0];
     b.max_run = hmeta[1];
     std::memcpy(b.box, hbox, sizeof(b.box));
@@ -766,13 +662,8 @@ __global__ __launch_bounds__(kBlock) void k_inst_order(const InstDev* __restrict
 
 hipError_t blas_build(Blas& b, hipStream_t s, bool cubic) {
   const TriSource src{b.verts, b.idx, b.nverts};
-  const uint32_t N = b.ntris;
-  return lbvh_build(
-      b, src, b.nverts, s,
-      [&](const uint32_t* ids_sorted) {
-        k_tris<<<blocks_for(N), kBlock, 0, s>>>(b.verts, b.idx, ids_sorted, N, b.tris, b.perm);
-      },
-      cubic ? 1 : 0);
+  // the leaf pass already wrote the leaf-order triangles
+  return lbvh_build(b, src, b.nverts, s, [](const uint32_t*) {}, cubic ? 1 : 0);
 }
 
 hipError_t blas_refit(Blas& b, hipStream_t s) {
@@ -781,23 +672,34 @@ hipError_t blas_refit(Blas& b, hipStream_t s) {
   const TriSource src{b.verts, b.idx, b.nverts};
   const int box_parts = int(std::min<uint64_t>(1024, std::max<uint64_t>(1, (b.nverts + kBlock - 1) / kBlock)));
   uint32_t* base = nullptr;
-  const size_t o_part = (N + 63) & ~size_t(63);
+  const uint32_t tops_cap = refit_tops_capacity(N);
+  const size_t o_tops = (N + 63) & ~size_t(63);
+  const size_t o_part = o_tops + ((tops_cap + 63) & ~size_t(63));
   HRT_CHECK(hipMallocAsync(reinterpret_cast<void**>(&base), 4 * (o_part + 8 * size_t(box_parts) + 16), s));
   uint32_t* arrive = base;
+  uint32_t* tops = base + o_tops;
   float4* part = reinterpret_cast<float4*>(base + o_part);
   float* box = reinterpret_cast<float*>(base + o_part + 8 * size_t(box_parts));
   k_box_partial<TriSource><<<box_parts, kBlock, 0, s>>>(src, part);
   k_box_final<<<1, kBlock, 0, s>>>(part, box_parts, box, box + 8, 0);
   k_fill<<<fill_blocks(N), kBlock, 0, s>>>(arrive, N, 0u);
-  k_leaves_refit<TriSource><<<uint32_t((N + kRefitChunk - 1) / kRefitChunk), kBlock, 0, s>>>(
-      src, b.perm, b.leaf_run, b.meta, uint32_t(N), b.pairs, b.parent, b.parent_leaf, b.node_range, arrive);
-  k_tris<<<blocks_for(N), kBlock, 0, s>>>(b.verts, b.idx, b.perm, uint32_t(N), b.tris, nullptr);
+  k_fill<<<1, kBlock, 0, s>>>(b.meta + 2, 2, 0u);  // top-entry count, overflow flag
+  // the kept permutation gives the leaf order; it is not rewritten
+  k_leaves<TriSource><<<blocks_for(N), kBlock, 0, s>>>(src, b.perm, b.leaf_run, b.meta, uint32_t(N), b.pairs,
+                                                       b.parent_leaf, b.tris, nullptr);
+  const OneTree tree{b.meta, tops_cap};
+  k_refit_chunks<OneTree><<<uint32_t((N + kRefitChunk - 1) / kRefitChunk), kRefitThreads, 0, s>>>(
+      tree, b.pairs, b.parent, b.parent_leaf, b.node_range, tops, b.meta + 2, b.meta + 3);
+  k_refit_tops<OneTree><<<1, kRefitTopsThreads, 0, s>>>(tree, b.pairs, b.parent, tops, b.meta + 2, arrive);
   hipError_t err = hipGetLastError();
   float hbox[8];
+  uint32_t hmeta[4];
   if (err == hipSuccess) err = hipMemcpyAsync(hbox, box, sizeof(hbox), hipMemcpyDeviceToHost, s);
+  if (err == hipSuccess) err = hipMemcpyAsync(hmeta, b.meta, sizeof(hmeta), hipMemcpyDeviceToHost, s);
   hipError_t ferr = hipFreeAsync(base, s);
   if (err == hipSuccess) err = ferr;
   if (err == hipSuccess) err = hipStreamSynchronize(s);
+  if (err == hipSuccess && hmeta[3] != 0u) err = hipErrorIllegalState;  // a refit list overflowed
   if (err != hipSuccess) return err;
   std::memcpy(b.box, hbox, sizeof(b.box));
   // boxes changed: the BVH4 nodes and the any-hit tree are made again on the next commit
@@ -889,26 +791,6 @@ hipError_t blas_build_bvh4(Blas& b, hipStream_t s) {
 // allocation, no memset, no synchronisation).
 // ---------------------------------------------------------------------------
 namespace hrt {
-
-size_t radix_pass_scratch_words(uint64_t n) {
-  const uint64_t ntiles = (n + kSortTile - 1) / kSortTile;
-  const uint64_t hist_words = uint64_t(kDigits) * ntiles;
-  return size_t(2 * hist_words + scan_temp_words(hist_words) + 128);
-}
-
-hipError_t radix_pass(const uint32_t* kin, const uint32_t* vin, uint32_t* kout, uint32_t* vout, uint64_t n, int shift,
-                      uint32_t* scratch, hipStream_t s) {
-  if (n == 0) return hipSuccess;
-  const uint32_t ntiles = uint32_t((n + kSortTile - 1) / kSortTile);
-  const uint64_t hist_words = uint64_t(kDigits) * ntiles;
-  uint32_t* hist = scratch;
-  uint32_t* hscan = scratch + ((hist_words + 63) & ~uint64_t(63));
-  uint32_t* stmp = hscan + ((hist_words + 63) & ~uint64_t(63));
-  k_radix_hist<<<ntiles, kBlock, 0, s>>>(kin, uint32_t(n), shift, hist, ntiles);
-  HRT_CHECK(exclusive_scan(hist, hscan, hist_words, stmp, s));
-  k_radix_scatter<<<ntiles, kBlock, 0, s>>>(kin, vin, kout, vout, uint32_t(n), shift, hscan, ntiles);
-  return hipGetLastError();
-}
 
 size_t scan_scratch_words(uint64_t n) { return scan_temp_words(n); }
 
