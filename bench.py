@@ -488,15 +488,18 @@ def _line(value, unit, ms, steps, warmup, workload, extra):
 def bench_eye(args) -> int:
     """The CPU-runnable eye-ray configuration: 512x512 pinhole rays, nearest
     hit on the LBVH2 of an icosphere (6 subdivisions, 81,920 tris) on a
-    2-triangle ground quad; primId and t per pixel, bit-exact with the oracle."""
+    2-triangle ground quad; primId and t per pixel, bit-exact with the oracle.
+    The rays are made on the device in the renderer's 8x8-superblock order
+    (hrt_eye_make_rays); the whole frame (rays, trace, colours) is timed too."""
     import torch
 
     import heavyrt_amd as H
     from heavyrt_amd import workloads as W
 
     v, f = W.eye_scene()
-    rays = W.eye_rays()
-    n = rays.shape[0]
+    width = height = 512
+    cam = W.eye_camera(width, height)
+    n = width * height
     ctx = H.Context(0)
     stream = torch.cuda.Stream()
     torch.cuda.set_stream(stream)
@@ -507,7 +510,10 @@ def bench_eye(args) -> int:
     sc.add_instance(sc.add_triangles(v, f))
     sc.commit()
     build_ms = (time.perf_counter() - t0) * 1e3
-    d_rays = torch.from_numpy(rays).cuda()
+    d_rays = torch.empty((n, 8), dtype=torch.float32, device="cuda")
+    ctx.eye_make_rays(cam[0], cam[1], width, height, d_rays, stream=sptr)
+    torch.cuda.synchronize()
+    rays = d_rays.cpu().numpy()
     d_hits = torch.empty((n, 2), dtype=torch.int32, device="cuda")
     d_st = torch.empty((n, 2), dtype=torch.int32, device="cuda")
     for _ in range(args.warmup):
@@ -519,8 +525,22 @@ def bench_eye(args) -> int:
     st = d_st.cpu().numpy().view(np.uint32).astype(np.int64)
     nbytes = float(np.sum(40 + 64 * st[:, 0] + 60 * st[:, 1]))
     mean = float(ms.mean())
+    # the whole frame: rays, trace, colours (hrt_eye_render)
+    d_color = torch.empty(n, dtype=torch.int32, device="cuda")
+    frame = lambda: sc.eye_render(cam[0], cam[1], width, height, W.EYE_PALETTE, d_color, stream=sptr)
+    frame()
+    frame_ms = _timed(stream, frame, args.steps)
+    # the same rays in row-major pixel order, for the effect of the tile order
+    xy_rows = torch.empty(n, dtype=torch.int32, device="cuda")
+    ctx.eye_make_rays(cam[0], cam[1], width, height, d_rays, xy_rows, stream=sptr)
+    torch.cuda.synchronize()
+    xy = xy_rows.cpu().numpy().view(np.uint32)
+    order = np.argsort((xy >> 16).astype(np.int64) * width + (xy & 0xFFFF))
+    d_rows = torch.from_numpy(rays[order]).cuda()
+    rows_ms = _timed(stream, lambda: sc.trace_nearest(d_rows, n, d_hits, H.HIT_COMPACT, stream=sptr), args.steps)
     result = _line(round(n / (mean * 1e-3) / 1e6, 2), "Mrays/s", round(mean, 4), args.steps, args.warmup,
-                   "eye: 512x512 pinhole rays, nearest hit, LBVH2 of an icosphere (81,920 tris) + ground quad",
+                   "eye: 512x512 pinhole rays (made on the device in 8x8 superblock order), nearest hit, "
+                   "LBVH2 of an icosphere (81,920 tris) + ground quad",
                    {"roofline": {"bound": "hbm", "kernel": "nearest-hit traversal (BVH2, reference order)",
                                  "achieved": round(nbytes / (mean * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBPS,
                                  "unit": "GB/s", "frac": round(nbytes / (mean * 1e-3) / 1e9 / HBM_PEAK_GBPS, 4),
@@ -528,13 +548,17 @@ def bench_eye(args) -> int:
                     "latency_ms": {"p50": round(float(np.percentile(ms, 50)), 4),
                                    "p99": round(float(np.percentile(ms, 99)), 4)},
                     "build_ms": round(build_ms, 3), "triangles": int(f.shape[0]), "rays": n,
-                    "visits_per_ray": round(float(st[:, 0].mean()), 3), "cpu_baseline": None})
+                    "visits_per_ray": round(float(st[:, 0].mean()), 3),
+                    "frame_ms": round(float(frame_ms.mean()), 4),
+                    "row_major_order_ms": round(float(rows_ms.mean()), 4), "cpu_baseline": None})
     from oracle import oracle as O
 
     threads = args.cpu_threads or min(16, os.cpu_count() or 1)
     ob = O.OracleBVH(v, f)
+    ref_rays, _ = O.eye_rays(cam[0], cam[1], width, height)
+    result["rays_bit_exact_vs_oracle"] = bool(np.array_equal(ref_rays.view(np.uint32), rays.view(np.uint32)))
     t0 = time.perf_counter()
-    t, p, _, _ = ob.trace(rays, threads=threads)
+    t, p, _, _ = ob.trace(ref_rays, threads=threads)
     cpu_s = time.perf_counter() - t0
     result["cpu_baseline"] = {"value": round(n / cpu_s / 1e6, 3), "unit": "Mrays/s", "cores": threads,
                               "kind": "port", "sample": f"the full 512x512 frame, {cpu_s * 1e3:.1f} ms"}

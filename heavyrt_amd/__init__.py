@@ -62,7 +62,7 @@ EXPORTS = [
     "hrt_scene_set_keep_build_arrays", "hrt_scene_clear_geom", "hrt_scene_clear_instances",
     "hrt_add_triangles", "hrt_add_instance", "hrt_commit", "hrt_trace_nearest", "hrt_trace_any",
     "hrt_trace_nearest_stats", "hrt_trace_any_stats", "hrt_query_nearest_one", "hrt_geom_count",
-    "hrt_instance_count", "hrt_debug_dump", "hrt_last_error", "hrt_rtao_make_rays",
+    "hrt_instance_count", "hrt_debug_dump", "hrt_last_error", "hrt_rtao_make_rays", "hrt_eye_make_rays", "hrt_eye_render",
     "hrt_trace_nearest_host", "hrt_trace_any_host", "hrt_update_triangles", "hrt_update_instance",
     "hrt_batch_create", "hrt_batch_destroy", "hrt_batch_buffers", "hrt_batch_run", "hrt_batch_build",
     "hrt_batch_trace", "hrt_batch_candidate_count", "hrt_batch_candidate_info", "hrt_batch_set_candidate",
@@ -132,6 +132,8 @@ def lib():
         L.hrt_instance_count.restype = u32
         L.hrt_debug_dump.argtypes = [vp, u32, i32, vp, sz]
         L.hrt_rtao_make_rays.argtypes = [vp, vp, vp, sz, u32, C.c_float, C.c_uint64, vp, vp]
+        L.hrt_eye_make_rays.argtypes = [vp, vp, vp, u32, u32, vp, vp, vp]
+        L.hrt_eye_render.argtypes = [vp, vp, vp, u32, u32, vp, u32, vp, vp]
         L.hrt_trace_nearest_host.argtypes = [vp, vp, sz, vp, vp]
         L.hrt_trace_any_host.argtypes = [vp, vp, sz, vp]
         L.hrt_update_triangles.argtypes = [vp, u32, vp, sz, sz, vp, sz, i32]
@@ -192,6 +194,15 @@ class Context:
     @property
     def stream(self) -> int:
         return int(lib().hrt_ctx_stream(self._h) or 0)
+
+    def eye_make_rays(self, proj_inv, world_view_inv, width: int, height: int, out_rays, out_xy=None,
+                      stream: int = 0):
+        """Eye rays of a pinhole camera in the 01_eye_rays renderer's superblock order
+        (hrt_eye_make_rays); matrices are 16 column-major floats."""
+        pi = np.ascontiguousarray(proj_inv, dtype=np.float32).reshape(16)
+        wi = np.ascontiguousarray(world_view_inv, dtype=np.float32).reshape(16)
+        _check(lib().hrt_eye_make_rays(self._h, _ptr(pi), _ptr(wi), width, height, _dev_ptr(out_rays),
+                                       _dev_ptr(out_xy) if out_xy is not None else None, stream or None))
 
     def close(self):
         """Destroys the context, and first every scene still open on it (a scene
@@ -306,6 +317,15 @@ class Scene:
                        stream: int = 0):
         _check(lib().hrt_rtao_make_rays(self._h, _dev_ptr(prim_rays), _dev_ptr(prim_hits_compact), npix, spp,
                                         float(tmax), seed & 0xFFFFFFFFFFFFFFFF, _dev_ptr(out_rays), stream or None))
+
+    def eye_render(self, proj_inv, world_view_inv, width: int, height: int, palette, out_color, stream: int = 0):
+        """The 01_eye_rays frame on the device (hrt_eye_render): out_color is a device buffer of
+        width * height words."""
+        pi = np.ascontiguousarray(proj_inv, dtype=np.float32).reshape(16)
+        wi = np.ascontiguousarray(world_view_inv, dtype=np.float32).reshape(16)
+        pal = np.ascontiguousarray(palette, dtype=np.uint32).reshape(-1)
+        _check(lib().hrt_eye_render(self._h, _ptr(pi), _ptr(wi), width, height, _ptr(pal), pal.size,
+                                    _dev_ptr(out_color), stream or None))
 
     # -- parity hooks -------------------------------------------------------
     def dump(self, geom_id: int, what: int) -> np.ndarray:

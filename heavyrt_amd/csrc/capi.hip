@@ -616,6 +616,62 @@ hrt_status hrt_rtao_make_rays(hrt_scene* s, const float* primaryRays, const void
   return HRT_OK;
 }
 
+namespace {
+hrt_status eye_args(const float* proj_inv, const float* world_view_inv, uint32_t width, uint32_t height,
+                    hrt::EyeCamera* cam) {
+  if (!proj_inv || !world_view_inv) return fail(HRT_ERR_INVALID_ARG, "camera matrix is null");
+  if (width == 0 || height == 0 || (width & 7u) || (height & 7u) || width > 0xFFFFu || height > 0xFFFFu)
+    return fail(HRT_ERR_INVALID_ARG, "width and height must be multiples of 8, at most 65535 (8x8 pixel tiles)");
+  std::memcpy(cam->proj_inv.m, proj_inv, sizeof(cam->proj_inv.m));
+  std::memcpy(cam->world_view_inv.m, world_view_inv, sizeof(cam->world_view_inv.m));
+  return HRT_OK;
+}
+}  // namespace
+
+hrt_status hrt_eye_make_rays(hrt_ctx* ctx, const float projInvColMajor16[16], const float worldViewInvColMajor16[16],
+                             uint32_t width, uint32_t height, float* outRays, uint32_t* outPackedXY, void* stream) {
+  if (!ctx || !outRays) return fail(HRT_ERR_INVALID_ARG, "null argument");
+  hrt::EyeCamera cam;
+  hrt_status st = eye_args(projInvColMajor16, worldViewInvColMajor16, width, height, &cam);
+  if (st != HRT_OK) return st;
+  hipStream_t hs = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e == hipSuccess) e = hrt::eye_make_rays(cam, width, height, reinterpret_cast<hrt::Ray*>(outRays), outPackedXY, hs);
+  if (e != hipSuccess) return fail_hip(e, "eye rays");
+  return HRT_OK;
+}
+
+hrt_status hrt_eye_render(hrt_scene* s, const float projInvColMajor16[16], const float worldViewInvColMajor16[16],
+                          uint32_t width, uint32_t height, const uint32_t* palette, uint32_t paletteSize,
+                          uint32_t* outColor, void* stream) {
+  if (!s || !outColor || !palette) return fail(HRT_ERR_INVALID_ARG, "null argument");
+  if (paletteSize == 0 || paletteSize > hrt::kEyePaletteMax)
+    return fail(HRT_ERR_INVALID_ARG, "the palette holds 1 to 64 colours");
+  if (!s->committed) return fail(HRT_ERR_NOT_COMMITTED, "CommitScene has not been called since the last change");
+  hrt::EyeCamera cam;
+  hrt_status st = eye_args(projInvColMajor16, worldViewInvColMajor16, width, height, &cam);
+  if (st != HRT_OK) return st;
+  hrt::EyePalette pal{};
+  std::memcpy(pal.c, palette, 4 * size_t(paletteSize));
+  pal.n = paletteSize;
+  const size_t n = size_t(width) * height;
+  hipStream_t hs = stream ? static_cast<hipStream_t>(stream) : s->ctx->stream;
+  hipError_t e = hipSetDevice(s->ctx->device);
+  if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
+  void* buf = nullptr;
+  if ((e = hipMallocAsync(&buf, 40 * n, hs)) != hipSuccess) return fail_hip(e, "eye render buffers");
+  float* rays = static_cast<float*>(buf);
+  uint32_t* hits = reinterpret_cast<uint32_t*>(static_cast<char*>(buf) + 32 * n);
+  e = hrt::eye_make_rays(cam, width, height, reinterpret_cast<hrt::Ray*>(rays), nullptr, hs);
+  st = e == hipSuccess ? trace_common(s, rays, n, hits, HRT_HIT_COMPACT, 0, nullptr, stream) : HRT_OK;
+  if (e == hipSuccess && st == HRT_OK) e = hrt::eye_colors(hits, width, height, pal, outColor, hs);
+  const hipError_t fe = hipFreeAsync(buf, hs);
+  if (st != HRT_OK) return st;
+  if (e == hipSuccess) e = fe;
+  if (e != hipSuccess) return fail_hip(e, "eye render");
+  return HRT_OK;
+}
+
 uint32_t hrt_geom_count(const hrt_scene* s) { return s ? uint32_t(s->geoms.size()) : 0u; }
 uint32_t hrt_instance_count(const hrt_scene* s) { return s ? uint32_t(s->instances.size()) : 0u; }
 

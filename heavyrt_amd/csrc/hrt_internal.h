@@ -138,4 +138,23 @@ hipError_t trace_two_level(const Blas& tlas, const InstDev* inst_leaf_order, con
 hipError_t rtao_make_rays(const Blas& b, const Ray* prim_rays, const void* prim_hits_compact, uint64_t npix,
                           uint32_t spp, float tmax, uint64_t seed, Ray* out, hipStream_t stream);
 
+// Eye rays of a pinhole camera, one per pixel, in SuperBlockIndex2DOpt order
+// (width and height multiples of 8); packed_xy (nullable) gets y << 16 | x.
+struct EyeCamera {
+  struct Mat {
+    float m[16];  // column-major
+  };
+  Mat proj_inv, world_view_inv;
+};
+constexpr uint32_t kEyePaletteMax = 64;
+struct EyePalette {
+  uint32_t c[kEyePaletteMax];
+  uint32_t n;
+};
+hipError_t eye_make_rays(const EyeCamera& cam, uint32_t width, uint32_t height, Ray* out, uint32_t* packed_xy,
+                         hipStream_t stream);
+// Pixel colours from compact hits of those rays (row-major image).
+hipError_t eye_colors(const void* hits_compact, uint32_t width, uint32_t height, const EyePalette& pal,
+                      uint32_t* out_color, hipStream_t stream);
+
 }  // namespace hrt

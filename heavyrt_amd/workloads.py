@@ -277,6 +277,47 @@ def eye_rays(width: int = 512, height: int = 512) -> np.ndarray:
     return pinhole_rays(width, height, eye=(0.0, 1.5, 4.0), target=(0.0, 1.0, 0.0), fov_y_deg=45.0)
 
 
+EYE_POS = (0.0, 1.5, 4.0)
+EYE_TARGET = (0.0, 1.0, 0.0)
+
+# EyeRayCaster::m_palette (examples/01_eye_rays/eye_ray.h:106-113)
+EYE_PALETTE = np.array([0xffe6194b, 0xff3cb44b, 0xffffe119, 0xff0082c8, 0xfff58231, 0xff911eb4, 0xff46f0f0,
+                        0xfff032e6, 0xffd2f53c, 0xfffabebe, 0xff008080, 0xffe6beff, 0xffaa6e28, 0xfffffac8,
+                        0xff800000, 0xffaaffc3, 0xff808000, 0xffffd8b1, 0xff000080, 0xff808080], dtype=np.uint32)
+
+
+def perspective(fov_y_deg: float, aspect: float, znear: float, zfar: float) -> np.ndarray:
+    """OpenGL-style projection (the role of LiteMath::perspectiveMatrix, eye_ray_host.cpp:314), float64."""
+    f = 1.0 / math.tan(math.radians(fov_y_deg) * 0.5)
+    return np.array([[f / aspect, 0, 0, 0], [0, f, 0, 0],
+                     [0, 0, (zfar + znear) / (znear - zfar), 2 * zfar * znear / (znear - zfar)],
+                     [0, 0, -1, 0]], dtype=np.float64)
+
+
+def look_at(eye, target, up=(0.0, 1.0, 0.0)) -> np.ndarray:
+    """World -> view matrix looking down -z (the role of LiteMath::lookAt), float64."""
+    eye = np.asarray(eye, dtype=np.float64)
+    z = eye - np.asarray(target, dtype=np.float64)
+    z /= np.linalg.norm(z)
+    x = np.cross(np.asarray(up, dtype=np.float64), z)
+    x /= np.linalg.norm(x)
+    y = np.cross(z, x)
+    m = np.eye(4)
+    m[0, :3], m[1, :3], m[2, :3] = x, y, z
+    m[:3, 3] = -m[:3, :3] @ eye
+    return m
+
+
+def eye_camera(width: int = 512, height: int = 512, eye=EYE_POS, target=EYE_TARGET, fov_y_deg: float = 45.0):
+    """(inverse projection, inverse world-view) as 16 column-major float32 each, as
+    EyeRayCaster keeps them in m_projInv / m_worldViewInv (eye_ray_host.cpp:312-318)."""
+    proj = perspective(fov_y_deg, width / height, 0.01, 100.0)
+    view = look_at(eye, target)
+    pi = np.linalg.inv(proj).astype(np.float32)
+    wi = np.linalg.inv(view).astype(np.float32)
+    return pi.reshape(4, 4).flatten(order="F"), wi.reshape(4, 4).flatten(order="F")
+
+
 RTAO_EYE = (1.0, 1.1, 0.15)
 RTAO_TARGET = (1.0, 0.0, 0.45)
 RTAO_FOV = 44.0

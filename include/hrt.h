@@ -203,6 +203,29 @@ hrt_status hrt_rtao_make_rays(hrt_scene* scene, const float* primaryRays, const 
                               size_t nPixels, uint32_t raysPerHit, float tMax, uint64_t seed, float* outRays,
                               void* stream);
 
+/* Device-side eye rays of the 01_eye_rays renderer (EyeRayCaster::
+ * kernel_PackXY + kernel_InitEyeRay, examples/01_eye_rays/eye_ray.cpp:26-47,
+ * 114-138; EyeRayDirNormalized, transform_ray3f,
+ * examples/render_common.h:6-36): one ray per pixel from the inverse
+ * projection and inverse world-view matrices (column-major), pixel centres,
+ * tNear = 0, tFar = FLT_MAX.  Ray k belongs to the pixel the reference's
+ * SuperBlockIndex2DOpt order gives thread k (8x8 superblocks of 4x4 blocks),
+ * so width and height must be multiples of 8.  outPackedXY (nullable) gets
+ * y << 16 | x per ray, the reference's m_packedXY.  Device pointers;
+ * asynchronous on `stream`. */
+hrt_status hrt_eye_make_rays(hrt_ctx* ctx, const float projInvColMajor16[16], const float worldViewInvColMajor16[16],
+                             uint32_t width, uint32_t height, float* outRays, uint32_t* outPackedXY, void* stream);
+
+/* The 01_eye_rays frame on the device (EyeRayCaster::Render /
+ * CastRaySingleBlock, eye_ray_host.cpp:329-345, with kernel_RayTrace,
+ * eye_ray.cpp:50-75): eye rays as above, nearest hits against the committed
+ * scene, and outColor[y * width + x] = 0 on a miss, else
+ * palette[primId % paletteSize] (paletteSize 1..64, host array).  outColor
+ * is a device buffer of width * height words; asynchronous on `stream`. */
+hrt_status hrt_eye_render(hrt_scene* scene, const float projInvColMajor16[16], const float worldViewInvColMajor16[16],
+                          uint32_t width, uint32_t height, const uint32_t* palette, uint32_t paletteSize,
+                          uint32_t* outColor, void* stream);
+
 /* ISceneObject::GetGeomNum / GetInstNum (CrossRT.h:163-164). */
 uint32_t hrt_geom_count(const hrt_scene* scene);
 uint32_t hrt_instance_count(const hrt_scene* scene);

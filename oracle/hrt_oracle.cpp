@@ -19,6 +19,12 @@
 //   BVH2 traversal        core/BVH2CommonRT.cpp:9-141 (+ raytrace_common.h:38-80)
 //   any-hit rule          core/BVH4HalfRT.cpp:258-331 (tFar <= 0 => stop at first hit)
 //   brute force           core/BruteForceRT.cpp:184-265
+//   eye rays              examples/01_eye_rays/eye_ray.cpp:26-47, 114-138
+//                         (kernel_InitEyeRay, SuperBlockIndex2DOpt,
+//                         kernel_PackXY); examples/render_common.h:6-36
+//                         (EyeRayDirNormalized, transform_ray3f).  LiteMath's
+//                         float4x4 * float4 is taken as each row's products
+//                         summed left to right, normalize(u) = u / sqrt(dot(u, u))
 //
 // Floating point: every float3 operation is scalar, left to right
 // (dot = x*x' + y*y' + z*z'), and the file is compiled with
@@ -1568,6 +1574,58 @@ void hrto_scene_trace(void* h, int mode, const float* rays, uint64_t n, uint32_t
       counters[2 * i + 0] = uint32_t(c.pairs);
       counters[2 * i + 1] = uint32_t(c.tris);
     }
+  }
+}
+
+
+// SuperBlockIndex2DOpt (eye_ray.cpp:114-131): pixel -> thread index.
+uint32_t hrto_superblock_offset(uint32_t x, uint32_t y, uint32_t width) {
+  const uint32_t inBlockIdX = x & 3u, inBlockIdY = y & 3u;
+  const uint32_t localIndex = inBlockIdY * 4 + inBlockIdX;
+  const uint32_t wBlocks = width >> 2;
+  const uint32_t blockX = x >> 2, blockY = y >> 2;
+  const uint32_t inHBlockIdX = blockX & 1u, inHBlockIdY = blockY & 1u;
+  const uint32_t localIndexH = inHBlockIdY * 2 + inHBlockIdX;
+  const uint32_t wBlocksH = wBlocks >> 1;
+  const uint32_t blockHX = blockX >> 1, blockHY = blockY >> 1;
+  return (blockHX + blockHY * wBlocksH) * 64 + localIndexH * 16 + localIndex;
+}
+
+// kernel_PackXY over every pixel, then kernel_InitEyeRay for every thread
+// index.  Matrices are column-major.  rays: 8 floats per thread index.
+void hrto_eye_rays(const float* projInv, const float* worldViewInv, uint32_t width, uint32_t height, float* rays,
+                   uint32_t* packedXY) {
+  const size_t n = size_t(width) * height;
+  std::vector<uint32_t> xy(n, 0u);
+  for (uint32_t y = 0; y < height; y++)
+    for (uint32_t x = 0; x < width; x++) xy[hrto_superblock_offset(x, y, width)] = ((y << 16) & 0xFFFF0000u) | (x & 0xFFFFu);
+  auto mat_vec = [](const float* a, f4 v) -> f4 {
+    return {a[0] * v.x + a[4] * v.y + a[8] * v.z + a[12] * v.w, a[1] * v.x + a[5] * v.y + a[9] * v.z + a[13] * v.w,
+            a[2] * v.x + a[6] * v.y + a[10] * v.z + a[14] * v.w, a[3] * v.x + a[7] * v.y + a[11] * v.z + a[15] * v.w};
+  };
+  auto normalize = [](f3 u) -> f3 {
+    const float len = std::sqrt(dot3(u, u));
+    return {u.x / len, u.y / len, u.z / len};
+  };
+  for (size_t t = 0; t < n; t++) {
+    const uint32_t XY = xy[t];
+    const uint32_t x = XY & 0x0000FFFFu, y = (XY & 0xFFFF0000u) >> 16;
+    const float fx = (float(x) + 0.5f) / float(width), fy = (float(y) + 0.5f) / float(height);
+    // EyeRayDirNormalized
+    f4 pos = {2.0f * fx - 1.0f, -2.0f * fy + 1.0f, 0.0f, 1.0f};
+    pos = mat_vec(projInv, pos);
+    const float w = pos.w;
+    pos = {pos.x / w, pos.y / w, pos.z / w, pos.w / w};
+    f3 dir = normalize({pos.x, pos.y, pos.z});
+    // transform_ray3f
+    const f3 o = {0.0f, 0.0f, 0.0f};
+    const f4 p1 = mat_vec(worldViewInv, {o.x, o.y, o.z, 1.0f});
+    const f3 o2 = {o.x + 100.0f * dir.x, o.y + 100.0f * dir.y, o.z + 100.0f * dir.z};
+    const f4 p2 = mat_vec(worldViewInv, {o2.x, o2.y, o2.z, 1.0f});
+    dir = normalize({p2.x - p1.x, p2.y - p1.y, p2.z - p1.z});
+    const float r[8] = {p1.x, p1.y, p1.z, 0.0f, dir.x, dir.y, dir.z, 3.402823466e+38f};
+    std::memcpy(rays + 8 * t, r, sizeof(r));
+    if (packedXY) packedXY[t] = XY;
   }
 }
 

@@ -64,6 +64,10 @@ def lib():
         L.hrto_free.argtypes = [vp]
         L.hrto_refit.restype = C.c_int
         L.hrto_refit.argtypes = [vp, vp, u32, vp, u32]
+        L.hrto_superblock_offset.restype = u32
+        L.hrto_superblock_offset.argtypes = [u32, u32, u32]
+        L.hrto_eye_rays.restype = None
+        L.hrto_eye_rays.argtypes = [vp, vp, u32, u32, vp, vp]
         L.hrto_sizes.argtypes = [vp, vp]
         L.hrto_scene_box.argtypes = [vp, vp]
         L.hrto_codes.argtypes = [vp, vp, vp, vp]
@@ -326,3 +330,20 @@ def slab(o, inv, bmin, bmax) -> np.ndarray:
     out = np.zeros(2, dtype=np.float32)
     lib().hrto_slab(*(_ptr(a) for a in args), _ptr(out))
     return out
+
+
+def superblock_offset(x: int, y: int, width: int) -> int:
+    """SuperBlockIndex2DOpt (examples/01_eye_rays/eye_ray.cpp:114-131): pixel -> thread index."""
+    return int(lib().hrto_superblock_offset(x, y, width))
+
+
+def eye_rays(proj_inv_col_major: np.ndarray, world_view_inv_col_major: np.ndarray, width: int, height: int):
+    """kernel_PackXY + kernel_InitEyeRay (eye_ray.cpp:26-47, 133-138): rays (n, 8) in thread order and
+    the packed y << 16 | x of each thread."""
+    pi = np.ascontiguousarray(proj_inv_col_major, dtype=np.float32).reshape(16)
+    wi = np.ascontiguousarray(world_view_inv_col_major, dtype=np.float32).reshape(16)
+    n = width * height
+    rays = np.zeros((n, 8), dtype=np.float32)
+    xy = np.zeros(n, dtype=np.uint32)
+    lib().hrto_eye_rays(_ptr(pi), _ptr(wi), width, height, _ptr(rays), _ptr(xy))
+    return rays, xy

@@ -1,0 +1,61 @@
+"""Oracle eye rays (EyeRayCaster::kernel_PackXY / kernel_InitEyeRay, examples/01_eye_rays/eye_ray.cpp:26-47,
+114-138): the superblock order is a bijection onto 8x8 tiles, and the rays equal a float64 pinhole camera."""
+import numpy as np
+import pytest
+
+from heavyrt_amd import workloads as W
+from oracle import oracle as O
+
+
+def test_superblock_offset_known_answers():
+    # 4x4 blocks, 2x2 blocks per superblock, superblocks row by row
+    assert O.superblock_offset(0, 0, 16) == 0
+    assert O.superblock_offset(3, 0, 16) == 3
+    assert O.superblock_offset(0, 1, 16) == 4
+    assert O.superblock_offset(4, 0, 16) == 16
+    assert O.superblock_offset(0, 4, 16) == 32
+    assert O.superblock_offset(4, 4, 16) == 48
+    assert O.superblock_offset(8, 0, 16) == 64
+    assert O.superblock_offset(0, 8, 16) == 128
+    assert O.superblock_offset(15, 15, 16) == 255
+
+
+@pytest.mark.parametrize("w,h", [(16, 8), (64, 40), (512, 512)])
+def test_superblock_order_is_tiles_of_64(w, h):
+    _, xy = O.eye_rays(*W.eye_camera(w, h), w, h)
+    x, y = xy & 0xFFFF, xy >> 16
+    assert np.unique(xy).size == w * h
+    assert x.max() == w - 1 and y.max() == h - 1
+    # each run of 64 thread indices is one 8x8 tile
+    tx, ty = (x // 8).reshape(-1, 64), (y // 8).reshape(-1, 64)
+    assert np.all(tx == tx[:, :1]) and np.all(ty == ty[:, :1])
+    for t in range(0, w * h, 997):
+        assert O.superblock_offset(int(x[t]), int(y[t]), w) == t
+
+
+def test_eye_rays_equal_float64_pinhole():
+    w = h = 64
+    rays, xy = O.eye_rays(*W.eye_camera(w, h), w, h)
+    ref = W.pinhole_rays(w, h, eye=W.EYE_POS, target=W.EYE_TARGET, fov_y_deg=45.0)
+    pix = (xy >> 16).astype(np.int64) * w + (xy & 0xFFFF)
+    ref = ref[pix]
+    np.testing.assert_allclose(rays[:, 0:3], ref[:, 0:3], atol=2e-6)
+    np.testing.assert_allclose(rays[:, 4:7], ref[:, 4:7], atol=2e-6)
+    np.testing.assert_allclose(np.linalg.norm(rays[:, 4:7].astype(np.float64), axis=1), 1.0, atol=1e-6)
+    assert np.all(rays[:, 3] == 0.0) and np.all(rays[:, 7] == np.float32(3.402823466e38))
+
+
+def test_eye_rays_mirror_about_the_centre():
+    # y is flipped (row 0 at the top): the top row looks up, the left column looks left
+    w = h = 32
+    rays, xy = O.eye_rays(*W.eye_camera(w, h, eye=(0, 0, 5), target=(0, 0, 0)), w, h)
+    x, y = xy & 0xFFFF, xy >> 16
+    d = rays[:, 4:7]
+    assert np.all(d[y == 0, 1] > 0) and np.all(d[y == h - 1, 1] < 0)
+    assert np.all(d[x == 0, 0] < 0) and np.all(d[x == w - 1, 0] > 0)
+    # pixels mirrored about the centre get mirrored directions
+    k = {(int(a), int(b)): i for i, (a, b) in enumerate(zip(x, y))}
+    for (a, b), i in list(k.items())[::37]:
+        j = k[(w - 1 - a, h - 1 - b)]
+        np.testing.assert_allclose(d[i, 0:2], -d[j, 0:2], atol=1e-6)
+        np.testing.assert_allclose(d[i, 2], d[j, 2], atol=1e-6)
