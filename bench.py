@@ -6,8 +6,9 @@ primary rays traced nearest-hit (BVH2 node pairs, the reference's visiting
 order, bit-exact with BVH2CommonRT); 16 cosine-weighted AO rays per primary
 hit made on the device (tMax = 0.1 x scene diagonal); all AO rays traced
 any-hit on FP16 BVH4 nodes (BVH4HalfRT's node format, boxes rounded outward)
-of a second LBVH over the same triangles built on a cubic Morton grid
-(any-hit answers do not depend on the tree; the tests check them against the
+of a second tree over the same triangles: leaves from an LBVH on a cubic
+Morton grid, the hierarchy above them by agglomerative clustering (any-hit
+answers do not depend on the tree; the tests check them against the
 oracle and every run checks them against the reference tree's BVH2, FP32 BVH4
 and FP16 BVH4).  Scene and rays
 are resident in HBM before the timed region starts.
@@ -42,7 +43,7 @@ if ROOT not in sys.path:
 HBM_PEAK_GBPS = 8000.0
 SPP = 16
 SEED = 0x2545F4914F6CDD1D
-NODE_BYTES = {"bvh2": 64, "bvh4": 128, "bvh4_fp16": 64, "bvh4_fp16_cubic": 64}
+NODE_BYTES = {"bvh2": 64, "bvh4": 128, "bvh4_fp16": 64, "bvh4_fp16_anytree": 64}
 # HBM bytes per any-hit launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
 # this workload, scaled by the same counters on a device copy of known size.
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "rtao_anyhit_traffic.json")
@@ -69,9 +70,9 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--any-layout", choices=["bvh2", "bvh4", "bvh4_fp16_cubic"], default="bvh4_fp16_cubic",
+    ap.add_argument("--any-layout", choices=["bvh2", "bvh4", "bvh4_fp16_anytree"], default="bvh4_fp16_anytree",
                     help="tree for the AO any-hit rays: bvh2 (the reference tree's pairs), bvh4 (its FP32 "
-                         "BVH4), bvh4_fp16_cubic (FP16 BVH4 of the cubic-grid any-hit tree)")
+                         "BVH4), bvh4_fp16_anytree (FP16 BVH4 of the any-hit tree: cubic-grid leaves, clustered hierarchy)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true", help="skip timing the other any-hit layouts")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, os.cpu_count())")
@@ -122,10 +123,10 @@ def main() -> int:
         return bench_serving(args)
     dev = torch.device("cuda:0")
     layouts = {"bvh2": H.LAYOUT_BVH2, "bvh4": H.LAYOUT_BVH4, "bvh4_fp16": H.LAYOUT_BVH4_FP16,
-               "bvh4_fp16_cubic": H.LAYOUT_AUTO_FP16}
+               "bvh4_fp16_anytree": H.LAYOUT_AUTO_FP16}
     # the committed layout of a step: nearest hits on BVH2 in the reference
     # order, any-hit on the chosen tree
-    step_layout = {"bvh2": H.LAYOUT_BVH2, "bvh4": H.LAYOUT_AUTO, "bvh4_fp16_cubic": H.LAYOUT_AUTO_FP16}
+    step_layout = {"bvh2": H.LAYOUT_BVH2, "bvh4": H.LAYOUT_AUTO, "bvh4_fp16_anytree": H.LAYOUT_AUTO_FP16}
 
     log("generating scene")
     verts, faces = W.rtao_scene()
@@ -145,7 +146,7 @@ def main() -> int:
     scene.commit(layout=H.LAYOUT_BVH4)  # BVH4 collapse (FP32 + FP16 nodes)
     collapse_ms = (time.perf_counter() - t0) * 1e3
     t0 = time.perf_counter()
-    scene.commit(layout=H.LAYOUT_AUTO_FP16)  # cubic-grid any-hit tree + its BVH4
+    scene.commit(layout=H.LAYOUT_AUTO_FP16)  # the any-hit tree + its BVH4
     any_tree_ms = (time.perf_counter() - t0) * 1e3
     log(f"build {faces.shape[0]} tris: {build_ms:.2f} ms (host copy + LBVH), BVH4 collapse {collapse_ms:.2f} ms, "
         f"any-hit tree {any_tree_ms:.2f} ms")
@@ -226,7 +227,7 @@ def main() -> int:
     variants = {}
     if not args.no_variants:
         ref_vis = d_vis.clone()
-        for name in ("bvh2", "bvh4", "bvh4_fp16", "bvh4_fp16_cubic"):
+        for name in ("bvh2", "bvh4", "bvh4_fp16", "bvh4_fp16_anytree"):
             scene.commit(layout=layouts[name])
             out = torch.zeros_like(d_vis)
             for _ in range(2):

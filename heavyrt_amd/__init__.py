@@ -331,7 +331,7 @@ class Scene:
     def dump(self, geom_id: int, what: int) -> np.ndarray:
         flag = what & DUMP_ANY_TREE
         what &= ~DUMP_ANY_TREE
-        sizes = np.zeros(5, dtype=np.uint32)
+        sizes = np.zeros(6, dtype=np.uint32)
         _check(lib().hrt_debug_dump(self._h, geom_id, DUMP_SIZES | flag, _ptr(sizes), sizes.nbytes))
         if what == DUMP_SIZES:
             return sizes

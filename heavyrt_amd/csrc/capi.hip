@@ -697,8 +697,8 @@ hrt_status hrt_debug_dump(hrt_scene* s, uint32_t geomId, int what, void* host_ou
   if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
   if (what == HRT_DUMP_SIZES) {
     if (need(20)) return HRT_ERR_INVALID_ARG;
-    uint32_t v[5] = {N, L, b.max_run, npairs, b.nquads};
-    std::memcpy(host_out, v, 20);
+    uint32_t v[6] = {N, L, b.max_run, npairs, b.nquads, b.clustered ? 1u : 0u};
+    std::memcpy(host_out, v, bytes >= 24 ? 24 : 20);
     return HRT_OK;
   }
   if (what == HRT_DUMP_SCENE_BOX) {

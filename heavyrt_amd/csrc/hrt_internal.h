@@ -59,8 +59,13 @@ struct Blas {
   uint32_t* ucodes = nullptr;      // leaves
   uint32_t* karras_lr = nullptr;   // 2 * (leaves - 1): children in the reference's left/right numbering
   bool keep_debug = false;
-  // Second tree over the same triangles for any-hit queries, built on a cubic
-  // Morton grid (made on demand by blas_build_any_tree; owned).
+  // true when the hierarchy above the leaves is the agglomerative one
+  // (blas_ploc): pairs, parent and karras_lr hold it; parent_leaf and
+  // node_range still describe the Morton hierarchy and are not used
+  bool clustered = false;
+  // Second tree over the same triangles for any-hit queries: leaves on a
+  // cubic Morton grid, hierarchy by blas_ploc (made on demand by
+  // blas_build_any_tree; owned).
   Blas* any_tree = nullptr;
 };
 
@@ -87,6 +92,12 @@ hipError_t blas_refit(Blas& b, hipStream_t stream);
 // Leaves b.any_tree null when some Morton cell of the cubic grid holds more
 // than 127 triangles.  Synchronises the stream.
 hipError_t blas_build_any_tree(Blas& b, hipStream_t stream);
+
+// Replaces the hierarchy above b's leaves with the agglomerative one (PLOC,
+// ploc.hip); leaves, leaf order and triangles stay.  *replaced is false (and
+// b unchanged) when the tree would be deeper than the BVH4 traversal stack
+// allows (32 internal levels).  Synchronises the stream.
+hipError_t blas_ploc(Blas& b, hipStream_t stream, bool* replaced);
 // Collapses the built BVH2 into FP32 and FP16 BVH4 nodes (idempotent);
 // synchronises the stream.
 hipError_t blas_build_bvh4(Blas& b, hipStream_t stream);

@@ -726,6 +726,8 @@ hipError_t blas_build_any_tree(Blas& b, hipStream_t s) {
   if (e == hipSuccess) e = blas_build(*a, s, true);
   // a run longer than the 7-bit leaf count cannot be stored: keep the
   // reference tree for any-hit queries then
+  bool replaced = false;
+  if (e == hipSuccess && a->max_run <= 127) e = blas_ploc(*a, s, &replaced);
   if (e == hipSuccess && a->max_run <= 127) e = blas_build_bvh4(*a, s);
   if (e != hipSuccess || a->max_run > 127) {
     blas_free(*a);

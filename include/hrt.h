@@ -93,7 +93,9 @@ enum { HRT_BUILD_LOW = 0, HRT_BUILD_MEDIUM = 1, HRT_BUILD_HIGH = 2, HRT_BUILD_RE
 /* Debug / parity dumps (hrt_debug_dump `what`).  Available for geometries
  * added after hrt_scene_set_keep_build_arrays(scene, 1). */
 enum {
-  HRT_DUMP_SIZES = 0,        /* uint32[5]: triangles, leaves, max leaf size, pairs, BVH4 nodes */
+  HRT_DUMP_SIZES = 0,        /* uint32[5]: triangles, leaves, max leaf size, pairs, BVH4 nodes;
+                                a 6th word (if the buffer holds 24 bytes): 1 when the hierarchy
+                                above the leaves is the agglomerative one (any-hit tree) */
   HRT_DUMP_CODES_INPUT = 1,  /* uint32[ntris]: Morton code per input triangle */
   HRT_DUMP_CODES_SORTED = 2, /* uint32[ntris]: codes after the (code, id) sort */
   HRT_DUMP_IDS_SORTED = 3,   /* uint32[ntris]: triangle ids after the sort */

@@ -19,6 +19,7 @@
 //   BVH2 traversal        core/BVH2CommonRT.cpp:9-141 (+ raytrace_common.h:38-80)
 //   any-hit rule          core/BVH4HalfRT.cpp:258-331 (tFar <= 0 => stop at first hit)
 //   brute force           core/BruteForceRT.cpp:184-265
+//   agglomerative tree    no reference counterpart (the engine's any-hit tree)
 //   eye rays              examples/01_eye_rays/eye_ray.cpp:26-47, 114-138
 //                         (kernel_InitEyeRay, SuperBlockIndex2DOpt,
 //                         kernel_PackXY); examples/render_common.h:6-36
@@ -35,6 +36,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <vector>
 
 #ifdef _OPENMP
@@ -748,6 +750,94 @@ static Hit trace_brute(const Build& b, f4 posNear, f4 dirFar) {
 }
 
 // ---------------------------------------------------------------------------
+// Agglomerative hierarchy over the leaves (PLOC: Meister & Bittner 2018,
+// "Parallel Locally-Ordered Clustering for Bounding Volume Hierarchy
+// Construction").  No reference counterpart: the engine's any-hit tree may
+// differ from the reference's tree, since an any-hit answer does not depend
+// on the tree.  Restated so that the GPU builder's nodes can be checked one
+// for one.  The rule, step by step:
+//   clusters = the leaves in Morton order, each with its box;
+//   repeat until one cluster is left:
+//     each cluster i picks the j != i within [i - radius, i + radius] that
+//     minimises the union box's half area (ex*ey + ey*ez + ez*ex, NaN taken
+//     as +inf), ties to the smaller j -- a strict order on pairs, so the
+//     globally best pair always picks itself and every round merges;
+//     every mutual pair (i < j) becomes a new internal node, left = i,
+//     right = j, numbered downwards from L - 2 in increasing i; it takes
+//     position i and j is dropped; the survivors keep their order.
+// The root ends up as node 0.  Boxes are the exact min/max unions, so the
+// LR layout (internal nodes 0..L-2, leaves L-1+m unchanged) feeds the BVH4
+// collapse as a Karras tree does.
+// ---------------------------------------------------------------------------
+static inline float union_half_area(f3 alo, f3 ahi, f3 blo, f3 bhi) {
+  const f3 lo = {mn(alo.x, blo.x), mn(alo.y, blo.y), mn(alo.z, blo.z)};
+  const f3 hi = {mx(ahi.x, bhi.x), mx(ahi.y, bhi.y), mx(ahi.z, bhi.z)};
+  const float ex = hi.x - lo.x, ey = hi.y - lo.y, ez = hi.z - lo.z;
+  const float a = ex * ey + ey * ez + ez * ex;
+  return (a == a) ? a : std::numeric_limits<float>::infinity();
+}
+
+static void ploc(Build& b, uint32_t radius) {
+  const uint32_t L = b.L;
+  if (L < 2) return;
+  const uint32_t Lm1 = L - 1;
+  struct Cl {
+    uint32_t node;
+    f3 lo, hi;
+  };
+  std::vector<Cl> cur(L), nxt;
+  for (uint32_t m = 0; m < L; m++) cur[m] = {Lm1 + m, b.lr[Lm1 + m].bmin, b.lr[Lm1 + m].bmax};
+  std::vector<uint32_t> nn;
+  uint32_t next_id = L - 2;
+  while (cur.size() > 1) {
+    const uint32_t n = uint32_t(cur.size());
+    nn.assign(n, 0);
+    for (uint32_t i = 0; i < n; i++) {
+      const uint32_t lo = i > radius ? i - radius : 0, hi = std::min(n - 1, i + radius);
+      float best = 0.0f;
+      uint32_t bj = 0xFFFFFFFFu;
+      for (uint32_t j = lo; j <= hi; j++) {
+        if (j == i) continue;
+        const float a = union_half_area(cur[i].lo, cur[i].hi, cur[j].lo, cur[j].hi);
+        if (bj == 0xFFFFFFFFu || a < best) {
+          best = a;
+          bj = j;
+        }
+      }
+      nn[i] = bj;
+    }
+    nxt.clear();
+    uint32_t merged = 0;
+    for (uint32_t i = 0; i < n; i++) {
+      const uint32_t j = nn[i];
+      const bool mutual = nn[j] == i;
+      if (mutual && i > j) continue;  // merged into j's node
+      if (!mutual) {
+        nxt.push_back(cur[i]);
+        continue;
+      }
+      const uint32_t id = next_id - merged++;
+      Node nd;
+      nd.bmin = {mn(cur[i].lo.x, cur[j].lo.x), mn(cur[i].lo.y, cur[j].lo.y), mn(cur[i].lo.z, cur[j].lo.z)};
+      nd.bmax = {mx(cur[i].hi.x, cur[j].hi.x), mx(cur[i].hi.y, cur[j].hi.y), mx(cur[i].hi.z, cur[j].hi.z)};
+      nd.left = cur[i].node;
+      nd.escape = cur[j].node;
+      b.lr[id] = nd;
+      nxt.push_back({id, nd.bmin, nd.bmax});
+    }
+    next_id -= merged;
+    cur.swap(nxt);
+  }
+  // subtree counts for the peach layout (parents have smaller ids than their children)
+  for (uint32_t i = Lm1; i-- > 0;)
+    b.temp_count[i] = b.temp_count[b.lr[i].left] + b.temp_count[b.lr[i].escape] + 2;
+  b.peach.clear();
+  build_peach(b);
+  b.q32.clear();
+  b.q16.clear();
+}
+
+// ---------------------------------------------------------------------------
 // BVH4: collapse of the Karras BVH2, FP16 boxes, traversal
 // ---------------------------------------------------------------------------
 // Collapse rule (core/builders/cbvh_core.cpp:263-352,
@@ -1277,6 +1367,10 @@ void* hrto_build(const float* vert4, uint32_t nVert, const uint32_t* idx, uint32
 }
 
 void hrto_free(void* h) { delete static_cast<Build*>(h); }
+
+// Replaces the hierarchy above the leaves with the agglomerative one (ploc);
+// leaves, leaf order and triangles stay.
+void hrto_ploc(void* h, uint32_t radius) { ploc(*static_cast<Build*>(h), radius); }
 
 // Refit (BuildQuality BUILD_REFIT, CrossRT.h:14): new vertex positions (and
 // optionally indices, same triangle count), hierarchy and leaf order kept;

@@ -64,6 +64,8 @@ def lib():
         L.hrto_free.argtypes = [vp]
         L.hrto_refit.restype = C.c_int
         L.hrto_refit.argtypes = [vp, vp, u32, vp, u32]
+        L.hrto_ploc.restype = None
+        L.hrto_ploc.argtypes = [vp, u32]
         L.hrto_superblock_offset.restype = u32
         L.hrto_superblock_offset.argtypes = [u32, u32, u32]
         L.hrto_eye_rays.restype = None
@@ -162,6 +164,10 @@ class OracleBVH:
                              0 if idx is None else self.idx.size)
         if r != 0:
             raise ValueError("a refit keeps the triangle count")
+
+    def ploc(self, radius: int = 16):
+        """Replaces the hierarchy above the leaves with the agglomerative (PLOC) one; leaves stay."""
+        lib().hrto_ploc(self._h, radius)
 
     def scene_box(self):
         out = np.zeros(12, dtype=np.float32)

@@ -122,7 +122,7 @@ def test_bvh4_any_hit(gpu_ctx, name, gen, layout):
         mode = O.MODE4_FP16 if layout in (H.LAYOUT_BVH4_FP16, H.LAYOUT_AUTO_FP16) else O.MODE4_FP32
         ob4 = ob
         if layout == H.LAYOUT_AUTO_FP16 and has_any_tree(s):
-            ob4 = O.OracleBVH(v, f, cubic=True)  # AUTO_FP16 answers any-hit on the cubic-grid tree
+            ob4 = oracle_any_tree(v, f)[0]  # AUTO_FP16 answers any-hit on its own any-hit tree
         _, p4, _, _, nq, nt = ob4.trace4(anyr, mode, counters=True)
         gs = trace(s, rays, "any", stats=True)
         np.testing.assert_array_equal(gs["hit"], p4 != INVALID)
@@ -131,7 +131,6 @@ def test_bvh4_any_hit(gpu_ctx, name, gen, layout):
     s.close()
 
 
-@pytest.mark.parametrize("layout", [H.LAYOUT_AUTO, H.LAYOUT_AUTO_FP16], ids=["auto", "auto_fp16"])
 def has_any_tree(scene) -> bool:
     try:
         scene.dump(0, H.DUMP_SIZES | H.DUMP_ANY_TREE)
@@ -140,20 +139,39 @@ def has_any_tree(scene) -> bool:
         return False
 
 
+def oracle_any_tree(v, f):
+    """The oracle's any-hit tree: leaves of the cubic-grid LBVH, hierarchy by agglomerative clustering
+    (radius 8) unless that tree has more than 32 internal levels (the BVH4 stack bound)."""
+    ob = O.OracleBVH(v, f, cubic=True)
+    if ob.max_run > 127 or ob.leaves < 3:
+        return ob, False
+    pl = O.OracleBVH(v, f, cubic=True)
+    pl.ploc(8)
+    lr = pl.lr_nodes()
+    L = pl.leaves
+    depth = np.zeros(2 * L - 1, dtype=np.int64)
+    for i in range(L - 1):  # parents have smaller ids than their children
+        depth[int(lr[i, 3])] = depth[i] + 1
+        depth[int(lr[i, 7])] = depth[i] + 1
+    return (pl, True) if depth.max() <= 32 else (ob, False)
+
+
 @pytest.mark.parametrize("name,gen", CASES, ids=[c[0] for c in CASES])
 def test_any_tree_bit_exact(gpu_ctx, name, gen):
-    """HRT_LAYOUT_AUTO_FP16 builds a second tree on a cubic Morton grid for
-    any-hit queries: its build arrays and FP16 BVH4 nodes equal the oracle's
-    cubic-grid build bit for bit."""
+    """HRT_LAYOUT_AUTO_FP16 builds a second tree for any-hit queries: leaves on
+    a cubic Morton grid, the hierarchy above them by agglomerative clustering.
+    Its build arrays, node layout and FP16 BVH4 nodes equal the oracle's bit
+    for bit."""
     v, f = gen()
     s = H.Scene(gpu_ctx, keep_build_arrays=True)
     s.add_instance(s.add_triangles(v, f))
     s.commit(layout=H.LAYOUT_AUTO_FP16)
-    ob = O.OracleBVH(v, f, cubic=True)
+    ob, clustered = oracle_any_tree(v, f)
     if ob.max_run > 127:
         assert not has_any_tree(s)
         s.close()
         return
+    assert int(s.dump(0, H.DUMP_SIZES | H.DUMP_ANY_TREE)[5]) == int(clustered)
     check_build(s, ob, flag=H.DUMP_ANY_TREE)
     boxes16, refs16 = gpu_quads16_as_floats(s.dump(0, H.DUMP_QUADS16 | H.DUMP_ANY_TREE))
     oq16 = ob.quads(True)
