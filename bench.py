@@ -43,7 +43,7 @@ if ROOT not in sys.path:
 HBM_PEAK_GBPS = 8000.0
 SPP = 16
 SEED = 0x2545F4914F6CDD1D
-NODE_BYTES = {"bvh2": 64, "bvh4": 128, "bvh4_fp16": 64, "bvh4_fp16_anytree": 64}
+NODE_BYTES = {"bvh2": 64, "bvh4": 128, "bvh4_fp16": 64, "bvh4_fp16_anytree": 64, "bvh8_fp16_anytree": 128}
 # HBM bytes per any-hit launch from rocprofv3 FETCH_SIZE / WRITE_SIZE runs of
 # this workload, scaled by the same counters on a device copy of known size.
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "rtao_anyhit_traffic.json")
@@ -70,9 +70,11 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--any-layout", choices=["bvh2", "bvh4", "bvh4_fp16_anytree"], default="bvh4_fp16_anytree",
+    ap.add_argument("--any-layout", choices=["bvh2", "bvh4", "bvh4_fp16_anytree", "bvh8_fp16_anytree"],
+                    default="bvh4_fp16_anytree",
                     help="tree for the AO any-hit rays: bvh2 (the reference tree's pairs), bvh4 (its FP32 "
-                         "BVH4), bvh4_fp16_anytree (FP16 BVH4 of the any-hit tree: cubic-grid leaves, clustered hierarchy)")
+                         "BVH4), bvh4_fp16_anytree (FP16 BVH4 of the any-hit tree: cubic-grid leaves, clustered hierarchy), "
+                         "bvh8_fp16_anytree (the same tree in 8-wide FP16 nodes)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true", help="skip timing the other any-hit layouts")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, os.cpu_count())")
@@ -123,10 +125,11 @@ def main() -> int:
         return bench_serving(args)
     dev = torch.device("cuda:0")
     layouts = {"bvh2": H.LAYOUT_BVH2, "bvh4": H.LAYOUT_BVH4, "bvh4_fp16": H.LAYOUT_BVH4_FP16,
-               "bvh4_fp16_anytree": H.LAYOUT_AUTO_FP16}
+               "bvh4_fp16_anytree": H.LAYOUT_AUTO_FP16, "bvh8_fp16_anytree": H.LAYOUT_AUTO_FP16_WIDE}
     # the committed layout of a step: nearest hits on BVH2 in the reference
     # order, any-hit on the chosen tree
-    step_layout = {"bvh2": H.LAYOUT_BVH2, "bvh4": H.LAYOUT_AUTO, "bvh4_fp16_anytree": H.LAYOUT_AUTO_FP16}
+    step_layout = {"bvh2": H.LAYOUT_BVH2, "bvh4": H.LAYOUT_AUTO, "bvh4_fp16_anytree": H.LAYOUT_AUTO_FP16,
+                   "bvh8_fp16_anytree": H.LAYOUT_AUTO_FP16_WIDE}
 
     log("generating scene")
     verts, faces = W.rtao_scene()
@@ -227,7 +230,7 @@ def main() -> int:
     variants = {}
     if not args.no_variants:
         ref_vis = d_vis.clone()
-        for name in ("bvh2", "bvh4", "bvh4_fp16", "bvh4_fp16_anytree"):
+        for name in ("bvh2", "bvh4", "bvh4_fp16", "bvh4_fp16_anytree", "bvh8_fp16_anytree"):
             scene.commit(layout=layouts[name])
             out = torch.zeros_like(d_vis)
             for _ in range(2):

@@ -238,12 +238,15 @@ hrt_status trace_common(hrt_scene* s, const float* rays, size_t n, void* out, in
     const int lay = s->layout;
     if (s->two_level)
       e = hrt::trace_two_level(s->tlas, s->inst_dev ? s->inst_dev + s->instances.size() : nullptr, a, hs);
-    else if (lay == HRT_LAYOUT_BVH2 || ((lay == HRT_LAYOUT_AUTO || lay == HRT_LAYOUT_AUTO_FP16) && !anyhit))
+    const bool auto16 = lay == HRT_LAYOUT_AUTO_FP16 || lay == HRT_LAYOUT_AUTO_FP16_WIDE;
+    if (lay == HRT_LAYOUT_BVH2 || ((lay == HRT_LAYOUT_AUTO || auto16) && !anyhit))
       e = hrt::trace_blas(*b, a, hs);
-    else if (lay == HRT_LAYOUT_AUTO_FP16 && b->any_tree)
-      e = hrt::trace_blas4(*b->any_tree, a, true, hs);  // any-hit on the cubic-grid tree
+    else if (lay == HRT_LAYOUT_AUTO_FP16_WIDE && b->any_tree && b->any_tree->octs16)
+      e = hrt::trace_blas8(*b->any_tree, a, hs);  // any-hit on the any-hit tree's 8-wide nodes
+    else if (auto16 && b->any_tree)
+      e = hrt::trace_blas4(*b->any_tree, a, true, hs);  // any-hit on the any-hit tree
     else
-      e = hrt::trace_blas4(*b, a, lay == HRT_LAYOUT_BVH4_FP16 || lay == HRT_LAYOUT_AUTO_FP16, hs);
+      e = hrt::trace_blas4(*b, a, lay == HRT_LAYOUT_BVH4_FP16 || auto16, hs);
   }
   if (e != hipSuccess) return fail_hip(e, "trace");
   return HRT_OK;
@@ -473,7 +476,8 @@ hrt_status hrt_update_instance(hrt_scene* s, uint32_t instId, const float colMaj
 hrt_status hrt_commit(hrt_scene* s, int quality, int layout) {
   (void)quality;
   if (!s) return fail(HRT_ERR_INVALID_ARG, "scene is null");
-  if (layout < HRT_LAYOUT_BVH2 || layout > HRT_LAYOUT_AUTO_FP16) return fail(HRT_ERR_INVALID_ARG, "unknown layout");
+  if (layout < HRT_LAYOUT_BVH2 || layout > HRT_LAYOUT_AUTO_FP16_WIDE)
+    return fail(HRT_ERR_INVALID_ARG, "unknown layout");
   s->committed = false;
   free_tlas(s);
   const bool single = s->instances.size() == 1 && s->instances[0].identity;
@@ -489,14 +493,15 @@ hrt_status hrt_commit(hrt_scene* s, int quality, int layout) {
   } else if (layout != HRT_LAYOUT_BVH2) {
     for (const Instance& in : s->instances) {
       hrt::Blas& b = s->geoms[in.geom];
-      if (layout == HRT_LAYOUT_BVH4_FP16 || layout == HRT_LAYOUT_AUTO_FP16) {
+      const bool auto16 = layout == HRT_LAYOUT_AUTO_FP16 || layout == HRT_LAYOUT_AUTO_FP16_WIDE;
+      if (layout == HRT_LAYOUT_BVH4_FP16 || auto16) {
         // FP16 boxes are only conservative inside the half range (cbvh_fat.cpp:447-448
         // maps every value beyond +-65504 to +-65504)
         for (int k = 0; k < 3; k++)
           if (!(b.box[k] >= -65504.0f && b.box[4 + k] <= 65504.0f))
             return fail(HRT_ERR_UNSUPPORTED, "FP16 nodes need every coordinate within +-65504");
       }
-      if (layout == HRT_LAYOUT_AUTO_FP16) {
+      if (auto16) {
         // any-hit tree on a cubic Morton grid; the reference tree's BVH4 only
         // when the cubic one cannot be stored
         if ((e = hrt::blas_build_any_tree(b, s->ctx->stream)) != hipSuccess) return fail_hip(e, "any-hit tree");
@@ -685,7 +690,7 @@ hrt_status hrt_debug_dump(hrt_scene* s, uint32_t geomId, int what, void* host_ou
   const bool any_tree = (what & HRT_DUMP_ANY_TREE) != 0;
   what &= ~HRT_DUMP_ANY_TREE;
   if (any_tree && (geomId == HRT_TLAS_ID || !s->geoms[geomId].any_tree))
-    return fail(HRT_ERR_INVALID_ARG, "no any-hit tree (commit with HRT_LAYOUT_AUTO_FP16)");
+    return fail(HRT_ERR_INVALID_ARG, "no any-hit tree (commit with HRT_LAYOUT_AUTO_FP16 or _WIDE)");
   const hrt::Blas& b = (geomId == HRT_TLAS_ID) ? s->tlas : (any_tree ? *s->geoms[geomId].any_tree : s->geoms[geomId]);
   const uint32_t N = b.ntris, L = b.leaves;
   const uint32_t npairs = std::max<uint32_t>(L > 1 ? L - 1 : 1, 1);
@@ -697,8 +702,8 @@ hrt_status hrt_debug_dump(hrt_scene* s, uint32_t geomId, int what, void* host_ou
   if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
   if (what == HRT_DUMP_SIZES) {
     if (need(20)) return HRT_ERR_INVALID_ARG;
-    uint32_t v[6] = {N, L, b.max_run, npairs, b.nquads, b.clustered ? 1u : 0u};
-    std::memcpy(host_out, v, bytes >= 24 ? 24 : 20);
+    uint32_t v[7] = {N, L, b.max_run, npairs, b.nquads, b.clustered ? 1u : 0u, b.nocts};
+    std::memcpy(host_out, v, bytes >= 28 ? 28 : (bytes >= 24 ? 24 : 20));
     return HRT_OK;
   }
   if (what == HRT_DUMP_SCENE_BOX) {
@@ -713,6 +718,13 @@ hrt_status hrt_debug_dump(hrt_scene* s, uint32_t geomId, int what, void* host_ou
     if (need(n)) return HRT_ERR_INVALID_ARG;
     e = hipMemcpy(host_out, what == HRT_DUMP_QUADS32 ? static_cast<const void*>(b.quads32) : b.quads16, n,
                   hipMemcpyDeviceToHost);
+    return e == hipSuccess ? HRT_OK : fail_hip(e, "dump");
+  }
+  if (what == HRT_DUMP_OCTS16) {
+    if (!b.octs16) return fail(HRT_ERR_INVALID_ARG, "no BVH8 nodes (commit with HRT_LAYOUT_AUTO_FP16_WIDE)");
+    const size_t n = sizeof(hrt::Oct16) * size_t(b.nocts);
+    if (need(n)) return HRT_ERR_INVALID_ARG;
+    e = hipMemcpy(host_out, b.octs16, n, hipMemcpyDeviceToHost);
     return e == hipSuccess ? HRT_OK : fail_hip(e, "dump");
   }
   if (what == HRT_DUMP_PAIRS) {
