@@ -64,7 +64,7 @@ EXPORTS = [
     "hrt_scene_set_keep_build_arrays", "hrt_scene_clear_geom", "hrt_scene_clear_instances",
     "hrt_add_triangles", "hrt_add_instance", "hrt_commit", "hrt_trace_nearest", "hrt_trace_any",
     "hrt_trace_nearest_stats", "hrt_trace_any_stats", "hrt_query_nearest_one", "hrt_geom_count",
-    "hrt_instance_count", "hrt_debug_dump", "hrt_last_error", "hrt_rtao_make_rays", "hrt_eye_make_rays", "hrt_eye_render",
+    "hrt_instance_count", "hrt_debug_dump", "hrt_last_error", "hrt_rtao_make_rays", "hrt_eye_make_rays", "hrt_eye_render", "hrt_scene_serialize", "hrt_scene_deserialize",
     "hrt_trace_nearest_host", "hrt_trace_any_host", "hrt_update_triangles", "hrt_update_instance",
     "hrt_batch_create", "hrt_batch_destroy", "hrt_batch_buffers", "hrt_batch_run", "hrt_batch_build",
     "hrt_batch_trace", "hrt_batch_candidate_count", "hrt_batch_candidate_info", "hrt_batch_set_candidate",
@@ -136,6 +136,8 @@ def lib():
         L.hrt_rtao_make_rays.argtypes = [vp, vp, vp, sz, u32, C.c_float, C.c_uint64, vp, vp]
         L.hrt_eye_make_rays.argtypes = [vp, vp, vp, u32, u32, vp, vp, vp]
         L.hrt_eye_render.argtypes = [vp, vp, vp, u32, u32, vp, u32, vp, vp]
+        L.hrt_scene_serialize.argtypes = [vp, vp, sz, C.POINTER(C.c_size_t)]
+        L.hrt_scene_deserialize.argtypes = [vp, vp, sz, C.POINTER(C.c_void_p)]
         L.hrt_trace_nearest_host.argtypes = [vp, vp, sz, vp, vp]
         L.hrt_trace_any_host.argtypes = [vp, vp, sz, vp]
         L.hrt_update_triangles.argtypes = [vp, u32, vp, sz, sz, vp, sz, i32]
@@ -319,6 +321,25 @@ class Scene:
                        stream: int = 0):
         _check(lib().hrt_rtao_make_rays(self._h, _dev_ptr(prim_rays), _dev_ptr(prim_hits_compact), npix, spp,
                                         float(tmax), seed & 0xFFFFFFFFFFFFFFFF, _dev_ptr(out_rays), stream or None))
+
+    def serialize(self) -> bytes:
+        """The committed scene with its trees as bytes (hrt_scene_serialize), for a warm start."""
+        n = C.c_size_t(0)
+        _check(lib().hrt_scene_serialize(self._h, None, 0, C.byref(n)))
+        buf = np.empty(n.value, dtype=np.uint8)
+        _check(lib().hrt_scene_serialize(self._h, _ptr(buf), buf.size, C.byref(n)))
+        return buf.tobytes()
+
+    @classmethod
+    def deserialize(cls, ctx: "Context", data: bytes) -> "Scene":
+        """A committed scene from serialize()'s bytes (hrt_scene_deserialize); no tree is rebuilt."""
+        s = cls.__new__(cls)
+        s.ctx = ctx
+        s._h = C.c_void_p()
+        buf = np.frombuffer(data, dtype=np.uint8)
+        _check(lib().hrt_scene_deserialize(ctx._h, _ptr(buf), buf.size, C.byref(s._h)))
+        _live_scenes.add(s)
+        return s
 
     def eye_render(self, proj_inv, world_view_inv, width: int, height: int, palette, out_color, stream: int = 0):
         """The 01_eye_rays frame on the device (hrt_eye_render): out_color is a device buffer of

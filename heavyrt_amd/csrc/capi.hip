@@ -236,10 +236,11 @@ hrt_status trace_common(hrt_scene* s, const float* rays, size_t n, void* out, in
   hipError_t e = hipSetDevice(s->ctx->device);
   if (e == hipSuccess) {
     const int lay = s->layout;
+    const bool auto16 = lay == HRT_LAYOUT_AUTO_FP16 || lay == HRT_LAYOUT_AUTO_FP16_WIDE;
+    // b is null for a two-level scene: every branch below it reads b
     if (s->two_level)
       e = hrt::trace_two_level(s->tlas, s->inst_dev ? s->inst_dev + s->instances.size() : nullptr, a, hs);
-    const bool auto16 = lay == HRT_LAYOUT_AUTO_FP16 || lay == HRT_LAYOUT_AUTO_FP16_WIDE;
-    if (lay == HRT_LAYOUT_BVH2 || ((lay == HRT_LAYOUT_AUTO || auto16) && !anyhit))
+    else if (lay == HRT_LAYOUT_BVH2 || ((lay == HRT_LAYOUT_AUTO || auto16) && !anyhit))
       e = hrt::trace_blas(*b, a, hs);
     else if (lay == HRT_LAYOUT_AUTO_FP16_WIDE && b->any_tree && b->any_tree->octs16)
       e = hrt::trace_blas8(*b->any_tree, a, hs);  // any-hit on the any-hit tree's 8-wide nodes
@@ -674,6 +675,239 @@ hrt_status hrt_eye_render(hrt_scene* s, const float projInvColMajor16[16], const
   if (st != HRT_OK) return st;
   if (e == hipSuccess) e = fe;
   if (e != hipSuccess) return fail_hip(e, "eye render");
+  return HRT_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Scene snapshots (warm start).  Layout, little-endian:
+//   "HRTSCN01", u32 version, i32 layout, u32 geometries, u32 instances,
+//   per geometry a tree record, per instance u32 geometry + 16 floats
+//   (column-major), u64 hash of everything before it (snap_hash).
+// A tree record: u32 nverts, ntris, leaves, max_run, nquads, nocts,
+//   clustered, has_any_tree; float box[8], zscale[4]; then the device arrays
+//   at their allocated sizes (vertices, indices, pairs, triangles,
+//   permutation, parents, leaf runs, leaf parents, node ranges, meta, BVH4
+//   FP32 and FP16 nodes, BVH8 nodes); then the any-hit tree's record if any.
+// Loading uploads the arrays and commits: no LBVH, BVH4 or any-hit tree is
+// rebuilt; an instanced scene's top level is (it is small).
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr char kSnapMagic[8] = {'H', 'R', 'T', 'S', 'C', 'N', '0', '1'};
+constexpr uint32_t kSnapVersion = 1;
+
+// Writes into the caller's buffer (sized beforehand by snapshot_bytes); a
+// write past `cap` is dropped and flagged.
+struct SnapWriter {
+  uint8_t* out;
+  size_t cap, at = 0;
+  bool overflow = false;
+  bool room(size_t n) {
+    if (n > cap - at) overflow = true;
+    return !overflow;
+  }
+  void put(const void* p, size_t n) {
+    if (!room(n)) return;
+    std::memcpy(out + at, p, n);
+    at += n;
+  }
+  template <class T>
+  void val(T v) { put(&v, sizeof(T)); }
+  hipError_t dev(const void* d, size_t n) {
+    if (!room(n)) return hipSuccess;
+    hipError_t e = n ? hipMemcpy(out + at, d, n, hipMemcpyDeviceToHost) : hipSuccess;
+    at += n;
+    return e;
+  }
+};
+
+struct SnapReader {
+  const uint8_t* p;
+  size_t n, at = 0;
+  bool get(void* out, size_t k) {
+    if (k > n - at) return false;
+    std::memcpy(out, p + at, k);
+    at += k;
+    return true;
+  }
+  template <class T>
+  bool val(T& v) { return get(&v, sizeof(T)); }
+  hipError_t dev(void* d, size_t k) {
+    if (k > n - at) return hipErrorInvalidValue;
+    hipError_t e = k ? hipMemcpy(d, p + at, k, hipMemcpyHostToDevice) : hipSuccess;
+    at += k;
+    return e;
+  }
+};
+
+// FNV-1a over little-endian 64-bit words, then the tail bytes one by one
+// (one multiply per 8 bytes; every step is a bijection of the state, so any
+// single changed word changes the result).
+uint64_t snap_hash(const uint8_t* p, size_t n) {
+  constexpr uint64_t kPrime = 0x100000001b3ull;
+  uint64_t h = 0xcbf29ce484222325ull;
+  size_t i = 0;
+  for (; i + 8 <= n; i += 8) {
+    uint64_t w;
+    std::memcpy(&w, p + i, 8);
+    h = (h ^ w) * kPrime;
+  }
+  for (; i < n; i++) h = (h ^ p[i]) * kPrime;
+  return h;
+}
+
+size_t npairs_of(const hrt::Blas& b) { return std::max<size_t>(b.ntris > 1 ? b.ntris - 1 : 1, 1); }
+
+// Bytes of one tree record (and its any-hit tree's), as write_tree lays it out.
+size_t tree_bytes(const hrt::Blas& b) {
+  const size_t np = npairs_of(b), nt = b.ntris;
+  size_t n = 32 + sizeof(b.box) + sizeof(b.zscale);
+  n += 16ull * b.nverts + 12 * nt + sizeof(hrt::NodePair) * np + sizeof(hrt::TriPre) * nt + 4 * nt + 4 * np +
+       4 * nt + 4 * nt + 8 * np + 256 + (sizeof(hrt::Quad32) + sizeof(hrt::Quad16)) * size_t(b.nquads) +
+       sizeof(hrt::Oct16) * size_t(b.nocts);
+  return n + (b.any_tree ? tree_bytes(*b.any_tree) : 0);
+}
+
+size_t snapshot_bytes(const hrt_scene* s) {
+  size_t n = sizeof(kSnapMagic) + 16;
+  for (const hrt::Blas& b : s->geoms) n += tree_bytes(b);
+  return n + s->instances.size() * (4 + sizeof(float) * 16) + 8;
+}
+
+hipError_t write_tree(SnapWriter& w, const hrt::Blas& b) {
+  const uint32_t head[8] = {b.nverts, b.ntris, b.leaves, b.max_run, b.nquads, b.nocts, b.clustered ? 1u : 0u,
+                            b.any_tree ? 1u : 0u};
+  w.put(head, sizeof(head));
+  w.put(b.box, sizeof(b.box));
+  w.put(b.zscale, sizeof(b.zscale));
+  const size_t np = npairs_of(b), nt = b.ntris;
+  hipError_t e = hipSuccess;
+  const std::pair<const void*, size_t> arrays[] = {
+      {b.verts, 16ull * b.nverts}, {b.idx, 12ull * nt}, {b.pairs, sizeof(hrt::NodePair) * np},
+      {b.tris, sizeof(hrt::TriPre) * nt}, {b.perm, 4ull * nt}, {b.parent, 4ull * np},
+      {b.leaf_run, 4ull * nt}, {b.parent_leaf, 4ull * nt}, {b.node_range, 8ull * np}, {b.meta, 256},
+      {b.quads32, sizeof(hrt::Quad32) * b.nquads}, {b.quads16, sizeof(hrt::Quad16) * b.nquads},
+      {b.octs16, sizeof(hrt::Oct16) * b.nocts}};
+  for (const auto& a : arrays)
+    if ((e = w.dev(a.first, a.second)) != hipSuccess) return e;
+  return b.any_tree ? write_tree(w, *b.any_tree) : hipSuccess;
+}
+
+// Reads one tree record into b (allocated here).  Returns HRT_OK or an error
+// status with the message set.
+hrt_status read_tree(SnapReader& r, hrt::Blas& b, int depth) {
+  uint32_t head[8];
+  float box[8], zs[4];
+  if (!r.val(head) || !r.val(box) || !r.val(zs)) return fail(HRT_ERR_INVALID_ARG, "snapshot is truncated");
+  const uint32_t nverts = head[0], ntris = head[1];
+  if (nverts == 0 || ntris == 0 || ntris > 0x00FFFFFFu || head[2] == 0 || head[2] > ntris || head[7] > 1 ||
+      (head[7] && depth > 0))
+    return fail(HRT_ERR_INVALID_ARG, "snapshot holds an invalid tree record");
+  hipError_t e = hrt::blas_alloc(b, nverts, ntris, false);
+  if (e != hipSuccess) return fail_hip(e, "snapshot allocation");
+  b.leaves = head[2];
+  b.max_run = head[3];
+  b.clustered = head[6] != 0;
+  std::memcpy(b.box, box, sizeof(b.box));
+  std::memcpy(b.zscale, zs, sizeof(b.zscale));
+  if (head[4] && ((e = hipMalloc(&b.quads32, sizeof(hrt::Quad32) * head[4])) != hipSuccess ||
+                  (e = hipMalloc(&b.quads16, sizeof(hrt::Quad16) * head[4])) != hipSuccess))
+    return fail_hip(e, "snapshot allocation");
+  b.nquads = head[4];
+  if (head[5] && (e = hipMalloc(&b.octs16, sizeof(hrt::Oct16) * head[5])) != hipSuccess)
+    return fail_hip(e, "snapshot allocation");
+  b.nocts = head[5];
+  const size_t np = npairs_of(b), nt = ntris;
+  const std::pair<void*, size_t> arrays[] = {
+      {b.verts, 16ull * nverts}, {b.idx, 12ull * nt}, {b.pairs, sizeof(hrt::NodePair) * np},
+      {b.tris, sizeof(hrt::TriPre) * nt}, {b.perm, 4ull * nt}, {b.parent, 4ull * np},
+      {b.leaf_run, 4ull * nt}, {b.parent_leaf, 4ull * nt}, {b.node_range, 8ull * np}, {b.meta, 256},
+      {b.quads32, sizeof(hrt::Quad32) * b.nquads}, {b.quads16, sizeof(hrt::Quad16) * b.nquads},
+      {b.octs16, sizeof(hrt::Oct16) * b.nocts}};
+  for (const auto& a : arrays)
+    if ((e = r.dev(a.first, a.second)) != hipSuccess)
+      return e == hipErrorInvalidValue ? fail(HRT_ERR_INVALID_ARG, "snapshot is truncated") : fail_hip(e, "snapshot upload");
+  if (head[7]) {
+    b.any_tree = new (std::nothrow) hrt::Blas();
+    if (!b.any_tree) return fail(HRT_ERR_OUT_OF_MEMORY, "host allocation failed");
+    return read_tree(r, *b.any_tree, depth + 1);
+  }
+  return HRT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+hrt_status hrt_scene_serialize(hrt_scene* s, void* out, size_t capacity, size_t* size) {
+  if (!s || !size) return fail(HRT_ERR_INVALID_ARG, "scene/size is null");
+  if (!s->committed) return fail(HRT_ERR_NOT_COMMITTED, "only a committed scene is saved");
+  const size_t total = snapshot_bytes(s);
+  *size = total;
+  if (!out) return HRT_OK;
+  if (capacity < total) return fail(HRT_ERR_INVALID_ARG, "output buffer too small (query the size first)");
+  SnapWriter w{static_cast<uint8_t*>(out), total};
+  w.put(kSnapMagic, sizeof(kSnapMagic));
+  w.val(kSnapVersion);
+  w.val(int32_t(s->layout));
+  w.val(uint32_t(s->geoms.size()));
+  w.val(uint32_t(s->instances.size()));
+  hipError_t e = hipSetDevice(s->ctx->device);
+  if (e == hipSuccess) e = hipStreamSynchronize(s->ctx->stream);
+  for (size_t g = 0; e == hipSuccess && g < s->geoms.size(); g++) e = write_tree(w, s->geoms[g]);
+  if (e != hipSuccess) return fail_hip(e, "snapshot download");
+  for (const Instance& in : s->instances) {
+    w.val(uint32_t(in.geom));
+    w.put(in.fwd, sizeof(in.fwd));
+  }
+  if (w.overflow || w.at + 8 != total) return fail(HRT_ERR_INVALID_ARG, "internal: snapshot size mismatch");
+  w.val(snap_hash(w.out, w.at));
+  return HRT_OK;
+}
+
+hrt_status hrt_scene_deserialize(hrt_ctx* ctx, const void* data, size_t size, hrt_scene** out) {
+  if (!ctx || !data || !out) return fail(HRT_ERR_INVALID_ARG, "null argument");
+  *out = nullptr;
+  const uint8_t* p = static_cast<const uint8_t*>(data);
+  if (size < sizeof(kSnapMagic) + 24 || std::memcmp(p, kSnapMagic, sizeof(kSnapMagic)) != 0)
+    return fail(HRT_ERR_INVALID_ARG, "not a scene snapshot");
+  uint64_t want = 0;
+  std::memcpy(&want, p + size - 8, 8);
+  if (snap_hash(p, size - 8) != want) return fail(HRT_ERR_INVALID_ARG, "scene snapshot is corrupted (hash mismatch)");
+  SnapReader r{p, size - 8};
+  r.at = sizeof(kSnapMagic);
+  uint32_t version = 0, ngeom = 0, ninst = 0;
+  int32_t layout = 0;
+  if (!r.val(version) || !r.val(layout) || !r.val(ngeom) || !r.val(ninst))
+    return fail(HRT_ERR_INVALID_ARG, "snapshot is truncated");
+  if (version != kSnapVersion) return fail(HRT_ERR_UNSUPPORTED, "unknown snapshot version");
+  hrt_scene* s = nullptr;
+  hrt_status st = hrt_scene_create(ctx, &s);
+  if (st != HRT_OK) return st;
+  hipError_t e = hipSetDevice(ctx->device);
+  if (e != hipSuccess) {
+    hrt_scene_destroy(s);
+    return fail_hip(e, "hipSetDevice");
+  }
+  for (uint32_t g = 0; st == HRT_OK && g < ngeom; g++) {
+    s->geoms.emplace_back();
+    st = read_tree(r, s->geoms.back(), 0);
+  }
+  for (uint32_t i = 0; st == HRT_OK && i < ninst; i++) {
+    uint32_t geom = 0;
+    float m[16];
+    if (!r.val(geom) || !r.val(m)) st = fail(HRT_ERR_INVALID_ARG, "snapshot is truncated");
+    else st = hrt_add_instance(s, geom, m, nullptr);
+  }
+  if (st == HRT_OK && r.at != r.n) st = fail(HRT_ERR_INVALID_ARG, "snapshot has trailing bytes");
+  if (st == HRT_OK) st = hrt_commit(s, HRT_BUILD_MEDIUM, layout);
+  if (st != HRT_OK) {
+    hrt_scene_destroy(s);
+    return st;
+  }
+  *out = s;
   return HRT_OK;
 }
 

@@ -196,9 +196,12 @@ hipError_t blas_ploc(Blas& b, hipStream_t s, bool* replaced) {
   NodePair* npairs = nullptr;
   uint32_t *nparent = nullptr, *nlr = nullptr;
   uint32_t* base = nullptr;
-  hipError_t e = hipMalloc(&npairs, sizeof(NodePair) * ni);
-  if (e == hipSuccess) e = hipMalloc(&nparent, 4ull * ni);
-  if (e == hipSuccess && b.karras_lr) e = hipMalloc(&nlr, 8ull * ni);
+  // the same capacities as blas_alloc's (pairs for every triangle but one),
+  // so the swapped arrays can stand in for the originals everywhere
+  const size_t cap = std::max<size_t>(b.ntris > 1 ? b.ntris - 1 : 1, 1);
+  hipError_t e = hipMalloc(&npairs, sizeof(NodePair) * cap);
+  if (e == hipSuccess) e = hipMalloc(&nparent, 4ull * cap);
+  if (e == hipSuccess && b.karras_lr) e = hipMalloc(&nlr, 8ull * cap);
   if (e == hipSuccess) e = hipMallocAsync(reinterpret_cast<void**>(&base), 4 * words, s);
   bool ok = false;
   if (e == hipSuccess) {

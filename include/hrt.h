@@ -157,6 +157,22 @@ hrt_status hrt_add_instance(hrt_scene* scene, uint32_t geomId, const float colMa
 hrt_status hrt_update_triangles(hrt_scene* scene, uint32_t geomId, const float* vpos, size_t nVert,
                                 size_t vStrideBytes, const uint32_t* idx, size_t nIdx, int quality);
 
+/* Warm start: a committed scene (every geometry with its trees -- LBVH node
+ * pairs, leaf-order triangles, BVH4/BVH8 nodes, the any-hit tree -- the
+ * instances and the layout) as one byte buffer.  Call with out = NULL to get
+ * the size in *size, then with a buffer of at least that size.  No
+ * counterpart in the reference (its engines rebuild from geometry; the
+ * reference lists serialised BVHs for serving warm start as future work). */
+hrt_status hrt_scene_serialize(hrt_scene* scene, void* out, size_t capacity, size_t* size);
+
+/* A committed scene from hrt_scene_serialize's bytes on this context's
+ * device: the arrays are uploaded, nothing but an instanced scene's top level
+ * is rebuilt, and traces give the saved scene's answers bit for bit.  Fails
+ * with HRT_ERR_INVALID_ARG for a buffer that is not a snapshot, is truncated
+ * or whose hash does not match, and with HRT_ERR_UNSUPPORTED for an unknown
+ * snapshot version. */
+hrt_status hrt_scene_deserialize(hrt_ctx* ctx, const void* data, size_t size, hrt_scene** out);
+
 /* ISceneObject::UpdateInstance (CrossRT.h:125-130; a stub in the reference,
  * BVH2CommonRT_host.cpp:166-170): a new column-major matrix for an existing
  * instance; the scene must be committed again. */
