@@ -189,6 +189,34 @@ __device__ inline float2 slab(float3 o, float3 inv, float bminx, float bminy, fl
   return make_float2(fmaxf(tmin, fminf(lo2, hi2)), fminf(tmax, fmaxf(lo2, hi2)));
 }
 
+// -(o * inv) per axis, made once per ray for slab_fused.
+__device__ inline float3 neg_o_inv(float3 o, float3 inv) {
+  return make_float3(-(o.x * inv.x), -(o.y * inv.y), -(o.z * inv.z));
+}
+
+// The slab test in the fused form t = fma(b, inv, -(o * inv)): one rounding
+// per plane instead of two, and with an FP16 box value as a direct operand of
+// v_fma_mix_f32 (widening FP16 to FP32 is exact) no conversion either.  It is
+// used only by the any-hit traversals of the FP16 nodes, where box tests prune
+// and never order the answer: a ray's any-hit answer is whether some triangle
+// passes the Moller-Trumbore test, whichever boxes are visited, as long as no
+// box holding such a triangle is rejected.  Both forms round each plane
+// distance by about half an ulp of the larger of |b| and |o| (the difference
+// b - o, or the product o * inv, is rounded), against boxes widened outward
+// by at least one FP16 ulp.
+__device__ inline float2 slab_fused(float3 oi, float3 inv, float bminx, float bminy, float bminz, float bmaxx,
+                                    float bmaxy, float bmaxz) {
+  const float lo = fmaf(bminx, inv.x, oi.x);
+  const float hi = fmaf(bmaxx, inv.x, oi.x);
+  const float lo1 = fmaf(bminy, inv.y, oi.y);
+  const float hi1 = fmaf(bmaxy, inv.y, oi.y);
+  const float lo2 = fmaf(bminz, inv.z, oi.z);
+  const float hi2 = fmaf(bmaxz, inv.z, oi.z);
+  const float tmin = fmaxf(fminf(lo, hi), fminf(lo1, hi1));
+  const float tmax = fminf(fmaxf(lo, hi), fmaxf(lo1, hi1));
+  return make_float2(fmaxf(tmin, fminf(lo2, hi2)), fminf(tmax, fmaxf(lo2, hi2)));
+}
+
 struct HitRec {
   float t;
   uint32_t prim;  // original triangle id, kInvalid on miss

@@ -24,7 +24,7 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kBlock4 = 64;
-constexpr int kRefillAt4 = 16;
+constexpr int kRefillAt4 = 32;
 constexpr uint32_t kClaim = 128;  // ray indices a wave claims from the queue at once
 constexpr float kMaxFloat = 3.402823466e+38f;
 
@@ -38,9 +38,11 @@ __global__ void k_reset_counter4(uint32_t* c) { *c = 0u; }
 
 // Loads one BVH4 node, tests its four children, and returns the entry
 // distances (kMaxFloat for a miss) and references in reference order.
-template <bool kFp16>
-__device__ inline void visit_quad(const void* __restrict__ quads, uint32_t q, float3 o, float3 inv, float tnear,
-                                  float tcull, float tm[4], uint32_t ch[4]) {
+// kFused (FP16 nodes of any-hit traversals only): slab_fused with oi = -(o * inv).
+template <bool kFp16, bool kFused = false>
+__device__ inline void visit_quad(const void* __restrict__ quads, uint32_t q, float3 o, float3 inv, float3 oi,
+                                  float tnear, float tcull, float tm[4], uint32_t ch[4]) {
+  static_assert(kFp16 || !kFused, "the fused box test is for FP16 nodes");
   float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
   if (kFp16) {
     const uint4* p = reinterpret_cast<const uint4*>(static_cast<const Quad16*>(quads) + q);
@@ -69,7 +71,8 @@ __device__ inline void visit_quad(const void* __restrict__ quads, uint32_t q, fl
   }
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    const float2 t = slab(o, inv, lox[k], loy[k], loz[k], hix[k], hiy[k], hiz[k]);
+    const float2 t = kFused ? slab_fused(oi, inv, lox[k], loy[k], loz[k], hix[k], hiy[k], hiz[k])
+                            : slab(o, inv, lox[k], loy[k], loz[k], hix[k], hiy[k], hiz[k]);
     // An unused slot (box +1e9..-1e9, empty leaf 0x80000000) would pass the
     // min/max slab test as the box [-1e9, 1e9]; it holds no triangle, so it
     // is never entered.
@@ -125,6 +128,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_simple(const void* __restric
   const float tnear = r.pos_near.w;
   const bool any = kForceAny || (r.dir_far.w <= 0.0f);
   const float3 inv = safe_inverse(d);
+  const float3 oi = neg_o_inv(o, inv);
   HitRec h;
   h.t = kForceAny ? fabsf(r.dir_far.w) : (any ? -r.dir_far.w : r.dir_far.w);
   h.prim = kInvalid;
@@ -137,7 +141,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_simple(const void* __restric
     if ((cur & kLeafBit) == 0) {
       float tm[4];
       uint32_t ch[4];
-      visit_quad<kFp16>(quads, cur, o, inv, tnear, h.t, tm, ch);
+      visit_quad<kFp16, kForceAny && kFp16>(quads, cur, o, inv, oi, tnear, h.t, tm, ch);
       if (kStats) nvisits++;
       if (tm[3] < kMaxFloat) stack[(top++) * kBlock4] = ch[3];
       if (tm[2] < kMaxFloat) stack[(top++) * kBlock4] = ch[2];
@@ -185,7 +189,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_any_persistent(const void* _
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
   bool active = false;
   uint64_t rid = 0;
-  float3 o = make_float3(0.f, 0.f, 0.f), d = o, inv = o;
+  float3 o = make_float3(0.f, 0.f, 0.f), d = o, inv = o, oi = o;
   float tnear = 0.f;
   HitRec h;
   h.t = 0.f;
@@ -211,6 +215,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_any_persistent(const void* _
           h.t = fabsf(r.dir_far.w);
           h.prim = kInvalid;
           inv = safe_inverse(d);
+          oi = neg_o_inv(o, inv);
           top = 0;
           cur = 0;
           leaf = 0;
@@ -221,7 +226,8 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_any_persistent(const void* _
     if (__ballot(active) == 0) break;
 
     while (true) {
-      // inner phase: park the first leaf found, keep descending speculatively
+      // inner phase: park the first leaf found; a lane with a parked leaf
+      // keeps descending while any lane of the wave still looks for one
       while (true) {
         if (active && leaf == 0 && (cur & kLeafBit) != 0 && cur != kInvalid) {
           leaf = cur;
@@ -232,7 +238,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_any_persistent(const void* _
         if (active && (cur & kLeafBit) == 0) {
           float tm[4];
           uint32_t ch[4];
-          visit_quad<kFp16>(quads, cur, o, inv, tnear, h.t, tm, ch);
+          visit_quad<kFp16, kFp16>(quads, cur, o, inv, oi, tnear, h.t, tm, ch);
           if (tm[3] < kMaxFloat) stk.push(top, ch[3]);
           if (tm[2] < kMaxFloat) stk.push(top, ch[2]);
           if (tm[1] < kMaxFloat) stk.push(top, ch[1]);
@@ -308,13 +314,16 @@ hipError_t launch_simple4(const Blas& b, const TraceArgs& a, hipStream_t stream)
 // (19 comparators, swap when strictly less); the oracle orders them with the
 // same network, so the counting variant's visits equal the oracle's.
 // ---------------------------------------------------------------------------
-__device__ inline void visit_oct(const Oct16* __restrict__ octs, uint32_t q, float3 o, float3 inv, float tnear,
-                                 float tcull, float tm[8], uint32_t ch[8]) {
+template <bool kFused>
+__device__ inline void visit_oct(const Oct16* __restrict__ octs, uint32_t q, float3 o, float3 inv, float3 oi,
+                                 float tnear, float tcull, float tm[8], uint32_t ch[8]) {
   const uint4* p = reinterpret_cast<const uint4*>(octs + q);
 #pragma unroll
   for (int k = 0; k < 8; k++) {
     const uint4 w = p[k];
-    const float2 t = slab(o, inv, h2f_lo(w.x), h2f_hi(w.x), h2f_lo(w.y), h2f_hi(w.y), h2f_lo(w.z), h2f_hi(w.z));
+    const float2 t =
+        kFused ? slab_fused(oi, inv, h2f_lo(w.x), h2f_hi(w.x), h2f_lo(w.y), h2f_hi(w.y), h2f_lo(w.z), h2f_hi(w.z))
+               : slab(o, inv, h2f_lo(w.x), h2f_hi(w.x), h2f_lo(w.y), h2f_hi(w.y), h2f_lo(w.z), h2f_hi(w.z));
     ch[k] = w.w;
     const bool hit = (t.x <= t.y) && (t.y >= tnear) && (t.x <= tcull) && ch[k] != kEmptyLeaf;
     tm[k] = hit ? t.x : kMaxFloat;
@@ -353,6 +362,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace8_simple(const Oct16* __restri
   const float tnear = r.pos_near.w;
   const bool any = kForceAny || (r.dir_far.w <= 0.0f);
   const float3 inv = safe_inverse(d);
+  const float3 oi = neg_o_inv(o, inv);
   HitRec h;
   h.t = kForceAny ? fabsf(r.dir_far.w) : (any ? -r.dir_far.w : r.dir_far.w);
   h.prim = kInvalid;
@@ -365,7 +375,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace8_simple(const Oct16* __restri
     if ((cur & kLeafBit) == 0) {
       float tm[8];
       uint32_t ch[8];
-      visit_oct(octs, cur, o, inv, tnear, h.t, tm, ch);
+      visit_oct<kForceAny>(octs, cur, o, inv, oi, tnear, h.t, tm, ch);
       if (kStats) nvisits++;
 #pragma unroll
       for (int k = 7; k >= 1; k--)
@@ -408,7 +418,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace8_any_persistent(const Oct16* 
   const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
   bool active = false;
   uint64_t rid = 0;
-  float3 o = make_float3(0.f, 0.f, 0.f), d = o, inv = o;
+  float3 o = make_float3(0.f, 0.f, 0.f), d = o, inv = o, oi = o;
   float tnear = 0.f;
   HitRec h;
   h.t = 0.f;
@@ -434,6 +444,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace8_any_persistent(const Oct16* 
           h.t = fabsf(r.dir_far.w);
           h.prim = kInvalid;
           inv = safe_inverse(d);
+          oi = neg_o_inv(o, inv);
           top = 0;
           cur = 0;
           leaf = 0;
@@ -454,7 +465,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace8_any_persistent(const Oct16* 
         if (active && (cur & kLeafBit) == 0) {
           float tm[8];
           uint32_t ch[8];
-          visit_oct(octs, cur, o, inv, tnear, h.t, tm, ch);
+          visit_oct<true>(octs, cur, o, inv, oi, tnear, h.t, tm, ch);
 #pragma unroll
           for (int k = 7; k >= 1; k--)
             if (tm[k] < kMaxFloat) stk.push(top, ch[k]);

@@ -550,6 +550,23 @@ static inline void slab(f3 o, f3 inv, f3 bmin, f3 bmax, float& tmin_out, float& 
   tmax_out = mn(tmax, mx(lo2, hi2));
 }
 
+// The engine's fused form of the same test for the any-hit traversals of FP16
+// nodes: t = fma(b, inv, -(o * inv)) per plane (one rounding after the
+// product o * inv); the same min/max reduction.
+static inline void slab_fused(f3 o, f3 inv, f3 bmin, f3 bmax, float& tmin_out, float& tmax_out) {
+  const float nx = -(o.x * inv.x), ny = -(o.y * inv.y), nz = -(o.z * inv.z);
+  const float lo = std::fma(bmin.x, inv.x, nx);
+  const float hi = std::fma(bmax.x, inv.x, nx);
+  const float lo1 = std::fma(bmin.y, inv.y, ny);
+  const float hi1 = std::fma(bmax.y, inv.y, ny);
+  const float lo2 = std::fma(bmin.z, inv.z, nz);
+  const float hi2 = std::fma(bmax.z, inv.z, nz);
+  const float tmin = mx(mn(lo, hi), mn(lo1, hi1));
+  const float tmax = mn(mx(lo, hi), mx(lo1, hi1));
+  tmin_out = mx(tmin, mn(lo2, hi2));
+  tmax_out = mn(tmax, mx(lo2, hi2));
+}
+
 // Moller-Trumbore with the reference's 1e-6 slack, BVH2CommonRT.cpp:26-45:
 // t, u, v and whether the barycentric and t > tNear conditions hold.
 static inline bool mt_eval(f3 o, f3 d, float tNear, f3 A, f3 B, f3 C, float& t, float& u, float& v) {
@@ -1004,7 +1021,9 @@ static inline bool mt_test_lex(f3 o, f3 d, float tNear, f3 A, f3 B, f3 C, Hit& h
 // BVH4HalfRT::BVH4Traverse order (BVH4HalfRT.cpp:73-256): four slab tests,
 // misses get MAXFLOAT, five-comparator network on the entry distances, push
 // w, z, y (if hit), continue with x.
-static Hit trace_bvh4(const Build& b, const std::vector<Quad>& quads, f4 posNear, f4 dirFar, Counters* cnt) {
+// fused: the box tests in slab_fused form (the engine's any-hit calls on FP16 nodes).
+static Hit trace_bvh4(const Build& b, const std::vector<Quad>& quads, f4 posNear, f4 dirFar, Counters* cnt,
+                      bool fused = false) {
   const bool anyhit = (dirFar.w <= 0.0f);
   const float tFar = anyhit ? -dirFar.w : dirFar.w;
   Hit h;
@@ -1028,7 +1047,8 @@ static Hit trace_bvh4(const Build& b, const std::vector<Quad>& quads, f4 posNear
       uint32_t ch[4];
       for (int k = 0; k < 4; k++) {
         float a, c;
-        slab(o, inv, q.c[k].bmin, q.c[k].bmax, a, c);
+        if (fused) slab_fused(o, inv, q.c[k].bmin, q.c[k].bmax, a, c);
+        else slab(o, inv, q.c[k].bmin, q.c[k].bmax, a, c);
         // An unused slot (+1e9, -1e9) is inside out; the min/max slab test
         // turns it into the box [-1e9, 1e9], so the reference enters it and
         // finds its empty leaf.  Skipping slots whose reference is the empty
@@ -1142,7 +1162,8 @@ static void collapse8(const Build& b, std::vector<Oct>& octs) {
 // strictly less), the hit children are deferred farthest first and the
 // nearest is entered.  The engine's kernels use the same network, so their
 // visit counts equal these.
-static Hit trace_bvh8(const Build& b, const std::vector<Oct>& octs, f4 posNear, f4 dirFar, Counters* cnt) {
+static Hit trace_bvh8(const Build& b, const std::vector<Oct>& octs, f4 posNear, f4 dirFar, Counters* cnt,
+                      bool fused = false) {
   const bool anyhit = (dirFar.w <= 0.0f);
   const float tFar = anyhit ? -dirFar.w : dirFar.w;
   Hit h;
@@ -1168,7 +1189,8 @@ static Hit trace_bvh8(const Build& b, const std::vector<Oct>& octs, f4 posNear, 
       uint32_t ch[8];
       for (int k = 0; k < 8; k++) {
         float a, c;
-        slab(o, inv, q.c[k].bmin, q.c[k].bmax, a, c);
+        if (fused) slab_fused(o, inv, q.c[k].bmin, q.c[k].bmax, a, c);
+        else slab(o, inv, q.c[k].bmin, q.c[k].bmax, a, c);
         const bool hit = (a <= c) && (c >= tNear) && (a <= h.t) && q.c[k].ref != 0x80000000u;
         tm[k] = hit ? a : MAXF;
         ch[k] = q.c[k].ref;
@@ -1709,8 +1731,9 @@ void hrto_octs(void* h, uint32_t* out) {
     }
 }
 
-// BVH8 traversal on the FP16 nodes, (t, primId) rule.
-void hrto_trace8(void* h, const float* rays, uint64_t n, uint32_t* hits, uint32_t* counters, int threads) {
+// BVH8 traversal on the FP16 nodes, (t, primId) rule; fused != 0: slab_fused box tests.
+void hrto_trace8(void* h, const float* rays, uint64_t n, uint32_t* hits, uint32_t* counters, int threads,
+                 int fused) {
   Build* b = static_cast<Build*>(h);
   if (b->o16.empty()) collapse8(*b, b->o16);
   const std::vector<Oct>& q = b->o16;
@@ -1723,7 +1746,7 @@ void hrto_trace8(void* h, const float* rays, uint64_t n, uint32_t* hits, uint32_
     std::memcpy(&p, rays + 8 * i, 16);
     std::memcpy(&d, rays + 8 * i + 4, 16);
     Counters c;
-    Hit r = trace_bvh8(*b, q, p, d, counters ? &c : nullptr);
+    Hit r = trace_bvh8(*b, q, p, d, counters ? &c : nullptr, fused != 0);
     std::memcpy(hits + 6 * i, &r, sizeof(Hit));
     if (counters) {
       counters[2 * i + 0] = uint32_t(c.pairs);
@@ -1732,11 +1755,11 @@ void hrto_trace8(void* h, const float* rays, uint64_t n, uint32_t* hits, uint32_
   }
 }
 
-// BVH4 traversal (mode 0: FP32 boxes, 1: FP16 boxes) and brute force with
-// the (t, primId) rule (mode 2).
+// BVH4 traversal (mode 0: FP32 boxes, 1: FP16 boxes, 3: FP16 boxes with
+// slab_fused box tests) and brute force with the (t, primId) rule (mode 2).
 void hrto_trace4(void* h, int mode, const float* rays, uint64_t n, uint32_t* hits, uint32_t* counters, int threads) {
   const Build* b = static_cast<Build*>(h);
-  const std::vector<Quad>* q = (mode == 2) ? nullptr : &quads_for(b, mode);
+  const std::vector<Quad>* q = (mode == 2) ? nullptr : &quads_for(b, mode == 3 ? 1 : mode);
 #ifdef _OPENMP
   if (threads > 0) omp_set_num_threads(threads);
 #pragma omp parallel for schedule(dynamic, 256) if (threads != 1)
@@ -1746,7 +1769,7 @@ void hrto_trace4(void* h, int mode, const float* rays, uint64_t n, uint32_t* hit
     std::memcpy(&p, rays + 8 * i, 16);
     std::memcpy(&d, rays + 8 * i + 4, 16);
     Counters c;
-    Hit r = q ? trace_bvh4(*b, *q, p, d, counters ? &c : nullptr) : trace_brute_lex(*b, p, d);
+    Hit r = q ? trace_bvh4(*b, *q, p, d, counters ? &c : nullptr, mode == 3) : trace_brute_lex(*b, p, d);
     std::memcpy(hits + 6 * i, &r, sizeof(Hit));
     if (counters && q) {
       counters[2 * i + 0] = uint32_t(c.pairs);
@@ -1769,6 +1792,10 @@ void hrto_safe_inverse(const float* d3, float* out3) {
 }
 void hrto_slab(const float* o3, const float* inv3, const float* bmin3, const float* bmax3, float* out2) {
   slab(f3{o3[0], o3[1], o3[2]}, f3{inv3[0], inv3[1], inv3[2]}, f3{bmin3[0], bmin3[1], bmin3[2]},
+       f3{bmax3[0], bmax3[1], bmax3[2]}, out2[0], out2[1]);
+}
+void hrto_slab_fused(const float* o3, const float* inv3, const float* bmin3, const float* bmax3, float* out2) {
+  slab_fused(f3{o3[0], o3[1], o3[2]}, f3{inv3[0], inv3[1], inv3[2]}, f3{bmin3[0], bmin3[1], bmin3[2]},
        f3{bmax3[0], bmax3[1], bmax3[2]}, out2[0], out2[1]);
 }
 

@@ -27,6 +27,7 @@ MODE_BRUTE = 2
 MODE4_FP32 = 0
 MODE4_FP16 = 1
 MODE4_BRUTE = 2
+MODE4_FP16_FUSED = 3  # FP16 nodes, box tests in the fused form of the engine's any-hit calls
 
 
 def f2h(v: float) -> int:
@@ -69,7 +70,7 @@ def lib():
         L.hrto_octs.restype = None
         L.hrto_octs.argtypes = [vp, vp]
         L.hrto_trace8.restype = None
-        L.hrto_trace8.argtypes = [vp, vp, u64, vp, vp, C.c_int]
+        L.hrto_trace8.argtypes = [vp, vp, u64, vp, vp, C.c_int, C.c_int]
         L.hrto_ploc.restype = None
         L.hrto_ploc.argtypes = [vp, u32]
         L.hrto_superblock_offset.restype = u32
@@ -92,6 +93,7 @@ def lib():
         L.hrto_pack_leaf.argtypes = [u32, u32]
         L.hrto_safe_inverse.argtypes = [vp, vp]
         L.hrto_slab.argtypes = [vp, vp, vp, vp, vp]
+        L.hrto_slab_fused.argtypes = [vp, vp, vp, vp, vp]
         L.hrto_quad_count.restype = u64
         L.hrto_quad_count.argtypes = [vp]
         L.hrto_quads.argtypes = [vp, C.c_int, vp]
@@ -243,13 +245,14 @@ class OracleBVH:
         lib().hrto_octs(self._h, _ptr(out))
         return out
 
-    def trace8(self, rays: np.ndarray, counters: bool = False, threads: int = 0):
-        """BVH8 traversal on FP16 nodes, nearest hit by (t, primId).  Returns (t, primId, u, v[, nodes, tris])."""
+    def trace8(self, rays: np.ndarray, counters: bool = False, threads: int = 0, fused: bool = False):
+        """BVH8 traversal on FP16 nodes, nearest hit by (t, primId); fused: the box tests in the fused
+        form of the engine's any-hit calls.  Returns (t, primId, u, v[, nodes, tris])."""
         rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
         n = rays.shape[0]
         hits = np.zeros((n, 6), dtype=np.uint32)
         cnt = np.zeros((n, 2), dtype=np.uint32) if counters else None
-        lib().hrto_trace8(self._h, _ptr(rays), n, _ptr(hits), _ptr(cnt) if counters else None, threads)
+        lib().hrto_trace8(self._h, _ptr(rays), n, _ptr(hits), _ptr(cnt) if counters else None, threads, int(fused))
         out = (hits[:, 0].view(np.float32).copy(), hits[:, 1].copy(), hits[:, 4].view(np.float32).copy(),
                hits[:, 5].view(np.float32).copy())
         return out + (cnt[:, 0].copy(), cnt[:, 1].copy()) if counters else out
@@ -359,6 +362,13 @@ def slab(o, inv, bmin, bmax) -> np.ndarray:
     args = [np.asarray(a, dtype=np.float32) for a in (o, inv, bmin, bmax)]
     out = np.zeros(2, dtype=np.float32)
     lib().hrto_slab(*(_ptr(a) for a in args), _ptr(out))
+    return out
+
+
+def slab_fused(o, inv, bmin, bmax) -> np.ndarray:
+    args = [np.asarray(a, dtype=np.float32) for a in (o, inv, bmin, bmax)]
+    out = np.zeros(2, dtype=np.float32)
+    lib().hrto_slab_fused(*(_ptr(a) for a in args), _ptr(out))
     return out
 
 

@@ -119,7 +119,8 @@ def test_bvh4_any_hit(gpu_ctx, name, gen, layout):
     g = trace(s, rays, "any")
     np.testing.assert_array_equal(g["hit"], p2 != INVALID)
     if layout != H.LAYOUT_AUTO:
-        mode = O.MODE4_FP16 if layout in (H.LAYOUT_BVH4_FP16, H.LAYOUT_AUTO_FP16) else O.MODE4_FP32
+        # hrt_trace_any on FP16 nodes tests boxes in the fused form
+        mode = O.MODE4_FP16_FUSED if layout in (H.LAYOUT_BVH4_FP16, H.LAYOUT_AUTO_FP16) else O.MODE4_FP32
         ob4 = ob
         if layout == H.LAYOUT_AUTO_FP16 and has_any_tree(s):
             ob4 = oracle_any_tree(v, f)[0]  # AUTO_FP16 answers any-hit on its own any-hit tree
@@ -279,7 +280,7 @@ def test_bvh8_any_hit(gpu_ctx, name, gen):
     np.testing.assert_array_equal(g["hit"], p2 != INVALID)
     if has_any_tree(s):
         ob, _ = oracle_any_tree(v, f)
-        _, p8, _, _, nn, nt = ob.trace8(anyr, counters=True)
+        _, p8, _, _, nn, nt = ob.trace8(anyr, counters=True, fused=True)
         gs = trace(s, rays, "any", stats=True)
         np.testing.assert_array_equal(gs["hit"], p8 != INVALID)
         np.testing.assert_array_equal(gs["pairs"], nn)

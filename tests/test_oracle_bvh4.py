@@ -7,7 +7,12 @@ Collapse: every BVH4 child box equals the BVH2 node box it came from, every
 triangle sits in exactly one reachable leaf, and the node count equals the
 number of even-depth BVH2 internal nodes.  Traversal: FP32 and FP16 BVH4 give
 the same (t, primId) as a brute force under the same (t, primId) rule.
+Slab tests: both forms (the reference's (b - o) * inv and the engine's fused
+fma(b, inv, -(o * inv)) of its FP16 any-hit calls) equal the correctly rounded
+results of exact rational arithmetic, and any-hit answers do not depend on
+the form.
 """
+from fractions import Fraction
 import numpy as np
 import pytest
 
@@ -170,3 +175,73 @@ def test_bvh4_any_hit_matches_bvh2(name, v, f):
     _, p16, _, _ = b.trace4(rays, O.MODE4_FP16)
     np.testing.assert_array_equal(p32 != 0xFFFFFFFF, p2 != 0xFFFFFFFF)
     np.testing.assert_array_equal(p16 != 0xFFFFFFFF, p2 != 0xFFFFFFFF)
+
+
+def _rn32(x: Fraction) -> np.float32:
+    """x rounded to the nearest float32, ties to even (exact: the candidates are compared as rationals)."""
+    f = np.float32(float(x))
+    best = None
+    for c in (np.nextafter(f, np.float32(-np.inf)), f, np.nextafter(f, np.float32(np.inf))):
+        if not np.isfinite(c):
+            continue
+        e = abs(Fraction(float(c)) - x)
+        if best is None or e < best[0] or (e == best[0] and int(np.float32(c).view(np.uint32)) % 2 == 0):
+            best = (e, np.float32(c))
+    return best[1]
+
+
+def _slab_exact(o, inv, lo, hi, fused):
+    ts = []
+    for a in range(3):
+        ia = Fraction(float(inv[a]))
+        if fused:
+            noi = -_rn32(Fraction(float(o[a])) * ia)
+            t0 = _rn32(Fraction(float(lo[a])) * ia + Fraction(float(noi)))
+            t1 = _rn32(Fraction(float(hi[a])) * ia + Fraction(float(noi)))
+        else:
+            t0 = _rn32(ia * Fraction(float(_rn32(Fraction(float(lo[a])) - Fraction(float(o[a]))))))
+            t1 = _rn32(ia * Fraction(float(_rn32(Fraction(float(hi[a])) - Fraction(float(o[a]))))))
+        ts.append((min(t0, t1), max(t0, t1)))
+    tmin = max(max(ts[0][0], ts[1][0]), ts[2][0])
+    tmax = min(min(ts[0][1], ts[1][1]), ts[2][1])
+    return np.array([tmin, tmax], np.float32)
+
+
+def test_slab_forms_equal_exact_rounding():
+    rng = np.random.default_rng(31)
+    for k in range(400):
+        scale = 10.0 ** rng.integers(-3, 4)
+        o = (rng.standard_normal(3) * scale).astype(np.float32)
+        d = rng.standard_normal(3).astype(np.float32)
+        inv = O.safe_inverse(d)
+        lo = (rng.standard_normal(3) * scale).astype(np.float32)
+        hi = (lo + np.abs(rng.standard_normal(3)) * scale).astype(np.float32)
+        for fused, fn in ((False, O.slab), (True, O.slab_fused)):
+            got = fn(o, inv, lo, hi)
+            want = _slab_exact(o, inv, lo, hi, fused)
+            np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32), err_msg=f"{k} fused={fused}")
+    # the two forms do round differently somewhere (else the fused restatement pins nothing)
+    diffs = 0
+    for k in range(400):
+        o = rng.standard_normal(3).astype(np.float32) * np.float32(50.0)
+        inv = O.safe_inverse(rng.standard_normal(3).astype(np.float32))
+        lo = rng.standard_normal(3).astype(np.float32)
+        hi = lo + np.float32(0.5)
+        diffs += int(np.any(O.slab(o, inv, lo, hi) != O.slab_fused(o, inv, lo, hi)))
+    assert diffs > 0
+
+
+@pytest.mark.parametrize("name,v,f", MESHES, ids=[m[0] for m in MESHES])
+def test_fused_box_test_keeps_any_hit_answers(name, v, f):
+    rays = _rays(v, 3000, 1300)
+    rays[:, 7] = -0.4 * float(np.max(v.max(0) - v.min(0)))
+    b = O.OracleBVH(v, f)
+    _, pb, _, _ = b.trace4(rays, O.MODE4_BRUTE)
+    _, p16, _, _, n16, _ = b.trace4(rays, O.MODE4_FP16, counters=True)
+    _, pf, _, _, nf, _ = b.trace4(rays, O.MODE4_FP16_FUSED, counters=True)
+    np.testing.assert_array_equal(pf != 0xFFFFFFFF, pb != 0xFFFFFFFF)
+    np.testing.assert_array_equal(p16 != 0xFFFFFFFF, pb != 0xFFFFFFFF)
+    _, p8, _, _ = b.trace8(rays, fused=True)
+    np.testing.assert_array_equal(p8 != 0xFFFFFFFF, pb != 0xFFFFFFFF)
+    # the fused form prunes at most a few boxes differently
+    assert abs(int(nf.sum()) - int(n16.sum())) <= 0.01 * int(n16.sum())
