@@ -49,14 +49,15 @@ NODE_BYTES = {"bvh2": 64, "bvh4": 128, "bvh4_fp16": 64, "bvh4_fp16_anytree": 64,
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "rtao_anyhit_traffic.json")
 
 
-def measured_traffic(layout: str):
-    """Profiled HBM bytes per launch for `layout`, or None if not profiled."""
+def measured_traffic(layout: str, ao_pass: str):
+    """Profiled HBM bytes per launch for `layout` and AO pass kind ("fused" or
+    "two_step"), or None if not profiled."""
     try:
         with open(TRAFFIC_FILE) as fh:
             rec = json.load(fh)
     except (OSError, ValueError):
         return None
-    if rec.get("any_hit_layout") != layout:
+    if rec.get("any_hit_layout") != layout or rec.get("ao_pass", "two_step") != ao_pass:
         return None
     return float(rec["hbm_bytes_per_launch"])
 
@@ -77,6 +78,9 @@ def main() -> int:
                          "bvh8_fp16_anytree (the same tree in 8-wide FP16 nodes)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-variants", action="store_true", help="skip timing the other any-hit layouts")
+    ap.add_argument("--two-step", action="store_true",
+                    help="rtao: write the AO rays to HBM and trace them (hrt_rtao_make_rays + hrt_trace_any) "
+                         "instead of the fused AO pass (hrt_rtao_trace)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = min(16, os.cpu_count())")
     ap.add_argument("--workload", choices=["rtao", "eye", "batched", "precision", "serving"], default="rtao",
                     help="rtao (default): the headline RTAO step; eye: 512x512 eye rays on the icosphere "
@@ -176,10 +180,15 @@ def main() -> int:
         nearest()
         if ev is not None:
             ev[1].record(stream)
-        scene.rtao_make_rays(d_prim, d_hits, npix, SPP, tmax, SEED, d_ao, stream=sptr)
-        if ev is not None:
-            ev[2].record(stream)
-        scene.trace_any(d_ao, n_ao, d_vis, stream=sptr)
+        if args.two_step:
+            scene.rtao_make_rays(d_prim, d_hits, npix, SPP, tmax, SEED, d_ao, stream=sptr)
+            if ev is not None:
+                ev[2].record(stream)
+            scene.trace_any(d_ao, n_ao, d_vis, stream=sptr)
+        else:
+            if ev is not None:
+                ev[2].record(stream)
+            scene.rtao_trace(d_prim, d_hits, npix, SPP, tmax, SEED, d_vis, stream=sptr)
         if ev is not None:
             ev[3].record(stream)
 
@@ -202,8 +211,39 @@ def main() -> int:
     any_ms = np.array([e[2].elapsed_time(e[3]) for e in events])
     rays_per_step = npix + n_ao
     value = rays_per_step / (ms_per_step * 1e-3) / 1e6
+    ao_kind = "raygen + any-hit" if args.two_step else "fused AO pass"
     log(f"step {ms_per_step:.3f} ms wall; nearest {near_ms.mean():.3f} ms, raygen {gen_ms.mean():.3f} ms, "
-        f"any-hit[{args.any_layout}] {any_ms.mean():.3f} ms")
+        f"any-hit[{args.any_layout}, {ao_kind}] {any_ms.mean():.3f} ms")
+
+    # the same AO rays through HBM (hrt_rtao_make_rays + hrt_trace_any): the
+    # counters, the layout comparison and the CPU baseline read them
+    d_vis_timed = d_vis.clone()
+    scene.rtao_make_rays(d_prim, d_hits, npix, SPP, tmax, SEED, d_ao, stream=sptr)
+    d_vis2 = torch.zeros_like(d_vis)
+    scene.trace_any(d_ao, n_ao, d_vis2, stream=sptr)
+    torch.cuda.synchronize()
+    ao_identical = bool(torch.equal(d_vis2, d_vis_timed))
+    ev2 = [[torch.cuda.Event(enable_timing=True) for _ in range(3)] for _ in range(5)]
+    for e in ev2:
+        e[0].record(stream)
+        scene.rtao_make_rays(d_prim, d_hits, npix, SPP, tmax, SEED, d_ao, stream=sptr)
+        e[1].record(stream)
+        scene.trace_any(d_ao, n_ao, d_vis2, stream=sptr)
+        e[2].record(stream)
+    ev3 = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(5)]
+    for e in ev3:
+        e[0].record(stream)
+        scene.rtao_trace(d_prim, d_hits, npix, SPP, tmax, SEED, d_vis2, stream=sptr)
+        e[1].record(stream)
+    torch.cuda.synchronize()
+    ao_pass = {
+        "fused_ms": round(float(np.mean([e[0].elapsed_time(e[1]) for e in ev3])), 4),
+        "two_step_raygen_ms": round(float(np.mean([e[0].elapsed_time(e[1]) for e in ev2])), 4),
+        "two_step_any_hit_ms": round(float(np.mean([e[1].elapsed_time(e[2]) for e in ev2])), 4),
+        "identical": ao_identical,
+    }
+    log(f"AO pass: fused {ao_pass['fused_ms']:.3f} ms; two-step raygen {ao_pass['two_step_raygen_ms']:.3f} + "
+        f"any-hit {ao_pass['two_step_any_hit_ms']:.3f} ms; identical {ao_identical}")
 
     # traversal counters (counting variant, untimed)
     def counters(layout_name: str, any_hit: bool):
@@ -216,12 +256,15 @@ def main() -> int:
             chk = torch.zeros_like(d_hits)
             scene.trace_nearest(d_prim, npix, chk, H.HIT_COMPACT, stream=sptr, stats=st)
         torch.cuda.synchronize()
-        same = bool(torch.equal(chk, d_vis if any_hit else d_hits))
+        same = bool(torch.equal(chk, d_vis_timed if any_hit else d_hits))
         return st.cpu().numpy().view(np.uint32).astype(np.int64), same
 
     sp, same_p = counters("bvh2", False)
     sa, same_a = counters(args.any_layout, True)
-    bytes_any = float(np.sum(33 + NODE_BYTES[args.any_layout] * sa[:, 0] + 60 * sa[:, 1]))
+    # per AO ray: its 32-byte record (the fused pass reads a 32-byte frame per
+    # pixel instead), the 1-byte answer, node and triangle bytes
+    ray_bytes = 32.0 if args.two_step else 32.0 / SPP
+    bytes_any = float(np.sum(ray_bytes + 1 + NODE_BYTES[args.any_layout] * sa[:, 0] + 60 * sa[:, 1]))
     bytes_near = float(np.sum(40 + 64 * sp[:, 0] + 60 * sp[:, 1]))
     achieved_any = bytes_any / (any_ms.mean() * 1e-3) / 1e9
     achieved_near = bytes_near / (near_ms.mean() * 1e-3) / 1e9
@@ -229,7 +272,7 @@ def main() -> int:
     # the other any-hit layouts, same rays, same answers
     variants = {}
     if not args.no_variants:
-        ref_vis = d_vis.clone()
+        ref_vis = d_vis_timed
         for name in ("bvh2", "bvh4", "bvh4_fp16", "bvh4_fp16_anytree", "bvh8_fp16_anytree"):
             scene.commit(layout=layouts[name])
             out = torch.zeros_like(d_vis)
@@ -262,7 +305,9 @@ def main() -> int:
         "data": "synthetic",
         "config": {
             "workload": "rtao: 1024x1024 primary nearest-hit (LBVH2, reference order) + 16 AO any-hit rays per "
-                        f"hit ({args.any_layout} of the same LBVH), 1,072,576-tri heightfield+boxes",
+                        f"hit ({args.any_layout}; "
+                        + ("rays written to HBM, then traced" if args.two_step else
+                           "rays made inside the any-hit traversal") + "), 1,072,576-tri heightfield+boxes",
             "triangles": int(faces.shape[0]),
             "primary_rays": int(npix),
             "ao_rays": int(n_ao),
@@ -271,12 +316,12 @@ def main() -> int:
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"any-hit traversal ({args.any_layout})",
+            "kernel": f"any-hit traversal ({args.any_layout}, {ao_kind})",
             "achieved": round(achieved_any, 1),
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": round(achieved_any / HBM_PEAK_GBPS, 4),
-            "traffic": measured_traffic(args.any_layout),
+            "traffic": measured_traffic(args.any_layout, "two_step" if args.two_step else "fused"),
             "traffic_source": "profiles/rtao_anyhit_traffic.json (rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0)",
             "bytes_per_launch": bytes_any,
             "mean_launch_ms": round(float(any_ms.mean()), 4),
@@ -293,6 +338,7 @@ def main() -> int:
         "bvh4_collapse_ms": round(collapse_ms, 3),
         "any_hit_tree_build_ms": round(any_tree_ms, 3),
         "counting_variant_identical": same_p and same_a,
+        "ao_pass": ao_pass,
         "any_hit_layouts": variants,
         "cpu_baseline": None,
     }
@@ -306,7 +352,7 @@ def main() -> int:
         ao_host = d_ao.cpu().numpy()
         ao_host[:, 7] = -np.abs(ao_host[:, 7])  # any-hit convention: negative tFar
         hits_host = d_hits.cpu().numpy().view(np.uint32)
-        vis_host = d_vis.cpu().numpy().astype(bool)
+        vis_host = d_vis_timed.cpu().numpy().astype(bool)
         t0 = time.perf_counter()
         tp, pp, _, _ = ob.trace(prim_rays, threads=threads)
         _, pa, _, _ = ob.trace(ao_host, threads=threads)

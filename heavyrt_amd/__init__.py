@@ -64,7 +64,7 @@ EXPORTS = [
     "hrt_scene_set_keep_build_arrays", "hrt_scene_clear_geom", "hrt_scene_clear_instances",
     "hrt_add_triangles", "hrt_add_instance", "hrt_commit", "hrt_trace_nearest", "hrt_trace_any",
     "hrt_trace_nearest_stats", "hrt_trace_any_stats", "hrt_query_nearest_one", "hrt_geom_count",
-    "hrt_instance_count", "hrt_debug_dump", "hrt_last_error", "hrt_rtao_make_rays", "hrt_eye_make_rays", "hrt_eye_render", "hrt_scene_serialize", "hrt_scene_deserialize",
+    "hrt_instance_count", "hrt_debug_dump", "hrt_last_error", "hrt_rtao_make_rays", "hrt_rtao_trace", "hrt_eye_make_rays", "hrt_eye_render", "hrt_scene_serialize", "hrt_scene_deserialize",
     "hrt_trace_nearest_host", "hrt_trace_any_host", "hrt_update_triangles", "hrt_update_instance",
     "hrt_batch_create", "hrt_batch_destroy", "hrt_batch_buffers", "hrt_batch_run", "hrt_batch_build",
     "hrt_batch_trace", "hrt_batch_candidate_count", "hrt_batch_candidate_info", "hrt_batch_set_candidate",
@@ -134,6 +134,7 @@ def lib():
         L.hrt_instance_count.restype = u32
         L.hrt_debug_dump.argtypes = [vp, u32, i32, vp, sz]
         L.hrt_rtao_make_rays.argtypes = [vp, vp, vp, sz, u32, C.c_float, C.c_uint64, vp, vp]
+        L.hrt_rtao_trace.argtypes = [vp, vp, vp, sz, u32, C.c_float, C.c_uint64, vp, vp]
         L.hrt_eye_make_rays.argtypes = [vp, vp, vp, u32, u32, vp, vp, vp]
         L.hrt_eye_render.argtypes = [vp, vp, vp, u32, u32, vp, u32, vp, vp]
         L.hrt_scene_serialize.argtypes = [vp, vp, sz, C.POINTER(C.c_size_t)]
@@ -321,6 +322,12 @@ class Scene:
                        stream: int = 0):
         _check(lib().hrt_rtao_make_rays(self._h, _dev_ptr(prim_rays), _dev_ptr(prim_hits_compact), npix, spp,
                                         float(tmax), seed & 0xFFFFFFFFFFFFFFFF, _dev_ptr(out_rays), stream or None))
+
+    def rtao_trace(self, prim_rays, prim_hits_compact, npix: int, spp: int, tmax: float, seed: int, out_hits,
+                   stream: int = 0):
+        """rtao_make_rays + trace_any in one call (hrt_rtao_trace): one byte per AO ray in out_hits."""
+        _check(lib().hrt_rtao_trace(self._h, _dev_ptr(prim_rays), _dev_ptr(prim_hits_compact), npix, spp,
+                                    float(tmax), seed & 0xFFFFFFFFFFFFFFFF, _dev_ptr(out_hits), stream or None))
 
     def serialize(self) -> bytes:
         """The committed scene with its trees as bytes (hrt_scene_serialize), for a warm start."""

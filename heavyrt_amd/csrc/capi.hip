@@ -622,6 +622,57 @@ hrt_status hrt_rtao_make_rays(hrt_scene* s, const float* primaryRays, const void
   return HRT_OK;
 }
 
+hrt_status hrt_rtao_trace(hrt_scene* s, const float* primaryRays, const void* primaryHitsCompact, size_t nPixels,
+                          uint32_t raysPerHit, float tMax, uint64_t seed, uint8_t* outHits, void* stream) {
+  const hrt::Blas* b = nullptr;
+  uint32_t inst = 0;
+  hrt_status st = select_blas(s, &b, &inst);
+  if (st != HRT_OK) return st;
+  if (nPixels == 0 || raysPerHit == 0) return HRT_OK;
+  if (!primaryRays || !primaryHitsCompact || !outHits) return fail(HRT_ERR_INVALID_ARG, "null buffer");
+  const uint64_t n = uint64_t(nPixels) * raysPerHit;
+  if (n / raysPerHit != nPixels || n > 0xFFFFFFFFull)
+    return fail(HRT_ERR_INVALID_ARG, "nPixels * raysPerHit must stay below 2^32");
+  hipStream_t hs = stream ? static_cast<hipStream_t>(stream) : s->ctx->stream;
+  hipError_t e = hipSetDevice(s->ctx->device);
+  if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
+  const hrt::Ray* prim = reinterpret_cast<const hrt::Ray*>(primaryRays);
+  // the tree any-hit rays use, as in trace_common
+  const int lay = s->layout;
+  const bool auto16 = lay == HRT_LAYOUT_AUTO_FP16 || lay == HRT_LAYOUT_AUTO_FP16_WIDE;
+  const bool wide = lay == HRT_LAYOUT_AUTO_FP16_WIDE && b->any_tree && b->any_tree->octs16;
+  if (lay == HRT_LAYOUT_BVH2 || wide) {
+    // BVH2 and 8-wide nodes: the rays go through HBM
+    void* rays = nullptr;
+    if ((e = hipMallocAsync(&rays, sizeof(hrt::Ray) * n, hs)) != hipSuccess) return fail_hip(e, "rtao rays");
+    hrt::TraceArgs a;
+    a.rays = static_cast<const hrt::Ray*>(rays);
+    a.n = n;
+    a.out = outHits;
+    a.format = HRT_HIT_FULL;
+    a.anyhit = 1;
+    a.stats = nullptr;
+    a.inst_id = inst;
+    a.geom_id = s->instances[inst].geom;
+    e = hrt::rtao_make_rays(*b, prim, primaryHitsCompact, nPixels, raysPerHit, tMax, seed,
+                            static_cast<hrt::Ray*>(rays), hs);
+    if (e == hipSuccess) e = wide ? hrt::trace_blas8(*b->any_tree, a, hs) : hrt::trace_blas(*b, a, hs);
+    const hipError_t fe = hipFreeAsync(rays, hs);
+    if (e == hipSuccess) e = fe;
+    return e == hipSuccess ? HRT_OK : fail_hip(e, "rtao trace");
+  }
+  const hrt::Blas& tree = (auto16 && b->any_tree) ? *b->any_tree : *b;
+  const bool fp16 = lay == HRT_LAYOUT_BVH4_FP16 || auto16;
+  void* frames = nullptr;
+  if ((e = hipMallocAsync(&frames, 32ull * nPixels, hs)) != hipSuccess) return fail_hip(e, "rtao frames");
+  hrt::AoFrame* f = static_cast<hrt::AoFrame*>(frames);
+  e = hrt::rtao_frames(*b, prim, primaryHitsCompact, nPixels, f, hs);
+  if (e == hipSuccess) e = hrt::trace_ao4(tree, fp16, f, nPixels, raysPerHit, tMax, seed, outHits, hs);
+  const hipError_t fe = hipFreeAsync(frames, hs);
+  if (e == hipSuccess) e = fe;
+  return e == hipSuccess ? HRT_OK : fail_hip(e, "rtao trace");
+}
+
 namespace {
 hrt_status eye_args(const float* proj_inv, const float* world_view_inv, uint32_t width, uint32_t height,
                     hrt::EyeCamera* cam) {

@@ -156,6 +156,15 @@ hipError_t trace_two_level(const Blas& tlas, const InstDev* inst_leaf_order, con
 // Ambient-occlusion rays from primary hits (compact records), spp per pixel.
 hipError_t rtao_make_rays(const Blas& b, const Ray* prim_rays, const void* prim_hits_compact, uint64_t npix,
                           uint32_t spp, float tmax, uint64_t seed, Ray* out, hipStream_t stream);
+// One AO frame per pixel (ao.h) for trace_ao4.
+struct AoFrame;
+hipError_t rtao_frames(const Blas& b, const Ray* prim_rays, const void* prim_hits_compact, uint64_t npix,
+                       AoFrame* out, hipStream_t stream);
+// Any-hit of every AO sample of npix frames on b's BVH4 nodes (FP16 or FP32),
+// each ray made in registers exactly as rtao_make_rays writes it; out gets one
+// byte per sample.  npix * spp must stay below 2^32.
+hipError_t trace_ao4(const Blas& b, bool fp16, const AoFrame* frames, uint64_t npix, uint32_t spp, float tmax,
+                     uint64_t seed, uint8_t* out, hipStream_t stream);
 
 // Eye rays of a pinhole camera, one per pixel, in SuperBlockIndex2DOpt order
 // (width and height multiples of 8); packed_xy (nullable) gets y << 16 | x.

@@ -1,33 +1,15 @@
-// rtao.hip -- device-side ambient-occlusion ray generation.
-//
-// The reference renderer builds AO rays one at a time on the host
-// (RTAO::kernel_InitAORay, examples/02_rtao/rtao.cpp:147-165, with
-// MapSampleToCosineDistribution / OffsRayPos, examples/render_common.h:56-112):
-// cosine-weighted directions about the surface normal, the origin pushed off
-// the surface by epsilonOfPos(hit) along the normal, tNear = 0,
-// tFar = the AO radius.  Here one lane makes one ray, directly in HBM, from
-// the primary hit: geometric normal of the hit triangle facing the viewer,
-// hit point = origin + 0.9999995 * t * dir (rtao.cpp:92), two PCG32 numbers
-// per ray (sequence = global ray index, so the rays do not depend on the
-// launch shape).  Pixels whose primary ray missed get an empty ray
-// (tNear = tFar = 0) that can hit nothing.
+// rtao.hip -- device-side ambient-occlusion ray generation: one lane makes
+// one AO ray in HBM (k_ao_rays), or one pixel's frame for the traversal that
+// makes its rays in registers (k_ao_frames).  The arithmetic is in ao.h.
 #include <hip/hip_runtime.h>
 
+#include "ao.h"
 #include "hrt_internal.h"
 
 namespace hrt {
 namespace {
 
 constexpr int kBlock = 256;
-constexpr float kGEps = 2e-5f;  // render_common.h:88
-
-__device__ inline uint32_t pcg32_step(uint64_t& state, uint64_t inc) {
-  const uint64_t old = state;
-  state = old * 6364136223846793005ull + inc;
-  const uint32_t xs = uint32_t(((old >> 18u) ^ old) >> 27u);
-  const uint32_t rot = uint32_t(old >> 59u);
-  return (xs >> rot) | (xs << ((32u - rot) & 31u));
-}
 
 __global__ __launch_bounds__(kBlock) void k_ao_rays(const Ray* __restrict__ prim_rays,
                                                     const uint2* __restrict__ prim_hits,
@@ -36,56 +18,18 @@ __global__ __launch_bounds__(kBlock) void k_ao_rays(const Ray* __restrict__ prim
                                                     uint32_t spp, float tmax, uint64_t seed, Ray* __restrict__ out) {
   const uint64_t k = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (k >= npix * spp) return;
-  const uint64_t pix = k / spp;
-  const uint2 h = prim_hits[pix];
-  Ray r;
-  if (h.y >= ntris) {  // miss (or an id outside this geometry)
-    r.pos_near = make_float4(0.f, 0.f, 0.f, 0.f);
-    r.dir_far = make_float4(0.f, 1.f, 0.f, 0.f);
-    out[k] = r;
-    return;
-  }
-  const Ray p = prim_rays[pix];
-  const float t = __uint_as_float(h.x);
-  const float3 o = make_float3(p.pos_near.x, p.pos_near.y, p.pos_near.z);
-  const float3 d = make_float3(p.dir_far.x, p.dir_far.y, p.dir_far.z);
-  const float tt = 0.9999995f * t;
-  const float3 hp = make_float3(o.x + tt * d.x, o.y + tt * d.y, o.z + tt * d.z);
-  const float4 A = verts[idx[3 * h.y + 0]], B = verts[idx[3 * h.y + 1]], C = verts[idx[3 * h.y + 2]];
-  const float3 e1 = make_float3(B.x - A.x, B.y - A.y, B.z - A.z);
-  const float3 e2 = make_float3(C.x - A.x, C.y - A.y, C.z - A.z);
-  float3 n = make_float3(e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z, e1.x * e2.y - e1.y * e2.x);
-  const float inv_len = 1.0f / sqrtf(n.x * n.x + n.y * n.y + n.z * n.z);
-  n = make_float3(n.x * inv_len, n.y * inv_len, n.z * inv_len);
-  if (n.x * d.x + n.y * d.y + n.z * d.z > 0.0f) n = make_float3(-n.x, -n.y, -n.z);
+  const AoFrame f = ao_frame(prim_rays, prim_hits, verts, idx, ntris, k / spp);
+  out[k] = ao_ray(f, k, tmax, seed);
+}
 
-  // PCG32(seed, sequence k): pcg32_srandom_r then two draws
-  const uint64_t inc = (k << 1u) | 1u;
-  uint64_t st = 0;
-  (void)pcg32_step(st, inc);
-  st += seed;
-  (void)pcg32_step(st, inc);
-  const float r1 = float(pcg32_step(st, inc) >> 8) * (1.0f / 16777216.0f);
-  const float r2 = float(pcg32_step(st, inc) >> 8) * (1.0f / 16777216.0f);
-  // cosine-weighted hemisphere (power 1 lobe, render_common.h:62-70)
-  float sphi, cphi;
-  sincosf(6.28318530717958647692f * r1, &sphi, &cphi);
-  const float cth = sqrtf(1.0f - r2);
-  const float sth = sqrtf(r2);
-  // orthonormal basis about n (Duff et al., "Building an Orthonormal Basis, Revisited", JCGT 2017)
-  const float sgn = copysignf(1.0f, n.z);
-  const float a = -1.0f / (sgn + n.z);
-  const float b = n.x * n.y * a;
-  const float3 tx = make_float3(1.0f + sgn * n.x * n.x * a, sgn * b, -sgn * n.x);
-  const float3 ty = make_float3(b, sgn + n.y * n.y * a, -n.y);
-  const float lx = sth * cphi, ly = sth * sphi;
-  const float3 dir = make_float3(tx.x * lx + ty.x * ly + n.x * cth, tx.y * lx + ty.y * ly + n.y * cth,
-                                 tx.z * lx + ty.z * ly + n.z * cth);
-  // OffsRayPos: the direction is in n's hemisphere, so the offset is along +n
-  const float m = fmaxf(fmaxf(fabsf(hp.x), fmaxf(fabsf(hp.y), fabsf(hp.z))), 2.0f * kGEps) * kGEps;
-  r.pos_near = make_float4(hp.x + m * n.x, hp.y + m * n.y, hp.z + m * n.z, 0.0f);
-  r.dir_far = make_float4(dir.x, dir.y, dir.z, tmax);
-  out[k] = r;
+__global__ __launch_bounds__(kBlock) void k_ao_frames(const Ray* __restrict__ prim_rays,
+                                                      const uint2* __restrict__ prim_hits,
+                                                      const float4* __restrict__ verts,
+                                                      const uint32_t* __restrict__ idx, uint32_t ntris,
+                                                      uint64_t npix, AoFrame* __restrict__ out) {
+  const uint64_t p = uint64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (p >= npix) return;
+  out[p] = ao_frame(prim_rays, prim_hits, verts, idx, ntris, p);
 }
 
 }  // namespace
@@ -97,6 +41,15 @@ hipError_t rtao_make_rays(const Blas& b, const Ray* prim_rays, const void* prim_
   const uint32_t blocks = uint32_t((n + kBlock - 1) / kBlock);
   k_ao_rays<<<blocks, kBlock, 0, stream>>>(prim_rays, static_cast<const uint2*>(prim_hits_compact), b.verts, b.idx,
                                            b.ntris, npix, spp, tmax, seed, out);
+  return hipGetLastError();
+}
+
+hipError_t rtao_frames(const Blas& b, const Ray* prim_rays, const void* prim_hits_compact, uint64_t npix,
+                       AoFrame* out, hipStream_t stream) {
+  if (npix == 0) return hipSuccess;
+  const uint32_t blocks = uint32_t((npix + kBlock - 1) / kBlock);
+  k_ao_frames<<<blocks, kBlock, 0, stream>>>(prim_rays, static_cast<const uint2*>(prim_hits_compact), b.verts,
+                                             b.idx, b.ntris, npix, out);
   return hipGetLastError();
 }
 

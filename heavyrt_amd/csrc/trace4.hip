@@ -17,6 +17,7 @@
 
 #include <algorithm>
 
+#include "ao.h"
 #include "hrt_internal.h"
 
 namespace hrt {
@@ -175,11 +176,27 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_simple(const void* __restric
 // the wave tests leaves together; the answer does not depend on the order.
 // (Nearest hits stay on k_trace4_simple: on deep, heavily overlapping trees
 // its 48-entry LDS stack beats a short LDS stack with global spill.)
-template <bool kFp16>
+// Src gives ray i: StoredRays reads it from HBM, AoRays makes AO sample i
+// from its pixel's frame.  A ray with an empty interval (tFar <= tNear, or a
+// NaN bound) can pass no triangle's t test and finishes at once.
+struct StoredRays {
+  const Ray* __restrict__ rays;
+  __device__ Ray ray(uint64_t i) const { return rays[i]; }
+};
+
+struct AoRays {
+  const AoFrame* __restrict__ frames;
+  uint32_t spp;
+  float tmax;
+  uint64_t seed;
+  // 32-bit division: the launcher keeps npix * spp below 2^32
+  __device__ Ray ray(uint64_t i) const { return ao_ray(frames[uint32_t(i) / spp], i, tmax, seed); }
+};
+
+template <bool kFp16, class Src>
 __global__ __launch_bounds__(kBlock4) void k_trace4_any_persistent(const void* __restrict__ quads,
-                                                                   const TriPre* __restrict__ tris,
-                                                                   const Ray* __restrict__ rays, uint64_t n,
-                                                                   uint8_t* __restrict__ out,
+                                                                   const TriPre* __restrict__ tris, Src src,
+                                                                   uint64_t n, uint8_t* __restrict__ out,
                                                                    uint32_t* __restrict__ queue,
                                                                    uint32_t* __restrict__ spill) {
   __shared__ uint32_t stack_mem[kShortStack * kBlock4];
@@ -208,7 +225,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_any_persistent(const void* _
       if (!active) {
         if (idx < n) {
           rid = idx;
-          const Ray r = rays[idx];
+          const Ray r = src.ray(idx);
           o = make_float3(r.pos_near.x, r.pos_near.y, r.pos_near.z);
           d = make_float3(r.dir_far.x, r.dir_far.y, r.dir_far.z);
           tnear = r.pos_near.w;
@@ -217,7 +234,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_any_persistent(const void* _
           inv = safe_inverse(d);
           oi = neg_o_inv(o, inv);
           top = 0;
-          cur = 0;
+          cur = (h.t > tnear) ? 0u : kInvalid;
           leaf = 0;
           active = true;
         }
@@ -263,14 +280,14 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_any_persistent(const void* _
   }
 }
 
-template <bool kFp16>
-hipError_t launch_any4(const Blas& b, const TraceArgs& a, hipStream_t stream) {
+template <bool kFp16, class Src>
+hipError_t launch_any4(const Blas& b, Src src, uint64_t n, uint8_t* out, hipStream_t stream) {
   int dev = 0, cus = 256, per_cu = 0;
   (void)hipGetDevice(&dev);
   (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  const void* kfn = reinterpret_cast<const void*>(&k_trace4_any_persistent<kFp16>);
+  const void* kfn = reinterpret_cast<const void*>(&k_trace4_any_persistent<kFp16, Src>);
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kBlock4, 0) != hipSuccess || per_cu <= 0) per_cu = 8;
-  const uint64_t want = (a.n + kBlock4 - 1) / kBlock4;
+  const uint64_t want = (n + kBlock4 - 1) / kBlock4;
   const int grid = int(std::min<uint64_t>(want, uint64_t(std::max(cus, 1)) * uint64_t(per_cu)));
   const void* q = kFp16 ? static_cast<const void*>(b.quads16) : static_cast<const void*>(b.quads32);
   // queue counter + spill area for stack entries beyond the LDS part
@@ -279,8 +296,7 @@ hipError_t launch_any4(const Blas& b, const TraceArgs& a, hipStream_t stream) {
   hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&buf), sizeof(uint32_t) * (64 + spill_words), stream);
   if (e != hipSuccess) return e;
   k_reset_counter4<<<1, 1, 0, stream>>>(buf);
-  k_trace4_any_persistent<kFp16><<<grid, kBlock4, 0, stream>>>(q, b.tris, a.rays, a.n, static_cast<uint8_t*>(a.out),
-                                                               buf, buf + 64);
+  k_trace4_any_persistent<kFp16, Src><<<grid, kBlock4, 0, stream>>>(q, b.tris, src, n, out, buf, buf + 64);
   e = hipGetLastError();
   hipError_t f = hipFreeAsync(buf, stream);
   return e != hipSuccess ? e : f;
@@ -539,10 +555,23 @@ hipError_t trace_blas8(const Blas& b, const TraceArgs& a, hipStream_t stream) {
   return launch_simple8(b, a, stream);
 }
 
+hipError_t trace_ao4(const Blas& b, bool fp16, const AoFrame* frames, uint64_t npix, uint32_t spp, float tmax,
+                     uint64_t seed, uint8_t* out, hipStream_t stream) {
+  const uint64_t n = npix * spp;
+  if (n == 0) return hipSuccess;
+  if (!b.quads32 || !b.quads16 || n > 0xFFFFFFFFull) return hipErrorInvalidValue;
+  const AoRays src{frames, spp, tmax, seed};
+  return fp16 ? launch_any4<true>(b, src, n, out, stream) : launch_any4<false>(b, src, n, out, stream);
+}
+
 hipError_t trace_blas4(const Blas& b, const TraceArgs& a, bool fp16, hipStream_t stream) {
   if (a.n == 0) return hipSuccess;
   if (!b.quads32 || !b.quads16) return hipErrorInvalidValue;
-  if (a.anyhit && !a.stats) return fp16 ? launch_any4<true>(b, a, stream) : launch_any4<false>(b, a, stream);
+  if (a.anyhit && !a.stats) {
+    const StoredRays src{a.rays};
+    uint8_t* out = static_cast<uint8_t*>(a.out);
+    return fp16 ? launch_any4<true>(b, src, a.n, out, stream) : launch_any4<false>(b, src, a.n, out, stream);
+  }
   return fp16 ? launch_simple4<true>(b, a, stream) : launch_simple4<false>(b, a, stream);
 }
 

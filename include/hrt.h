@@ -227,6 +227,18 @@ hrt_status hrt_rtao_make_rays(hrt_scene* scene, const float* primaryRays, const 
                               size_t nPixels, uint32_t raysPerHit, float tMax, uint64_t seed, float* outRays,
                               void* stream);
 
+/* The AO pass of the RTAO renderer in one call: the same rays as
+ * hrt_rtao_make_rays (same arguments), traced as hrt_trace_any traces them,
+ * one byte per ray in outHits (1 = occluded), ray k = pixel * raysPerHit + s.
+ * With BVH4 nodes (HRT_LAYOUT_BVH4, _BVH4_FP16, _AUTO, _AUTO_FP16) the
+ * traversal makes each ray in registers from a 32-byte per-pixel frame, so no
+ * ray touches HBM; other layouts write the rays to a temporary buffer first.
+ * nPixels * raysPerHit must stay below 2^32.  No counterpart in the reference
+ * (its renderer calls RayQuery_AnyHit per ray). */
+hrt_status hrt_rtao_trace(hrt_scene* scene, const float* primaryRays, const void* primaryHitsCompact,
+                          size_t nPixels, uint32_t raysPerHit, float tMax, uint64_t seed, uint8_t* outHits,
+                          void* stream);
+
 /* Device-side eye rays of the 01_eye_rays renderer (EyeRayCaster::
  * kernel_PackXY + kernel_InitEyeRay, examples/01_eye_rays/eye_ray.cpp:26-47,
  * 114-138; EyeRayDirNormalized, transform_ray3f,

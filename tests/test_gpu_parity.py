@@ -190,6 +190,11 @@ def test_rtao_config_parity(gpu_ctx):
     occl = vis[live].mean()
     say(f"rtao occlusion fraction {occl:.3f}")
     assert 0.01 < occl < 0.99
+    # the fused AO pass (rays made inside the traversal) gives the same bytes
+    dvis2 = torch.full((npix * spp,), 7, dtype=torch.uint8, device="cuda")
+    scene.rtao_trace(dprim, dhits, npix, spp, tmax, 1234, dvis2, stream=stream)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(dvis2.cpu().numpy(), dvis.cpu().numpy())
     scene.close()
 
 
