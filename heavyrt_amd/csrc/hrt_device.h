@@ -194,6 +194,13 @@ __device__ inline float3 neg_o_inv(float3 o, float3 inv) {
   return make_float3(-(o.x * inv.x), -(o.y * inv.y), -(o.z * inv.z));
 }
 
+// slab_fused needs a finite -(o * inv) on every axis.  It overflows for an
+// axis-parallel direction (safe_inverse gives +-1e36) with |o| above ~340 on
+// that axis; such a ray keeps the unfused slab.
+__device__ inline bool fused_ok(float3 oi) {
+  return __builtin_isfinite(oi.x) && __builtin_isfinite(oi.y) && __builtin_isfinite(oi.z);
+}
+
 // The slab test in the fused form t = fma(b, inv, -(o * inv)): one rounding
 // per plane instead of two, and with an FP16 box value as a direct operand of
 // v_fma_mix_f32 (widening FP16 to FP32 is exact) no conversion either.  It is
@@ -203,7 +210,9 @@ __device__ inline float3 neg_o_inv(float3 o, float3 inv) {
 // box holding such a triangle is rejected.  Both forms round each plane
 // distance by about half an ulp of the larger of |b| and |o| (the difference
 // b - o, or the product o * inv, is rounded), against boxes widened outward
-// by at least one FP16 ulp.
+// by at least one FP16 ulp.  Valid only while fused_ok(oi): with a finite
+// -(o * inv) each plane is the exact b * inv - o * inv rounded once (no
+// intermediate overflow), so its sign and order are right.
 __device__ inline float2 slab_fused(float3 oi, float3 inv, float bminx, float bminy, float bminz, float bmaxx,
                                     float bmaxy, float bmaxz) {
   const float lo = fmaf(bminx, inv.x, oi.x);

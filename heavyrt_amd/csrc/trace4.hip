@@ -39,41 +39,49 @@ __global__ void k_reset_counter4(uint32_t* c) { *c = 0u; }
 
 // Loads one BVH4 node, tests its four children, and returns the entry
 // distances (kMaxFloat for a miss) and references in reference order.
-// kFused (FP16 nodes of any-hit traversals only): slab_fused with oi = -(o * inv).
+// kFused (FP16 nodes of any-hit traversals only): slab_fused with
+// oi = -(o * inv) for a lane whose fz = fused_ok(oi), slab otherwise.
 template <bool kFp16, bool kFused = false>
 __device__ inline void visit_quad(const void* __restrict__ quads, uint32_t q, float3 o, float3 inv, float3 oi,
-                                  float tnear, float tcull, float tm[4], uint32_t ch[4]) {
+                                  bool fz, float tnear, float tcull, float tm[4], uint32_t ch[4]) {
   static_assert(kFp16 || !kFused, "the fused box test is for FP16 nodes");
-  float lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+  float2 tt[4];
   if (kFp16) {
+    // per child {minx | miny << 16, minz | maxx << 16, maxy | maxz << 16, ref};
+    // the FP16 values are widened inside each branch, so the fused one feeds
+    // them straight to v_fma_mix_f32
     const uint4* p = reinterpret_cast<const uint4*>(static_cast<const Quad16*>(quads) + q);
+    uint4 w[4];
 #pragma unroll
     for (int k = 0; k < 4; k++) {
-      const uint4 w = p[k];
-      lox[k] = h2f_lo(w.x);
-      loy[k] = h2f_hi(w.x);
-      loz[k] = h2f_lo(w.y);
-      hix[k] = h2f_hi(w.y);
-      hiy[k] = h2f_lo(w.z);
-      hiz[k] = h2f_hi(w.z);
-      ch[k] = w.w;
+      w[k] = p[k];
+      ch[k] = w[k].w;
+    }
+    if (kFused && fz) {
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        tt[k] = slab_fused(oi, inv, h2f_lo(w[k].x), h2f_hi(w[k].x), h2f_lo(w[k].y), h2f_hi(w[k].y),
+                           h2f_lo(w[k].z), h2f_hi(w[k].z));
+    } else {
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        tt[k] = slab(o, inv, h2f_lo(w[k].x), h2f_hi(w[k].x), h2f_lo(w[k].y), h2f_hi(w[k].y), h2f_lo(w[k].z),
+                     h2f_hi(w[k].z));
     }
   } else {
     const float4* p = reinterpret_cast<const float4*>(static_cast<const Quad32*>(quads) + q);
     const float4 a = p[0], b = p[1], c = p[2], d = p[3], e = p[4], f = p[5];
     const uint4 r = reinterpret_cast<const uint4*>(p)[6];
-    lox[0] = a.x; lox[1] = a.y; lox[2] = a.z; lox[3] = a.w;
-    hix[0] = b.x; hix[1] = b.y; hix[2] = b.z; hix[3] = b.w;
-    loy[0] = c.x; loy[1] = c.y; loy[2] = c.z; loy[3] = c.w;
-    hiy[0] = d.x; hiy[1] = d.y; hiy[2] = d.z; hiy[3] = d.w;
-    loz[0] = e.x; loz[1] = e.y; loz[2] = e.z; loz[3] = e.w;
-    hiz[0] = f.x; hiz[1] = f.y; hiz[2] = f.z; hiz[3] = f.w;
+    const float lox[4] = {a.x, a.y, a.z, a.w}, hix[4] = {b.x, b.y, b.z, b.w};
+    const float loy[4] = {c.x, c.y, c.z, c.w}, hiy[4] = {d.x, d.y, d.z, d.w};
+    const float loz[4] = {e.x, e.y, e.z, e.w}, hiz[4] = {f.x, f.y, f.z, f.w};
     ch[0] = r.x; ch[1] = r.y; ch[2] = r.z; ch[3] = r.w;
+#pragma unroll
+    for (int k = 0; k < 4; k++) tt[k] = slab(o, inv, lox[k], loy[k], loz[k], hix[k], hiy[k], hiz[k]);
   }
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    const float2 t = kFused ? slab_fused(oi, inv, lox[k], loy[k], loz[k], hix[k], hiy[k], hiz[k])
-                            : slab(o, inv, lox[k], loy[k], loz[k], hix[k], hiy[k], hiz[k]);
+    const float2 t = tt[k];
     // An unused slot (box +1e9..-1e9, empty leaf 0x80000000) would pass the
     // min/max slab test as the box [-1e9, 1e9]; it holds no triangle, so it
     // is never entered.
@@ -130,6 +138,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_simple(const void* __restric
   const bool any = kForceAny || (r.dir_far.w <= 0.0f);
   const float3 inv = safe_inverse(d);
   const float3 oi = neg_o_inv(o, inv);
+  const bool fz = fused_ok(oi);
   HitRec h;
   h.t = kForceAny ? fabsf(r.dir_far.w) : (any ? -r.dir_far.w : r.dir_far.w);
   h.prim = kInvalid;
@@ -142,7 +151,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_simple(const void* __restric
     if ((cur & kLeafBit) == 0) {
       float tm[4];
       uint32_t ch[4];
-      visit_quad<kFp16, kForceAny && kFp16>(quads, cur, o, inv, oi, tnear, h.t, tm, ch);
+      visit_quad<kFp16, kForceAny && kFp16>(quads, cur, o, inv, oi, fz, tnear, h.t, tm, ch);
       if (kStats) nvisits++;
       if (tm[3] < kMaxFloat) stack[(top++) * kBlock4] = ch[3];
       if (tm[2] < kMaxFloat) stack[(top++) * kBlock4] = ch[2];
@@ -194,7 +203,7 @@ struct AoRays {
 };
 
 template <bool kFp16, class Src>
-__global__ __launch_bounds__(kBlock4) void k_trace4_any_persistent(const void* __restrict__ quads,
+__global__ __launch_bounds__(kBlock4, 8) void k_trace4_any_persistent(const void* __restrict__ quads,
                                                                    const TriPre* __restrict__ tris, Src src,
                                                                    uint64_t n, uint8_t* __restrict__ out,
                                                                    uint32_t* __restrict__ queue,
@@ -207,6 +216,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_any_persistent(const void* _
   bool active = false;
   uint64_t rid = 0;
   float3 o = make_float3(0.f, 0.f, 0.f), d = o, inv = o, oi = o;
+  bool fz = true;
   float tnear = 0.f;
   HitRec h;
   h.t = 0.f;
@@ -233,6 +243,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_any_persistent(const void* _
           h.prim = kInvalid;
           inv = safe_inverse(d);
           oi = neg_o_inv(o, inv);
+          fz = fused_ok(oi);
           top = 0;
           cur = (h.t > tnear) ? 0u : kInvalid;
           leaf = 0;
@@ -255,7 +266,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_any_persistent(const void* _
         if (active && (cur & kLeafBit) == 0) {
           float tm[4];
           uint32_t ch[4];
-          visit_quad<kFp16, kFp16>(quads, cur, o, inv, oi, tnear, h.t, tm, ch);
+          visit_quad<kFp16, kFp16>(quads, cur, o, inv, oi, fz, tnear, h.t, tm, ch);
           if (tm[3] < kMaxFloat) stk.push(top, ch[3]);
           if (tm[2] < kMaxFloat) stk.push(top, ch[2]);
           if (tm[1] < kMaxFloat) stk.push(top, ch[1]);
@@ -332,15 +343,27 @@ hipError_t launch_simple4(const Blas& b, const TraceArgs& a, hipStream_t stream)
 // ---------------------------------------------------------------------------
 template <bool kFused>
 __device__ inline void visit_oct(const Oct16* __restrict__ octs, uint32_t q, float3 o, float3 inv, float3 oi,
-                                 float tnear, float tcull, float tm[8], uint32_t ch[8]) {
+                                 bool fz, float tnear, float tcull, float tm[8], uint32_t ch[8]) {
   const uint4* p = reinterpret_cast<const uint4*>(octs + q);
+  uint4 w[8];
+#pragma unroll
+  for (int k = 0; k < 8; k++) w[k] = p[k];
+  float2 tt[8];
+  if (kFused && fz) {
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      tt[k] = slab_fused(oi, inv, h2f_lo(w[k].x), h2f_hi(w[k].x), h2f_lo(w[k].y), h2f_hi(w[k].y), h2f_lo(w[k].z),
+                         h2f_hi(w[k].z));
+  } else {
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      tt[k] = slab(o, inv, h2f_lo(w[k].x), h2f_hi(w[k].x), h2f_lo(w[k].y), h2f_hi(w[k].y), h2f_lo(w[k].z),
+                   h2f_hi(w[k].z));
+  }
 #pragma unroll
   for (int k = 0; k < 8; k++) {
-    const uint4 w = p[k];
-    const float2 t =
-        kFused ? slab_fused(oi, inv, h2f_lo(w.x), h2f_hi(w.x), h2f_lo(w.y), h2f_hi(w.y), h2f_lo(w.z), h2f_hi(w.z))
-               : slab(o, inv, h2f_lo(w.x), h2f_hi(w.x), h2f_lo(w.y), h2f_hi(w.y), h2f_lo(w.z), h2f_hi(w.z));
-    ch[k] = w.w;
+    const float2 t = tt[k];
+    ch[k] = w[k].w;
     const bool hit = (t.x <= t.y) && (t.y >= tnear) && (t.x <= tcull) && ch[k] != kEmptyLeaf;
     tm[k] = hit ? t.x : kMaxFloat;
   }
@@ -379,6 +402,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace8_simple(const Oct16* __restri
   const bool any = kForceAny || (r.dir_far.w <= 0.0f);
   const float3 inv = safe_inverse(d);
   const float3 oi = neg_o_inv(o, inv);
+  const bool fz = fused_ok(oi);
   HitRec h;
   h.t = kForceAny ? fabsf(r.dir_far.w) : (any ? -r.dir_far.w : r.dir_far.w);
   h.prim = kInvalid;
@@ -391,7 +415,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace8_simple(const Oct16* __restri
     if ((cur & kLeafBit) == 0) {
       float tm[8];
       uint32_t ch[8];
-      visit_oct<kForceAny>(octs, cur, o, inv, oi, tnear, h.t, tm, ch);
+      visit_oct<kForceAny>(octs, cur, o, inv, oi, fz, tnear, h.t, tm, ch);
       if (kStats) nvisits++;
 #pragma unroll
       for (int k = 7; k >= 1; k--)
@@ -435,6 +459,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace8_any_persistent(const Oct16* 
   bool active = false;
   uint64_t rid = 0;
   float3 o = make_float3(0.f, 0.f, 0.f), d = o, inv = o, oi = o;
+  bool fz = true;
   float tnear = 0.f;
   HitRec h;
   h.t = 0.f;
@@ -461,6 +486,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace8_any_persistent(const Oct16* 
           h.prim = kInvalid;
           inv = safe_inverse(d);
           oi = neg_o_inv(o, inv);
+          fz = fused_ok(oi);
           top = 0;
           cur = 0;
           leaf = 0;
@@ -481,7 +507,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace8_any_persistent(const Oct16* 
         if (active && (cur & kLeafBit) == 0) {
           float tm[8];
           uint32_t ch[8];
-          visit_oct<true>(octs, cur, o, inv, oi, tnear, h.t, tm, ch);
+          visit_oct<true>(octs, cur, o, inv, oi, fz, tnear, h.t, tm, ch);
 #pragma unroll
           for (int k = 7; k >= 1; k--)
             if (tm[k] < kMaxFloat) stk.push(top, ch[k]);

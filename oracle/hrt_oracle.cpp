@@ -552,7 +552,8 @@ static inline void slab(f3 o, f3 inv, f3 bmin, f3 bmax, float& tmin_out, float& 
 
 // The engine's fused form of the same test for the any-hit traversals of FP16
 // nodes: t = fma(b, inv, -(o * inv)) per plane (one rounding after the
-// product o * inv); the same min/max reduction.
+// product o * inv); the same min/max reduction.  Used only for a ray whose
+// -(o * inv) is finite on every axis (fused_ok); other rays keep slab.
 static inline void slab_fused(f3 o, f3 inv, f3 bmin, f3 bmax, float& tmin_out, float& tmax_out) {
   const float nx = -(o.x * inv.x), ny = -(o.y * inv.y), nz = -(o.z * inv.z);
   const float lo = std::fma(bmin.x, inv.x, nx);
@@ -565,6 +566,10 @@ static inline void slab_fused(f3 o, f3 inv, f3 bmin, f3 bmax, float& tmin_out, f
   const float tmax = mn(mx(lo, hi), mx(lo1, hi1));
   tmin_out = mx(tmin, mn(lo2, hi2));
   tmax_out = mn(tmax, mx(lo2, hi2));
+}
+
+static inline bool fused_ok(f3 o, f3 inv) {
+  return std::isfinite(-(o.x * inv.x)) && std::isfinite(-(o.y * inv.y)) && std::isfinite(-(o.z * inv.z));
 }
 
 // Moller-Trumbore with the reference's 1e-6 slack, BVH2CommonRT.cpp:26-45:
@@ -1034,6 +1039,7 @@ static Hit trace_bvh4(const Build& b, const std::vector<Quad>& quads, f4 posNear
   const f3 d = {dirFar.x, dirFar.y, dirFar.z};
   const float tNear = posNear.w;
   const f3 inv = safe_inverse(d);
+  const bool use_fused = fused && fused_ok(o, inv);
   uint32_t stack[64];
   int top = 0;
   uint32_t cur = 0;
@@ -1047,7 +1053,7 @@ static Hit trace_bvh4(const Build& b, const std::vector<Quad>& quads, f4 posNear
       uint32_t ch[4];
       for (int k = 0; k < 4; k++) {
         float a, c;
-        if (fused) slab_fused(o, inv, q.c[k].bmin, q.c[k].bmax, a, c);
+        if (use_fused) slab_fused(o, inv, q.c[k].bmin, q.c[k].bmax, a, c);
         else slab(o, inv, q.c[k].bmin, q.c[k].bmax, a, c);
         // An unused slot (+1e9, -1e9) is inside out; the min/max slab test
         // turns it into the box [-1e9, 1e9], so the reference enters it and
@@ -1174,6 +1180,7 @@ static Hit trace_bvh8(const Build& b, const std::vector<Oct>& octs, f4 posNear, 
   const f3 d = {dirFar.x, dirFar.y, dirFar.z};
   const float tNear = posNear.w;
   const f3 inv = safe_inverse(d);
+  const bool use_fused = fused && fused_ok(o, inv);
   uint32_t stack[128];
   int top = 0;
   uint32_t cur = 0;
@@ -1189,7 +1196,7 @@ static Hit trace_bvh8(const Build& b, const std::vector<Oct>& octs, f4 posNear, 
       uint32_t ch[8];
       for (int k = 0; k < 8; k++) {
         float a, c;
-        if (fused) slab_fused(o, inv, q.c[k].bmin, q.c[k].bmax, a, c);
+        if (use_fused) slab_fused(o, inv, q.c[k].bmin, q.c[k].bmax, a, c);
         else slab(o, inv, q.c[k].bmin, q.c[k].bmax, a, c);
         const bool hit = (a <= c) && (c >= tNear) && (a <= h.t) && q.c[k].ref != 0x80000000u;
         tm[k] = hit ? a : MAXF;

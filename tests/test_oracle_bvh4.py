@@ -245,3 +245,37 @@ def test_fused_box_test_keeps_any_hit_answers(name, v, f):
     np.testing.assert_array_equal(p8 != 0xFFFFFFFF, pb != 0xFFFFFFFF)
     # the fused form prunes at most a few boxes differently
     assert abs(int(nf.sum()) - int(n16.sum())) <= 0.01 * int(n16.sum())
+
+
+def far_axis_rays(n: int = 2000):
+    """Rays straight down (d.x = d.z = 0) over a sphere centred at x = 500: -(o * inv)
+    overflows on x and z, where safe_inverse gives +-1e36."""
+    v, f = W.icosphere(3, 10.0, (500.0, 0.0, 0.0))
+    r = W.uniform01(W.pcg32(n, 2, 1400))
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0] = 488.0 + 24.0 * r[:, 0]
+    rays[:, 1] = 30.0
+    rays[:, 2] = -12.0 + 24.0 * r[:, 1]
+    rays[:, 5] = -1.0
+    rays[:, 7] = -100.0  # any-hit
+    return v, f, rays
+
+
+def test_fused_box_test_falls_back_where_o_inv_overflows():
+    v, f, rays = far_axis_rays()
+    inv = O.safe_inverse(rays[0, 4:7])
+    o = rays[0, 0:3]
+    lo = np.array([o[0] - 5.0, -10.0, o[2] - 5.0], np.float32)
+    hi = np.array([o[0] + 5.0, 10.0, o[2] + 5.0], np.float32)
+    a = O.slab(o, inv, lo, hi)
+    assert a[0] <= a[1]  # the ray passes through the box
+    b = O.slab_fused(o, inv, lo, hi)
+    assert not (b[0] <= b[1])  # the raw fused form loses it: -(o * inv) is -inf
+    bvh = O.OracleBVH(v, f)
+    _, pb, _, _ = bvh.trace4(rays, O.MODE4_BRUTE)
+    _, pf, _, _ = bvh.trace4(rays, O.MODE4_FP16_FUSED)
+    _, p8, _, _ = bvh.trace8(rays, fused=True)
+    hit = pb != 0xFFFFFFFF
+    assert 0.2 < hit.mean() < 1.0
+    np.testing.assert_array_equal(pf != 0xFFFFFFFF, hit)
+    np.testing.assert_array_equal(p8 != 0xFFFFFFFF, hit)
