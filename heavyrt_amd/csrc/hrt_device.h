@@ -341,6 +341,31 @@ __device__ inline float h2f_lo(uint32_t w) {
 }
 __device__ inline float h2f_hi(uint32_t w) { return float(__builtin_bit_cast(_Float16, uint16_t(w >> 16))); }
 
+// slab_fused on one FP16 child record {minx | miny << 16, minz | maxx << 16,
+// maxy | maxz << 16, ref} of a BVH4/BVH8 node.  Per axis, one v_perm_b32
+// puts the bound the ray meets first (the minimum where inv >= 0, else the
+// maximum) in the low half and the other in the high half, so the entry and
+// exit distances need no min/max: for a box with lo <= hi, fma(lo, inv, oi)
+// <= fma(hi, inv, oi) when inv > 0 (rounding is monotonic) and the reverse
+// when inv < 0, so the values equal slab_fused's.  sel comes from
+// near_far_selectors(inv), once per ray.
+__device__ inline uint3 near_far_selectors(float3 inv) {
+  // byte_permute({S0, S1}, sel): bytes 0-3 of S1, then 4-7 of S0
+  return make_uint3(inv.x >= 0.0f ? 0x07060100u : 0x01000706u,   // S0 = w.y (maxx hi), S1 = w.x (minx lo)
+                    inv.y >= 0.0f ? 0x05040302u : 0x03020504u,   // S0 = w.z (maxy lo), S1 = w.x (miny hi)
+                    inv.z >= 0.0f ? 0x07060100u : 0x01000706u);  // S0 = w.z (maxz hi), S1 = w.y (minz lo)
+}
+
+__device__ inline float2 slab_fused_nf(float3 oi, float3 inv, uint4 w, uint3 sel) {
+  const uint32_t bx = __builtin_amdgcn_perm(w.y, w.x, sel.x);
+  const uint32_t by = __builtin_amdgcn_perm(w.z, w.x, sel.y);
+  const uint32_t bz = __builtin_amdgcn_perm(w.z, w.y, sel.z);
+  const float nx = fmaf(h2f_lo(bx), inv.x, oi.x), fx = fmaf(h2f_hi(bx), inv.x, oi.x);
+  const float ny = fmaf(h2f_lo(by), inv.y, oi.y), fy = fmaf(h2f_hi(by), inv.y, oi.y);
+  const float nz = fmaf(h2f_lo(bz), inv.z, oi.z), fz = fmaf(h2f_hi(bz), inv.z, oi.z);
+  return make_float2(fmaxf(fmaxf(nx, ny), nz), fminf(fminf(fx, fy), fz));
+}
+
 // FP32 -> FP16 bits, round to nearest even (v_cvt_f16_f32 in the default mode).
 __device__ inline uint32_t f2h_rne(float f) { return __builtin_bit_cast(uint16_t, _Float16(f)); }
 

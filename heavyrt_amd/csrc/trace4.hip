@@ -58,10 +58,9 @@ __device__ inline void visit_quad(const void* __restrict__ quads, uint32_t q, fl
       ch[k] = w[k].w;
     }
     if (kFused && fz) {
+      const uint3 sel = near_far_selectors(inv);
 #pragma unroll
-      for (int k = 0; k < 4; k++)
-        tt[k] = slab_fused(oi, inv, h2f_lo(w[k].x), h2f_hi(w[k].x), h2f_lo(w[k].y), h2f_hi(w[k].y),
-                           h2f_lo(w[k].z), h2f_hi(w[k].z));
+      for (int k = 0; k < 4; k++) tt[k] = slab_fused_nf(oi, inv, w[k], sel);
     } else {
 #pragma unroll
       for (int k = 0; k < 4; k++)
@@ -350,10 +349,9 @@ __device__ inline void visit_oct(const Oct16* __restrict__ octs, uint32_t q, flo
   for (int k = 0; k < 8; k++) w[k] = p[k];
   float2 tt[8];
   if (kFused && fz) {
+    const uint3 sel = near_far_selectors(inv);
 #pragma unroll
-    for (int k = 0; k < 8; k++)
-      tt[k] = slab_fused(oi, inv, h2f_lo(w[k].x), h2f_hi(w[k].x), h2f_lo(w[k].y), h2f_hi(w[k].y), h2f_lo(w[k].z),
-                         h2f_hi(w[k].z));
+    for (int k = 0; k < 8; k++) tt[k] = slab_fused_nf(oi, inv, w[k], sel);
   } else {
 #pragma unroll
     for (int k = 0; k < 8; k++)
