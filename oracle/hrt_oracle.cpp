@@ -552,20 +552,22 @@ static inline void slab(f3 o, f3 inv, f3 bmin, f3 bmax, float& tmin_out, float& 
 
 // The engine's fused form of the same test for the any-hit traversals of FP16
 // nodes: t = fma(b, inv, -(o * inv)) per plane (one rounding after the
-// product o * inv); the same min/max reduction.  Used only for a ray whose
-// -(o * inv) is finite on every axis (fused_ok); other rays keep slab.
+// product o * inv).  Per axis the bound the ray meets first is the minimum
+// where inv >= 0, else the maximum; entry = the largest of those distances,
+// exit = the smallest of the others.  For a box with min <= max this equals
+// the min/max reduction (rounding is monotonic); an inverted box (the
+// +65504 .. -65504 of an unused slot) comes out empty.  Used only for a ray
+// whose -(o * inv) is finite on every axis (fused_ok); other rays keep slab.
 static inline void slab_fused(f3 o, f3 inv, f3 bmin, f3 bmax, float& tmin_out, float& tmax_out) {
   const float nx = -(o.x * inv.x), ny = -(o.y * inv.y), nz = -(o.z * inv.z);
-  const float lo = std::fma(bmin.x, inv.x, nx);
-  const float hi = std::fma(bmax.x, inv.x, nx);
-  const float lo1 = std::fma(bmin.y, inv.y, ny);
-  const float hi1 = std::fma(bmax.y, inv.y, ny);
-  const float lo2 = std::fma(bmin.z, inv.z, nz);
-  const float hi2 = std::fma(bmax.z, inv.z, nz);
-  const float tmin = mx(mn(lo, hi), mn(lo1, hi1));
-  const float tmax = mn(mx(lo, hi), mx(lo1, hi1));
-  tmin_out = mx(tmin, mn(lo2, hi2));
-  tmax_out = mn(tmax, mx(lo2, hi2));
+  const float ax = std::fma(inv.x >= 0.0f ? bmin.x : bmax.x, inv.x, nx);
+  const float bx = std::fma(inv.x >= 0.0f ? bmax.x : bmin.x, inv.x, nx);
+  const float ay = std::fma(inv.y >= 0.0f ? bmin.y : bmax.y, inv.y, ny);
+  const float by = std::fma(inv.y >= 0.0f ? bmax.y : bmin.y, inv.y, ny);
+  const float az = std::fma(inv.z >= 0.0f ? bmin.z : bmax.z, inv.z, nz);
+  const float bz = std::fma(inv.z >= 0.0f ? bmax.z : bmin.z, inv.z, nz);
+  tmin_out = mx(mx(ax, ay), az);
+  tmax_out = mn(mn(bx, by), bz);
 }
 
 static inline bool fused_ok(f3 o, f3 inv) {

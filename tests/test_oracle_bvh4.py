@@ -220,6 +220,16 @@ def test_slab_forms_equal_exact_rounding():
             got = fn(o, inv, lo, hi)
             want = _slab_exact(o, inv, lo, hi, fused)
             np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32), err_msg=f"{k} fused={fused}")
+    # an unused slot's inverted FP16 box: the min/max form turns it into a box, the fused one
+    # (near/far bound by the sign of inv) finds it empty
+    for k in range(200):
+        o = rng.standard_normal(3).astype(np.float32)
+        inv = O.safe_inverse(rng.standard_normal(3).astype(np.float32))
+        lo = np.full(3, 65504.0, np.float32)
+        hi = np.full(3, -65504.0, np.float32)
+        a, b = O.slab(o, inv, lo, hi)
+        c, d = O.slab_fused(o, inv, lo, hi)
+        assert a <= b and not (c <= d)
     # the two forms do round differently somewhere (else the fused restatement pins nothing)
     diffs = 0
     for k in range(400):
