@@ -24,8 +24,8 @@
 // whether some triangle is hit, which does not depend on the order, so it
 // traverses speculatively (Aila & Laine, "Understanding the Efficiency of Ray
 // Traversal on GPUs", HPG 2009): a lane that has found a leaf parks it and
-// keeps descending until every lane of the wave holds a leaf, then the whole
-// wave intersects leaves together.
+// keeps descending until all but a few lanes of the wave hold a leaf, then
+// those lanes intersect their leaves together.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -40,6 +40,9 @@ constexpr int kWavesPerBlock = 1;
 constexpr int kTraceBlock = kWave * kWavesPerBlock;
 constexpr int kRefillAt = 16;  // refill when at least this many lanes are idle
 constexpr uint32_t kClaim = 128;  // ray indices a wave claims from the queue at once
+// Speculative any-hit leaves the node loop for the leaf phase once this many
+// lanes or fewer still look for a leaf (and some lane holds one).
+constexpr uint32_t kLeafPhaseAt = 8;
 
 struct HitOutFull {  // CRT_Hit (CrossRT.h:20-27)
   float t;
@@ -197,7 +200,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_persistent(const NodePair
             cur = (top > 0) ? stk.pop(top) : kInvalid;
           }
           const bool searching = active && leaf == 0 && (cur & kLeafBit) == 0;
-          if (!__any(searching)) break;
+          const uint32_t nsearch = __popcll(__ballot(searching));
+          if (nsearch == 0 || (nsearch <= kLeafPhaseAt && __any(leaf != 0))) break;
           if (active && (cur & kLeafBit) == 0) {
             const float4* pp = reinterpret_cast<const float4*>(pairs + cur);
             const float4 lmin = pp[0], lmax = pp[1], rmin = pp[2], rmax = pp[3];

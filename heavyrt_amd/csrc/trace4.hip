@@ -25,7 +25,10 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kBlock4 = 64;
-constexpr int kRefillAt4 = 32;
+constexpr int kRefillAt4 = 40;
+// The speculative any-hit kernels leave the node loop for the leaf phase once
+// this many lanes or fewer still look for a leaf (0 would wait for the last).
+constexpr uint32_t kLeafPhaseAt = 8;
 constexpr uint32_t kClaim = 128;  // ray indices a wave claims from the queue at once
 constexpr float kMaxFloat = 3.402823466e+38f;
 
@@ -254,14 +257,18 @@ __global__ __launch_bounds__(kBlock4, 8) void k_trace4_any_persistent(const void
 
     while (true) {
       // inner phase: park the first leaf found; a lane with a parked leaf
-      // keeps descending while any lane of the wave still looks for one
+      // keeps descending while enough lanes of the wave still look for one
       while (true) {
         if (active && leaf == 0 && (cur & kLeafBit) != 0 && cur != kInvalid) {
           leaf = cur;
           cur = (top > 0) ? stk.pop(top) : kInvalid;
         }
+        // leave for the leaf phase once at most kLeafPhaseAt lanes still
+        // search and some lane holds a leaf (the slowest searchers would
+        // otherwise keep the wave half idle)
         const bool searching = active && leaf == 0 && (cur & kLeafBit) == 0;
-        if (!__any(searching)) break;
+        const uint32_t nsearch = __popcll(__ballot(searching));
+        if (nsearch == 0 || (nsearch <= kLeafPhaseAt && __any(leaf != 0))) break;
         if (active && (cur & kLeafBit) == 0) {
           float tm[4];
           uint32_t ch[4];
@@ -500,8 +507,12 @@ __global__ __launch_bounds__(kBlock4) void k_trace8_any_persistent(const Oct16* 
           leaf = cur;
           cur = (top > 0) ? stk.pop(top) : kInvalid;
         }
+        // leave for the leaf phase once at most kLeafPhaseAt lanes still
+        // search and some lane holds a leaf (the slowest searchers would
+        // otherwise keep the wave half idle)
         const bool searching = active && leaf == 0 && (cur & kLeafBit) == 0;
-        if (!__any(searching)) break;
+        const uint32_t nsearch = __popcll(__ballot(searching));
+        if (nsearch == 0 || (nsearch <= kLeafPhaseAt && __any(leaf != 0))) break;
         if (active && (cur & kLeafBit) == 0) {
           float tm[8];
           uint32_t ch[8];
