@@ -40,6 +40,10 @@ constexpr int kWavesPerBlock = 1;
 constexpr int kTraceBlock = kWave * kWavesPerBlock;
 constexpr int kRefillAt = 16;  // refill when at least this many lanes are idle
 constexpr uint32_t kClaim = 128;  // ray indices a wave claims from the queue at once
+// One-ray-per-lane kernel: stack entries per lane in LDS (16 KB per
+// 256-thread block, so LDS allows 8 waves per SIMD); deeper entries, up to
+// kStackDepth, go to a per-lane global spill area.
+constexpr int kSimpleLds = 16;
 // Speculative any-hit leaves the node loop for the leaf phase once this many
 // lanes or fewer still look for a leaf (and some lane holds one).
 constexpr uint32_t kLeafPhaseAt = 8;
@@ -58,11 +62,12 @@ template <bool kForceAny, bool kStats>
 __global__ __launch_bounds__(256) void k_trace_simple(const NodePair* __restrict__ pairs,
                                                       const TriPre* __restrict__ tris,
                                                       const Ray* __restrict__ rays, uint64_t n, void* out,
-                                                      int format, uint2 ids, uint32_t* __restrict__ stats) {
-  __shared__ uint32_t stack_mem[kStackDepth * 256];
+                                                      int format, uint2 ids, uint32_t* __restrict__ stats,
+                                                      uint32_t* __restrict__ spill) {
+  __shared__ uint32_t stack_mem[kSimpleLds * 256];
   const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
   if (i >= n) return;
-  uint32_t* stack = stack_mem + threadIdx.x;
+  const ShortStack<kSimpleLds, 256> stk{stack_mem + threadIdx.x, spill + i, gridDim.x * 256u};
   const Ray r = rays[i];
   const float3 o = make_float3(r.pos_near.x, r.pos_near.y, r.pos_near.z);
   const float3 d = make_float3(r.dir_far.x, r.dir_far.y, r.dir_far.z);
@@ -78,7 +83,7 @@ __global__ __launch_bounds__(256) void k_trace_simple(const NodePair* __restrict
   uint32_t npairs = 0, ntests = 0;
   int top = 0;
   uint32_t cur = 0;
-  while (top >= 0) {
+  while (true) {
     bool pop;
     if ((cur & kLeafBit) == 0) {
       const float4* pp = reinterpret_cast<const float4*>(pairs + cur);
@@ -94,8 +99,7 @@ __global__ __launch_bounds__(256) void k_trace_simple(const NodePair* __restrict
       if (h0 && h1) {
         const bool left_first = t0.x <= t1.x;
         cur = left_first ? c0 : c1;
-        stack[top * 256] = left_first ? c1 : c0;
-        top++;
+        stk.push(top, left_first ? c1 : c0);
       }
     } else {
       pop = true;
@@ -107,8 +111,8 @@ __global__ __launch_bounds__(256) void k_trace_simple(const NodePair* __restrict
       }
     }
     if (pop) {
-      top--;
-      cur = stack[max(top, 0) * 256];
+      if (top == 0) break;
+      cur = stk.pop(top);
     }
   }
   if (kStats) {
@@ -310,22 +314,28 @@ hipError_t launch_persistent(const Blas& b, const TraceArgs& a, hipStream_t stre
 
 hipError_t trace_blas(const Blas& b, const TraceArgs& a, hipStream_t stream) {
   if (a.n == 0) return hipSuccess;
-  if (a.stats) {
-    // counting variant: one ray per lane in the reference's order
-    const uint32_t blocks = uint32_t((a.n + 255) / 256);
-    if (a.anyhit)
-      k_trace_simple<true, true><<<blocks, 256, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format, make_uint2(a.inst_id, a.geom_id), a.stats);
-    else
-      k_trace_simple<false, true><<<blocks, 256, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format, make_uint2(a.inst_id, a.geom_id),
-                                                               a.stats);
-    return hipGetLastError();
-  }
-  if (a.anyhit) return launch_persistent<true, true>(b, a, stream);
-  // Nearest hits: one ray per lane.  Rays that arrive in screen order stay
-  // coherent within a wave, which the refilling kernel gives up.
+  if (a.anyhit && !a.stats) return launch_persistent<true, true>(b, a, stream);
+  // One ray per lane: the counting variant (the reference's order) and
+  // nearest hits (rays that arrive in screen order stay coherent within a
+  // wave, which the refilling kernel gives up).
   const uint32_t blocks = uint32_t((a.n + 255) / 256);
-  k_trace_simple<false, false><<<blocks, 256, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format, make_uint2(a.inst_id, a.geom_id), nullptr);
-  return hipGetLastError();
+  uint32_t* spill = nullptr;
+  hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&spill),
+                                sizeof(uint32_t) * size_t(blocks) * 256 * (kStackDepth - kSimpleLds), stream);
+  if (e != hipSuccess) return e;
+  const uint2 ids = make_uint2(a.inst_id, a.geom_id);
+  if (a.stats && a.anyhit)
+    k_trace_simple<true, true><<<blocks, 256, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format, ids, a.stats,
+                                                            spill);
+  else if (a.stats)
+    k_trace_simple<false, true><<<blocks, 256, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format, ids,
+                                                             a.stats, spill);
+  else
+    k_trace_simple<false, false><<<blocks, 256, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format, ids,
+                                                              nullptr, spill);
+  e = hipGetLastError();
+  const hipError_t f = hipFreeAsync(spill, stream);
+  return e != hipSuccess ? e : f;
 }
 
 }  // namespace hrt

@@ -12,7 +12,9 @@
 //
 // The stack holds at most 45 entries: BVH4 nodes are the even-depth nodes of
 // a BVH2 with at most 30 internal levels (15 node levels, 3 deferred children
-// each); 48 LDS entries per lane, interleaved by lane.
+// each; 48 for a clustered any-hit tree, whose depth its build checks).  Every
+// kernel keeps 16 entries per lane in LDS, interleaved by lane, and the rest
+// in a per-lane global spill area, so LDS does not limit occupancy.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -26,6 +28,11 @@ namespace {
 constexpr int kWave = 64;
 constexpr int kBlock4 = 64;
 constexpr int kRefillAt4 = 40;
+// LDS stack entries per lane of the one-ray-per-lane kernels (the rest spill
+// to global memory).  Deep, incoherent nearest-hit walks push often, so these
+// keep a longer LDS part than the persistent kernels' kShortStack.
+constexpr int kSimpleLds4 = 48;
+constexpr int kSimpleLds8 = 32;
 // The speculative any-hit kernels leave the node loop for the leaf phase once
 // this many lanes or fewer still look for a leaf (0 would wait for the last).
 constexpr uint32_t kLeafPhaseAt = 8;
@@ -128,11 +135,12 @@ template <bool kForceAny, bool kFp16, bool kStats>
 __global__ __launch_bounds__(kBlock4) void k_trace4_simple(const void* __restrict__ quads,
                                                            const TriPre* __restrict__ tris,
                                                            const Ray* __restrict__ rays, uint64_t n, void* out,
-                                                           int format, uint2 ids, uint32_t* __restrict__ stats) {
-  __shared__ uint32_t stack_mem[kStackDepth4 * kBlock4];
+                                                           int format, uint2 ids, uint32_t* __restrict__ stats,
+                                                           uint32_t* __restrict__ spill) {
+  __shared__ uint32_t stack_mem[kSimpleLds4 * kBlock4];
   const uint64_t i = uint64_t(blockIdx.x) * kBlock4 + threadIdx.x;
   if (i >= n) return;
-  uint32_t* stack = stack_mem + threadIdx.x;
+  const ShortStack<kSimpleLds4, kBlock4> stk{stack_mem + threadIdx.x, spill + i, gridDim.x * kBlock4};
   const Ray r = rays[i];
   const float3 o = make_float3(r.pos_near.x, r.pos_near.y, r.pos_near.z);
   const float3 d = make_float3(r.dir_far.x, r.dir_far.y, r.dir_far.z);
@@ -155,9 +163,9 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_simple(const void* __restric
       uint32_t ch[4];
       visit_quad<kFp16, kForceAny && kFp16>(quads, cur, o, inv, oi, fz, tnear, h.t, tm, ch);
       if (kStats) nvisits++;
-      if (tm[3] < kMaxFloat) stack[(top++) * kBlock4] = ch[3];
-      if (tm[2] < kMaxFloat) stack[(top++) * kBlock4] = ch[2];
-      if (tm[1] < kMaxFloat) stack[(top++) * kBlock4] = ch[1];
+      if (tm[3] < kMaxFloat) stk.push(top, ch[3]);
+      if (tm[2] < kMaxFloat) stk.push(top, ch[2]);
+      if (tm[1] < kMaxFloat) stk.push(top, ch[1]);
       if (tm[0] < kMaxFloat) cur = ch[0];
       else pop = true;
     }
@@ -171,7 +179,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_simple(const void* __restric
     }
     if (pop) {
       if (top == 0) break;
-      cur = stack[(--top) * kBlock4];
+      cur = stk.pop(top);
     }
   }
   if (kStats) {
@@ -323,22 +331,30 @@ template <bool kFp16>
 hipError_t launch_simple4(const Blas& b, const TraceArgs& a, hipStream_t stream) {
   const uint32_t blocks = uint32_t((a.n + kBlock4 - 1) / kBlock4);
   const void* q = kFp16 ? static_cast<const void*>(b.quads16) : static_cast<const void*>(b.quads32);
+  // stack entries beyond the LDS part, per lane
+  uint32_t* spill = nullptr;
+  hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&spill),
+                                sizeof(uint32_t) * (size_t(blocks) * kBlock4 * (kStackDepth4 - kSimpleLds4) + 64), stream);
+  if (e != hipSuccess) return e;
+  const uint2 ids = make_uint2(a.inst_id, a.geom_id);
   if (a.anyhit) {
     if (a.stats)
-      k_trace4_simple<true, kFp16, true><<<blocks, kBlock4, 0, stream>>>(q, b.tris, a.rays, a.n, a.out, a.format, make_uint2(a.inst_id, a.geom_id),
-                                                                        a.stats);
+      k_trace4_simple<true, kFp16, true><<<blocks, kBlock4, 0, stream>>>(q, b.tris, a.rays, a.n, a.out, a.format, ids,
+                                                                        a.stats, spill);
     else
-      k_trace4_simple<true, kFp16, false><<<blocks, kBlock4, 0, stream>>>(q, b.tris, a.rays, a.n, a.out, a.format, make_uint2(a.inst_id, a.geom_id),
-                                                                         nullptr);
+      k_trace4_simple<true, kFp16, false><<<blocks, kBlock4, 0, stream>>>(q, b.tris, a.rays, a.n, a.out, a.format,
+                                                                         ids, nullptr, spill);
   } else {
     if (a.stats)
-      k_trace4_simple<false, kFp16, true><<<blocks, kBlock4, 0, stream>>>(q, b.tris, a.rays, a.n, a.out, a.format, make_uint2(a.inst_id, a.geom_id),
-                                                                         a.stats);
+      k_trace4_simple<false, kFp16, true><<<blocks, kBlock4, 0, stream>>>(q, b.tris, a.rays, a.n, a.out, a.format,
+                                                                         ids, a.stats, spill);
     else
       k_trace4_simple<false, kFp16, false><<<blocks, kBlock4, 0, stream>>>(q, b.tris, a.rays, a.n, a.out,
-                                                                          a.format, make_uint2(a.inst_id, a.geom_id), nullptr);
+                                                                          a.format, ids, nullptr, spill);
   }
-  return hipGetLastError();
+  e = hipGetLastError();
+  const hipError_t f = hipFreeAsync(spill, stream);
+  return e != hipSuccess ? e : f;
 }
 
 // ---------------------------------------------------------------------------
@@ -395,11 +411,12 @@ template <bool kForceAny, bool kStats>
 __global__ __launch_bounds__(kBlock4) void k_trace8_simple(const Oct16* __restrict__ octs,
                                                            const TriPre* __restrict__ tris,
                                                            const Ray* __restrict__ rays, uint64_t n, void* out,
-                                                           int format, uint2 ids, uint32_t* __restrict__ stats) {
-  __shared__ uint32_t stack_mem[kStackDepth8 * kBlock4];
+                                                           int format, uint2 ids, uint32_t* __restrict__ stats,
+                                                           uint32_t* __restrict__ spill) {
+  __shared__ uint32_t stack_mem[kSimpleLds8 * kBlock4];
   const uint64_t i = uint64_t(blockIdx.x) * kBlock4 + threadIdx.x;
   if (i >= n) return;
-  uint32_t* stack = stack_mem + threadIdx.x;
+  const ShortStack<kSimpleLds8, kBlock4> stk{stack_mem + threadIdx.x, spill + i, gridDim.x * kBlock4};
   const Ray r = rays[i];
   const float3 o = make_float3(r.pos_near.x, r.pos_near.y, r.pos_near.z);
   const float3 d = make_float3(r.dir_far.x, r.dir_far.y, r.dir_far.z);
@@ -424,7 +441,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace8_simple(const Oct16* __restri
       if (kStats) nvisits++;
 #pragma unroll
       for (int k = 7; k >= 1; k--)
-        if (tm[k] < kMaxFloat) stack[(top++) * kBlock4] = ch[k];
+        if (tm[k] < kMaxFloat) stk.push(top, ch[k]);
       if (tm[0] < kMaxFloat) cur = ch[0];
       else pop = true;
     }
@@ -438,7 +455,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace8_simple(const Oct16* __restri
     }
     if (pop) {
       if (top == 0) break;
-      cur = stack[(--top) * kBlock4];
+      cur = stk.pop(top);
     }
   }
   if (kStats) {
@@ -562,23 +579,29 @@ hipError_t launch_any8(const Blas& b, const TraceArgs& a, hipStream_t stream) {
 
 hipError_t launch_simple8(const Blas& b, const TraceArgs& a, hipStream_t stream) {
   const uint32_t blocks = uint32_t((a.n + kBlock4 - 1) / kBlock4);
+  uint32_t* spill = nullptr;
+  hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&spill),
+                                sizeof(uint32_t) * (size_t(blocks) * kBlock4 * (kStackDepth8 - kSimpleLds8) + 64), stream);
+  if (e != hipSuccess) return e;
   const uint2 ids = make_uint2(a.inst_id, a.geom_id);
   if (a.anyhit) {
     if (a.stats)
       k_trace8_simple<true, true><<<blocks, kBlock4, 0, stream>>>(b.octs16, b.tris, a.rays, a.n, a.out, a.format, ids,
-                                                                  a.stats);
+                                                                  a.stats, spill);
     else
       k_trace8_simple<true, false><<<blocks, kBlock4, 0, stream>>>(b.octs16, b.tris, a.rays, a.n, a.out, a.format,
-                                                                   ids, nullptr);
+                                                                   ids, nullptr, spill);
   } else {
     if (a.stats)
       k_trace8_simple<false, true><<<blocks, kBlock4, 0, stream>>>(b.octs16, b.tris, a.rays, a.n, a.out, a.format,
-                                                                   ids, a.stats);
+                                                                   ids, a.stats, spill);
     else
       k_trace8_simple<false, false><<<blocks, kBlock4, 0, stream>>>(b.octs16, b.tris, a.rays, a.n, a.out, a.format,
-                                                                    ids, nullptr);
+                                                                    ids, nullptr, spill);
   }
-  return hipGetLastError();
+  e = hipGetLastError();
+  const hipError_t f = hipFreeAsync(spill, stream);
+  return e != hipSuccess ? e : f;
 }
 
 }  // namespace
