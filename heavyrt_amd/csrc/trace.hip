@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 
 #include "hrt_internal.h"
 
@@ -40,9 +41,12 @@ constexpr int kWavesPerBlock = 1;
 constexpr int kTraceBlock = kWave * kWavesPerBlock;
 constexpr int kRefillAt = 16;  // refill when at least this many lanes are idle
 constexpr uint32_t kClaim = 128;  // ray indices a wave claims from the queue at once
-// One-ray-per-lane kernel: stack entries per lane in LDS (16 KB per
-// 256-thread block, so LDS allows 8 waves per SIMD); deeper entries, up to
-// kStackDepth, go to a per-lane global spill area.
+// One-ray-per-lane kernel: stack entries per lane in LDS.  With 16 (16 KB
+// per 256-thread block) LDS allows 8 waves per SIMD and deeper entries, up to
+// kStackDepth, go to a per-lane global spill area; with kStackDepth nothing
+// spills but LDS allows 5 waves per SIMD.  The launcher takes the full LDS
+// stack when its whole grid is resident at once (a small batch gains nothing
+// from more waves and would pay for the spills), the short one otherwise.
 constexpr int kSimpleLds = 16;
 // Speculative any-hit leaves the node loop for the leaf phase once this many
 // lanes or fewer still look for a leaf (and some lane holds one).
@@ -58,16 +62,16 @@ __global__ void k_reset_counter(uint32_t* c) { *c = 0u; }
 
 // Reference-order reference kernel (one ray per lane, no refill).  Kept for
 // the counting variant used to report the reference's traversal counters.
-template <bool kForceAny, bool kStats>
+template <bool kForceAny, bool kStats, int kLds>
 __global__ __launch_bounds__(256) void k_trace_simple(const NodePair* __restrict__ pairs,
                                                       const TriPre* __restrict__ tris,
                                                       const Ray* __restrict__ rays, uint64_t n, void* out,
                                                       int format, uint2 ids, uint32_t* __restrict__ stats,
                                                       uint32_t* __restrict__ spill) {
-  __shared__ uint32_t stack_mem[kSimpleLds * 256];
+  __shared__ uint32_t stack_mem[kLds * 256];
   const uint64_t i = uint64_t(blockIdx.x) * 256 + threadIdx.x;
   if (i >= n) return;
-  const ShortStack<kSimpleLds, 256> stk{stack_mem + threadIdx.x, spill + i, gridDim.x * 256u};
+  const ShortStack<kLds, 256> stk{stack_mem + threadIdx.x, spill + i, gridDim.x * 256u};
   const Ray r = rays[i];
   const float3 o = make_float3(r.pos_near.x, r.pos_near.y, r.pos_near.z);
   const float3 d = make_float3(r.dir_far.x, r.dir_far.y, r.dir_far.z);
@@ -310,6 +314,40 @@ hipError_t launch_persistent(const Blas& b, const TraceArgs& a, hipStream_t stre
   return e != hipSuccess ? e : f;
 }
 
+template <int kLds>
+hipError_t launch_simple(const Blas& b, const TraceArgs& a, uint32_t blocks, uint32_t* spill, hipStream_t stream) {
+  const uint2 ids = make_uint2(a.inst_id, a.geom_id);
+  if (a.stats && a.anyhit)
+    k_trace_simple<true, true, kLds><<<blocks, 256, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format, ids,
+                                                                  a.stats, spill);
+  else if (a.stats)
+    k_trace_simple<false, true, kLds><<<blocks, 256, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format, ids,
+                                                                   a.stats, spill);
+  else
+    k_trace_simple<false, false, kLds><<<blocks, 256, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format,
+                                                                    ids, nullptr, spill);
+  return hipGetLastError();
+}
+
+// Blocks of the full-LDS-stack kernel resident on the current device at
+// once, queried once per device (a per-call query shows up as launch latency
+// on small batches).
+uint32_t resident_full_stack_blocks() {
+  static std::atomic<uint32_t> cache[64];
+  int dev = 0, cus = 0, per_cu = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0) return 0;
+  if (dev < 64) {
+    const uint32_t v = cache[dev].load(std::memory_order_relaxed);
+    if (v) return v;
+  }
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) return 0;
+  const void* kfn = reinterpret_cast<const void*>(&k_trace_simple<false, false, kStackDepth>);
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, 256, 0) != hipSuccess || per_cu <= 0) return 0;
+  const uint32_t v = uint32_t(cus) * uint32_t(per_cu);
+  if (dev < 64) cache[dev].store(v, std::memory_order_relaxed);
+  return v;
+}
+
 }  // namespace
 
 hipError_t trace_blas(const Blas& b, const TraceArgs& a, hipStream_t stream) {
@@ -319,21 +357,12 @@ hipError_t trace_blas(const Blas& b, const TraceArgs& a, hipStream_t stream) {
   // nearest hits (rays that arrive in screen order stay coherent within a
   // wave, which the refilling kernel gives up).
   const uint32_t blocks = uint32_t((a.n + 255) / 256);
+  if (blocks <= resident_full_stack_blocks()) return launch_simple<kStackDepth>(b, a, blocks, nullptr, stream);
   uint32_t* spill = nullptr;
   hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&spill),
                                 sizeof(uint32_t) * size_t(blocks) * 256 * (kStackDepth - kSimpleLds), stream);
   if (e != hipSuccess) return e;
-  const uint2 ids = make_uint2(a.inst_id, a.geom_id);
-  if (a.stats && a.anyhit)
-    k_trace_simple<true, true><<<blocks, 256, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format, ids, a.stats,
-                                                            spill);
-  else if (a.stats)
-    k_trace_simple<false, true><<<blocks, 256, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format, ids,
-                                                             a.stats, spill);
-  else
-    k_trace_simple<false, false><<<blocks, 256, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format, ids,
-                                                              nullptr, spill);
-  e = hipGetLastError();
+  e = launch_simple<kSimpleLds>(b, a, blocks, spill, stream);
   const hipError_t f = hipFreeAsync(spill, stream);
   return e != hipSuccess ? e : f;
 }
