@@ -201,31 +201,6 @@ __device__ inline bool fused_ok(float3 oi) {
   return __builtin_isfinite(oi.x) && __builtin_isfinite(oi.y) && __builtin_isfinite(oi.z);
 }
 
-// The slab test in the fused form t = fma(b, inv, -(o * inv)): one rounding
-// per plane instead of two, and with an FP16 box value as a direct operand of
-// v_fma_mix_f32 (widening FP16 to FP32 is exact) no conversion either.  It is
-// used only by the any-hit traversals of the FP16 nodes, where box tests prune
-// and never order the answer: a ray's any-hit answer is whether some triangle
-// passes the Moller-Trumbore test, whichever boxes are visited, as long as no
-// box holding such a triangle is rejected.  Both forms round each plane
-// distance by about half an ulp of the larger of |b| and |o| (the difference
-// b - o, or the product o * inv, is rounded), against boxes widened outward
-// by at least one FP16 ulp.  Valid only while fused_ok(oi): with a finite
-// -(o * inv) each plane is the exact b * inv - o * inv rounded once (no
-// intermediate overflow), so its sign and order are right.
-__device__ inline float2 slab_fused(float3 oi, float3 inv, float bminx, float bminy, float bminz, float bmaxx,
-                                    float bmaxy, float bmaxz) {
-  const float lo = fmaf(bminx, inv.x, oi.x);
-  const float hi = fmaf(bmaxx, inv.x, oi.x);
-  const float lo1 = fmaf(bminy, inv.y, oi.y);
-  const float hi1 = fmaf(bmaxy, inv.y, oi.y);
-  const float lo2 = fmaf(bminz, inv.z, oi.z);
-  const float hi2 = fmaf(bmaxz, inv.z, oi.z);
-  const float tmin = fmaxf(fminf(lo, hi), fminf(lo1, hi1));
-  const float tmax = fminf(fmaxf(lo, hi), fmaxf(lo1, hi1));
-  return make_float2(fmaxf(tmin, fminf(lo2, hi2)), fminf(tmax, fmaxf(lo2, hi2)));
-}
-
 struct HitRec {
   float t;
   uint32_t prim;  // original triangle id, kInvalid on miss
@@ -341,14 +316,28 @@ __device__ inline float h2f_lo(uint32_t w) {
 }
 __device__ inline float h2f_hi(uint32_t w) { return float(__builtin_bit_cast(_Float16, uint16_t(w >> 16))); }
 
-// slab_fused on one FP16 child record {minx | miny << 16, minz | maxx << 16,
-// maxy | maxz << 16, ref} of a BVH4/BVH8 node.  Per axis, one v_perm_b32
-// puts the bound the ray meets first (the minimum where inv >= 0, else the
-// maximum) in the low half and the other in the high half, so the entry and
-// exit distances need no min/max: for a box with lo <= hi, fma(lo, inv, oi)
-// <= fma(hi, inv, oi) when inv > 0 (rounding is monotonic) and the reverse
-// when inv < 0, so the values equal slab_fused's.  sel comes from
-// near_far_selectors(inv), once per ray.
+// The slab test in the fused form t = fma(b, inv, -(o * inv)) on one FP16
+// child record {minx | miny << 16, minz | maxx << 16, maxy | maxz << 16, ref}
+// of a BVH4/BVH8 node.  One rounding per plane instead of two, and each FP16
+// bound is a direct operand of v_fma_mix_f32 (widening FP16 to FP32 is
+// exact), so no conversion either.
+//
+// Per axis, one v_perm_b32 puts the bound the ray meets first (the minimum
+// where inv >= 0, else the maximum) in the low half and the other in the high
+// half, so entry and exit need no min/max: for a box with lo <= hi,
+// fma(lo, inv, oi) <= fma(hi, inv, oi) when inv > 0 (rounding is monotonic)
+// and the reverse when inv < 0, so the distances equal the min/max form's.
+// sel comes from near_far_selectors(inv), once per ray.
+//
+// Used only by the any-hit traversals of the FP16 nodes, where box tests prune
+// and never order the answer: a ray's any-hit answer is whether some triangle
+// passes the Moller-Trumbore test, whichever boxes are visited, as long as no
+// box holding such a triangle is rejected.  This form and slab round each
+// plane distance by about half an ulp of the larger of |b| and |o| (the
+// product o * inv, or the difference b - o, is rounded), against boxes
+// widened outward by at least one FP16 ulp.  Valid only while fused_ok(oi):
+// with a finite -(o * inv) each plane is the exact b * inv - o * inv rounded
+// once (no intermediate overflow), so its sign and order are right.
 __device__ inline uint3 near_far_selectors(float3 inv) {
   // byte_permute({S0, S1}, sel): bytes 0-3 of S1, then 4-7 of S0
   return make_uint3(inv.x >= 0.0f ? 0x07060100u : 0x01000706u,   // S0 = w.y (maxx hi), S1 = w.x (minx lo)
@@ -356,7 +345,7 @@ __device__ inline uint3 near_far_selectors(float3 inv) {
                     inv.z >= 0.0f ? 0x07060100u : 0x01000706u);  // S0 = w.z (maxz hi), S1 = w.y (minz lo)
 }
 
-__device__ inline float2 slab_fused_nf(float3 oi, float3 inv, uint4 w, uint3 sel) {
+__device__ inline float2 slab_fused(float3 oi, float3 inv, uint4 w, uint3 sel) {
   const uint32_t bx = __builtin_amdgcn_perm(w.y, w.x, sel.x);
   const uint32_t by = __builtin_amdgcn_perm(w.z, w.x, sel.y);
   const uint32_t bz = __builtin_amdgcn_perm(w.z, w.y, sel.z);

@@ -29,8 +29,9 @@ constexpr int kWave = 64;
 constexpr int kBlock4 = 64;
 constexpr int kRefillAt4 = 40;
 // LDS stack entries per lane of the one-ray-per-lane kernels (the rest spill
-// to global memory).  Deep, incoherent nearest-hit walks push often, so these
-// keep a longer LDS part than the persistent kernels' kShortStack.
+// to global memory).  Deep, incoherent nearest-hit walks push often and
+// measured slower with spills, so BVH4 keeps its whole stack in LDS; BVH8
+// (80 entries) keeps 32.
 constexpr int kSimpleLds4 = 48;
 constexpr int kSimpleLds8 = 32;
 // The speculative any-hit kernels leave the node loop for the leaf phase once
@@ -70,7 +71,7 @@ __device__ inline void visit_quad(const void* __restrict__ quads, uint32_t q, fl
     if (kFused && fz) {
       const uint3 sel = near_far_selectors(inv);
 #pragma unroll
-      for (int k = 0; k < 4; k++) tt[k] = slab_fused_nf(oi, inv, w[k], sel);
+      for (int k = 0; k < 4; k++) tt[k] = slab_fused(oi, inv, w[k], sel);
     } else {
 #pragma unroll
       for (int k = 0; k < 4; k++)
@@ -331,11 +332,14 @@ template <bool kFp16>
 hipError_t launch_simple4(const Blas& b, const TraceArgs& a, hipStream_t stream) {
   const uint32_t blocks = uint32_t((a.n + kBlock4 - 1) / kBlock4);
   const void* q = kFp16 ? static_cast<const void*>(b.quads16) : static_cast<const void*>(b.quads32);
-  // stack entries beyond the LDS part, per lane
+  // stack entries beyond the LDS part, per lane (none while LDS holds them all)
   uint32_t* spill = nullptr;
-  hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&spill),
-                                sizeof(uint32_t) * (size_t(blocks) * kBlock4 * (kStackDepth4 - kSimpleLds4) + 64), stream);
-  if (e != hipSuccess) return e;
+  hipError_t e = hipSuccess;
+  if (kSimpleLds4 < kStackDepth4 &&
+      (e = hipMallocAsync(reinterpret_cast<void**>(&spill),
+                          sizeof(uint32_t) * size_t(blocks) * kBlock4 * (kStackDepth4 - kSimpleLds4), stream)) !=
+          hipSuccess)
+    return e;
   const uint2 ids = make_uint2(a.inst_id, a.geom_id);
   if (a.anyhit) {
     if (a.stats)
@@ -353,7 +357,7 @@ hipError_t launch_simple4(const Blas& b, const TraceArgs& a, hipStream_t stream)
                                                                           a.format, ids, nullptr, spill);
   }
   e = hipGetLastError();
-  const hipError_t f = hipFreeAsync(spill, stream);
+  const hipError_t f = spill ? hipFreeAsync(spill, stream) : hipSuccess;
   return e != hipSuccess ? e : f;
 }
 
@@ -374,7 +378,7 @@ __device__ inline void visit_oct(const Oct16* __restrict__ octs, uint32_t q, flo
   if (kFused && fz) {
     const uint3 sel = near_far_selectors(inv);
 #pragma unroll
-    for (int k = 0; k < 8; k++) tt[k] = slab_fused_nf(oi, inv, w[k], sel);
+    for (int k = 0; k < 8; k++) tt[k] = slab_fused(oi, inv, w[k], sel);
   } else {
 #pragma unroll
     for (int k = 0; k < 8; k++)
@@ -581,7 +585,7 @@ hipError_t launch_simple8(const Blas& b, const TraceArgs& a, hipStream_t stream)
   const uint32_t blocks = uint32_t((a.n + kBlock4 - 1) / kBlock4);
   uint32_t* spill = nullptr;
   hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&spill),
-                                sizeof(uint32_t) * (size_t(blocks) * kBlock4 * (kStackDepth8 - kSimpleLds8) + 64), stream);
+                                sizeof(uint32_t) * size_t(blocks) * kBlock4 * (kStackDepth8 - kSimpleLds8), stream);
   if (e != hipSuccess) return e;
   const uint2 ids = make_uint2(a.inst_id, a.geom_id);
   if (a.anyhit) {
