@@ -235,7 +235,7 @@ __global__ __launch_bounds__(kBlock4, 8) void k_trace4_any_persistent(const void
   h.u = h.v = 0.f;
   int top = 0;
   uint32_t cur = kInvalid;
-  uint32_t leaf = 0;
+  uint32_t leaf = 0, leaf2 = 0;  // parked leaves (leaf2 only while leaf is held)
   RayPool pool;  // ray indices claimed by this wave
 
   while (true) {
@@ -258,6 +258,7 @@ __global__ __launch_bounds__(kBlock4, 8) void k_trace4_any_persistent(const void
           top = 0;
           cur = (h.t > tnear) ? 0u : kInvalid;
           leaf = 0;
+          leaf2 = 0;
           active = true;
         }
       }
@@ -265,11 +266,13 @@ __global__ __launch_bounds__(kBlock4, 8) void k_trace4_any_persistent(const void
     if (__ballot(active) == 0) break;
 
     while (true) {
-      // inner phase: park the first leaf found; a lane with a parked leaf
-      // keeps descending while enough lanes of the wave still look for one
+      // inner phase: park the first two leaves found; a lane with a parked
+      // leaf keeps descending while enough lanes of the wave still look for
+      // one
       while (true) {
-        if (active && leaf == 0 && (cur & kLeafBit) != 0 && cur != kInvalid) {
-          leaf = cur;
+        if (active && (cur & kLeafBit) != 0 && cur != kInvalid && (leaf == 0 || leaf2 == 0)) {
+          if (leaf == 0) leaf = cur;
+          else leaf2 = cur;
           cur = (top > 0) ? stk.pop(top) : kInvalid;
         }
         // leave for the leaf phase once at most kLeafPhaseAt lanes still
@@ -292,9 +295,15 @@ __global__ __launch_bounds__(kBlock4, 8) void k_trace4_any_persistent(const void
       // leaf phase
       if (active && leaf != 0) {
         const uint32_t s = leaf_start(leaf), c = leaf_count(leaf);
-        for (uint32_t k = s; k < s + c; k++)
-          if (mt_test_lex(o, d, tnear, tris[k], h)) break;
+        bool hit = false;
+        for (uint32_t k = s; k < s + c && !hit; k++) hit = mt_test_lex(o, d, tnear, tris[k], h);
+        if (!hit && leaf2 != 0) {
+          const uint32_t s2 = leaf_start(leaf2), c2 = leaf_count(leaf2);
+          for (uint32_t k = s2; k < s2 + c2; k++)
+            if (mt_test_lex(o, d, tnear, tris[k], h)) break;
+        }
         leaf = 0;
+        leaf2 = 0;
       }
       if (active && (h.prim != kInvalid || cur == kInvalid)) {
         out[rid] = (h.prim != kInvalid) ? 1 : 0;
