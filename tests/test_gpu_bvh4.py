@@ -321,3 +321,34 @@ def test_any_hit_axis_parallel_far_from_origin(gpu_ctx, layout):
     np.testing.assert_array_equal(gs["pairs"], nq)
     np.testing.assert_array_equal(gs["tris"], nt)
     s.close()
+
+
+@pytest.mark.parametrize("layout", [H.LAYOUT_BVH2, H.LAYOUT_BVH4, H.LAYOUT_BVH4_FP16, H.LAYOUT_AUTO_FP16,
+                                    H.LAYOUT_AUTO_FP16_WIDE],
+                         ids=["bvh2", "bvh4", "bvh4_fp16", "auto_fp16", "auto_fp16_wide"])
+def test_any_hit_interval_edge_cases(gpu_ctx, layout):
+    """Empty and reversed (tNear, tFar) intervals, a negative tNear and a NaN bound: the any-hit
+    kernels finish such rays at once; answers equal the oracle's brute force."""
+    v, f = W.icosphere(3, 1.0, (0.0, 0.0, 0.0))
+    rays = rays_for(v, 6000, 2100)
+    n = rays.shape[0]
+    span = float(np.max(v.max(0) - v.min(0)))
+    kind = np.arange(n) % 6
+    rays[:, 3] = 0.0
+    rays[:, 7] = 3.0 * span                       # origins sit 1.5 extents from the centre
+    rays[kind == 1, 7] = 0.0                      # tFar = tNear = 0
+    rays[kind == 2, 3] = 0.5 * span               # tNear > tFar
+    rays[kind == 2, 7] = 0.25 * span
+    rays[kind == 3, 3] = -span                    # negative tNear
+    rays[kind == 4, 7] = np.float32(np.nan)       # NaN tFar
+    rays[kind == 5, 3] = 1.2 * span               # a proper interval that starts late
+    say(f"any-hit interval edge cases, layout {layout}")
+    s = scene_with(gpu_ctx, v, f, layout)
+    anyr = rays.copy()
+    anyr[:, 7] = -anyr[:, 7]
+    _, pb, _, _ = O.OracleBVH(v, f).trace4(anyr, O.MODE4_BRUTE)
+    g = trace(s, rays, "any")
+    np.testing.assert_array_equal(g["hit"], pb != INVALID)
+    assert not g["hit"][(kind == 1) | (kind == 2) | (kind == 4)].any()
+    assert g["hit"][kind == 3].any() and g["hit"][kind == 0].any()
+    s.close()
