@@ -64,7 +64,7 @@ EXPORTS = [
     "hrt_scene_set_keep_build_arrays", "hrt_scene_clear_geom", "hrt_scene_clear_instances",
     "hrt_add_triangles", "hrt_add_instance", "hrt_commit", "hrt_trace_nearest", "hrt_trace_any",
     "hrt_trace_nearest_stats", "hrt_trace_any_stats", "hrt_query_nearest_one", "hrt_geom_count",
-    "hrt_instance_count", "hrt_debug_dump", "hrt_last_error", "hrt_rtao_make_rays", "hrt_rtao_trace", "hrt_eye_make_rays", "hrt_eye_render", "hrt_scene_serialize", "hrt_scene_deserialize",
+    "hrt_instance_count", "hrt_debug_dump", "hrt_last_error", "hrt_rtao_make_rays", "hrt_rtao_trace", "hrt_rtao_render", "hrt_eye_make_rays", "hrt_eye_render", "hrt_scene_serialize", "hrt_scene_deserialize",
     "hrt_trace_nearest_host", "hrt_trace_any_host", "hrt_update_triangles", "hrt_update_instance",
     "hrt_batch_create", "hrt_batch_destroy", "hrt_batch_buffers", "hrt_batch_run", "hrt_batch_build",
     "hrt_batch_trace", "hrt_batch_candidate_count", "hrt_batch_candidate_info", "hrt_batch_set_candidate",
@@ -135,6 +135,7 @@ def lib():
         L.hrt_debug_dump.argtypes = [vp, u32, i32, vp, sz]
         L.hrt_rtao_make_rays.argtypes = [vp, vp, vp, sz, u32, C.c_float, C.c_uint64, vp, vp]
         L.hrt_rtao_trace.argtypes = [vp, vp, vp, sz, u32, C.c_float, C.c_uint64, vp, vp]
+        L.hrt_rtao_render.argtypes = [vp, vp, vp, u32, u32, u32, C.c_float, C.c_uint64, C.c_float, vp, vp]
         L.hrt_eye_make_rays.argtypes = [vp, vp, vp, u32, u32, vp, vp, vp]
         L.hrt_eye_render.argtypes = [vp, vp, vp, u32, u32, vp, u32, vp, vp]
         L.hrt_scene_serialize.argtypes = [vp, vp, sz, C.POINTER(C.c_size_t)]
@@ -347,6 +348,15 @@ class Scene:
         _check(lib().hrt_scene_deserialize(ctx._h, _ptr(buf), buf.size, C.byref(s._h)))
         _live_scenes.add(s)
         return s
+
+    def rtao_render(self, proj_inv, world_view_inv, width: int, height: int, spp: int, tmax: float, seed: int,
+                    out_color, power: float = 1.0, stream: int = 0):
+        """The 02_rtao frame on the device (hrt_rtao_render): out_color is a device buffer of
+        width * height words."""
+        pi = np.ascontiguousarray(proj_inv, dtype=np.float32).reshape(16)
+        wi = np.ascontiguousarray(world_view_inv, dtype=np.float32).reshape(16)
+        _check(lib().hrt_rtao_render(self._h, _ptr(pi), _ptr(wi), width, height, spp, float(tmax),
+                                     seed & 0xFFFFFFFFFFFFFFFF, float(power), _dev_ptr(out_color), stream or None))
 
     def eye_render(self, proj_inv, world_view_inv, width: int, height: int, palette, out_color, stream: int = 0):
         """The 01_eye_rays frame on the device (hrt_eye_render): out_color is a device buffer of

@@ -622,6 +622,58 @@ hrt_status hrt_rtao_make_rays(hrt_scene* s, const float* primaryRays, const void
   return HRT_OK;
 }
 
+}  // extern "C"
+
+namespace {
+// The AO pass of one frame: AO rays of every primary hit, traced for any hit
+// on the tree any-hit rays use in this layout (as trace_common chooses it).
+// BVH4 layouts make each ray inside the traversal from a per-pixel frame;
+// BVH2 and 8-wide nodes go through a temporary ray buffer.
+hipError_t rtao_pass(hrt_scene* s, const hrt::Blas* b, uint32_t inst, const hrt::Ray* prim, const void* hits,
+                     uint64_t npix, uint32_t spp, float tmax, uint64_t seed, uint8_t* out, hipStream_t hs) {
+  const uint64_t n = npix * spp;
+  const int lay = s->layout;
+  const bool auto16 = lay == HRT_LAYOUT_AUTO_FP16 || lay == HRT_LAYOUT_AUTO_FP16_WIDE;
+  const bool wide = lay == HRT_LAYOUT_AUTO_FP16_WIDE && b->any_tree && b->any_tree->octs16;
+  hipError_t e = hipSuccess;
+  if (lay == HRT_LAYOUT_BVH2 || wide) {
+    void* rays = nullptr;
+    if ((e = hipMallocAsync(&rays, sizeof(hrt::Ray) * n, hs)) != hipSuccess) return e;
+    hrt::TraceArgs a;
+    a.rays = static_cast<const hrt::Ray*>(rays);
+    a.n = n;
+    a.out = out;
+    a.format = HRT_HIT_FULL;
+    a.anyhit = 1;
+    a.stats = nullptr;
+    a.inst_id = inst;
+    a.geom_id = s->instances[inst].geom;
+    e = hrt::rtao_make_rays(*b, prim, hits, npix, spp, tmax, seed, static_cast<hrt::Ray*>(rays), hs);
+    if (e == hipSuccess) e = wide ? hrt::trace_blas8(*b->any_tree, a, hs) : hrt::trace_blas(*b, a, hs);
+    const hipError_t fe = hipFreeAsync(rays, hs);
+    return e == hipSuccess ? fe : e;
+  }
+  const hrt::Blas& tree = (auto16 && b->any_tree) ? *b->any_tree : *b;
+  const bool fp16 = lay == HRT_LAYOUT_BVH4_FP16 || auto16;
+  void* frames = nullptr;
+  if ((e = hipMallocAsync(&frames, 32ull * npix, hs)) != hipSuccess) return e;
+  hrt::AoFrame* f = static_cast<hrt::AoFrame*>(frames);
+  e = hrt::rtao_frames(*b, prim, hits, npix, f, hs);
+  if (e == hipSuccess) e = hrt::trace_ao4(tree, fp16, f, npix, spp, tmax, seed, out, hs);
+  const hipError_t fe = hipFreeAsync(frames, hs);
+  return e == hipSuccess ? fe : e;
+}
+
+hrt_status rtao_counts(size_t nPixels, uint32_t raysPerHit) {
+  const uint64_t n = uint64_t(nPixels) * raysPerHit;
+  if (raysPerHit && (n / raysPerHit != nPixels || n > 0xFFFFFFFFull))
+    return fail(HRT_ERR_INVALID_ARG, "nPixels * raysPerHit must stay below 2^32");
+  return HRT_OK;
+}
+}  // namespace
+
+extern "C" {
+
 hrt_status hrt_rtao_trace(hrt_scene* s, const float* primaryRays, const void* primaryHitsCompact, size_t nPixels,
                           uint32_t raysPerHit, float tMax, uint64_t seed, uint8_t* outHits, void* stream) {
   const hrt::Blas* b = nullptr;
@@ -630,46 +682,12 @@ hrt_status hrt_rtao_trace(hrt_scene* s, const float* primaryRays, const void* pr
   if (st != HRT_OK) return st;
   if (nPixels == 0 || raysPerHit == 0) return HRT_OK;
   if (!primaryRays || !primaryHitsCompact || !outHits) return fail(HRT_ERR_INVALID_ARG, "null buffer");
-  const uint64_t n = uint64_t(nPixels) * raysPerHit;
-  if (n / raysPerHit != nPixels || n > 0xFFFFFFFFull)
-    return fail(HRT_ERR_INVALID_ARG, "nPixels * raysPerHit must stay below 2^32");
+  if ((st = rtao_counts(nPixels, raysPerHit)) != HRT_OK) return st;
   hipStream_t hs = stream ? static_cast<hipStream_t>(stream) : s->ctx->stream;
   hipError_t e = hipSetDevice(s->ctx->device);
-  if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
-  const hrt::Ray* prim = reinterpret_cast<const hrt::Ray*>(primaryRays);
-  // the tree any-hit rays use, as in trace_common
-  const int lay = s->layout;
-  const bool auto16 = lay == HRT_LAYOUT_AUTO_FP16 || lay == HRT_LAYOUT_AUTO_FP16_WIDE;
-  const bool wide = lay == HRT_LAYOUT_AUTO_FP16_WIDE && b->any_tree && b->any_tree->octs16;
-  if (lay == HRT_LAYOUT_BVH2 || wide) {
-    // BVH2 and 8-wide nodes: the rays go through HBM
-    void* rays = nullptr;
-    if ((e = hipMallocAsync(&rays, sizeof(hrt::Ray) * n, hs)) != hipSuccess) return fail_hip(e, "rtao rays");
-    hrt::TraceArgs a;
-    a.rays = static_cast<const hrt::Ray*>(rays);
-    a.n = n;
-    a.out = outHits;
-    a.format = HRT_HIT_FULL;
-    a.anyhit = 1;
-    a.stats = nullptr;
-    a.inst_id = inst;
-    a.geom_id = s->instances[inst].geom;
-    e = hrt::rtao_make_rays(*b, prim, primaryHitsCompact, nPixels, raysPerHit, tMax, seed,
-                            static_cast<hrt::Ray*>(rays), hs);
-    if (e == hipSuccess) e = wide ? hrt::trace_blas8(*b->any_tree, a, hs) : hrt::trace_blas(*b, a, hs);
-    const hipError_t fe = hipFreeAsync(rays, hs);
-    if (e == hipSuccess) e = fe;
-    return e == hipSuccess ? HRT_OK : fail_hip(e, "rtao trace");
-  }
-  const hrt::Blas& tree = (auto16 && b->any_tree) ? *b->any_tree : *b;
-  const bool fp16 = lay == HRT_LAYOUT_BVH4_FP16 || auto16;
-  void* frames = nullptr;
-  if ((e = hipMallocAsync(&frames, 32ull * nPixels, hs)) != hipSuccess) return fail_hip(e, "rtao frames");
-  hrt::AoFrame* f = static_cast<hrt::AoFrame*>(frames);
-  e = hrt::rtao_frames(*b, prim, primaryHitsCompact, nPixels, f, hs);
-  if (e == hipSuccess) e = hrt::trace_ao4(tree, fp16, f, nPixels, raysPerHit, tMax, seed, outHits, hs);
-  const hipError_t fe = hipFreeAsync(frames, hs);
-  if (e == hipSuccess) e = fe;
+  if (e == hipSuccess)
+    e = rtao_pass(s, b, inst, reinterpret_cast<const hrt::Ray*>(primaryRays), primaryHitsCompact, nPixels,
+                  raysPerHit, tMax, seed, outHits, hs);
   return e == hipSuccess ? HRT_OK : fail_hip(e, "rtao trace");
 }
 
@@ -726,6 +744,41 @@ hrt_status hrt_eye_render(hrt_scene* s, const float projInvColMajor16[16], const
   if (st != HRT_OK) return st;
   if (e == hipSuccess) e = fe;
   if (e != hipSuccess) return fail_hip(e, "eye render");
+  return HRT_OK;
+}
+
+hrt_status hrt_rtao_render(hrt_scene* s, const float projInvColMajor16[16], const float worldViewInvColMajor16[16],
+                           uint32_t width, uint32_t height, uint32_t raysPerHit, float tMax, uint64_t seed, float power,
+                           uint32_t* outColor, void* stream) {
+  const hrt::Blas* b = nullptr;
+  uint32_t inst = 0;
+  hrt_status st = select_blas(s, &b, &inst);
+  if (st != HRT_OK) return st;
+  if (!outColor) return fail(HRT_ERR_INVALID_ARG, "null argument");
+  if (raysPerHit == 0) return fail(HRT_ERR_INVALID_ARG, "raysPerHit must be at least 1");
+  hrt::EyeCamera cam;
+  if ((st = eye_args(projInvColMajor16, worldViewInvColMajor16, width, height, &cam)) != HRT_OK) return st;
+  const size_t n = size_t(width) * height;
+  if ((st = rtao_counts(n, raysPerHit)) != HRT_OK) return st;
+  hipStream_t hs = stream ? static_cast<hipStream_t>(stream) : s->ctx->stream;
+  hipError_t e = hipSetDevice(s->ctx->device);
+  if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
+  // eye rays (32 B), compact hits (8 B) and AO bytes (raysPerHit) per pixel
+  void* buf = nullptr;
+  if ((e = hipMallocAsync(&buf, (40 + size_t(raysPerHit)) * n, hs)) != hipSuccess)
+    return fail_hip(e, "rtao render buffers");
+  float* rays = static_cast<float*>(buf);
+  uint32_t* hits = reinterpret_cast<uint32_t*>(static_cast<char*>(buf) + 32 * n);
+  uint8_t* occ = static_cast<uint8_t*>(buf) + 40 * n;
+  e = hrt::eye_make_rays(cam, width, height, reinterpret_cast<hrt::Ray*>(rays), nullptr, hs);
+  st = e == hipSuccess ? trace_common(s, rays, n, hits, HRT_HIT_COMPACT, 0, nullptr, stream) : HRT_OK;
+  if (e == hipSuccess && st == HRT_OK)
+    e = rtao_pass(s, b, inst, reinterpret_cast<const hrt::Ray*>(rays), hits, n, raysPerHit, tMax, seed, occ, hs);
+  if (e == hipSuccess && st == HRT_OK) e = hrt::ao_colors(hits, occ, width, height, raysPerHit, power, outColor, hs);
+  const hipError_t fe = hipFreeAsync(buf, hs);
+  if (st != HRT_OK) return st;
+  if (e == hipSuccess) e = fe;
+  if (e != hipSuccess) return fail_hip(e, "rtao render");
   return HRT_OK;
 }
 

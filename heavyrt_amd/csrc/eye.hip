@@ -85,7 +85,39 @@ __global__ __launch_bounds__(kBlock) void k_eye_colors(const uint2* __restrict__
   out_color[px.y * width + px.x] = prim == 0xFFFFFFFFu ? 0u : pal.c[prim % pal.n];
 }
 
+// The 02_rtao frame's colour (RTAO::kernel_AO2Color and floatToGrayRGBA,
+// examples/02_rtao/rtao.cpp:15-24, 62-72): 0 where the eye ray missed, else
+// the occluded fraction of the pixel's spp AO rays, raised to `power`,
+// clamped to [0, 1] and truncated to a gray level with alpha 255.
+__global__ __launch_bounds__(kBlock) void k_ao_colors(const uint2* __restrict__ hits,
+                                                      const uint8_t* __restrict__ occluded, uint32_t width,
+                                                      uint32_t height, uint32_t spp, float power,
+                                                      uint32_t* __restrict__ out_color) {
+  const uint32_t tid = blockIdx.x * kBlock + threadIdx.x;
+  if (tid >= width * height) return;
+  const uint2 px = superblock_pixel(tid, width);
+  uint32_t c = 0u;
+  if (hits[tid].y != 0xFFFFFFFFu) {
+    uint32_t count = 0;
+    for (uint32_t k = 0; k < spp; k++) count += occluded[uint64_t(tid) * spp + k] ? 1u : 0u;
+    float v = float(count) * (1.0f / float(spp));
+    if (power != 1.0f) v = powf(v, power);
+    const uint32_t g = uint32_t(fminf(fmaxf(v, 0.0f), 1.0f) * 255.0f);
+    c = g | (g << 8) | (g << 16) | (255u << 24);
+  }
+  out_color[px.y * width + px.x] = c;
+}
+
 }  // namespace
+
+hipError_t ao_colors(const void* hits_compact, const uint8_t* occluded, uint32_t width, uint32_t height,
+                     uint32_t spp, float power, uint32_t* out_color, hipStream_t stream) {
+  const uint64_t n = uint64_t(width) * height;
+  if (n == 0) return hipSuccess;
+  k_ao_colors<<<uint32_t((n + kBlock - 1) / kBlock), kBlock, 0, stream>>>(
+      static_cast<const uint2*>(hits_compact), occluded, width, height, spp, power, out_color);
+  return hipGetLastError();
+}
 
 hipError_t eye_make_rays(const EyeCamera& cam, uint32_t width, uint32_t height, Ray* out, uint32_t* packed_xy,
                          hipStream_t stream) {

@@ -184,5 +184,9 @@ hipError_t eye_make_rays(const EyeCamera& cam, uint32_t width, uint32_t height, 
 // Pixel colours from compact hits of those rays (row-major image).
 hipError_t eye_colors(const void* hits_compact, uint32_t width, uint32_t height, const EyePalette& pal,
                       uint32_t* out_color, hipStream_t stream);
+// The 02_rtao frame's colours from compact primary hits (superblock order)
+// and one occlusion byte per AO ray (spp per pixel).
+hipError_t ao_colors(const void* hits_compact, const uint8_t* occluded, uint32_t width, uint32_t height,
+                     uint32_t spp, float power, uint32_t* out_color, hipStream_t stream);
 
 }  // namespace hrt

@@ -262,6 +262,18 @@ hrt_status hrt_eye_render(hrt_scene* scene, const float projInvColMajor16[16], c
                           uint32_t width, uint32_t height, const uint32_t* palette, uint32_t paletteSize,
                           uint32_t* outColor, void* stream);
 
+/* The 02_rtao renderer's frame on the device (RTAO::CalcAO for every pixel,
+ * examples/02_rtao/rtao.cpp:46-72): eye rays as hrt_eye_make_rays makes them,
+ * nearest hits as hrt_trace_nearest finds them, raysPerHit AO rays per hit as
+ * hrt_rtao_trace traces them, then kernel_AO2Color: outColor[y * width + x]
+ * (a device image of width * height words) is 0 where the eye ray missed,
+ * else the occluded fraction of its AO rays raised to `power`, clamped to
+ * [0, 1] and truncated to a gray level, with alpha 255.  Width and height
+ * must be multiples of 8; the scene must hold one identity instance. */
+hrt_status hrt_rtao_render(hrt_scene* scene, const float projInvColMajor16[16], const float worldViewInvColMajor16[16],
+                           uint32_t width, uint32_t height, uint32_t raysPerHit, float tMax, uint64_t seed, float power,
+                           uint32_t* outColor, void* stream);
+
 /* ISceneObject::GetGeomNum / GetInstNum (CrossRT.h:163-164). */
 uint32_t hrt_geom_count(const hrt_scene* scene);
 uint32_t hrt_instance_count(const hrt_scene* scene);

@@ -80,3 +80,79 @@ def test_rtao_trace_errors(gpu_ctx):
         s.rtao_trace(d, d, 1, 1, 1.0, 0, o)
     assert e.value.status == 5
     s.close()
+
+
+@pytest.mark.parametrize("layout", [H.LAYOUT_BVH2, H.LAYOUT_AUTO_FP16], ids=["bvh2", "auto_fp16"])
+def test_rtao_render_matches_oracle(gpu_ctx, layout):
+    """hrt_rtao_render's image equals kernel_AO2Color applied to the oracle's answers: primary hits
+    of the device eye rays and any-hit answers for the AO rays hrt_rtao_make_rays makes from them."""
+    import torch
+
+    from oracle import oracle as O
+
+    v, f = W.rtao_scene(128, 64, 150)
+    width, height, spp, seed = 64, 48, 8, 99
+    n = width * height
+    tmax = 0.1 * W.scene_diagonal(v)
+    # a camera over the heightfield toward the horizon: about half the pixels see sky
+    pi, wi = W.eye_camera(width, height, eye=(1.0, 0.45, -0.3), target=(1.0, 0.25, 0.8), fov_y_deg=60.0)
+    s = H.Scene(gpu_ctx)
+    s.add_instance(s.add_triangles(v, f))
+    s.commit(layout=layout)
+    stream = torch.cuda.current_stream().cuda_stream
+    rays = torch.empty((n, 8), dtype=torch.float32, device="cuda")
+    xy = torch.empty(n, dtype=torch.int32, device="cuda")
+    gpu_ctx.eye_make_rays(pi, wi, width, height, rays, xy, stream=stream)
+    img = torch.full((n,), 7, dtype=torch.int32, device="cuda")
+    s.rtao_render(pi, wi, width, height, spp, tmax, seed, img, stream=stream)
+    img2 = torch.full((n,), 7, dtype=torch.int32, device="cuda")
+    s.rtao_render(pi, wi, width, height, spp, tmax, seed, img2, power=2.0, stream=stream)
+    hits = torch.zeros((n, 2), dtype=torch.int32, device="cuda")
+    s.trace_nearest(rays, n, hits, H.HIT_COMPACT, stream=stream)
+    ao = torch.empty((n * spp, 8), dtype=torch.float32, device="cuda")
+    s.rtao_make_rays(rays, hits, n, spp, tmax, seed, ao, stream=stream)
+    torch.cuda.synchronize()
+    say(f"rtao render layout {layout}")
+    ob = O.OracleBVH(v, f)
+    t, p, _, _ = ob.trace(rays.cpu().numpy())
+    h = hits.cpu().numpy().view(np.uint32)
+    np.testing.assert_array_equal(h[:, 1], p)
+    np.testing.assert_array_equal(h[:, 0], t.view(np.uint32))
+    anyr = ao.cpu().numpy()
+    anyr[:, 7] = -np.abs(anyr[:, 7])
+    _, pa, _, _ = ob.trace(anyr)
+    count = (pa != 0xFFFFFFFF).reshape(n, spp).sum(axis=1)
+    frac = count.astype(np.float32) * (np.float32(1.0) / np.float32(spp))
+    packed = xy.cpu().numpy().view(np.uint32)
+    px, py = packed & 0xFFFF, packed >> 16
+
+    def image(val):
+        g = (np.clip(val, np.float32(0.0), np.float32(1.0)) * np.float32(255.0)).astype(np.uint32)
+        c = np.where(p == 0xFFFFFFFF, 0, g | (g << 8) | (g << 16) | (255 << 24)).astype(np.uint32)
+        out = np.zeros(n, np.uint32)
+        out[py * width + px] = c
+        return out
+
+    np.testing.assert_array_equal(img.cpu().numpy().view(np.uint32), image(frac))
+    assert (p == 0xFFFFFFFF).any() and (p != 0xFFFFFFFF).any() and 0 < (count > 0).mean() < 1
+    # another power goes through powf: within one gray level of numpy's float32 power
+    got2 = img2.cpu().numpy().view(np.uint32) & 0xFF
+    want2 = image(np.power(frac, np.float32(2.0))) & 0xFF
+    assert np.max(np.abs(got2.astype(np.int64) - want2.astype(np.int64))) <= 1
+    s.close()
+
+
+def test_rtao_render_errors(gpu_ctx):
+    import torch
+
+    v, f = W.icosphere(2)
+    s = H.Scene(gpu_ctx)
+    s.add_instance(s.add_triangles(v, f))
+    s.commit()
+    pi, wi = W.eye_camera(64, 64)
+    img = torch.zeros(64 * 64, dtype=torch.int32, device="cuda")
+    for w, hgt, spp in ((60, 64, 4), (64, 64, 0)):  # width not a multiple of 8, no AO rays
+        with pytest.raises(H.HrtError) as e:
+            s.rtao_render(pi, wi, w, hgt, spp, 1.0, 0, img)
+        assert e.value.status == 1
+    s.close()
