@@ -137,7 +137,8 @@ def main() -> int:
 
     log("generating scene")
     verts, faces = W.rtao_scene()
-    prim_rays = W.rtao_primary_rays()
+    # one eye ray per pixel, laid out in the renderers' 8x8-superblock order
+    prim_rays = W.rtao_primary_rays(order="superblock")
     npix = prim_rays.shape[0]
     tmax = 0.1 * W.scene_diagonal(verts)
     n_ao = npix * SPP
@@ -245,6 +246,24 @@ def main() -> int:
     log(f"AO pass: fused {ao_pass['fused_ms']:.3f} ms; two-step raygen {ao_pass['two_step_raygen_ms']:.3f} + "
         f"any-hit {ao_pass['two_step_any_hit_ms']:.3f} ms; identical {ao_identical}")
 
+    # the whole 02_rtao frame on the device (hrt_rtao_render): eye rays of the
+    # same camera in the renderer's 8x8-superblock order, nearest hits, the
+    # fused AO pass and the AO image
+    cam_pi, cam_wi = W.eye_camera(1024, 1024, eye=W.RTAO_EYE, target=W.RTAO_TARGET, fov_y_deg=W.RTAO_FOV)
+    d_img = torch.zeros(npix, dtype=torch.int32, device=dev)
+    for _ in range(2):
+        scene.rtao_render(cam_pi, cam_wi, 1024, 1024, SPP, tmax, SEED, d_img, stream=sptr)
+    evf = [[torch.cuda.Event(enable_timing=True) for _ in range(2)] for _ in range(5)]
+    for e in evf:
+        e[0].record(stream)
+        scene.rtao_render(cam_pi, cam_wi, 1024, 1024, SPP, tmax, SEED, d_img, stream=sptr)
+        e[1].record(stream)
+    torch.cuda.synchronize()
+    frame_ms = float(np.mean([e[0].elapsed_time(e[1]) for e in evf]))
+    frame = {"frame_ms": round(frame_ms, 4), "mrays_s": round(rays_per_step / (frame_ms * 1e-3) / 1e6, 1),
+             "note": "hrt_rtao_render, 1024x1024, 16 AO rays per hit, eye rays made on the device"}
+    log(f"RTAO frame on the device: {frame_ms:.3f} ms")
+
     # traversal counters (counting variant, untimed)
     def counters(layout_name: str, any_hit: bool):
         scene.commit(layout=step_layout[layout_name])
@@ -304,7 +323,8 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "rtao: 1024x1024 primary nearest-hit (LBVH2, reference order) + 16 AO any-hit rays per "
+            "workload": "rtao: 1024x1024 primary nearest-hit (LBVH2, reference order; rays in 8x8-superblock "
+                        "pixel order) + 16 AO any-hit rays per "
                         f"hit ({args.any_layout}; "
                         + ("rays written to HBM, then traced" if args.two_step else
                            "rays made inside the any-hit traversal") + "), 1,072,576-tri heightfield+boxes",
@@ -339,6 +359,7 @@ def main() -> int:
         "any_hit_tree_build_ms": round(any_tree_ms, 3),
         "counting_variant_identical": same_p and same_a,
         "ao_pass": ao_pass,
+        "device_frame": frame,
         "any_hit_layouts": variants,
         "cpu_baseline": None,
     }

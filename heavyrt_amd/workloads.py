@@ -323,8 +323,23 @@ RTAO_TARGET = (1.0, 0.0, 0.45)
 RTAO_FOV = 44.0
 
 
-def rtao_primary_rays(width: int = 1024, height: int = 1024) -> np.ndarray:
-    return pinhole_rays(width, height, eye=RTAO_EYE, target=RTAO_TARGET, fov_y_deg=RTAO_FOV)
+def superblock_order(width: int, height: int) -> np.ndarray:
+    """Row-major pixel index (y * width + x) of each thread in the renderers' 8x8-superblock
+    packing (SuperBlockIndex2DOpt, examples/01_eye_rays/eye_ray.cpp:114-131): superblocks of four
+    4x4 blocks, in rows of superblocks.  width and height must be multiples of 8."""
+    tid = np.arange(width * height, dtype=np.int64)
+    sb, in_h, in_b = tid >> 6, (tid >> 4) & 3, tid & 15
+    wb = width >> 3
+    bx = (sb % wb) * 2 + (in_h & 1)
+    by = (sb // wb) * 2 + (in_h >> 1)
+    return ((by * 4 + (in_b >> 2)) * width + bx * 4 + (in_b & 3)).astype(np.int64)
+
+
+def rtao_primary_rays(width: int = 1024, height: int = 1024, order: str = "rows") -> np.ndarray:
+    """RTAO eye rays, one per pixel: in row-major order, or ("superblock") in the renderers'
+    8x8-superblock order."""
+    rays = pinhole_rays(width, height, eye=RTAO_EYE, target=RTAO_TARGET, fov_y_deg=RTAO_FOV)
+    return rays[superblock_order(width, height)] if order == "superblock" else rays
 
 
 def incoherent_rays(n: int, stream: int = STREAM_RAYS, tfar: float = float(FLT_MAX)) -> np.ndarray:

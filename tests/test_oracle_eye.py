@@ -59,3 +59,13 @@ def test_eye_rays_mirror_about_the_centre():
         j = k[(w - 1 - a, h - 1 - b)]
         np.testing.assert_allclose(d[i, 0:2], -d[j, 0:2], atol=1e-6)
         np.testing.assert_allclose(d[i, 2], d[j, 2], atol=1e-6)
+
+
+def test_superblock_order_inverts_the_oracle_packing():
+    """workloads.superblock_order (the host-side ray layout of the RTAO bench) is the inverse of the
+    oracle's SuperBlockIndex2DOpt restatement on every pixel."""
+    for w, h in ((64, 48), (128, 128)):
+        order = W.superblock_order(w, h)
+        assert sorted(order.tolist()) == list(range(w * h))
+        tid = np.array([O.superblock_offset(int(i % w), int(i // w), w) for i in order])
+        np.testing.assert_array_equal(tid, np.arange(w * h))
