@@ -27,6 +27,11 @@
 //                         (EyeRayDirNormalized, transform_ray3f).  LiteMath's
 //                         float4x4 * float4 is taken as each row's products
 //                         summed left to right, normalize(u) = u / sqrt(dot(u, u))
+//   AO rays               examples/02_rtao/rtao.cpp:92, 147-165 (hit point,
+//                         kernel_InitAORay); examples/render_common.h:56-112
+//                         (MapSampleToCosineDistribution, OffsRayPos).  The
+//                         draws are PCG32 (pcg-c-basic), sequence = global
+//                         sample index, in place of the renderer's QMC table
 //
 // Floating point: every float3 operation is scalar, left to right
 // (dot = x*x' + y*y' + z*z'), and the file is compiled with
@@ -1900,6 +1905,93 @@ uint32_t hrto_superblock_offset(uint32_t x, uint32_t y, uint32_t width) {
 
 // kernel_PackXY over every pixel, then kernel_InitEyeRay for every thread
 // index.  Matrices are column-major.  rays: 8 floats per thread index.
+static uint32_t pcg32_step(uint64_t& state, uint64_t inc) {
+  const uint64_t old = state;
+  state = old * 6364136223846793005ull + inc;
+  const uint32_t xs = uint32_t(((old >> 18u) ^ old) >> 27u);
+  const uint32_t rot = uint32_t(old >> 59u);
+  return (xs >> rot) | (xs << ((32u - rot) & 31u));
+}
+
+// AO rays from primary hits, the engine's ray generator (device code ao.h)
+// restated with the same float operations: per pixel the hit point
+// o + 0.9999995 * t * d (rtao.cpp:92), the viewer-facing unit geometric
+// normal, the OffsRayPos offset (render_common.h:75-112; kGEps = 2e-5);
+// per sample k = pixel * spp + s two PCG32 draws (sequence k, seed), the
+// cosine-weighted direction (render_common.h:62-70) in Duff et al.'s basis
+// about the normal, tNear = 0, tFar = tMax.  A missed pixel gets the empty
+// ray (0,0,0,0), (0,1,0,0).  verts3: 3 floats per vertex; hits: compact
+// (t bits, primId) per pixel.
+// n draws of PCG32 seeded with (initstate = seed, initseq = seq), the
+// pcg32_srandom_r / pcg32_random_r pair of O'Neill's pcg-c-basic (the
+// generator the AO rays draw from).
+void hrto_pcg32(uint64_t seed, uint64_t seq, uint32_t n, uint32_t* out) {
+  const uint64_t inc = (seq << 1u) | 1u;
+  uint64_t st = 0;
+  (void)pcg32_step(st, inc);
+  st += seed;
+  (void)pcg32_step(st, inc);
+  for (uint32_t i = 0; i < n; i++) out[i] = pcg32_step(st, inc);
+}
+
+void hrto_ao_rays(const float* verts3, const uint32_t* idx, uint32_t ntris, const float* prim_rays,
+                  const uint32_t* hits, uint64_t npix, uint32_t spp, float tmax, uint64_t seed, float* out) {
+  const float kGEps = 2e-5f;
+  auto step = pcg32_step;
+  for (uint64_t pix = 0; pix < npix; pix++) {
+    const uint32_t prim = hits[2 * pix + 1];
+    float* r = out + 8 * pix * spp;
+    if (prim >= ntris) {
+      for (uint32_t sIdx = 0; sIdx < spp; sIdx++) {
+        const float e[8] = {0.f, 0.f, 0.f, 0.f, 0.f, 1.f, 0.f, 0.f};
+        std::memcpy(r + 8 * sIdx, e, sizeof(e));
+      }
+      continue;
+    }
+    const float* p = prim_rays + 8 * pix;
+    float t;
+    std::memcpy(&t, &hits[2 * pix], 4);
+    const f3 o = {p[0], p[1], p[2]}, d = {p[4], p[5], p[6]};
+    const float tt = 0.9999995f * t;
+    const f3 hp = {o.x + tt * d.x, o.y + tt * d.y, o.z + tt * d.z};
+    const float* A = verts3 + 3 * size_t(idx[3 * size_t(prim) + 0]);
+    const float* B = verts3 + 3 * size_t(idx[3 * size_t(prim) + 1]);
+    const float* C = verts3 + 3 * size_t(idx[3 * size_t(prim) + 2]);
+    const f3 e1 = {B[0] - A[0], B[1] - A[1], B[2] - A[2]};
+    const f3 e2 = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
+    f3 n = {e1.y * e2.z - e1.z * e2.y, e1.z * e2.x - e1.x * e2.z, e1.x * e2.y - e1.y * e2.x};
+    const float inv_len = 1.0f / std::sqrt(n.x * n.x + n.y * n.y + n.z * n.z);
+    n = {n.x * inv_len, n.y * inv_len, n.z * inv_len};
+    if (n.x * d.x + n.y * d.y + n.z * d.z > 0.0f) n = {-n.x, -n.y, -n.z};
+    const float m = std::fmax(std::fmax(std::fabs(hp.x), std::fmax(std::fabs(hp.y), std::fabs(hp.z))), 2.0f * kGEps) * kGEps;
+    const f3 org = {hp.x + m * n.x, hp.y + m * n.y, hp.z + m * n.z};
+    for (uint32_t sIdx = 0; sIdx < spp; sIdx++) {
+      const uint64_t k = pix * spp + sIdx;
+      const uint64_t inc = (k << 1u) | 1u;
+      uint64_t st = 0;
+      (void)step(st, inc);
+      st += seed;
+      (void)step(st, inc);
+      const float r1 = float(step(st, inc) >> 8) * (1.0f / 16777216.0f);
+      const float r2 = float(step(st, inc) >> 8) * (1.0f / 16777216.0f);
+      const float phi = 6.28318530717958647692f * r1;
+      const float sphi = std::sin(phi), cphi = std::cos(phi);
+      const float cth = std::sqrt(1.0f - r2);
+      const float sth = std::sqrt(r2);
+      const float sgn = std::copysign(1.0f, n.z);
+      const float a = -1.0f / (sgn + n.z);
+      const float b = n.x * n.y * a;
+      const f3 tx = {1.0f + sgn * n.x * n.x * a, sgn * b, -sgn * n.x};
+      const f3 ty = {b, sgn + n.y * n.y * a, -n.y};
+      const float lx = sth * cphi, ly = sth * sphi;
+      const float ray[8] = {org.x, org.y, org.z, 0.0f,
+                            tx.x * lx + ty.x * ly + n.x * cth, tx.y * lx + ty.y * ly + n.y * cth,
+                            tx.z * lx + ty.z * ly + n.z * cth, tmax};
+      std::memcpy(r + 8 * sIdx, ray, sizeof(ray));
+    }
+  }
+}
+
 void hrto_eye_rays(const float* projInv, const float* worldViewInv, uint32_t width, uint32_t height, float* rays,
                    uint32_t* packedXY) {
   const size_t n = size_t(width) * height;

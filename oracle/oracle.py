@@ -93,6 +93,10 @@ def lib():
         L.hrto_pack_leaf.argtypes = [u32, u32]
         L.hrto_safe_inverse.argtypes = [vp, vp]
         L.hrto_slab.argtypes = [vp, vp, vp, vp, vp]
+        L.hrto_pcg32.restype = None
+        L.hrto_pcg32.argtypes = [u64, u64, u32, vp]
+        L.hrto_ao_rays.restype = None
+        L.hrto_ao_rays.argtypes = [vp, vp, C.c_uint32, vp, vp, u64, C.c_uint32, C.c_float, C.c_uint64, vp]
         L.hrto_slab_fused.argtypes = [vp, vp, vp, vp, vp]
         L.hrto_quad_count.restype = u64
         L.hrto_quad_count.argtypes = [vp]
@@ -375,6 +379,27 @@ def slab_fused(o, inv, bmin, bmax) -> np.ndarray:
 def superblock_offset(x: int, y: int, width: int) -> int:
     """SuperBlockIndex2DOpt (examples/01_eye_rays/eye_ray.cpp:114-131): pixel -> thread index."""
     return int(lib().hrto_superblock_offset(x, y, width))
+
+
+def pcg32(seed: int, seq: int, n: int) -> np.ndarray:
+    """n PCG32 draws for (initstate seed, initseq seq), the generator behind ao_rays."""
+    out = np.zeros(n, dtype=np.uint32)
+    lib().hrto_pcg32(int(seed) & 0xFFFFFFFFFFFFFFFF, int(seq) & 0xFFFFFFFFFFFFFFFF, int(n), _ptr(out))
+    return out
+
+
+def ao_rays(verts: np.ndarray, idx: np.ndarray, prim_rays: np.ndarray, hits_compact: np.ndarray, spp: int,
+            tmax: float, seed: int) -> np.ndarray:
+    """The engine's AO ray generator (ao.h) restated: (npix * spp, 8) rays from primary rays and
+    their compact hits (t bits, primId)."""
+    v = np.ascontiguousarray(verts, dtype=np.float32).reshape(-1, 3)
+    f = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1)
+    pr = np.ascontiguousarray(prim_rays, dtype=np.float32).reshape(-1, 8)
+    h = np.ascontiguousarray(hits_compact, dtype=np.uint32).reshape(-1, 2)
+    out = np.zeros((pr.shape[0] * spp, 8), dtype=np.float32)
+    lib().hrto_ao_rays(_ptr(v), _ptr(f), f.size // 3, _ptr(pr), _ptr(h), pr.shape[0], spp, float(tmax),
+                       seed & 0xFFFFFFFFFFFFFFFF, _ptr(out))
+    return out
 
 
 def eye_rays(proj_inv_col_major: np.ndarray, world_view_inv_col_major: np.ndarray, width: int, height: int):

@@ -156,3 +156,48 @@ def test_rtao_render_errors(gpu_ctx):
             s.rtao_render(pi, wi, w, hgt, spp, 1.0, 0, img)
         assert e.value.status == 1
     s.close()
+
+
+@pytest.mark.parametrize("layout", [H.LAYOUT_BVH2, H.LAYOUT_AUTO_FP16], ids=["bvh2", "auto_fp16"])
+def test_rtao_make_rays_match_the_oracle(gpu_ctx, layout):
+    """hrt_rtao_make_rays against the oracle's generator on the oracle's own primary hits: origins,
+    tNear and tFar bit for bit, directions within 4e-7 (sincosf is the device math library's on one
+    side and the C library's on the other, both within about an ulp)."""
+    import torch
+
+    from oracle import oracle as O
+
+    v, f = W.rtao_scene(128, 64, 150)
+    lo, hi = v.min(0), v.max(0)
+    c = 0.5 * (lo + hi)
+    ext = float(np.max(hi - lo))
+    eye = c + np.array([0.2, 0.35, 1.1]) * ext
+    prim = W.pinhole_rays(64, 40, eye=tuple(eye), target=tuple(c + np.array([0.0, 0.15 * ext, 0.0])),
+                          fov_y_deg=60.0)
+    t, pid, _, _ = O.OracleBVH(v, f).trace(prim)
+    hits = np.zeros((prim.shape[0], 2), dtype=np.uint32)
+    hits[:, 0] = t.view(np.uint32)
+    hits[:, 1] = pid
+    live = pid != 0xFFFFFFFF
+    assert live.any() and not live.all()
+    s = H.Scene(gpu_ctx)
+    s.add_instance(s.add_triangles(v, f))
+    s.commit(layout=layout)
+    stream = torch.cuda.current_stream().cuda_stream
+    npix = prim.shape[0]
+    for spp, seed in ((1, 3), (5, 0xDA3E39CB94B95BDB)):
+        say(f"rtao_make_rays vs oracle layout {layout} spp {spp}")
+        tmax = 0.1 * W.scene_diagonal(v)
+        ref = O.ao_rays(v, f, prim, hits, spp, tmax, seed)
+        dprim = torch.from_numpy(prim).cuda()
+        dhits = torch.from_numpy(hits.view(np.int32)).cuda()
+        out = torch.full((npix * spp, 8), 7.0, dtype=torch.float32, device="cuda")
+        s.rtao_make_rays(dprim, dhits, npix, spp, tmax, seed, out, stream=stream)
+        torch.cuda.synchronize()
+        got = out.cpu().numpy()
+        np.testing.assert_array_equal(got[:, 0:4].view(np.uint32), ref[:, 0:4].view(np.uint32))
+        np.testing.assert_array_equal(got[:, 7].view(np.uint32), ref[:, 7].view(np.uint32))
+        np.testing.assert_allclose(got[:, 4:7], ref[:, 4:7], rtol=0, atol=4e-7)
+        miss = np.repeat(~live, spp)
+        np.testing.assert_array_equal(got[miss], ref[miss])
+    s.close()
