@@ -21,12 +21,21 @@ constexpr float MAXFLOAT_ = 3.402823466e+38f;
 #endif
 constexpr float DEG_TO_RAD = 3.14159265358979323846f / 180.0f;
 constexpr float M_TWOPI_ = 6.28318530717958647692f;
+#ifndef M_TWOPI
+#define M_TWOPI 6.28318530717958647692
+#endif
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+constexpr float INV_PI = 0.31830988618379067154f;
+constexpr float INV_TWOPI = 0.15915494309189533577f;
 
 struct float2 {
   float x, y;
   float2() : x(0), y(0) {}
   float2(float a, float b) : x(a), y(b) {}
   explicit float2(float a) : x(a), y(a) {}
+  explicit float2(const float* p) : x(p[0]), y(p[1]) {}
   float& operator[](int i) { return (&x)[i]; }
   float operator[](int i) const { return (&x)[i]; }
 };
@@ -36,6 +45,7 @@ struct float3 {
   float3() : x(0), y(0), z(0) {}
   float3(float a, float b, float c) : x(a), y(b), z(c) {}
   explicit float3(float a) : x(a), y(a), z(a) {}
+  explicit float3(const float* p) : x(p[0]), y(p[1]), z(p[2]) {}
   float& operator[](int i) { return (&x)[i]; }
   float operator[](int i) const { return (&x)[i]; }
 };
@@ -45,15 +55,36 @@ struct float4 {
   float4() : x(0), y(0), z(0), w(0) {}
   float4(float a, float b, float c, float d) : x(a), y(b), z(c), w(d) {}
   explicit float4(float a) : x(a), y(a), z(a), w(a) {}
+  explicit float4(const float* p) : x(p[0]), y(p[1]), z(p[2]), w(p[3]) {}
   float& operator[](int i) { return (&x)[i]; }
   float operator[](int i) const { return (&x)[i]; }
 };
 
 struct int2 { int x, y; int2() : x(0), y(0) {} int2(int a, int b) : x(a), y(b) {} };
-struct int4 { int x, y, z, w; int4() : x(0), y(0), z(0), w(0) {} int4(int a, int b, int c, int d) : x(a), y(b), z(c), w(d) {} };
+struct int3 { int x, y, z; int3() : x(0), y(0), z(0) {} int3(int a, int b, int c) : x(a), y(b), z(c) {} };
+// int4(float4) truncates toward zero per component, as a C cast does.
+struct int4 {
+  int x, y, z, w;
+  int4() : x(0), y(0), z(0), w(0) {}
+  int4(int a, int b, int c, int d) : x(a), y(b), z(c), w(d) {}
+  explicit int4(float4 f) : x(int(f.x)), y(int(f.y)), z(int(f.z)), w(int(f.w)) {}
+};
 struct uint2 { uint x, y; uint2() : x(0), y(0) {} uint2(uint a, uint b) : x(a), y(b) {} };
 struct uint3 { uint x, y, z; uint3() : x(0), y(0), z(0) {} uint3(uint a, uint b, uint c) : x(a), y(b), z(c) {} };
 struct uint4 { uint x, y, z, w; uint4() : x(0), y(0), z(0), w(0) {} uint4(uint a, uint b, uint c, uint d) : x(a), y(b), z(c), w(d) {} };
+typedef unsigned short ushort;
+typedef unsigned char uchar;
+struct ushort2 { ushort x, y; ushort2() : x(0), y(0) {} ushort2(ushort a, ushort b) : x(a), y(b) {} };
+struct ushort4 {
+  ushort x, y, z, w;
+  ushort4() : x(0), y(0), z(0), w(0) {}
+  ushort4(ushort a, ushort b, ushort c, ushort d) : x(a), y(b), z(c), w(d) {}
+};
+struct uchar4 {
+  uchar x, y, z, w;
+  uchar4() : x(0), y(0), z(0), w(0) {}
+  uchar4(uchar a, uchar b, uchar c, uchar d) : x(a), y(b), z(c), w(d) {}
+};
 
 // --- float3 ---------------------------------------------------------------
 inline float3 operator+(float3 a, float3 b) { return float3(a.x + b.x, a.y + b.y, a.z + b.z); }
@@ -90,11 +121,35 @@ inline float4 min(float4 a, float4 b) {
 inline float4 max(float4 a, float4 b) {
   return float4(std::fmax(a.x, b.x), std::fmax(a.y, b.y), std::fmax(a.z, b.z), std::fmax(a.w, b.w));
 }
+inline float4 operator-(float4 a) { return float4(-a.x, -a.y, -a.z, -a.w); }
+inline float4 operator+(float4 a, float s) { return float4(a.x + s, a.y + s, a.z + s, a.w + s); }
+inline float4 operator-(float4 a, float s) { return float4(a.x - s, a.y - s, a.z - s, a.w - s); }
+inline float4 operator/(float4 a, float4 b) { return float4(a.x / b.x, a.y / b.y, a.z / b.z, a.w / b.w); }
+inline float4& operator+=(float4& a, float4 b) { a = a + b; return a; }
+inline float4& operator-=(float4& a, float4 b) { a = a - b; return a; }
+inline float4& operator*=(float4& a, float s) { a = a * s; return a; }
+inline float length(float4 a) { return std::sqrt(dot(a, a)); }
+inline float4 normalize(float4 a) { return a * (1.0f / length(a)); }
+inline float length3f(float4 a) { return std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+inline float4 abs(float4 a) { return float4(std::fabs(a.x), std::fabs(a.y), std::fabs(a.z), std::fabs(a.w)); }
+inline float3 abs(float3 a) { return float3(std::fabs(a.x), std::fabs(a.y), std::fabs(a.z)); }
+inline float4 clamp(float4 v, float lo, float hi) {
+  return float4(v.x < lo ? lo : (v.x > hi ? hi : v.x), v.y < lo ? lo : (v.y > hi ? hi : v.y),
+                v.z < lo ? lo : (v.z > hi ? hi : v.z), v.w < lo ? lo : (v.w > hi ? hi : v.w));
+}
+inline float3 clamp(float3 v, float lo, float hi) {
+  return float3(v.x < lo ? lo : (v.x > hi ? hi : v.x), v.y < lo ? lo : (v.y > hi ? hi : v.y),
+                v.z < lo ? lo : (v.z > hi ? hi : v.z));
+}
+inline float dot(float a, float b) { return a * b; }
 inline float3 to_float3(float4 a) { return float3(a.x, a.y, a.z); }
 inline float4 to_float4(float3 a, float w) { return float4(a.x, a.y, a.z, w); }
 inline float as_float(uint u) { float f; std::memcpy(&f, &u, 4); return f; }
 inline uint as_uint(float f) { uint u; std::memcpy(&u, &f, 4); return u; }
 inline float clamp(float v, float lo, float hi) { return v < lo ? lo : (v > hi ? hi : v); }
+inline int clamp(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+inline uint clamp(uint v, uint lo, uint hi) { return v < lo ? lo : (v > hi ? hi : v); }
+inline float lerp(float a, float b, float t) { return a + t * (b - a); }
 inline int sign(int v) { return (v > 0) - (v < 0); }
 inline float sign(float v) { return (v > 0.0f) ? 1.0f : ((v < 0.0f) ? -1.0f : 0.0f); }
 
@@ -120,9 +175,18 @@ struct float4x4 {
 inline float4 operator*(const float4x4& m, float4 v) {
   return m.m_col[0] * v.x + m.m_col[1] * v.y + m.m_col[2] * v.z + m.m_col[3] * v.w;
 }
+// A point (w = 1) through the matrix, without a perspective divide.
+inline float3 operator*(const float4x4& m, float3 v) { return to_float3(m * to_float4(v, 1.0f)); }
+inline float4 mul(const float4x4& m, float4 v) { return m * v; }
+inline float3 mul(const float4x4& m, float3 v) { return m * v; }
 inline float4x4 operator*(const float4x4& a, const float4x4& b) {
   float4x4 r;
   for (int c = 0; c < 4; c++) r.m_col[c] = a * b.m_col[c];
+  return r;
+}
+inline float4x4 transpose(const float4x4& m) {
+  float4x4 r;
+  for (int c = 0; c < 4; c++) r.set_row(c, m.get_col(c));
   return r;
 }
 inline float4x4 translate4x4(float3 t) {
@@ -212,7 +276,21 @@ struct Box4f {
     boxMin = min(boxMin, p);
     boxMax = max(boxMax, p);
   }
+  void include(const Box4f& b) {
+    boxMin = min(boxMin, b.boxMin);
+    boxMax = max(boxMax, b.boxMax);
+  }
   float4 center() const { return (boxMin + boxMax) * 0.5f; }
+};
+
+struct BBox3f {
+  float3 boxMin = float3(+MAXFLOAT, +MAXFLOAT, +MAXFLOAT);
+  float3 boxMax = float3(-MAXFLOAT, -MAXFLOAT, -MAXFLOAT);
+  void include(float3 p) {
+    boxMin = min(boxMin, p);
+    boxMax = max(boxMax, p);
+  }
+  float3 center() const { return (boxMin + boxMax) * 0.5f; }
 };
 
 }  // namespace LiteMath
