@@ -395,3 +395,42 @@ def batched_workload(n_scenes: int = 256, tris=(1e3, 2e5), rays=(4096, 262144), 
         out.append((v, f, incoherent_rays(int(nr[s]), stream=1001 + 2 * s)))
     return out
 
+
+
+def serving_scenes(count: int = 16, tris=(5e4, 2e6), stream: int = STREAM_SERVE):
+    """Serving workload: `count` resident scenes with triangle counts
+    log-uniform in `tris`; even ids are random soups, odd ids heightfields of
+    2nz x nz quads with nz = round(sqrt(n / 4)).  Returns [(verts, faces)]."""
+    nt = log_uniform_int(count, tris[0], tris[1], stream)
+    out = []
+    for k in range(count):
+        if k % 2 == 0:
+            out.append(soup(int(nt[k]), stream=stream + 10 + k))
+        else:
+            nz = max(8, int(round(math.sqrt(nt[k] / 4.0))))
+            v, f, _ = heightfield(2 * nz, nz)
+            out.append((v, f))
+    return out
+
+
+def serving_camera(verts: np.ndarray):
+    """Eye point, target and AO radius of a serving scene's 1024x1024 request pool."""
+    lo, hi = verts.min(0), verts.max(0)
+    c = 0.5 * (lo + hi)
+    ext = float(np.max(hi - lo))
+    eye = c + np.array([0.35, 0.8, 1.6]) * ext
+    return tuple(eye), tuple(c), 0.1 * ext
+
+
+def serving_requests(count: int, qps: float, n_scenes: int = 16, pool: int = 1 << 20, sizes=(1e3, 1e6),
+                     stream: int = STREAM_SERVE + 100):
+    """A Poisson request stream: sizes log-uniform in `sizes`, exponential
+    gaps at rate `qps`, scene ids uniform, 50/50 nearest-hit / any-hit, and a
+    start offset into the scene's ray pool of `pool` rays.  Returns a dict of
+    arrays (size, arrival_s, scene, any_hit, start)."""
+    sz = log_uniform_int(count, sizes[0], sizes[1], stream)
+    u = uniform01(pcg32(count, 4, stream + 100))
+    gaps = -np.log(np.maximum(u[:, 0], 1e-12)) / qps
+    return {"size": sz, "arrival_s": np.cumsum(gaps), "scene": np.minimum((u[:, 1] * n_scenes).astype(np.int64),
+                                                                          n_scenes - 1),
+            "any_hit": u[:, 2] < 0.5, "start": (u[:, 3] * (pool - sz)).astype(np.int64)}

@@ -148,15 +148,20 @@ def test_eye_rays_full_config(gpu_ctx):
     scene.close()
 
 
-def test_rtao_config_parity(gpu_ctx):
+@pytest.mark.parametrize("layout", ["bvh2", "auto_fp16"])
+def test_rtao_config_parity(gpu_ctx, layout):
     """1M-triangle heightfield + 2,000 boxes; 1024x1024 primaries (nearest),
-    16 AO rays per hit made on the device, any-hit, all compared exactly."""
+    16 AO rays per hit made on the device, any-hit, all compared exactly; on
+    the reference tree's BVH2 and on the timed layout (nearest on BVH2, any-hit
+    on the FP16 BVH4 nodes of the any-hit tree)."""
     import torch
 
     v, f = W.rtao_scene()
-    say(f"rtao build {f.shape[0]} tris")
+    say(f"rtao build {f.shape[0]} tris, layout {layout}")
     ob = O.OracleBVH(v, f)
-    scene = make_scene(gpu_ctx, v, f)
+    scene = H.Scene(gpu_ctx, keep_build_arrays=True)
+    scene.add_instance(scene.add_triangles(v, f))
+    scene.commit(layout={"bvh2": H.LAYOUT_BVH2, "auto_fp16": H.LAYOUT_AUTO_FP16}[layout])
     check_build(scene, ob)
     prim = W.rtao_primary_rays()
     npix = prim.shape[0]
