@@ -29,7 +29,6 @@ LAYOUT_BVH4 = 1
 LAYOUT_BVH4_FP16 = 2
 LAYOUT_AUTO = 3
 LAYOUT_AUTO_FP16 = 4
-LAYOUT_AUTO_FP16_WIDE = 5
 
 DUMP_SIZES = 0
 DUMP_CODES_INPUT = 1
@@ -43,7 +42,6 @@ DUMP_SCENE_BOX = 8
 DUMP_QUADS32 = 9
 DUMP_QUADS16 = 10
 DUMP_PAIRS = 11
-DUMP_OCTS16 = 12
 DUMP_ANY_TREE = 0x100  # OR'ed into a dump kind: the any-hit tree of HRT_LAYOUT_AUTO_FP16
 # hrt_debug_dump geometry id that selects the top-level tree of an instanced scene
 TLAS_ID = 0xFFFFFFFF
@@ -371,7 +369,7 @@ class Scene:
     def dump(self, geom_id: int, what: int) -> np.ndarray:
         flag = what & DUMP_ANY_TREE
         what &= ~DUMP_ANY_TREE
-        sizes = np.zeros(7, dtype=np.uint32)
+        sizes = np.zeros(6, dtype=np.uint32)
         _check(lib().hrt_debug_dump(self._h, geom_id, DUMP_SIZES | flag, _ptr(sizes), sizes.nbytes))
         if what == DUMP_SIZES:
             return sizes
@@ -380,8 +378,6 @@ class Scene:
             out = np.zeros((nq, 32), dtype=np.uint32)
         elif what == DUMP_QUADS16:
             out = np.zeros((nq, 16), dtype=np.uint32)
-        elif what == DUMP_OCTS16:
-            out = np.zeros((int(sizes[6]), 32), dtype=np.uint32)
         elif what == DUMP_SCENE_BOX:
             out = np.zeros(12, dtype=np.float32)
         elif what == DUMP_PAIRS:

@@ -236,14 +236,12 @@ hrt_status trace_common(hrt_scene* s, const float* rays, size_t n, void* out, in
   hipError_t e = hipSetDevice(s->ctx->device);
   if (e == hipSuccess) {
     const int lay = s->layout;
-    const bool auto16 = lay == HRT_LAYOUT_AUTO_FP16 || lay == HRT_LAYOUT_AUTO_FP16_WIDE;
+    const bool auto16 = lay == HRT_LAYOUT_AUTO_FP16;
     // b is null for a two-level scene: every branch below it reads b
     if (s->two_level)
       e = hrt::trace_two_level(s->tlas, s->inst_dev ? s->inst_dev + s->instances.size() : nullptr, a, hs);
     else if (lay == HRT_LAYOUT_BVH2 || ((lay == HRT_LAYOUT_AUTO || auto16) && !anyhit))
       e = hrt::trace_blas(*b, a, hs);
-    else if (lay == HRT_LAYOUT_AUTO_FP16_WIDE && b->any_tree && b->any_tree->octs16)
-      e = hrt::trace_blas8(*b->any_tree, a, hs);  // any-hit on the any-hit tree's 8-wide nodes
     else if (auto16 && b->any_tree)
       e = hrt::trace_blas4(*b->any_tree, a, true, hs);  // any-hit on the any-hit tree
     else
@@ -477,7 +475,7 @@ hrt_status hrt_update_instance(hrt_scene* s, uint32_t instId, const float colMaj
 hrt_status hrt_commit(hrt_scene* s, int quality, int layout) {
   (void)quality;
   if (!s) return fail(HRT_ERR_INVALID_ARG, "scene is null");
-  if (layout < HRT_LAYOUT_BVH2 || layout > HRT_LAYOUT_AUTO_FP16_WIDE)
+  if (layout < HRT_LAYOUT_BVH2 || layout > HRT_LAYOUT_AUTO_FP16)
     return fail(HRT_ERR_INVALID_ARG, "unknown layout");
   s->committed = false;
   free_tlas(s);
@@ -494,7 +492,7 @@ hrt_status hrt_commit(hrt_scene* s, int quality, int layout) {
   } else if (layout != HRT_LAYOUT_BVH2) {
     for (const Instance& in : s->instances) {
       hrt::Blas& b = s->geoms[in.geom];
-      const bool auto16 = layout == HRT_LAYOUT_AUTO_FP16 || layout == HRT_LAYOUT_AUTO_FP16_WIDE;
+      const bool auto16 = layout == HRT_LAYOUT_AUTO_FP16;
       if (layout == HRT_LAYOUT_BVH4_FP16 || auto16) {
         // FP16 boxes are only conservative inside the half range (cbvh_fat.cpp:447-448
         // maps every value beyond +-65504 to +-65504)
@@ -628,15 +626,14 @@ namespace {
 // The AO pass of one frame: AO rays of every primary hit, traced for any hit
 // on the tree any-hit rays use in this layout (as trace_common chooses it).
 // BVH4 layouts make each ray inside the traversal from a per-pixel frame;
-// BVH2 and 8-wide nodes go through a temporary ray buffer.
+// BVH2 goes through a temporary ray buffer.
 hipError_t rtao_pass(hrt_scene* s, const hrt::Blas* b, uint32_t inst, const hrt::Ray* prim, const void* hits,
                      uint64_t npix, uint32_t spp, float tmax, uint64_t seed, uint8_t* out, hipStream_t hs) {
   const uint64_t n = npix * spp;
   const int lay = s->layout;
-  const bool auto16 = lay == HRT_LAYOUT_AUTO_FP16 || lay == HRT_LAYOUT_AUTO_FP16_WIDE;
-  const bool wide = lay == HRT_LAYOUT_AUTO_FP16_WIDE && b->any_tree && b->any_tree->octs16;
+  const bool auto16 = lay == HRT_LAYOUT_AUTO_FP16;
   hipError_t e = hipSuccess;
-  if (lay == HRT_LAYOUT_BVH2 || wide) {
+  if (lay == HRT_LAYOUT_BVH2) {
     void* rays = nullptr;
     if ((e = hipMallocAsync(&rays, sizeof(hrt::Ray) * n, hs)) != hipSuccess) return e;
     hrt::TraceArgs a;
@@ -649,7 +646,7 @@ hipError_t rtao_pass(hrt_scene* s, const hrt::Blas* b, uint32_t inst, const hrt:
     a.inst_id = inst;
     a.geom_id = s->instances[inst].geom;
     e = hrt::rtao_make_rays(*b, prim, hits, npix, spp, tmax, seed, static_cast<hrt::Ray*>(rays), hs);
-    if (e == hipSuccess) e = wide ? hrt::trace_blas8(*b->any_tree, a, hs) : hrt::trace_blas(*b, a, hs);
+    if (e == hipSuccess) e = hrt::trace_blas(*b, a, hs);
     const hipError_t fe = hipFreeAsync(rays, hs);
     return e == hipSuccess ? fe : e;
   }
@@ -789,11 +786,11 @@ hrt_status hrt_rtao_render(hrt_scene* s, const float projInvColMajor16[16], cons
 //   "HRTSCN01", u32 version, i32 layout, u32 geometries, u32 instances,
 //   per geometry a tree record, per instance u32 geometry + 16 floats
 //   (column-major), u64 hash of everything before it (snap_hash).
-// A tree record: u32 nverts, ntris, leaves, max_run, nquads, nocts,
+// A tree record: u32 nverts, ntris, leaves, max_run, nquads, reserved (0),
 //   clustered, has_any_tree; float box[8], zscale[4]; then the device arrays
 //   at their allocated sizes (vertices, indices, pairs, triangles,
 //   permutation, parents, leaf runs, leaf parents, node ranges, meta, BVH4
-//   FP32 and FP16 nodes, BVH8 nodes); then the any-hit tree's record if any.
+//   FP32 and FP16 nodes); then the any-hit tree's record if any.
 // Loading uploads the arrays and commits: no LBVH, BVH4 or any-hit tree is
 // rebuilt; an instanced scene's top level is (it is small).
 // ---------------------------------------------------------------------------
@@ -869,8 +866,7 @@ size_t tree_bytes(const hrt::Blas& b) {
   const size_t np = npairs_of(b), nt = b.ntris;
   size_t n = 32 + sizeof(b.box) + sizeof(b.zscale);
   n += 16ull * b.nverts + 12 * nt + sizeof(hrt::NodePair) * np + sizeof(hrt::TriPre) * nt + 4 * nt + 4 * np +
-       4 * nt + 4 * nt + 8 * np + 256 + (sizeof(hrt::Quad32) + sizeof(hrt::Quad16)) * size_t(b.nquads) +
-       sizeof(hrt::Oct16) * size_t(b.nocts);
+       4 * nt + 4 * nt + 8 * np + 256 + (sizeof(hrt::Quad32) + sizeof(hrt::Quad16)) * size_t(b.nquads);
   return n + (b.any_tree ? tree_bytes(*b.any_tree) : 0);
 }
 
@@ -881,7 +877,7 @@ size_t snapshot_bytes(const hrt_scene* s) {
 }
 
 hipError_t write_tree(SnapWriter& w, const hrt::Blas& b) {
-  const uint32_t head[8] = {b.nverts, b.ntris, b.leaves, b.max_run, b.nquads, b.nocts, b.clustered ? 1u : 0u,
+  const uint32_t head[8] = {b.nverts, b.ntris, b.leaves, b.max_run, b.nquads, 0u, b.clustered ? 1u : 0u,
                             b.any_tree ? 1u : 0u};
   w.put(head, sizeof(head));
   w.put(b.box, sizeof(b.box));
@@ -892,23 +888,149 @@ hipError_t write_tree(SnapWriter& w, const hrt::Blas& b) {
       {b.verts, 16ull * b.nverts}, {b.idx, 12ull * nt}, {b.pairs, sizeof(hrt::NodePair) * np},
       {b.tris, sizeof(hrt::TriPre) * nt}, {b.perm, 4ull * nt}, {b.parent, 4ull * np},
       {b.leaf_run, 4ull * nt}, {b.parent_leaf, 4ull * nt}, {b.node_range, 8ull * np}, {b.meta, 256},
-      {b.quads32, sizeof(hrt::Quad32) * b.nquads}, {b.quads16, sizeof(hrt::Quad16) * b.nquads},
-      {b.octs16, sizeof(hrt::Oct16) * b.nocts}};
+      {b.quads32, sizeof(hrt::Quad32) * b.nquads}, {b.quads16, sizeof(hrt::Quad16) * b.nquads}};
   for (const auto& a : arrays)
     if ((e = w.dev(a.first, a.second)) != hipSuccess) return e;
   return b.any_tree ? write_tree(w, *b.any_tree) : hipSuccess;
 }
 
-// Reads one tree record into b (allocated here).  Returns HRT_OK or an error
-// status with the message set.
+// Host views of one tree record's arrays inside the snapshot bytes.
+struct TreeSpans {
+  const uint32_t* idx;
+  const hrt::NodePair* pairs;
+  const hrt::TriPre* tris;
+  const uint32_t* perm;
+  const uint32_t* parent;
+  const uint32_t* leaf_run;
+  const uint32_t* parent_leaf;
+  const uint2* node_range;
+  const uint32_t* meta;
+  const hrt::Quad32* quads32;
+  const hrt::Quad16* quads16;
+};
+
+// Child references of one tree level form a tree: every internal reference
+// below `count`, no node but the root (0) referenced, every other node
+// referenced exactly once, every leaf inside the triangle array, and no
+// internal node deeper than `max_depth` (the traversal stacks are sized for
+// that depth).  refs(k, out) writes node k's `fan` child references.
+template <class Refs>
+const char* check_tree(uint32_t count, int fan, uint32_t ntris, uint32_t max_depth, Refs refs) {
+  std::vector<uint32_t> depth(count, 0xFFFFFFFFu);
+  std::vector<uint32_t> order;
+  order.reserve(count);
+  depth[0] = 0;
+  order.push_back(0);
+  uint32_t ch[4];
+  for (size_t at = 0; at < order.size(); at++) {
+    const uint32_t k = order[at];
+    refs(k, ch);
+    for (int i = 0; i < fan; i++) {
+      const uint32_t r = ch[i];
+      if (r & hrt::kLeafBit) {
+        if (uint64_t(hrt::leaf_start(r)) + hrt::leaf_count(r) > ntris) return "a leaf reaches past the triangles";
+        continue;
+      }
+      if (r == 0 || r >= count) return "a child reference is out of range";
+      if (depth[r] != 0xFFFFFFFFu) return "a node is referenced twice";
+      depth[r] = depth[k] + 1;
+      if (depth[r] > max_depth) return "the tree is deeper than the traversal stack allows";
+      order.push_back(r);
+    }
+  }
+  return nullptr;
+}
+
+// Every index and reference a kernel follows stays inside its array, and the
+// trees are trees of bounded depth, so a corrupted or crafted snapshot cannot
+// make a kernel read or write out of bounds or loop.
+const char* check_tree_record(const uint32_t head[8], const TreeSpans& t, bool any_tree) {
+  const uint32_t nverts = head[0], ntris = head[1], leaves = head[2], max_run = head[3], nquads = head[4];
+  if (max_run == 0 || max_run > 127) return "bad longest leaf";
+  if (t.meta[0] != leaves || t.meta[1] != max_run) return "meta words disagree with the header";
+  for (size_t i = 0; i < 3ull * ntris; i++)
+    if (t.idx[i] >= nverts) return "an index reaches past the vertices";
+  std::vector<uint8_t> seen(ntris, 0);
+  for (uint32_t i = 0; i < ntris; i++) {
+    const uint32_t p = t.perm[i];
+    if (p >= ntris || seen[p]) return "the leaf permutation is not a permutation";
+    seen[p] = 1;
+    if (__builtin_bit_cast(uint32_t, t.tris[i].a.w) >= ntris) return "a triangle id is out of range";
+  }
+  const uint32_t np = leaves > 1 ? leaves - 1 : 1;
+  for (uint32_t k = 0; k < np; k++) {
+    if (k > 0 && (t.parent[k] >> 1) >= np) return "a parent reference is out of range";
+    if (leaves > 1 && (t.node_range[k].x > t.node_range[k].y || t.node_range[k].y >= leaves))
+      return "a node range is out of range";
+  }
+  for (uint32_t i = 0; i < leaves; i++)  // a one-leaf tree has no parent records
+    if (t.leaf_run[i] >= ntris || (leaves > 1 && (t.parent_leaf[i] >> 1) >= np)) return "a leaf record is out of range";
+  // node pairs: the BVH2 kernels keep kStackDepth pending entries, at most one per level
+  const char* err = check_tree(np, 2, ntris, any_tree ? 0xFFFFFFFFu : uint32_t(hrt::kStackDepth - 1),
+                               [&](uint32_t k, uint32_t* c) {
+                                 c[0] = __builtin_bit_cast(uint32_t, t.pairs[k].lmin.w);
+                                 c[1] = __builtin_bit_cast(uint32_t, t.pairs[k].rmin.w);
+                               });
+  if (err) return err;
+  if (nquads) {
+    // BVH4 nodes: at most three pending entries per level in kStackDepth4
+    const uint32_t d4 = uint32_t(hrt::kStackDepth4 / 3 - 1);
+    if ((err = check_tree(nquads, 4, ntris, d4, [&](uint32_t k, uint32_t* c) {
+           c[0] = t.quads32[k].ref.x;
+           c[1] = t.quads32[k].ref.y;
+           c[2] = t.quads32[k].ref.z;
+           c[3] = t.quads32[k].ref.w;
+         })))
+      return err;
+    for (uint32_t k = 0; k < nquads; k++)
+      for (int i = 0; i < 4; i++)
+        if (t.quads16[k].c[i].w != (&t.quads32[k].ref.x)[i]) return "FP16 and FP32 nodes disagree";
+  }
+  return nullptr;
+}
+
+// Reads one tree record into b (allocated here).  The record is checked on
+// the host before anything is uploaded.  Returns HRT_OK or an error status
+// with the message set.
 hrt_status read_tree(SnapReader& r, hrt::Blas& b, int depth) {
   uint32_t head[8];
   float box[8], zs[4];
   if (!r.val(head) || !r.val(box) || !r.val(zs)) return fail(HRT_ERR_INVALID_ARG, "snapshot is truncated");
   const uint32_t nverts = head[0], ntris = head[1];
-  if (nverts == 0 || ntris == 0 || ntris > 0x00FFFFFFu || head[2] == 0 || head[2] > ntris || head[7] > 1 ||
-      (head[7] && depth > 0))
+  if (nverts == 0 || ntris == 0 || ntris > 0x00FFFFFFu || head[2] == 0 || head[2] > ntris || head[5] != 0 ||
+      head[7] > 1 || (head[7] && depth > 0))
     return fail(HRT_ERR_INVALID_ARG, "snapshot holds an invalid tree record");
+  const size_t np = std::max<size_t>(ntris > 1 ? ntris - 1 : 1, 1), nt = ntris, nq = head[4];
+  const size_t sizes[] = {16ull * nverts, 12ull * nt, sizeof(hrt::NodePair) * np, sizeof(hrt::TriPre) * nt,
+                          4ull * nt, 4ull * np, 4ull * nt, 4ull * nt, 8ull * np, 256,
+                          sizeof(hrt::Quad32) * nq, sizeof(hrt::Quad16) * nq};
+  constexpr int kArrays = sizeof(sizes) / sizeof(sizes[0]);
+  const uint8_t* at[kArrays];
+  size_t off = r.at;
+  for (int i = 0; i < kArrays; i++) {
+    if (sizes[i] > r.n - off) return fail(HRT_ERR_INVALID_ARG, "snapshot is truncated");
+    at[i] = r.p + off;
+    off += sizes[i];
+  }
+  // the arrays are read in place; records are 4-byte aligned in the buffer
+  // when it is, and copied otherwise
+  std::vector<uint8_t> copy;
+  if ((reinterpret_cast<uintptr_t>(r.p + r.at) & 15u) != 0) {
+    copy.assign(r.p + r.at, r.p + off);
+    size_t o = 0;
+    for (int i = 0; i < kArrays; i++) {
+      at[i] = copy.data() + o;
+      o += sizes[i];
+    }
+  }
+  const TreeSpans spans{reinterpret_cast<const uint32_t*>(at[1]),      reinterpret_cast<const hrt::NodePair*>(at[2]),
+                        reinterpret_cast<const hrt::TriPre*>(at[3]),   reinterpret_cast<const uint32_t*>(at[4]),
+                        reinterpret_cast<const uint32_t*>(at[5]),      reinterpret_cast<const uint32_t*>(at[6]),
+                        reinterpret_cast<const uint32_t*>(at[7]),      reinterpret_cast<const uint2*>(at[8]),
+                        reinterpret_cast<const uint32_t*>(at[9]),      reinterpret_cast<const hrt::Quad32*>(at[10]),
+                        reinterpret_cast<const hrt::Quad16*>(at[11])};
+  if (const char* bad = check_tree_record(head, spans, depth > 0))
+    return fail(HRT_ERR_INVALID_ARG, std::string("snapshot holds an invalid tree record: ") + bad);
   hipError_t e = hrt::blas_alloc(b, nverts, ntris, false);
   if (e != hipSuccess) return fail_hip(e, "snapshot allocation");
   b.leaves = head[2];
@@ -916,23 +1038,16 @@ hrt_status read_tree(SnapReader& r, hrt::Blas& b, int depth) {
   b.clustered = head[6] != 0;
   std::memcpy(b.box, box, sizeof(b.box));
   std::memcpy(b.zscale, zs, sizeof(b.zscale));
-  if (head[4] && ((e = hipMalloc(&b.quads32, sizeof(hrt::Quad32) * head[4])) != hipSuccess ||
-                  (e = hipMalloc(&b.quads16, sizeof(hrt::Quad16) * head[4])) != hipSuccess))
+  if (nq && ((e = hipMalloc(&b.quads32, sizeof(hrt::Quad32) * nq)) != hipSuccess ||
+             (e = hipMalloc(&b.quads16, sizeof(hrt::Quad16) * nq)) != hipSuccess))
     return fail_hip(e, "snapshot allocation");
   b.nquads = head[4];
-  if (head[5] && (e = hipMalloc(&b.octs16, sizeof(hrt::Oct16) * head[5])) != hipSuccess)
-    return fail_hip(e, "snapshot allocation");
-  b.nocts = head[5];
-  const size_t np = npairs_of(b), nt = ntris;
-  const std::pair<void*, size_t> arrays[] = {
-      {b.verts, 16ull * nverts}, {b.idx, 12ull * nt}, {b.pairs, sizeof(hrt::NodePair) * np},
-      {b.tris, sizeof(hrt::TriPre) * nt}, {b.perm, 4ull * nt}, {b.parent, 4ull * np},
-      {b.leaf_run, 4ull * nt}, {b.parent_leaf, 4ull * nt}, {b.node_range, 8ull * np}, {b.meta, 256},
-      {b.quads32, sizeof(hrt::Quad32) * b.nquads}, {b.quads16, sizeof(hrt::Quad16) * b.nquads},
-      {b.octs16, sizeof(hrt::Oct16) * b.nocts}};
-  for (const auto& a : arrays)
-    if ((e = r.dev(a.first, a.second)) != hipSuccess)
-      return e == hipErrorInvalidValue ? fail(HRT_ERR_INVALID_ARG, "snapshot is truncated") : fail_hip(e, "snapshot upload");
+  void* dst[kArrays] = {b.verts, b.idx,         b.pairs,      b.tris, b.perm,    b.parent,
+                        b.leaf_run, b.parent_leaf, b.node_range, b.meta, b.quads32, b.quads16};
+  for (int i = 0; i < kArrays; i++)
+    if (sizes[i] && (e = hipMemcpy(dst[i], at[i], sizes[i], hipMemcpyHostToDevice)) != hipSuccess)
+      return fail_hip(e, "snapshot upload");
+  r.at = off;
   if (head[7]) {
     b.any_tree = new (std::nothrow) hrt::Blas();
     if (!b.any_tree) return fail(HRT_ERR_OUT_OF_MEMORY, "host allocation failed");
@@ -1028,7 +1143,7 @@ hrt_status hrt_debug_dump(hrt_scene* s, uint32_t geomId, int what, void* host_ou
   const bool any_tree = (what & HRT_DUMP_ANY_TREE) != 0;
   what &= ~HRT_DUMP_ANY_TREE;
   if (any_tree && (geomId == HRT_TLAS_ID || !s->geoms[geomId].any_tree))
-    return fail(HRT_ERR_INVALID_ARG, "no any-hit tree (commit with HRT_LAYOUT_AUTO_FP16 or _WIDE)");
+    return fail(HRT_ERR_INVALID_ARG, "no any-hit tree (commit with HRT_LAYOUT_AUTO_FP16)");
   const hrt::Blas& b = (geomId == HRT_TLAS_ID) ? s->tlas : (any_tree ? *s->geoms[geomId].any_tree : s->geoms[geomId]);
   const uint32_t N = b.ntris, L = b.leaves;
   const uint32_t npairs = std::max<uint32_t>(L > 1 ? L - 1 : 1, 1);
@@ -1040,8 +1155,8 @@ hrt_status hrt_debug_dump(hrt_scene* s, uint32_t geomId, int what, void* host_ou
   if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
   if (what == HRT_DUMP_SIZES) {
     if (need(20)) return HRT_ERR_INVALID_ARG;
-    uint32_t v[7] = {N, L, b.max_run, npairs, b.nquads, b.clustered ? 1u : 0u, b.nocts};
-    std::memcpy(host_out, v, bytes >= 28 ? 28 : (bytes >= 24 ? 24 : 20));
+    uint32_t v[6] = {N, L, b.max_run, npairs, b.nquads, b.clustered ? 1u : 0u};
+    std::memcpy(host_out, v, bytes >= 24 ? 24 : 20);
     return HRT_OK;
   }
   if (what == HRT_DUMP_SCENE_BOX) {
@@ -1056,13 +1171,6 @@ hrt_status hrt_debug_dump(hrt_scene* s, uint32_t geomId, int what, void* host_ou
     if (need(n)) return HRT_ERR_INVALID_ARG;
     e = hipMemcpy(host_out, what == HRT_DUMP_QUADS32 ? static_cast<const void*>(b.quads32) : b.quads16, n,
                   hipMemcpyDeviceToHost);
-    return e == hipSuccess ? HRT_OK : fail_hip(e, "dump");
-  }
-  if (what == HRT_DUMP_OCTS16) {
-    if (!b.octs16) return fail(HRT_ERR_INVALID_ARG, "no BVH8 nodes (commit with HRT_LAYOUT_AUTO_FP16_WIDE)");
-    const size_t n = sizeof(hrt::Oct16) * size_t(b.nocts);
-    if (need(n)) return HRT_ERR_INVALID_ARG;
-    e = hipMemcpy(host_out, b.octs16, n, hipMemcpyDeviceToHost);
     return e == hipSuccess ? HRT_OK : fail_hip(e, "dump");
   }
   if (what == HRT_DUMP_PAIRS) {

@@ -147,18 +147,6 @@ struct __attribute__((aligned(64))) Quad16 {
 // most 30 internal levels), at most 3 deferred children per level.
 constexpr int kStackDepth4 = 48;
 
-// BVH8 node with FP16 boxes, 128 bytes: eight children in Quad16's per-child
-// format.  A node stands for a binary node whose depth is a multiple of three;
-// its children are that node's descendants three levels down (a leaf met
-// earlier is taken as it is), left to right; unused slots as in Quad32.
-struct __attribute__((aligned(128))) Oct16 {
-  uint4 c[8];
-};
-
-// Deepest BVH8 path: at most 11 node levels (depths 0, 3, ..., 30 of a binary
-// tree of at most 32 internal levels), at most 7 deferred children per level.
-constexpr int kStackDepth8 = 80;
-
 struct Ray {  // 32-byte ray record: posAndNear, dirAndFar (CrossRT.h:143)
   float4 pos_near;
   float4 dir_far;
@@ -318,7 +306,7 @@ __device__ inline float h2f_hi(uint32_t w) { return float(__builtin_bit_cast(_Fl
 
 // The slab test in the fused form t = fma(b, inv, -(o * inv)) on one FP16
 // child record {minx | miny << 16, minz | maxx << 16, maxy | maxz << 16, ref}
-// of a BVH4/BVH8 node.  One rounding per plane instead of two, and each FP16
+// of a BVH4 node.  One rounding per plane instead of two, and each FP16
 // bound is a direct operand of v_fma_mix_f32 (widening FP16 to FP32 is
 // exact), so no conversion either.
 //

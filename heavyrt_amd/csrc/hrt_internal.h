@@ -51,9 +51,6 @@ struct Blas {
   uint32_t nquads = 0;
   Quad32* quads32 = nullptr;
   Quad16* quads16 = nullptr;
-  // BVH8 collapse with FP16 boxes (made on demand by blas_build_bvh8)
-  uint32_t nocts = 0;
-  Oct16* octs16 = nullptr;
   // parity hooks (kept only when requested)
   uint32_t* codes_in = nullptr;    // Morton code per input triangle
   uint32_t* codes_sorted = nullptr;
@@ -104,9 +101,6 @@ hipError_t blas_ploc(Blas& b, hipStream_t stream, bool* replaced);
 // Collapses the built BVH2 into FP32 and FP16 BVH4 nodes (idempotent);
 // synchronises the stream.
 hipError_t blas_build_bvh4(Blas& b, hipStream_t stream);
-// BVH8 nodes with FP16 boxes (the any-hit tree's wide layout) unless they
-// exist.  Synchronises the stream.
-hipError_t blas_build_bvh8(Blas& b, hipStream_t stream);
 
 // Top-level LBVH over instance boxes.  `t` is allocated with blas_alloc(t,
 // 2 * ninst, ninst, ...) and t.verts holds the world boxes (min, max per
@@ -145,8 +139,6 @@ hipError_t trace_blas(const Blas& b, const TraceArgs& a, hipStream_t stream);
 
 // BVH4 traversal; fp16 selects the FP16 nodes.  Nearest hit by (t, primId).
 hipError_t trace_blas4(const Blas& b, const TraceArgs& a, bool fp16, hipStream_t stream);
-// BVH8 traversal on b.octs16 (FP16 boxes), nearest by (t, primId) or any-hit.
-hipError_t trace_blas8(const Blas& b, const TraceArgs& a, hipStream_t stream);
 
 // Two-level traversal: the TLAS (`tlas`, built by tlas_build) with instance
 // records in leaf order, each instance's BVH2 in object space.  Full hit

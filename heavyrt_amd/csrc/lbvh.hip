@@ -507,62 +507,6 @@ __global__ __launch_bounds__(kBlock) void k_collapse4(const NodePair* __restrict
   q16[q] = h;
 }
 
-// flag[k] = 1 when internal node k's depth is a multiple of three (it
-// becomes a BVH8 node)
-__global__ __launch_bounds__(kBlock) void k_depth_mod3(const uint32_t* __restrict__ parent_int, uint32_t ninternal,
-                                                       uint32_t* __restrict__ flag) {
-  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-  if (k >= ninternal) return;
-  uint32_t d = 0, p = k;
-  while (p != 0 && d < 64u) {  // at most 32 levels: checked when the tree is built
-    p = parent_int[p] >> 1;
-    d++;
-  }
-  flag[k] = (d % 3u == 0u) ? 1u : 0u;
-}
-
-// BVH8 node of internal node k: its descendants three levels down, a leaf
-// met earlier taken as it is, left to right; FP16 boxes rounded outward.
-__global__ __launch_bounds__(kBlock) void k_collapse8(const NodePair* __restrict__ pairs,
-                                                      const uint32_t* __restrict__ flag,
-                                                      const uint32_t* __restrict__ oidx, uint32_t ninternal,
-                                                      Oct16* __restrict__ out) {
-  const uint32_t k = blockIdx.x * kBlock + threadIdx.x;
-  const bool single = (ninternal == 0);  // the root is a leaf: one node from pair 0
-  if (single ? k != 0 : (k >= ninternal || !flag[k])) return;
-  Child c[8];
-  int n = 0;
-  for (uint32_t s1 = 0; s1 < 2; s1++) {
-    const Child a = pair_child(pairs, k, s1);
-    if (single || (a.ref & kLeafBit) != 0) {
-      c[n++] = a;
-      continue;
-    }
-    for (uint32_t s2 = 0; s2 < 2; s2++) {
-      const Child b = pair_child(pairs, a.ref, s2);
-      if ((b.ref & kLeafBit) != 0) {
-        c[n++] = b;
-        continue;
-      }
-      c[n++] = pair_child(pairs, b.ref, 0);
-      c[n++] = pair_child(pairs, b.ref, 1);
-    }
-  }
-  for (; n < 8; n++) {
-    c[n].lo = make_float3(1e9f, 1e9f, 1e9f);
-    c[n].hi = make_float3(-1e9f, -1e9f, -1e9f);
-    c[n].ref = pack_leaf(0, 0);
-  }
-  Oct16 o;
-  for (int i = 0; i < 8; i++) {
-    const uint32_t ref = (c[i].ref & kLeafBit) ? c[i].ref : oidx[c[i].ref];
-    o.c[i] = make_uint4(round_to_neg(c[i].lo.x) | (round_to_neg(c[i].lo.y) << 16),
-                        round_to_neg(c[i].lo.z) | (round_to_pos(c[i].hi.x) << 16),
-                        round_to_pos(c[i].hi.y) | (round_to_pos(c[i].hi.z) << 16), ref);
-  }
-  out[single ? 0u : oidx[k]] = o;
-}
-
 inline uint32_t blocks_for(uint64_t n) { return uint32_t((n + kBlock - 1) / kBlock); }
 
 }  // namespace
@@ -601,7 +545,7 @@ void blas_free(Blas& b) {
   }
   void* ptrs[] = {b.leaf_run,   b.parent_leaf, b.node_range, b.meta,
                   b.verts,      b.idx,        b.pairs,      b.tris,      b.perm,   b.parent,
-                  b.quads32,    b.quads16,    b.octs16,     b.codes_in,   b.codes_sorted, b.ids_sorted,
+                  b.quads32,    b.quads16,    b.codes_in,   b.codes_sorted, b.ids_sorted,
                   b.run_start,  b.ucodes,     b.karras_lr};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
@@ -761,12 +705,9 @@ hipError_t blas_refit(Blas& b, hipStream_t s) {
   // boxes changed: the BVH4 nodes and the any-hit tree are made again on the next commit
   if (b.quads32) (void)hipFree(b.quads32);
   if (b.quads16) (void)hipFree(b.quads16);
-  if (b.octs16) (void)hipFree(b.octs16);
   b.quads32 = nullptr;
   b.quads16 = nullptr;
-  b.octs16 = nullptr;
   b.nquads = 0;
-  b.nocts = 0;
   if (b.any_tree) {
     blas_free(*b.any_tree);
     delete b.any_tree;
@@ -788,7 +729,6 @@ hipError_t blas_build_any_tree(Blas& b, hipStream_t s) {
   bool replaced = false;
   if (e == hipSuccess && a->max_run <= 127) e = blas_ploc(*a, s, &replaced);
   if (e == hipSuccess && a->max_run <= 127) e = blas_build_bvh4(*a, s);
-  if (e == hipSuccess && a->max_run <= 127) e = blas_build_bvh8(*a, s);
   if (e != hipSuccess || a->max_run > 127) {
     blas_free(*a);
     delete a;
@@ -842,43 +782,6 @@ hipError_t blas_build_bvh4(Blas& b, hipStream_t s) {
     b.quads32 = nullptr;
     b.quads16 = nullptr;
     b.nquads = 0;
-  }
-  return err;
-}
-
-hipError_t blas_build_bvh8(Blas& b, hipStream_t s) {
-  if (b.octs16) return hipSuccess;
-  const uint32_t ninternal = b.leaves > 1 ? b.leaves - 1 : 0;
-  const size_t nflag = std::max<uint32_t>(ninternal, 1);
-  const size_t w_scan = scan_temp_words(nflag);
-  uint32_t* base = nullptr;
-  HRT_CHECK(hipMallocAsync(reinterpret_cast<void**>(&base), 4 * (2 * nflag + w_scan + 64), s));
-  uint32_t *flag = base, *oidx = base + nflag, *stmp = base + 2 * nflag;
-  hipError_t err = hipSuccess;
-  do {
-    uint32_t no = 1;
-    if (ninternal > 0) {
-      k_depth_mod3<<<blocks_for(ninternal), kBlock, 0, s>>>(b.parent, ninternal, flag);
-      if ((err = exclusive_scan(flag, oidx, ninternal, stmp, s)) != hipSuccess) break;
-      uint32_t last[2];
-      if ((err = hipMemcpyAsync(&last[0], oidx + ninternal - 1, 4, hipMemcpyDeviceToHost, s)) != hipSuccess) break;
-      if ((err = hipMemcpyAsync(&last[1], flag + ninternal - 1, 4, hipMemcpyDeviceToHost, s)) != hipSuccess) break;
-      if ((err = hipStreamSynchronize(s)) != hipSuccess) break;
-      no = last[0] + last[1];
-    }
-    if ((err = hipMalloc(&b.octs16, sizeof(Oct16) * no)) != hipSuccess) break;
-    k_collapse8<<<blocks_for(std::max<uint32_t>(ninternal, 1)), kBlock, 0, s>>>(b.pairs, flag, oidx, ninternal,
-                                                                               b.octs16);
-    if ((err = hipGetLastError()) != hipSuccess) break;
-    b.nocts = no;
-  } while (false);
-  hipError_t ferr = hipFreeAsync(base, s);
-  if (err == hipSuccess) err = ferr;
-  if (err == hipSuccess) err = hipStreamSynchronize(s);
-  if (err != hipSuccess) {
-    if (b.octs16) (void)hipFree(b.octs16);
-    b.octs16 = nullptr;
-    b.nocts = 0;
   }
   return err;
 }

@@ -1,5 +1,4 @@
-// trace4.hip -- BVH4 nearest-hit / any-hit traversal (FP32 or FP16 boxes),
-// and the same on the BVH8 nodes (FP16 boxes) of the any-hit tree.
+// trace4.hip -- BVH4 nearest-hit / any-hit traversal (FP32 or FP16 boxes).
 //
 // Visiting order follows BVH4HalfRT::BVH4Traverse (core/BVH4HalfRT.cpp:73-256):
 // the four child boxes are slab-tested in FP32 (FP16 boxes are widened to FP32
@@ -30,10 +29,8 @@ constexpr int kBlock4 = 64;
 constexpr int kRefillAt4 = 40;
 // LDS stack entries per lane of the one-ray-per-lane kernels (the rest spill
 // to global memory).  Deep, incoherent nearest-hit walks push often and
-// measured slower with spills, so BVH4 keeps its whole stack in LDS; BVH8
-// (80 entries) keeps 32.
+// measured slower with spills, so BVH4 keeps its whole stack in LDS.
 constexpr int kSimpleLds4 = 48;
-constexpr int kSimpleLds8 = 32;
 // The speculative any-hit kernels leave the node loop for the leaf phase once
 // this many lanes or fewer still look for a leaf (0 would wait for the last).
 constexpr uint32_t kLeafPhaseAt = 8;
@@ -370,261 +367,7 @@ hipError_t launch_simple4(const Blas& b, const TraceArgs& a, hipStream_t stream)
   return e != hipSuccess ? e : f;
 }
 
-// ---------------------------------------------------------------------------
-// BVH8 (no reference counterpart): eight FP16 children per 128-byte node.
-// The eight entry distances are ordered by Batcher's odd-even merge network
-// (19 comparators, swap when strictly less); the oracle orders them with the
-// same network, so the counting variant's visits equal the oracle's.
-// ---------------------------------------------------------------------------
-template <bool kFused>
-__device__ inline void visit_oct(const Oct16* __restrict__ octs, uint32_t q, float3 o, float3 inv, float3 oi,
-                                 bool fz, float tnear, float tcull, float tm[8], uint32_t ch[8]) {
-  const uint4* p = reinterpret_cast<const uint4*>(octs + q);
-  uint4 w[8];
-#pragma unroll
-  for (int k = 0; k < 8; k++) w[k] = p[k];
-  float2 tt[8];
-  if (kFused && fz) {
-    const uint3 sel = near_far_selectors(inv);
-#pragma unroll
-    for (int k = 0; k < 8; k++) tt[k] = slab_fused(oi, inv, w[k], sel);
-  } else {
-#pragma unroll
-    for (int k = 0; k < 8; k++)
-      tt[k] = slab(o, inv, h2f_lo(w[k].x), h2f_hi(w[k].x), h2f_lo(w[k].y), h2f_hi(w[k].y), h2f_lo(w[k].z),
-                   h2f_hi(w[k].z));
-  }
-#pragma unroll
-  for (int k = 0; k < 8; k++) {
-    const float2 t = tt[k];
-    ch[k] = w[k].w;
-    const bool hit = (t.x <= t.y) && (t.y >= tnear) && (t.x <= tcull) && ch[k] != kEmptyLeaf;
-    tm[k] = hit ? t.x : kMaxFloat;
-  }
-  auto cswap = [&](int i, int j) {
-    const bool s = tm[j] < tm[i];
-    const float ti = tm[i], tj = tm[j];
-    const uint32_t ci = ch[i], cj = ch[j];
-    tm[i] = s ? tj : ti;
-    tm[j] = s ? ti : tj;
-    ch[i] = s ? cj : ci;
-    ch[j] = s ? ci : cj;
-  };
-  cswap(0, 1); cswap(2, 3); cswap(4, 5); cswap(6, 7);
-  cswap(0, 2); cswap(1, 3); cswap(4, 6); cswap(5, 7);
-  cswap(1, 2); cswap(5, 6);
-  cswap(0, 4); cswap(1, 5); cswap(2, 6); cswap(3, 7);
-  cswap(2, 4); cswap(3, 5);
-  cswap(1, 2); cswap(3, 4); cswap(5, 6);
-}
-
-// One ray per lane (also the counting variant: nodes visited, triangles in
-// visited leaves).
-template <bool kForceAny, bool kStats>
-__global__ __launch_bounds__(kBlock4) void k_trace8_simple(const Oct16* __restrict__ octs,
-                                                           const TriPre* __restrict__ tris,
-                                                           const Ray* __restrict__ rays, uint64_t n, void* out,
-                                                           int format, uint2 ids, uint32_t* __restrict__ stats,
-                                                           uint32_t* __restrict__ spill) {
-  __shared__ uint32_t stack_mem[kSimpleLds8 * kBlock4];
-  const uint64_t i = uint64_t(blockIdx.x) * kBlock4 + threadIdx.x;
-  if (i >= n) return;
-  const ShortStack<kSimpleLds8, kBlock4> stk{stack_mem + threadIdx.x, spill + i, gridDim.x * kBlock4};
-  const Ray r = rays[i];
-  const float3 o = make_float3(r.pos_near.x, r.pos_near.y, r.pos_near.z);
-  const float3 d = make_float3(r.dir_far.x, r.dir_far.y, r.dir_far.z);
-  const float tnear = r.pos_near.w;
-  const bool any = kForceAny || (r.dir_far.w <= 0.0f);
-  const float3 inv = safe_inverse(d);
-  const float3 oi = neg_o_inv(o, inv);
-  const bool fz = fused_ok(oi);
-  HitRec h;
-  h.t = kForceAny ? fabsf(r.dir_far.w) : (any ? -r.dir_far.w : r.dir_far.w);
-  h.prim = kInvalid;
-  h.u = h.v = 0.0f;
-  uint32_t nvisits = 0, ntests = 0;
-  int top = 0;
-  uint32_t cur = 0;
-  while (true) {
-    bool pop = false;
-    if ((cur & kLeafBit) == 0) {
-      float tm[8];
-      uint32_t ch[8];
-      visit_oct<kForceAny>(octs, cur, o, inv, oi, fz, tnear, h.t, tm, ch);
-      if (kStats) nvisits++;
-#pragma unroll
-      for (int k = 7; k >= 1; k--)
-        if (tm[k] < kMaxFloat) stk.push(top, ch[k]);
-      if (tm[0] < kMaxFloat) cur = ch[0];
-      else pop = true;
-    }
-    if (!pop && (cur & kLeafBit) != 0 && cur != kInvalid) {
-      const uint32_t s = leaf_start(cur), c = leaf_count(cur);
-      if (kStats) ntests += c;
-      for (uint32_t k = s; k < s + c; k++)
-        if (mt_test_lex(o, d, tnear, tris[k], h) && any) break;
-      pop = true;
-      if (any && h.prim != kInvalid) break;
-    }
-    if (pop) {
-      if (top == 0) break;
-      cur = stk.pop(top);
-    }
-  }
-  if (kStats) {
-    stats[2 * i + 0] = nvisits;
-    stats[2 * i + 1] = ntests;
-  }
-  write_hit<kForceAny>(out, i, format, ids, h);
-}
-
-// Persistent speculative any-hit over BVH8 nodes: the BVH4 kernel's scheme
-// (wave-local ray pool, park the first leaf, keep descending).
-__global__ __launch_bounds__(kBlock4) void k_trace8_any_persistent(const Oct16* __restrict__ octs,
-                                                                   const TriPre* __restrict__ tris,
-                                                                   const Ray* __restrict__ rays, uint64_t n,
-                                                                   uint8_t* __restrict__ out,
-                                                                   uint32_t* __restrict__ queue,
-                                                                   uint32_t* __restrict__ spill) {
-  __shared__ uint32_t stack_mem[kShortStack * kBlock4];
-  const int lane = threadIdx.x & (kWave - 1);
-  const ShortStack<kShortStack, kBlock4> stk{stack_mem + threadIdx.x, spill + blockIdx.x * kBlock4 + threadIdx.x,
-                                             gridDim.x * kBlock4};
-  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
-  bool active = false;
-  uint64_t rid = 0;
-  float3 o = make_float3(0.f, 0.f, 0.f), d = o, inv = o, oi = o;
-  bool fz = true;
-  float tnear = 0.f;
-  HitRec h;
-  h.t = 0.f;
-  h.prim = kInvalid;
-  h.u = h.v = 0.f;
-  int top = 0;
-  uint32_t cur = kInvalid;
-  uint32_t leaf = 0;
-  RayPool pool;
-
-  while (true) {
-    const uint64_t idle = __ballot(!active);
-    const uint32_t nidle = __popcll(idle);
-    if (!pool.drained() && (nidle >= kRefillAt4 || nidle == kWave)) {
-      const uint64_t idx = pool.take<kClaim>(nidle, __popcll(idle & lt_mask), queue, n);
-      if (!active) {
-        if (idx < n) {
-          rid = idx;
-          const Ray r = rays[idx];
-          o = make_float3(r.pos_near.x, r.pos_near.y, r.pos_near.z);
-          d = make_float3(r.dir_far.x, r.dir_far.y, r.dir_far.z);
-          tnear = r.pos_near.w;
-          h.t = fabsf(r.dir_far.w);
-          h.prim = kInvalid;
-          inv = safe_inverse(d);
-          oi = neg_o_inv(o, inv);
-          fz = fused_ok(oi);
-          top = 0;
-          cur = 0;
-          leaf = 0;
-          active = true;
-        }
-      }
-    }
-    if (__ballot(active) == 0) break;
-
-    while (true) {
-      while (true) {
-        if (active && leaf == 0 && (cur & kLeafBit) != 0 && cur != kInvalid) {
-          leaf = cur;
-          cur = (top > 0) ? stk.pop(top) : kInvalid;
-        }
-        // leave for the leaf phase once at most kLeafPhaseAt lanes still
-        // search and some lane holds a leaf (the slowest searchers would
-        // otherwise keep the wave half idle)
-        const bool searching = active && leaf == 0 && (cur & kLeafBit) == 0;
-        const uint32_t nsearch = __popcll(__ballot(searching));
-        if (nsearch == 0 || (nsearch <= kLeafPhaseAt && __any(leaf != 0))) break;
-        if (active && (cur & kLeafBit) == 0) {
-          float tm[8];
-          uint32_t ch[8];
-          visit_oct<true>(octs, cur, o, inv, oi, fz, tnear, h.t, tm, ch);
-#pragma unroll
-          for (int k = 7; k >= 1; k--)
-            if (tm[k] < kMaxFloat) stk.push(top, ch[k]);
-          if (tm[0] < kMaxFloat) cur = ch[0];
-          else cur = (top > 0) ? stk.pop(top) : kInvalid;
-        }
-      }
-      if (active && leaf != 0) {
-        const uint32_t s = leaf_start(leaf), c = leaf_count(leaf);
-        for (uint32_t k = s; k < s + c; k++)
-          if (mt_test_lex(o, d, tnear, tris[k], h)) break;
-        leaf = 0;
-      }
-      if (active && (h.prim != kInvalid || cur == kInvalid)) {
-        out[rid] = (h.prim != kInvalid) ? 1 : 0;
-        active = false;
-      }
-      const uint32_t nact = __popcll(__ballot(active));
-      if (nact == 0 || (!pool.drained() && kWave - nact >= kRefillAt4)) break;
-    }
-  }
-}
-
-hipError_t launch_any8(const Blas& b, const TraceArgs& a, hipStream_t stream) {
-  int dev = 0, cus = 256, per_cu = 0;
-  (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  const void* kfn = reinterpret_cast<const void*>(&k_trace8_any_persistent);
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kBlock4, 0) != hipSuccess || per_cu <= 0) per_cu = 8;
-  const uint64_t want = (a.n + kBlock4 - 1) / kBlock4;
-  const int grid = int(std::min<uint64_t>(want, uint64_t(std::max(cus, 1)) * uint64_t(per_cu)));
-  const size_t spill_words = size_t(grid) * kBlock4 * (kStackDepth8 - kShortStack);
-  uint32_t* buf = nullptr;
-  hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&buf), sizeof(uint32_t) * (64 + spill_words), stream);
-  if (e != hipSuccess) return e;
-  k_reset_counter4<<<1, 1, 0, stream>>>(buf);
-  k_trace8_any_persistent<<<grid, kBlock4, 0, stream>>>(b.octs16, b.tris, a.rays, a.n, static_cast<uint8_t*>(a.out),
-                                                        buf, buf + 64);
-  e = hipGetLastError();
-  hipError_t f = hipFreeAsync(buf, stream);
-  return e != hipSuccess ? e : f;
-}
-
-hipError_t launch_simple8(const Blas& b, const TraceArgs& a, hipStream_t stream) {
-  const uint32_t blocks = uint32_t((a.n + kBlock4 - 1) / kBlock4);
-  uint32_t* spill = nullptr;
-  hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&spill),
-                                sizeof(uint32_t) * size_t(blocks) * kBlock4 * (kStackDepth8 - kSimpleLds8), stream);
-  if (e != hipSuccess) return e;
-  const uint2 ids = make_uint2(a.inst_id, a.geom_id);
-  if (a.anyhit) {
-    if (a.stats)
-      k_trace8_simple<true, true><<<blocks, kBlock4, 0, stream>>>(b.octs16, b.tris, a.rays, a.n, a.out, a.format, ids,
-                                                                  a.stats, spill);
-    else
-      k_trace8_simple<true, false><<<blocks, kBlock4, 0, stream>>>(b.octs16, b.tris, a.rays, a.n, a.out, a.format,
-                                                                   ids, nullptr, spill);
-  } else {
-    if (a.stats)
-      k_trace8_simple<false, true><<<blocks, kBlock4, 0, stream>>>(b.octs16, b.tris, a.rays, a.n, a.out, a.format,
-                                                                   ids, a.stats, spill);
-    else
-      k_trace8_simple<false, false><<<blocks, kBlock4, 0, stream>>>(b.octs16, b.tris, a.rays, a.n, a.out, a.format,
-                                                                    ids, nullptr, spill);
-  }
-  e = hipGetLastError();
-  const hipError_t f = hipFreeAsync(spill, stream);
-  return e != hipSuccess ? e : f;
-}
-
 }  // namespace
-
-hipError_t trace_blas8(const Blas& b, const TraceArgs& a, hipStream_t stream) {
-  if (a.n == 0) return hipSuccess;
-  if (!b.octs16) return hipErrorInvalidValue;
-  if (a.anyhit && !a.stats) return launch_any8(b, a, stream);
-  return launch_simple8(b, a, stream);
-}
 
 hipError_t trace_ao4(const Blas& b, bool fp16, const AoFrame* frames, uint64_t npix, uint32_t spp, float tmax,
                      uint64_t seed, uint8_t* out, hipStream_t stream) {

@@ -222,8 +222,6 @@ ISceneObject* CreateSceneRT(const char* a_implName, const char* a_buildName, con
     layout = HRT_LAYOUT_BVH4_FP16;
   else if (name.find("HipBVH4") != std::string::npos)
     layout = HRT_LAYOUT_BVH4;
-  else if (name.find("HipAutoHalfWide") != std::string::npos)
-    layout = HRT_LAYOUT_AUTO_FP16_WIDE;
   else if (name.find("HipAutoHalf") != std::string::npos)
     layout = HRT_LAYOUT_AUTO_FP16;
   else if (name.find("HipAuto") != std::string::npos)

@@ -81,16 +81,13 @@ enum { HRT_HIT_FULL = 0, HRT_HIT_COMPACT = 1 };
  *              extent), the hierarchy above them by agglomerative clustering;
  *              it visits fewer nodes.  Any-hit answers do not depend on the
  *              tree.  If a cubic-grid cell holds more than 127 triangles,
- *              any-hit uses the reference tree's FP16 BVH4 instead.
- *   AUTO_FP16_WIDE  as AUTO_FP16 with the any-hit tree collapsed to 8-wide
- *              nodes with FP16 boxes (no reference counterpart). */
+ *              any-hit uses the reference tree's FP16 BVH4 instead. */
 enum {
   HRT_LAYOUT_BVH2 = 0,
   HRT_LAYOUT_BVH4 = 1,
   HRT_LAYOUT_BVH4_FP16 = 2,
   HRT_LAYOUT_AUTO = 3,
-  HRT_LAYOUT_AUTO_FP16 = 4,
-  HRT_LAYOUT_AUTO_FP16_WIDE = 5
+  HRT_LAYOUT_AUTO_FP16 = 4
 };
 enum { HRT_BUILD_LOW = 0, HRT_BUILD_MEDIUM = 1, HRT_BUILD_HIGH = 2, HRT_BUILD_REFIT = 3 }; /* BuildQuality, CrossRT.h:9-15 */
 
@@ -99,8 +96,7 @@ enum { HRT_BUILD_LOW = 0, HRT_BUILD_MEDIUM = 1, HRT_BUILD_HIGH = 2, HRT_BUILD_RE
 enum {
   HRT_DUMP_SIZES = 0,        /* uint32[5]: triangles, leaves, max leaf size, pairs, BVH4 nodes;
                                 a 6th word (if the buffer holds 24 bytes): 1 when the hierarchy
-                                above the leaves is the agglomerative one (any-hit tree);
-                                a 7th (28 bytes): BVH8 nodes */
+                                above the leaves is the agglomerative one (any-hit tree) */
   HRT_DUMP_CODES_INPUT = 1,  /* uint32[ntris]: Morton code per input triangle */
   HRT_DUMP_CODES_SORTED = 2, /* uint32[ntris]: codes after the (code, id) sort */
   HRT_DUMP_IDS_SORTED = 3,   /* uint32[ntris]: triangle ids after the sort */
@@ -113,8 +109,7 @@ enum {
   HRT_DUMP_QUADS32 = 9,      /* uint32[nquads*32]: BVH4 nodes, FP32 boxes (after a BVH4 commit) */
   HRT_DUMP_QUADS16 = 10,     /* uint32[nquads*16]: BVH4 nodes, FP16 boxes */
   HRT_DUMP_PAIRS = 11,       /* uint32[pairs*16]: the BVH2 node pairs as traversed */
-  HRT_DUMP_OCTS16 = 12,      /* uint32[nodes*32]: BVH8 nodes with FP16 boxes (any-hit tree) */
-  HRT_DUMP_ANY_TREE = 0x100  /* OR'ed into `what`: the same array of the any-hit tree (HRT_LAYOUT_AUTO_FP16[_WIDE]) */
+  HRT_DUMP_ANY_TREE = 0x100  /* OR'ed into `what`: the same array of the any-hit tree (HRT_LAYOUT_AUTO_FP16) */
 };
 
 /* Context: one device, one HIP stream.  No counterpart in the reference
@@ -158,7 +153,7 @@ hrt_status hrt_update_triangles(hrt_scene* scene, uint32_t geomId, const float* 
                                 size_t vStrideBytes, const uint32_t* idx, size_t nIdx, int quality);
 
 /* Warm start: a committed scene (every geometry with its trees -- LBVH node
- * pairs, leaf-order triangles, BVH4/BVH8 nodes, the any-hit tree -- the
+ * pairs, leaf-order triangles, BVH4 nodes, the any-hit tree -- the
  * instances and the layout) as one byte buffer.  Call with out = NULL to get
  * the size in *size, then with a buffer of at least that size.  No
  * counterpart in the reference (its engines rebuild from geometry; the

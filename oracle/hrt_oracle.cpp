@@ -119,7 +119,6 @@ static inline Node dummy_node() {  // cbvh.h:28-36
 // BVH4 child (box + reference) and node, see collapse4 below
 struct Child4 { f3 bmin, bmax; uint32_t ref; };
 struct Quad { Child4 c[4]; };
-struct Oct { Child4 c[8]; };
 
 struct Build {
   // inputs (float4 positions with w = 1, triangle indices)
@@ -153,7 +152,6 @@ struct Build {
   // BVH4 collapse (FP32 boxes, FP16-rounded boxes), made on first use
   std::vector<Quad> q32, q16;
   // BVH8 collapse (FP16-rounded boxes), made on first use
-  std::vector<struct Oct> o16;
 };
 
 static size_t power_of_two(size_t n) {  // lbvh.cpp:76-82
@@ -868,7 +866,6 @@ static void ploc(Build& b, uint32_t radius) {
   build_peach(b);
   b.q32.clear();
   b.q16.clear();
-  b.o16.clear();
 }
 
 // ---------------------------------------------------------------------------
@@ -1081,138 +1078,6 @@ static Hit trace_bvh4(const Build& b, const std::vector<Quad>& quads, f4 posNear
       if (tm[3] < MAXF) stack[top++] = ch[3];
       if (tm[2] < MAXF) stack[top++] = ch[2];
       if (tm[1] < MAXF) stack[top++] = ch[1];
-      if (tm[0] < MAXF) cur = ch[0];
-      pop = !(tm[0] < MAXF);
-      if (cnt && uint64_t(top) > cnt->max_stack) cnt->max_stack = uint64_t(top);
-    }
-    if (!pop && (cur & LEAF_BIT) != 0 && cur != 0xFFFFFFFFu) {
-      const uint32_t start = leaf_start(cur), count = leaf_count(cur);
-      if (cnt) { cnt->leaves++; cnt->tris += count; }
-      for (uint32_t t = start; t < start + count; t++) {
-        const uint32_t* ix = &b.idx_leaf_order[3 * size_t(t)];
-        if (mt_test_lex(o, d, tNear, vpos(b, ix[0]), vpos(b, ix[1]), vpos(b, ix[2]), h, b.perm[t]) && anyhit) break;
-      }
-      pop = true;
-      if (anyhit && h.primId != 0xFFFFFFFFu) break;
-    }
-    if (pop) {
-      top--;
-      cur = stack[std::max(top, 0)];
-    }
-  }
-  return h;
-}
-
-// ---------------------------------------------------------------------------
-// BVH8 (no reference counterpart: the engine's wide any-hit nodes).  A BVH8
-// node replaces a binary internal node whose depth is a multiple of three;
-// its children are that node's descendants three levels down, an internal
-// descendant at a shallower level being expanded and a leaf taken as it is,
-// in left-to-right order: at most eight.  Unused slots as in collapse4.
-// Nodes are numbered in binary index order; boxes are FP16-rounded outward
-// as BVH4HalfRT rounds them (cbvh_fat.cpp:447-448).
-// ---------------------------------------------------------------------------
-static void collapse8(const Build& b, std::vector<Oct>& octs) {
-  octs.clear();
-  const uint32_t L = b.L;
-  const Node* lr = b.lr.data();
-  auto empty_child = []() {
-    Child4 e;
-    e.bmin = {1e9f, 1e9f, 1e9f};
-    e.bmax = {-1e9f, -1e9f, -1e9f};
-    e.ref = pack_leaf(0, 0);
-    return e;
-  };
-  if (L == 1) {  // root is a leaf: one node holding it and the peach dummy
-    Oct o;
-    o.c[0] = Child4{lr[0].bmin, lr[0].bmax, lr[0].left};
-    o.c[1] = Child4{{0, 0, 0}, {0, 0, 0}, 0x80000000u};
-    for (int k = 2; k < 8; k++) o.c[k] = empty_child();
-    for (auto& c : o.c) c = fp16_child(c);
-    octs.push_back(o);
-    return;
-  }
-  std::vector<int> depth(L - 1, -1);
-  depth[0] = 0;
-  std::vector<uint32_t> st = {0};
-  while (!st.empty()) {
-    uint32_t k = st.back();
-    st.pop_back();
-    for (uint32_t c : {lr[k].left, lr[k].escape})
-      if (c < L - 1) { depth[c] = depth[k] + 1; st.push_back(c); }
-  }
-  std::vector<uint32_t> oidx(L - 1, 0xFFFFFFFFu);
-  uint32_t no = 0;
-  for (uint32_t k = 0; k + 1 < L; k++)
-    if (depth[k] % 3 == 0) oidx[k] = no++;
-  octs.resize(no);
-  auto child_of = [&](uint32_t idx) {
-    Child4 c;
-    c.bmin = lr[idx].bmin;
-    c.bmax = lr[idx].bmax;
-    c.ref = (idx >= L - 1) ? lr[idx].left : oidx[idx];
-    return c;
-  };
-  for (uint32_t k = 0; k + 1 < L; k++) {
-    if (oidx[k] == 0xFFFFFFFFu) continue;
-    Oct& o = octs[oidx[k]];
-    int n = 0;
-    for (uint32_t a : {lr[k].left, lr[k].escape}) {
-      if (a >= L - 1) { o.c[n++] = child_of(a); continue; }
-      for (uint32_t c : {lr[a].left, lr[a].escape}) {
-        if (c >= L - 1) { o.c[n++] = child_of(c); continue; }
-        o.c[n++] = child_of(lr[c].left);
-        o.c[n++] = child_of(lr[c].escape);
-      }
-    }
-    while (n < 8) o.c[n++] = empty_child();
-    for (auto& c : o.c) c = fp16_child(c);
-  }
-}
-
-// BVH8 traversal with the (t, primId) rule: the eight entry distances are
-// ordered by Batcher's odd-even merge network (19 comparators, swap when
-// strictly less), the hit children are deferred farthest first and the
-// nearest is entered.  The engine's kernels use the same network, so their
-// visit counts equal these.
-static Hit trace_bvh8(const Build& b, const std::vector<Oct>& octs, f4 posNear, f4 dirFar, Counters* cnt,
-                      bool fused = false) {
-  const bool anyhit = (dirFar.w <= 0.0f);
-  const float tFar = anyhit ? -dirFar.w : dirFar.w;
-  Hit h;
-  h.t = tFar;
-  h.primId = h.instId = h.geomId = 0xFFFFFFFFu;
-  h.u = h.v = 0.0f;
-  const f3 o = {posNear.x, posNear.y, posNear.z};
-  const f3 d = {dirFar.x, dirFar.y, dirFar.z};
-  const float tNear = posNear.w;
-  const f3 inv = safe_inverse(d);
-  const bool use_fused = fused && fused_ok(o, inv);
-  uint32_t stack[128];
-  int top = 0;
-  uint32_t cur = 0;
-  const float MAXF = 3.402823466e+38f;
-  static const int net[19][2] = {{0, 1}, {2, 3}, {4, 5}, {6, 7}, {0, 2}, {1, 3}, {4, 6}, {5, 7}, {1, 2}, {5, 6},
-                                 {0, 4}, {1, 5}, {2, 6}, {3, 7}, {2, 4}, {3, 5}, {1, 2}, {3, 4}, {5, 6}};
-  while (top >= 0) {
-    bool pop = false;
-    if ((cur & LEAF_BIT) == 0) {
-      const Oct& q = octs[cur];
-      if (cnt) cnt->pairs++;
-      float tm[8];
-      uint32_t ch[8];
-      for (int k = 0; k < 8; k++) {
-        float a, c;
-        if (use_fused) slab_fused(o, inv, q.c[k].bmin, q.c[k].bmax, a, c);
-        else slab(o, inv, q.c[k].bmin, q.c[k].bmax, a, c);
-        const bool hit = (a <= c) && (c >= tNear) && (a <= h.t) && q.c[k].ref != 0x80000000u;
-        tm[k] = hit ? a : MAXF;
-        ch[k] = q.c[k].ref;
-      }
-      for (const auto& e : net)
-        if (tm[e[1]] < tm[e[0]]) { std::swap(tm[e[0]], tm[e[1]]); std::swap(ch[e[0]], ch[e[1]]); }
-      for (int k = 7; k >= 1; k--)
-        if (tm[k] < MAXF) stack[top++] = ch[k];
       if (tm[0] < MAXF) cur = ch[0];
       pop = !(tm[0] < MAXF);
       if (cnt && uint64_t(top) > cnt->max_stack) cnt->max_stack = uint64_t(top);
@@ -1723,50 +1588,6 @@ void hrto_quads(void* h, int fp16, uint32_t* out) {
       std::memcpy(w + 4, &q[i].c[k].bmax, 12);
       w[7] = 0;
     }
-}
-
-// BVH8 nodes (FP16 boxes after packing and unpacking): octs[i] = 8 children
-// x 8 words (bmin xyz, ref, bmax xyz, 0).
-uint64_t hrto_oct_count(void* h) {
-  Build* b = static_cast<Build*>(h);
-  if (b->o16.empty()) collapse8(*b, b->o16);
-  return b->o16.size();
-}
-void hrto_octs(void* h, uint32_t* out) {
-  Build* b = static_cast<Build*>(h);
-  if (b->o16.empty()) collapse8(*b, b->o16);
-  for (size_t i = 0; i < b->o16.size(); i++)
-    for (int k = 0; k < 8; k++) {
-      uint32_t* w = out + 64 * i + 8 * k;
-      std::memcpy(w + 0, &b->o16[i].c[k].bmin, 12);
-      w[3] = b->o16[i].c[k].ref;
-      std::memcpy(w + 4, &b->o16[i].c[k].bmax, 12);
-      w[7] = 0;
-    }
-}
-
-// BVH8 traversal on the FP16 nodes, (t, primId) rule; fused != 0: slab_fused box tests.
-void hrto_trace8(void* h, const float* rays, uint64_t n, uint32_t* hits, uint32_t* counters, int threads,
-                 int fused) {
-  Build* b = static_cast<Build*>(h);
-  if (b->o16.empty()) collapse8(*b, b->o16);
-  const std::vector<Oct>& q = b->o16;
-#ifdef _OPENMP
-  if (threads > 0) omp_set_num_threads(threads);
-#pragma omp parallel for schedule(dynamic, 256) if (threads != 1)
-#endif
-  for (int64_t i = 0; i < int64_t(n); i++) {
-    f4 p, d;
-    std::memcpy(&p, rays + 8 * i, 16);
-    std::memcpy(&d, rays + 8 * i + 4, 16);
-    Counters c;
-    Hit r = trace_bvh8(*b, q, p, d, counters ? &c : nullptr, fused != 0);
-    std::memcpy(hits + 6 * i, &r, sizeof(Hit));
-    if (counters) {
-      counters[2 * i + 0] = uint32_t(c.pairs);
-      counters[2 * i + 1] = uint32_t(c.tris);
-    }
-  }
 }
 
 // BVH4 traversal (mode 0: FP32 boxes, 1: FP16 boxes, 3: FP16 boxes with

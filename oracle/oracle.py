@@ -65,12 +65,6 @@ def lib():
         L.hrto_free.argtypes = [vp]
         L.hrto_refit.restype = C.c_int
         L.hrto_refit.argtypes = [vp, vp, u32, vp, u32]
-        L.hrto_oct_count.restype = u64
-        L.hrto_oct_count.argtypes = [vp]
-        L.hrto_octs.restype = None
-        L.hrto_octs.argtypes = [vp, vp]
-        L.hrto_trace8.restype = None
-        L.hrto_trace8.argtypes = [vp, vp, u64, vp, vp, C.c_int, C.c_int]
         L.hrto_ploc.restype = None
         L.hrto_ploc.argtypes = [vp, u32]
         L.hrto_superblock_offset.restype = u32
@@ -241,25 +235,6 @@ class OracleBVH:
         out = np.zeros((n, 4, 8), dtype=np.uint32)
         lib().hrto_quads(self._h, 1 if fp16 else 0, _ptr(out))
         return out
-
-    def octs(self) -> np.ndarray:
-        """(nocts, 8, 8) uint32 BVH8 nodes, FP16-rounded boxes: per child bmin xyz, ref, bmax xyz, 0."""
-        n = int(lib().hrto_oct_count(self._h))
-        out = np.zeros((n, 8, 8), dtype=np.uint32)
-        lib().hrto_octs(self._h, _ptr(out))
-        return out
-
-    def trace8(self, rays: np.ndarray, counters: bool = False, threads: int = 0, fused: bool = False):
-        """BVH8 traversal on FP16 nodes, nearest hit by (t, primId); fused: the box tests in the fused
-        form of the engine's any-hit calls.  Returns (t, primId, u, v[, nodes, tris])."""
-        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
-        n = rays.shape[0]
-        hits = np.zeros((n, 6), dtype=np.uint32)
-        cnt = np.zeros((n, 2), dtype=np.uint32) if counters else None
-        lib().hrto_trace8(self._h, _ptr(rays), n, _ptr(hits), _ptr(cnt) if counters else None, threads, int(fused))
-        out = (hits[:, 0].view(np.float32).copy(), hits[:, 1].copy(), hits[:, 4].view(np.float32).copy(),
-               hits[:, 5].view(np.float32).copy())
-        return out + (cnt[:, 0].copy(), cnt[:, 1].copy()) if counters else out
 
     def trace4(self, rays: np.ndarray, mode: int = MODE4_FP32, counters: bool = False, threads: int = 0):
         """BVH4 traversal (MODE4_FP32 / MODE4_FP16) or brute force (MODE4_BRUTE),

@@ -237,59 +237,7 @@ def test_precision_soup_fp16_vs_fp32_identical(gpu_ctx):
     assert rel.mean() < 1e-5
 
 
-def gpu_octs16_as_floats(q: np.ndarray):
-    """(no, 32) words -> boxes (no, 8, 6) float32 and refs (no, 8)."""
-    w = q.reshape(-1, 8, 4)
-    halves = w[:, :, 0:3].copy().view(np.uint16).reshape(-1, 8, 6)
-    return halves.view(np.float16).astype(np.float32), w[:, :, 3]
-
-
-@pytest.mark.parametrize("name,gen", CASES, ids=[c[0] for c in CASES])
-def test_bvh8_nodes_bit_exact(gpu_ctx, name, gen):
-    """HRT_LAYOUT_AUTO_FP16_WIDE: the any-hit tree's 8-wide FP16 nodes equal the oracle's collapse8 of the
-    same tree (boxes and references)."""
-    v, f = gen()
-    s = scene_with(gpu_ctx, v, f, H.LAYOUT_AUTO_FP16_WIDE)
-    ob, _ = oracle_any_tree(v, f)
-    if ob.max_run > 127:
-        assert not has_any_tree(s)
-        s.close()
-        return
-    boxes, refs = gpu_octs16_as_floats(s.dump(0, H.DUMP_OCTS16 | H.DUMP_ANY_TREE))
-    oo = ob.octs()
-    assert boxes.shape[0] == oo.shape[0]
-    np.testing.assert_array_equal(refs, oo[:, :, 3])
-    want = np.concatenate([oo[:, :, 0:3], oo[:, :, 4:7]], axis=2).view(np.float32)
-    np.testing.assert_array_equal(boxes.view(np.uint32), want.view(np.uint32))
-    s.close()
-
-
-@pytest.mark.parametrize("name,gen", CASES, ids=[c[0] for c in CASES])
-def test_bvh8_any_hit(gpu_ctx, name, gen):
-    """Any-hit on the 8-wide nodes: the reference tree's answers; the counting variant's node visits and
-    triangle tests equal the oracle's BVH8 traversal of the same tree."""
-    v, f = gen()
-    rays = rays_for(v, 20000, 1900)
-    rays[:, 7] = 0.5 * float(np.max(v.max(0) - v.min(0)))
-    say(f"bvh8 any {name}")
-    s = scene_with(gpu_ctx, v, f, H.LAYOUT_AUTO_FP16_WIDE)
-    anyr = rays.copy()
-    anyr[:, 7] = -anyr[:, 7]
-    _, p2, _, _ = O.OracleBVH(v, f).trace(anyr)
-    g = trace(s, rays, "any")
-    np.testing.assert_array_equal(g["hit"], p2 != INVALID)
-    if has_any_tree(s):
-        ob, _ = oracle_any_tree(v, f)
-        _, p8, _, _, nn, nt = ob.trace8(anyr, counters=True, fused=True)
-        gs = trace(s, rays, "any", stats=True)
-        np.testing.assert_array_equal(gs["hit"], p8 != INVALID)
-        np.testing.assert_array_equal(gs["pairs"], nn)
-        np.testing.assert_array_equal(gs["tris"], nt)
-    s.close()
-
-
-@pytest.mark.parametrize("layout", [H.LAYOUT_BVH4_FP16, H.LAYOUT_AUTO_FP16, H.LAYOUT_AUTO_FP16_WIDE],
-                         ids=["bvh4_fp16", "auto_fp16", "auto_fp16_wide"])
+@pytest.mark.parametrize("layout", [H.LAYOUT_BVH4_FP16, H.LAYOUT_AUTO_FP16], ids=["bvh4_fp16", "auto_fp16"])
 def test_any_hit_axis_parallel_far_from_origin(gpu_ctx, layout):
     """Rays with d.x = d.z = 0 over geometry at x = 500: -(o * inv) overflows there, so these lanes
     keep the unfused box test; answers equal brute force and counts equal the oracle's."""
@@ -311,10 +259,7 @@ def test_any_hit_axis_parallel_far_from_origin(gpu_ctx, layout):
         *_, nq, nt = ob.trace4(anyr, O.MODE4_FP16_FUSED, counters=True)
     elif has_any_tree(s):
         ob = oracle_any_tree(v, f)[0]
-        if layout == H.LAYOUT_AUTO_FP16:
-            *_, nq, nt = ob.trace4(anyr, O.MODE4_FP16_FUSED, counters=True)
-        else:
-            *_, nq, nt = ob.trace8(anyr, counters=True, fused=True)
+        *_, nq, nt = ob.trace4(anyr, O.MODE4_FP16_FUSED, counters=True)
     else:
         s.close()
         return
@@ -323,9 +268,8 @@ def test_any_hit_axis_parallel_far_from_origin(gpu_ctx, layout):
     s.close()
 
 
-@pytest.mark.parametrize("layout", [H.LAYOUT_BVH2, H.LAYOUT_BVH4, H.LAYOUT_BVH4_FP16, H.LAYOUT_AUTO_FP16,
-                                    H.LAYOUT_AUTO_FP16_WIDE],
-                         ids=["bvh2", "bvh4", "bvh4_fp16", "auto_fp16", "auto_fp16_wide"])
+@pytest.mark.parametrize("layout", [H.LAYOUT_BVH2, H.LAYOUT_BVH4, H.LAYOUT_BVH4_FP16, H.LAYOUT_AUTO_FP16],
+                         ids=["bvh2", "bvh4", "bvh4_fp16", "auto_fp16"])
 def test_any_hit_interval_edge_cases(gpu_ctx, layout):
     """Empty and reversed (tNear, tFar) intervals, a negative tNear and a NaN bound: the any-hit
     kernels finish such rays at once; answers equal the oracle's brute force."""

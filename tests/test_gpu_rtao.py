@@ -10,9 +10,8 @@ from heavyrt_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
 
-LAYOUTS = [H.LAYOUT_BVH2, H.LAYOUT_BVH4, H.LAYOUT_BVH4_FP16, H.LAYOUT_AUTO, H.LAYOUT_AUTO_FP16,
-           H.LAYOUT_AUTO_FP16_WIDE]
-IDS = ["bvh2", "bvh4", "bvh4_fp16", "auto", "auto_fp16", "auto_fp16_wide"]
+LAYOUTS = [H.LAYOUT_BVH2, H.LAYOUT_BVH4, H.LAYOUT_BVH4_FP16, H.LAYOUT_AUTO, H.LAYOUT_AUTO_FP16]
+IDS = ["bvh2", "bvh4", "bvh4_fp16", "auto", "auto_fp16"]
 
 
 def two_step_and_fused(scene, prim, spp, tmax, seed):

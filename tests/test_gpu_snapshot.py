@@ -22,8 +22,8 @@ def same_traces(a, b, rays):
     np.testing.assert_array_equal(trace(a, anyr, "any")["hit"], trace(b, anyr, "any")["hit"])
 
 
-@pytest.mark.parametrize("layout", [H.LAYOUT_BVH2, H.LAYOUT_BVH4_FP16, H.LAYOUT_AUTO_FP16, H.LAYOUT_AUTO_FP16_WIDE],
-                         ids=["bvh2", "bvh4_fp16", "auto_fp16", "auto_fp16_wide"])
+@pytest.mark.parametrize("layout", [H.LAYOUT_BVH2, H.LAYOUT_BVH4_FP16, H.LAYOUT_AUTO_FP16],
+                         ids=["bvh2", "bvh4_fp16", "auto_fp16"])
 def test_snapshot_single_geometry(gpu_ctx, layout):
     v, f = W.rtao_scene(256, 128, 300)
     s = H.Scene(gpu_ctx)
@@ -37,12 +37,9 @@ def test_snapshot_single_geometry(gpu_ctx, layout):
         np.testing.assert_array_equal(r.dump(0, what).view(np.uint32), s.dump(0, what).view(np.uint32))
     if layout == H.LAYOUT_BVH4_FP16:
         np.testing.assert_array_equal(r.dump(0, H.DUMP_QUADS16), s.dump(0, H.DUMP_QUADS16))
-    if layout in (H.LAYOUT_AUTO_FP16, H.LAYOUT_AUTO_FP16_WIDE):
+    if layout == H.LAYOUT_AUTO_FP16:
         for what in (H.DUMP_SIZES, H.DUMP_PAIRS, H.DUMP_QUADS16):
             np.testing.assert_array_equal(r.dump(0, what | H.DUMP_ANY_TREE), s.dump(0, what | H.DUMP_ANY_TREE))
-        if layout == H.LAYOUT_AUTO_FP16_WIDE:
-            np.testing.assert_array_equal(r.dump(0, H.DUMP_OCTS16 | H.DUMP_ANY_TREE),
-                                          s.dump(0, H.DUMP_OCTS16 | H.DUMP_ANY_TREE))
     rays = rays_for(v, 20_000, 5300)
     same_traces(s, r, rays)
     # a second snapshot of the restored scene is the same bytes
@@ -127,3 +124,46 @@ def test_snapshot_errors(gpu_ctx):
     # the untouched bytes still load
     H.Scene.deserialize(gpu_ctx, bytes(data)).close()
     s.close()
+
+
+def test_snapshot_crafted_records_are_refused(gpu_ctx):
+    """Records behind a matching hash whose references would lead a kernel out of its arrays (a child
+    past the node pairs, a leaf past the triangles, the root as a child, a duplicated leaf-order entry,
+    an index past the vertices, a triangle id past the triangles, a leaf of 200 triangles) are refused
+    on the host before anything is uploaded."""
+    v, f = W.icosphere(2)
+    s = H.Scene(gpu_ctx)
+    s.add_instance(s.add_triangles(v, f))
+    s.commit()
+    body = bytes(s.serialize()[:-8])
+    s.close()
+    head = np.frombuffer(body[24:56], dtype="<u4")
+    nverts, ntris, leaves = int(head[0]), int(head[1]), int(head[2])
+    np_alloc = max(ntris - 1, 1)
+    at = 24 + 80
+    off = {}
+    for name, size in (("verts", 16 * nverts), ("idx", 12 * ntris), ("pairs", 64 * np_alloc),
+                       ("tris", 48 * ntris), ("perm", 4 * ntris)):
+        off[name] = at
+        at += size
+
+    def with_word(byte_off: int, value: int) -> bytes:
+        b = bytearray(body)
+        b[byte_off:byte_off + 4] = int(value).to_bytes(4, "little")
+        return bytes(b)
+
+    pair = lambda k, side: off["pairs"] + 64 * k + 32 * side + 12  # child reference word
+    cases = {
+        "child past the pairs": with_word(pair(0, 0), leaves + 3),
+        "root as a child": with_word(pair(0, 1), 0),
+        "leaf past the triangles": with_word(pair(0, 0), 0x80000000 | (5 << 24) | (ntris - 2)),
+        "longest leaf of 200": with_word(24 + 12, 200),
+        "duplicated leaf-order entry": with_word(off["perm"] + 4, int(np.frombuffer(body[off["perm"]:off["perm"] + 4],
+                                                                                    dtype="<u4")[0])),
+        "index past the vertices": with_word(off["idx"], nverts),
+        "triangle id past the triangles": with_word(off["tris"] + 12, ntris),
+    }
+    for name, bad in cases.items():
+        assert bad != body, name
+        assert status_of(gpu_ctx, rehashed(bad)) == 1, name
+    H.Scene.deserialize(gpu_ctx, rehashed(body)).close()
