@@ -29,6 +29,8 @@ LAYOUT_BVH4 = 1
 LAYOUT_BVH4_FP16 = 2
 LAYOUT_AUTO = 3
 LAYOUT_AUTO_FP16 = 4
+TRI_MT = 0
+TRI_WATERTIGHT = 1
 
 DUMP_SIZES = 0
 DUMP_CODES_INPUT = 1
@@ -60,7 +62,8 @@ STATUS_NAMES = {
 EXPORTS = [
     "hrt_init", "hrt_shutdown", "hrt_ctx_stream", "hrt_scene_create", "hrt_scene_destroy",
     "hrt_scene_set_keep_build_arrays", "hrt_scene_clear_geom", "hrt_scene_clear_instances",
-    "hrt_add_triangles", "hrt_add_instance", "hrt_commit", "hrt_trace_nearest", "hrt_trace_any",
+    "hrt_add_triangles", "hrt_add_instance", "hrt_scene_set_triangle_test", "hrt_commit", "hrt_trace_nearest",
+    "hrt_trace_any",
     "hrt_trace_nearest_stats", "hrt_trace_any_stats", "hrt_query_nearest_one", "hrt_geom_count",
     "hrt_instance_count", "hrt_debug_dump", "hrt_last_error", "hrt_rtao_make_rays", "hrt_rtao_trace", "hrt_rtao_render", "hrt_eye_make_rays", "hrt_eye_render", "hrt_scene_serialize", "hrt_scene_deserialize",
     "hrt_trace_nearest_host", "hrt_trace_any_host", "hrt_update_triangles", "hrt_update_instance",
@@ -121,6 +124,7 @@ def lib():
         L.hrt_add_triangles.argtypes = [vp, vp, sz, sz, vp, sz, i32, C.POINTER(u32)]
         L.hrt_add_instance.argtypes = [vp, u32, vp, C.POINTER(u32)]
         L.hrt_commit.argtypes = [vp, i32, i32]
+        L.hrt_scene_set_triangle_test.argtypes = [vp, i32]
         L.hrt_trace_nearest.argtypes = [vp, vp, sz, vp, i32, vp]
         L.hrt_trace_any.argtypes = [vp, vp, sz, vp, vp]
         L.hrt_trace_nearest_stats.argtypes = [vp, vp, sz, vp, i32, vp, vp]
@@ -282,6 +286,10 @@ class Scene:
     def update_instance(self, inst_id: int, matrix_col_major: np.ndarray):
         m = np.ascontiguousarray(np.asarray(matrix_col_major, np.float32).reshape(16))
         _check(lib().hrt_update_instance(self._h, inst_id, _ptr(m)))
+
+    def set_triangle_test(self, test: int):
+        """TRI_MT (the reference's test) or TRI_WATERTIGHT, from the next commit on."""
+        _check(lib().hrt_scene_set_triangle_test(self._h, test))
 
     def commit(self, quality: int = 1, layout: int = LAYOUT_BVH2):
         _check(lib().hrt_commit(self._h, quality, layout))

@@ -135,6 +135,8 @@ struct hrt_scene {
   bool committed = false;
   bool keep_debug = false;
   int layout = HRT_LAYOUT_BVH2;
+  int tri_test = HRT_TRI_MT;  // takes effect at the next commit
+  int committed_tri_test = HRT_TRI_MT;
   // instanced scenes: top-level tree over the instance boxes and the
   // instance records (by instance id, then in the tree's leaf order)
   bool two_level = false;
@@ -232,6 +234,7 @@ hrt_status trace_common(hrt_scene* s, const float* rays, size_t n, void* out, in
   a.stats = stats;
   a.inst_id = inst;
   a.geom_id = s->two_level ? 0u : s->instances[inst].geom;
+  a.watertight = s->committed_tri_test == HRT_TRI_WATERTIGHT;
   hipStream_t hs = stream ? static_cast<hipStream_t>(stream) : s->ctx->stream;
   hipError_t e = hipSetDevice(s->ctx->device);
   if (e == hipSuccess) {
@@ -258,6 +261,7 @@ hrt_status scene_single_blas(hrt_scene* s, const Blas** out) {
   uint32_t inst = 0;
   return select_blas(s, out, &inst);
 }
+int scene_triangle_test(const hrt_scene* s) { return s->committed_tri_test; }
 }  // namespace hrt
 
 extern "C" {
@@ -480,6 +484,9 @@ hrt_status hrt_commit(hrt_scene* s, int quality, int layout) {
   s->committed = false;
   free_tlas(s);
   const bool single = s->instances.size() == 1 && s->instances[0].identity;
+  const bool wt = s->tri_test == HRT_TRI_WATERTIGHT;
+  if (wt && !single)
+    return fail(HRT_ERR_UNSUPPORTED, "the watertight triangle test traces scenes with one identity instance");
   hipError_t e = hipSetDevice(s->ctx->device);
   if (e != hipSuccess) return fail_hip(e, "hipSetDevice");
   if (!single) {
@@ -511,8 +518,24 @@ hrt_status hrt_commit(hrt_scene* s, int quality, int layout) {
       }
     }
   }
+  if (wt) {
+    hrt::Blas& b = s->geoms[s->instances[0].geom];
+    if ((e = hrt::blas_build_wt_tris(b, s->ctx->stream)) != hipSuccess ||
+        (b.any_tree && (e = hrt::blas_build_wt_tris(*b.any_tree, s->ctx->stream)) != hipSuccess) ||
+        (e = hipStreamSynchronize(s->ctx->stream)) != hipSuccess)
+      return fail_hip(e, "watertight triangle records");
+  }
   s->layout = layout;
+  s->committed_tri_test = s->tri_test;
   s->committed = true;
+  return HRT_OK;
+}
+
+hrt_status hrt_scene_set_triangle_test(hrt_scene* s, int test) {
+  if (!s) return fail(HRT_ERR_INVALID_ARG, "scene is null");
+  if (test != HRT_TRI_MT && test != HRT_TRI_WATERTIGHT) return fail(HRT_ERR_INVALID_ARG, "unknown triangle test");
+  s->tri_test = test;
+  s->committed = false;
   return HRT_OK;
 }
 
@@ -645,6 +668,7 @@ hipError_t rtao_pass(hrt_scene* s, const hrt::Blas* b, uint32_t inst, const hrt:
     a.stats = nullptr;
     a.inst_id = inst;
     a.geom_id = s->instances[inst].geom;
+    a.watertight = s->committed_tri_test == HRT_TRI_WATERTIGHT;
     e = hrt::rtao_make_rays(*b, prim, hits, npix, spp, tmax, seed, static_cast<hrt::Ray*>(rays), hs);
     if (e == hipSuccess) e = hrt::trace_blas(*b, a, hs);
     const hipError_t fe = hipFreeAsync(rays, hs);
@@ -656,7 +680,8 @@ hipError_t rtao_pass(hrt_scene* s, const hrt::Blas* b, uint32_t inst, const hrt:
   if ((e = hipMallocAsync(&frames, 32ull * npix, hs)) != hipSuccess) return e;
   hrt::AoFrame* f = static_cast<hrt::AoFrame*>(frames);
   e = hrt::rtao_frames(*b, prim, hits, npix, f, hs);
-  if (e == hipSuccess) e = hrt::trace_ao4(tree, fp16, f, npix, spp, tmax, seed, out, hs);
+  if (e == hipSuccess)
+    e = hrt::trace_ao4(tree, fp16, s->committed_tri_test == HRT_TRI_WATERTIGHT, f, npix, spp, tmax, seed, out, hs);
   const hipError_t fe = hipFreeAsync(frames, hs);
   return e == hipSuccess ? fe : e;
 }
@@ -1070,7 +1095,7 @@ hrt_status hrt_scene_serialize(hrt_scene* s, void* out, size_t capacity, size_t*
   SnapWriter w{static_cast<uint8_t*>(out), total};
   w.put(kSnapMagic, sizeof(kSnapMagic));
   w.val(kSnapVersion);
-  w.val(int32_t(s->layout));
+  w.val(int32_t(s->layout | (s->committed_tri_test << 8)));
   w.val(uint32_t(s->geoms.size()));
   w.val(uint32_t(s->instances.size()));
   hipError_t e = hipSetDevice(s->ctx->device);
@@ -1121,7 +1146,8 @@ hrt_status hrt_scene_deserialize(hrt_ctx* ctx, const void* data, size_t size, hr
     else st = hrt_add_instance(s, geom, m, nullptr);
   }
   if (st == HRT_OK && r.at != r.n) st = fail(HRT_ERR_INVALID_ARG, "snapshot has trailing bytes");
-  if (st == HRT_OK) st = hrt_commit(s, HRT_BUILD_MEDIUM, layout);
+  if (st == HRT_OK) st = hrt_scene_set_triangle_test(s, layout >> 8);
+  if (st == HRT_OK) st = hrt_commit(s, HRT_BUILD_MEDIUM, layout & 0xFF);
   if (st != HRT_OK) {
     hrt_scene_destroy(s);
     return st;

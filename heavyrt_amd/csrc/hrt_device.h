@@ -252,6 +252,114 @@ __device__ inline bool mt_test_lex(float3 o, float3 d, float tnear, const TriPre
   return false;
 }
 
+// ---------------------------------------------------------------------------
+// Triangle-test policies.  The traversal kernels take one as a template
+// parameter; a lane makes its ray's policy object once (per refill) and tests
+// every triangle of its leaves with it.
+//   MtTest  the reference's Moller-Trumbore with the 1e-6 barycentric slack
+//           (mt_eval) on TriPre records {A, id}, B-A, C-A.
+//   WtTest  the watertight test of Woop, Benthin and Wald ("Watertight
+//           Ray/Triangle Intersection", JCGT 2(1), 2013) in FP32 on records
+//           {A, id}, B, C (absolute vertices).  The ray's largest direction
+//           axis becomes z (lowest axis on ties; x and y swapped when that
+//           component is negative, which keeps the winding), the vertices
+//           are moved to the ray origin and sheared so the ray runs along z,
+//           and the three 2D edge functions U, V, W decide: all >= 0 or all
+//           <= 0 is a hit.  Two triangles that share an edge evaluate its
+//           function from the same sheared vertices with the operands
+//           exchanged, so they get exactly opposite values (or both zero):
+//           no ray passes between them, with no slack.  An edge value of
+//           zero counts as inside on both sides (the paper recomputes zeros
+//           in double; this FP32 form leaves such a ray to both triangles,
+//           which keeps the surface closed).  det = U + V + W; a zero det
+//           (a degenerate triangle, or the ray in its plane) is a miss;
+//           t = T / det with T = U Az + V Bz + W Cz, coords[0] = W / det
+//           (the weight of C) and coords[1] = V / det (the weight of B), the
+//           same meaning as the reference's u and v.
+// ---------------------------------------------------------------------------
+struct MtTest {
+  __device__ MtTest() {}
+  __device__ MtTest(float3, float3) {}
+  __device__ bool eval(float3 o, float3 d, float tnear, const TriPre& tri, float& t, float& u, float& v) const {
+    return mt_eval(o, d, tnear, tri, t, u, v);
+  }
+};
+
+__device__ inline float axis_of(float3 v, uint32_t k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+struct WtTest {
+  uint32_t axes;  // kx | ky << 2 | kz << 4
+  float sx, sy, sz;
+  __device__ WtTest() : axes(0), sx(0.f), sy(0.f), sz(0.f) {}
+  __device__ WtTest(float3, float3 d) {
+    const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+    const uint32_t kz = (ax >= ay) ? (ax >= az ? 0u : 2u) : (ay >= az ? 1u : 2u);
+    uint32_t kx = kz == 2u ? 0u : kz + 1u;
+    uint32_t ky = kx == 2u ? 0u : kx + 1u;
+    const float dz = axis_of(d, kz);
+    if (dz < 0.0f) {
+      const uint32_t k = kx;
+      kx = ky;
+      ky = k;
+    }
+    axes = kx | (ky << 2) | (kz << 4);
+    sx = axis_of(d, kx) / dz;
+    sy = axis_of(d, ky) / dz;
+    sz = 1.0f / dz;
+  }
+  __device__ bool eval(float3 o, float3, float tnear, const TriPre& tri, float& t, float& u, float& v) const {
+    const uint32_t kx = axes & 3u, ky = (axes >> 2) & 3u, kz = axes >> 4;
+    const float3 A = make_float3(tri.a.x - o.x, tri.a.y - o.y, tri.a.z - o.z);
+    const float3 B = make_float3(tri.e1.x - o.x, tri.e1.y - o.y, tri.e1.z - o.z);
+    const float3 C = make_float3(tri.e2.x - o.x, tri.e2.y - o.y, tri.e2.z - o.z);
+    const float Akz = axis_of(A, kz), Bkz = axis_of(B, kz), Ckz = axis_of(C, kz);
+    const float Ax = axis_of(A, kx) - sx * Akz, Ay = axis_of(A, ky) - sy * Akz;
+    const float Bx = axis_of(B, kx) - sx * Bkz, By = axis_of(B, ky) - sy * Bkz;
+    const float Cx = axis_of(C, kx) - sx * Ckz, Cy = axis_of(C, ky) - sy * Ckz;
+    const float U = Cx * By - Cy * Bx;
+    const float V = Ax * Cy - Ay * Cx;
+    const float W = Bx * Ay - By * Ax;
+    if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+    const float det = U + V + W;
+    if (det == 0.0f) return false;
+    const float T = U * (sz * Akz) + V * (sz * Bkz) + W * (sz * Ckz);
+    const float rcp = 1.0f / det;
+    t = T * rcp;
+    u = W * rcp;
+    v = V * rcp;
+    return t > tnear;
+  }
+};
+
+// The two acceptance rules with a triangle-test policy: the first strictly
+// closer hit (BVH2), and (t, primId) (BVH4).
+template <class Tri>
+__device__ inline bool tri_test(const Tri& k, float3 o, float3 d, float tnear, const TriPre& tri, HitRec& h) {
+  float t, u, v;
+  if (k.eval(o, d, tnear, tri, t, u, v) && t < h.t) {
+    h.t = t;
+    h.prim = __float_as_uint(tri.a.w);
+    h.u = u;
+    h.v = v;
+    return true;
+  }
+  return false;
+}
+
+template <class Tri>
+__device__ inline bool tri_test_lex(const Tri& k, float3 o, float3 d, float tnear, const TriPre& tri, HitRec& h) {
+  float t, u, v;
+  const uint32_t prim = __float_as_uint(tri.a.w);
+  if (k.eval(o, d, tnear, tri, t, u, v) && (t < h.t || (t == h.t && h.prim != kInvalid && prim < h.prim))) {
+    h.t = t;
+    h.prim = prim;
+    h.u = u;
+    h.v = v;
+    return true;
+  }
+  return false;
+}
+
 // Two-level rule: inside one instance the first strictly closer hit wins, as
 // in mt_test; a hit at the same distance in an instance with a smaller id
 // replaces one from a larger id, so across instances the answer does not

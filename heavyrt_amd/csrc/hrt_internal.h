@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/hrt.h"
@@ -27,6 +28,8 @@ hrt_status fail_hip(hipError_t e, const char* what);
 struct Blas;
 // The geometry tree of a committed scene with exactly one identity instance.
 hrt_status scene_single_blas(hrt_scene* s, const Blas** out);
+// The triangle test (HRT_TRI_*) of a committed scene's traces.
+int scene_triangle_test(const hrt_scene* s);
 
 // Device buffers of one bottom-level structure (one geometry).
 struct Blas {
@@ -40,6 +43,7 @@ struct Blas {
   uint32_t* idx = nullptr;    // input index buffer (3 * ntris)
   NodePair* pairs = nullptr;  // max(leaves - 1, 1) pairs
   TriPre* tris = nullptr;     // ntris, leaf order
+  TriPre* tris_wt = nullptr;  // ntris, leaf order, absolute vertices {A, id}, B, C (watertight test; on demand)
   uint32_t* perm = nullptr;   // leaf-order position -> original triangle id
   uint32_t* parent = nullptr; // internal node -> (parent << 1) | side (Karras step)
   // kept from the build for refits (hierarchy unchanged, boxes recomputed)
@@ -101,6 +105,9 @@ hipError_t blas_ploc(Blas& b, hipStream_t stream, bool* replaced);
 // Collapses the built BVH2 into FP32 and FP16 BVH4 nodes (idempotent);
 // synchronises the stream.
 hipError_t blas_build_bvh4(Blas& b, hipStream_t stream);
+// Leaf-order triangles with absolute vertices for the watertight test
+// (idempotent).  Asynchronous on `stream`.
+hipError_t blas_build_wt_tris(Blas& b, hipStream_t stream);
 
 // Top-level LBVH over instance boxes.  `t` is allocated with blas_alloc(t,
 // 2 * ninst, ninst, ...) and t.verts holds the world boxes (min, max per
@@ -125,6 +132,11 @@ hipError_t fill_words(uint32_t* p, uint64_t n, uint32_t v, hipStream_t stream);
 // Traversal ------------------------------------------------------------------
 enum HitFormat : int { kHitFull = 0, kHitCompact = 1 };
 
+// Workgroups of `kernel` (block threads, no dynamic LDS) resident on the
+// current device at once: CUs x occupancy.  Queried once per device and
+// kernel (a query per call shows up as launch latency on small batches).
+uint32_t resident_grid(const void* kernel, int block);
+
 struct TraceArgs {
   const Ray* rays;
   uint64_t n;
@@ -134,7 +146,14 @@ struct TraceArgs {
   uint32_t* stats;  // optional per-ray [pairs visited, triangles tested]
   uint32_t inst_id = 0;  // written into full hit records on a hit
   uint32_t geom_id = 0;
+  bool watertight = false;  // WtTest on Blas::tris_wt instead of the reference's MtTest
 };
+
+// The triangle records a test policy reads.
+template <class Tri>
+inline const TriPre* tri_records(const Blas& b) {
+  return std::is_same<Tri, WtTest>::value ? b.tris_wt : b.tris;
+}
 hipError_t trace_blas(const Blas& b, const TraceArgs& a, hipStream_t stream);
 
 // BVH4 traversal; fp16 selects the FP16 nodes.  Nearest hit by (t, primId).
@@ -155,8 +174,8 @@ hipError_t rtao_frames(const Blas& b, const Ray* prim_rays, const void* prim_hit
 // Any-hit of every AO sample of npix frames on b's BVH4 nodes (FP16 or FP32),
 // each ray made in registers exactly as rtao_make_rays writes it; out gets one
 // byte per sample.  npix * spp must stay below 2^32.
-hipError_t trace_ao4(const Blas& b, bool fp16, const AoFrame* frames, uint64_t npix, uint32_t spp, float tmax,
-                     uint64_t seed, uint8_t* out, hipStream_t stream);
+hipError_t trace_ao4(const Blas& b, bool fp16, bool watertight, const AoFrame* frames, uint64_t npix, uint32_t spp,
+                     float tmax, uint64_t seed, uint8_t* out, hipStream_t stream);
 
 // Eye rays of a pinhole camera, one per pixel, in SuperBlockIndex2DOpt order
 // (width and height multiples of 8); packed_xy (nullable) gets y << 16 | x.

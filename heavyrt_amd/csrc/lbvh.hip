@@ -507,6 +507,25 @@ __global__ __launch_bounds__(kBlock) void k_collapse4(const NodePair* __restrict
   q16[q] = h;
 }
 
+// Leaf-order triangle j with its absolute vertices for the watertight test:
+// {A, original id}, {B, 0}, {C, 0} (the vertices exactly as stored, so
+// triangles that share a vertex see the same values).
+__global__ __launch_bounds__(kBlock) void k_wt_tris(const float4* __restrict__ verts, const uint32_t* __restrict__ idx,
+                                                    const uint32_t* __restrict__ perm, uint32_t ntris,
+                                                    TriPre* __restrict__ out) {
+  const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+  if (j >= ntris) return;
+  const uint32_t t = perm[j];
+  const float4 A = verts[idx[3 * size_t(t) + 0]];
+  const float4 B = verts[idx[3 * size_t(t) + 1]];
+  const float4 C = verts[idx[3 * size_t(t) + 2]];
+  TriPre r;
+  r.a = make_float4(A.x, A.y, A.z, __uint_as_float(t));
+  r.e1 = make_float4(B.x, B.y, B.z, 0.0f);
+  r.e2 = make_float4(C.x, C.y, C.z, 0.0f);
+  out[j] = r;
+}
+
 inline uint32_t blocks_for(uint64_t n) { return uint32_t((n + kBlock - 1) / kBlock); }
 
 }  // namespace
@@ -543,7 +562,7 @@ void blas_free(Blas& b) {
     delete b.any_tree;
     b.any_tree = nullptr;
   }
-  void* ptrs[] = {b.leaf_run,   b.parent_leaf, b.node_range, b.meta,
+  void* ptrs[] = {b.leaf_run,   b.parent_leaf, b.node_range, b.meta,       b.tris_wt,
                   b.verts,      b.idx,        b.pairs,      b.tris,      b.perm,   b.parent,
                   b.quads32,    b.quads16,    b.codes_in,   b.codes_sorted, b.ids_sorted,
                   b.run_start,  b.ucodes,     b.karras_lr};
@@ -702,7 +721,10 @@ hipError_t blas_refit(Blas& b, hipStream_t s) {
   if (err == hipSuccess && hmeta[3] != 0u) err = hipErrorIllegalState;  // a refit list overflowed
   if (err != hipSuccess) return err;
   std::memcpy(b.box, hbox, sizeof(b.box));
-  // boxes changed: the BVH4 nodes and the any-hit tree are made again on the next commit
+  // boxes changed: the BVH4 nodes, the watertight records and the any-hit
+  // tree are made again on the next commit
+  if (b.tris_wt) (void)hipFree(b.tris_wt);
+  b.tris_wt = nullptr;
   if (b.quads32) (void)hipFree(b.quads32);
   if (b.quads16) (void)hipFree(b.quads16);
   b.quads32 = nullptr;
@@ -744,6 +766,13 @@ hipError_t tlas_build(Blas& t, const InstDev* by_id, InstDev* leaf_order, hipStr
   return lbvh_build(t, src, N, s, [&](const uint32_t* ids_sorted) {
     k_inst_order<<<blocks_for(N), kBlock, 0, s>>>(by_id, ids_sorted, N, leaf_order, t.perm);
   });
+}
+
+hipError_t blas_build_wt_tris(Blas& b, hipStream_t s) {
+  if (b.tris_wt) return hipSuccess;
+  HRT_CHECK(hipMalloc(&b.tris_wt, sizeof(TriPre) * size_t(b.ntris)));
+  k_wt_tris<<<blocks_for(b.ntris), kBlock, 0, s>>>(b.verts, b.idx, b.perm, b.ntris, b.tris_wt);
+  return hipGetLastError();
 }
 
 hipError_t blas_build_bvh4(Blas& b, hipStream_t s) {

@@ -465,6 +465,8 @@ hrt_status hrt_server_add_scene(hrt_server* sv, hrt_scene* scene, uint32_t* scen
   const hrt::Blas* b = nullptr;
   hrt_status st = hrt::scene_single_blas(scene, &b);
   if (st != HRT_OK) return st;
+  if (hrt::scene_triangle_test(scene) != HRT_TRI_MT)
+    return fail(HRT_ERR_UNSUPPORTED, "the server traces with the reference's triangle test");
   std::lock_guard<std::mutex> lk(sv->mu);
   sv->scenes.push_back(b);
   if (sceneId) *sceneId = uint32_t(sv->scenes.size() - 1);

@@ -29,7 +29,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <atomic>
+#include <mutex>
+#include <vector>
 
 #include "hrt_internal.h"
 
@@ -62,7 +63,7 @@ __global__ void k_reset_counter(uint32_t* c) { *c = 0u; }
 
 // Reference-order reference kernel (one ray per lane, no refill).  Kept for
 // the counting variant used to report the reference's traversal counters.
-template <bool kForceAny, bool kStats, int kLds>
+template <bool kForceAny, bool kStats, int kLds, class Tri = MtTest>
 __global__ __launch_bounds__(256) void k_trace_simple(const NodePair* __restrict__ pairs,
                                                       const TriPre* __restrict__ tris,
                                                       const Ray* __restrict__ rays, uint64_t n, void* out,
@@ -79,6 +80,7 @@ __global__ __launch_bounds__(256) void k_trace_simple(const NodePair* __restrict
   const bool any = kForceAny || (r.dir_far.w <= 0.0f);
   const float tfar = kForceAny ? fabsf(r.dir_far.w) : (any ? -r.dir_far.w : r.dir_far.w);
   const float3 inv = safe_inverse(d);
+  const Tri tt(o, d);
   HitRec h;
   h.t = tfar;
   h.prim = kInvalid;
@@ -110,7 +112,7 @@ __global__ __launch_bounds__(256) void k_trace_simple(const NodePair* __restrict
       if (cur != kInvalid) {
         const uint32_t s = leaf_start(cur), c = leaf_count(cur);
         if (kStats) ntests += c;
-        for (uint32_t k = s; k < s + c; k++) mt_test(o, d, tnear, tris[k], h);
+        for (uint32_t k = s; k < s + c; k++) tri_test(tt, o, d, tnear, tris[k], h);
         if (any && h.prim != kInvalid) break;
       }
     }
@@ -139,7 +141,7 @@ __global__ __launch_bounds__(256) void k_trace_simple(const NodePair* __restrict
 // Persistent wave64 traversal with ballot-based refill.
 //   kForceAny: every ray is an any-hit query (hrt_trace_any)
 //   kSpec:     speculative while-while traversal (any-hit only)
-template <bool kForceAny, bool kSpec>
+template <bool kForceAny, bool kSpec, class Tri = MtTest>
 __global__ __launch_bounds__(kTraceBlock) void k_trace_persistent(const NodePair* __restrict__ pairs,
                                                                   const TriPre* __restrict__ tris,
                                                                   const Ray* __restrict__ rays, uint64_t n,
@@ -159,6 +161,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_persistent(const NodePair
   float3 o = make_float3(0.f, 0.f, 0.f), d = o, inv = o;
   float tnear = 0.f;
   bool any = kForceAny;
+  Tri tt;
   HitRec h;
   h.t = 0.f;
   h.prim = kInvalid;
@@ -186,6 +189,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_persistent(const NodePair
           h.prim = kInvalid;
           h.u = h.v = 0.f;
           inv = safe_inverse(d);
+          tt = Tri(o, d);
           top = 0;
           cur = 0;
           leaf = 0;
@@ -233,7 +237,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_persistent(const NodePair
         if (active && leaf != 0) {
           const uint32_t s = leaf_start(leaf), c = leaf_count(leaf);
           for (uint32_t k = s; k < s + c; k++) {
-            if (mt_test(o, d, tnear, tris[k], h)) break;  // any-hit: first hit decides
+            if (tri_test(tt, o, d, tnear, tris[k], h)) break;  // any-hit: first hit decides
           }
           leaf = 0;
         }
@@ -260,7 +264,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_persistent(const NodePair
             }
           } else if (cur != kInvalid) {
             const uint32_t s = leaf_start(cur), c = leaf_count(cur);
-            for (uint32_t k = s; k < s + c; k++) mt_test(o, d, tnear, tris[k], h);
+            for (uint32_t k = s; k < s + c; k++) tri_test(tt, o, d, tnear, tris[k], h);
             cur = (top > 0) ? stk.pop(top) : kInvalid;
           }
           done = (cur == kInvalid) || (any && h.prim != kInvalid);
@@ -286,85 +290,93 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_persistent(const NodePair
   }
 }
 
-// Resident-wave count of the persistent kernel on the current device.
-int persistent_grid(const void* kernel) {
-  int dev = 0, cus = 0, per_cu = 0;
-  if (hipGetDevice(&dev) != hipSuccess) return 1024;
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, kTraceBlock, 0) != hipSuccess || per_cu <= 0)
-    per_cu = 8;
-  return cus * per_cu;
-}
-
-template <bool kAny, bool kSpec>
+template <bool kAny, bool kSpec, class Tri>
 hipError_t launch_persistent(const Blas& b, const TraceArgs& a, hipStream_t stream) {
-  const void* kfn = reinterpret_cast<const void*>(&k_trace_persistent<kAny, kSpec>);
+  const void* kfn = reinterpret_cast<const void*>(&k_trace_persistent<kAny, kSpec, Tri>);
   uint64_t want = (a.n + kTraceBlock - 1) / kTraceBlock;
-  const int grid = int(std::min<uint64_t>(want, uint64_t(persistent_grid(kfn))));
+  const int grid = int(std::min<uint64_t>(want, resident_grid(kfn, kTraceBlock)));
   // queue counter + spill area for stack entries beyond the LDS part
   const size_t spill_words = size_t(grid) * kTraceBlock * (kStackDepth - kShortStack);
   uint32_t* buf = nullptr;
   hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&buf), sizeof(uint32_t) * (64 + spill_words), stream);
   if (e != hipSuccess) return e;
   k_reset_counter<<<1, 1, 0, stream>>>(buf);
-  k_trace_persistent<kAny, kSpec><<<grid, kTraceBlock, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format, make_uint2(a.inst_id, a.geom_id),
-                                                                   buf, buf + 64);
+  k_trace_persistent<kAny, kSpec, Tri><<<grid, kTraceBlock, 0, stream>>>(
+      b.pairs, tri_records<Tri>(b), a.rays, a.n, a.out, a.format, make_uint2(a.inst_id, a.geom_id), buf, buf + 64);
   e = hipGetLastError();
   hipError_t f = hipFreeAsync(buf, stream);
   return e != hipSuccess ? e : f;
 }
 
-template <int kLds>
+template <int kLds, class Tri>
 hipError_t launch_simple(const Blas& b, const TraceArgs& a, uint32_t blocks, uint32_t* spill, hipStream_t stream) {
   const uint2 ids = make_uint2(a.inst_id, a.geom_id);
+  const TriPre* tris = tri_records<Tri>(b);
   if (a.stats && a.anyhit)
-    k_trace_simple<true, true, kLds><<<blocks, 256, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format, ids,
-                                                                  a.stats, spill);
+    k_trace_simple<true, true, kLds, Tri><<<blocks, 256, 0, stream>>>(b.pairs, tris, a.rays, a.n, a.out, a.format,
+                                                                       ids, a.stats, spill);
   else if (a.stats)
-    k_trace_simple<false, true, kLds><<<blocks, 256, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format, ids,
-                                                                   a.stats, spill);
+    k_trace_simple<false, true, kLds, Tri><<<blocks, 256, 0, stream>>>(b.pairs, tris, a.rays, a.n, a.out, a.format,
+                                                                        ids, a.stats, spill);
   else
-    k_trace_simple<false, false, kLds><<<blocks, 256, 0, stream>>>(b.pairs, b.tris, a.rays, a.n, a.out, a.format,
-                                                                    ids, nullptr, spill);
+    k_trace_simple<false, false, kLds, Tri><<<blocks, 256, 0, stream>>>(b.pairs, tris, a.rays, a.n, a.out,
+                                                                         a.format, ids, nullptr, spill);
   return hipGetLastError();
 }
 
-// Blocks of the full-LDS-stack kernel resident on the current device at
-// once, queried once per device (a per-call query shows up as launch latency
-// on small batches).
-uint32_t resident_full_stack_blocks() {
-  static std::atomic<uint32_t> cache[64];
-  int dev = 0, cus = 0, per_cu = 0;
-  if (hipGetDevice(&dev) != hipSuccess || dev < 0) return 0;
-  if (dev < 64) {
-    const uint32_t v = cache[dev].load(std::memory_order_relaxed);
-    if (v) return v;
-  }
-  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) return 0;
-  const void* kfn = reinterpret_cast<const void*>(&k_trace_simple<false, false, kStackDepth>);
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, 256, 0) != hipSuccess || per_cu <= 0) return 0;
-  const uint32_t v = uint32_t(cus) * uint32_t(per_cu);
-  if (dev < 64) cache[dev].store(v, std::memory_order_relaxed);
-  return v;
-}
-
-}  // namespace
-
-hipError_t trace_blas(const Blas& b, const TraceArgs& a, hipStream_t stream) {
-  if (a.n == 0) return hipSuccess;
-  if (a.anyhit && !a.stats) return launch_persistent<true, true>(b, a, stream);
+template <class Tri>
+hipError_t trace_blas_with(const Blas& b, const TraceArgs& a, hipStream_t stream) {
+  if (a.anyhit && !a.stats) return launch_persistent<true, true, Tri>(b, a, stream);
   // One ray per lane: the counting variant (the reference's order) and
   // nearest hits (rays that arrive in screen order stay coherent within a
   // wave, which the refilling kernel gives up).
   const uint32_t blocks = uint32_t((a.n + 255) / 256);
-  if (blocks <= resident_full_stack_blocks()) return launch_simple<kStackDepth>(b, a, blocks, nullptr, stream);
+  if (blocks <= resident_grid(reinterpret_cast<const void*>(&k_trace_simple<false, false, kStackDepth, Tri>), 256))
+    return launch_simple<kStackDepth, Tri>(b, a, blocks, nullptr, stream);
   uint32_t* spill = nullptr;
   hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&spill),
                                 sizeof(uint32_t) * size_t(blocks) * 256 * (kStackDepth - kSimpleLds), stream);
   if (e != hipSuccess) return e;
-  e = launch_simple<kSimpleLds>(b, a, blocks, spill, stream);
+  e = launch_simple<kSimpleLds, Tri>(b, a, blocks, spill, stream);
   const hipError_t f = hipFreeAsync(spill, stream);
   return e != hipSuccess ? e : f;
+}
+
+}  // namespace
+
+uint32_t resident_grid(const void* kernel, int block) {
+  struct Entry {
+    int dev;
+    const void* kernel;
+    int block;
+    uint32_t grid;
+  };
+  static std::mutex mu;
+  static std::vector<Entry> cache;
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) return 1024;
+  {
+    std::lock_guard<std::mutex> lock(mu);
+    for (const Entry& e : cache)
+      if (e.dev == dev && e.kernel == kernel && e.block == block) return e.grid;
+  }
+  int cus = 0, per_cu = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) cus = 256;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess || per_cu <= 0)
+    per_cu = 1;
+  const uint32_t grid = uint32_t(cus) * uint32_t(per_cu);
+  std::lock_guard<std::mutex> lock(mu);
+  cache.push_back(Entry{dev, kernel, block, grid});
+  return grid;
+}
+
+hipError_t trace_blas(const Blas& b, const TraceArgs& a, hipStream_t stream) {
+  if (a.n == 0) return hipSuccess;
+  if (a.watertight) {
+    if (!b.tris_wt) return hipErrorInvalidValue;
+    return trace_blas_with<WtTest>(b, a, stream);
+  }
+  return trace_blas_with<MtTest>(b, a, stream);
 }
 
 }  // namespace hrt

@@ -31,6 +31,22 @@ constexpr int kRefillAt4 = 40;
 // to global memory).  Deep, incoherent nearest-hit walks push often and
 // measured slower with spills, so BVH4 keeps its whole stack in LDS.
 constexpr int kSimpleLds4 = 48;
+// LDS stack entries per lane of the refilling nearest-hit kernel.
+#ifndef HRT_NEAREST4_LDS
+#define HRT_NEAREST4_LDS 32
+#endif
+constexpr int kNearestLds4 = HRT_NEAREST4_LDS;
+// Nearest-hit batches larger than one resident grid: the refilling kernel
+// (1) or one ray per lane (0).
+#ifndef HRT_NEAREST4_PERSISTENT
+#define HRT_NEAREST4_PERSISTENT 1
+#endif
+// Nearest-hit batches of at least this many rays are sorted into coherent
+// waves first (0: never).
+#ifndef HRT_SORT_RAYS4_MIN
+#define HRT_SORT_RAYS4_MIN 0
+#endif
+constexpr uint64_t kSortRaysMin4 = HRT_SORT_RAYS4_MIN;
 // The speculative any-hit kernels leave the node loop for the leaf phase once
 // this many lanes or fewer still look for a leaf (0 would wait for the last).
 constexpr uint32_t kLeafPhaseAt = 8;
@@ -129,16 +145,21 @@ __device__ inline void write_hit(void* out, uint64_t i, int format, uint2 ids, c
 
 // One ray per lane, the reference's order step by step (also the counting
 // variant: quads visited, triangles in visited leaves).
-template <bool kForceAny, bool kFp16, bool kStats>
+template <bool kForceAny, bool kFp16, bool kStats, class Tri = MtTest>
 __global__ __launch_bounds__(kBlock4) void k_trace4_simple(const void* __restrict__ quads,
                                                            const TriPre* __restrict__ tris,
                                                            const Ray* __restrict__ rays, uint64_t n, void* out,
                                                            int format, uint2 ids, uint32_t* __restrict__ stats,
-                                                           uint32_t* __restrict__ spill) {
+                                                           uint32_t* __restrict__ spill,
+                                                           const uint32_t* const* __restrict__ order_slot) {
   __shared__ uint32_t stack_mem[kSimpleLds4 * kBlock4];
-  const uint64_t i = uint64_t(blockIdx.x) * kBlock4 + threadIdx.x;
+  const uint64_t lane_id = uint64_t(blockIdx.x) * kBlock4 + threadIdx.x;
+  if (lane_id >= n) return;
+  const uint32_t* __restrict__ order = order_slot ? *order_slot : nullptr;
+  // thread k traces ray order[k] when the batch was sorted into coherent waves
+  const uint64_t i = order ? uint64_t(order[lane_id]) : lane_id;
   if (i >= n) return;
-  const ShortStack<kSimpleLds4, kBlock4> stk{stack_mem + threadIdx.x, spill + i, gridDim.x * kBlock4};
+  const ShortStack<kSimpleLds4, kBlock4> stk{stack_mem + threadIdx.x, spill + lane_id, gridDim.x * kBlock4};
   const Ray r = rays[i];
   const float3 o = make_float3(r.pos_near.x, r.pos_near.y, r.pos_near.z);
   const float3 d = make_float3(r.dir_far.x, r.dir_far.y, r.dir_far.z);
@@ -147,6 +168,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_simple(const void* __restric
   const float3 inv = safe_inverse(d);
   const float3 oi = neg_o_inv(o, inv);
   const bool fz = fused_ok(oi);
+  const Tri tt(o, d);
   HitRec h;
   h.t = kForceAny ? fabsf(r.dir_far.w) : (any ? -r.dir_far.w : r.dir_far.w);
   h.prim = kInvalid;
@@ -171,7 +193,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_simple(const void* __restric
       const uint32_t s = leaf_start(cur), c = leaf_count(cur);
       if (kStats) ntests += c;
       for (uint32_t k = s; k < s + c; k++)
-        if (mt_test_lex(o, d, tnear, tris[k], h) && any) break;
+        if (tri_test_lex(tt, o, d, tnear, tris[k], h) && any) break;
       pop = true;
       if (any && h.prim != kInvalid) break;
     }
@@ -210,7 +232,7 @@ struct AoRays {
   __device__ Ray ray(uint64_t i) const { return ao_ray(frames[uint32_t(i) / spp], i, tmax, seed); }
 };
 
-template <bool kFp16, class Src>
+template <bool kFp16, class Src, class Tri = MtTest>
 __global__ __launch_bounds__(kBlock4, 8) void k_trace4_any_persistent(const void* __restrict__ quads,
                                                                    const TriPre* __restrict__ tris, Src src,
                                                                    uint64_t n, uint8_t* __restrict__ out,
@@ -226,6 +248,7 @@ __global__ __launch_bounds__(kBlock4, 8) void k_trace4_any_persistent(const void
   float3 o = make_float3(0.f, 0.f, 0.f), d = o, inv = o, oi = o;
   bool fz = true;
   float tnear = 0.f;
+  Tri tt;
   HitRec h;
   h.t = 0.f;
   h.prim = kInvalid;
@@ -252,6 +275,7 @@ __global__ __launch_bounds__(kBlock4, 8) void k_trace4_any_persistent(const void
           inv = safe_inverse(d);
           oi = neg_o_inv(o, inv);
           fz = fused_ok(oi);
+          tt = Tri(o, d);
           top = 0;
           cur = (h.t > tnear) ? 0u : kInvalid;
           leaf = 0;
@@ -293,11 +317,11 @@ __global__ __launch_bounds__(kBlock4, 8) void k_trace4_any_persistent(const void
       if (active && leaf != 0) {
         const uint32_t s = leaf_start(leaf), c = leaf_count(leaf);
         bool hit = false;
-        for (uint32_t k = s; k < s + c && !hit; k++) hit = mt_test_lex(o, d, tnear, tris[k], h);
+        for (uint32_t k = s; k < s + c && !hit; k++) hit = tri_test_lex(tt, o, d, tnear, tris[k], h);
         if (!hit && leaf2 != 0) {
           const uint32_t s2 = leaf_start(leaf2), c2 = leaf_count(leaf2);
           for (uint32_t k = s2; k < s2 + c2; k++)
-            if (mt_test_lex(o, d, tnear, tris[k], h)) break;
+            if (tri_test_lex(tt, o, d, tnear, tris[k], h)) break;
         }
         leaf = 0;
         leaf2 = 0;
@@ -312,15 +336,117 @@ __global__ __launch_bounds__(kBlock4, 8) void k_trace4_any_persistent(const void
   }
 }
 
-template <bool kFp16, class Src>
+// Persistent nearest-hit over BVH4 nodes with ballot-based refill from a
+// wave-local ray pool.  One ray per lane keeps a wave until its slowest ray
+// finishes; on incoherent rays (hundreds of node visits, widely spread)
+// refilling idle lanes keeps the waves full instead.  The walk is the
+// speculative while-while of the any-hit kernel: a lane parks the first leaf
+// it reaches and keeps descending while the wave still searches, then the
+// wave tests its parked leaves together.  The nearest hit under the
+// (t, primId) rule does not depend on the order nodes and leaves are visited
+// in: boxes are culled against the current best t, which only decreases, and
+// a box is kept when its entry distance equals it, so no box holding a
+// triangle at or below the final t is ever skipped.  The answer is the one
+// k_trace4_simple gives.  A ray with tFar <= 0 (any-hit by the ray's own
+// convention) stops at its first hit, as there.
+// kLds stack entries per lane live in LDS, the rest (to kStackDepth4) in a
+// per-lane global spill area.
+template <bool kFp16, int kLds, class Tri = MtTest>
+__global__ __launch_bounds__(kBlock4) void k_trace4_nearest_persistent(const void* __restrict__ quads,
+                                                                       const TriPre* __restrict__ tris,
+                                                                       const Ray* __restrict__ rays, uint64_t n,
+                                                                       void* out, int format, uint2 ids,
+                                                                       uint32_t* __restrict__ queue,
+                                                                       uint32_t* __restrict__ spill,
+                                                                       const uint32_t* const* __restrict__ order_slot) {
+  __shared__ uint32_t stack_mem[kLds * kBlock4];
+  const uint32_t* __restrict__ order = order_slot ? *order_slot : nullptr;
+  const int lane = threadIdx.x & (kWave - 1);
+  const ShortStack<kLds, kBlock4> stk{stack_mem + threadIdx.x, spill + blockIdx.x * kBlock4 + threadIdx.x,
+                                      gridDim.x * kBlock4};
+  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
+  bool active = false, any = false;
+  uint64_t rid = 0;
+  float3 o = make_float3(0.f, 0.f, 0.f), d = o, inv = o;
+  float tnear = 0.f;
+  Tri tt;
+  HitRec h;
+  h.t = 0.f;
+  h.prim = kInvalid;
+  h.u = h.v = 0.f;
+  int top = 0;
+  uint32_t cur = kInvalid;
+  uint32_t leaf = 0;  // parked leaf, 0 = none
+  RayPool pool;
+
+  while (true) {
+    const uint64_t idle = __ballot(!active);
+    const uint32_t nidle = __popcll(idle);
+    if (!pool.drained() && (nidle >= kRefillAt4 || nidle == kWave)) {
+      const uint64_t slot = pool.take<kClaim>(nidle, __popcll(idle & lt_mask), queue, n);
+      const uint64_t idx = (order && slot < n) ? uint64_t(order[slot]) : slot;
+      if (!active && idx < n) {
+        rid = idx;
+        const Ray r = rays[idx];
+        o = make_float3(r.pos_near.x, r.pos_near.y, r.pos_near.z);
+        d = make_float3(r.dir_far.x, r.dir_far.y, r.dir_far.z);
+        tnear = r.pos_near.w;
+        any = r.dir_far.w <= 0.0f;
+        h.t = any ? -r.dir_far.w : r.dir_far.w;
+        h.prim = kInvalid;
+        h.u = h.v = 0.f;
+        inv = safe_inverse(d);
+        tt = Tri(o, d);
+        top = 0;
+        // an empty interval admits no hit: t > tNear and t < tFar
+        cur = (h.t > tnear) ? 0u : kInvalid;
+        leaf = 0;
+        active = true;
+      }
+    }
+    if (__ballot(active) == 0) break;
+
+    while (true) {
+      while (true) {
+        if (active && leaf == 0 && (cur & kLeafBit) != 0 && cur != kInvalid) {
+          leaf = cur;
+          cur = (top > 0) ? stk.pop(top) : kInvalid;
+        }
+        const bool searching = active && leaf == 0 && (cur & kLeafBit) == 0;
+        const uint32_t nsearch = __popcll(__ballot(searching));
+        if (nsearch == 0 || (nsearch <= kLeafPhaseAt && __any(leaf != 0))) break;
+        if (active && (cur & kLeafBit) == 0) {
+          float tm[4];
+          uint32_t ch[4];
+          visit_quad<kFp16>(quads, cur, o, inv, o, true, tnear, h.t, tm, ch);
+          if (tm[3] < kMaxFloat) stk.push(top, ch[3]);
+          if (tm[2] < kMaxFloat) stk.push(top, ch[2]);
+          if (tm[1] < kMaxFloat) stk.push(top, ch[1]);
+          if (tm[0] < kMaxFloat) cur = ch[0];
+          else cur = (top > 0) ? stk.pop(top) : kInvalid;
+        }
+      }
+      if (active && leaf != 0) {
+        const uint32_t s = leaf_start(leaf), c = leaf_count(leaf);
+        for (uint32_t k = s; k < s + c; k++)
+          if (tri_test_lex(tt, o, d, tnear, tris[k], h) && any) break;
+        leaf = 0;
+      }
+      if (active && ((any && h.prim != kInvalid) || (leaf == 0 && cur == kInvalid))) {
+        write_hit<false>(out, rid, format, ids, h);
+        active = false;
+      }
+      const uint32_t nact = __popcll(__ballot(active));
+      if (nact == 0 || (!pool.drained() && kWave - nact >= kRefillAt4)) break;
+    }
+  }
+}
+
+template <bool kFp16, class Src, class Tri>
 hipError_t launch_any4(const Blas& b, Src src, uint64_t n, uint8_t* out, hipStream_t stream) {
-  int dev = 0, cus = 256, per_cu = 0;
-  (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  const void* kfn = reinterpret_cast<const void*>(&k_trace4_any_persistent<kFp16, Src>);
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kfn, kBlock4, 0) != hipSuccess || per_cu <= 0) per_cu = 8;
+  const void* kfn = reinterpret_cast<const void*>(&k_trace4_any_persistent<kFp16, Src, Tri>);
   const uint64_t want = (n + kBlock4 - 1) / kBlock4;
-  const int grid = int(std::min<uint64_t>(want, uint64_t(std::max(cus, 1)) * uint64_t(per_cu)));
+  const int grid = int(std::min<uint64_t>(want, resident_grid(kfn, kBlock4)));
   const void* q = kFp16 ? static_cast<const void*>(b.quads16) : static_cast<const void*>(b.quads32);
   // queue counter + spill area for stack entries beyond the LDS part
   const size_t spill_words = size_t(grid) * kBlock4 * (kStackDepth4 - kShortStack);
@@ -328,14 +454,84 @@ hipError_t launch_any4(const Blas& b, Src src, uint64_t n, uint8_t* out, hipStre
   hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&buf), sizeof(uint32_t) * (64 + spill_words), stream);
   if (e != hipSuccess) return e;
   k_reset_counter4<<<1, 1, 0, stream>>>(buf);
-  k_trace4_any_persistent<kFp16, Src><<<grid, kBlock4, 0, stream>>>(q, b.tris, src, n, out, buf, buf + 64);
+  k_trace4_any_persistent<kFp16, Src, Tri><<<grid, kBlock4, 0, stream>>>(q, tri_records<Tri>(b), src, n, out, buf,
+                                                                          buf + 64);
   e = hipGetLastError();
   hipError_t f = hipFreeAsync(buf, stream);
   return e != hipSuccess ? e : f;
 }
 
-template <bool kFp16>
-hipError_t launch_simple4(const Blas& b, const TraceArgs& a, hipStream_t stream) {
+// Sort key of a ray for coherent waves: 7 bits of direction (the cube face of
+// the largest |d| component, 6 faces, times a 4x4 grid of the other two
+// components divided by it) above a 24-bit Morton code of the origin on a
+// 256^3 grid over the scene box (clamped).  Rays with similar directions and
+// nearby origins then share waves, so their node and triangle fetches hit the
+// same cache lines.  A NaN component lands in cell 0.
+__device__ inline uint32_t spread8(uint32_t x) {
+  x = (x | (x << 8)) & 0x0000F00Fu;
+  x = (x | (x << 4)) & 0x000C30C3u;
+  x = (x | (x << 2)) & 0x00249249u;
+  return x;
+}
+
+__global__ __launch_bounds__(256) void k_ray_keys(const Ray* __restrict__ rays, uint32_t n, float3 lo, float3 scale,
+                                                  uint32_t* __restrict__ keys, uint32_t* __restrict__ ids) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const Ray r = rays[i];
+  const float dx = r.dir_far.x, dy = r.dir_far.y, dz = r.dir_far.z;
+  const float ax = fabsf(dx), ay = fabsf(dy), az = fabsf(dz);
+  uint32_t face;
+  float m, p, q;
+  if (ax >= ay && ax >= az) {
+    face = dx < 0.0f ? 1u : 0u;
+    m = ax, p = dy, q = dz;
+  } else if (ay >= az) {
+    face = dy < 0.0f ? 3u : 2u;
+    m = ay, p = dx, q = dz;
+  } else {
+    face = dz < 0.0f ? 5u : 4u;
+    m = az, p = dx, q = dy;
+  }
+  const float inv_m = m > 0.0f ? 1.0f / m : 0.0f;
+  const uint32_t cp = uint32_t(fminf(fmaxf((p * inv_m + 1.0f) * 2.0f, 0.0f), 3.0f));
+  const uint32_t cq = uint32_t(fminf(fmaxf((q * inv_m + 1.0f) * 2.0f, 0.0f), 3.0f));
+  const uint32_t cx = uint32_t(fminf(fmaxf((r.pos_near.x - lo.x) * scale.x, 0.0f), 255.0f));
+  const uint32_t cy = uint32_t(fminf(fmaxf((r.pos_near.y - lo.y) * scale.y, 0.0f), 255.0f));
+  const uint32_t cz = uint32_t(fminf(fmaxf((r.pos_near.z - lo.z) * scale.z, 0.0f), 255.0f));
+  keys[i] = ((face * 16u + cp * 4u + cq) << 24) | spread8(cx) | (spread8(cy) << 1) | (spread8(cz) << 2);
+  ids[i] = i;
+}
+
+// The order the trace kernel reads: the sorted one, or none (unsorted) when
+// the sort failed (its look-back gave up: *err != 0) and left it incomplete.
+__global__ void k_order_or_null(const uint32_t* __restrict__ err, const uint32_t* order, const uint32_t** slot) {
+  *slot = (*err == 0u) ? order : nullptr;
+}
+
+template <bool kFp16, class Tri>
+hipError_t launch_nearest4(const Blas& b, const TraceArgs& a, const uint32_t* const* order_slot,
+                           hipStream_t stream) {
+  const void* kfn = reinterpret_cast<const void*>(&k_trace4_nearest_persistent<kFp16, kNearestLds4, Tri>);
+  const uint64_t want = (a.n + kBlock4 - 1) / kBlock4;
+  const int grid = int(std::min<uint64_t>(want, resident_grid(kfn, kBlock4)));
+  const void* q = kFp16 ? static_cast<const void*>(b.quads16) : static_cast<const void*>(b.quads32);
+  // queue counter + spill area for stack entries beyond the LDS part
+  const size_t spill_words = size_t(grid) * kBlock4 * (kStackDepth4 - kNearestLds4);
+  uint32_t* buf = nullptr;
+  hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&buf), sizeof(uint32_t) * (64 + spill_words), stream);
+  if (e != hipSuccess) return e;
+  k_reset_counter4<<<1, 1, 0, stream>>>(buf);
+  k_trace4_nearest_persistent<kFp16, kNearestLds4, Tri><<<grid, kBlock4, 0, stream>>>(
+      q, tri_records<Tri>(b), a.rays, a.n, a.out, a.format, make_uint2(a.inst_id, a.geom_id), buf, buf + 64, order_slot);
+  e = hipGetLastError();
+  hipError_t f = hipFreeAsync(buf, stream);
+  return e != hipSuccess ? e : f;
+}
+
+template <bool kFp16, class Tri>
+hipError_t launch_simple4(const Blas& b, const TraceArgs& a, const uint32_t* const* order_slot,
+                          hipStream_t stream) {
   const uint32_t blocks = uint32_t((a.n + kBlock4 - 1) / kBlock4);
   const void* q = kFp16 ? static_cast<const void*>(b.quads16) : static_cast<const void*>(b.quads32);
   // stack entries beyond the LDS part, per lane (none while LDS holds them all)
@@ -347,46 +543,109 @@ hipError_t launch_simple4(const Blas& b, const TraceArgs& a, hipStream_t stream)
           hipSuccess)
     return e;
   const uint2 ids = make_uint2(a.inst_id, a.geom_id);
+  const TriPre* tris = tri_records<Tri>(b);
   if (a.anyhit) {
     if (a.stats)
-      k_trace4_simple<true, kFp16, true><<<blocks, kBlock4, 0, stream>>>(q, b.tris, a.rays, a.n, a.out, a.format, ids,
-                                                                        a.stats, spill);
+      k_trace4_simple<true, kFp16, true, Tri><<<blocks, kBlock4, 0, stream>>>(q, tris, a.rays, a.n, a.out, a.format,
+                                                                             ids, a.stats, spill, nullptr);
     else
-      k_trace4_simple<true, kFp16, false><<<blocks, kBlock4, 0, stream>>>(q, b.tris, a.rays, a.n, a.out, a.format,
-                                                                         ids, nullptr, spill);
+      k_trace4_simple<true, kFp16, false, Tri><<<blocks, kBlock4, 0, stream>>>(q, tris, a.rays, a.n, a.out, a.format,
+                                                                              ids, nullptr, spill, nullptr);
   } else {
     if (a.stats)
-      k_trace4_simple<false, kFp16, true><<<blocks, kBlock4, 0, stream>>>(q, b.tris, a.rays, a.n, a.out, a.format,
-                                                                         ids, a.stats, spill);
+      k_trace4_simple<false, kFp16, true, Tri><<<blocks, kBlock4, 0, stream>>>(q, tris, a.rays, a.n, a.out, a.format,
+                                                                              ids, a.stats, spill, nullptr);
     else
-      k_trace4_simple<false, kFp16, false><<<blocks, kBlock4, 0, stream>>>(q, b.tris, a.rays, a.n, a.out,
-                                                                          a.format, ids, nullptr, spill);
+      k_trace4_simple<false, kFp16, false, Tri><<<blocks, kBlock4, 0, stream>>>(q, tris, a.rays, a.n, a.out,
+                                                                               a.format, ids, nullptr, spill, order_slot);
   }
   e = hipGetLastError();
   const hipError_t f = spill ? hipFreeAsync(spill, stream) : hipSuccess;
   return e != hipSuccess ? e : f;
 }
 
+// Nearest hits of a large batch: the rays are sorted into coherent waves
+// first (k_ray_keys, then the onesweep radix sort of the engine's builder),
+// and each thread traces the ray the order gives it; the answers land at the
+// rays' own indices, so the output is what an unsorted trace writes.
+template <bool kFp16, class Tri>
+hipError_t launch_nearest4_sorted(const Blas& b, const TraceArgs& a, hipStream_t stream) {
+  const uint64_t n = a.n;
+  const size_t sort_words = radix_sort_scratch_words(n, 4);
+  // keys, ids (twice), radix scratch, the error word and the order slot
+  const size_t words = 4 * n + sort_words + 64;
+  uint32_t* base = nullptr;
+  hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&base), sizeof(uint32_t) * words, stream);
+  if (e != hipSuccess) return e;
+  // base[0..1]: the order slot (8-byte aligned), base[2]: the sort's error word
+  const uint32_t** slot = reinterpret_cast<const uint32_t**>(base);
+  uint32_t* err = base + 2;
+  uint32_t *ka = base + 64, *va = ka + n, *kb = ka + 2 * n, *vb = ka + 3 * n;
+  uint32_t* scratch = ka + 4 * n;
+  const float3 lo = make_float3(b.box[0], b.box[1], b.box[2]);
+  const float ex = b.box[4] - b.box[0], ey = b.box[5] - b.box[1], ez = b.box[6] - b.box[2];
+  const float3 scale = make_float3(ex > 0.f ? 256.f / ex : 0.f, ey > 0.f ? 256.f / ey : 0.f, ez > 0.f ? 256.f / ez : 0.f);
+  e = fill_words(err, 1, 0u, stream);
+  if (e == hipSuccess) {
+    k_ray_keys<<<uint32_t((n + 255) / 256), 256, 0, stream>>>(a.rays, uint32_t(n), lo, scale, ka, va);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = radix_sort(ka, va, kb, vb, n, 0, 4, scratch, err, stream);
+  const uint32_t* order = va;  // four passes: the sorted pairs end in (ka, va)
+  if (e == hipSuccess) {
+    k_order_or_null<<<1, 1, 0, stream>>>(err, order, slot);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess)
+    e = HRT_NEAREST4_PERSISTENT ? launch_nearest4<kFp16, Tri>(b, a, slot, stream)
+                                : launch_simple4<kFp16, Tri>(b, a, slot, stream);
+  const hipError_t f = hipFreeAsync(base, stream);
+  return e != hipSuccess ? e : f;
+}
+
+template <class Tri>
+hipError_t trace_blas4_with(const Blas& b, const TraceArgs& a, bool fp16, hipStream_t stream) {
+  if (a.anyhit && !a.stats) {
+    const StoredRays src{a.rays};
+    uint8_t* out = static_cast<uint8_t*>(a.out);
+    return fp16 ? launch_any4<true, StoredRays, Tri>(b, src, a.n, out, stream)
+                : launch_any4<false, StoredRays, Tri>(b, src, a.n, out, stream);
+  }
+  if (!a.stats && !a.anyhit && kSortRaysMin4 > 0 && a.n >= kSortRaysMin4)
+    return fp16 ? launch_nearest4_sorted<true, Tri>(b, a, stream) : launch_nearest4_sorted<false, Tri>(b, a, stream);
+  // one ray per lane for the counting variant and for batches whose whole
+  // grid is resident at once (nothing to refill); refilling waves otherwise
+  const uint64_t blocks = (a.n + kBlock4 - 1) / kBlock4;
+  const void* simple = fp16 ? reinterpret_cast<const void*>(&k_trace4_simple<false, true, false, Tri>)
+                            : reinterpret_cast<const void*>(&k_trace4_simple<false, false, false, Tri>);
+  if (!HRT_NEAREST4_PERSISTENT || a.stats || a.anyhit || blocks <= resident_grid(simple, kBlock4))
+    return fp16 ? launch_simple4<true, Tri>(b, a, nullptr, stream) : launch_simple4<false, Tri>(b, a, nullptr, stream);
+  return fp16 ? launch_nearest4<true, Tri>(b, a, nullptr, stream) : launch_nearest4<false, Tri>(b, a, nullptr, stream);
+}
+
 }  // namespace
 
-hipError_t trace_ao4(const Blas& b, bool fp16, const AoFrame* frames, uint64_t npix, uint32_t spp, float tmax,
-                     uint64_t seed, uint8_t* out, hipStream_t stream) {
+hipError_t trace_ao4(const Blas& b, bool fp16, bool watertight, const AoFrame* frames, uint64_t npix, uint32_t spp,
+                     float tmax, uint64_t seed, uint8_t* out, hipStream_t stream) {
   const uint64_t n = npix * spp;
   if (n == 0) return hipSuccess;
-  if (!b.quads32 || !b.quads16 || n > 0xFFFFFFFFull) return hipErrorInvalidValue;
+  if (!b.quads32 || !b.quads16 || n > 0xFFFFFFFFull || (watertight && !b.tris_wt)) return hipErrorInvalidValue;
   const AoRays src{frames, spp, tmax, seed};
-  return fp16 ? launch_any4<true>(b, src, n, out, stream) : launch_any4<false>(b, src, n, out, stream);
+  if (watertight)
+    return fp16 ? launch_any4<true, AoRays, WtTest>(b, src, n, out, stream)
+                : launch_any4<false, AoRays, WtTest>(b, src, n, out, stream);
+  return fp16 ? launch_any4<true, AoRays, MtTest>(b, src, n, out, stream)
+              : launch_any4<false, AoRays, MtTest>(b, src, n, out, stream);
 }
 
 hipError_t trace_blas4(const Blas& b, const TraceArgs& a, bool fp16, hipStream_t stream) {
   if (a.n == 0) return hipSuccess;
   if (!b.quads32 || !b.quads16) return hipErrorInvalidValue;
-  if (a.anyhit && !a.stats) {
-    const StoredRays src{a.rays};
-    uint8_t* out = static_cast<uint8_t*>(a.out);
-    return fp16 ? launch_any4<true>(b, src, a.n, out, stream) : launch_any4<false>(b, src, a.n, out, stream);
+  if (a.watertight) {
+    if (!b.tris_wt) return hipErrorInvalidValue;
+    return trace_blas4_with<WtTest>(b, a, fp16, stream);
   }
-  return fp16 ? launch_simple4<true>(b, a, stream) : launch_simple4<false>(b, a, stream);
+  return trace_blas4_with<MtTest>(b, a, fp16, stream);
 }
 
 }  // namespace hrt

@@ -91,6 +91,16 @@ enum {
 };
 enum { HRT_BUILD_LOW = 0, HRT_BUILD_MEDIUM = 1, HRT_BUILD_HIGH = 2, HRT_BUILD_REFIT = 3 }; /* BuildQuality, CrossRT.h:9-15 */
 
+/* Ray/triangle test of a scene's traces (hrt_scene_set_triangle_test):
+ *   MT          the reference's Moller-Trumbore with its 1e-6 barycentric
+ *               slack (BVH2CommonRT.cpp:26-37); bit-exact parity mode.
+ *   WATERTIGHT  the watertight FP32 test of Woop, Benthin and Wald (JCGT
+ *               2(1), 2013): no slack, and no ray passes between two
+ *               triangles that share an edge or a vertex (an edge value of
+ *               exactly zero counts for both).  t, u and v keep their
+ *               meaning.  No counterpart in the reference. */
+enum { HRT_TRI_MT = 0, HRT_TRI_WATERTIGHT = 1 };
+
 /* Debug / parity dumps (hrt_debug_dump `what`).  Available for geometries
  * added after hrt_scene_set_keep_build_arrays(scene, 1). */
 enum {
@@ -172,6 +182,14 @@ hrt_status hrt_scene_deserialize(hrt_ctx* ctx, const void* data, size_t size, hr
  * BVH2CommonRT_host.cpp:166-170): a new column-major matrix for an existing
  * instance; the scene must be committed again. */
 hrt_status hrt_update_instance(hrt_scene* scene, uint32_t instId, const float colMajor16[16]);
+
+/* The ray/triangle test (HRT_TRI_*) of every trace of the scene from its
+ * next commit on: hrt_trace_*, the host batches, the per-ray query and the
+ * AO, eye and RTAO passes.  The watertight test needs a scene with one
+ * identity instance (commit fails with HRT_ERR_UNSUPPORTED otherwise); the
+ * server and batches trace with the reference's test.  No counterpart in the
+ * reference (its only test is BVH2CommonRT.cpp:26-37). */
+hrt_status hrt_scene_set_triangle_test(hrt_scene* scene, int test);
 
 /* ISceneObject::CommitScene (CrossRT.h:115; BVH2CommonRT_host.cpp:118-129).
  * A scene with exactly one instance whose matrix is the identity is traced on

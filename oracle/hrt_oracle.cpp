@@ -20,7 +20,10 @@
 //   any-hit rule          core/BVH4HalfRT.cpp:258-331 (tFar <= 0 => stop at first hit)
 //   brute force           core/BruteForceRT.cpp:184-265
 //   agglomerative tree    no reference counterpart (the engine's any-hit tree)
-//   BVH8 nodes            no reference counterpart (the engine's wide any-hit nodes)
+//   watertight test       no reference counterpart: Woop, Benthin and Wald,
+//                         "Watertight Ray/Triangle Intersection", JCGT 2(1),
+//                         2013 (section 3, the FP32 form without the
+//                         double-precision recomputation of zero edges)
 //   eye rays              examples/01_eye_rays/eye_ray.cpp:26-47, 114-138
 //                         (kernel_InitEyeRay, SuperBlockIndex2DOpt,
 //                         kernel_PackXY); examples/render_common.h:6-36
@@ -577,9 +580,52 @@ static inline bool fused_ok(f3 o, f3 inv) {
   return std::isfinite(-(o.x * inv.x)) && std::isfinite(-(o.y * inv.y)) && std::isfinite(-(o.z * inv.z));
 }
 
+// The watertight test (Woop, Benthin, Wald 2013) in FP32, the published
+// steps: the ray's largest |direction| axis becomes z (the lowest axis on
+// ties; x and y exchanged when that component is negative), the vertices are
+// taken relative to the origin and sheared by (-dx/dz, -dy/dz, 1/dz), the 2D
+// edge functions U (edge BC), V (CA), W (AB) must not have mixed signs (a
+// zero counts as either), det = U + V + W must not be zero, and
+// t = (U Az + V Bz + W Cz) / det.  u = W / det (the weight of C) and
+// v = V / det (the weight of B), the reference's meaning of u and v.  Every
+// operation rounds once, in the order written.
+static inline float axis_of(f3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+static inline bool wt_eval(f3 o, f3 d, float tNear, f3 A0, f3 B0, f3 C0, float& t, float& u, float& v) {
+  const float ax = std::fabs(d.x), ay = std::fabs(d.y), az = std::fabs(d.z);
+  const int kz = (ax >= ay) ? (ax >= az ? 0 : 2) : (ay >= az ? 1 : 2);
+  int kx = (kz + 1) % 3, ky = (kx + 1) % 3;
+  const float dz = axis_of(d, kz);
+  if (dz < 0.0f) std::swap(kx, ky);
+  const float sx = axis_of(d, kx) / dz, sy = axis_of(d, ky) / dz, sz = 1.0f / dz;
+  const f3 A = sub3(A0, o), B = sub3(B0, o), C = sub3(C0, o);
+  const float Ax = axis_of(A, kx) - sx * axis_of(A, kz), Ay = axis_of(A, ky) - sy * axis_of(A, kz);
+  const float Bx = axis_of(B, kx) - sx * axis_of(B, kz), By = axis_of(B, ky) - sy * axis_of(B, kz);
+  const float Cx = axis_of(C, kx) - sx * axis_of(C, kz), Cy = axis_of(C, ky) - sy * axis_of(C, kz);
+  const float U = Cx * By - Cy * Bx;
+  const float V = Ax * Cy - Ay * Cx;
+  const float W = Bx * Ay - By * Ax;
+  if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+  const float det = U + V + W;
+  if (det == 0.0f) return false;
+  const float T = U * (sz * axis_of(A, kz)) + V * (sz * axis_of(B, kz)) + W * (sz * axis_of(C, kz));
+  const float rcp = 1.0f / det;
+  t = T * rcp;
+  u = W * rcp;
+  v = V * rcp;
+  return t > tNear;
+}
+
+// The triangle test every traversal below uses: 0 = the reference's
+// Moller-Trumbore, 1 = the watertight test.  Set by the Python wrapper for the
+// duration of one call (hrto_set_triangle_test); not for concurrent calls
+// that want different tests.
+static int g_tri_test = 0;
+
 // Moller-Trumbore with the reference's 1e-6 slack, BVH2CommonRT.cpp:26-45:
 // t, u, v and whether the barycentric and t > tNear conditions hold.
 static inline bool mt_eval(f3 o, f3 d, float tNear, f3 A, f3 B, f3 C, float& t, float& u, float& v) {
+  if (g_tri_test == 1) return wt_eval(o, d, tNear, A, B, C, t, u, v);
   const f3 e1 = sub3(B, A);
   const f3 e2 = sub3(C, A);
   const f3 pv = cross3(d, e2);
@@ -1610,6 +1656,49 @@ void hrto_trace4(void* h, int mode, const float* rays, uint64_t n, uint32_t* hit
       counters[2 * i + 0] = uint32_t(c.pairs);
       counters[2 * i + 1] = uint32_t(c.tris);
     }
+  }
+}
+
+void hrto_set_triangle_test(int test) { g_tri_test = test; }
+
+// One triangle test: out = {t, u, v, accepted (0/1)}; test 0 = Moller-Trumbore, 1 = watertight.
+void hrto_tri_eval(int test, const float* o3, const float* d3, float tnear, const float* a3, const float* b3,
+                   const float* c3, float* out4) {
+  const f3 o{o3[0], o3[1], o3[2]}, d{d3[0], d3[1], d3[2]};
+  const f3 A{a3[0], a3[1], a3[2]}, B{b3[0], b3[1], b3[2]}, C{c3[0], c3[1], c3[2]};
+  float t = 0.f, u = 0.f, v = 0.f;
+  const bool ok = test == 1 ? wt_eval(o, d, tnear, A, B, C, t, u, v) : mt_eval(o, d, tnear, A, B, C, t, u, v);
+  out4[0] = t;
+  out4[1] = u;
+  out4[2] = v;
+  out4[3] = ok ? 1.0f : 0.0f;
+}
+
+// Brute force: for every ray, how many triangles the test accepts with t in
+// (tNear, |tFar|) -- the closed-surface checks count these.
+void hrto_tri_hit_counts(int test, const float* v4, uint32_t nverts, const uint32_t* idx, uint32_t ntris,
+                         const float* rays, uint64_t n, uint32_t* counts, int threads) {
+#ifdef _OPENMP
+  if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(dynamic, 64) if (threads != 1)
+#endif
+  for (int64_t i = 0; i < int64_t(n); i++) {
+    const float* r = rays + 8 * i;
+    const f3 o{r[0], r[1], r[2]}, d{r[4], r[5], r[6]};
+    const float tfar = std::fabs(r[7]);
+    uint32_t c = 0;
+    for (uint32_t k = 0; k < ntris; k++) {
+      f3 P[3];
+      for (int j = 0; j < 3; j++) {
+        const uint32_t vi = idx[3 * size_t(k) + j];
+        P[j] = vi < nverts ? f3{v4[4 * vi], v4[4 * vi + 1], v4[4 * vi + 2]} : f3{0, 0, 0};
+      }
+      float t, u, v;
+      const bool ok = test == 1 ? wt_eval(o, d, r[3], P[0], P[1], P[2], t, u, v)
+                                : mt_eval(o, d, r[3], P[0], P[1], P[2], t, u, v);
+      if (ok && t < tfar) c++;
+    }
+    counts[i] = c;
   }
 }
 

@@ -29,6 +29,9 @@ MODE4_FP16 = 1
 MODE4_BRUTE = 2
 MODE4_FP16_FUSED = 3  # FP16 nodes, box tests in the fused form of the engine's any-hit calls
 
+TRI_MT = 0          # the reference's Moller-Trumbore (1e-6 slack)
+TRI_WATERTIGHT = 1  # Woop, Benthin, Wald 2013, FP32
+
 
 def f2h(v: float) -> int:
     return int(lib().hrto_f2h(v))
@@ -96,6 +99,12 @@ def lib():
         L.hrto_quad_count.argtypes = [vp]
         L.hrto_quads.argtypes = [vp, C.c_int, vp]
         L.hrto_trace4.argtypes = [vp, C.c_int, vp, u64, vp, vp, C.c_int]
+        L.hrto_set_triangle_test.restype = None
+        L.hrto_set_triangle_test.argtypes = [C.c_int]
+        L.hrto_tri_eval.restype = None
+        L.hrto_tri_eval.argtypes = [C.c_int, vp, vp, C.c_float, vp, vp, vp, vp]
+        L.hrto_tri_hit_counts.restype = None
+        L.hrto_tri_hit_counts.argtypes = [C.c_int, vp, u32, vp, u32, vp, u64, vp, C.c_int]
         for name in ("hrto_f2h", "hrto_round_neg", "hrto_round_pos"):
             getattr(L, name).restype = u32
             getattr(L, name).argtypes = [C.c_float]
@@ -117,6 +126,41 @@ def lib():
 
 def _ptr(a: np.ndarray):
     return a.ctypes.data_as(C.c_void_p)
+
+
+class triangle_test:
+    """Context manager: the oracle's traversals use `test` (TRI_MT or TRI_WATERTIGHT) inside the block."""
+
+    def __init__(self, test: int):
+        self.test = int(test)
+
+    def __enter__(self):
+        lib().hrto_set_triangle_test(self.test)
+        return self
+
+    def __exit__(self, *exc):
+        lib().hrto_set_triangle_test(TRI_MT)
+        return False
+
+
+def tri_eval(test: int, o, d, tnear: float, A, B, C):
+    """One triangle test: (t, u, v, accepted)."""
+    args = [np.ascontiguousarray(x, dtype=np.float32) for x in (o, d, A, B, C)]
+    out = np.zeros(4, dtype=np.float32)
+    lib().hrto_tri_eval(int(test), _ptr(args[0]), _ptr(args[1]), float(tnear), _ptr(args[2]), _ptr(args[3]),
+                        _ptr(args[4]), _ptr(out))
+    return float(out[0]), float(out[1]), float(out[2]), bool(out[3])
+
+
+def tri_hit_counts(test: int, verts, idx, rays, threads: int = 0) -> np.ndarray:
+    """Per ray, the number of triangles the test accepts with t in (tNear, |tFar|) (brute force)."""
+    v4 = as_vert4(verts)
+    f = np.ascontiguousarray(idx, dtype=np.uint32).reshape(-1)
+    r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+    out = np.zeros(r.shape[0], dtype=np.uint32)
+    lib().hrto_tri_hit_counts(int(test), _ptr(v4), v4.shape[0], _ptr(f), f.size // 3, _ptr(r), r.shape[0],
+                              _ptr(out), threads)
+    return out
 
 
 def as_vert4(verts: np.ndarray) -> np.ndarray:

@@ -103,6 +103,37 @@ def test_bvh4_nearest_bit_exact(gpu_ctx, name, gen, layout):
     s.close()
 
 
+@pytest.mark.parametrize("layout", [H.LAYOUT_BVH4, H.LAYOUT_BVH4_FP16], ids=["fp32", "fp16"])
+def test_bvh4_nearest_refilling_waves(gpu_ctx, layout):
+    """A batch larger than one resident grid of the one-ray-per-lane kernel goes to the refilling
+    kernel (speculative while-while, wave-local ray pool): full hits equal the oracle's BVH4
+    traversal under the (t, primId) rule on every ray, including rays that are any-hit by their own
+    tFar <= 0, empty and reversed intervals and a NaN bound."""
+    v, f = W.soup(150_000, stream=W.STREAM_TEST + 5)
+    n = 600_000
+    rays = W.incoherent_rays(n, stream=W.STREAM_TEST + 6)
+    kind = np.arange(n) % 10
+    rays[kind == 1, 7] = -rays[kind == 1, 7]      # any-hit by the ray's own convention
+    rays[kind == 2, 7] = 0.0                      # tFar = tNear = 0: empty
+    rays[kind == 3, 3] = 0.5                      # tNear > tFar
+    rays[kind == 3, 7] = 0.25
+    rays[kind == 4, 7] = np.float32(np.nan)
+    rays[kind == 5, 7] = 0.05                     # short rays
+    say(f"bvh4 nearest refilling waves, layout {layout}")
+    s = scene_with(gpu_ctx, v, f, layout)
+    g = trace(s, rays, "full")
+    s.close()
+    mode = O.MODE4_FP16 if layout == H.LAYOUT_BVH4_FP16 else O.MODE4_FP32
+    t, p, u, vv = O.OracleBVH(v, f).trace4(rays, mode)
+    np.testing.assert_array_equal(g["prim"], p)
+    np.testing.assert_array_equal(g["t"].view(np.uint32), t.view(np.uint32))
+    np.testing.assert_array_equal(g["u"].view(np.uint32), u.view(np.uint32))
+    np.testing.assert_array_equal(g["v"].view(np.uint32), vv.view(np.uint32))
+    hit = p != INVALID
+    assert 0.05 < hit[kind == 0].mean() < 0.999
+    assert not hit[(kind == 2) | (kind == 3) | (kind == 4)].any()
+
+
 @pytest.mark.parametrize("name,gen", CASES, ids=[c[0] for c in CASES])
 @pytest.mark.parametrize("layout", [H.LAYOUT_BVH4, H.LAYOUT_BVH4_FP16, H.LAYOUT_AUTO, H.LAYOUT_AUTO_FP16],
                          ids=["fp32", "fp16", "auto", "auto_fp16"])

@@ -1,0 +1,180 @@
+"""The oracle's watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2(1),
+2013), pinned without the engine: known answers, an exact rational
+recomputation of the inside/outside decision and of t, u, v, a closed mesh
+that no ray from its inside escapes, invariance under a cyclic relabelling of
+the axes, and the BVH traversals against brute force with the same test."""
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+from conftest import VSGF_MESHES, vsgf_path
+from heavyrt_amd import workloads as W
+from oracle import oracle as O
+
+WT, MT = O.TRI_WATERTIGHT, O.TRI_MT
+INVALID = 0xFFFFFFFF
+
+
+def test_known_answers():
+    A, B, C = [0, 0, 0], [1, 0, 0], [0, 1, 0]
+    t, u, v, ok = O.tri_eval(WT, [0.25, 0.25, 1.0], [0, 0, -1], 0.0, A, B, C)
+    assert ok and (t, u, v) == (1.0, 0.25, 0.25)  # u = weight of C, v = weight of B
+    t, u, v, ok = O.tri_eval(WT, [0.25, 0.25, -2.0], [0, 0, 1], 0.0, A, B, C)  # from below: two-sided
+    assert ok and (t, u, v) == (2.0, 0.25, 0.25)
+    assert not O.tri_eval(WT, [0.75, 0.75, 1.0], [0, 0, -1], 0.0, A, B, C)[3]  # outside the hypotenuse
+    assert not O.tri_eval(WT, [0.25, 0.25, 1.0], [0, 0, 1], 0.0, A, B, C)[3]   # behind the origin
+    assert not O.tri_eval(WT, [0.25, 0.25, 1.0], [0, 0, -1], 1.5, A, B, C)[3]  # before tNear
+    assert not O.tri_eval(WT, [0.25, 0.25, 0.0], [1, 0, 0], 0.0, A, B, C)[3]   # in the plane: det = 0
+    assert not O.tri_eval(WT, [0.25, 0.25, 1.0], [0, 0, -1], 0.0, A, A, C)[3]  # degenerate triangle
+    # exactly on an edge and on a vertex: inside (a zero edge value counts)
+    assert O.tri_eval(WT, [0.5, 0.0, 1.0], [0, 0, -1], 0.0, A, B, C)[3]
+    assert O.tri_eval(WT, [0.0, 0.0, 1.0], [0, 0, -1], 0.0, A, B, C)[3]
+    # the reference's test keeps its 1e-6 slack; the watertight one has none
+    eps_out = [0.5, -4e-7, 1.0]
+    assert O.tri_eval(MT, eps_out, [0, 0, -1], 0.0, A, B, C)[3]
+    assert not O.tri_eval(WT, eps_out, [0, 0, -1], 0.0, A, B, C)[3]
+
+
+def _exact(o, d, A, B, C):
+    """Exact rational ray/triangle: (inside incl. edges, t, weight of B, weight of C) or None."""
+    F = lambda x: Fraction(float(x))
+    o, d, A, B, C = ([F(c) for c in p] for p in (o, d, A, B, C))
+    e1 = [B[i] - A[i] for i in range(3)]
+    e2 = [C[i] - A[i] for i in range(3)]
+    cross = lambda a, b: [a[1] * b[2] - a[2] * b[1], a[2] * b[0] - a[0] * b[2], a[0] * b[1] - a[1] * b[0]]
+    dot = lambda a, b: sum(a[i] * b[i] for i in range(3))
+    p = cross(d, e2)
+    det = dot(e1, p)
+    if det == 0:
+        return None
+    s = [o[i] - A[i] for i in range(3)]
+    q = cross(s, e1)
+    b = dot(s, p) / det  # weight of B
+    c = dot(q, d) / det  # weight of C
+    t = dot(e2, q) / det
+    return (b >= 0 and c >= 0 and b + c <= 1), t, b, c
+
+
+def test_decision_and_values_match_exact_arithmetic():
+    r = W.uniform01(W.pcg32(3000, 15, 4100)).astype(np.float64)
+    checked = 0
+    rel_t, err_uv = [], []
+    for k in range(r.shape[0]):
+        A, B, C = r[k, 0:3] * 2 - 1, r[k, 3:6] * 2 - 1, r[k, 6:9] * 2 - 1
+        o = r[k, 9:12] * 4 - 2
+        # aim at a point of the triangle's plane with barycentrics in [-0.3, 1.3]
+        bb, cc = r[k, 12] * 1.6 - 0.3, r[k, 13] * 1.6 - 0.3
+        target = A + bb * (B - A) + cc * (C - A)
+        d = target - o
+        d /= np.linalg.norm(d)
+        args = [np.float32(x) for x in (o, d, A, B, C)]
+        ex = _exact(*args)
+        if ex is None:
+            continue
+        inside, t, wb, wc = ex
+        margin = min(abs(float(wb)), abs(float(wc)), abs(1 - float(wb) - float(wc)))
+        if margin < 1e-4:
+            continue  # near an edge the FP32 rounding of the edge values may decide either way
+        t32, u32, v32, ok = O.tri_eval(WT, args[0], args[1], -1e30, args[2], args[3], args[4])
+        assert ok == inside, k
+        if inside:
+            rel_t.append(abs(t32 - float(t)) / max(1e-3, abs(float(t))))
+            err_uv.append(max(abs(u32 - float(wc)), abs(v32 - float(wb))))
+        checked += 1
+    assert checked > 2500
+    # FP32 rounding: the bulk at a few ulps, the tail set by ill-conditioned (grazing) triangles
+    rel_t, err_uv = np.array(rel_t), np.array(err_uv)
+    assert np.median(rel_t) < 1e-6 and np.percentile(rel_t, 99) < 1e-4 and rel_t.max() < 1e-3
+    assert np.median(err_uv) < 1e-6 and np.percentile(err_uv, 99) < 1e-4 and err_uv.max() < 1e-3
+
+
+def test_cyclic_axis_relabelling_gives_the_same_bits():
+    r = W.uniform01(W.pcg32(500, 15, 4200))
+    for k in range(r.shape[0]):
+        A, B, C = r[k, 0:3] * 2 - 1, r[k, 3:6] * 2 - 1, r[k, 6:9] * 2 - 1
+        o = r[k, 9:12] * 4 - 2
+        d = (A + B + C) / 3 + 0.2 * (r[k, 12:15] - 0.5) - o
+        d = (d / np.linalg.norm(d)).astype(np.float32)
+        base = O.tri_eval(WT, o, d, 0.0, A, B, C)
+        for shift in (1, 2):
+            rot = lambda p: np.roll(np.asarray(p, np.float32), shift)
+            got = O.tri_eval(WT, rot(o), rot(d), 0.0, rot(A), rot(B), rot(C))
+            assert got[3] == base[3], k
+            if base[3]:
+                assert np.float32(got[0]).view(np.uint32) == np.float32(base[0]).view(np.uint32), k
+
+
+def closed_sphere_rays():
+    """An icosphere (closed, 1,280 triangles) and rays from points inside it
+    aimed exactly at every vertex and every edge midpoint (as float32)."""
+    v, f = W.icosphere(3, 1.0, (0.3, -0.2, 0.1))
+    edges = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+    edges = np.unique(np.sort(edges, axis=1), axis=0)
+    targets = np.concatenate([v, 0.5 * (v[edges[:, 0]] + v[edges[:, 1]])]).astype(np.float32)
+    origins = np.array([[0.3, -0.2, 0.1], [0.55, 0.1, -0.3], [0.0, -0.6, 0.4]], np.float32)
+    rays = []
+    for o in origins:
+        d = targets - o
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        r = np.zeros((targets.shape[0], 8), np.float32)
+        r[:, 0:3] = o
+        r[:, 4:7] = d
+        r[:, 7] = 10.0
+        rays.append(r)
+    return v, f, np.concatenate(rays)
+
+
+def test_no_ray_escapes_a_closed_mesh():
+    v, f, rays = closed_sphere_rays()
+    counts = O.tri_hit_counts(WT, v, f, rays)
+    assert counts.min() >= 1
+    # a ray through an edge is claimed by its two triangles at most, through a vertex by its fan (<= 6 here)
+    assert counts.max() <= 6
+    # most rays aimed at an edge or a vertex land in exactly one triangle (no zero edge value)
+    assert (counts == 1).mean() > 0.5
+
+
+def test_leaky_without_slack_is_caught():
+    """The closed-mesh check finds a leak when there is one: the same sphere with every triangle
+    shrunk by a relative 1e-5 about its centroid (hairline gaps along every edge) lets rays
+    aimed at edges and vertices through."""
+    v, f, rays = closed_sphere_rays()
+    tri = v[f].astype(np.float64)
+    c = tri.mean(axis=1, keepdims=True)
+    shrunk = (c + (tri - c) * (1 - 1e-5)).astype(np.float32)
+    sv = shrunk.reshape(-1, 3)
+    sf = np.arange(sv.shape[0], dtype=np.uint32).reshape(-1, 3)
+    counts = O.tri_hit_counts(WT, sv, sf, rays)
+    assert counts.min() == 0
+
+
+@pytest.mark.parametrize("name", VSGF_MESHES)
+def test_traversals_equal_brute_force(name):
+    v, f = W.load_vsgf(vsgf_path(name))
+    lo, hi = v.min(0), v.max(0)
+    ext = float(np.max(hi - lo)) + 1e-3
+    r = W.uniform01(W.pcg32(3000, 6, 4300))
+    rays = np.zeros((3000, 8), np.float32)
+    rays[:, 0:3] = lo - 0.5 * ext + 2 * ext * r[:, 0:3]
+    tgt = lo + (hi - lo) * r[:, 3:6]
+    d = tgt - rays[:, 0:3]
+    rays[:, 4:7] = d / np.linalg.norm(d, axis=1, keepdims=True)
+    rays[:, 7] = 10 * ext
+    b = O.OracleBVH(v, f)
+    with O.triangle_test(WT):
+        t2, p2, _, _ = b.trace(rays)
+        tb, pb, _, _ = b.trace(rays, O.MODE_BRUTE)
+        t4, p4, _, _ = b.trace4(rays, O.MODE4_FP16)
+        t4b, p4b, _, _ = b.trace4(rays, O.MODE4_BRUTE)
+    np.testing.assert_array_equal(t2.view(np.uint32), tb.view(np.uint32))
+    np.testing.assert_array_equal(t4.view(np.uint32), t4b.view(np.uint32))
+    np.testing.assert_array_equal(p4, p4b)
+    # the BVH2 order picks the first of equal-t hits, the BVH4 rule the smaller id: the same t
+    np.testing.assert_array_equal(t2.view(np.uint32), t4.view(np.uint32))
+    counts = O.tri_hit_counts(WT, v, f, rays)
+    np.testing.assert_array_equal(counts > 0, p2 != INVALID)
+    # the context manager restores the reference's test
+    tm, pm, _, _ = b.trace(rays)
+    tmb, pmb, _, _ = b.trace(rays, O.MODE_BRUTE)
+    np.testing.assert_array_equal(tm.view(np.uint32), tmb.view(np.uint32))
