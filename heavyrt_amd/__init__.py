@@ -184,9 +184,11 @@ def _ptr(a: np.ndarray):
 
 
 def _dev_ptr(x) -> int:
-    """Device address of a torch tensor (or an int address)."""
+    """Address of a torch tensor, a numpy array (host) or an int address."""
     if isinstance(x, int):
         return x
+    if isinstance(x, np.ndarray):
+        return int(x.ctypes.data)
     return int(x.data_ptr())
 
 
@@ -316,6 +318,21 @@ class Scene:
         else:
             _check(lib().hrt_trace_any_stats(self._h, _dev_ptr(rays), n, _dev_ptr(out), _dev_ptr(stats),
                                              stream or None))
+
+    def trace_nearest_host(self, rays: np.ndarray, stats: bool = False):
+        """Host-memory batch (hrt_trace_nearest_host): (n, 8) uint32 hit records[, (n, 2) counters]."""
+        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        out = np.zeros((r.shape[0], 8), dtype=np.uint32)
+        st = np.zeros((r.shape[0], 2), dtype=np.uint32) if stats else None
+        _check(lib().hrt_trace_nearest_host(self._h, _ptr(r), r.shape[0], _ptr(out), _ptr(st) if stats else None))
+        return (out, st) if stats else out
+
+    def trace_any_host(self, rays: np.ndarray) -> np.ndarray:
+        """Host-memory batch (hrt_trace_any_host): one bool per ray."""
+        r = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        out = np.zeros(r.shape[0], dtype=np.uint8)
+        _check(lib().hrt_trace_any_host(self._h, _ptr(r), r.shape[0], _ptr(out)))
+        return out.astype(bool)
 
     def query_nearest_one(self, pos_near, dir_far):
         p = np.ascontiguousarray(pos_near, dtype=np.float32)
@@ -584,11 +601,12 @@ class Server:
         _check(lib().hrt_server_add_scene(self._h, scene._h, C.byref(sid)))
         return sid.value
 
-    def submit(self, scene_id: int, rays: np.ndarray, out: np.ndarray, any_hit: bool = False) -> int:
-        """rays: pinned (n, 8) float32; out: pinned (n, 2) uint32 (nearest) or (n,) uint8 (any-hit)."""
+    def submit(self, scene_id: int, rays, out, any_hit: bool = False) -> int:
+        """rays: (n, 8) float32; out: (n, 2) uint32 (nearest) or (n,) uint8 (any-hit); each a pinned host
+        array (pinned_array) or a device tensor on the server's device."""
         n = rays.shape[0]
         rid = C.c_uint64()
-        _check(lib().hrt_server_submit(self._h, scene_id, 1 if any_hit else 0, rays.ctypes.data, n, out.ctypes.data,
+        _check(lib().hrt_server_submit(self._h, scene_id, 1 if any_hit else 0, _dev_ptr(rays), n, _dev_ptr(out),
                                        C.byref(rid)))
         self._keep[rid.value] = (rays, out)
         return rid.value

@@ -77,7 +77,7 @@ HARNESS_BIN = os.path.join(ROOT, "tests", "cpp", "_build", "scene_harness")
 def build_harness() -> str:
     """Renderer-style C++ caller of ISceneObject used by the GPU tests."""
     os.makedirs(os.path.dirname(HARNESS_BIN), exist_ok=True)
-    _run(["g++", "-O2", "-std=c++17", "-Wall", "-I", INCLUDE, "-o", HARNESS_BIN, HARNESS_SRC,
+    _run(["g++", "-O2", "-std=c++17", "-fopenmp", "-Wall", "-I", INCLUDE, "-o", HARNESS_BIN, HARNESS_SRC,
           "-L", HERE, "-lhrt_scene", "-lhrt", f"-Wl,-rpath,{HERE}", "-Wl,-rpath,$ORIGIN/../../../heavyrt_amd"])
     return HARNESS_BIN
 

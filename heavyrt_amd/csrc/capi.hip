@@ -13,6 +13,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
+#include <condition_variable>
 #include <mutex>
 #include <new>
 #include <string>
@@ -137,21 +139,47 @@ struct hrt_scene {
   int layout = HRT_LAYOUT_BVH2;
   int tri_test = HRT_TRI_MT;  // takes effect at the next commit
   int committed_tri_test = HRT_TRI_MT;
+  // servers holding the scene (hrt_server_add_scene): its trees must not change
+  std::atomic<int> served{0};
   // instanced scenes: top-level tree over the instance boxes and the
   // instance records (by instance id, then in the tree's leaf order)
   bool two_level = false;
   hrt::Blas tlas;
   hrt::InstDev* inst_dev = nullptr;
-  // staging for hrt_query_nearest_one
-  std::mutex one_mu;
   // staging for the host-memory batch calls (grown on demand, guarded by one_mu)
+  std::mutex one_mu;
   void* host_dev = nullptr;
   size_t host_dev_bytes = 0;
-  hrt::Ray* one_ray = nullptr;
-  hrt_hit* one_hit = nullptr;
+  // hrt_query_nearest_one: concurrent calls are traced together (below)
+  struct OneQueue {
+    struct Waiter {
+      float ray[8];
+      hrt_hit* out;
+      hrt_status* status;
+      std::string* message;
+    };
+    std::mutex mu;
+    std::condition_variable cv;
+    bool running = false;      // a caller is tracing a batch
+    uint64_t open_gen = 1;     // the batch new calls join
+    uint64_t done_gen = 0;     // the last batch that finished
+    std::vector<Waiter> open;  // calls of the open batch
+    // pinned host staging the trace kernel reads and writes in place
+    float* rays = nullptr;
+    hrt_hit* hits = nullptr;
+    size_t cap = 0;
+    hipStream_t stream = nullptr;
+  } one;
 };
 
 namespace {
+
+// Mutators refuse a scene that a server holds: the server walks its trees.
+hrt_status check_mutable(const hrt_scene* s) {
+  if (s->served.load() > 0)
+    return fail(HRT_ERR_INVALID_ARG, "the scene is resident in a server (destroy the server before changing it)");
+  return HRT_OK;
+}
 
 void free_tlas(hrt_scene* s) {
   if (s->tlas.pairs || s->inst_dev) {
@@ -262,6 +290,33 @@ hrt_status scene_single_blas(hrt_scene* s, const Blas** out) {
   return select_blas(s, out, &inst);
 }
 int scene_triangle_test(const hrt_scene* s) { return s->committed_tri_test; }
+
+TreeRef scene_any_hit_tree(const hrt_scene* s) {
+  // the branch trace_common takes for an any-hit batch of a single-tree scene
+  const Blas& b = s->geoms[s->instances[0].geom];
+  const int lay = s->layout;
+  TreeRef t;
+  if (lay == HRT_LAYOUT_BVH2 || !b.quads32) {
+    t.nodes = b.pairs;
+    t.tris = b.tris;
+    t.kind = 0;
+  } else if (lay == HRT_LAYOUT_AUTO_FP16 && b.any_tree) {
+    t.nodes = b.any_tree->quads16;
+    t.tris = b.any_tree->tris;
+    t.kind = 1;
+  } else if (lay == HRT_LAYOUT_BVH4_FP16 || lay == HRT_LAYOUT_AUTO_FP16) {
+    t.nodes = b.quads16;
+    t.tris = b.tris;
+    t.kind = 1;
+  } else {
+    t.nodes = b.quads32;
+    t.tris = b.tris;
+    t.kind = 2;
+  }
+  return t;
+}
+
+void scene_serve_ref(hrt_scene* s, int delta) { s->served += delta; }
 }  // namespace hrt
 
 extern "C" {
@@ -316,6 +371,7 @@ hrt_status hrt_scene_set_keep_build_arrays(hrt_scene* s, int keep) {
 
 hrt_status hrt_scene_clear_geom(hrt_scene* s) {
   if (!s) return fail(HRT_ERR_INVALID_ARG, "scene is null");
+  if (hrt_status st = check_mutable(s)) return st;
   (void)hipSetDevice(s->ctx->device);
   (void)hipStreamSynchronize(s->ctx->stream);
   free_tlas(s);
@@ -328,15 +384,22 @@ hrt_status hrt_scene_clear_geom(hrt_scene* s) {
 
 void hrt_scene_destroy(hrt_scene* s) {
   if (!s) return;
+  if (s->served.load() > 0) {
+    // a server still walks its trees: keep everything rather than free memory in use
+    std::fprintf(stderr, "hrt_scene_destroy: the scene is resident in a server; it is kept until the process ends\n");
+    return;
+  }
   hrt_scene_clear_geom(s);
-  if (s->one_ray) (void)hipFree(s->one_ray);
   if (s->host_dev) (void)hipFree(s->host_dev);
-  if (s->one_hit) (void)hipFree(s->one_hit);
+  if (s->one.rays) (void)hipHostFree(s->one.rays);
+  if (s->one.hits) (void)hipHostFree(s->one.hits);
+  if (s->one.stream) (void)hipStreamDestroy(s->one.stream);
   delete s;
 }
 
 hrt_status hrt_scene_clear_instances(hrt_scene* s) {
   if (!s) return fail(HRT_ERR_INVALID_ARG, "scene is null");
+  if (hrt_status st = check_mutable(s)) return st;
   free_tlas(s);
   s->instances.clear();
   s->committed = false;
@@ -347,6 +410,7 @@ hrt_status hrt_add_triangles(hrt_scene* s, const float* vpos, size_t nVert, size
                              const uint32_t* idx, size_t nIdx, int quality, uint32_t* outGeomId) {
   (void)quality;  // every quality level builds the LBVH (the reference's cbvh_lbvh preset)
   if (!s || !vpos || !idx) return fail(HRT_ERR_INVALID_ARG, "scene/vpos/idx is null");
+  if (hrt_status st = check_mutable(s)) return st;
   if (vStrideBytes == 0) vStrideBytes = 12;
   if (vStrideBytes % 4 != 0 || vStrideBytes < 12) return fail(HRT_ERR_INVALID_ARG, "vertex stride must be >= 12 and a multiple of 4");
   const size_t ntris = nIdx / 3;
@@ -385,6 +449,7 @@ hrt_status hrt_add_triangles(hrt_scene* s, const float* vpos, size_t nVert, size
 
 hrt_status hrt_add_instance(hrt_scene* s, uint32_t geomId, const float colMajor16[16], uint32_t* outInstId) {
   if (!s || !colMajor16) return fail(HRT_ERR_INVALID_ARG, "scene/matrix is null");
+  if (hrt_status st = check_mutable(s)) return st;
   if (geomId >= s->geoms.size()) return fail(HRT_ERR_INVALID_ARG, "no such geometry");
   Instance in;
   in.geom = geomId;
@@ -401,6 +466,7 @@ hrt_status hrt_update_triangles(hrt_scene* s, uint32_t geomId, const float* vpos
                                 const uint32_t* idx, size_t nIdx, int quality) {
   if (!s || !vpos) return fail(HRT_ERR_INVALID_ARG, "scene/vpos is null");
   if (geomId >= s->geoms.size()) return fail(HRT_ERR_INVALID_ARG, "no such geometry");
+  if (hrt_status st = check_mutable(s)) return st;
   if (vStrideBytes == 0) vStrideBytes = 12;
   if (vStrideBytes % 4 != 0 || vStrideBytes < 12) return fail(HRT_ERR_INVALID_ARG, "vertex stride must be >= 12 and a multiple of 4");
   hrt::Blas& b = s->geoms[geomId];
@@ -466,6 +532,7 @@ hrt_status hrt_update_triangles(hrt_scene* s, uint32_t geomId, const float* vpos
 
 hrt_status hrt_update_instance(hrt_scene* s, uint32_t instId, const float colMajor16[16]) {
   if (!s || !colMajor16) return fail(HRT_ERR_INVALID_ARG, "scene/matrix is null");
+  if (hrt_status st = check_mutable(s)) return st;
   if (instId >= s->instances.size()) return fail(HRT_ERR_INVALID_ARG, "no such instance");
   Instance in = s->instances[instId];
   std::memcpy(in.fwd, colMajor16, sizeof(in.fwd));
@@ -479,6 +546,7 @@ hrt_status hrt_update_instance(hrt_scene* s, uint32_t instId, const float colMaj
 hrt_status hrt_commit(hrt_scene* s, int quality, int layout) {
   (void)quality;
   if (!s) return fail(HRT_ERR_INVALID_ARG, "scene is null");
+  if (hrt_status st = check_mutable(s)) return st;
   if (layout < HRT_LAYOUT_BVH2 || layout > HRT_LAYOUT_AUTO_FP16)
     return fail(HRT_ERR_INVALID_ARG, "unknown layout");
   s->committed = false;
@@ -533,6 +601,7 @@ hrt_status hrt_commit(hrt_scene* s, int quality, int layout) {
 
 hrt_status hrt_scene_set_triangle_test(hrt_scene* s, int test) {
   if (!s) return fail(HRT_ERR_INVALID_ARG, "scene is null");
+  if (hrt_status st = check_mutable(s)) return st;
   if (test != HRT_TRI_MT && test != HRT_TRI_WATERTIGHT) return fail(HRT_ERR_INVALID_ARG, "unknown triangle test");
   s->tri_test = test;
   s->committed = false;
@@ -604,25 +673,74 @@ hrt_status hrt_trace_any_host(hrt_scene* s, const float* rays, size_t n, uint8_t
   return host_batch(s, rays, n, out, 1, 1, nullptr);
 }
 
+// The per-ray path.  A call joins the open batch of its scene; if no batch
+// is being traced, the caller traces the open batch (its own ray and every
+// ray that joined meanwhile) as one launch and hands each waiting caller its
+// hit, otherwise it waits for its batch.  So concurrent per-ray calls -- a
+// renderer's OpenMP loop of RayQuery_NearestHit -- share launches instead of
+// queueing one launch each.  Rays and hits stay in pinned host memory that
+// the kernel reads and writes in place: one launch and one wait per batch.
 hrt_status hrt_query_nearest_one(hrt_scene* s, const float posNear[4], const float dirFar[4], hrt_hit* out) {
   if (!s || !posNear || !dirFar || !out) return fail(HRT_ERR_INVALID_ARG, "null argument");
-  std::lock_guard<std::mutex> lock(s->one_mu);
-  hipError_t e = hipSetDevice(s->ctx->device);
-  if (e == hipSuccess && !s->one_ray) e = hipMalloc(&s->one_ray, sizeof(hrt::Ray));
-  if (e == hipSuccess && !s->one_hit) e = hipMalloc(&s->one_hit, sizeof(hrt_hit));
-  if (e != hipSuccess) return fail_hip(e, "query staging");
-  float r[8];
-  std::memcpy(r, posNear, 16);
-  std::memcpy(r + 4, dirFar, 16);
-  hipStream_t hs = s->ctx->stream;
-  if ((e = hipMemcpyAsync(s->one_ray, r, sizeof(r), hipMemcpyHostToDevice, hs)) != hipSuccess)
-    return fail_hip(e, "query upload");
-  hrt_status st = trace_common(s, reinterpret_cast<const float*>(s->one_ray), 1, s->one_hit, HRT_HIT_FULL, 0, nullptr,
-                               hs);
-  if (st != HRT_OK) return st;
-  if ((e = hipMemcpyAsync(out, s->one_hit, sizeof(hrt_hit), hipMemcpyDeviceToHost, hs)) != hipSuccess)
-    return fail_hip(e, "query download");
-  if ((e = hipStreamSynchronize(hs)) != hipSuccess) return fail_hip(e, "query sync");
+  auto& q = s->one;
+  hrt_status status = HRT_OK;
+  std::string message;
+  std::unique_lock<std::mutex> lk(q.mu);
+  const uint64_t gen = q.open_gen;
+  hrt_scene::OneQueue::Waiter w;
+  std::memcpy(w.ray, posNear, 16);
+  std::memcpy(w.ray + 4, dirFar, 16);
+  w.out = out;
+  w.status = &status;
+  w.message = &message;
+  q.open.push_back(w);
+  while (q.done_gen < gen) {
+    if (q.running) {
+      q.cv.wait(lk);
+      continue;
+    }
+    // lead: trace the open batch (it holds this call's ray)
+    q.running = true;
+    std::vector<hrt_scene::OneQueue::Waiter> batch;
+    batch.swap(q.open);
+    const uint64_t this_gen = q.open_gen++;
+    lk.unlock();
+    const size_t n = batch.size();
+    hipError_t e = hipSetDevice(s->ctx->device);
+    if (e == hipSuccess && !q.stream) e = hipStreamCreateWithFlags(&q.stream, hipStreamNonBlocking);
+    if (e == hipSuccess && n > q.cap) {
+      if (q.rays) (void)hipHostFree(q.rays);
+      if (q.hits) (void)hipHostFree(q.hits);
+      q.rays = nullptr;
+      q.hits = nullptr;
+      q.cap = 0;
+      const size_t cap = std::max<size_t>(64, 2 * n);
+      e = hipHostMalloc(reinterpret_cast<void**>(&q.rays), 32 * cap, hipHostMallocDefault);
+      if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&q.hits), sizeof(hrt_hit) * cap, hipHostMallocDefault);
+      if (e == hipSuccess) q.cap = cap;
+    }
+    hrt_status st = HRT_OK;
+    std::string msg;
+    if (e != hipSuccess) {
+      st = fail_hip(e, "query staging");
+    } else {
+      for (size_t i = 0; i < n; i++) std::memcpy(q.rays + 8 * i, batch[i].ray, 32);
+      st = trace_common(s, q.rays, n, q.hits, HRT_HIT_FULL, 0, nullptr, q.stream);
+      if (st == HRT_OK && (e = hipStreamSynchronize(q.stream)) != hipSuccess) st = fail_hip(e, "query sync");
+    }
+    if (st != HRT_OK) msg = hrt::g_last_error;
+    lk.lock();
+    for (size_t i = 0; i < n; i++) {
+      if (st == HRT_OK) *batch[i].out = q.hits[i];
+      *batch[i].status = st;
+      *batch[i].message = msg;
+    }
+    q.done_gen = this_gen;
+    q.running = false;
+    q.cv.notify_all();
+  }
+  lk.unlock();
+  if (status != HRT_OK) return fail(status, message);
   return HRT_OK;
 }
 

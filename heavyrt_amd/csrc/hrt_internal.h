@@ -164,6 +164,38 @@ hipError_t trace_blas4(const Blas& b, const TraceArgs& a, bool fp16, hipStream_t
 // records carry the instance and geometry ids of the hit.
 hipError_t trace_two_level(const Blas& tlas, const InstDev* inst_leaf_order, const TraceArgs& a, hipStream_t stream);
 
+// Serving: one request inside a step, 48 bytes.  nodes is the tree the
+// request's rays walk (BVH2 node pairs, or FP16/FP32 BVH4 nodes for the
+// any-hit kernels), tris its leaf-order triangles; rays (device) and out
+// (device: compact hits for nearest, bytes for any-hit) are the request's.
+struct ServeSeg {
+  const void* nodes;
+  const TriPre* tris;
+  const Ray* rays;
+  void* out;
+  uint32_t any;  // 1: any-hit query
+  uint32_t pad[3];
+};
+static_assert(sizeof(ServeSeg) == 48, "ServeSeg is 48 bytes");
+// The any-hit rays of nseg requests (ray offsets off[0..nseg]) over their
+// BVH4 trees (FP16 or FP32 nodes for all of them), one persistent launch of
+// at most `grid` workgroups; *queue must be zero, spill holds
+// grid * 64 * (kStackDepth4 - kShortStack) words.
+hipError_t serve_any4(bool fp16, const uint32_t* off, const ServeSeg* segs, uint32_t nseg, uint32_t nrays,
+                      uint32_t grid, uint32_t* queue, uint32_t* spill, hipStream_t stream);
+uint32_t serve_any4_resident_grid(bool fp16);
+// The tree hrt_trace_any walks on a committed scene with one identity
+// instance: kind 0 = BVH2 node pairs, 1 = FP16 BVH4 nodes, 2 = FP32 BVH4
+// nodes.
+struct TreeRef {
+  const void* nodes = nullptr;
+  const TriPre* tris = nullptr;
+  int kind = 0;
+};
+TreeRef scene_any_hit_tree(const hrt_scene* s);
+// A server holds the scene (count > 0): its mutators refuse to run.
+void scene_serve_ref(hrt_scene* s, int delta);
+
 // Ambient-occlusion rays from primary hits (compact records), spp per pixel.
 hipError_t rtao_make_rays(const Blas& b, const Ray* prim_rays, const void* prim_hits_compact, uint64_t npix,
                           uint32_t spp, float tmax, uint64_t seed, Ray* out, hipStream_t stream);

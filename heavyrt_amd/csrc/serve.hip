@@ -7,24 +7,27 @@
 //
 //   submit   the request joins a FIFO with its arrival time
 //   admit    the scheduler thread forms a STEP from the head of the FIFO:
-//            requests are admitted while their ray and result buffers fit in
-//            what is left of the HBM budget (first-fit in one device arena)
-//            and the step has room; a request that does not fit waits for
-//            memory freed by earlier ones, the order is kept
-//   run      one step = copy every admitted request's rays in, ONE persistent
-//            trace launch over all of them (each ray finds its request by a
-//            binary search over the step's ray offsets), copy every result
+//            requests are admitted while their staging buffers fit in what
+//            is left of the HBM budget (first-fit in one device arena) and
+//            the step has room; a request that does not fit waits for memory
+//            freed by earlier ones, the order is kept.  Requests whose rays
+//            and results are already device buffers need no staging.
+//   run      one step = copy the host requests' rays in, then one persistent
+//            trace launch per kind of tree over all of the step's requests
+//            of that kind (each ray finds its request by a binary search over
+//            the step's ray offsets), then copy the host requests' results
 //            out; up to `slots` steps are in flight on their own streams, so
 //            requests that arrive while others are on the GPU are admitted
 //            into the next step and their work overlaps on the device
 //   retire   when a step's event has completed its requests are done and
 //            their memory returns to the budget
 //
-// Each request's rays are walked exactly as hrt_trace_nearest /
-// hrt_trace_any walk them on the scene's BVH2 (BVH2CommonRT.cpp:49-141,
-// first strictly closer hit; any-hit stops at the first hit), so a result is
-// the same alone and in any mix.  Request buffers are pinned host memory
-// (hrt_host_alloc), copied with asynchronous DMA.
+// Each request's rays walk the tree hrt_trace_nearest / hrt_trace_any walk
+// on that scene: nearest hits on the BVH2 in BVH2CommonRT's order
+// (BVH2CommonRT.cpp:49-141, first strictly closer hit), any-hit on the
+// scene's any-hit tree (the FP16 BVH4 of the any-hit tree for
+// HRT_LAYOUT_AUTO_FP16, else its BVH4 nodes, else its BVH2), stopping at the
+// first hit.  So a result is the same alone and in any mix.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -47,17 +50,6 @@ constexpr int kWaveS = 64;
 constexpr int kRefillS = 16;
 constexpr uint32_t kClaimS = 128;
 constexpr uint32_t kMaxPerStep = 256;
-
-// One request inside a step (48 bytes).
-struct ServeSeg {
-  const NodePair* pairs;
-  const TriPre* tris;
-  const Ray* rays;
-  void* out;        // uint2 (t, primId) per ray for nearest, uint8 for any-hit
-  uint32_t any;     // 1: any-hit query
-  uint32_t pad[3];
-};
-static_assert(sizeof(ServeSeg) == 48, "ServeSeg is 48 bytes");
 
 // Step table in device memory: [queue counter + pad: 64 words]
 // [ray offsets: kMaxPerStep + 1 words, padded] [segments].
@@ -110,7 +102,7 @@ __global__ __launch_bounds__(kWaveS) void k_serve(const uint8_t* __restrict__ ta
         const uint32_t sg = seg_of_step(off, nseg, rid);
         const ServeSeg& S = segs[sg];
         loc = rid - off[sg];
-        P = S.pairs;
+        P = static_cast<const NodePair*>(S.nodes);
         Tr = S.tris;
         out = S.out;
         any_out = S.any != 0;
@@ -180,14 +172,17 @@ namespace {
 using hrt::fail;
 using hrt::fail_hip;
 
+constexpr int kKinds = 3;  // trees a step launches over: BVH2 pairs, FP16 BVH4, FP32 BVH4
+
 struct Request {
   uint64_t id = 0;
   uint32_t scene = 0;
   int any = 0;
-  const float* rays = nullptr;  // pinned host
-  void* out = nullptr;          // pinned host
+  const float* rays = nullptr;  // pinned host or device
+  void* out = nullptr;          // pinned host or device
+  bool rays_dev = false, out_dev = false;
   uint32_t n = 0;
-  size_t bytes = 0;             // device bytes: rays + results
+  size_t bytes = 0;             // staging bytes in the arena (host buffers only)
   size_t dev_off = 0;
   hrt::Clock::time_point t_submit, t_admit, t_done;
   int state = 0;                // 0 queued, 1 running, 2 done, 3 failed
@@ -202,6 +197,10 @@ struct Arena {  // first-fit over [0, size) with coalescing free list
     free[0] = sz;
   }
   bool alloc(size_t n, size_t* off) {
+    if (n == 0) {
+      *off = 0;
+      return true;
+    }
     for (auto it = free.begin(); it != free.end(); ++it) {
       if (it->second >= n) {
         *off = it->first;
@@ -214,6 +213,7 @@ struct Arena {  // first-fit over [0, size) with coalescing free list
     return false;
   }
   void release(size_t off, size_t n) {
+    if (n == 0) return;
     auto it = free.emplace(off, n).first;
     auto nx = std::next(it);
     if (nx != free.end() && it->first + it->second == nx->first) {
@@ -233,23 +233,32 @@ struct Arena {  // first-fit over [0, size) with coalescing free list
 struct Step {
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
-  uint8_t* tab_host = nullptr;  // pinned
+  uint8_t* tab_host = nullptr;  // pinned, kKinds tables
   uint8_t* tab_dev = nullptr;
   uint32_t* spill = nullptr;
   std::vector<uint64_t> reqs;
   bool busy = false;
 };
 
+// A resident scene as the server reads it (pointers taken at hrt_server_add_scene;
+// the scene refuses changes while a server holds it).
+struct ServedScene {
+  hrt_scene* scene = nullptr;
+  const hrt::NodePair* pairs = nullptr;
+  const hrt::TriPre* tris = nullptr;
+  hrt::TreeRef any;  // the tree any-hit requests walk
+};
+
 }  // namespace
 
 struct hrt_server {
   hrt_ctx* ctx = nullptr;
-  std::vector<const hrt::Blas*> scenes;
+  std::vector<ServedScene> scenes;
   size_t budget = 0;
   uint8_t* arena_dev = nullptr;
   Arena arena;
   std::vector<Step> steps;
-  uint32_t grid = 1;
+  uint32_t grid[kKinds] = {1, 1, 1};
   size_t spill_words = 0;
   std::mutex mu;
   std::condition_variable cv_work, cv_done;
@@ -259,7 +268,7 @@ struct hrt_server {
   bool stop = false;
   std::thread worker;
   // statistics
-  uint64_t completed = 0, rays_done = 0, nsteps = 0, step_reqs = 0;
+  uint64_t completed = 0, failed = 0, rays_done = 0, nsteps = 0, step_reqs = 0;
   size_t in_flight = 0, peak_in_flight = 0;
   hipError_t fault = hipSuccess;
 };
@@ -268,49 +277,93 @@ namespace {
 
 size_t align_up(size_t x, size_t a) { return (x + a - 1) / a * a; }
 
-size_t request_bytes(uint32_t n, int any) { return align_up(32ull * n, 256) + align_up((any ? 1ull : 8ull) * n, 256); }
+size_t result_bytes(uint32_t n, int any) { return (any ? 1ull : 8ull) * n; }
+
+// Arena bytes of a request: staging for whichever of its buffers is in host memory.
+size_t staging_bytes(const Request& r) {
+  return (r.rays_dev ? 0 : align_up(32ull * r.n, 256)) + (r.out_dev ? 0 : align_up(result_bytes(r.n, r.any), 256));
+}
 
 // Launches one step with the admitted requests (lock held by the caller).
 hipError_t launch_step(hrt_server* sv, Step& st) {
   using namespace hrt;
   const uint32_t k = uint32_t(st.reqs.size());
-  uint32_t* words = reinterpret_cast<uint32_t*>(st.tab_host);
-  std::memset(st.tab_host, 0, 256);  // queue counter
-  uint32_t* off = words + kTabOffsWords;
-  ServeSeg* segs = reinterpret_cast<ServeSeg*>(st.tab_host + kTabSegByte);
-  uint32_t R = 0;
+  uint32_t nk[kKinds] = {0, 0, 0}, rk[kKinds] = {0, 0, 0};
   hipError_t e = hipSuccess;
+  for (int t = 0; t < kKinds; t++) std::memset(st.tab_host + t * kTabBytes, 0, 256);  // queue counters
   for (uint32_t i = 0; i < k && e == hipSuccess; i++) {
     Request& q = sv->reqs[st.reqs[i]];
-    const Blas* b = sv->scenes[q.scene];
+    const ServedScene& sc = sv->scenes[q.scene];
     uint8_t* base = sv->arena_dev + q.dev_off;
-    off[i] = R;
-    R += q.n;
+    const Ray* d_rays = q.rays_dev ? reinterpret_cast<const Ray*>(q.rays) : reinterpret_cast<const Ray*>(base);
+    void* d_out = q.out_dev ? q.out : base + (q.rays_dev ? 0 : align_up(32ull * q.n, 256));
     ServeSeg sg;
     std::memset(&sg, 0, sizeof(sg));
-    sg.pairs = b->pairs;
-    sg.tris = b->tris;
-    sg.rays = reinterpret_cast<const Ray*>(base);
-    sg.out = base + align_up(32ull * q.n, 256);
+    int kind = 0;
+    if (q.any && sc.any.kind != 0) {
+      kind = sc.any.kind;
+      sg.nodes = sc.any.nodes;
+      sg.tris = sc.any.tris;
+    } else {
+      sg.nodes = sc.pairs;
+      sg.tris = sc.tris;
+    }
+    sg.rays = d_rays;
+    sg.out = d_out;
     sg.any = q.any ? 1u : 0u;
-    segs[i] = sg;
-    e = hipMemcpyAsync(base, q.rays, 32ull * q.n, hipMemcpyHostToDevice, st.stream);
+    uint8_t* tab = st.tab_host + kind * kTabBytes;
+    uint32_t* off = reinterpret_cast<uint32_t*>(tab) + kTabOffsWords;
+    off[nk[kind]] = rk[kind];
+    reinterpret_cast<ServeSeg*>(tab + kTabSegByte)[nk[kind]] = sg;
+    nk[kind]++;
+    rk[kind] += q.n;
+    if (!q.rays_dev) e = hipMemcpyAsync(base, q.rays, 32ull * q.n, hipMemcpyHostToDevice, st.stream);
   }
-  off[k] = R;
-  if (e == hipSuccess) e = hipMemcpyAsync(st.tab_dev, st.tab_host, kTabBytes, hipMemcpyHostToDevice, st.stream);
-  if (e == hipSuccess && R > 0) {
-    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(sv->grid, (R + kWaveS - 1) / kWaveS));
-    k_serve<<<grid, kWaveS, 0, st.stream>>>(st.tab_dev, k, R, st.spill);
-    e = hipGetLastError();
+  for (int t = 0; t < kKinds; t++) (reinterpret_cast<uint32_t*>(st.tab_host + t * kTabBytes) + kTabOffsWords)[nk[t]] = rk[t];
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(st.tab_dev, st.tab_host, kKinds * kTabBytes, hipMemcpyHostToDevice, st.stream);
+  for (int t = 0; t < kKinds && e == hipSuccess; t++) {
+    if (rk[t] == 0) continue;
+    uint8_t* tab = st.tab_dev + t * kTabBytes;
+    const uint32_t grid = std::max<uint32_t>(1, std::min<uint32_t>(sv->grid[t], (rk[t] + kWaveS - 1) / kWaveS));
+    if (t == 0) {
+      k_serve<<<grid, kWaveS, 0, st.stream>>>(tab, nk[t], rk[t], st.spill);
+      e = hipGetLastError();
+    } else {
+      e = serve_any4(t == 1, reinterpret_cast<const uint32_t*>(tab) + kTabOffsWords,
+                     reinterpret_cast<const ServeSeg*>(tab + kTabSegByte), nk[t], rk[t], grid,
+                     reinterpret_cast<uint32_t*>(tab), st.spill, st.stream);
+    }
   }
   for (uint32_t i = 0; i < k && e == hipSuccess; i++) {
     Request& q = sv->reqs[st.reqs[i]];
-    uint8_t* base = sv->arena_dev + q.dev_off;
-    e = hipMemcpyAsync(q.out, base + align_up(32ull * q.n, 256), (q.any ? 1ull : 8ull) * q.n,
-                       hipMemcpyDeviceToHost, st.stream);
+    if (q.out_dev) continue;
+    uint8_t* base = sv->arena_dev + q.dev_off + (q.rays_dev ? 0 : align_up(32ull * q.n, 256));
+    e = hipMemcpyAsync(q.out, base, result_bytes(q.n, q.any), hipMemcpyDeviceToHost, st.stream);
   }
   if (e == hipSuccess) e = hipEventRecord(st.done, st.stream);
   return e;
+}
+
+// Completes a step's requests (state 2 done, 3 failed) and returns their
+// memory to the budget.
+void retire_step(hrt_server* sv, Step& st, bool ok) {
+  const auto now = hrt::Clock::now();
+  for (uint64_t id : st.reqs) {
+    Request& r = sv->reqs[id];
+    r.state = ok ? 2 : 3;
+    r.t_done = now;
+    sv->arena.release(r.dev_off, r.bytes);
+    sv->in_flight -= r.bytes;
+    if (ok) {
+      sv->completed++;
+      sv->rays_done += r.n;
+    } else {
+      sv->failed++;
+    }
+  }
+  st.reqs.clear();
+  st.busy = false;
 }
 
 void worker_loop(hrt_server* sv) {
@@ -323,19 +376,8 @@ void worker_loop(hrt_server* sv) {
       if (!st.busy) continue;
       hipError_t q = hipEventQuery(st.done);
       if (q == hipErrorNotReady) continue;
-      const auto now = hrt::Clock::now();
-      for (uint64_t id : st.reqs) {
-        Request& r = sv->reqs[id];
-        r.state = (q == hipSuccess) ? 2 : 3;
-        r.t_done = now;
-        sv->arena.release(r.dev_off, r.bytes);
-        sv->in_flight -= r.bytes;
-        sv->completed++;
-        sv->rays_done += r.n;
-      }
       if (q != hipSuccess && sv->fault == hipSuccess) sv->fault = q;
-      st.reqs.clear();
-      st.busy = false;
+      retire_step(sv, st, q == hipSuccess);
       progressed = true;
     }
     if (progressed) sv->cv_done.notify_all();
@@ -356,11 +398,18 @@ void worker_loop(hrt_server* sv) {
           sv->queue.pop_front();
         }
         if (st.reqs.empty()) break;
-        hipError_t e = launch_step(sv, st);
-        st.busy = true;
         sv->nsteps++;
         sv->step_reqs += st.reqs.size();
-        if (e != hipSuccess && sv->fault == hipSuccess) sv->fault = e;
+        st.busy = true;
+        hipError_t e = launch_step(sv, st);
+        if (e != hipSuccess) {
+          // nothing of the step may have run: its requests fail now (an
+          // event that was never recorded would read as complete)
+          if (sv->fault == hipSuccess) sv->fault = e;
+          (void)hipStreamSynchronize(st.stream);
+          retire_step(sv, st, false);
+          sv->cv_done.notify_all();
+        }
         progressed = true;
       }
     }
@@ -392,7 +441,21 @@ void destroy_server(hrt_server* sv) {
     if (st.spill) (void)hipFree(st.spill);
   }
   if (sv->arena_dev) (void)hipFree(sv->arena_dev);
+  for (ServedScene& sc : sv->scenes) hrt::scene_serve_ref(sc.scene, -1);
   delete sv;
+}
+
+// Where a request buffer lives: 0 pinned host memory, 1 device memory of the
+// server's device, -1 neither (pageable host memory, another device).
+int buffer_kind(const void* p, int device) {
+  hipPointerAttribute_t at;
+  if (hipPointerGetAttributes(&at, p) != hipSuccess) {
+    (void)hipGetLastError();
+    return -1;
+  }
+  if (at.type == hipMemoryTypeHost) return 0;
+  if (at.type == hipMemoryTypeDevice && at.device == device) return 1;
+  return -1;
 }
 
 }  // namespace
@@ -423,19 +486,23 @@ hrt_status hrt_server_create(hrt_ctx* ctx, uint64_t budgetBytes, uint32_t stepsI
   sv->budget = align_up(budgetBytes, 256);
   hipError_t e = hipSetDevice(ctx->device);
   if (e == hipSuccess) e = hipMalloc(&sv->arena_dev, sv->budget);
-  int dev = 0, cus = 256, per_cu = 8;
-  (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  (void)hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_serve), kWaveS, 0);
   // steps in flight share the chip: each gets its share of the resident waves
-  sv->grid = uint32_t(std::max(1, cus * std::max(per_cu, 1) / int(stepsInFlight)));
-  sv->spill_words = size_t(sv->grid) * kWaveS * (kStackDepth - kShortStack);
+  const uint32_t resident[kKinds] = {resident_grid(reinterpret_cast<const void*>(&k_serve), kWaveS),
+                                     serve_any4_resident_grid(true), serve_any4_resident_grid(false)};
+  uint32_t gmax = 1;
+  for (int t = 0; t < kKinds; t++) {
+    sv->grid[t] = std::max<uint32_t>(1, resident[t] / stepsInFlight);
+    gmax = std::max(gmax, sv->grid[t]);
+  }
+  // spill words per lane: the deeper of the BVH2 and BVH4 stacks beyond LDS
+  sv->spill_words = size_t(gmax) * kWaveS * size_t(std::max(kStackDepth, kStackDepth4) - kShortStack);
   sv->steps.resize(stepsInFlight);
   for (Step& st : sv->steps) {
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&st.stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&st.done, hipEventDisableTiming);
-    if (e == hipSuccess) e = hipHostMalloc(reinterpret_cast<void**>(&st.tab_host), kTabBytes, hipHostMallocDefault);
-    if (e == hipSuccess) e = hipMalloc(&st.tab_dev, kTabBytes);
+    if (e == hipSuccess)
+      e = hipHostMalloc(reinterpret_cast<void**>(&st.tab_host), kKinds * kTabBytes, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc(&st.tab_dev, kKinds * kTabBytes);
     if (e == hipSuccess) e = hipMalloc(&st.spill, 4 * sv->spill_words);
   }
   if (e != hipSuccess) {
@@ -467,8 +534,14 @@ hrt_status hrt_server_add_scene(hrt_server* sv, hrt_scene* scene, uint32_t* scen
   if (st != HRT_OK) return st;
   if (hrt::scene_triangle_test(scene) != HRT_TRI_MT)
     return fail(HRT_ERR_UNSUPPORTED, "the server traces with the reference's triangle test");
+  ServedScene sc;
+  sc.scene = scene;
+  sc.pairs = b->pairs;
+  sc.tris = b->tris;
+  sc.any = hrt::scene_any_hit_tree(scene);
   std::lock_guard<std::mutex> lk(sv->mu);
-  sv->scenes.push_back(b);
+  hrt::scene_serve_ref(scene, +1);
+  sv->scenes.push_back(sc);
   if (sceneId) *sceneId = uint32_t(sv->scenes.size() - 1);
   return HRT_OK;
 }
@@ -478,13 +551,10 @@ hrt_status hrt_server_submit(hrt_server* sv, uint32_t sceneId, int anyHit, const
   if (!sv || !requestId) return fail(HRT_ERR_INVALID_ARG, "null argument");
   if (n == 0 || n > 0x7FFFFFFFull) return fail(HRT_ERR_INVALID_ARG, "1 to 2^31-1 rays per request");
   if (!rays || !out) return fail(HRT_ERR_INVALID_ARG, "rays/out is null");
-  for (const void* p : {static_cast<const void*>(rays), static_cast<const void*>(out)}) {
-    hipPointerAttribute_t at;
-    if (hipPointerGetAttributes(&at, p) != hipSuccess || at.type != hipMemoryTypeHost) {
-      (void)hipGetLastError();
-      return fail(HRT_ERR_INVALID_ARG, "request buffers must be pinned host memory (hrt_host_alloc)");
-    }
-  }
+  const int kr = buffer_kind(rays, sv->ctx->device), ko = buffer_kind(out, sv->ctx->device);
+  if (kr < 0 || ko < 0)
+    return fail(HRT_ERR_INVALID_ARG,
+                "request buffers must be pinned host memory (hrt_host_alloc) or device memory of the server's device");
   std::lock_guard<std::mutex> lk(sv->mu);
   if (sceneId >= sv->scenes.size()) return fail(HRT_ERR_INVALID_ARG, "no such scene");
   if (sv->fault != hipSuccess) return fail_hip(sv->fault, "server");
@@ -494,8 +564,10 @@ hrt_status hrt_server_submit(hrt_server* sv, uint32_t sceneId, int anyHit, const
   r.any = anyHit ? 1 : 0;
   r.rays = rays;
   r.out = out;
+  r.rays_dev = kr == 1;
+  r.out_dev = ko == 1;
   r.n = uint32_t(n);
-  r.bytes = request_bytes(r.n, r.any);
+  r.bytes = staging_bytes(r);
   if (r.bytes > sv->budget) return fail(HRT_ERR_OUT_OF_MEMORY, "request larger than the server's HBM budget");
   r.t_submit = hrt::Clock::now();
   sv->reqs[r.id] = r;
@@ -522,7 +594,7 @@ hrt_status hrt_server_wait(hrt_server* sv, uint64_t requestId, double* latencyMs
   sv->cv_done.wait(lk, [&] { return it->second.state >= 2; });
   const Request r = it->second;
   sv->reqs.erase(it);
-  if (r.state == 3) return fail_hip(sv->fault, "request");
+  if (r.state == 3) return fail_hip(sv->fault == hipSuccess ? hipErrorUnknown : sv->fault, "request");
   using ms = std::chrono::duration<double, std::milli>;
   if (latencyMs) *latencyMs = ms(r.t_done - r.t_submit).count();
   if (queueMs) *queueMs = ms(r.t_admit - r.t_submit).count();

@@ -31,22 +31,13 @@ constexpr int kRefillAt4 = 40;
 // to global memory).  Deep, incoherent nearest-hit walks push often and
 // measured slower with spills, so BVH4 keeps its whole stack in LDS.
 constexpr int kSimpleLds4 = 48;
-// LDS stack entries per lane of the refilling nearest-hit kernel.
-#ifndef HRT_NEAREST4_LDS
-#define HRT_NEAREST4_LDS 32
-#endif
-constexpr int kNearestLds4 = HRT_NEAREST4_LDS;
-// Nearest-hit batches larger than one resident grid: the refilling kernel
-// (1) or one ray per lane (0).
-#ifndef HRT_NEAREST4_PERSISTENT
-#define HRT_NEAREST4_PERSISTENT 1
-#endif
 // Nearest-hit batches of at least this many rays are sorted into coherent
-// waves first (0: never).
-#ifndef HRT_SORT_RAYS4_MIN
-#define HRT_SORT_RAYS4_MIN 0
-#endif
-constexpr uint64_t kSortRaysMin4 = HRT_SORT_RAYS4_MIN;
+// waves before they are traced.  On 4,194,304 incoherent rays in a
+// 4,194,304-triangle soup the FP16 walk fetches 6 % fewer bytes from beyond
+// the L2 and takes 21.8 instead of 24.0 ms; a refilling persistent kernel
+// (wave-local ray pool, speculative while-while) was no faster there (24.8
+// ms unsorted, 23.3 sorted): the walk waits on memory, not on idle lanes.
+constexpr uint64_t kSortRaysMin4 = 1u << 20;
 // The speculative any-hit kernels leave the node loop for the leaf phase once
 // this many lanes or fewer still look for a leaf (0 would wait for the last).
 constexpr uint32_t kLeafPhaseAt = 8;
@@ -216,20 +207,46 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_simple(const void* __restric
 // (Nearest hits stay on k_trace4_simple: on deep, heavily overlapping trees
 // its 48-entry LDS stack beats a short LDS stack with global spill.)
 // Src gives ray i: StoredRays reads it from HBM, AoRays makes AO sample i
-// from its pixel's frame.  A ray with an empty interval (tFar <= tNear, or a
-// NaN bound) can pass no triangle's t test and finishes at once.
+// from its pixel's frame; both trace every ray in the launch's tree and
+// write answer i at out[i].  ServeRays (kSegments) belongs to the server:
+// ray i is ray `slot` of one request, which names its own tree and answer
+// buffer.  A ray with an empty interval (tFar <= tNear, or a NaN bound) can
+// pass no triangle's t test and finishes at once.
 struct StoredRays {
+  static constexpr bool kSegments = false;
   const Ray* __restrict__ rays;
   __device__ Ray ray(uint64_t i) const { return rays[i]; }
 };
 
 struct AoRays {
+  static constexpr bool kSegments = false;
   const AoFrame* __restrict__ frames;
   uint32_t spp;
   float tmax;
   uint64_t seed;
   // 32-bit division: the launcher keeps npix * spp below 2^32
   __device__ Ray ray(uint64_t i) const { return ao_ray(frames[uint32_t(i) / spp], i, tmax, seed); }
+};
+
+struct ServeRays {
+  static constexpr bool kSegments = true;
+  const uint32_t* __restrict__ off;  // nseg + 1 ray offsets
+  const ServeSeg* __restrict__ segs;
+  uint32_t nseg;
+  __device__ Ray ray(uint64_t i, const void*& nodes, const TriPre*& tris, uint8_t*& out, uint64_t& slot) const {
+    uint32_t lo = 0, hi = nseg;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (off[mid] <= uint32_t(i)) lo = mid;
+      else hi = mid;
+    }
+    const ServeSeg& S = segs[lo];
+    nodes = S.nodes;
+    tris = S.tris;
+    out = static_cast<uint8_t*>(S.out);
+    slot = i - off[lo];
+    return S.rays[slot];
+  }
 };
 
 template <bool kFp16, class Src, class Tri = MtTest>
@@ -257,6 +274,11 @@ __global__ __launch_bounds__(kBlock4, 8) void k_trace4_any_persistent(const void
   uint32_t cur = kInvalid;
   uint32_t leaf = 0, leaf2 = 0;  // parked leaves (leaf2 only while leaf is held)
   RayPool pool;  // ray indices claimed by this wave
+  // the tree and answer buffer of this lane's ray (a per-request source);
+  // a launch-wide source keeps the kernel arguments
+  const void* lane_quads = quads;
+  const TriPre* lane_tris = tris;
+  uint8_t* lane_out = out;
 
   while (true) {
     const uint64_t idle = __ballot(!active);
@@ -265,8 +287,13 @@ __global__ __launch_bounds__(kBlock4, 8) void k_trace4_any_persistent(const void
       const uint64_t idx = pool.take<kClaim>(nidle, __popcll(idle & lt_mask), queue, n);
       if (!active) {
         if (idx < n) {
-          rid = idx;
-          const Ray r = src.ray(idx);
+          Ray r;
+          if constexpr (Src::kSegments) {
+            r = src.ray(idx, lane_quads, lane_tris, lane_out, rid);
+          } else {
+            rid = idx;
+            r = src.ray(idx);
+          }
           o = make_float3(r.pos_near.x, r.pos_near.y, r.pos_near.z);
           d = make_float3(r.dir_far.x, r.dir_far.y, r.dir_far.z);
           tnear = r.pos_near.w;
@@ -305,7 +332,7 @@ __global__ __launch_bounds__(kBlock4, 8) void k_trace4_any_persistent(const void
         if (active && (cur & kLeafBit) == 0) {
           float tm[4];
           uint32_t ch[4];
-          visit_quad<kFp16, kFp16>(quads, cur, o, inv, oi, fz, tnear, h.t, tm, ch);
+          visit_quad<kFp16, kFp16>(Src::kSegments ? lane_quads : quads, cur, o, inv, oi, fz, tnear, h.t, tm, ch);
           if (tm[3] < kMaxFloat) stk.push(top, ch[3]);
           if (tm[2] < kMaxFloat) stk.push(top, ch[2]);
           if (tm[1] < kMaxFloat) stk.push(top, ch[1]);
@@ -315,125 +342,20 @@ __global__ __launch_bounds__(kBlock4, 8) void k_trace4_any_persistent(const void
       }
       // leaf phase
       if (active && leaf != 0) {
+        const TriPre* T = Src::kSegments ? lane_tris : tris;
         const uint32_t s = leaf_start(leaf), c = leaf_count(leaf);
         bool hit = false;
-        for (uint32_t k = s; k < s + c && !hit; k++) hit = tri_test_lex(tt, o, d, tnear, tris[k], h);
+        for (uint32_t k = s; k < s + c && !hit; k++) hit = tri_test_lex(tt, o, d, tnear, T[k], h);
         if (!hit && leaf2 != 0) {
           const uint32_t s2 = leaf_start(leaf2), c2 = leaf_count(leaf2);
           for (uint32_t k = s2; k < s2 + c2; k++)
-            if (tri_test_lex(tt, o, d, tnear, tris[k], h)) break;
+            if (tri_test_lex(tt, o, d, tnear, T[k], h)) break;
         }
         leaf = 0;
         leaf2 = 0;
       }
       if (active && (h.prim != kInvalid || cur == kInvalid)) {
-        out[rid] = (h.prim != kInvalid) ? 1 : 0;
-        active = false;
-      }
-      const uint32_t nact = __popcll(__ballot(active));
-      if (nact == 0 || (!pool.drained() && kWave - nact >= kRefillAt4)) break;
-    }
-  }
-}
-
-// Persistent nearest-hit over BVH4 nodes with ballot-based refill from a
-// wave-local ray pool.  One ray per lane keeps a wave until its slowest ray
-// finishes; on incoherent rays (hundreds of node visits, widely spread)
-// refilling idle lanes keeps the waves full instead.  The walk is the
-// speculative while-while of the any-hit kernel: a lane parks the first leaf
-// it reaches and keeps descending while the wave still searches, then the
-// wave tests its parked leaves together.  The nearest hit under the
-// (t, primId) rule does not depend on the order nodes and leaves are visited
-// in: boxes are culled against the current best t, which only decreases, and
-// a box is kept when its entry distance equals it, so no box holding a
-// triangle at or below the final t is ever skipped.  The answer is the one
-// k_trace4_simple gives.  A ray with tFar <= 0 (any-hit by the ray's own
-// convention) stops at its first hit, as there.
-// kLds stack entries per lane live in LDS, the rest (to kStackDepth4) in a
-// per-lane global spill area.
-template <bool kFp16, int kLds, class Tri = MtTest>
-__global__ __launch_bounds__(kBlock4) void k_trace4_nearest_persistent(const void* __restrict__ quads,
-                                                                       const TriPre* __restrict__ tris,
-                                                                       const Ray* __restrict__ rays, uint64_t n,
-                                                                       void* out, int format, uint2 ids,
-                                                                       uint32_t* __restrict__ queue,
-                                                                       uint32_t* __restrict__ spill,
-                                                                       const uint32_t* const* __restrict__ order_slot) {
-  __shared__ uint32_t stack_mem[kLds * kBlock4];
-  const uint32_t* __restrict__ order = order_slot ? *order_slot : nullptr;
-  const int lane = threadIdx.x & (kWave - 1);
-  const ShortStack<kLds, kBlock4> stk{stack_mem + threadIdx.x, spill + blockIdx.x * kBlock4 + threadIdx.x,
-                                      gridDim.x * kBlock4};
-  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
-  bool active = false, any = false;
-  uint64_t rid = 0;
-  float3 o = make_float3(0.f, 0.f, 0.f), d = o, inv = o;
-  float tnear = 0.f;
-  Tri tt;
-  HitRec h;
-  h.t = 0.f;
-  h.prim = kInvalid;
-  h.u = h.v = 0.f;
-  int top = 0;
-  uint32_t cur = kInvalid;
-  uint32_t leaf = 0;  // parked leaf, 0 = none
-  RayPool pool;
-
-  while (true) {
-    const uint64_t idle = __ballot(!active);
-    const uint32_t nidle = __popcll(idle);
-    if (!pool.drained() && (nidle >= kRefillAt4 || nidle == kWave)) {
-      const uint64_t slot = pool.take<kClaim>(nidle, __popcll(idle & lt_mask), queue, n);
-      const uint64_t idx = (order && slot < n) ? uint64_t(order[slot]) : slot;
-      if (!active && idx < n) {
-        rid = idx;
-        const Ray r = rays[idx];
-        o = make_float3(r.pos_near.x, r.pos_near.y, r.pos_near.z);
-        d = make_float3(r.dir_far.x, r.dir_far.y, r.dir_far.z);
-        tnear = r.pos_near.w;
-        any = r.dir_far.w <= 0.0f;
-        h.t = any ? -r.dir_far.w : r.dir_far.w;
-        h.prim = kInvalid;
-        h.u = h.v = 0.f;
-        inv = safe_inverse(d);
-        tt = Tri(o, d);
-        top = 0;
-        // an empty interval admits no hit: t > tNear and t < tFar
-        cur = (h.t > tnear) ? 0u : kInvalid;
-        leaf = 0;
-        active = true;
-      }
-    }
-    if (__ballot(active) == 0) break;
-
-    while (true) {
-      while (true) {
-        if (active && leaf == 0 && (cur & kLeafBit) != 0 && cur != kInvalid) {
-          leaf = cur;
-          cur = (top > 0) ? stk.pop(top) : kInvalid;
-        }
-        const bool searching = active && leaf == 0 && (cur & kLeafBit) == 0;
-        const uint32_t nsearch = __popcll(__ballot(searching));
-        if (nsearch == 0 || (nsearch <= kLeafPhaseAt && __any(leaf != 0))) break;
-        if (active && (cur & kLeafBit) == 0) {
-          float tm[4];
-          uint32_t ch[4];
-          visit_quad<kFp16>(quads, cur, o, inv, o, true, tnear, h.t, tm, ch);
-          if (tm[3] < kMaxFloat) stk.push(top, ch[3]);
-          if (tm[2] < kMaxFloat) stk.push(top, ch[2]);
-          if (tm[1] < kMaxFloat) stk.push(top, ch[1]);
-          if (tm[0] < kMaxFloat) cur = ch[0];
-          else cur = (top > 0) ? stk.pop(top) : kInvalid;
-        }
-      }
-      if (active && leaf != 0) {
-        const uint32_t s = leaf_start(leaf), c = leaf_count(leaf);
-        for (uint32_t k = s; k < s + c; k++)
-          if (tri_test_lex(tt, o, d, tnear, tris[k], h) && any) break;
-        leaf = 0;
-      }
-      if (active && ((any && h.prim != kInvalid) || (leaf == 0 && cur == kInvalid))) {
-        write_hit<false>(out, rid, format, ids, h);
+        (Src::kSegments ? lane_out : out)[rid] = (h.prim != kInvalid) ? 1 : 0;
         active = false;
       }
       const uint32_t nact = __popcll(__ballot(active));
@@ -510,26 +432,6 @@ __global__ void k_order_or_null(const uint32_t* __restrict__ err, const uint32_t
 }
 
 template <bool kFp16, class Tri>
-hipError_t launch_nearest4(const Blas& b, const TraceArgs& a, const uint32_t* const* order_slot,
-                           hipStream_t stream) {
-  const void* kfn = reinterpret_cast<const void*>(&k_trace4_nearest_persistent<kFp16, kNearestLds4, Tri>);
-  const uint64_t want = (a.n + kBlock4 - 1) / kBlock4;
-  const int grid = int(std::min<uint64_t>(want, resident_grid(kfn, kBlock4)));
-  const void* q = kFp16 ? static_cast<const void*>(b.quads16) : static_cast<const void*>(b.quads32);
-  // queue counter + spill area for stack entries beyond the LDS part
-  const size_t spill_words = size_t(grid) * kBlock4 * (kStackDepth4 - kNearestLds4);
-  uint32_t* buf = nullptr;
-  hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&buf), sizeof(uint32_t) * (64 + spill_words), stream);
-  if (e != hipSuccess) return e;
-  k_reset_counter4<<<1, 1, 0, stream>>>(buf);
-  k_trace4_nearest_persistent<kFp16, kNearestLds4, Tri><<<grid, kBlock4, 0, stream>>>(
-      q, tri_records<Tri>(b), a.rays, a.n, a.out, a.format, make_uint2(a.inst_id, a.geom_id), buf, buf + 64, order_slot);
-  e = hipGetLastError();
-  hipError_t f = hipFreeAsync(buf, stream);
-  return e != hipSuccess ? e : f;
-}
-
-template <bool kFp16, class Tri>
 hipError_t launch_simple4(const Blas& b, const TraceArgs& a, const uint32_t* const* order_slot,
                           hipStream_t stream) {
   const uint32_t blocks = uint32_t((a.n + kBlock4 - 1) / kBlock4);
@@ -596,9 +498,7 @@ hipError_t launch_nearest4_sorted(const Blas& b, const TraceArgs& a, hipStream_t
     k_order_or_null<<<1, 1, 0, stream>>>(err, order, slot);
     e = hipGetLastError();
   }
-  if (e == hipSuccess)
-    e = HRT_NEAREST4_PERSISTENT ? launch_nearest4<kFp16, Tri>(b, a, slot, stream)
-                                : launch_simple4<kFp16, Tri>(b, a, slot, stream);
+  if (e == hipSuccess) e = launch_simple4<kFp16, Tri>(b, a, slot, stream);
   const hipError_t f = hipFreeAsync(base, stream);
   return e != hipSuccess ? e : f;
 }
@@ -611,16 +511,9 @@ hipError_t trace_blas4_with(const Blas& b, const TraceArgs& a, bool fp16, hipStr
     return fp16 ? launch_any4<true, StoredRays, Tri>(b, src, a.n, out, stream)
                 : launch_any4<false, StoredRays, Tri>(b, src, a.n, out, stream);
   }
-  if (!a.stats && !a.anyhit && kSortRaysMin4 > 0 && a.n >= kSortRaysMin4)
+  if (!a.stats && !a.anyhit && a.n >= kSortRaysMin4)
     return fp16 ? launch_nearest4_sorted<true, Tri>(b, a, stream) : launch_nearest4_sorted<false, Tri>(b, a, stream);
-  // one ray per lane for the counting variant and for batches whose whole
-  // grid is resident at once (nothing to refill); refilling waves otherwise
-  const uint64_t blocks = (a.n + kBlock4 - 1) / kBlock4;
-  const void* simple = fp16 ? reinterpret_cast<const void*>(&k_trace4_simple<false, true, false, Tri>)
-                            : reinterpret_cast<const void*>(&k_trace4_simple<false, false, false, Tri>);
-  if (!HRT_NEAREST4_PERSISTENT || a.stats || a.anyhit || blocks <= resident_grid(simple, kBlock4))
-    return fp16 ? launch_simple4<true, Tri>(b, a, nullptr, stream) : launch_simple4<false, Tri>(b, a, nullptr, stream);
-  return fp16 ? launch_nearest4<true, Tri>(b, a, nullptr, stream) : launch_nearest4<false, Tri>(b, a, nullptr, stream);
+  return fp16 ? launch_simple4<true, Tri>(b, a, nullptr, stream) : launch_simple4<false, Tri>(b, a, nullptr, stream);
 }
 
 }  // namespace
@@ -636,6 +529,26 @@ hipError_t trace_ao4(const Blas& b, bool fp16, bool watertight, const AoFrame* f
                 : launch_any4<false, AoRays, WtTest>(b, src, n, out, stream);
   return fp16 ? launch_any4<true, AoRays, MtTest>(b, src, n, out, stream)
               : launch_any4<false, AoRays, MtTest>(b, src, n, out, stream);
+}
+
+hipError_t serve_any4(bool fp16, const uint32_t* off, const ServeSeg* segs, uint32_t nseg, uint32_t nrays,
+                      uint32_t grid, uint32_t* queue, uint32_t* spill, hipStream_t stream) {
+  if (nrays == 0) return hipSuccess;
+  const ServeRays src{off, segs, nseg};
+  grid = std::max<uint32_t>(1, std::min<uint32_t>(grid, (nrays + kBlock4 - 1) / kBlock4));
+  if (fp16)
+    k_trace4_any_persistent<true, ServeRays, MtTest><<<grid, kBlock4, 0, stream>>>(nullptr, nullptr, src, nrays,
+                                                                                   nullptr, queue, spill);
+  else
+    k_trace4_any_persistent<false, ServeRays, MtTest><<<grid, kBlock4, 0, stream>>>(nullptr, nullptr, src, nrays,
+                                                                                    nullptr, queue, spill);
+  return hipGetLastError();
+}
+
+uint32_t serve_any4_resident_grid(bool fp16) {
+  return fp16 ? resident_grid(reinterpret_cast<const void*>(&k_trace4_any_persistent<true, ServeRays, MtTest>), kBlock4)
+              : resident_grid(reinterpret_cast<const void*>(&k_trace4_any_persistent<false, ServeRays, MtTest>),
+                              kBlock4);
 }
 
 hipError_t trace_blas4(const Blas& b, const TraceArgs& a, bool fp16, hipStream_t stream) {

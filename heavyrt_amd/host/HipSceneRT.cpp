@@ -57,6 +57,7 @@ struct HipSceneRT : public ISceneObject {
     if (m_scene) report("ClearGeom", hrt_scene_clear_geom(m_scene));
     m_geomBoxes.clear();
     m_instCount = 0;
+    m_identity.clear();
   }
 
   uint32_t AddGeom_Triangles3f(const float* a_vpos3f, size_t a_vertNumber, const uint32_t* a_triIndices,
@@ -99,6 +100,7 @@ struct HipSceneRT : public ISceneObject {
   void ClearScene() override {
     if (m_scene) report("ClearScene", hrt_scene_clear_instances(m_scene));
     m_instCount = 0;
+    m_identity.clear();
   }
 
   void CommitScene(BuildQuality a_qualityLevel) override {
@@ -111,14 +113,20 @@ struct HipSceneRT : public ISceneObject {
     if (!m_scene) return id;
     hrt_status st = hrt_add_instance(m_scene, a_geomId, a_matrix.data(), &id);
     report("AddInstance", st);
-    if (st == HRT_OK) m_instCount++;
+    if (st == HRT_OK) {
+      m_instCount++;
+      m_identity.push_back(is_identity(a_matrix));
+    }
     return st == HRT_OK ? id : 0xFFFFFFFFu;
   }
 
   // CrossRT.h:125-130: a new matrix for an existing instance; traced after
   // the next CommitScene.
   void UpdateInstance(uint32_t a_instanceId, const LiteMath::float4x4& a_matrix) override {
-    if (m_scene) report("UpdateInstance", hrt_update_instance(m_scene, a_instanceId, a_matrix.data()));
+    if (!m_scene) return;
+    const hrt_status st = hrt_update_instance(m_scene, a_instanceId, a_matrix.data());
+    report("UpdateInstance", st);
+    if (st == HRT_OK && a_instanceId < m_identity.size()) m_identity[a_instanceId] = is_identity(a_matrix);
   }
 
   CRT_Hit RayQuery_NearestHit(LiteMath::float4 posAndNear, LiteMath::float4 dirAndFar) override {
@@ -175,7 +183,11 @@ struct HipSceneRT : public ISceneObject {
   MetricStats GetStats() override {
     MetricStats s{};
     const double rays = m_rays ? double(m_rays) : 1.0;
-    const double nodes = (m_layout == 0 || m_layout == 3) ? 2.0 * double(m_pairs) : 4.0 * double(m_pairs);
+    // nearest hits walk BVH4 nodes (4 per visit) only for a BVH4 layout on a
+    // scene of one identity instance; every other scene walks BVH2 pairs
+    const bool quads = (m_layout == HRT_LAYOUT_BVH4 || m_layout == HRT_LAYOUT_BVH4_FP16) && m_identity.size() == 1 &&
+                       m_identity[0];
+    const double nodes = (quads ? 4.0 : 2.0) * double(m_pairs);
     s.avgNC = float(nodes / rays);
     s.avgTC = float(double(m_tris) / rays);
     s.avgBLB = float((32.0 * nodes + 60.0 * double(m_tris)) / rays);
@@ -210,6 +222,14 @@ struct HipSceneRT : public ISceneObject {
   std::vector<LiteMath::float4> m_geomBoxes;
   uint32_t m_instCount = 0;
   uint64_t m_pairs = 0, m_tris = 0, m_rays = 0;
+  std::vector<bool> m_identity;  // per instance: the matrix is the identity
+
+  static bool is_identity(const LiteMath::float4x4& m) {
+    for (int c = 0; c < 4; c++)
+      for (int r = 0; r < 4; r++)
+        if (m(r, c) != (r == c ? 1.0f : 0.0f)) return false;
+    return true;
+  }
 };
 
 }  // namespace

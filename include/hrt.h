@@ -132,6 +132,8 @@ void* hrt_ctx_stream(hrt_ctx* ctx);
 
 /* Replaces `new X` in CreateSceneRT (FactoryRT.cpp:29-78) / DeleteSceneRT (:80). */
 hrt_status hrt_scene_create(hrt_ctx* ctx, hrt_scene** out);
+/* Destroy the servers that hold a scene first: a scene still held by a
+ * server is kept (with a message on stderr) until the process ends. */
 void hrt_scene_destroy(hrt_scene* scene);
 /* Keep builder intermediates for hrt_debug_dump (parity hooks). */
 hrt_status hrt_scene_set_keep_build_arrays(hrt_scene* scene, int keep);
@@ -347,11 +349,15 @@ hrt_status hrt_batch_dump(hrt_batch* batch, uint32_t scene, int what, void* host
  * CrossRT.h:143-151).  A server keeps committed scenes resident and takes
  * requests (a ray batch for one scene, nearest or any) from any thread.  Its
  * scheduler thread admits queued requests in arrival order while their
- * device buffers (32 B per ray + the results) fit in the HBM budget, runs all
- * admitted requests of a step as one trace launch, keeps up to stepsInFlight
- * steps on the device at once, and admits new arrivals into the next step
- * while earlier ones run.  Results equal hrt_trace_nearest (compact hits,
- * reference order) / hrt_trace_any on the same rays, alone or in any mix. */
+ * staging buffers (32 B per ray + the results, for buffers in host memory)
+ * fit in the HBM budget, runs all admitted requests of a step as one trace
+ * launch per kind of tree, keeps up to stepsInFlight steps on the device at
+ * once, and admits new arrivals into the next step while earlier ones run.
+ * Nearest-hit requests walk the scene's BVH2 in the reference order (compact
+ * hits, equal to hrt_trace_nearest for the BVH2, AUTO and AUTO_FP16
+ * layouts); any-hit requests walk the tree hrt_trace_any walks (the any-hit
+ * tree's FP16 BVH4 for AUTO_FP16) and equal it.  Results are the same alone
+ * or in any mix. */
 typedef struct hrt_server hrt_server;
 /* Pinned host memory for request rays and results (hipHostMalloc). */
 hrt_status hrt_host_alloc(size_t bytes, void** out);
@@ -359,12 +365,15 @@ void hrt_host_free(void* p);
 hrt_status hrt_server_create(hrt_ctx* ctx, uint64_t budgetBytes, uint32_t stepsInFlight, hrt_server** out);
 /* Waits for every submitted request, then frees the server. */
 void hrt_server_destroy(hrt_server* server);
-/* A committed scene with one identity instance; it must outlive the server. */
+/* A committed scene with one identity instance and the reference's triangle
+ * test.  While a server holds it, the scene's mutators (clear, add, update,
+ * commit, set_triangle_test) fail with HRT_ERR_INVALID_ARG. */
 hrt_status hrt_server_add_scene(hrt_server* server, hrt_scene* scene, uint32_t* sceneId);
 /* rays: n x 8 floats, out: n x hrt_hit_compact (nearest) or n bytes (any-hit);
- * both pinned (hrt_host_alloc) and untouched by the caller until the request
- * is done.  Fails with HRT_ERR_OUT_OF_MEMORY if the request alone exceeds
- * the budget. */
+ * each either pinned host memory (hrt_host_alloc; staged through the HBM
+ * budget) or device memory of the server's device (read and written in
+ * place), untouched by the caller until the request is done.  Fails with
+ * HRT_ERR_OUT_OF_MEMORY if the request's staging alone exceeds the budget. */
 hrt_status hrt_server_submit(hrt_server* server, uint32_t sceneId, int anyHit, const float* rays, size_t n,
                              void* out, uint64_t* requestId);
 /* 1 = done, 0 = pending, negative = failed or unknown. */

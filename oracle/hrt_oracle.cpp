@@ -46,6 +46,7 @@
 #include <cstdio>
 #include <cstring>
 #include <limits>
+#include <unordered_set>
 #include <vector>
 
 #ifdef _OPENMP
@@ -661,21 +662,54 @@ static inline f3 vpos(const Build& b, uint32_t vi) {
 
 struct Counters { uint64_t pairs = 0, tris = 0, leaves = 0, max_stack = 0; };
 
+// The reference's traversal metrics as shipped: CrossRT.h:31 defines
+// ENABLE_METRICS, and BVH2TraverseC32 then keeps, per visited node pair,
+// NC/BLB and -- for each of the five treelet sizes (CrossRT.h:32-33) -- long
+// jumps, cache-line changes and a set of touched lines (BVH2CommonRT.cpp:
+// 57-84), per leaf LC/TC/BLB (:121-126), per push and pop SOC/SBL
+// (:108-111, :134-138).  Only the CPU baseline's "as shipped" timing uses it.
+struct RefMetrics {
+  static constexpr int kTreelets = 5;
+  static constexpr uint32_t kSizes[kTreelets] = {128, 512, 4096, 16384, 131072};
+  uint64_t NC = 0, LC = 0, TC = 0, BLB = 0, SOC = 0, SBL = 0;
+  uint64_t LJC[kTreelets] = {0}, CMC[kTreelets] = {0};
+  std::unordered_set<uint32_t> WSS[kTreelets];
+};
+constexpr uint32_t RefMetrics::kSizes[RefMetrics::kTreelets];
+
 // The traversal loop of BVH2CommonRT::BVH2TraverseC32 in object space, with
 // an incoming hit (culling uses its distance) and a triangle acceptance rule
 // accept(A, B, C, original id) that updates the hit.  Returns true when an
 // any-hit query has found its hit.
 template <class Accept>
 static bool visit_bvh2(const Build& b, f3 o, f3 d, float tNear, bool anyhit, Hit& h, Counters* cnt,
-                       Accept&& accept) {
+                       Accept&& accept, RefMetrics* m = nullptr) {
   const f3 inv = safe_inverse(d);
   uint32_t stack[80];
   int top = 0;
   uint32_t off = 0;
+  uint32_t off_old = 0;
   const Node* nodes = b.peach.data();
   while (top >= 0) {
     bool pop = false;
     if ((off & LEAF_BIT) == 0) {
+      if (m) {
+        m->NC += 2;
+        m->BLB += 2 * sizeof(Node);
+        if (off_old != off) {
+          for (int i = 0; i < RefMetrics::kTreelets; i++) {
+            const double jump = std::abs(double(off) - double(off_old)) * double(sizeof(Node));
+            if (jump >= double(RefMetrics::kSizes[i])) m->LJC[i]++;
+            const uint32_t line_old = uint32_t(off_old * sizeof(Node)) / RefMetrics::kSizes[i];
+            const uint32_t line_new = uint32_t(off * sizeof(Node)) / RefMetrics::kSizes[i];
+            if (line_old != line_new) {
+              m->CMC[i]++;
+              m->WSS[i].insert(line_new);
+            }
+          }
+          off_old = off;
+        }
+      }
       const Node n0 = nodes[off + 0];
       const Node n1 = nodes[off + 1];
       if (cnt) cnt->pairs++;
@@ -691,10 +725,19 @@ static bool visit_bvh2(const Build& b, f3 o, f3 d, float tNear, bool anyhit, Hit
         stack[top] = (a0 <= a1) ? n1.left : n0.left;
         top++;
         if (cnt && uint64_t(top) > cnt->max_stack) cnt->max_stack = uint64_t(top);
+        if (m) {
+          m->SOC++;
+          m->SBL += sizeof(uint32_t);
+        }
       }
     } else if (off != 0xFFFFFFFFu) {
       const uint32_t start = leaf_start(off), count = leaf_count(off);
       if (cnt) { cnt->leaves++; cnt->tris += count; }
+      if (m) {
+        m->LC++;
+        m->TC += count;
+        m->BLB += count * (3 * sizeof(uint32_t) + 3 * sizeof(f4));
+      }
       for (uint32_t t = start; t < start + count; t++) {
         const uint32_t* ix = &b.idx_leaf_order[3 * size_t(t)];
         accept(vpos(b, ix[0]), vpos(b, ix[1]), vpos(b, ix[2]), b.perm[t]);
@@ -705,6 +748,10 @@ static bool visit_bvh2(const Build& b, f3 o, f3 d, float tNear, bool anyhit, Hit
     if (pop) {
       top--;
       off = stack[std::max(top, 0)];
+      if (m) {
+        m->SOC++;
+        m->SBL += sizeof(uint32_t);
+      }
     }
   }
   return false;
@@ -713,7 +760,7 @@ static bool visit_bvh2(const Build& b, f3 o, f3 d, float tNear, bool anyhit, Hit
 // BVH2CommonRT::BVH2TraverseC32 over the peach layout (BVH2CommonRT.cpp:49-141).
 // primId is reported as the ORIGINAL triangle id (the REMAP_PRIM_ID rule of
 // BVH2CommonRT.cpp:270-276); the leaf-order position is perm^-1 of it.
-static Hit trace_bvh2common(const Build& b, f4 posNear, f4 dirFar, Counters* cnt) {
+static Hit trace_bvh2common(const Build& b, f4 posNear, f4 dirFar, Counters* cnt, RefMetrics* m = nullptr) {
   bool anyhit = (dirFar.w <= 0.0f);
   float tFar = anyhit ? -dirFar.w : dirFar.w;
   Hit h;
@@ -724,7 +771,7 @@ static Hit trace_bvh2common(const Build& b, f4 posNear, f4 dirFar, Counters* cnt
   const f3 d = {dirFar.x, dirFar.y, dirFar.z};
   const float tNear = posNear.w;
   visit_bvh2(b, o, d, tNear, anyhit, h, cnt,
-             [&](f3 A, f3 B, f3 C, uint32_t prim) { mt_test(o, d, tNear, A, B, C, h, prim); });
+             [&](f3 A, f3 B, f3 C, uint32_t prim) { mt_test(o, d, tNear, A, B, C, h, prim); }, m);
   return h;
 }
 
@@ -1660,6 +1707,27 @@ void hrto_trace4(void* h, int mode, const float* rays, uint64_t n, uint32_t* hit
 }
 
 void hrto_set_triangle_test(int test) { g_tri_test = test; }
+
+// BVH2CommonRT traversal on one thread with the reference's metrics
+// bookkeeping on (the "as shipped" CPU baseline).  hits as hrto_trace;
+// out[0..5]: NC, LC, TC, BLB, SOC, touched 128-byte lines (WSS[0]).
+void hrto_trace_metrics(void* h, const float* rays, uint64_t n, uint32_t* hits, uint64_t* out6) {
+  const Build* b = static_cast<Build*>(h);
+  RefMetrics m;
+  for (uint64_t i = 0; i < n; i++) {
+    f4 p, d;
+    std::memcpy(&p, rays + 8 * i, 16);
+    std::memcpy(&d, rays + 8 * i + 4, 16);
+    const Hit r = trace_bvh2common(*b, p, d, nullptr, &m);
+    std::memcpy(hits + 6 * i, &r, sizeof(Hit));
+  }
+  out6[0] = m.NC;
+  out6[1] = m.LC;
+  out6[2] = m.TC;
+  out6[3] = m.BLB;
+  out6[4] = m.SOC;
+  out6[5] = m.WSS[0].size();
+}
 
 // One triangle test: out = {t, u, v, accepted (0/1)}; test 0 = Moller-Trumbore, 1 = watertight.
 void hrto_tri_eval(int test, const float* o3, const float* d3, float tnear, const float* a3, const float* b3,

@@ -99,6 +99,8 @@ def lib():
         L.hrto_quad_count.argtypes = [vp]
         L.hrto_quads.argtypes = [vp, C.c_int, vp]
         L.hrto_trace4.argtypes = [vp, C.c_int, vp, u64, vp, vp, C.c_int]
+        L.hrto_trace_metrics.restype = None
+        L.hrto_trace_metrics.argtypes = [vp, vp, u64, vp, vp]
         L.hrto_set_triangle_test.restype = None
         L.hrto_set_triangle_test.argtypes = [C.c_int]
         L.hrto_tri_eval.restype = None
@@ -271,6 +273,16 @@ class OracleBVH:
         if counters:
             return t, prim, u, v, cnt[:, 0].copy(), cnt[:, 1].copy()
         return t, prim, u, v
+
+    def trace_metrics(self, rays: np.ndarray):
+        """BVH2CommonRT traversal on one thread with the reference's metrics on (as shipped).
+        Returns (t, primId, metrics dict)."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        hits = np.zeros((rays.shape[0], 6), dtype=np.uint32)
+        m = np.zeros(6, dtype=np.uint64)
+        lib().hrto_trace_metrics(self._h, _ptr(rays), rays.shape[0], _ptr(hits), _ptr(m))
+        names = ("NC", "LC", "TC", "BLB", "SOC", "WSS128")
+        return hits[:, 0].view(np.float32).copy(), hits[:, 1].copy(), {k: int(x) for k, x in zip(names, m)}
 
     # -- BVH4 (collapse of the same LBVH) ---------------------------------
     def quads(self, fp16: bool = False) -> np.ndarray:

@@ -4,13 +4,22 @@
 // rtao.cpp:183 for RayQuery_AnyHit), then repeats the queries through the
 // batch extensions.
 //
-//   scene_harness <impl> <mesh.bin> <rays.bin> <out.bin>
+//   scene_harness <impl> <mesh.bin> <rays.bin> <out.bin> [threads]
 //   mesh.bin: u32 nVert, u32 nIdx, float4 vertices[nVert], u32 indices[nIdx]
 //   rays.bin: u32 n, float rays[n][8] (posAndNear, dirAndFar)
 //   out.bin : CRT_Hit per-ray[n], CRT_Hit batch[n], u8 anyPerRay[n],
 //             u8 anyBatch[n], u32 geomNum, u32 instNum
+//   threads  > 0: also runs the per-ray loop from that many OpenMP threads
+//            (eye_ray_host.cpp:327-363 issues RayQuery_NearestHit from an
+//            OpenMP loop), times every call and the whole loop, checks the
+//            hits against the sequential per-ray pass and prints one JSON
+//            line {"per_ray_loop": ...}
+#include <algorithm>
+#include <chrono>
 #include <cstdint>
 #include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "heavyrt/CrossRT.h"
@@ -25,10 +34,11 @@ static bool read_exact(FILE* f, T* dst, size_t n) {
 }
 
 int main(int argc, char** argv) {
-  if (argc != 5) {
-    std::fprintf(stderr, "usage: %s impl mesh.bin rays.bin out.bin\n", argv[0]);
+  if (argc != 5 && argc != 6) {
+    std::fprintf(stderr, "usage: %s impl mesh.bin rays.bin out.bin [threads]\n", argv[0]);
     return 2;
   }
+  const int threads = argc == 6 ? std::atoi(argv[5]) : 0;
   FILE* fm = std::fopen(argv[2], "rb");
   FILE* fr = std::fopen(argv[3], "rb");
   if (!fm || !fr) return 3;
@@ -60,6 +70,30 @@ int main(int argc, char** argv) {
   std::vector<char> tmp(n);
   HipSceneRT_AnyHitBatch(scene, pos.data(), dir.data(), n, reinterpret_cast<bool*>(tmp.data()));
   for (uint32_t i = 0; i < n; i++) any_batch[i] = tmp[i] ? 1 : 0;
+  if (threads > 0) {
+    using clk = std::chrono::steady_clock;
+    std::vector<CRT_Hit> par(n);
+    std::vector<float> us(n);
+    for (int pass = 0; pass < 2; pass++) {  // the first pass warms the staging up
+      const auto t0 = clk::now();
+#pragma omp parallel for num_threads(threads) schedule(dynamic, 64)
+      for (int64_t i = 0; i < int64_t(n); i++) {
+        const auto a = clk::now();
+        par[i] = scene->RayQuery_NearestHit(pos[i], dir[i]);
+        us[i] = std::chrono::duration<float, std::micro>(clk::now() - a).count();
+      }
+      const double loop_ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+      if (pass == 0) continue;
+      uint32_t mism = 0;
+      for (uint32_t i = 0; i < n; i++) mism += std::memcmp(&par[i], &per_ray[i], sizeof(CRT_Hit)) != 0;
+      std::vector<float> srt(us);
+      std::sort(srt.begin(), srt.end());
+      std::printf("{\"per_ray_loop\": {\"rays\": %u, \"threads\": %d, \"loop_ms\": %.3f, \"mrays_s\": %.4f, "
+                  "\"call_us_p50\": %.2f, \"call_us_p99\": %.2f, \"mismatches_vs_sequential\": %u}}\n",
+                  n, threads, loop_ms, n / (loop_ms * 1e3), srt[n / 2], srt[std::min<size_t>(n - 1, size_t(0.99 * n))],
+                  mism);
+    }
+  }
   const MetricStats st = scene->GetStats();
   const uint32_t counts[2] = {scene->GetGeomNum(), scene->GetInstNum()};
   std::printf("%s: %u rays, avg nodes %.2f, avg tris %.2f, avg bytes %.1f\n", scene->Name(), n, st.avgNC, st.avgTC,

@@ -104,13 +104,13 @@ def test_bvh4_nearest_bit_exact(gpu_ctx, name, gen, layout):
 
 
 @pytest.mark.parametrize("layout", [H.LAYOUT_BVH4, H.LAYOUT_BVH4_FP16], ids=["fp32", "fp16"])
-def test_bvh4_nearest_refilling_waves(gpu_ctx, layout):
-    """A batch larger than one resident grid of the one-ray-per-lane kernel goes to the refilling
-    kernel (speculative while-while, wave-local ray pool): full hits equal the oracle's BVH4
-    traversal under the (t, primId) rule on every ray, including rays that are any-hit by their own
-    tFar <= 0, empty and reversed intervals and a NaN bound."""
+def test_bvh4_nearest_sorted_waves(gpu_ctx, layout):
+    """A batch of 2^20 rays or more is sorted into coherent waves before the trace (each thread
+    traces the ray the order gives it): full hits equal the oracle's BVH4 traversal under the
+    (t, primId) rule on every ray, at every ray's own index, including rays that are any-hit by
+    their own tFar <= 0, empty and reversed intervals, a NaN bound and a NaN direction."""
     v, f = W.soup(150_000, stream=W.STREAM_TEST + 5)
-    n = 600_000
+    n = 1_100_000
     rays = W.incoherent_rays(n, stream=W.STREAM_TEST + 6)
     kind = np.arange(n) % 10
     rays[kind == 1, 7] = -rays[kind == 1, 7]      # any-hit by the ray's own convention
@@ -119,7 +119,8 @@ def test_bvh4_nearest_refilling_waves(gpu_ctx, layout):
     rays[kind == 3, 7] = 0.25
     rays[kind == 4, 7] = np.float32(np.nan)
     rays[kind == 5, 7] = 0.05                     # short rays
-    say(f"bvh4 nearest refilling waves, layout {layout}")
+    rays[kind == 6, 4] = np.float32(np.nan)       # a NaN direction sorts into cell 0
+    say(f"bvh4 nearest sorted waves, layout {layout}")
     s = scene_with(gpu_ctx, v, f, layout)
     g = trace(s, rays, "full")
     s.close()
@@ -131,7 +132,7 @@ def test_bvh4_nearest_refilling_waves(gpu_ctx, layout):
     np.testing.assert_array_equal(g["v"].view(np.uint32), vv.view(np.uint32))
     hit = p != INVALID
     assert 0.05 < hit[kind == 0].mean() < 0.999
-    assert not hit[(kind == 2) | (kind == 3) | (kind == 4)].any()
+    assert not hit[(kind == 2) | (kind == 3) | (kind == 4) | (kind == 6)].any()
 
 
 @pytest.mark.parametrize("name,gen", CASES, ids=[c[0] for c in CASES])

@@ -130,3 +130,46 @@ def test_refuses_bad_requests(gpu_ctx):
     inst.close()
     for s, _ in scenes:
         s.close()
+
+
+def test_device_requests_any_hit_tree_and_frozen_scenes(gpu_ctx):
+    """Requests with device buffers (no staging) mix with pinned-host ones; any-hit requests on an
+    AUTO_FP16 scene walk its any-hit tree and equal hrt_trace_any; a served scene refuses changes
+    until the server is gone."""
+    import torch
+
+    v, f = W.soup(30_000, stream=7100)
+    s = H.Scene(gpu_ctx)
+    s.add_instance(s.add_triangles(v, f))
+    s.commit(layout=H.LAYOUT_AUTO_FP16)
+    srv = H.Server(gpu_ctx, 8 << 20, steps_in_flight=2)
+    sid = srv.add_scene(s)
+    with pytest.raises(H.HrtError) as e:
+        s.commit(layout=H.LAYOUT_BVH2)
+    assert e.value.status == 1
+    with pytest.raises(H.HrtError):
+        s.add_triangles(v, f)
+    reqs = []
+    for i in range(24):
+        n = 1000 + 997 * i
+        anyh = i % 2 == 1
+        rays = request_rays(v, n, 7200 + i)
+        if i % 3 == 0:  # device buffers
+            d_rays = torch.from_numpy(np.asarray(rays)).cuda()
+            d_out = torch.zeros(n, dtype=torch.uint8, device="cuda") if anyh else \
+                torch.zeros((n, 2), dtype=torch.int32, device="cuda")
+            torch.cuda.synchronize()
+            reqs.append((anyh, rays, d_out, srv.submit(sid, d_rays, d_out, any_hit=anyh), d_rays))
+        else:
+            out = H.pinned_array((n,), np.uint8) if anyh else H.pinned_array((n, 2), np.uint32)
+            reqs.append((anyh, rays, out, srv.submit(sid, rays, out, any_hit=anyh), None))
+    for anyh, rays, out, rid, _ in reqs:
+        srv.wait(rid)
+        got = out.cpu().numpy().view(np.uint32) if hasattr(out, "cpu") and not anyh else \
+            (out.cpu().numpy() if hasattr(out, "cpu") else np.asarray(out))
+        np.testing.assert_array_equal(got, solo_result(s, rays, anyh))
+    st = srv.stats()
+    assert st["completed"] == len(reqs)
+    srv.close()
+    s.commit(layout=H.LAYOUT_BVH2)  # free again once the server is gone
+    s.close()

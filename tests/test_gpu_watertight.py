@@ -83,10 +83,10 @@ def test_no_ray_escapes_a_closed_mesh(gpu_ctx, layout):
 
 
 @pytest.mark.parametrize("layout", [H.LAYOUT_BVH4, H.LAYOUT_BVH4_FP16], ids=["fp32", "fp16"])
-def test_watertight_refilling_waves(gpu_ctx, layout):
-    """A batch larger than one resident grid (the refilling kernel) under the watertight test."""
+def test_watertight_sorted_waves(gpu_ctx, layout):
+    """A batch large enough to be sorted into coherent waves, under the watertight test."""
     v, f = W.soup(150_000, stream=W.STREAM_TEST + 8)
-    rays = W.incoherent_rays(600_000, stream=W.STREAM_TEST + 9)
+    rays = W.incoherent_rays(1_100_000, stream=W.STREAM_TEST + 9)
     s = wt_scene(gpu_ctx, v, f, layout)
     g = trace(s, rays, "full")
     s.close()

@@ -349,3 +349,18 @@ def test_leaf_count_wraps_past_127_like_the_reference():
     b = O.OracleBVH(v, f)
     assert b.max_run == 130
     assert O.pack_leaf(0, 130) == (LEAF | (2 << 24))
+
+
+def test_as_shipped_metrics_mode_matches_the_counters():
+    """The reference's metrics bookkeeping (the CPU baseline's "as shipped" run) changes no hit, and its
+    NC / TC equal twice the pair visits / the triangle tests of the counting traversal."""
+    v, f = W.eye_scene()
+    rays = W.eye_rays()[::7]
+    ob = O.OracleBVH(v, f)
+    t, p, m = ob.trace_metrics(rays)
+    t2, p2, _, _, pairs, tris = ob.trace(rays, counters=True)
+    np.testing.assert_array_equal(t.view(np.uint32), t2.view(np.uint32))
+    np.testing.assert_array_equal(p, p2)
+    assert m["NC"] == 2 * int(pairs.sum()) and m["TC"] == int(tris.sum())
+    assert m["BLB"] == 32 * m["NC"] + 60 * m["TC"]
+    assert 0 < m["WSS128"] <= m["NC"]
