@@ -69,7 +69,7 @@ EXPORTS = [
     "hrt_trace_nearest_host", "hrt_trace_any_host", "hrt_update_triangles", "hrt_update_instance",
     "hrt_batch_create", "hrt_batch_destroy", "hrt_batch_buffers", "hrt_batch_run", "hrt_batch_build",
     "hrt_batch_trace", "hrt_batch_candidate_count", "hrt_batch_candidate_info", "hrt_batch_set_candidate",
-    "hrt_batch_get_candidate", "hrt_batch_autotune", "hrt_batch_dump",
+    "hrt_batch_get_candidate", "hrt_batch_autotune", "hrt_batch_dump", "hrt_batch_status",
     "hrt_host_alloc", "hrt_host_free", "hrt_server_create", "hrt_server_destroy", "hrt_server_add_scene",
     "hrt_server_submit", "hrt_server_poll", "hrt_server_wait", "hrt_server_stats",
 ]
@@ -159,6 +159,7 @@ def lib():
         L.hrt_batch_get_candidate.argtypes = [vp]
         L.hrt_batch_autotune.argtypes = [vp, i32, C.POINTER(i32), vp]
         L.hrt_batch_dump.argtypes = [vp, u32, i32, vp, sz]
+        L.hrt_batch_status.argtypes = [vp, C.POINTER(u32)]
         L.hrt_host_alloc.argtypes = [sz, C.POINTER(vp)]
         L.hrt_host_free.argtypes = [vp]
         L.hrt_host_free.restype = None
@@ -533,6 +534,15 @@ class Batch:
         ch = C.c_int()
         _check(lib().hrt_batch_autotune(self._h, reps, C.byref(ch), _ptr(ms)))
         return ch.value, ms
+
+    def status(self) -> None:
+        """Raises HrtError (with .scene set) when the last build could not store some scene's tree."""
+        bad = C.c_uint32(0xFFFFFFFF)
+        st = lib().hrt_batch_status(self._h, C.byref(bad))
+        if st != 0:
+            err = HrtError(st, lib().hrt_last_error().decode(errors="replace"))
+            err.scene = None if bad.value == 0xFFFFFFFF else int(bad.value)
+            raise err
 
     def dump(self, scene: int, what: int) -> np.ndarray:
         sizes = np.zeros(5, dtype=np.uint32)

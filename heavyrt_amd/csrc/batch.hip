@@ -963,6 +963,29 @@ hrt_status hrt_batch_autotune(hrt_batch* b, int reps, int* chosen, float* msPerC
   return HRT_OK;
 }
 
+hrt_status hrt_batch_status(hrt_batch* b, uint32_t* badScene) {
+  if (!b) return fail(HRT_ERR_INVALID_ARG, "batch is null");
+  if (badScene) *badScene = 0xFFFFFFFFu;
+  if (!b->built) return fail(HRT_ERR_NOT_COMMITTED, "the batch has not been built");
+  hipError_t e = hipSetDevice(b->ctx->device);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  std::vector<uint32_t> mr(b->S);
+  uint32_t rerr = 0;
+  if (e == hipSuccess) e = hipMemcpy(mr.data(), b->max_run, 4ull * b->S, hipMemcpyDeviceToHost);
+  if (e == hipSuccess) e = hipMemcpy(&rerr, b->refit_err, 4, hipMemcpyDeviceToHost);
+  if (e != hipSuccess) return fail_hip(e, "batch status");
+  for (uint32_t s = 0; s < b->S; s++) {
+    if (mr[s] > 127) {
+      if (badScene) *badScene = s;
+      return fail(HRT_ERR_LEAF_OVERFLOW, "scene " + std::to_string(s) + ": " + std::to_string(mr[s]) +
+                                             " triangles share one Morton code; the 7-bit leaf count "
+                                             "(raytrace_common.h:40) cannot hold them, so its hits are not valid");
+    }
+  }
+  if (rerr) return fail(HRT_ERR_UNSUPPORTED, "a device-side check of the build failed (sort look-back or refit list)");
+  return HRT_OK;
+}
+
 // Per-scene results after a build: sizes (5 words: triangles, leaves, longest
 // run, pairs reserved, first pair), node pairs (16 words each), permutation.
 hrt_status hrt_batch_dump(hrt_batch* b, uint32_t scene, int what, void* host_out, size_t bytes) {

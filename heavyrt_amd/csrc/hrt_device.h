@@ -454,20 +454,27 @@ __device__ inline float2 slab_fused(float3 oi, float3 inv, uint4 w, uint3 sel) {
 // FP32 -> FP16 bits, round to nearest even (v_cvt_f16_f32 in the default mode).
 __device__ inline uint32_t f2h_rne(float f) { return __builtin_bit_cast(uint16_t, _Float16(f)); }
 
-// half.hpp nextafter (half.hpp:4291-4308) for non-NaN halves.
-__device__ inline uint32_t h_nextafter(uint32_t from, uint32_t to) {
-  const uint32_t fa = from & 0x7FFFu, ta = to & 0x7FFFu;
-  if (fa > 0x7C00u || ta > 0x7C00u) return from;
-  if (from == to || !(fa | ta)) return to;
-  if (!fa) return (to & 0x8000u) + 1u;
-  const uint32_t of = from ^ (0x8000u | (0x8000u - (from >> 15)));
-  const uint32_t ot = to ^ (0x8000u | (0x8000u - (to >> 15)));
-  const uint32_t up = ((from >> 15) ^ uint32_t(of < ot)) << 1;
-  return (from + up - 1u) & 0xFFFFu;
+// One binary16 step toward -65504 (0xFBFF) and toward +65504 (0x7BFF), the
+// two directions cbvh_fat.cpp:447-448 take after rounding: away from the
+// target's side a step shrinks the magnitude (the bit pattern minus one),
+// toward it the magnitude grows (plus one); either zero steps to the
+// smallest subnormal of the target's sign; the target itself stays, and the
+// infinity beyond it steps back onto it.  NaN stays.
+__device__ inline uint32_t h_step_down(uint32_t h) {
+  if ((h & 0x7FFFu) > 0x7C00u || h == 0xFBFFu) return h;
+  if (h == 0xFC00u) return 0xFBFFu;
+  if ((h & 0x7FFFu) == 0u) return 0x8001u;
+  return (h & 0x8000u) ? h + 1u : h - 1u;
+}
+__device__ inline uint32_t h_step_up(uint32_t h) {
+  if ((h & 0x7FFFu) > 0x7C00u || h == 0x7BFFu) return h;
+  if (h == 0x7C00u) return 0x7BFFu;
+  if ((h & 0x7FFFu) == 0u) return 0x0001u;
+  return (h & 0x8000u) ? h - 1u : h + 1u;
 }
 // cbvh_fat.cpp:447-448: round to nearest, then one step toward -/+ max half.
-__device__ inline uint32_t round_to_neg(float v) { return h_nextafter(f2h_rne(v), 0xFBFFu); }
-__device__ inline uint32_t round_to_pos(float v) { return h_nextafter(f2h_rne(v), 0x7BFFu); }
+__device__ inline uint32_t round_to_neg(float v) { return h_step_down(f2h_rne(v)); }
+__device__ inline uint32_t round_to_pos(float v) { return h_step_up(f2h_rne(v)); }
 
 // ---------------------------------------------------------------------------
 // Builder helpers shared by the single-scene and batched LBVH builders

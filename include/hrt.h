@@ -339,6 +339,12 @@ int hrt_batch_get_candidate(const hrt_batch* batch);
  * rays (best of reps), selects the fastest; msPerCandidate (nullable) gets
  * hrt_batch_candidate_count() timings.  Synchronous. */
 hrt_status hrt_batch_autotune(hrt_batch* batch, int reps, int* chosen, float* msPerCandidate);
+/* Synchronous check of the last build: HRT_OK when every scene's tree was
+ * stored; HRT_ERR_LEAF_OVERFLOW (with *badScene, nullable, the first such
+ * scene) when more than 127 triangles of a scene share one Morton code --
+ * that scene's hits are not valid, as hrt_add_triangles would have refused
+ * it; HRT_ERR_UNSUPPORTED when a device-side check of the build failed. */
+hrt_status hrt_batch_status(hrt_batch* batch, uint32_t* badScene);
 /* Per-scene parity hooks after a build: HRT_DUMP_SIZES (5 words: triangles,
  * leaves, longest run, pairs, first pair), HRT_DUMP_PAIRS, HRT_DUMP_PERM. */
 hrt_status hrt_batch_dump(hrt_batch* batch, uint32_t scene, int what, void* host_out, size_t bytes);

@@ -1060,15 +1060,21 @@ static uint16_t f2h_rne(float f) {
   return uint16_t(sign | q);
 }
 
-// half.hpp nextafter (half.hpp:4291-4308) restricted to non-NaN inputs.
+// nextafter(from, to) on binary16 (the step half.hpp's nextafter takes), for
+// the two targets the packing uses (+-65504): the half's value as a signed
+// integer key (+0 and -0 both 0, -x = -key(x); keys are in value order),
+// one step toward the target's key, back to a half.  A step onto zero keeps
+// the sign of `from`; a step off zero lands on the smallest subnormal of the
+// target's sign; the target itself and NaN stay.
 static uint16_t h_nextafter(uint16_t from, uint16_t to) {
-  const uint32_t fa = from & 0x7FFFu, ta = to & 0x7FFFu;
-  if (fa > 0x7C00u || ta > 0x7C00u) return from;
-  if (from == to || !(fa | ta)) return to;
-  if (!fa) return uint16_t((to & 0x8000u) + 1u);
-  auto ord = [](uint32_t v) { return v ^ (0x8000u | (0x8000u - (v >> 15))); };
-  const uint32_t up = ((from >> 15) ^ uint32_t(ord(from) < ord(to))) << 1;
-  return uint16_t(from + up - 1u);
+  if ((from & 0x7FFFu) > 0x7C00u) return from;
+  if (from == to) return to;
+  auto key = [](uint16_t h) { return (h & 0x8000u) ? -int32_t(h & 0x7FFFu) : int32_t(h & 0x7FFFu); };
+  const int32_t kf = key(from), kt = key(to);
+  if (kf == kt) return to;  // the other zero
+  const int32_t k = kf + (kt > kf ? 1 : -1);
+  if (k == 0) return uint16_t(from & 0x8000u);
+  return k > 0 ? uint16_t(k) : uint16_t(0x8000u | uint32_t(-k));
 }
 static uint16_t round_to_neg(float v) { return h_nextafter(f2h_rne(v), 0xFBFFu); }  // cbvh_fat.cpp:447
 static uint16_t round_to_pos(float v) { return h_nextafter(f2h_rne(v), 0x7BFFu); }  // cbvh_fat.cpp:448

@@ -136,3 +136,31 @@ def test_batch_rerun_with_new_inputs(gpu_ctx):
         np.testing.assert_array_equal(hb[:, 1], p)
     assert sizes == [(x[0].shape[0], x[1].shape[0], x[2].shape[0]) for x in moved]
     b.close()
+
+
+def test_batch_reports_a_leaf_overflow(gpu_ctx):
+    """A scene with 130 triangles on one Morton code cannot be stored in 7-bit leaf counts: the batch's
+    status names it (hrt_add_triangles refuses the same scene); the other scenes' hits stay right."""
+    import torch
+
+    tri = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float32)
+    bad = (tri, np.array([[0, 1, 2]] * 130, np.uint32), W.incoherent_rays(500, stream=6100))
+    good = W.batched_workload(2, tris=(500, 2000), rays=(1000, 2000), stream=6200)
+    scenes = [good[0], bad, good[1]]
+    b = make_batch(gpu_ctx, scenes)
+    b.run(torch.cuda.current_stream().cuda_stream)
+    with pytest.raises(H.HrtError) as e:
+        b.status()
+    assert e.value.status == 6 and e.value.scene == 1
+    hits = b.hits()
+    for s in (0, 2):
+        v, f, rays = scenes[s]
+        t, p, _, _ = O.OracleBVH(v, f).trace(rays)
+        hb = hits[b.ray_off[s]:b.ray_off[s + 1]]
+        np.testing.assert_array_equal(hb[:, 0], t.view(np.uint32))
+        np.testing.assert_array_equal(hb[:, 1], p)
+    b.close()
+    ok = make_batch(gpu_ctx, good)
+    ok.run(torch.cuda.current_stream().cuda_stream)
+    ok.status()
+    ok.close()

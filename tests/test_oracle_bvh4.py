@@ -54,6 +54,22 @@ def test_outward_rounding_known_answers():
         assert O.round_to_neg(x) == 0xFBFF
 
 
+def test_outward_step_equals_numpy_nextafter_on_every_half():
+    """For every non-NaN binary16 value (both zeros and infinities included), the outward step equals
+    numpy's IEEE nextafter toward -65504 and toward +65504 (a different implementation)."""
+    bits = np.arange(0x10000, dtype=np.uint32).astype(np.uint16)
+    h = bits.view(np.float16)
+    keep = ~np.isnan(h)
+    want_neg = np.nextafter(h[keep], np.float16(-65504.0)).view(np.uint16)
+    want_pos = np.nextafter(h[keep], np.float16(65504.0)).view(np.uint16)
+    vals = h[keep].astype(np.float32)
+    got_neg = np.array([O.round_to_neg(float(x)) for x in vals], dtype=np.uint16)
+    got_pos = np.array([O.round_to_pos(float(x)) for x in vals], dtype=np.uint16)
+    # a float that is exactly a half rounds to that half first; -0.0 and +0.0 stay distinct
+    np.testing.assert_array_equal(got_neg, want_neg)
+    np.testing.assert_array_equal(got_pos, want_pos)
+
+
 def test_outward_rounding_is_strictly_conservative_in_range():
     rng = np.random.default_rng(3)
     xs = np.concatenate([rng.uniform(-1, 1, 20000), rng.uniform(-60000, 60000, 5000),
