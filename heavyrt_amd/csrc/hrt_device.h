@@ -177,6 +177,39 @@ __device__ inline float2 slab(float3 o, float3 inv, float bminx, float bminy, fl
   return make_float2(fmaxf(tmin, fminf(lo2, hi2)), fminf(tmax, fmaxf(lo2, hi2)));
 }
 
+// The box test of the watertight mode, made conservative as Ize shows
+// ("Robust BVH Ray Traversal", JCGT 2(2), 2013): a plane distance
+// inv * (b - o) carries three roundings (the reciprocal, the difference, the
+// product), a relative error within gamma3 = 3u / (1 - 3u), u = 2^-24, so the
+// exit distance is scaled by 1 + 2 gamma3, taken as the next float above it
+// (1 + 4 * 2^-23), and a box the exact ray meets is never rejected.  An axis
+// whose direction component lies within +-1e-36 (where safe_inverse puts
+// +-1e36 in place of 1/d) is parallel: its slab is every t when
+// lo <= o <= hi and no t otherwise, so a ray running inside a box face keeps
+// the box.
+constexpr float kRobustExit = 1.000000476837158203125f;
+__device__ inline uint32_t parallel_axes(float3 d) {
+  return (fabsf(d.x) > 1.0e-36f ? 0u : 1u) | (fabsf(d.y) > 1.0e-36f ? 0u : 2u) | (fabsf(d.z) > 1.0e-36f ? 0u : 4u);
+}
+__device__ inline void robust_axis(float o, float inv, bool par, float lo, float hi, float& a, float& b) {
+  const float t0 = inv * (lo - o), t1 = inv * (hi - o);
+  a = fminf(t0, t1);
+  b = fmaxf(t0, t1);
+  if (par) {
+    const bool in = lo <= o && o <= hi;
+    a = in ? -__builtin_inff() : __builtin_inff();
+    b = in ? __builtin_inff() : -__builtin_inff();
+  }
+}
+__device__ inline float2 slab_robust(float3 o, float3 inv, uint32_t par, float bminx, float bminy, float bminz,
+                                     float bmaxx, float bmaxy, float bmaxz) {
+  float ax, bx, ay, by, az, bz;
+  robust_axis(o.x, inv.x, (par & 1u) != 0u, bminx, bmaxx, ax, bx);
+  robust_axis(o.y, inv.y, (par & 2u) != 0u, bminy, bmaxy, ay, by);
+  robust_axis(o.z, inv.z, (par & 4u) != 0u, bminz, bmaxz, az, bz);
+  return make_float2(fmaxf(fmaxf(ax, ay), az), fminf(fminf(bx, by), bz) * kRobustExit);
+}
+
 // -(o * inv) per axis, made once per ray for slab_fused.
 __device__ inline float3 neg_o_inv(float3 o, float3 inv) {
   return make_float3(-(o.x * inv.x), -(o.y * inv.y), -(o.z * inv.z));
@@ -277,18 +310,28 @@ __device__ inline bool mt_test_lex(float3 o, float3 d, float tnear, const TriPre
 //           (the weight of C) and coords[1] = V / det (the weight of B), the
 //           same meaning as the reference's u and v.
 // ---------------------------------------------------------------------------
+// A triangle-test policy also carries its box test: the reference's slab for
+// its Moller-Trumbore, the conservative slab_robust for the watertight test
+// (a watertight triangle test behind a box test that rounds a touched box
+// away would still let rays through).
 struct MtTest {
+  static constexpr bool kRobustBoxes = false;
   __device__ MtTest() {}
   __device__ MtTest(float3, float3) {}
   __device__ bool eval(float3 o, float3 d, float tnear, const TriPre& tri, float& t, float& u, float& v) const {
     return mt_eval(o, d, tnear, tri, t, u, v);
+  }
+  __device__ float2 box(float3 o, float3 inv, float bminx, float bminy, float bminz, float bmaxx, float bmaxy,
+                        float bmaxz) const {
+    return slab(o, inv, bminx, bminy, bminz, bmaxx, bmaxy, bmaxz);
   }
 };
 
 __device__ inline float axis_of(float3 v, uint32_t k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
 
 struct WtTest {
-  uint32_t axes;  // kx | ky << 2 | kz << 4
+  static constexpr bool kRobustBoxes = true;
+  uint32_t axes;  // kx | ky << 2 | kz << 4 | parallel_axes(d) << 6
   float sx, sy, sz;
   __device__ WtTest() : axes(0), sx(0.f), sy(0.f), sz(0.f) {}
   __device__ WtTest(float3, float3 d) {
@@ -302,13 +345,13 @@ struct WtTest {
       kx = ky;
       ky = k;
     }
-    axes = kx | (ky << 2) | (kz << 4);
+    axes = kx | (ky << 2) | (kz << 4) | (parallel_axes(d) << 6);
     sx = axis_of(d, kx) / dz;
     sy = axis_of(d, ky) / dz;
     sz = 1.0f / dz;
   }
   __device__ bool eval(float3 o, float3, float tnear, const TriPre& tri, float& t, float& u, float& v) const {
-    const uint32_t kx = axes & 3u, ky = (axes >> 2) & 3u, kz = axes >> 4;
+    const uint32_t kx = axes & 3u, ky = (axes >> 2) & 3u, kz = (axes >> 4) & 3u;
     const float3 A = make_float3(tri.a.x - o.x, tri.a.y - o.y, tri.a.z - o.z);
     const float3 B = make_float3(tri.e1.x - o.x, tri.e1.y - o.y, tri.e1.z - o.z);
     const float3 C = make_float3(tri.e2.x - o.x, tri.e2.y - o.y, tri.e2.z - o.z);
@@ -328,6 +371,10 @@ struct WtTest {
     u = W * rcp;
     v = V * rcp;
     return t > tnear;
+  }
+  __device__ float2 box(float3 o, float3 inv, float bminx, float bminy, float bminz, float bmaxx, float bmaxy,
+                        float bmaxz) const {
+    return slab_robust(o, inv, axes >> 6, bminx, bminy, bminz, bmaxx, bmaxy, bmaxz);
   }
 };
 

@@ -95,8 +95,8 @@ __global__ __launch_bounds__(256) void k_trace_simple(const NodePair* __restrict
       const float4* pp = reinterpret_cast<const float4*>(pairs + cur);
       const float4 lmin = pp[0], lmax = pp[1], rmin = pp[2], rmax = pp[3];
       if (kStats) npairs++;
-      const float2 t0 = slab(o, inv, lmin.x, lmin.y, lmin.z, lmax.x, lmax.y, lmax.z);
-      const float2 t1 = slab(o, inv, rmin.x, rmin.y, rmin.z, rmax.x, rmax.y, rmax.z);
+      const float2 t0 = tt.box(o, inv, lmin.x, lmin.y, lmin.z, lmax.x, lmax.y, lmax.z);
+      const float2 t1 = tt.box(o, inv, rmin.x, rmin.y, rmin.z, rmax.x, rmax.y, rmax.z);
       const bool h0 = (t0.x <= t0.y) && (t0.y >= tnear) && (t0.x <= h.t);
       const bool h1 = (t1.x <= t1.y) && (t1.y >= tnear) && (t1.x <= h.t);
       const uint32_t c0 = __float_as_uint(lmin.w), c1 = __float_as_uint(rmin.w);
@@ -217,8 +217,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_persistent(const NodePair
           if (active && (cur & kLeafBit) == 0) {
             const float4* pp = reinterpret_cast<const float4*>(pairs + cur);
             const float4 lmin = pp[0], lmax = pp[1], rmin = pp[2], rmax = pp[3];
-            const float2 t0 = slab(o, inv, lmin.x, lmin.y, lmin.z, lmax.x, lmax.y, lmax.z);
-            const float2 t1 = slab(o, inv, rmin.x, rmin.y, rmin.z, rmax.x, rmax.y, rmax.z);
+            const float2 t0 = tt.box(o, inv, lmin.x, lmin.y, lmin.z, lmax.x, lmax.y, lmax.z);
+            const float2 t1 = tt.box(o, inv, rmin.x, rmin.y, rmin.z, rmax.x, rmax.y, rmax.z);
             const bool h0 = (t0.x <= t0.y) && (t0.y >= tnear) && (t0.x <= h.t);
             const bool h1 = (t1.x <= t1.y) && (t1.y >= tnear) && (t1.x <= h.t);
             const uint32_t c0 = __float_as_uint(lmin.w), c1 = __float_as_uint(rmin.w);
@@ -248,8 +248,8 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_persistent(const NodePair
           if ((cur & kLeafBit) == 0) {
             const float4* pp = reinterpret_cast<const float4*>(pairs + cur);
             const float4 lmin = pp[0], lmax = pp[1], rmin = pp[2], rmax = pp[3];
-            const float2 t0 = slab(o, inv, lmin.x, lmin.y, lmin.z, lmax.x, lmax.y, lmax.z);
-            const float2 t1 = slab(o, inv, rmin.x, rmin.y, rmin.z, rmax.x, rmax.y, rmax.z);
+            const float2 t0 = tt.box(o, inv, lmin.x, lmin.y, lmin.z, lmax.x, lmax.y, lmax.z);
+            const float2 t1 = tt.box(o, inv, rmin.x, rmin.y, rmin.z, rmax.x, rmax.y, rmax.z);
             const bool h0 = (t0.x <= t0.y) && (t0.y >= tnear) && (t0.x <= h.t);
             const bool h1 = (t1.x <= t1.y) && (t1.y >= tnear) && (t1.x <= h.t);
             const uint32_t c0 = __float_as_uint(lmin.w), c1 = __float_as_uint(rmin.w);

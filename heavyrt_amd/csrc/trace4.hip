@@ -52,15 +52,16 @@ struct HitOutFull {
 
 __global__ void k_reset_counter4(uint32_t* c) { *c = 0u; }
 
-// Loads one BVH4 node, tests its four children, and returns the entry
-// distances (kMaxFloat for a miss) and references in reference order.
-// kFused (FP16 nodes of any-hit traversals only): slab_fused with
+// Loads one BVH4 node, tests its four children with the box test of the
+// triangle-test policy tt, and returns the entry distances (kMaxFloat for a
+// miss) and references in reference order.  kFused (FP16 nodes of any-hit
+// traversals under the reference's test only): slab_fused with
 // oi = -(o * inv) for a lane whose fz = fused_ok(oi), slab otherwise.
-template <bool kFp16, bool kFused = false>
+template <bool kFp16, bool kFused, class Tri>
 __device__ inline void visit_quad(const void* __restrict__ quads, uint32_t q, float3 o, float3 inv, float3 oi,
-                                  bool fz, float tnear, float tcull, float tm[4], uint32_t ch[4]) {
+                                  bool fz, const Tri& tt, float tnear, float tcull, float tm[4], uint32_t ch[4]) {
   static_assert(kFp16 || !kFused, "the fused box test is for FP16 nodes");
-  float2 tt[4];
+  float2 bt[4];
   if (kFp16) {
     // per child {minx | miny << 16, minz | maxx << 16, maxy | maxz << 16, ref};
     // the FP16 values are widened inside each branch, so the fused one feeds
@@ -72,15 +73,15 @@ __device__ inline void visit_quad(const void* __restrict__ quads, uint32_t q, fl
       w[k] = p[k];
       ch[k] = w[k].w;
     }
-    if (kFused && fz) {
+    if (kFused && !Tri::kRobustBoxes && fz) {
       const uint3 sel = near_far_selectors(inv);
 #pragma unroll
-      for (int k = 0; k < 4; k++) tt[k] = slab_fused(oi, inv, w[k], sel);
+      for (int k = 0; k < 4; k++) bt[k] = slab_fused(oi, inv, w[k], sel);
     } else {
 #pragma unroll
       for (int k = 0; k < 4; k++)
-        tt[k] = slab(o, inv, h2f_lo(w[k].x), h2f_hi(w[k].x), h2f_lo(w[k].y), h2f_hi(w[k].y), h2f_lo(w[k].z),
-                     h2f_hi(w[k].z));
+        bt[k] = tt.box(o, inv, h2f_lo(w[k].x), h2f_hi(w[k].x), h2f_lo(w[k].y), h2f_hi(w[k].y), h2f_lo(w[k].z),
+                       h2f_hi(w[k].z));
     }
   } else {
     const float4* p = reinterpret_cast<const float4*>(static_cast<const Quad32*>(quads) + q);
@@ -91,11 +92,11 @@ __device__ inline void visit_quad(const void* __restrict__ quads, uint32_t q, fl
     const float loz[4] = {e.x, e.y, e.z, e.w}, hiz[4] = {f.x, f.y, f.z, f.w};
     ch[0] = r.x; ch[1] = r.y; ch[2] = r.z; ch[3] = r.w;
 #pragma unroll
-    for (int k = 0; k < 4; k++) tt[k] = slab(o, inv, lox[k], loy[k], loz[k], hix[k], hiy[k], hiz[k]);
+    for (int k = 0; k < 4; k++) bt[k] = tt.box(o, inv, lox[k], loy[k], loz[k], hix[k], hiy[k], hiz[k]);
   }
 #pragma unroll
   for (int k = 0; k < 4; k++) {
-    const float2 t = tt[k];
+    const float2 t = bt[k];
     // An unused slot (box +1e9..-1e9, empty leaf 0x80000000) would pass the
     // min/max slab test as the box [-1e9, 1e9]; it holds no triangle, so it
     // is never entered.
@@ -172,7 +173,7 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_simple(const void* __restric
     if ((cur & kLeafBit) == 0) {
       float tm[4];
       uint32_t ch[4];
-      visit_quad<kFp16, kForceAny && kFp16>(quads, cur, o, inv, oi, fz, tnear, h.t, tm, ch);
+      visit_quad<kFp16, kForceAny && kFp16>(quads, cur, o, inv, oi, fz, tt, tnear, h.t, tm, ch);
       if (kStats) nvisits++;
       if (tm[3] < kMaxFloat) stk.push(top, ch[3]);
       if (tm[2] < kMaxFloat) stk.push(top, ch[2]);
@@ -332,7 +333,8 @@ __global__ __launch_bounds__(kBlock4, 8) void k_trace4_any_persistent(const void
         if (active && (cur & kLeafBit) == 0) {
           float tm[4];
           uint32_t ch[4];
-          visit_quad<kFp16, kFp16>(Src::kSegments ? lane_quads : quads, cur, o, inv, oi, fz, tnear, h.t, tm, ch);
+          visit_quad<kFp16, kFp16>(Src::kSegments ? lane_quads : quads, cur, o, inv, oi, fz, tt, tnear, h.t, tm,
+                                   ch);
           if (tm[3] < kMaxFloat) stk.push(top, ch[3]);
           if (tm[2] < kMaxFloat) stk.push(top, ch[2]);
           if (tm[1] < kMaxFloat) stk.push(top, ch[1]);

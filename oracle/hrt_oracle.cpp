@@ -544,7 +544,14 @@ static inline f3 safe_inverse(f3 d) {  // raytrace_common.h:57-65
           1.0f / (std::fabs(d.z) > eps ? d.z : std::copysign(eps, d.z))};
 }
 
-static inline void slab(f3 o, f3 inv, f3 bmin, f3 bmax, float& tmin_out, float& tmax_out) {  // raytrace_common.h:67-80
+// The triangle test every traversal below uses: 0 = the reference's
+// Moller-Trumbore, 1 = the watertight test (which brings the conservative
+// box test slab_robust with it).  Set by the Python wrapper for the duration
+// of one call (hrto_set_triangle_test); not for concurrent calls that want
+// different tests.
+static int g_tri_test = 0;
+
+static inline void slab_ref(f3 o, f3 inv, f3 bmin, f3 bmax, float& tmin_out, float& tmax_out) {  // raytrace_common.h:67-80
   const float lo = inv.x * (bmin.x - o.x);
   const float hi = inv.x * (bmax.x - o.x);
   const float lo1 = inv.y * (bmin.y - o.y);
@@ -575,6 +582,39 @@ static inline void slab_fused(f3 o, f3 inv, f3 bmin, f3 bmax, float& tmin_out, f
   const float bz = std::fma(inv.z >= 0.0f ? bmax.z : bmin.z, inv.z, nz);
   tmin_out = mx(mx(ax, ay), az);
   tmax_out = mn(mn(bx, by), bz);
+}
+
+// The box test of the watertight mode, conservative as in Ize, "Robust BVH
+// Ray Traversal" (JCGT 2(2), 2013): each plane distance inv * (b - o) has three
+// roundings (reciprocal, difference, product), so its relative error is within
+// gamma3 = 3u / (1 - 3u), u = 2^-24; the exit distance is multiplied by
+// 1 + 2 gamma3, taken as the next float above (1 + 4 * 2^-23).  An axis whose
+// direction component is within +-1e-36 (where safe_inverse stands in +-1e36
+// for 1/d) counts as parallel: all t when lo <= o <= hi, none otherwise.
+static const float kRobustExit = 1.000000476837158203125f;
+static inline void robust_axis(float o, float inv, float d, float lo, float hi, float& a, float& b) {
+  const float t0 = inv * (lo - o), t1 = inv * (hi - o);
+  a = mn(t0, t1);
+  b = mx(t0, t1);
+  if (!(std::fabs(d) > 1.0e-36f)) {
+    const bool in = lo <= o && o <= hi;
+    a = in ? -INFINITY : INFINITY;
+    b = in ? INFINITY : -INFINITY;
+  }
+}
+static inline void slab_robust(f3 o, f3 d, f3 inv, f3 bmin, f3 bmax, float& tmin_out, float& tmax_out) {
+  float ax, bx, ay, by, az, bz;
+  robust_axis(o.x, inv.x, d.x, bmin.x, bmax.x, ax, bx);
+  robust_axis(o.y, inv.y, d.y, bmin.y, bmax.y, ay, by);
+  robust_axis(o.z, inv.z, d.z, bmin.z, bmax.z, az, bz);
+  tmin_out = mx(mx(ax, ay), az);
+  tmax_out = mn(mn(bx, by), bz) * kRobustExit;
+}
+
+// The box test of the current triangle test.
+static inline void slab(f3 o, f3 d, f3 inv, f3 bmin, f3 bmax, float& tmin_out, float& tmax_out) {
+  if (g_tri_test == 1) slab_robust(o, d, inv, bmin, bmax, tmin_out, tmax_out);
+  else slab_ref(o, inv, bmin, bmax, tmin_out, tmax_out);
 }
 
 static inline bool fused_ok(f3 o, f3 inv) {
@@ -616,12 +656,6 @@ static inline bool wt_eval(f3 o, f3 d, float tNear, f3 A0, f3 B0, f3 C0, float& 
   v = V * rcp;
   return t > tNear;
 }
-
-// The triangle test every traversal below uses: 0 = the reference's
-// Moller-Trumbore, 1 = the watertight test.  Set by the Python wrapper for the
-// duration of one call (hrto_set_triangle_test); not for concurrent calls
-// that want different tests.
-static int g_tri_test = 0;
 
 // Moller-Trumbore with the reference's 1e-6 slack, BVH2CommonRT.cpp:26-45:
 // t, u, v and whether the barycentric and t > tNear conditions hold.
@@ -714,8 +748,8 @@ static bool visit_bvh2(const Build& b, f3 o, f3 d, float tNear, bool anyhit, Hit
       const Node n1 = nodes[off + 1];
       if (cnt) cnt->pairs++;
       float a0, b0, a1, b1;
-      slab(o, inv, n0.bmin, n0.bmax, a0, b0);
-      slab(o, inv, n1.bmin, n1.bmax, a1, b1);
+      slab(o, d, inv, n0.bmin, n0.bmax, a0, b0);
+      slab(o, d, inv, n1.bmin, n1.bmax, a1, b1);
       const bool h0 = (a0 <= b0) && (b0 >= tNear) && (a0 <= h.t);
       const bool h1 = (a1 <= b1) && (b1 >= tNear) && (a1 <= h.t);
       pop = (!h0 && !h1);
@@ -795,7 +829,7 @@ static Hit trace_lr(const Build& b, f4 posNear, f4 dirFar) {
   if (b.L == 1) {  // root is a leaf: its box is tested first, as in the peach layout
     const uint32_t leaf = nodes[0].left;
     float a0, b0;
-    slab(o, inv, nodes[0].bmin, nodes[0].bmax, a0, b0);
+    slab(o, d, inv, nodes[0].bmin, nodes[0].bmax, a0, b0);
     if (!((a0 <= b0) && (b0 >= tNear) && (a0 <= h.t))) return h;
     for (uint32_t t = leaf_start(leaf); t < leaf_start(leaf) + leaf_count(leaf); t++) {
       const uint32_t* ix = &b.idx_leaf_order[3 * size_t(t)];
@@ -811,8 +845,8 @@ static Hit trace_lr(const Build& b, f4 posNear, f4 dirFar) {
     const Node n0 = nodes[p.left];
     const Node n1 = nodes[p.escape];
     float a0, b0, a1, b1;
-    slab(o, inv, n0.bmin, n0.bmax, a0, b0);
-    slab(o, inv, n1.bmin, n1.bmax, a1, b1);
+    slab(o, d, inv, n0.bmin, n0.bmax, a0, b0);
+    slab(o, d, inv, n1.bmin, n1.bmax, a1, b1);
     const bool h0 = (a0 <= b0) && (b0 >= tNear) && (a0 <= h.t);
     const bool h1 = (a1 <= b1) && (b1 >= tNear) && (a1 <= h.t);
     uint32_t next = 0xFFFFFFFFu;
@@ -863,7 +897,7 @@ static Hit trace_brute(const Build& b, f4 posNear, f4 dirFar) {
   {
     f3 lo = {b.bbmin.x, b.bbmin.y, b.bbmin.z}, hi = {b.bbmax.x, b.bbmax.y, b.bbmax.z};
     float a, c;
-    slab(o, safe_inverse(d), lo, hi, a, c);
+    slab(o, d, safe_inverse(d), lo, hi, a, c);
     if (!((a <= c) && (c >= tNear) && (a <= tFar))) return h;
   }
   for (uint32_t t = 0; t < b.ntris; t++) {
@@ -1142,7 +1176,8 @@ static Hit trace_bvh4(const Build& b, const std::vector<Quad>& quads, f4 posNear
   const f3 d = {dirFar.x, dirFar.y, dirFar.z};
   const float tNear = posNear.w;
   const f3 inv = safe_inverse(d);
-  const bool use_fused = fused && fused_ok(o, inv);
+  // the watertight mode's robust box test replaces the fused one
+  const bool use_fused = fused && fused_ok(o, inv) && g_tri_test != 1;
   uint32_t stack[64];
   int top = 0;
   uint32_t cur = 0;
@@ -1157,7 +1192,7 @@ static Hit trace_bvh4(const Build& b, const std::vector<Quad>& quads, f4 posNear
       for (int k = 0; k < 4; k++) {
         float a, c;
         if (use_fused) slab_fused(o, inv, q.c[k].bmin, q.c[k].bmax, a, c);
-        else slab(o, inv, q.c[k].bmin, q.c[k].bmax, a, c);
+        else slab(o, d, inv, q.c[k].bmin, q.c[k].bmax, a, c);
         // An unused slot (+1e9, -1e9) is inside out; the min/max slab test
         // turns it into the box [-1e9, 1e9], so the reference enters it and
         // finds its empty leaf.  Skipping slots whose reference is the empty
@@ -1366,7 +1401,7 @@ static Hit trace_scene(const Scene& s, f4 posNear, f4 dirFar, Counters* cnt) {
   };
   auto box_hit = [&](const Node& n, float& tmin) {
     float a, b;
-    slab(wo, winv, n.bmin, n.bmax, a, b);
+    slab(wo, wd, winv, n.bmin, n.bmax, a, b);
     tmin = a;
     return (a <= b) && (b >= tNear) && (a <= h.t);
   };
@@ -1789,8 +1824,13 @@ void hrto_safe_inverse(const float* d3, float* out3) {
   out3[0] = r.x; out3[1] = r.y; out3[2] = r.z;
 }
 void hrto_slab(const float* o3, const float* inv3, const float* bmin3, const float* bmax3, float* out2) {
-  slab(f3{o3[0], o3[1], o3[2]}, f3{inv3[0], inv3[1], inv3[2]}, f3{bmin3[0], bmin3[1], bmin3[2]},
+  slab_ref(f3{o3[0], o3[1], o3[2]}, f3{inv3[0], inv3[1], inv3[2]}, f3{bmin3[0], bmin3[1], bmin3[2]},
        f3{bmax3[0], bmax3[1], bmax3[2]}, out2[0], out2[1]);
+}
+void hrto_slab_robust(const float* o3, const float* d3, const float* bmin3, const float* bmax3, float* out2) {
+  const f3 d{d3[0], d3[1], d3[2]};
+  slab_robust(f3{o3[0], o3[1], o3[2]}, d, safe_inverse(d), f3{bmin3[0], bmin3[1], bmin3[2]},
+              f3{bmax3[0], bmax3[1], bmax3[2]}, out2[0], out2[1]);
 }
 void hrto_slab_fused(const float* o3, const float* inv3, const float* bmin3, const float* bmax3, float* out2) {
   slab_fused(f3{o3[0], o3[1], o3[2]}, f3{inv3[0], inv3[1], inv3[2]}, f3{bmin3[0], bmin3[1], bmin3[2]},

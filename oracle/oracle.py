@@ -95,6 +95,7 @@ def lib():
         L.hrto_ao_rays.restype = None
         L.hrto_ao_rays.argtypes = [vp, vp, C.c_uint32, vp, vp, u64, C.c_uint32, C.c_float, C.c_uint64, vp]
         L.hrto_slab_fused.argtypes = [vp, vp, vp, vp, vp]
+        L.hrto_slab_robust.argtypes = [vp, vp, vp, vp, vp]
         L.hrto_quad_count.restype = u64
         L.hrto_quad_count.argtypes = [vp]
         L.hrto_quads.argtypes = [vp, C.c_int, vp]
@@ -397,6 +398,14 @@ def slab(o, inv, bmin, bmax) -> np.ndarray:
     args = [np.asarray(a, dtype=np.float32) for a in (o, inv, bmin, bmax)]
     out = np.zeros(2, dtype=np.float32)
     lib().hrto_slab(*(_ptr(a) for a in args), _ptr(out))
+    return out
+
+
+def slab_robust(o, d, bmin, bmax) -> np.ndarray:
+    """The watertight mode's conservative box test: (entry, exit * (1 + 4 * 2^-23)), parallel axes exact."""
+    args = [np.asarray(a, dtype=np.float32) for a in (o, d, bmin, bmax)]
+    out = np.zeros(2, dtype=np.float32)
+    lib().hrto_slab_robust(*(_ptr(a) for a in args), _ptr(out))
     return out
 
 

@@ -14,7 +14,7 @@ from gpu_util import say, trace
 from heavyrt_amd import workloads as W
 from oracle import oracle as O
 from test_gpu_parity import rays_for
-from test_oracle_watertight import closed_sphere_rays
+from test_oracle_watertight import SPHERES, closed_sphere_rays
 
 pytestmark = pytest.mark.gpu
 
@@ -70,12 +70,19 @@ def test_watertight_nearest_and_any_equal_oracle(gpu_ctx, name, gen, layout):
     s.close()
 
 
+@pytest.mark.parametrize("level,radius,center", SPHERES, ids=["unit", "tiny", "far", "offset"])
 @pytest.mark.parametrize("layout", LAYOUTS, ids=LAYOUT_IDS)
-def test_no_ray_escapes_a_closed_mesh(gpu_ctx, layout):
-    v, f, rays = closed_sphere_rays()
+def test_no_ray_escapes_a_closed_mesh(gpu_ctx, layout, level, radius, center):
+    """Rays aimed exactly at every vertex and edge midpoint of a closed icosphere from inside it:
+    every one hits, nearest hits equal the oracle's traversal bit for bit (the conservative box test
+    included), and every any-hit query answers yes."""
+    v, f, rays = closed_sphere_rays(level, radius, center)
     s = wt_scene(gpu_ctx, v, f, layout)
-    g = trace(s, rays, "compact")
+    g = trace(s, rays, "full")
     assert np.all(g["prim"] != INVALID)
+    t, p, _, _ = oracle_nearest(O.OracleBVH(v, f), rays, layout)
+    np.testing.assert_array_equal(g["prim"], p)
+    np.testing.assert_array_equal(g["t"].view(np.uint32), t.view(np.uint32))
     anyr = rays.copy()
     anyr[:, 7] = -anyr[:, 7]
     assert np.all(trace(s, anyr, "any")["hit"])

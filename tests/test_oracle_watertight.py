@@ -105,14 +105,18 @@ def test_cyclic_axis_relabelling_gives_the_same_bits():
                 assert np.float32(got[0]).view(np.uint32) == np.float32(base[0]).view(np.uint32), k
 
 
-def closed_sphere_rays():
-    """An icosphere (closed, 1,280 triangles) and rays from points inside it
-    aimed exactly at every vertex and every edge midpoint (as float32)."""
-    v, f = W.icosphere(3, 1.0, (0.3, -0.2, 0.1))
+def closed_sphere_rays(level=3, radius=1.0, center=(0.3, -0.2, 0.1), offsets=((0, 0, 0), (0.25, 0.3, -0.4),
+                                                                           (-0.3, -0.4, 0.3))):
+    """An icosphere (closed; 1,280 triangles at level 3) and rays from points
+    inside it (the centre and two more, offsets in units of the radius) aimed
+    exactly at every vertex and every edge midpoint (as float32).  From the
+    centre many rays run parallel to a coordinate plane (a direction component
+    of exactly 0), some of them inside a face of a leaf's box."""
+    v, f = W.icosphere(level, radius, center)
     edges = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
     edges = np.unique(np.sort(edges, axis=1), axis=0)
     targets = np.concatenate([v, 0.5 * (v[edges[:, 0]] + v[edges[:, 1]])]).astype(np.float32)
-    origins = np.array([[0.3, -0.2, 0.1], [0.55, 0.1, -0.3], [0.0, -0.6, 0.4]], np.float32)
+    origins = (np.asarray(center, np.float64) + radius * np.asarray(offsets, np.float64)).astype(np.float32)
     rays = []
     for o in origins:
         d = targets - o
@@ -120,7 +124,7 @@ def closed_sphere_rays():
         r = np.zeros((targets.shape[0], 8), np.float32)
         r[:, 0:3] = o
         r[:, 4:7] = d
-        r[:, 7] = 10.0
+        r[:, 7] = 10.0 * radius
         rays.append(r)
     return v, f, np.concatenate(rays)
 
@@ -133,6 +137,75 @@ def test_no_ray_escapes_a_closed_mesh():
     assert counts.max() <= 6
     # most rays aimed at an edge or a vertex land in exactly one triangle (no zero edge value)
     assert (counts == 1).mean() > 0.5
+
+
+SPHERES = [(3, 1.0, (0.3, -0.2, 0.1)), (2, 0.01, (-0.004, 0.002, 0.003)), (4, 40.0, (1000.0, -500.0, 250.0)),
+           (3, 3.0, (-7.5, 12.25, 0.0))]
+
+
+@pytest.mark.parametrize("level,radius,center", SPHERES, ids=["unit", "tiny", "far", "offset"])
+def test_no_ray_escapes_through_the_trees(level, radius, center):
+    """The traversals keep watertightness: rays aimed exactly at every vertex and edge of a closed
+    mesh from inside it find a hit through the BVH2, the FP32 and FP16 BVH4 (and the any-hit box form)
+    of the reference tree and of the clustered any-hit tree, because the watertight mode's box test is
+    conservative."""
+    v, f, rays = closed_sphere_rays(level, radius, center)
+    anyr = rays.copy()
+    anyr[:, 7] = -anyr[:, 7]
+    trees = [O.OracleBVH(v, f), O.OracleBVH(v, f, cubic=True)]
+    trees[1].ploc(8)
+    with O.triangle_test(WT):
+        for b in trees:
+            assert np.all(b.trace(rays)[1] != INVALID)
+            assert np.all(b.trace(anyr)[1] != INVALID)
+            for mode in (O.MODE4_FP32, O.MODE4_FP16, O.MODE4_FP16_FUSED):
+                assert np.all(b.trace4(rays, mode)[1] != INVALID), mode
+
+
+def test_robust_box_test_known_answers():
+    lo, hi = np.zeros(3, np.float32), np.ones(3, np.float32)
+    # a generic ray: the entry is the reference slab's, the exit is scaled up by 1 + 4 * 2^-23
+    o, d = np.array([-1.0, 0.25, 0.5], np.float32), np.array([1.0, 0.1, 0.05], np.float32)
+    ref = O.slab(o, O.safe_inverse(d), lo, hi)
+    rob = O.slab_robust(o, d, lo, hi)
+    assert rob[0] == ref[0] and rob[1] == np.float32(ref[1] * np.float32(1 + 4 * 2.0 ** -23))
+    # a ray running inside the face x = 1 (d.x = 0): the reference slab rejects the box, the robust one keeps it
+    o, d = np.array([1.0, -1.0, 0.5], np.float32), np.array([0.0, 1.0, 0.0], np.float32)
+    ref = O.slab(o, O.safe_inverse(d), lo, hi)
+    rob = O.slab_robust(o, d, lo, hi)
+    assert not ref[0] <= ref[1]
+    assert rob[0] <= rob[1] and rob[0] == 1.0 and rob[1] >= 2.0
+    # one ulp outside that face: no t at all
+    o[0] = np.nextafter(np.float32(1.0), np.float32(2.0))
+    rob = O.slab_robust(o, d, lo, hi)
+    assert not rob[0] <= rob[1]
+    # behind the ray: rejected by the caller's exit >= tNear rule as before
+    o, d = np.array([2.0, 0.5, 0.5], np.float32), np.array([1.0, 0.0, 0.0], np.float32)
+    assert O.slab_robust(o, d, lo, hi)[1] < 0
+
+
+def test_robust_box_test_keeps_every_touched_corner():
+    """Rays aimed exactly at a box corner (as float32) from random origins: the reference slab
+    rounds some of those boxes away, the robust one keeps every one."""
+    r = W.uniform01(W.pcg32(20000, 9, 4400)).astype(np.float64)
+    lo = (r[:, 0:3] * 4 - 2).astype(np.float32)
+    hi = (lo + 0.01 + r[:, 3:6] * 3).astype(np.float32)
+    corner_sel = r[:, 6:9] < 0.5
+    corner = np.where(corner_sel, lo, hi)
+    o = (corner + (r[:, 0:3] * 2 - 1) * 10 ** (r[:, 8:9] * 6 - 3)).astype(np.float32)
+    ref_miss = rob_miss = 0
+    for k in range(r.shape[0]):
+        d = corner[k].astype(np.float64) - o[k]
+        n = np.linalg.norm(d)
+        if n == 0:
+            continue
+        d = (d / n).astype(np.float32)
+        a, b = O.slab(o[k], O.safe_inverse(d), lo[k], hi[k])
+        ref_miss += not (a <= b and b >= 0)
+        a, b = O.slab_robust(o[k], d, lo[k], hi[k])
+        rob_miss += not (a <= b and b >= 0)
+    assert rob_miss == 0
+    assert ref_miss > 0  # the case the robust form exists for does occur
 
 
 def test_leaky_without_slack_is_caught():
