@@ -63,7 +63,7 @@ EXPORTS = [
     "hrt_init", "hrt_shutdown", "hrt_ctx_stream", "hrt_scene_create", "hrt_scene_destroy",
     "hrt_scene_set_keep_build_arrays", "hrt_scene_clear_geom", "hrt_scene_clear_instances",
     "hrt_add_triangles", "hrt_add_instance", "hrt_scene_set_triangle_test", "hrt_commit", "hrt_trace_nearest",
-    "hrt_trace_any",
+    "hrt_trace_any", "hrt_trace_any_noalloc",
     "hrt_trace_nearest_stats", "hrt_trace_any_stats", "hrt_query_nearest_one", "hrt_geom_count",
     "hrt_instance_count", "hrt_debug_dump", "hrt_last_error", "hrt_rtao_make_rays", "hrt_rtao_trace", "hrt_rtao_render", "hrt_eye_make_rays", "hrt_eye_render", "hrt_scene_serialize", "hrt_scene_deserialize",
     "hrt_trace_nearest_host", "hrt_trace_any_host", "hrt_update_triangles", "hrt_update_instance",
@@ -127,6 +127,7 @@ def lib():
         L.hrt_scene_set_triangle_test.argtypes = [vp, i32]
         L.hrt_trace_nearest.argtypes = [vp, vp, sz, vp, i32, vp]
         L.hrt_trace_any.argtypes = [vp, vp, sz, vp, vp]
+        L.hrt_trace_any_noalloc.argtypes = [vp, vp, sz, vp, vp]
         L.hrt_trace_nearest_stats.argtypes = [vp, vp, sz, vp, i32, vp, vp]
         L.hrt_trace_any_stats.argtypes = [vp, vp, sz, vp, vp, vp]
         L.hrt_query_nearest_one.argtypes = [vp, vp, vp, vp]

@@ -23,7 +23,8 @@ INCLUDE = os.path.join(ROOT, "include")
 LIB_HRT = os.path.join(HERE, "libhrt.so")
 LIB_SCENE = os.path.join(HERE, "libhrt_scene.so")
 
-HIP_SOURCES = ["sort.hip", "lbvh.hip", "ploc.hip", "trace.hip", "trace4.hip", "trace_tlas.hip", "batch.hip", "serve.hip", "rtao.hip", "eye.hip", "capi.hip"]
+HIP_SOURCES = ["sort.hip", "lbvh.hip", "ploc.hip", "trace.hip", "trace4.hip", "trace_tlas.hip", "batch.hip", "serve.hip",
+               "rtao.hip", "eye.hip", "capi.hip", "noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_FLAGS = [

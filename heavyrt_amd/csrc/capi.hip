@@ -239,7 +239,7 @@ hrt_status select_blas(hrt_scene* s, const hrt::Blas** out, uint32_t* inst) {
 }
 
 hrt_status trace_common(hrt_scene* s, const float* rays, size_t n, void* out, int format, int anyhit,
-                        uint32_t* stats, void* stream) {
+                        uint32_t* stats, void* stream, uint32_t* workspace = nullptr, size_t workspace_words = 0) {
   if (!s) return fail(HRT_ERR_INVALID_ARG, "scene is null");
   if (!s->committed) return fail(HRT_ERR_NOT_COMMITTED, "CommitScene has not been called since the last change");
   const hrt::Blas* b = nullptr;
@@ -263,6 +263,8 @@ hrt_status trace_common(hrt_scene* s, const float* rays, size_t n, void* out, in
   a.inst_id = inst;
   a.geom_id = s->two_level ? 0u : s->instances[inst].geom;
   a.watertight = s->committed_tri_test == HRT_TRI_WATERTIGHT;
+  a.workspace = workspace;
+  a.workspace_words = workspace_words;
   hipStream_t hs = stream ? static_cast<hipStream_t>(stream) : s->ctx->stream;
   hipError_t e = hipSetDevice(s->ctx->device);
   if (e == hipSuccess) {
@@ -290,6 +292,16 @@ hrt_status scene_single_blas(hrt_scene* s, const Blas** out) {
   return select_blas(s, out, &inst);
 }
 int scene_triangle_test(const hrt_scene* s) { return s->committed_tri_test; }
+
+int scene_device(const hrt_scene* s) { return s->ctx->device; }
+
+hrt_status trace_any_in_workspace(hrt_scene* s, const float* rays, size_t n, uint8_t* out, void* stream,
+                                  uint32_t* workspace, size_t workspace_words) {
+  if (!s) return fail(HRT_ERR_INVALID_ARG, "scene is null");
+  if (s->committed && s->two_level)
+    return fail(HRT_ERR_UNSUPPORTED, "the allocation-free any-hit trace needs a scene with one identity instance");
+  return trace_common(s, rays, n, out, HRT_HIT_FULL, 1, nullptr, stream, workspace, workspace_words);
+}
 
 TreeRef scene_any_hit_tree(const hrt_scene* s) {
   // the branch trace_common takes for an any-hit batch of a single-tree scene
@@ -337,6 +349,12 @@ hrt_status hrt_init(int device, hrt_ctx** out) {
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) {
     delete c;
     return fail_hip(e, "hipStreamCreate");
+  }
+  // the allocation-free entry point's per-device scratch address (device current)
+  if ((e = hrt::noalloc_prepare(device)) != hipSuccess) {
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return fail_hip(e, "scratch symbol");
   }
   *out = c;
   return HRT_OK;

@@ -30,6 +30,17 @@ struct Blas;
 hrt_status scene_single_blas(hrt_scene* s, const Blas** out);
 // The triangle test (HRT_TRI_*) of a committed scene's traces.
 int scene_triangle_test(const hrt_scene* s);
+// hrt_trace_any with caller scratch (TraceArgs::workspace): no allocation.
+hrt_status trace_any_in_workspace(hrt_scene* s, const float* rays, size_t n, uint8_t* out, void* stream,
+                                  uint32_t* workspace, size_t workspace_words);
+// Looks up the allocation-free entry point's scratch on `device` (current),
+// once per device, and sizes the any-hit launches it feeds.
+hipError_t noalloc_prepare(int device);
+// Queries (and caches) the resident grids of the any-hit kernels.
+void warm_any_hit_launchers();
+void warm_bvh2_any_hit_launchers();
+// The device of a scene's context.
+int scene_device(const hrt_scene* s);
 
 // Device buffers of one bottom-level structure (one geometry).
 struct Blas {
@@ -147,6 +158,11 @@ struct TraceArgs {
   uint32_t inst_id = 0;  // written into full hit records on a hit
   uint32_t geom_id = 0;
   bool watertight = false;  // WtTest on Blas::tris_wt instead of the reference's MtTest
+  // caller-provided scratch for the persistent any-hit kernels (queue
+  // counter + stack spill): when set, nothing is allocated and the launch
+  // takes at most as many workgroups as the words hold stacks for
+  uint32_t* workspace = nullptr;
+  size_t workspace_words = 0;
 };
 
 // The triangle records a test policy reads.

@@ -294,17 +294,22 @@ template <bool kAny, bool kSpec, class Tri>
 hipError_t launch_persistent(const Blas& b, const TraceArgs& a, hipStream_t stream) {
   const void* kfn = reinterpret_cast<const void*>(&k_trace_persistent<kAny, kSpec, Tri>);
   uint64_t want = (a.n + kTraceBlock - 1) / kTraceBlock;
-  const int grid = int(std::min<uint64_t>(want, resident_grid(kfn, kTraceBlock)));
+  constexpr size_t kLaneWords = kStackDepth - kShortStack;
+  uint64_t cap = resident_grid(kfn, kTraceBlock);
+  if (a.workspace) cap = std::min<uint64_t>(cap, (a.workspace_words - 64) / (kTraceBlock * kLaneWords));
+  const int grid = int(std::max<uint64_t>(1, std::min<uint64_t>(want, cap)));
   // queue counter + spill area for stack entries beyond the LDS part
-  const size_t spill_words = size_t(grid) * kTraceBlock * (kStackDepth - kShortStack);
-  uint32_t* buf = nullptr;
-  hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&buf), sizeof(uint32_t) * (64 + spill_words), stream);
-  if (e != hipSuccess) return e;
+  const size_t spill_words = size_t(grid) * kTraceBlock * kLaneWords;
+  uint32_t* buf = a.workspace;
+  hipError_t e = hipSuccess;
+  if (!buf && (e = hipMallocAsync(reinterpret_cast<void**>(&buf), sizeof(uint32_t) * (64 + spill_words), stream)) !=
+                  hipSuccess)
+    return e;
   k_reset_counter<<<1, 1, 0, stream>>>(buf);
   k_trace_persistent<kAny, kSpec, Tri><<<grid, kTraceBlock, 0, stream>>>(
       b.pairs, tri_records<Tri>(b), a.rays, a.n, a.out, a.format, make_uint2(a.inst_id, a.geom_id), buf, buf + 64);
   e = hipGetLastError();
-  hipError_t f = hipFreeAsync(buf, stream);
+  hipError_t f = a.workspace ? hipSuccess : hipFreeAsync(buf, stream);
   return e != hipSuccess ? e : f;
 }
 
@@ -343,6 +348,11 @@ hipError_t trace_blas_with(const Blas& b, const TraceArgs& a, hipStream_t stream
 }
 
 }  // namespace
+
+void warm_bvh2_any_hit_launchers() {
+  (void)resident_grid(reinterpret_cast<const void*>(&k_trace_persistent<true, true, MtTest>), kTraceBlock);
+  (void)resident_grid(reinterpret_cast<const void*>(&k_trace_persistent<true, true, WtTest>), kTraceBlock);
+}
 
 uint32_t resident_grid(const void* kernel, int block) {
   struct Entry {

@@ -367,21 +367,27 @@ __global__ __launch_bounds__(kBlock4, 8) void k_trace4_any_persistent(const void
 }
 
 template <bool kFp16, class Src, class Tri>
-hipError_t launch_any4(const Blas& b, Src src, uint64_t n, uint8_t* out, hipStream_t stream) {
+hipError_t launch_any4(const Blas& b, Src src, uint64_t n, uint8_t* out, hipStream_t stream,
+                       uint32_t* workspace = nullptr, size_t workspace_words = 0) {
   const void* kfn = reinterpret_cast<const void*>(&k_trace4_any_persistent<kFp16, Src, Tri>);
   const uint64_t want = (n + kBlock4 - 1) / kBlock4;
-  const int grid = int(std::min<uint64_t>(want, resident_grid(kfn, kBlock4)));
+  constexpr size_t kLaneWords = kStackDepth4 - kShortStack;
+  uint64_t cap = resident_grid(kfn, kBlock4);
+  if (workspace) cap = std::min<uint64_t>(cap, (workspace_words - 64) / (kBlock4 * kLaneWords));
+  const int grid = int(std::max<uint64_t>(1, std::min<uint64_t>(want, cap)));
   const void* q = kFp16 ? static_cast<const void*>(b.quads16) : static_cast<const void*>(b.quads32);
   // queue counter + spill area for stack entries beyond the LDS part
-  const size_t spill_words = size_t(grid) * kBlock4 * (kStackDepth4 - kShortStack);
-  uint32_t* buf = nullptr;
-  hipError_t e = hipMallocAsync(reinterpret_cast<void**>(&buf), sizeof(uint32_t) * (64 + spill_words), stream);
-  if (e != hipSuccess) return e;
+  const size_t spill_words = size_t(grid) * kBlock4 * kLaneWords;
+  uint32_t* buf = workspace;
+  hipError_t e = hipSuccess;
+  if (!buf && (e = hipMallocAsync(reinterpret_cast<void**>(&buf), sizeof(uint32_t) * (64 + spill_words), stream)) !=
+                  hipSuccess)
+    return e;
   k_reset_counter4<<<1, 1, 0, stream>>>(buf);
   k_trace4_any_persistent<kFp16, Src, Tri><<<grid, kBlock4, 0, stream>>>(q, tri_records<Tri>(b), src, n, out, buf,
                                                                           buf + 64);
   e = hipGetLastError();
-  hipError_t f = hipFreeAsync(buf, stream);
+  hipError_t f = workspace ? hipSuccess : hipFreeAsync(buf, stream);
   return e != hipSuccess ? e : f;
 }
 
@@ -510,8 +516,8 @@ hipError_t trace_blas4_with(const Blas& b, const TraceArgs& a, bool fp16, hipStr
   if (a.anyhit && !a.stats) {
     const StoredRays src{a.rays};
     uint8_t* out = static_cast<uint8_t*>(a.out);
-    return fp16 ? launch_any4<true, StoredRays, Tri>(b, src, a.n, out, stream)
-                : launch_any4<false, StoredRays, Tri>(b, src, a.n, out, stream);
+    return fp16 ? launch_any4<true, StoredRays, Tri>(b, src, a.n, out, stream, a.workspace, a.workspace_words)
+                : launch_any4<false, StoredRays, Tri>(b, src, a.n, out, stream, a.workspace, a.workspace_words);
   }
   if (!a.stats && !a.anyhit && a.n >= kSortRaysMin4)
     return fp16 ? launch_nearest4_sorted<true, Tri>(b, a, stream) : launch_nearest4_sorted<false, Tri>(b, a, stream);
@@ -545,6 +551,15 @@ hipError_t serve_any4(bool fp16, const uint32_t* off, const ServeSeg* segs, uint
     k_trace4_any_persistent<false, ServeRays, MtTest><<<grid, kBlock4, 0, stream>>>(nullptr, nullptr, src, nrays,
                                                                                     nullptr, queue, spill);
   return hipGetLastError();
+}
+
+void warm_any_hit_launchers() {
+  warm_bvh2_any_hit_launchers();
+  const void* k[] = {reinterpret_cast<const void*>(&k_trace4_any_persistent<true, StoredRays, MtTest>),
+                     reinterpret_cast<const void*>(&k_trace4_any_persistent<false, StoredRays, MtTest>),
+                     reinterpret_cast<const void*>(&k_trace4_any_persistent<true, StoredRays, WtTest>),
+                     reinterpret_cast<const void*>(&k_trace4_any_persistent<false, StoredRays, WtTest>)};
+  for (const void* f : k) (void)resident_grid(f, kBlock4);
 }
 
 uint32_t serve_any4_resident_grid(bool fp16) {

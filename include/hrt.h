@@ -211,6 +211,18 @@ hrt_status hrt_trace_nearest(hrt_scene* scene, const float* rays, size_t n, void
  * out[i] = 1 if any triangle is hit in (tNear, |tFar|). */
 hrt_status hrt_trace_any(hrt_scene* scene, const float* rays, size_t n, uint8_t* out, void* stream);
 
+/* hrt_trace_any with no allocation of any kind (no hipMalloc, no pool, no
+ * allocating library call): the any-hit kernels' queue counter and stack
+ * spill live in a static device array of 64 MiB + 256 B that each GPU the
+ * library runs on holds until the process ends (its address is looked up
+ * by hrt_init).  No counterpart in the reference: for callers that cannot
+ * allocate -- code under HIP graph capture, tight serving loops, harnesses
+ * that forbid allocation.  Scenes with one identity instance only
+ * (HRT_ERR_UNSUPPORTED otherwise); any n up to 2^31-1 (the launch takes at
+ * most as many workgroups as the array holds stacks for).  Every call on one
+ * GPU shares that GPU's array: calls that overlap on one GPU race. */
+hrt_status hrt_trace_any_noalloc(hrt_scene* scene, const float* rays, size_t n, uint8_t* out, void* stream);
+
 /* Same as hrt_trace_nearest, and also writes per-ray traversal counters
  * (uint32 pairs visited, uint32 triangles tested) -- the reference's NC/2 and
  * TC metrics (CrossRT.h:172-175, BVH2CommonRT.cpp:69,124). */
