@@ -853,8 +853,9 @@ def bench_serving(args, cpu) -> dict:
 # HBM traffic from rocprofv3 counters (a child process per counter pass)
 # ---------------------------------------------------------------------------
 def counter_child(workloads) -> int:
-    """Runs under rocprofv3: a 1 GiB device copy (the calibration), then three
-    launches of each workload's dominant kernel after its setup."""
+    """Runs under rocprofv3: a 1 GiB device copy (the byte calibration), two
+    launches of the issue-rate calibration kernel (hrt_issue_probe), then
+    three launches of each workload's dominant kernel after its setup."""
     import torch
 
     import heavyrt_amd as H
@@ -867,6 +868,9 @@ def counter_child(workloads) -> int:
     torch.cuda.synchronize()
     del src, dst
     ctx = H.Context(0)
+    for _ in range(2):
+        ctx.issue_probe()
+    torch.cuda.synchronize()
     stream = torch.cuda.Stream()
     torch.cuda.set_stream(stream)
     sp = stream.cuda_stream
@@ -942,14 +946,20 @@ def measure_traffic(workloads, timeout_s: float = 240.0):
     """HBM bytes per launch of each workload's dominant kernel, from two
     rocprofv3 passes (FETCH_SIZE; WRITE_SIZE with the L2 hit/miss counts),
     each scaled by what the counter reads for the 1 GiB copy in the same
-    child.  Returns ({workload: {...}}, reason when nothing was measured)."""
+    child; and its VALU issue rate from a third pass (SQ_INSTS_VALU per
+    GRBM_GUI_ACTIVE cycle) against the calibration kernel hrt_issue_probe's
+    in the same child.  Returns ({workload: {...}}, reason when nothing was
+    measured)."""
     exe = shutil.which("rocprofv3")
     if not exe:
         return {}, "rocprofv3 not found"
     out = {w: {} for w in workloads}
     tmp = tempfile.mkdtemp(prefix="hrt_counters_")
     env = dict(os.environ, TMPDIR="/tmp")
-    passes = [("fetch", ["FETCH_SIZE"]), ("write", ["WRITE_SIZE", "TCC_HIT_sum", "TCC_MISS_sum"])]
+    passes = [("fetch", ["FETCH_SIZE"]), ("write", ["WRITE_SIZE", "TCC_HIT_sum", "TCC_MISS_sum"]),
+              ("issue", ["SQ_INSTS_VALU", "SQ_INSTS_SALU", "GRBM_GUI_ACTIVE"])]
+    issue = {w: [] for w in workloads}  # per launch: (VALU instructions, SALU instructions, GRBM cycles)
+    probe = []
     try:
         for tag, counters in passes:
             d = os.path.join(tmp, tag)
@@ -963,6 +973,21 @@ def measure_traffic(workloads, timeout_s: float = 240.0):
             rows = read_counter_csv(d)
             if not rows:
                 return {}, f"rocprofv3 pass {tag} wrote no counters (rc {r.returncode}): {r.stderr[-200:]}"
+            if tag == "issue":
+                per = {}
+                for did, name, cname, val in rows:
+                    per.setdefault(did, [name, {}])[1][cname] = val
+                for did in sorted(per):
+                    name, c = per[did]
+                    if not all(k in c for k in counters) or c["GRBM_GUI_ACTIVE"] <= 0:
+                        continue
+                    rec = (c["SQ_INSTS_VALU"], c["SQ_INSTS_SALU"], c["GRBM_GUI_ACTIVE"])
+                    if "k_issue_probe" in name:
+                        probe.append(rec)
+                    for w in workloads:
+                        if COUNTER_KERNELS[w] in name:
+                            issue[w].append(rec)
+                continue
             by_counter = {}
             for _, name, cname, val in rows:
                 by_counter.setdefault(cname, []).append((name, val))
@@ -992,6 +1017,14 @@ def measure_traffic(workloads, timeout_s: float = 240.0):
         miss = c.get("TCC_MISS_sum", {}).get("kernel_value")
         res[w] = {"hbm_read_bytes": fetch, "hbm_write_bytes": write,
                   "l2_hit_rate": (hit / (hit + miss)) if hit is not None and miss and hit + miss > 0 else None}
+        if probe and issue[w]:
+            last = issue[w][-3:]
+            rate = float(np.mean([v / g for v, _, g in last]))
+            peak = max(v / g for v, _, g in probe)
+            res[w]["issue"] = {"valu_instructions": round(float(np.mean([v for v, _, _ in last]))),
+                               "salu_instructions": round(float(np.mean([s for _, s, _ in last]))),
+                               "valu_per_cycle": round(rate, 2), "probe_valu_per_cycle": round(peak, 2),
+                               "frac": round(rate / peak, 4)}
     return res, ("" if res else "no workload kernel found in the counter output")
 
 
@@ -1007,6 +1040,12 @@ def attach_traffic(res: dict, meas: dict | None, reason: str) -> None:
                                    "launches), each scaled by the counter's reading for a 1 GiB device copy",
                  "hbm_read_bytes": round(meas["hbm_read_bytes"]), "hbm_write_bytes": round(meas["hbm_write_bytes"]),
                  "l2_hit_rate": None if meas["l2_hit_rate"] is None else round(meas["l2_hit_rate"], 4)})
+    if "issue" in meas:
+        # VALU instructions issued per GPU cycle (summed over every SIMD) against the calibration kernel's
+        # rate in the same run: how close the kernel is to the instruction-issue bound
+        roof["issue"] = dict(meas["issue"], source="rocprofv3 SQ_INSTS_VALU / GRBM_GUI_ACTIVE of this kernel "
+                                                   "(mean of 3 launches) over the same ratio for hrt_issue_probe")
+        roof["limit"] = "valu_issue" if meas["issue"]["frac"] > roof["frac"] else "hbm"
 
 
 # ---------------------------------------------------------------------------

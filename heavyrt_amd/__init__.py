@@ -70,8 +70,8 @@ EXPORTS = [
     "hrt_batch_create", "hrt_batch_destroy", "hrt_batch_buffers", "hrt_batch_run", "hrt_batch_build",
     "hrt_batch_trace", "hrt_batch_candidate_count", "hrt_batch_candidate_info", "hrt_batch_set_candidate",
     "hrt_batch_get_candidate", "hrt_batch_autotune", "hrt_batch_dump", "hrt_batch_status",
-    "hrt_host_alloc", "hrt_host_free", "hrt_server_create", "hrt_server_destroy", "hrt_server_add_scene",
-    "hrt_server_submit", "hrt_server_poll", "hrt_server_wait", "hrt_server_stats",
+    "hrt_issue_probe", "hrt_host_alloc", "hrt_host_free", "hrt_server_create", "hrt_server_destroy", "hrt_server_add_scene",
+    "hrt_server_submit", "hrt_server_poll", "hrt_server_wait", "hrt_server_stats", "hrt_server_tree_rays",
 ]
 
 
@@ -161,6 +161,7 @@ def lib():
         L.hrt_batch_autotune.argtypes = [vp, i32, C.POINTER(i32), vp]
         L.hrt_batch_dump.argtypes = [vp, u32, i32, vp, sz]
         L.hrt_batch_status.argtypes = [vp, C.POINTER(u32)]
+        L.hrt_issue_probe.argtypes = [vp, vp]
         L.hrt_host_alloc.argtypes = [sz, C.POINTER(vp)]
         L.hrt_host_free.argtypes = [vp]
         L.hrt_host_free.restype = None
@@ -172,6 +173,7 @@ def lib():
         L.hrt_server_poll.argtypes = [vp, C.c_uint64]
         L.hrt_server_wait.argtypes = [vp, C.c_uint64, C.POINTER(C.c_double), C.POINTER(C.c_double)]
         L.hrt_server_stats.argtypes = [vp, vp]
+        L.hrt_server_tree_rays.argtypes = [vp, vp]
         _lib = L
     return _lib
 
@@ -215,6 +217,10 @@ class Context:
         wi = np.ascontiguousarray(world_view_inv, dtype=np.float32).reshape(16)
         _check(lib().hrt_eye_make_rays(self._h, _ptr(pi), _ptr(wi), width, height, _dev_ptr(out_rays),
                                        _dev_ptr(out_xy) if out_xy is not None else None, stream or None))
+
+    def issue_probe(self, stream: int = 0):
+        """Launches the VALU issue-rate calibration kernel (hrt_issue_probe)."""
+        _check(lib().hrt_issue_probe(self._h, stream or None))
 
     def close(self):
         """Destroys the context, and first every scene still open on it (a scene
@@ -640,4 +646,10 @@ class Server:
         return {"completed": int(o[0]), "rays": int(o[1]), "steps": int(o[2]),
                 "mean_requests_per_step": int(o[3]) / 1000.0, "peak_bytes_in_flight": int(o[4]),
                 "budget_bytes": int(o[5])}
+
+    def tree_rays(self) -> dict:
+        """Rays launched so far per kind of tree."""
+        o = np.zeros(3, dtype=np.uint64)
+        _check(lib().hrt_server_tree_rays(self._h, _ptr(o)))
+        return {"bvh2": int(o[0]), "bvh4_fp16": int(o[1]), "bvh4_fp32": int(o[2])}
 

@@ -24,7 +24,7 @@ LIB_HRT = os.path.join(HERE, "libhrt.so")
 LIB_SCENE = os.path.join(HERE, "libhrt_scene.so")
 
 HIP_SOURCES = ["sort.hip", "lbvh.hip", "ploc.hip", "trace.hip", "trace4.hip", "trace_tlas.hip", "batch.hip", "serve.hip",
-               "rtao.hip", "eye.hip", "capi.hip", "noalloc.hip"]
+               "rtao.hip", "eye.hip", "capi.hip", "noalloc.hip", "probe.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_FLAGS = [

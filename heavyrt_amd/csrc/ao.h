@@ -105,4 +105,30 @@ __device__ inline Ray ao_ray(const AoFrame& f, uint64_t k, float tmax, uint64_t 
   return r;
 }
 
+// AO samples are traced in direction order inside runs of kAoOrderRun
+// consecutive samples: the wave that takes a run's next rays then holds rays
+// of one direction bin from neighbouring pixels, so their node and triangle
+// fetches share cache lines and their traversals take similar paths.
+// Any-hit answers do not depend on which lane traces which ray.
+constexpr uint32_t kAoOrderRun = 1024;
+constexpr uint32_t kAoBins = 24;
+// 24 direction bins: the cube face of the largest |d| component (the lower
+// axis on ties; 6 faces) times the signs of the two other components.
+__device__ inline uint32_t ao_dir_bin(float3 d) {
+  const float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+  uint32_t face;
+  float p, q;
+  if (ax >= ay && ax >= az) {
+    face = d.x < 0.0f ? 1u : 0u;
+    p = d.y, q = d.z;
+  } else if (ay >= az) {
+    face = d.y < 0.0f ? 3u : 2u;
+    p = d.x, q = d.z;
+  } else {
+    face = d.z < 0.0f ? 5u : 4u;
+    p = d.x, q = d.y;
+  }
+  return face * 4u + (p >= 0.0f ? 2u : 0u) + (q >= 0.0f ? 1u : 0u);
+}
+
 }  // namespace hrt

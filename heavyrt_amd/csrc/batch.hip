@@ -371,10 +371,8 @@ __global__ __launch_bounds__(64 * kWaves) void k_b_trace(const NodePair* __restr
                                                          uint32_t* __restrict__ spill) {
   constexpr int kBlk = 64 * kWaves;
   __shared__ uint32_t stack_mem[kLds * kBlk];
-  const int lane = threadIdx.x & 63;
   const uint32_t lanes = gridDim.x * kBlk;
   const ShortStack<kLds, kBlk> stk{stack_mem + threadIdx.x, spill + blockIdx.x * kBlk + threadIdx.x, lanes};
-  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
   bool active = false;
   uint32_t rid = 0;
@@ -395,7 +393,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_b_trace(const NodePair* __restr
     const uint64_t idle = __ballot(!active);
     const uint32_t nidle = __popcll(idle);
     if (!pool.drained() && (nidle >= uint32_t(kRefill) || nidle == 64)) {
-      const uint64_t idx = pool.take<kClaim>(nidle, __popcll(idle & lt_mask), queue, R);
+      const uint64_t idx = pool.take<kClaim>(nidle, lanes_below(idle), queue, R);
       if (!active) {
         if (idx < R) {
           rid = uint32_t(idx);

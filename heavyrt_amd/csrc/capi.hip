@@ -307,23 +307,18 @@ TreeRef scene_any_hit_tree(const hrt_scene* s) {
   // the branch trace_common takes for an any-hit batch of a single-tree scene
   const Blas& b = s->geoms[s->instances[0].geom];
   const int lay = s->layout;
+  const bool auto16 = lay == HRT_LAYOUT_AUTO_FP16;
   TreeRef t;
-  if (lay == HRT_LAYOUT_BVH2 || !b.quads32) {
-    t.nodes = b.pairs;
-    t.tris = b.tris;
-    t.kind = 0;
-  } else if (lay == HRT_LAYOUT_AUTO_FP16 && b.any_tree) {
-    t.nodes = b.any_tree->quads16;
-    t.tris = b.any_tree->tris;
-    t.kind = 1;
-  } else if (lay == HRT_LAYOUT_BVH4_FP16 || lay == HRT_LAYOUT_AUTO_FP16) {
-    t.nodes = b.quads16;
-    t.tris = b.tris;
-    t.kind = 1;
-  } else {
-    t.nodes = b.quads32;
-    t.tris = b.tris;
-    t.kind = 2;
+  t.nodes = b.pairs;
+  t.tris = b.tris;
+  t.kind = 0;
+  if (lay == HRT_LAYOUT_BVH2) return t;
+  if (auto16 && b.any_tree) {
+    if (b.any_tree->quads16) t = TreeRef{b.any_tree->quads16, b.any_tree->tris, 1};
+  } else if (lay == HRT_LAYOUT_BVH4_FP16 || auto16) {
+    if (b.quads16) t = TreeRef{b.quads16, b.tris, 1};
+  } else if (b.quads32) {
+    t = TreeRef{b.quads32, b.tris, 2};
   }
   return t;
 }
@@ -812,12 +807,16 @@ hipError_t rtao_pass(hrt_scene* s, const hrt::Blas* b, uint32_t inst, const hrt:
   }
   const hrt::Blas& tree = (auto16 && b->any_tree) ? *b->any_tree : *b;
   const bool fp16 = lay == HRT_LAYOUT_BVH4_FP16 || auto16;
+  // the frames (32 B per pixel), then the samples' direction order (2 B each)
   void* frames = nullptr;
-  if ((e = hipMallocAsync(&frames, 32ull * npix, hs)) != hipSuccess) return e;
+  if ((e = hipMallocAsync(&frames, 32ull * npix + 2ull * n, hs)) != hipSuccess) return e;
   hrt::AoFrame* f = static_cast<hrt::AoFrame*>(frames);
+  uint16_t* order = reinterpret_cast<uint16_t*>(static_cast<char*>(frames) + 32ull * npix);
   e = hrt::rtao_frames(*b, prim, hits, npix, f, hs);
+  if (e == hipSuccess) e = hrt::rtao_order(f, npix, spp, seed, order, hs);
   if (e == hipSuccess)
-    e = hrt::trace_ao4(tree, fp16, s->committed_tri_test == HRT_TRI_WATERTIGHT, f, npix, spp, tmax, seed, out, hs);
+    e = hrt::trace_ao4(tree, fp16, s->committed_tri_test == HRT_TRI_WATERTIGHT, f, order, npix, spp, tmax, seed, out,
+                       hs);
   const hipError_t fe = hipFreeAsync(frames, hs);
   return e == hipSuccess ? fe : e;
 }

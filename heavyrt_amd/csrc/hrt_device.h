@@ -91,6 +91,11 @@ constexpr int kShortStack = 16;  // LDS entries per lane in the persistent kerne
 // that every wave of the chip hits at each refill limits the whole kernel).
 // Every member is wave-uniform; take() must be called by all lanes of the
 // wave together.
+// Set bits of m in the lanes below this one (v_mbcnt_lo / v_mbcnt_hi).
+__device__ inline uint32_t lanes_below(uint64_t m) {
+  return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
+}
+
 struct RayPool {
   uint64_t next = 0, end = 0;  // indices claimed by the wave, not yet given out
   bool exhausted = false;      // the global queue has been run past n

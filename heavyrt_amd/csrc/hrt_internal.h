@@ -219,11 +219,16 @@ hipError_t rtao_make_rays(const Blas& b, const Ray* prim_rays, const void* prim_
 struct AoFrame;
 hipError_t rtao_frames(const Blas& b, const Ray* prim_rays, const void* prim_hits_compact, uint64_t npix,
                        AoFrame* out, hipStream_t stream);
+// The direction order of every AO sample of npix frames (ao.h, kAoOrderRun):
+// npix * spp 16-bit offsets for trace_ao4.
+hipError_t rtao_order(const AoFrame* frames, uint64_t npix, uint32_t spp, uint64_t seed,
+                      uint16_t* order, hipStream_t stream);
 // Any-hit of every AO sample of npix frames on b's BVH4 nodes (FP16 or FP32),
-// each ray made in registers exactly as rtao_make_rays writes it; out gets one
-// byte per sample.  npix * spp must stay below 2^32.
-hipError_t trace_ao4(const Blas& b, bool fp16, bool watertight, const AoFrame* frames, uint64_t npix, uint32_t spp,
-                     float tmax, uint64_t seed, uint8_t* out, hipStream_t stream);
+// each ray made in registers exactly as rtao_make_rays writes it, taken in the
+// order rtao_order wrote (nullable: sample order); out gets one byte per
+// sample.  npix * spp must stay below 2^32.
+hipError_t trace_ao4(const Blas& b, bool fp16, bool watertight, const AoFrame* frames, const uint16_t* order,
+                     uint64_t npix, uint32_t spp, float tmax, uint64_t seed, uint8_t* out, hipStream_t stream);
 
 // Eye rays of a pinhole camera, one per pixel, in SuperBlockIndex2DOpt order
 // (width and height multiples of 8); packed_xy (nullable) gets y << 16 | x.

@@ -73,10 +73,8 @@ __global__ __launch_bounds__(kWaveS) void k_serve(const uint8_t* __restrict__ ta
   uint32_t* queue = const_cast<uint32_t*>(reinterpret_cast<const uint32_t*>(table));
   const uint32_t* off = reinterpret_cast<const uint32_t*>(table) + kTabOffsWords;
   const ServeSeg* segs = reinterpret_cast<const ServeSeg*>(table + kTabSegByte);
-  const int lane = threadIdx.x & 63;
   const ShortStack<kShortStack, kWaveS> stk{stack_mem + threadIdx.x, spill + blockIdx.x * kWaveS + threadIdx.x,
                                             gridDim.x * kWaveS};
-  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   bool active = false;
   uint32_t rid = 0, loc = 0;
   const NodePair* P = nullptr;
@@ -96,7 +94,7 @@ __global__ __launch_bounds__(kWaveS) void k_serve(const uint8_t* __restrict__ ta
     const uint64_t idle = __ballot(!active);
     const uint32_t nidle = __popcll(idle);
     if (!pool.drained() && (nidle >= kRefillS || nidle == 64)) {
-      const uint64_t idx = pool.take<kClaimS>(nidle, __popcll(idle & lt_mask), queue, R);
+      const uint64_t idx = pool.take<kClaimS>(nidle, lanes_below(idle), queue, R);
       if (!active && idx < R) {
         rid = uint32_t(idx);
         const uint32_t sg = seg_of_step(off, nseg, rid);
@@ -269,6 +267,7 @@ struct hrt_server {
   std::thread worker;
   // statistics
   uint64_t completed = 0, failed = 0, rays_done = 0, nsteps = 0, step_reqs = 0;
+  uint64_t kind_rays[kKinds] = {0, 0, 0};  // rays launched per tree kind
   size_t in_flight = 0, peak_in_flight = 0;
   hipError_t fault = hipSuccess;
 };
@@ -334,6 +333,7 @@ hipError_t launch_step(hrt_server* sv, Step& st) {
                      reinterpret_cast<const ServeSeg*>(tab + kTabSegByte), nk[t], rk[t], grid,
                      reinterpret_cast<uint32_t*>(tab), st.spill, st.stream);
     }
+    if (e == hipSuccess) sv->kind_rays[t] += rk[t];
   }
   for (uint32_t i = 0; i < k && e == hipSuccess; i++) {
     Request& q = sv->reqs[st.reqs[i]];
@@ -603,6 +603,13 @@ hrt_status hrt_server_wait(hrt_server* sv, uint64_t requestId, double* latencyMs
 
 // out[0..5]: completed requests, rays, steps, mean requests per step x 1000,
 // peak bytes in flight, budget bytes.
+hrt_status hrt_server_tree_rays(hrt_server* sv, uint64_t out[3]) {
+  if (!sv || !out) return fail(HRT_ERR_INVALID_ARG, "null argument");
+  std::lock_guard<std::mutex> lk(sv->mu);
+  for (int t = 0; t < kKinds; t++) out[t] = sv->kind_rays[t];
+  return HRT_OK;
+}
+
 hrt_status hrt_server_stats(hrt_server* sv, uint64_t out[6]) {
   if (!sv || !out) return fail(HRT_ERR_INVALID_ARG, "null argument");
   std::lock_guard<std::mutex> lk(sv->mu);

@@ -120,7 +120,6 @@ __global__ __launch_bounds__(kOsThreads) void k_os_pass(const uint32_t* __restri
   const uint64_t tile_base = uint64_t(tile) * kOsTile;
 
   // 2. load and rank; wave w owns keys [w * 1024, w * 1024 + 1024) of the tile
-  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
   const uint64_t base = tile_base + uint64_t(w) * (kOsRounds * 64);
   uint32_t key[kOsRounds], val[kOsRounds], pos[kOsRounds];
 #pragma unroll
@@ -142,7 +141,7 @@ __global__ __launch_bounds__(kOsThreads) void k_os_pass(const uint32_t* __restri
       const uint64_t m = __ballot(bit);
       peers &= bit ? m : ~m;
     }
-    const uint32_t rank = __popcll(peers & lt_mask);
+    const uint32_t rank = lanes_below(peers);
     const uint32_t cnt = __popcll(peers);
     const uint32_t before = wcnt[w][digit];
     __builtin_amdgcn_wave_barrier();

@@ -149,11 +149,9 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_persistent(const NodePair
                                                                   uint32_t* __restrict__ queue,
                                                                   uint32_t* __restrict__ spill) {
   __shared__ uint32_t stack_mem[kShortStack * kTraceBlock];
-  const int lane = threadIdx.x & (kWave - 1);
   const uint32_t lanes = gridDim.x * kTraceBlock;
   const ShortStack<kShortStack, kTraceBlock> stk{stack_mem + threadIdx.x,
                                                  spill + blockIdx.x * kTraceBlock + threadIdx.x, lanes};
-  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
 
   // per-lane ray state
   bool active = false;
@@ -176,7 +174,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_persistent(const NodePair
     const uint64_t idle = __ballot(!active);
     const uint32_t nidle = __popcll(idle);
     if (!pool.drained() && (nidle >= kRefillAt || nidle == kWave)) {
-      const uint64_t idx = pool.take<kClaim>(nidle, __popcll(idle & lt_mask), queue, n);
+      const uint64_t idx = pool.take<kClaim>(nidle, lanes_below(idle), queue, n);
       if (!active) {
         if (idx < n) {
           rid = idx;

@@ -213,18 +213,25 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_simple(const void* __restric
 // ray i is ray `slot` of one request, which names its own tree and answer
 // buffer.  A ray with an empty interval (tFar <= tNear, or a NaN bound) can
 // pass no triangle's t test and finishes at once.
+// A launch-wide ray source gives the ray index taken at queue position i
+// (index) and that ray (ray).
 struct StoredRays {
   static constexpr bool kSegments = false;
   const Ray* __restrict__ rays;
+  __device__ uint64_t index(uint64_t i) const { return i; }
   __device__ Ray ray(uint64_t i) const { return rays[i]; }
 };
 
 struct AoRays {
   static constexpr bool kSegments = false;
   const AoFrame* __restrict__ frames;
+  const uint16_t* __restrict__ order;  // rtao_order's direction order, or null
   uint32_t spp;
   float tmax;
   uint64_t seed;
+  __device__ uint64_t index(uint64_t i) const {
+    return order ? (i & ~uint64_t(kAoOrderRun - 1)) + order[i] : i;
+  }
   // 32-bit division: the launcher keeps npix * spp below 2^32
   __device__ Ray ray(uint64_t i) const { return ao_ray(frames[uint32_t(i) / spp], i, tmax, seed); }
 };
@@ -257,10 +264,8 @@ __global__ __launch_bounds__(kBlock4, 8) void k_trace4_any_persistent(const void
                                                                    uint32_t* __restrict__ queue,
                                                                    uint32_t* __restrict__ spill) {
   __shared__ uint32_t stack_mem[kShortStack * kBlock4];
-  const int lane = threadIdx.x & (kWave - 1);
   const ShortStack<kShortStack, kBlock4> stk{stack_mem + threadIdx.x, spill + blockIdx.x * kBlock4 + threadIdx.x,
                                              gridDim.x * kBlock4};
-  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
   bool active = false;
   uint64_t rid = 0;
   float3 o = make_float3(0.f, 0.f, 0.f), d = o, inv = o, oi = o;
@@ -285,15 +290,15 @@ __global__ __launch_bounds__(kBlock4, 8) void k_trace4_any_persistent(const void
     const uint64_t idle = __ballot(!active);
     const uint32_t nidle = __popcll(idle);
     if (!pool.drained() && (nidle >= kRefillAt4 || nidle == kWave)) {
-      const uint64_t idx = pool.take<kClaim>(nidle, __popcll(idle & lt_mask), queue, n);
+      const uint64_t idx = pool.take<kClaim>(nidle, lanes_below(idle), queue, n);
       if (!active) {
         if (idx < n) {
           Ray r;
           if constexpr (Src::kSegments) {
             r = src.ray(idx, lane_quads, lane_tris, lane_out, rid);
           } else {
-            rid = idx;
-            r = src.ray(idx);
+            rid = src.index(idx);
+            r = src.ray(rid);
           }
           o = make_float3(r.pos_near.x, r.pos_near.y, r.pos_near.z);
           d = make_float3(r.dir_far.x, r.dir_far.y, r.dir_far.z);
@@ -526,12 +531,12 @@ hipError_t trace_blas4_with(const Blas& b, const TraceArgs& a, bool fp16, hipStr
 
 }  // namespace
 
-hipError_t trace_ao4(const Blas& b, bool fp16, bool watertight, const AoFrame* frames, uint64_t npix, uint32_t spp,
-                     float tmax, uint64_t seed, uint8_t* out, hipStream_t stream) {
+hipError_t trace_ao4(const Blas& b, bool fp16, bool watertight, const AoFrame* frames, const uint16_t* order,
+                     uint64_t npix, uint32_t spp, float tmax, uint64_t seed, uint8_t* out, hipStream_t stream) {
   const uint64_t n = npix * spp;
   if (n == 0) return hipSuccess;
   if (!b.quads32 || !b.quads16 || n > 0xFFFFFFFFull || (watertight && !b.tris_wt)) return hipErrorInvalidValue;
-  const AoRays src{frames, spp, tmax, seed};
+  const AoRays src{frames, order, spp, tmax, seed};
   if (watertight)
     return fp16 ? launch_any4<true, AoRays, WtTest>(b, src, n, out, stream)
                 : launch_any4<false, AoRays, WtTest>(b, src, n, out, stream);

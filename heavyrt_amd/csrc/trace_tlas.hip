@@ -74,11 +74,9 @@ __global__ __launch_bounds__(kBlock) void k_trace_two_level(const NodePair* __re
                                                             uint32_t* __restrict__ queue,
                                                             uint32_t* __restrict__ spill) {
   __shared__ uint32_t stack_mem[kShortStack * kBlock];
-  const int lane = threadIdx.x & (kWave - 1);
   const uint32_t lanes = gridDim.x * kBlock;
   const ShortStack<kShortStack, kBlock> stk{stack_mem + threadIdx.x, spill + blockIdx.x * kBlock + threadIdx.x,
                                              lanes};
-  const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (kWave - lane));
 
   bool active = false;
   uint64_t rid = 0;
@@ -135,7 +133,7 @@ __global__ __launch_bounds__(kBlock) void k_trace_two_level(const NodePair* __re
     const uint64_t idle = __ballot(!active);
     const uint32_t nidle = __popcll(idle);
     if (!pool.drained() && (nidle >= kRefillAt || nidle == kWave)) {
-      const uint64_t idx = pool.take<kClaim>(nidle, __popcll(idle & lt_mask), queue, n);
+      const uint64_t idx = pool.take<kClaim>(nidle, lanes_below(idle), queue, n);
       if (!active) {
         if (idx < n) {
           rid = idx;

@@ -312,6 +312,13 @@ uint32_t hrt_instance_count(const hrt_scene* scene);
 #define HRT_TLAS_ID 0xFFFFFFFFu
 hrt_status hrt_debug_dump(hrt_scene* scene, uint32_t geomId, int what, void* host_out, size_t bytes);
 
+/* Calibration kernel for issue-rate measurements (no reference counterpart):
+ * eight waves per SIMD run chains of FP32 min / max / multiply / select with
+ * no memory traffic, so under rocprofv3 its SQ_INSTS_VALU per GRBM_GUI_ACTIVE
+ * cycle is the VALU issue rate such code reaches; a traversal kernel's rate
+ * divided by it is that kernel's issue fraction.  Asynchronous on stream. */
+hrt_status hrt_issue_probe(hrt_ctx* ctx, void* stream);
+
 /* ---- Batches: many scenes built and traced in one HIP graph launch --------
  * No single counterpart in the reference, whose scenes are built and queried
  * one at a time (CrossRT.h:64-193).  A batch is S independent one-geometry
@@ -402,6 +409,11 @@ hrt_status hrt_server_wait(hrt_server* server, uint64_t requestId, double* laten
 /* out: completed requests, rays, steps, mean requests per step x 1000, peak
  * device bytes in flight, budget bytes. */
 hrt_status hrt_server_stats(hrt_server* server, uint64_t out[6]);
+/* out: rays launched so far on each kind of tree -- BVH2 node pairs, FP16
+ * BVH4 nodes, FP32 BVH4 nodes.  Nearest-hit requests walk the node pairs;
+ * any-hit requests walk the tree hrt_trace_any would (the any-hit tree of an
+ * HRT_LAYOUT_AUTO_FP16 scene).  No reference counterpart (serving). */
+hrt_status hrt_server_tree_rays(hrt_server* server, uint64_t out[3]);
 
 /* Thread-local description of the last failure in this thread. */
 const char* hrt_last_error(void);

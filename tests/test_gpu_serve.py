@@ -170,6 +170,11 @@ def test_device_requests_any_hit_tree_and_frozen_scenes(gpu_ctx):
         np.testing.assert_array_equal(got, solo_result(s, rays, anyh))
     st = srv.stats()
     assert st["completed"] == len(reqs)
+    # nearest-hit rays walked the node pairs, every any-hit ray the FP16 any-hit tree
+    tr = srv.tree_rays()
+    assert tr["bvh2"] == sum(r[1].shape[0] for r in reqs if not r[0])
+    assert tr["bvh4_fp16"] == sum(r[1].shape[0] for r in reqs if r[0])
+    assert tr["bvh4_fp32"] == 0
     srv.close()
     s.commit(layout=H.LAYOUT_BVH2)  # free again once the server is gone
     s.close()
