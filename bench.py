@@ -1045,7 +1045,9 @@ def attach_traffic(res: dict, meas: dict | None, reason: str) -> None:
         # rate in the same run: how close the kernel is to the instruction-issue bound
         roof["issue"] = dict(meas["issue"], source="rocprofv3 SQ_INSTS_VALU / GRBM_GUI_ACTIVE of this kernel "
                                                    "(mean of 3 launches) over the same ratio for hrt_issue_probe")
-        roof["limit"] = "valu_issue" if meas["issue"]["frac"] > roof["frac"] else "hbm"
+        top = max(meas["issue"]["frac"], roof["frac"])
+        roof["limit"] = ("latency" if top < 0.5 else
+                         "valu_issue" if meas["issue"]["frac"] > roof["frac"] else "hbm")
 
 
 # ---------------------------------------------------------------------------

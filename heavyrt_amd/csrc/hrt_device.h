@@ -67,11 +67,17 @@ struct __attribute__((aligned(16))) TriPre {
 // spill[(k - kLds) * spill_stride]).  Typical traversals stay well inside the
 // LDS part; the spill area makes the proven worst-case depth safe while the
 // LDS footprint stays small enough for full occupancy.
+// The LDS part is held as an LDS-address-space pointer: a pop then compiles
+// to a ds_read or a global load on its own branch, not to one flat load
+// through a pointer chosen between the two.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
 template <int kLds, int kStride>
 struct ShortStack {
-  uint32_t* lds;
+  lds_u32* lds;
   uint32_t* spill;
   uint32_t spill_stride;
+  __device__ ShortStack(uint32_t* lds_generic, uint32_t* spill_, uint32_t stride)
+      : lds((lds_u32*)(lds_generic)), spill(spill_), spill_stride(stride) {}
   __device__ inline void push(int& top, uint32_t v) const {
     if (top < kLds) lds[top * kStride] = v;
     else spill[size_t(top - kLds) * spill_stride] = v;
@@ -79,7 +85,23 @@ struct ShortStack {
   }
   __device__ inline uint32_t pop(int& top) const {
     --top;
-    return (top < kLds) ? lds[top * kStride] : spill[size_t(top - kLds) * spill_stride];
+    if (top < kLds) return lds[top * kStride];
+    return spill[size_t(top - kLds) * spill_stride];
+  }
+  // Pushes the first k of c3, c2, c1 counted from c1 (k = 3: c3, c2, c1;
+  // k = 1: c1), so c1 ends on top.  With room for three more entries in the
+  // LDS part, one store each and no spill test.
+  __device__ inline void push_top3(int& top, int k, uint32_t c1, uint32_t c2, uint32_t c3) const {
+    if (top + 3 <= kLds) {
+      if (k >= 1) lds[(top + k - 1) * kStride] = c1;
+      if (k >= 2) lds[(top + k - 2) * kStride] = c2;
+      if (k >= 3) lds[top * kStride] = c3;
+      top += k;
+    } else {
+      if (k >= 3) push(top, c3);
+      if (k >= 2) push(top, c2);
+      if (k >= 1) push(top, c1);
+    }
   }
 };
 constexpr int kShortStack = 16;  // LDS entries per lane in the persistent kernels

@@ -340,9 +340,9 @@ __global__ __launch_bounds__(kBlock4, 8) void k_trace4_any_persistent(const void
           uint32_t ch[4];
           visit_quad<kFp16, kFp16>(Src::kSegments ? lane_quads : quads, cur, o, inv, oi, fz, tt, tnear, h.t, tm,
                                    ch);
-          if (tm[3] < kMaxFloat) stk.push(top, ch[3]);
-          if (tm[2] < kMaxFloat) stk.push(top, ch[2]);
-          if (tm[1] < kMaxFloat) stk.push(top, ch[1]);
+          // sorted: the hit children are 0..k, the misses (kMaxFloat) last
+          stk.push_top3(top, int(tm[1] < kMaxFloat) + int(tm[2] < kMaxFloat) + int(tm[3] < kMaxFloat), ch[1],
+                        ch[2], ch[3]);
           if (tm[0] < kMaxFloat) cur = ch[0];
           else cur = (top > 0) ? stk.pop(top) : kInvalid;
         }
