@@ -358,7 +358,10 @@ __global__ __launch_bounds__(kB) void k_b_leaves(const float4* __restrict__ v, c
 // ---------------------------------------------------------------------------
 // trace
 // ---------------------------------------------------------------------------
-__global__ void k_b_reset(uint32_t* c) { *c = 0u; }
+// Clears the queue counters of the trace's XcdRayPool (one per XCD).
+__global__ void k_b_reset(uint32_t* c) {
+  if (threadIdx.x < kXcds) c[threadIdx.x * kXcdQueueStride] = 0u;
+}
 
 // Persistent nearest-hit over all scenes' rays.  kWaves waves per workgroup,
 // kLds stack entries per lane in LDS (the rest of the 32-entry worst case in
@@ -387,7 +390,7 @@ __global__ __launch_bounds__(64 * kWaves) void k_b_trace(const NodePair* __restr
   h.u = h.v = 0.f;
   int top = 0;
   uint32_t cur = kInvalid;
-  RayPool pool;  // ray indices claimed by this wave
+  XcdRayPool pool(blockIdx.x);  // ray indices claimed by this wave (its XCD's scenes first)
 
   while (true) {
     const uint64_t idle = __ballot(!active);
@@ -662,7 +665,7 @@ hipError_t enqueue_trace(hrt_batch* b, int cand, hipStream_t s) {
                     &b->hits};
     return hipLaunchKernel(c.fn, dim3(b->grid[cand]), dim3(threads), args, 0, s);
   }
-  k_b_reset<<<1, 1, 0, s>>>(b->queue);
+  k_b_reset<<<1, kXcds, 0, s>>>(b->queue);
   void* args[] = {&b->pairs, &b->tris, const_cast<SceneOffsets*>(&off), &b->rays, const_cast<uint32_t*>(&R),
                   &b->hits, &b->queue, &b->spill};
   return hipLaunchKernel(c.fn, dim3(b->grid[cand]), dim3(threads), args, 0, s);

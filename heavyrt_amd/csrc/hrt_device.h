@@ -113,6 +113,17 @@ constexpr int kShortStack = 16;  // LDS entries per lane in the persistent kerne
 // that every wave of the chip hits at each refill limits the whole kernel).
 // Every member is wave-uniform; take() must be called by all lanes of the
 // wave together.
+// The logical block a workgroup works on, so that each XCD's L2 serves one
+// contiguous stretch of the rays.  The dispatcher hands workgroups to the
+// eight XCDs in turn (workgroup b to XCD b mod 8); XCD x gets the x-th
+// eighth of the logical blocks, in order.  A permutation of [0, n).
+constexpr uint32_t kXcds = 8;
+__device__ inline uint32_t xcd_block(uint32_t b, uint32_t n) {
+  const uint32_t per = n / kXcds, rem = n % kXcds;
+  const uint32_t x = b % kXcds, i = b / kXcds;
+  return x * per + (x < rem ? x : rem) + i;
+}
+
 // Set bits of m in the lanes below this one (v_mbcnt_lo / v_mbcnt_hi).
 __device__ inline uint32_t lanes_below(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
@@ -145,6 +156,53 @@ struct RayPool {
     } else {
       next += (avail < nidle) ? avail : nidle;
     }
+    return idx < n ? idx : n;
+  }
+  __device__ inline bool drained() const { return exhausted && next >= end; }
+};
+
+// RayPool with one queue per XCD: [0, n) is cut into kXcds contiguous
+// eighths, and a wave claims from the eighth of its own XCD (workgroup b runs
+// on XCD b mod kXcds) until that is used up, then from the next eighths in
+// turn, so each XCD's L2 mostly serves one stretch of the rays.  The queue
+// holds the eighths' counters kXcdQueueStride words apart (all zero before
+// the launch).  Every member is wave-uniform; take() must be called by all
+// lanes of the wave together.
+constexpr uint32_t kXcdQueueStride = 8;
+constexpr uint32_t kXcdQueueWords = kXcds * kXcdQueueStride;
+struct XcdRayPool {
+  uint64_t next = 0, end = 0;  // indices claimed by the wave, not yet given out
+  uint32_t part = 0;           // the eighth being claimed from
+  uint32_t tried = 0;          // eighths found used up, from the wave's own one on
+  bool exhausted = false;      // every eighth is used up
+
+  __device__ explicit XcdRayPool(uint32_t block) : part(block % kXcds) {}
+
+  template <uint32_t kClaim>
+  __device__ inline uint64_t take(uint32_t nidle, uint32_t rank, uint32_t* queue, uint64_t n) {
+    const uint64_t avail = end - next;
+    uint64_t idx = (rank < avail) ? next + rank : n;
+    if (avail >= nidle || exhausted) {
+      next += (avail < nidle) ? avail : nidle;
+      return idx < n ? idx : n;
+    }
+    next = end;  // the first avail lanes took the rest
+    while (tried < kXcds) {
+      const uint32_t p = (part + tried) % kXcds;
+      const uint64_t lo = n * p / kXcds, hi = n * (p + 1) / kXcds;
+      uint32_t b = 0;
+      if ((threadIdx.x & 63) == 0) b = atomicAdd(queue + p * kXcdQueueStride, kClaim);
+      const uint64_t nb = lo + uint64_t(__shfl(b, 0));
+      if (nb < hi) {
+        end = (nb + kClaim < hi) ? nb + kClaim : hi;
+        if (rank >= avail && nb + (rank - avail) < end) idx = nb + (rank - avail);
+        next = nb + (nidle - avail);
+        if (next > end) next = end;
+        return idx < n ? idx : n;
+      }
+      tried++;
+    }
+    exhausted = true;
     return idx < n ? idx : n;
   }
   __device__ inline bool drained() const { return exhausted && next >= end; }

@@ -50,7 +50,7 @@ __device__ inline uint32_t ao_bin_fast(const AoFrame& f, uint64_t k, uint64_t se
   const float cth = __builtin_amdgcn_sqrtf(1.0f - r2), sth = __builtin_amdgcn_sqrtf(r2);
   const float3 n = make_float3(f.n.x, f.n.y, f.n.z);
   const float sgn = copysignf(1.0f, n.z);
-  const float a = -__builtin_amdgcn_rcpf(sgn + n.z);
+  const float a = f.n.w;
   const float b = n.x * n.y * a;
   const float lx = sth * cphi, ly = sth * sphi;
   return ao_dir_bin(make_float3((1.0f + sgn * n.x * n.x * a) * lx + b * ly + n.x * cth,

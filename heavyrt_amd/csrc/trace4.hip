@@ -145,9 +145,14 @@ __global__ __launch_bounds__(kBlock4) void k_trace4_simple(const void* __restric
                                                            uint32_t* __restrict__ spill,
                                                            const uint32_t* const* __restrict__ order_slot) {
   __shared__ uint32_t stack_mem[kSimpleLds4 * kBlock4];
-  const uint64_t lane_id = uint64_t(blockIdx.x) * kBlock4 + threadIdx.x;
-  if (lane_id >= n) return;
   const uint32_t* __restrict__ order = order_slot ? *order_slot : nullptr;
+  // a sorted batch: each XCD takes one contiguous eighth of the sorted order,
+  // so its L2 serves rays of similar direction and origin (5 % faster on 4 M
+  // incoherent rays); rays in their own order keep the dispatch order, whose
+  // interleaving balances the XCDs over uneven screen regions
+  const uint64_t lane_id =
+      uint64_t(order ? xcd_block(blockIdx.x, gridDim.x) : blockIdx.x) * kBlock4 + threadIdx.x;
+  if (lane_id >= n) return;
   // thread k traces ray order[k] when the batch was sorted into coherent waves
   const uint64_t i = order ? uint64_t(order[lane_id]) : lane_id;
   if (i >= n) return;
@@ -279,7 +284,7 @@ __global__ __launch_bounds__(kBlock4, 8) void k_trace4_any_persistent(const void
   int top = 0;
   uint32_t cur = kInvalid;
   uint32_t leaf = 0, leaf2 = 0;  // parked leaves (leaf2 only while leaf is held)
-  RayPool pool;  // ray indices claimed by this wave
+  RayPool pool;  // ray indices claimed by this wave (one queue: per-XCD queues measured 3 % slower here)
   // the tree and answer buffer of this lane's ray (a per-request source);
   // a launch-wide source keeps the kernel arguments
   const void* lane_quads = quads;
