@@ -106,13 +106,6 @@ struct ShortStack {
 };
 constexpr int kShortStack = 16;  // LDS entries per lane in the persistent kernels
 
-// Ray indices for the idle lanes of a persistent wave.  The wave claims
-// kClaim consecutive indices from the global queue with one atomic and hands
-// them out to its own idle lanes at later refills, so the single global
-// counter is hit once per kClaim rays instead of once per refill (a counter
-// that every wave of the chip hits at each refill limits the whole kernel).
-// Every member is wave-uniform; take() must be called by all lanes of the
-// wave together.
 // The logical block a workgroup works on, so that each XCD's L2 serves one
 // contiguous stretch of the rays.  The dispatcher hands workgroups to the
 // eight XCDs in turn (workgroup b to XCD b mod 8); XCD x gets the x-th
@@ -129,6 +122,13 @@ __device__ inline uint32_t lanes_below(uint64_t m) {
   return __builtin_amdgcn_mbcnt_hi(uint32_t(m >> 32), __builtin_amdgcn_mbcnt_lo(uint32_t(m), 0u));
 }
 
+// Ray indices for the idle lanes of a persistent wave.  The wave claims
+// kClaim consecutive indices from the global queue with one atomic and hands
+// them out to its own idle lanes at later refills, so the single global
+// counter is hit once per kClaim rays instead of once per refill (a counter
+// that every wave of the chip hits at each refill limits the whole kernel).
+// Every member is wave-uniform; take() must be called by all lanes of the
+// wave together.
 struct RayPool {
   uint64_t next = 0, end = 0;  // indices claimed by the wave, not yet given out
   bool exhausted = false;      // the global queue has been run past n
@@ -476,6 +476,15 @@ __device__ inline bool tri_test(const Tri& k, float3 o, float3 d, float tnear, c
     return true;
   }
   return false;
+}
+
+// An any-hit query's test: whether the triangle is hit in (tnear, tfar).  The
+// query ends at its first hit, so this equals tri_test / tri_test_lex with
+// the incoming distance tfar, without keeping t, u, v or the id.
+template <class Tri>
+__device__ inline bool tri_hit_before(const Tri& k, float3 o, float3 d, float tnear, const TriPre& tri, float tfar) {
+  float t, u, v;
+  return k.eval(o, d, tnear, tri, t, u, v) && t < tfar;
 }
 
 template <class Tri>

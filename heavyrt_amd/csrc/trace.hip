@@ -235,7 +235,10 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace_persistent(const NodePair
         if (active && leaf != 0) {
           const uint32_t s = leaf_start(leaf), c = leaf_count(leaf);
           for (uint32_t k = s; k < s + c; k++) {
-            if (tri_test(tt, o, d, tnear, tris[k], h)) break;  // any-hit: first hit decides
+            if (tri_hit_before(tt, o, d, tnear, tris[k], h.t)) {  // any-hit: the first hit decides
+              h.prim = 0u;
+              break;
+            }
           }
           leaf = 0;
         }

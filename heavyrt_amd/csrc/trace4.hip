@@ -357,12 +357,12 @@ __global__ __launch_bounds__(kBlock4, 8) void k_trace4_any_persistent(const void
         const TriPre* T = Src::kSegments ? lane_tris : tris;
         const uint32_t s = leaf_start(leaf), c = leaf_count(leaf);
         bool hit = false;
-        for (uint32_t k = s; k < s + c && !hit; k++) hit = tri_test_lex(tt, o, d, tnear, T[k], h);
+        for (uint32_t k = s; k < s + c && !hit; k++) hit = tri_hit_before(tt, o, d, tnear, T[k], h.t);
         if (!hit && leaf2 != 0) {
           const uint32_t s2 = leaf_start(leaf2), c2 = leaf_count(leaf2);
-          for (uint32_t k = s2; k < s2 + c2; k++)
-            if (tri_test_lex(tt, o, d, tnear, T[k], h)) break;
+          for (uint32_t k = s2; k < s2 + c2 && !hit; k++) hit = tri_hit_before(tt, o, d, tnear, T[k], h.t);
         }
+        if (hit) h.prim = 0u;  // any id but kInvalid: the query is answered
         leaf = 0;
         leaf2 = 0;
       }
