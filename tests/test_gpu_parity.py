@@ -272,3 +272,11 @@ def test_query_one_matches_batch(gpu_ctx):
         if prim != INVALID:
             assert inst == 0 and geom == 0
     scene.close()
+
+
+def test_issue_probe_runs(gpu_ctx):
+    """The issue-rate calibration kernel launches and completes on the context's device."""
+    import torch
+
+    gpu_ctx.issue_probe(torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
