@@ -88,19 +88,23 @@ struct ShortStack {
     if (top < kLds) return lds[top * kStride];
     return spill[size_t(top - kLds) * spill_stride];
   }
-  // Pushes the first k of c3, c2, c1 counted from c1 (k = 3: c3, c2, c1;
-  // k = 1: c1), so c1 ends on top.  With room for three more entries in the
-  // LDS part, one store each and no spill test.
-  __device__ inline void push_top3(int& top, int k, uint32_t c1, uint32_t c2, uint32_t c3) const {
+  // Pushes c3 if p3, then c2 if p2, then c1 if p1.  With room for three more
+  // entries in the LDS part, each value is stored at the top slot whether or
+  // not it is pushed (a slot at or above top holds nothing), so there is no
+  // branch.
+  __device__ inline void push_masked3(int& top, bool p3, uint32_t c3, bool p2, uint32_t c2, bool p1,
+                                      uint32_t c1) const {
     if (top + 3 <= kLds) {
-      if (k >= 1) lds[(top + k - 1) * kStride] = c1;
-      if (k >= 2) lds[(top + k - 2) * kStride] = c2;
-      if (k >= 3) lds[top * kStride] = c3;
-      top += k;
+      lds[top * kStride] = c3;
+      top += int(p3);
+      lds[top * kStride] = c2;
+      top += int(p2);
+      lds[top * kStride] = c1;
+      top += int(p1);
     } else {
-      if (k >= 3) push(top, c3);
-      if (k >= 2) push(top, c2);
-      if (k >= 1) push(top, c1);
+      if (p3) push(top, c3);
+      if (p2) push(top, c2);
+      if (p1) push(top, c1);
     }
   }
 };
