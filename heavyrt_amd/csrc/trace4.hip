@@ -97,10 +97,16 @@ __device__ inline void quad_boxes(const void* __restrict__ quads, uint32_t q, fl
 
 // Whether a child with box distances t and reference c is entered.  An unused
 // slot (box +1e9..-1e9, empty leaf 0x80000000) would pass the min/max slab
-// test as the box [-1e9, 1e9]; it holds no triangle, so it is never entered.
+// test as the box [-1e9, 1e9]; kSkipEmpty keeps it out.  slab_fused picks
+// each axis's near and far plane by the sign of the ray's direction, so it
+// finds the inverted box empty by itself; a caller that mostly uses it may
+// leave kSkipEmpty off, and a lane on the min/max form then enters the empty
+// leaf, which tests no triangle.
+template <bool kSkipEmpty = true>
 __device__ inline bool child_hit(float2 t, uint32_t c, float tnear, float tcull) {
-  return (t.x <= t.y) && (t.y >= tnear) && (t.x <= tcull) && c != kEmptyLeaf;
+  return (t.x <= t.y) && (t.y >= tnear) && (t.x <= tcull) && (!kSkipEmpty || c != kEmptyLeaf);
 }
+
 // quad_boxes, then the entry distances (kMaxFloat for a miss) sorted
 // ascending with their references, as the reference orders a node's children.
 template <bool kFp16, bool kFused, class Tri>
@@ -359,7 +365,7 @@ __global__ __launch_bounds__(kBlock4, 8) void k_trace4_any_persistent(const void
           quad_boxes<kFp16, kFp16>(Src::kSegments ? lane_quads : quads, cur, o, inv, oi, fz, tt, bt, ch);
           bool hk[4];
 #pragma unroll
-          for (int k = 0; k < 4; k++) hk[k] = child_hit(bt[k], ch[k], tnear, h.t);
+          for (int k = 0; k < 4; k++) hk[k] = child_hit<!(kFp16 && !Tri::kRobustBoxes)>(bt[k], ch[k], tnear, h.t);
           stk.push_masked3(top, hk[3] && (hk[0] || hk[1] || hk[2]), ch[3], hk[2] && (hk[0] || hk[1]), ch[2],
                            hk[1] && hk[0], ch[1]);
           if (hk[0]) cur = ch[0];
