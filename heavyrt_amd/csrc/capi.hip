@@ -894,9 +894,10 @@ hrt_status hrt_eye_render(hrt_scene* s, const float projInvColMajor16[16], const
   if ((e = hipMallocAsync(&buf, 40 * n, hs)) != hipSuccess) return fail_hip(e, "eye render buffers");
   float* rays = static_cast<float*>(buf);
   uint32_t* hits = reinterpret_cast<uint32_t*>(static_cast<char*>(buf) + 32 * n);
-  e = hrt::eye_make_rays(cam, width, height, reinterpret_cast<hrt::Ray*>(rays), nullptr, hs);
+  // row order: the image is all this call owes, and rows trace fastest
+  e = hrt::eye_make_rays(cam, width, height, reinterpret_cast<hrt::Ray*>(rays), nullptr, hs, true);
   st = e == hipSuccess ? trace_common(s, rays, n, hits, HRT_HIT_COMPACT, 0, nullptr, stream) : HRT_OK;
-  if (e == hipSuccess && st == HRT_OK) e = hrt::eye_colors(hits, width, height, pal, outColor, hs);
+  if (e == hipSuccess && st == HRT_OK) e = hrt::eye_colors(hits, width, height, pal, outColor, hs, true);
   const hipError_t fe = hipFreeAsync(buf, hs);
   if (st != HRT_OK) return st;
   if (e == hipSuccess) e = fe;

@@ -243,11 +243,13 @@ struct EyePalette {
   uint32_t c[kEyePaletteMax];
   uint32_t n;
 };
+// Eye rays in the reference's superblock order, or in row order (rows).
 hipError_t eye_make_rays(const EyeCamera& cam, uint32_t width, uint32_t height, Ray* out, uint32_t* packed_xy,
-                         hipStream_t stream);
-// Pixel colours from compact hits of those rays (row-major image).
+                         hipStream_t stream, bool rows = false);
+// Pixel colours from compact hits of those rays (row-major image); rows as
+// the rays were made.
 hipError_t eye_colors(const void* hits_compact, uint32_t width, uint32_t height, const EyePalette& pal,
-                      uint32_t* out_color, hipStream_t stream);
+                      uint32_t* out_color, hipStream_t stream, bool rows = false);
 // The 02_rtao frame's colours from compact primary hits (superblock order)
 // and one occlusion byte per AO ray (spp per pixel).
 hipError_t ao_colors(const void* hits_compact, const uint8_t* occluded, uint32_t width, uint32_t height,

@@ -281,7 +281,9 @@ hrt_status hrt_eye_make_rays(hrt_ctx* ctx, const float projInvColMajor16[16], co
 
 /* The 01_eye_rays frame on the device (EyeRayCaster::Render /
  * CastRaySingleBlock, eye_ray_host.cpp:329-345, with kernel_RayTrace,
- * eye_ray.cpp:50-75): eye rays as above, nearest hits against the committed
+ * eye_ray.cpp:50-75): eye rays as above (made and traced in row order
+ * rather than the superblock order, which only the image's speed sees),
+ * nearest hits against the committed
  * scene, and outColor[y * width + x] = 0 on a miss, else
  * palette[primId % paletteSize] (paletteSize 1..64, host array).  outColor
  * is a device buffer of width * height words; asynchronous on `stream`. */
