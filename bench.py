@@ -407,18 +407,24 @@ def bench_eye(args, cpu) -> dict:
     torch.cuda.synchronize()
     rays = d_rays.cpu().numpy()
     # the rays in the reference's superblock order, checked against the
-    # oracle below, are traced in row order (ray p is pixel p): a wave then
-    # covers 64 pixels of one row, which traces fastest of the orders tried
+    # oracle below, are traced in the XCD-stripe order hrt_eye_render uses
+    # (each XCD's workgroups get 4-row stripes of the screen), which traces
+    # fastest of the orders tried
     sb = W.superblock_order(width, height)
-    d_rows = torch.empty_like(d_rays)
-    d_rows[torch.from_numpy(sb).cuda()] = d_rays
+    xs = W.xcd_stripe_order(width, height)
+    by_pixel = torch.empty_like(d_rays)
+    by_pixel[torch.from_numpy(sb).cuda()] = d_rays
+    d_ord = by_pixel[torch.from_numpy(xs).cuda()].contiguous()
+    del by_pixel
     d_hits = torch.empty((n, 2), dtype=torch.int32, device="cuda")
     for _ in range(args.warmup):
-        sc.trace_nearest(d_rows, n, d_hits, H.HIT_COMPACT, stream=sptr)
-    ms = timed(stream, lambda: sc.trace_nearest(d_rows, n, d_hits, H.HIT_COMPACT, stream=sptr), args.steps)
-    hits = d_hits.cpu().numpy().view(np.uint32)[sb]  # back to the superblock order of `rays`
+        sc.trace_nearest(d_ord, n, d_hits, H.HIT_COMPACT, stream=sptr)
+    ms = timed(stream, lambda: sc.trace_nearest(d_ord, n, d_hits, H.HIT_COMPACT, stream=sptr), args.steps)
+    hits_by_pixel = np.empty((n, 2), dtype=np.uint32)
+    hits_by_pixel[xs] = d_hits.cpu().numpy().view(np.uint32)
+    hits = hits_by_pixel[sb]  # in the superblock order of `rays`
     d_st = torch.empty((n, 2), dtype=torch.int32, device="cuda")
-    sc.trace_nearest(d_rows, n, torch.empty_like(d_hits), H.HIT_COMPACT, stream=sptr, stats=d_st)
+    sc.trace_nearest(d_ord, n, torch.empty_like(d_hits), H.HIT_COMPACT, stream=sptr, stats=d_st)
     torch.cuda.synchronize()
     st = d_st.cpu().numpy().view(np.uint32).astype(np.int64)
     mean = float(ms.mean())
@@ -430,7 +436,7 @@ def bench_eye(args, cpu) -> dict:
     blb = float(np.sum(40 + 64 * st[:, 0] + 60 * st[:, 1]))
     res = line(round(n / (mean * 1e-3) / 1e6, 2), round(mean, 4), args.steps, args.warmup,
                {"workload": "eye: 512x512 pinhole rays (made on the device in the reference's 8x8-superblock "
-                            "order, traced in row order), nearest hit, "
+                            "order, traced in XCD stripes), nearest hit, "
                             "LBVH2 of an icosphere (81,920 tris) + ground quad", "triangles": int(f.shape[0]),
                 "rays": n},
                {"roofline": roofline("nearest-hit traversal (k_trace_simple, BVH2, reference order)", mean, alg, blb),
@@ -506,7 +512,10 @@ def bench_precision(args, cpu) -> dict:
     d_rays = torch.from_numpy(rays).cuda()
     res, timing = {}, {}
     out = torch.empty((n, 2), dtype=torch.int32, device="cuda")
-    for name, lay in (("fp32", H.LAYOUT_BVH4), ("fp16", H.LAYOUT_BVH4_FP16)):
+    # the reference tree's BVH4 (FP32, FP16 boxes), then the clustered second
+    # tree's (the timed layout last: the counters below walk it)
+    for name, lay in (("lbvh_fp32", H.LAYOUT_BVH4), ("lbvh_fp16", H.LAYOUT_BVH4_FP16), ("fp32", H.LAYOUT_PLOC),
+                      ("fp16", H.LAYOUT_PLOC_FP16)):
         sc.commit(layout=lay)
         for _ in range(args.warmup):
             sc.trace_nearest(d_rays, n, out, H.HIT_COMPACT, stream=sptr)
@@ -521,16 +530,21 @@ def bench_precision(args, cpu) -> dict:
                      stream=sptr, stats=st)
     torch.cuda.synchronize()
     stc = st.cpu().numpy().view(np.uint32).astype(np.int64)
-    sizes = sc.dump(0, H.DUMP_SIZES)
-    same = bool(np.array_equal(res["fp32"], res["fp16"]))
-    mism = int(np.sum(np.any(res["fp32"] != res["fp16"], axis=1)))
+    clustered = True
+    try:
+        sizes = sc.dump(0, H.DUMP_SIZES | H.DUMP_ANY_TREE)
+    except H.HrtError:  # no second tree: the PLOC layouts walk the reference tree
+        sizes, clustered = sc.dump(0, H.DUMP_SIZES), False
+    same = all(bool(np.array_equal(res["fp16"], res[k])) for k in ("fp32", "lbvh_fp32", "lbvh_fp16"))
+    mism = int(np.sum(np.any(res["fp32"] != res["fp16"], axis=1) | np.any(res["lbvh_fp32"] != res["fp16"], axis=1) |
+                      np.any(res["lbvh_fp16"] != res["fp16"], axis=1)))
     prim = res["fp16"][:, 1]
     hit = prim != INVALID
     t = res["fp16"][:, 0].view(np.float32)
     t64 = t_fp64(v, f, rays, prim)
     # the watertight test on the same tree and rays
     sc.set_triangle_test(H.TRI_WATERTIGHT)
-    sc.commit(layout=H.LAYOUT_BVH4_FP16)
+    sc.commit(layout=H.LAYOUT_PLOC_FP16)
     wt_ms = timed(stream, lambda: sc.trace_nearest(d_rays, n, out, H.HIT_COMPACT, stream=sptr), 3)
     wt = out.cpu().numpy().view(np.uint32).copy()
     wprim = wt[:, 1]
@@ -547,15 +561,23 @@ def bench_precision(args, cpu) -> dict:
     nq = int(sizes[4])
     alg = 40.0 * n + 64.0 * nq + 48.0 * n
     blb = float(np.mean(40 + 64 * stc[:, 0] + 60 * stc[:, 1])) * n
+
+    def rate(k):
+        ms = float(timing[k].mean())
+        return {"ms": round(ms, 4), "mrays_s": round(n / (ms * 1e-3) / 1e6, 2)}
+
     out_res = line(round(n / (ms16 * 1e-3) / 1e6, 2), round(ms16, 4), len(timing["fp16"]), args.warmup,
                    {"workload": "precision: 4,194,304-tri random soup, 4,194,304 incoherent nearest-hit rays, BVH4 "
-                                "with FP16 boxes rounded outward (value) vs FP32 boxes", "triangles": n, "rays": n},
-                   {"roofline": roofline("nearest-hit traversal (BVH4, FP16 nodes)", ms16, alg, blb),
-                    "fp32_nodes": {"ms": round(ms32, 4), "mrays_s": round(n / (ms32 * 1e-3) / 1e6, 2)},
-                    "fp16_nodes": {"ms": round(ms16, 4), "mrays_s": round(n / (ms16 * 1e-3) / 1e6, 2)},
+                                "with FP16 boxes rounded outward (value) vs FP32 boxes, on the clustered (PLOC) "
+                                "tree; the reference LBVH's BVH4 beside", "triangles": n, "rays": n},
+                   {"roofline": roofline("nearest-hit traversal (BVH4, FP16 nodes of the clustered tree)", ms16, alg,
+                                         blb),
+                    "fp32_nodes": rate("fp32"), "fp16_nodes": rate("fp16"),
+                    "reference_tree": {"fp32_nodes": rate("lbvh_fp32"), "fp16_nodes": rate("lbvh_fp16")},
+                    "clustered_tree": clustered,
                     "visits_per_ray": round(float(stc[:, 0].mean()), 2),
                     "tris_per_ray": round(float(stc[:, 1].mean()), 2),
-                    "hits_bit_identical_fp16_fp32": same, "mismatching_rays": mism,
+                    "hits_bit_identical_fp16_fp32_both_trees": same, "mismatching_rays": mism,
                     "hit_fraction": round(float(hit.mean()), 4),
                     "t_rel_error_vs_fp64": rel_stats(t, t64, hit),
                     "watertight": watertight,
@@ -942,8 +964,9 @@ def counter_child(workloads) -> int:
             sc.commit()
             made = torch.empty((512 * 512, 8), dtype=torch.float32, device="cuda")
             ctx.eye_make_rays(cam[0], cam[1], 512, 512, made, stream=sp)
-            rays = torch.empty_like(made)  # in row order, as bench_eye traces them
-            rays[torch.from_numpy(W.superblock_order(512, 512)).cuda()] = made
+            by_pixel = torch.empty_like(made)  # in XCD stripes, as bench_eye traces them
+            by_pixel[torch.from_numpy(W.superblock_order(512, 512)).cuda()] = made
+            rays = by_pixel[torch.from_numpy(W.xcd_stripe_order(512, 512)).cuda()].contiguous()
             hits = torch.empty((512 * 512, 2), dtype=torch.int32, device="cuda")
             for _ in range(3):
                 sc.trace_nearest(rays, 512 * 512, hits, H.HIT_COMPACT, stream=sp)
@@ -953,7 +976,7 @@ def counter_child(workloads) -> int:
             rays = torch.from_numpy(W.incoherent_rays(n, stream=W.STREAM_RAYS)).cuda()
             sc = H.Scene(ctx)
             sc.add_instance(sc.add_triangles(v, f))
-            sc.commit(layout=H.LAYOUT_BVH4_FP16)
+            sc.commit(layout=H.LAYOUT_PLOC_FP16)
             hits = torch.empty((n, 2), dtype=torch.int32, device="cuda")
             for _ in range(3):
                 sc.trace_nearest(rays, n, hits, H.HIT_COMPACT, stream=sp)

@@ -29,6 +29,8 @@ LAYOUT_BVH4 = 1
 LAYOUT_BVH4_FP16 = 2
 LAYOUT_AUTO = 3
 LAYOUT_AUTO_FP16 = 4
+LAYOUT_PLOC = 5  # nearest and any-hit on the FP32 BVH4 of the clustered second tree
+LAYOUT_PLOC_FP16 = 6  # the same with FP16 boxes
 TRI_MT = 0
 TRI_WATERTIGHT = 1
 
@@ -44,7 +46,7 @@ DUMP_SCENE_BOX = 8
 DUMP_QUADS32 = 9
 DUMP_QUADS16 = 10
 DUMP_PAIRS = 11
-DUMP_ANY_TREE = 0x100  # OR'ed into a dump kind: the any-hit tree of HRT_LAYOUT_AUTO_FP16
+DUMP_ANY_TREE = 0x100  # OR'ed into a dump kind: the second tree of HRT_LAYOUT_AUTO_FP16 / the PLOC layouts
 # hrt_debug_dump geometry id that selects the top-level tree of an instanced scene
 TLAS_ID = 0xFFFFFFFF
 

@@ -270,15 +270,17 @@ hrt_status trace_common(hrt_scene* s, const float* rays, size_t n, void* out, in
   if (e == hipSuccess) {
     const int lay = s->layout;
     const bool auto16 = lay == HRT_LAYOUT_AUTO_FP16;
+    const bool ploc = lay == HRT_LAYOUT_PLOC || lay == HRT_LAYOUT_PLOC_FP16;
+    const bool fp16 = lay == HRT_LAYOUT_BVH4_FP16 || auto16 || lay == HRT_LAYOUT_PLOC_FP16;
     // b is null for a two-level scene: every branch below it reads b
     if (s->two_level)
       e = hrt::trace_two_level(s->tlas, s->inst_dev ? s->inst_dev + s->instances.size() : nullptr, a, hs);
     else if (lay == HRT_LAYOUT_BVH2 || ((lay == HRT_LAYOUT_AUTO || auto16) && !anyhit))
       e = hrt::trace_blas(*b, a, hs);
-    else if (auto16 && b->any_tree)
-      e = hrt::trace_blas4(*b->any_tree, a, true, hs);  // any-hit on the any-hit tree
+    else if ((auto16 || ploc) && b->any_tree)
+      e = hrt::trace_blas4(*b->any_tree, a, fp16, hs);  // on the second (clustered) tree
     else
-      e = hrt::trace_blas4(*b, a, lay == HRT_LAYOUT_BVH4_FP16 || auto16, hs);
+      e = hrt::trace_blas4(*b, a, fp16, hs);
   }
   if (e != hipSuccess) return fail_hip(e, "trace");
   return HRT_OK;
@@ -308,17 +310,18 @@ TreeRef scene_any_hit_tree(const hrt_scene* s) {
   const Blas& b = s->geoms[s->instances[0].geom];
   const int lay = s->layout;
   const bool auto16 = lay == HRT_LAYOUT_AUTO_FP16;
+  const bool ploc = lay == HRT_LAYOUT_PLOC || lay == HRT_LAYOUT_PLOC_FP16;
+  const bool fp16 = lay == HRT_LAYOUT_BVH4_FP16 || auto16 || lay == HRT_LAYOUT_PLOC_FP16;
   TreeRef t;
   t.nodes = b.pairs;
   t.tris = b.tris;
   t.kind = 0;
   if (lay == HRT_LAYOUT_BVH2) return t;
-  if (auto16 && b.any_tree) {
-    if (b.any_tree->quads16) t = TreeRef{b.any_tree->quads16, b.any_tree->tris, 1};
-  } else if (lay == HRT_LAYOUT_BVH4_FP16 || auto16) {
-    if (b.quads16) t = TreeRef{b.quads16, b.tris, 1};
-  } else if (b.quads32) {
-    t = TreeRef{b.quads32, b.tris, 2};
+  const hrt::Blas& q = ((auto16 || ploc) && b.any_tree) ? *b.any_tree : b;
+  if (fp16) {
+    if (q.quads16) t = TreeRef{q.quads16, q.tris, 1};
+  } else if (q.quads32) {
+    t = TreeRef{q.quads32, q.tris, 2};
   }
   return t;
 }
@@ -560,7 +563,7 @@ hrt_status hrt_commit(hrt_scene* s, int quality, int layout) {
   (void)quality;
   if (!s) return fail(HRT_ERR_INVALID_ARG, "scene is null");
   if (hrt_status st = check_mutable(s)) return st;
-  if (layout < HRT_LAYOUT_BVH2 || layout > HRT_LAYOUT_AUTO_FP16)
+  if (layout < HRT_LAYOUT_BVH2 || layout > HRT_LAYOUT_PLOC_FP16)
     return fail(HRT_ERR_INVALID_ARG, "unknown layout");
   s->committed = false;
   free_tlas(s);
@@ -581,16 +584,17 @@ hrt_status hrt_commit(hrt_scene* s, int quality, int layout) {
     for (const Instance& in : s->instances) {
       hrt::Blas& b = s->geoms[in.geom];
       const bool auto16 = layout == HRT_LAYOUT_AUTO_FP16;
-      if (layout == HRT_LAYOUT_BVH4_FP16 || auto16) {
+      const bool second = auto16 || layout == HRT_LAYOUT_PLOC || layout == HRT_LAYOUT_PLOC_FP16;
+      if (layout == HRT_LAYOUT_BVH4_FP16 || auto16 || layout == HRT_LAYOUT_PLOC_FP16) {
         // FP16 boxes are only conservative inside the half range (cbvh_fat.cpp:447-448
         // maps every value beyond +-65504 to +-65504)
         for (int k = 0; k < 3; k++)
           if (!(b.box[k] >= -65504.0f && b.box[4 + k] <= 65504.0f))
             return fail(HRT_ERR_UNSUPPORTED, "FP16 nodes need every coordinate within +-65504");
       }
-      if (auto16) {
-        // any-hit tree on a cubic Morton grid; the reference tree's BVH4 only
-        // when the cubic one cannot be stored
+      if (second) {
+        // the second tree on a cubic Morton grid; the reference tree's BVH4
+        // only when the cubic one cannot be stored
         if ((e = hrt::blas_build_any_tree(b, s->ctx->stream)) != hipSuccess) return fail_hip(e, "any-hit tree");
         if (!b.any_tree && (e = hrt::blas_build_bvh4(b, s->ctx->stream)) != hipSuccess)
           return fail_hip(e, "BVH4 collapse");
@@ -805,8 +809,9 @@ hipError_t rtao_pass(hrt_scene* s, const hrt::Blas* b, uint32_t inst, const hrt:
     const hipError_t fe = hipFreeAsync(rays, hs);
     return e == hipSuccess ? fe : e;
   }
-  const hrt::Blas& tree = (auto16 && b->any_tree) ? *b->any_tree : *b;
-  const bool fp16 = lay == HRT_LAYOUT_BVH4_FP16 || auto16;
+  const bool ploc = lay == HRT_LAYOUT_PLOC || lay == HRT_LAYOUT_PLOC_FP16;
+  const hrt::Blas& tree = ((auto16 || ploc) && b->any_tree) ? *b->any_tree : *b;
+  const bool fp16 = lay == HRT_LAYOUT_BVH4_FP16 || auto16 || lay == HRT_LAYOUT_PLOC_FP16;
   // the frames (32 B per pixel), then the samples' direction order (2 B each)
   void* frames = nullptr;
   if ((e = hipMallocAsync(&frames, 32ull * npix + 2ull * n, hs)) != hipSuccess) return e;
@@ -894,7 +899,7 @@ hrt_status hrt_eye_render(hrt_scene* s, const float projInvColMajor16[16], const
   if ((e = hipMallocAsync(&buf, 40 * n, hs)) != hipSuccess) return fail_hip(e, "eye render buffers");
   float* rays = static_cast<float*>(buf);
   uint32_t* hits = reinterpret_cast<uint32_t*>(static_cast<char*>(buf) + 32 * n);
-  // row order: the image is all this call owes, and rows trace fastest
+  // XCD stripes: the image is all this call owes, and stripes trace fastest
   e = hrt::eye_make_rays(cam, width, height, reinterpret_cast<hrt::Ray*>(rays), nullptr, hs, true);
   st = e == hipSuccess ? trace_common(s, rays, n, hits, HRT_HIT_COMPACT, 0, nullptr, stream) : HRT_OK;
   if (e == hipSuccess && st == HRT_OK) e = hrt::eye_colors(hits, width, height, pal, outColor, hs, true);
@@ -1305,7 +1310,7 @@ hrt_status hrt_debug_dump(hrt_scene* s, uint32_t geomId, int what, void* host_ou
   const bool any_tree = (what & HRT_DUMP_ANY_TREE) != 0;
   what &= ~HRT_DUMP_ANY_TREE;
   if (any_tree && (geomId == HRT_TLAS_ID || !s->geoms[geomId].any_tree))
-    return fail(HRT_ERR_INVALID_ARG, "no any-hit tree (commit with HRT_LAYOUT_AUTO_FP16)");
+    return fail(HRT_ERR_INVALID_ARG, "no any-hit tree (commit with HRT_LAYOUT_AUTO_FP16 or a PLOC layout)");
   const hrt::Blas& b = (geomId == HRT_TLAS_ID) ? s->tlas : (any_tree ? *s->geoms[geomId].any_tree : s->geoms[geomId]);
   const uint32_t N = b.ntris, L = b.leaves;
   const uint32_t npairs = std::max<uint32_t>(L > 1 ? L - 1 : 1, 1);

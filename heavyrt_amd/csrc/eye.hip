@@ -11,10 +11,9 @@
 // (eye_ray.cpp:50-75).  Here one lane makes one ray directly in HBM, in that
 // same order for hrt_eye_make_rays (64 consecutive rays are one 8x8
 // superblock).  The whole frame on the device (hrt_eye_render) only owes the
-// image, so it makes, traces and colours its rays in row order: a wave then
-// holds 64 pixels of one row, and on the 512x512 eye frame the trace took
-// 0.144 instead of 0.164 ms (rows beat every tile shape tried: 16x4 0.147,
-// Morton 0.153, 16x16 0.155, 8x8 0.164).
+// image, so it makes, traces and colours its rays in the order that traces
+// fastest: XCD stripes (xcd_stripe_pixel).  On the 512x512 eye frame the
+// trace took 0.126 ms that way, 0.145 in row order, 0.164 in superblocks.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
@@ -36,10 +35,31 @@ __device__ inline uint2 superblock_pixel(uint32_t tid, uint32_t width) {
   return make_uint2(bx * 4u + (in_b & 3u), by * 4u + (in_b >> 2));
 }
 
-// Thread tid -> pixel in superblock order, or in row order (kRows).
-template <bool kRows>
-__device__ inline uint2 thread_pixel(uint32_t tid, uint32_t width) {
-  return kRows ? make_uint2(tid % width, tid / width) : superblock_pixel(tid, width);
+// Thread tid -> pixel in XCD stripes: the nearest-hit kernel runs rays
+// [256 b, 256 b + 256) as workgroup b, which the dispatcher hands to XCD
+// b mod 8, so ray tid is given a pixel of the rows r with (r / 4) mod 8 equal
+// to that XCD (4-row stripes, row order within each XCD's share).  Each XCD's
+// L2 then serves one eighth of the screen while the stripes keep every XCD's
+// share spread over the whole frame (contiguous eighths leave some XCDs the
+// costly regions).  For a height that is a multiple of 32 and a width that
+// is a multiple of 64 (each XCD's share is then a whole number of
+// workgroups); rows otherwise.  A permutation of the pixels either way.
+constexpr uint32_t kTraceGroup = 256;  // lanes per workgroup of the nearest-hit kernel
+constexpr uint32_t kStripeRows = 4;
+__device__ inline uint2 xcd_stripe_pixel(uint32_t tid, uint32_t width, uint32_t height) {
+  if (height % (kStripeRows * kXcds) != 0 || width % (kTraceGroup / kStripeRows) != 0)
+    return make_uint2(tid % width, tid / width);
+  const uint32_t b = tid / kTraceGroup, x = b % kXcds;
+  const uint32_t k = (b / kXcds) * kTraceGroup + tid % kTraceGroup;  // place in XCD x's share
+  const uint32_t rk = k / width;
+  const uint32_t row = (rk / kStripeRows) * (kStripeRows * kXcds) + x * kStripeRows + rk % kStripeRows;
+  return make_uint2(k % width, row);
+}
+
+// Thread tid -> pixel in superblock order, or in XCD stripes (kStripes).
+template <bool kStripes>
+__device__ inline uint2 thread_pixel(uint32_t tid, uint32_t width, uint32_t height) {
+  return kStripes ? xcd_stripe_pixel(tid, width, height) : superblock_pixel(tid, width);
 }
 
 // LiteMath float4x4 * float4 with a column-major matrix: each row's four
@@ -58,12 +78,12 @@ __device__ inline float3 normalize3(float3 u) {
   return make_float3(u.x / len, u.y / len, u.z / len);
 }
 
-template <bool kRows>
+template <bool kStripes>
 __global__ __launch_bounds__(kBlock) void k_eye_rays(EyeCamera cam, uint32_t width, uint32_t height,
                                                      Ray* __restrict__ out, uint32_t* __restrict__ packed_xy) {
   const uint32_t tid = blockIdx.x * kBlock + threadIdx.x;
   if (tid >= width * height) return;
-  const uint2 px = thread_pixel<kRows>(tid, width);
+  const uint2 px = thread_pixel<kStripes>(tid, width, height);
   const float x = (float(px.x) + 0.5f) / float(width);
   const float y = (float(px.y) + 0.5f) / float(height);
   // EyeRayDirNormalized
@@ -86,13 +106,13 @@ __global__ __launch_bounds__(kBlock) void k_eye_rays(EyeCamera cam, uint32_t wid
 }
 
 // out_color[y * width + x] = 0 on a miss, else palette[primId % size]
-template <bool kRows>
+template <bool kStripes>
 __global__ __launch_bounds__(kBlock) void k_eye_colors(const uint2* __restrict__ hits, uint32_t width,
                                                        uint32_t height, EyePalette pal,
                                                        uint32_t* __restrict__ out_color) {
   const uint32_t tid = blockIdx.x * kBlock + threadIdx.x;
   if (tid >= width * height) return;
-  const uint2 px = thread_pixel<kRows>(tid, width);
+  const uint2 px = thread_pixel<kStripes>(tid, width, height);
   const uint32_t prim = hits[tid].y;
   out_color[px.y * width + px.x] = prim == 0xFFFFFFFFu ? 0u : pal.c[prim % pal.n];
 }
@@ -132,22 +152,22 @@ hipError_t ao_colors(const void* hits_compact, const uint8_t* occluded, uint32_t
 }
 
 hipError_t eye_make_rays(const EyeCamera& cam, uint32_t width, uint32_t height, Ray* out, uint32_t* packed_xy,
-                         hipStream_t stream, bool rows) {
+                         hipStream_t stream, bool stripes) {
   const uint64_t n = uint64_t(width) * height;
   if (n == 0) return hipSuccess;
   const uint32_t blocks = uint32_t((n + kBlock - 1) / kBlock);
-  if (rows) k_eye_rays<true><<<blocks, kBlock, 0, stream>>>(cam, width, height, out, packed_xy);
+  if (stripes) k_eye_rays<true><<<blocks, kBlock, 0, stream>>>(cam, width, height, out, packed_xy);
   else k_eye_rays<false><<<blocks, kBlock, 0, stream>>>(cam, width, height, out, packed_xy);
   return hipGetLastError();
 }
 
 hipError_t eye_colors(const void* hits_compact, uint32_t width, uint32_t height, const EyePalette& pal,
-                      uint32_t* out_color, hipStream_t stream, bool rows) {
+                      uint32_t* out_color, hipStream_t stream, bool stripes) {
   const uint64_t n = uint64_t(width) * height;
   if (n == 0) return hipSuccess;
   const uint32_t blocks = uint32_t((n + kBlock - 1) / kBlock);
   const uint2* hits = static_cast<const uint2*>(hits_compact);
-  if (rows) k_eye_colors<true><<<blocks, kBlock, 0, stream>>>(hits, width, height, pal, out_color);
+  if (stripes) k_eye_colors<true><<<blocks, kBlock, 0, stream>>>(hits, width, height, pal, out_color);
   else k_eye_colors<false><<<blocks, kBlock, 0, stream>>>(hits, width, height, pal, out_color);
   return hipGetLastError();
 }

@@ -232,9 +232,13 @@ struct __attribute__((aligned(64))) Quad16 {
   uint4 c[4];
 };
 
-// Deepest BVH4 path: at most 15 node levels (the even depths of a BVH2 of at
-// most 30 internal levels), at most 3 deferred children per level.
-constexpr int kStackDepth4 = 48;
+// Deepest BVH4 path: at most kMaxLevels4 / 2 node levels (the even depths of
+// a BVH2 of at most kMaxLevels4 internal levels), at most 3 deferred children
+// per level.  An LBVH over unique 30-bit codes has at most 30 internal
+// levels; the clustered second tree is kept only within kMaxLevels4 (its
+// build measures the depth: 38 levels on a 4 M-triangle random soup).
+constexpr int kStackDepth4 = 63;
+constexpr int kMaxLevels4 = 2 * (kStackDepth4 / 3);
 
 struct Ray {  // 32-byte ray record: posAndNear, dirAndFar (CrossRT.h:143)
   float4 pos_near;

@@ -111,7 +111,7 @@ hipError_t blas_build_any_tree(Blas& b, hipStream_t stream);
 // Replaces the hierarchy above b's leaves with the agglomerative one (PLOC,
 // ploc.hip); leaves, leaf order and triangles stay.  *replaced is false (and
 // b unchanged) when the tree would be deeper than the BVH4 traversal stack
-// allows (32 internal levels).  Synchronises the stream.
+// allows (kMaxLevels4 internal levels).  Synchronises the stream.
 hipError_t blas_ploc(Blas& b, hipStream_t stream, bool* replaced);
 // Collapses the built BVH2 into FP32 and FP16 BVH4 nodes (idempotent);
 // synchronises the stream.
@@ -243,13 +243,14 @@ struct EyePalette {
   uint32_t c[kEyePaletteMax];
   uint32_t n;
 };
-// Eye rays in the reference's superblock order, or in row order (rows).
+// Eye rays in the reference's superblock order, or in XCD stripes (stripes:
+// the order the nearest-hit kernel traces fastest, eye.hip).
 hipError_t eye_make_rays(const EyeCamera& cam, uint32_t width, uint32_t height, Ray* out, uint32_t* packed_xy,
-                         hipStream_t stream, bool rows = false);
-// Pixel colours from compact hits of those rays (row-major image); rows as
-// the rays were made.
+                         hipStream_t stream, bool stripes = false);
+// Pixel colours from compact hits of those rays (row-major image); stripes
+// as the rays were made.
 hipError_t eye_colors(const void* hits_compact, uint32_t width, uint32_t height, const EyePalette& pal,
-                      uint32_t* out_color, hipStream_t stream, bool rows = false);
+                      uint32_t* out_color, hipStream_t stream, bool stripes = false);
 // The 02_rtao frame's colours from compact primary hits (superblock order)
 // and one occlusion byte per AO ray (spp per pixel).
 hipError_t ao_colors(const void* hits_compact, const uint8_t* occluded, uint32_t width, uint32_t height,

@@ -4,7 +4,7 @@
 // outputs: a ray-queue counter and a per-lane stack spill area.
 // hrt_trace_any takes it from the stream-ordered pool on every call; this
 // entry point takes it from `scratch`, a file-scope device array of
-// kScratchWords words (64 MiB + 256 B) that each GPU the library runs on holds
+// kScratchWords words (96 MiB + 256 B) that each GPU the library runs on holds
 // until the process ends.  Its device address is looked up once per device
 // by noalloc_prepare (hrt_init, with the device current) and kept in
 // g_scratch; host code never passes `scratch` itself to a kernel.  The queue
@@ -18,9 +18,12 @@
 
 namespace {
 
-// queue counter (64 words) + stacks for 8,192 workgroups of 64 lanes x 32
-// spill words (the deeper BVH4 stack; the BVH2 kernels need 16)
-constexpr size_t kScratchWords = 64 + size_t(8192) * 64 * 32;
+// queue counter (64 words) + stacks for 8,192 workgroups of 64 lanes x 48
+// spill words (the BVH4 kernels need kStackDepth4 - kShortStack = 47, the
+// BVH2 kernels 16)
+constexpr size_t kScratchWords = 64 + size_t(8192) * 64 * 48;
+static_assert(kScratchWords >= 64 + size_t(8192) * 64 * (hrt::kStackDepth4 - hrt::kShortStack),
+              "the scratch array holds stacks for 8,192 workgroups");
 
 __device__ __align__(256) uint32_t scratch[kScratchWords];
 

@@ -232,7 +232,8 @@ hipError_t blas_ploc(Blas& b, hipStream_t s, bool* replaced) {
       std::swap(ca, cb);
     } while (n > 1);
     if (e == hipSuccess) {
-      // the BVH4 traversal stack holds 48 entries: at most 32 internal levels
+      // the BVH4 traversal stack holds kStackDepth4 entries: at most
+      // kMaxLevels4 internal levels
       const uint32_t zero = 0u;
       uint32_t depth = 0;
       e = hipMemcpyAsync(tot, &zero, 4, hipMemcpyHostToDevice, s);
@@ -241,7 +242,7 @@ hipError_t blas_ploc(Blas& b, hipStream_t s, bool* replaced) {
         e = hipMemcpyAsync(&depth, tot, 4, hipMemcpyDeviceToHost, s);
       }
       if (e == hipSuccess) e = hipStreamSynchronize(s);
-      ok = e == hipSuccess && depth + 1 <= 32;
+      ok = e == hipSuccess && depth + 1 <= uint32_t(kMaxLevels4);
     }
   }
   if (base) {

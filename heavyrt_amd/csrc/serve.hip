@@ -25,9 +25,10 @@
 // Each request's rays walk the tree hrt_trace_nearest / hrt_trace_any walk
 // on that scene: nearest hits on the BVH2 in BVH2CommonRT's order
 // (BVH2CommonRT.cpp:49-141, first strictly closer hit), any-hit on the
-// scene's any-hit tree (the FP16 BVH4 of the any-hit tree for
-// HRT_LAYOUT_AUTO_FP16, else its BVH4 nodes, else its BVH2), stopping at the
-// first hit.  So a result is the same alone and in any mix.
+// scene's any-hit tree (the BVH4 of the clustered second tree for
+// HRT_LAYOUT_AUTO_FP16 and the PLOC layouts, else its BVH4 nodes, else its
+// BVH2), stopping at the first hit.  So a result is the same alone and in
+// any mix.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

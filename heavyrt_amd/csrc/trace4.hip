@@ -9,11 +9,13 @@
 // (t, primId), so FP16 and FP32 nodes -- which visit in different orders --
 // return identical hits.  tFar <= 0 or hrt_trace_any: stop at the first hit.
 //
-// The stack holds at most 45 entries: BVH4 nodes are the even-depth nodes of
-// a BVH2 with at most 30 internal levels (15 node levels, 3 deferred children
-// each; 48 for a clustered any-hit tree, whose depth its build checks).  Every
-// kernel keeps 16 entries per lane in LDS, interleaved by lane, and the rest
-// in a per-lane global spill area, so LDS does not limit occupancy.
+// The stack holds at most 45 entries on a tree over unique 30-bit codes (BVH4
+// nodes are the even-depth nodes of a BVH2 with at most 30 internal levels:
+// 15 node levels, 3 deferred children each) and at most kStackDepth4 = 63 on
+// the clustered second tree, whose build keeps it within 42 levels.  The
+// persistent kernels keep 16 entries per lane in LDS and the one-ray-per-lane
+// kernel 48, interleaved by lane; deeper entries go to a per-lane global
+// spill area, so LDS does not limit occupancy.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

@@ -183,10 +183,12 @@ struct HipSceneRT : public ISceneObject {
   MetricStats GetStats() override {
     MetricStats s{};
     const double rays = m_rays ? double(m_rays) : 1.0;
-    // nearest hits walk BVH4 nodes (4 per visit) only for a BVH4 layout on a
-    // scene of one identity instance; every other scene walks BVH2 pairs
-    const bool quads = (m_layout == HRT_LAYOUT_BVH4 || m_layout == HRT_LAYOUT_BVH4_FP16) && m_identity.size() == 1 &&
-                       m_identity[0];
+    // nearest hits walk BVH4 nodes (4 per visit) only for a BVH4 or PLOC
+    // layout on a scene of one identity instance; every other scene walks
+    // BVH2 pairs
+    const bool quads = (m_layout == HRT_LAYOUT_BVH4 || m_layout == HRT_LAYOUT_BVH4_FP16 ||
+                        m_layout == HRT_LAYOUT_PLOC || m_layout == HRT_LAYOUT_PLOC_FP16) &&
+                       m_identity.size() == 1 && m_identity[0];
     const double nodes = (quads ? 4.0 : 2.0) * double(m_pairs);
     s.avgNC = float(nodes / rays);
     s.avgTC = float(double(m_tris) / rays);
@@ -238,7 +240,11 @@ ISceneObject* CreateSceneRT(const char* a_implName, const char* a_buildName, con
   (void)a_layoutName;
   const std::string name(a_implName ? a_implName : "");
   int layout = HRT_LAYOUT_BVH2;
-  if (name.find("HipBVH4Half") != std::string::npos || name.find("HipBVH4_16") != std::string::npos)
+  if (name.find("HipPlocHalf") != std::string::npos)
+    layout = HRT_LAYOUT_PLOC_FP16;
+  else if (name.find("HipPloc") != std::string::npos)
+    layout = HRT_LAYOUT_PLOC;
+  else if (name.find("HipBVH4Half") != std::string::npos || name.find("HipBVH4_16") != std::string::npos)
     layout = HRT_LAYOUT_BVH4_FP16;
   else if (name.find("HipBVH4") != std::string::npos)
     layout = HRT_LAYOUT_BVH4;

@@ -335,6 +335,22 @@ def superblock_order(width: int, height: int) -> np.ndarray:
     return ((by * 4 + (in_b >> 2)) * width + bx * 4 + (in_b & 3)).astype(np.int64)
 
 
+def xcd_stripe_order(width: int, height: int, group: int = 256, stripe: int = 4, xcds: int = 8) -> np.ndarray:
+    """Row-major pixel index of each ray in the order hrt_eye_render traces (eye.hip,
+    xcd_stripe_pixel): workgroup b of `group` rays runs on XCD b % xcds and gets pixels of
+    the rows r with (r // stripe) % xcds == b % xcds, in row order.  Row order when the
+    height is not a multiple of stripe * xcds or the width not of group // stripe."""
+    tid = np.arange(width * height, dtype=np.int64)
+    if height % (stripe * xcds) or width % (group // stripe):
+        return tid
+    b = tid // group
+    x = b % xcds
+    k = (b // xcds) * group + tid % group
+    rk = k // width
+    row = (rk // stripe) * (stripe * xcds) + x * stripe + rk % stripe
+    return row * width + k % width
+
+
 def rtao_primary_rays(width: int = 1024, height: int = 1024, order: str = "rows") -> np.ndarray:
     """RTAO eye rays, one per pixel: in row-major order, or ("superblock") in the renderers'
     8x8-superblock order."""

@@ -78,7 +78,10 @@ struct ISceneObject {
 /// Factory by implementation name (core/FactoryRT.cpp:29-78).  The engine's
 /// names: "HipBVH2" (LBVH node pairs, BVH2CommonRT order), "HipBVH4"
 /// (BVH4, FP32 boxes), "HipBVH4Half" (BVH4, FP16 boxes), "HipAuto" (nearest
-/// on BVH2, any-hit on BVH4).  Any other name prints an error and returns the
+/// on BVH2, any-hit on BVH4), "HipAutoHalf" (nearest on BVH2, any-hit on the
+/// FP16 BVH4 of a clustered second tree), "HipPloc" / "HipPlocHalf" (both
+/// queries on the second tree's FP32 / FP16 BVH4).  Any other name prints an
+/// error and returns the
 /// "HipBVH2" engine, as the reference falls back to its default engine.
 ISceneObject* CreateSceneRT(const char* a_implName, const char* a_buildName, const char* a_layoutName);
 void DeleteSceneRT(ISceneObject* a_impl);

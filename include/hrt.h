@@ -21,7 +21,9 @@
  *   - Ray record: 8 floats = posAndNear (x, y, z, tNear), dirAndFar
  *     (x, y, z, tFar), the argument pair of RayQuery_NearestHit
  *     (CrossRT.h:143).  tFar <= 0 requests an any-hit query with
- *     tFar' = -tFar that stops at the first hit (BVH4HalfRT.cpp:260-262).
+ *     tFar' = -tFar that stops at the first hit (BVH4HalfRT.cpp:260-262);
+ *     which hit it reports depends on the walk, so on the tree a layout
+ *     walks (hit or miss does not).
  *   - primId in every hit is the ORIGINAL triangle index of the geometry
  *     (the REMAP_PRIM_ID rule, BVH2CommonRT.cpp:270-276, and BruteForceRT.cpp:255).
  */
@@ -81,13 +83,21 @@ enum { HRT_HIT_FULL = 0, HRT_HIT_COMPACT = 1 };
  *              extent), the hierarchy above them by agglomerative clustering;
  *              it visits fewer nodes.  Any-hit answers do not depend on the
  *              tree.  If a cubic-grid cell holds more than 127 triangles,
- *              any-hit uses the reference tree's FP16 BVH4 instead. */
+ *              any-hit uses the reference tree's FP16 BVH4 instead.
+ *   PLOC       nearest hits and any-hit both on the FP32 BVH4 of that second
+ *              tree.  Nearest hits by (t, primId), so they equal BVH4's; on
+ *              4 M incoherent rays in a 4 M-triangle soup it visits a third
+ *              fewer nodes.  Falls back to the reference tree's BVH4 as
+ *              AUTO_FP16 does.
+ *   PLOC_FP16  as PLOC with FP16 boxes (the second tree's FP16 BVH4). */
 enum {
   HRT_LAYOUT_BVH2 = 0,
   HRT_LAYOUT_BVH4 = 1,
   HRT_LAYOUT_BVH4_FP16 = 2,
   HRT_LAYOUT_AUTO = 3,
-  HRT_LAYOUT_AUTO_FP16 = 4
+  HRT_LAYOUT_AUTO_FP16 = 4,
+  HRT_LAYOUT_PLOC = 5,
+  HRT_LAYOUT_PLOC_FP16 = 6
 };
 enum { HRT_BUILD_LOW = 0, HRT_BUILD_MEDIUM = 1, HRT_BUILD_HIGH = 2, HRT_BUILD_REFIT = 3 }; /* BuildQuality, CrossRT.h:9-15 */
 
@@ -119,7 +129,7 @@ enum {
   HRT_DUMP_QUADS32 = 9,      /* uint32[nquads*32]: BVH4 nodes, FP32 boxes (after a BVH4 commit) */
   HRT_DUMP_QUADS16 = 10,     /* uint32[nquads*16]: BVH4 nodes, FP16 boxes */
   HRT_DUMP_PAIRS = 11,       /* uint32[pairs*16]: the BVH2 node pairs as traversed */
-  HRT_DUMP_ANY_TREE = 0x100  /* OR'ed into `what`: the same array of the any-hit tree (HRT_LAYOUT_AUTO_FP16) */
+  HRT_DUMP_ANY_TREE = 0x100  /* OR'ed into `what`: the same array of the any-hit tree (AUTO_FP16, PLOC layouts) */
 };
 
 /* Context: one device, one HIP stream.  No counterpart in the reference
@@ -213,7 +223,7 @@ hrt_status hrt_trace_any(hrt_scene* scene, const float* rays, size_t n, uint8_t*
 
 /* hrt_trace_any with no allocation of any kind (no hipMalloc, no pool, no
  * allocating library call): the any-hit kernels' queue counter and stack
- * spill live in a static device array of 64 MiB + 256 B that each GPU the
+ * spill live in a static device array of 96 MiB + 256 B that each GPU the
  * library runs on holds until the process ends (its address is looked up
  * by hrt_init).  No counterpart in the reference: for callers that cannot
  * allocate -- code under HIP graph capture, tight serving loops, harnesses
@@ -257,7 +267,7 @@ hrt_status hrt_rtao_make_rays(hrt_scene* scene, const float* primaryRays, const 
 /* The AO pass of the RTAO renderer in one call: the same rays as
  * hrt_rtao_make_rays (same arguments), traced as hrt_trace_any traces them,
  * one byte per ray in outHits (1 = occluded), ray k = pixel * raysPerHit + s.
- * With BVH4 nodes (HRT_LAYOUT_BVH4, _BVH4_FP16, _AUTO, _AUTO_FP16) the
+ * With BVH4 nodes (every layout but HRT_LAYOUT_BVH2) the
  * traversal makes each ray in registers from a 32-byte per-pixel frame, so no
  * ray touches HBM; other layouts write the rays to a temporary buffer first.
  * nPixels * raysPerHit must stay below 2^32.  No counterpart in the reference
@@ -281,8 +291,9 @@ hrt_status hrt_eye_make_rays(hrt_ctx* ctx, const float projInvColMajor16[16], co
 
 /* The 01_eye_rays frame on the device (EyeRayCaster::Render /
  * CastRaySingleBlock, eye_ray_host.cpp:329-345, with kernel_RayTrace,
- * eye_ray.cpp:50-75): eye rays as above (made and traced in row order
- * rather than the superblock order, which only the image's speed sees),
+ * eye_ray.cpp:50-75): eye rays as above (made and traced in an order of
+ * its own rather than the superblock order, which only the image's speed
+ * sees),
  * nearest hits against the committed
  * scene, and outColor[y * width + x] = 0 on a miss, else
  * palette[primId % paletteSize] (paletteSize 1..64, host array).  outColor

@@ -76,16 +76,39 @@ def test_bvh4_nodes_bit_exact(gpu_ctx, name, gen):
     s.close()
 
 
+FP16_LAYOUTS = (H.LAYOUT_BVH4_FP16, H.LAYOUT_AUTO_FP16, H.LAYOUT_PLOC_FP16)
+NEAREST4 = [H.LAYOUT_BVH4, H.LAYOUT_BVH4_FP16, H.LAYOUT_PLOC, H.LAYOUT_PLOC_FP16]
+NEAREST4_IDS = ["fp32", "fp16", "ploc", "ploc_fp16"]
+
+
+def walked_tree(scene, layout, v, f, ob):
+    """The oracle tree the scene's BVH4 traversals walk: the second (clustered) tree for the
+    AUTO_FP16 and PLOC layouts when the commit built one, else the reference tree ob."""
+    if layout in (H.LAYOUT_AUTO_FP16, H.LAYOUT_PLOC, H.LAYOUT_PLOC_FP16) and has_any_tree(scene):
+        return oracle_any_tree(v, f)[0]
+    return ob
+
+
 @pytest.mark.parametrize("name,gen", CASES, ids=[c[0] for c in CASES])
-@pytest.mark.parametrize("layout", [H.LAYOUT_BVH4, H.LAYOUT_BVH4_FP16], ids=["fp32", "fp16"])
+@pytest.mark.parametrize("layout", NEAREST4, ids=NEAREST4_IDS)
 def test_bvh4_nearest_bit_exact(gpu_ctx, name, gen, layout):
+    """Nearest hits bit-exact with the oracle's BVH4 traversal of the reference tree under the
+    (t, primId) rule, on every BVH4 layout: the PLOC layouts walk the clustered second tree, whose
+    hits are the same; their counters equal the oracle's walk of that tree."""
     v, f = gen()
     rays = rays_for(v, 20000, 1500)
     say(f"bvh4 nearest {name} layout {layout}")
     ob = O.OracleBVH(v, f)
     s = scene_with(gpu_ctx, v, f, layout)
-    mode = O.MODE4_FP16 if layout == H.LAYOUT_BVH4_FP16 else O.MODE4_FP32
+    mode = O.MODE4_FP16 if layout in FP16_LAYOUTS else O.MODE4_FP32
     t, p, u, vv, nq, nt = ob.trace4(rays, mode, counters=True)
+    walked = walked_tree(s, layout, v, f, ob)
+    if walked is not ob:
+        t2, p2, u2, v2, nq, nt = walked.trace4(rays, mode, counters=True)
+        np.testing.assert_array_equal(t2.view(np.uint32), t.view(np.uint32))
+        np.testing.assert_array_equal(p2, p)
+        np.testing.assert_array_equal(u2.view(np.uint32), u.view(np.uint32))
+        np.testing.assert_array_equal(v2.view(np.uint32), vv.view(np.uint32))
     g = trace(s, rays, "full", stats=True)
     np.testing.assert_array_equal(g["t"].view(np.uint32), t.view(np.uint32))
     np.testing.assert_array_equal(g["prim"], p)
@@ -103,12 +126,14 @@ def test_bvh4_nearest_bit_exact(gpu_ctx, name, gen, layout):
     s.close()
 
 
-@pytest.mark.parametrize("layout", [H.LAYOUT_BVH4, H.LAYOUT_BVH4_FP16], ids=["fp32", "fp16"])
+@pytest.mark.parametrize("layout", NEAREST4, ids=NEAREST4_IDS)
 def test_bvh4_nearest_sorted_waves(gpu_ctx, layout):
     """A batch of 2^20 rays or more is sorted into coherent waves before the trace (each thread
-    traces the ray the order gives it): full hits equal the oracle's BVH4 traversal under the
-    (t, primId) rule on every ray, at every ray's own index, including rays that are any-hit by
-    their own tFar <= 0, empty and reversed intervals, a NaN bound and a NaN direction."""
+    traces the ray the order gives it): full hits equal the oracle's BVH4 traversal of the tree
+    the layout walks under the (t, primId) rule on every ray, at every ray's own index, including
+    rays that are any-hit by their own tFar <= 0 (their hit is the first one the walk finds, so
+    it depends on the tree; their hit/miss answer and every nearest hit do not), empty and
+    reversed intervals, a NaN bound and a NaN direction."""
     v, f = W.soup(150_000, stream=W.STREAM_TEST + 5)
     n = 1_100_000
     rays = W.incoherent_rays(n, stream=W.STREAM_TEST + 6)
@@ -123,9 +148,17 @@ def test_bvh4_nearest_sorted_waves(gpu_ctx, layout):
     say(f"bvh4 nearest sorted waves, layout {layout}")
     s = scene_with(gpu_ctx, v, f, layout)
     g = trace(s, rays, "full")
+    ref = O.OracleBVH(v, f)
+    walked = walked_tree(s, layout, v, f, ref)
     s.close()
-    mode = O.MODE4_FP16 if layout == H.LAYOUT_BVH4_FP16 else O.MODE4_FP32
-    t, p, u, vv = O.OracleBVH(v, f).trace4(rays, mode)
+    mode = O.MODE4_FP16 if layout in FP16_LAYOUTS else O.MODE4_FP32
+    t, p, u, vv = walked.trace4(rays, mode)
+    if walked is not ref:
+        tr, pr, _, _ = ref.trace4(rays, mode)
+        near = kind != 1
+        np.testing.assert_array_equal(p[near], pr[near])
+        np.testing.assert_array_equal(t[near].view(np.uint32), tr[near].view(np.uint32))
+        np.testing.assert_array_equal(p[~near] != INVALID, pr[~near] != INVALID)
     np.testing.assert_array_equal(g["prim"], p)
     np.testing.assert_array_equal(g["t"].view(np.uint32), t.view(np.uint32))
     np.testing.assert_array_equal(g["u"].view(np.uint32), u.view(np.uint32))
@@ -136,8 +169,9 @@ def test_bvh4_nearest_sorted_waves(gpu_ctx, layout):
 
 
 @pytest.mark.parametrize("name,gen", CASES, ids=[c[0] for c in CASES])
-@pytest.mark.parametrize("layout", [H.LAYOUT_BVH4, H.LAYOUT_BVH4_FP16, H.LAYOUT_AUTO, H.LAYOUT_AUTO_FP16],
-                         ids=["fp32", "fp16", "auto", "auto_fp16"])
+@pytest.mark.parametrize("layout", [H.LAYOUT_BVH4, H.LAYOUT_BVH4_FP16, H.LAYOUT_AUTO, H.LAYOUT_AUTO_FP16,
+                                    H.LAYOUT_PLOC, H.LAYOUT_PLOC_FP16],
+                         ids=["fp32", "fp16", "auto", "auto_fp16", "ploc", "ploc_fp16"])
 def test_bvh4_any_hit(gpu_ctx, name, gen, layout):
     v, f = gen()
     rays = rays_for(v, 20000, 1700)
@@ -152,10 +186,8 @@ def test_bvh4_any_hit(gpu_ctx, name, gen, layout):
     np.testing.assert_array_equal(g["hit"], p2 != INVALID)
     if layout != H.LAYOUT_AUTO:
         # hrt_trace_any on FP16 nodes tests boxes in the fused form
-        mode = O.MODE4_FP16_FUSED if layout in (H.LAYOUT_BVH4_FP16, H.LAYOUT_AUTO_FP16) else O.MODE4_FP32
-        ob4 = ob
-        if layout == H.LAYOUT_AUTO_FP16 and has_any_tree(s):
-            ob4 = oracle_any_tree(v, f)[0]  # AUTO_FP16 answers any-hit on its own any-hit tree
+        mode = O.MODE4_FP16_FUSED if layout in FP16_LAYOUTS else O.MODE4_FP32
+        ob4 = walked_tree(s, layout, v, f, ob)  # AUTO_FP16 and PLOC answer any-hit on the second tree
         _, p4, _, _, nq, nt = ob4.trace4(anyr, mode, counters=True)
         gs = trace(s, rays, "any", stats=True)
         np.testing.assert_array_equal(gs["hit"], p4 != INVALID)
@@ -174,7 +206,7 @@ def has_any_tree(scene) -> bool:
 
 def oracle_any_tree(v, f):
     """The oracle's any-hit tree: leaves of the cubic-grid LBVH, hierarchy by agglomerative clustering
-    (radius 8) unless that tree has more than 32 internal levels (the BVH4 stack bound)."""
+    (radius 8) unless that tree has more than 42 internal levels (the BVH4 stack bound)."""
     ob = O.OracleBVH(v, f, cubic=True)
     if ob.max_run > 127 or ob.leaves < 3:
         return ob, False
@@ -186,7 +218,7 @@ def oracle_any_tree(v, f):
     for i in range(L - 1):  # parents have smaller ids than their children
         depth[int(lr[i, 3])] = depth[i] + 1
         depth[int(lr[i, 7])] = depth[i] + 1
-    return (pl, True) if depth.max() <= 32 else (ob, False)
+    return (pl, True) if depth.max() <= 42 else (ob, False)
 
 
 @pytest.mark.parametrize("name,gen", CASES, ids=[c[0] for c in CASES])
@@ -242,21 +274,24 @@ def test_fp16_layout_rejects_out_of_range_scene(gpu_ctx):
 
 def test_precision_soup_fp16_vs_fp32_identical(gpu_ctx):
     """Precision workload shape at reduced size: random soup, incoherent rays;
-    FP16 and FP32 BVH4 nodes return identical hits; t error vs FP64 reported."""
+    FP16 and FP32 BVH4 nodes return identical hits, on the reference tree and
+    on the clustered second tree (PLOC layouts); t error vs FP64 reported."""
     v, f = W.soup(500_000, stream=W.STREAM_SOUP)
     rays = W.incoherent_rays(500_000, stream=W.STREAM_RAYS)
     say("precision soup 500k")
-    s32 = scene_with(gpu_ctx, v, f, H.LAYOUT_BVH4)
-    s16 = scene_with(gpu_ctx, v, f, H.LAYOUT_BVH4_FP16)
-    try:
-        a = trace(s32, rays, "full")
-        b = trace(s16, rays, "full")
-    finally:
-        s32.close()
-        s16.close()
-    for k in ("t", "u", "v"):
-        np.testing.assert_array_equal(a[k].view(np.uint32), b[k].view(np.uint32))
-    np.testing.assert_array_equal(a["prim"], b["prim"])
+    res = {}
+    for lay in (H.LAYOUT_BVH4, H.LAYOUT_BVH4_FP16, H.LAYOUT_PLOC, H.LAYOUT_PLOC_FP16):
+        sc = scene_with(gpu_ctx, v, f, lay)
+        try:
+            res[lay] = trace(sc, rays, "full")
+        finally:
+            sc.close()
+    a = res[H.LAYOUT_BVH4]
+    for lay in (H.LAYOUT_BVH4_FP16, H.LAYOUT_PLOC, H.LAYOUT_PLOC_FP16):
+        b = res[lay]
+        for k in ("t", "u", "v"):
+            np.testing.assert_array_equal(a[k].view(np.uint32), b[k].view(np.uint32))
+        np.testing.assert_array_equal(a["prim"], b["prim"])
     ob = O.OracleBVH(v, f)
     hit = a["prim"] != INVALID
     t64 = ob.t_fp64(rays[hit], a["prim"][hit])
@@ -269,7 +304,8 @@ def test_precision_soup_fp16_vs_fp32_identical(gpu_ctx):
     assert rel.mean() < 1e-5
 
 
-@pytest.mark.parametrize("layout", [H.LAYOUT_BVH4_FP16, H.LAYOUT_AUTO_FP16], ids=["bvh4_fp16", "auto_fp16"])
+@pytest.mark.parametrize("layout", [H.LAYOUT_BVH4_FP16, H.LAYOUT_AUTO_FP16, H.LAYOUT_PLOC_FP16],
+                         ids=["bvh4_fp16", "auto_fp16", "ploc_fp16"])
 def test_any_hit_axis_parallel_far_from_origin(gpu_ctx, layout):
     """Rays with d.x = d.z = 0 over geometry at x = 500: -(o * inv) overflows there, so these lanes
     keep the unfused box test; answers equal brute force and counts equal the oracle's."""
@@ -300,8 +336,9 @@ def test_any_hit_axis_parallel_far_from_origin(gpu_ctx, layout):
     s.close()
 
 
-@pytest.mark.parametrize("layout", [H.LAYOUT_BVH2, H.LAYOUT_BVH4, H.LAYOUT_BVH4_FP16, H.LAYOUT_AUTO_FP16],
-                         ids=["bvh2", "bvh4", "bvh4_fp16", "auto_fp16"])
+@pytest.mark.parametrize("layout", [H.LAYOUT_BVH2, H.LAYOUT_BVH4, H.LAYOUT_BVH4_FP16, H.LAYOUT_AUTO_FP16,
+                                    H.LAYOUT_PLOC, H.LAYOUT_PLOC_FP16],
+                         ids=["bvh2", "bvh4", "bvh4_fp16", "auto_fp16", "ploc", "ploc_fp16"])
 def test_any_hit_interval_edge_cases(gpu_ctx, layout):
     """Empty and reversed (tNear, tFar) intervals, a negative tNear and a NaN bound: the any-hit
     kernels finish such rays at once; answers equal the oracle's brute force."""

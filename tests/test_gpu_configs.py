@@ -47,18 +47,23 @@ def test_precision_config_full_size(gpu_ctx):
     v, f = W.soup(n, stream=W.STREAM_SOUP)
     rays = W.incoherent_rays(n, stream=W.STREAM_RAYS)
     say(f"precision: {n} tris x {n} rays")
-    s32 = scene_with(gpu_ctx, v, f, H.LAYOUT_BVH4)
-    s16 = scene_with(gpu_ctx, v, f, H.LAYOUT_BVH4_FP16)
-    try:
-        a = trace(s32, rays, "full")
-        b = trace(s16, rays, "full")
-    finally:
-        s32.close()
-        s16.close()
-    for k in ("t", "u", "v"):
-        np.testing.assert_array_equal(a[k].view(np.uint32), b[k].view(np.uint32), err_msg=k)
-    np.testing.assert_array_equal(a["prim"], b["prim"])
-    say(f"precision: FP16 = FP32 on all rays; oracle BVH4 on {host_threads()} threads")
+    # the timed layout (FP16 nodes of the clustered tree), its FP32 twin, and
+    # the reference tree's FP16 and FP32 BVH4
+    res = {}
+    for lay in (H.LAYOUT_PLOC_FP16, H.LAYOUT_PLOC, H.LAYOUT_BVH4_FP16, H.LAYOUT_BVH4):
+        sc = scene_with(gpu_ctx, v, f, lay)
+        try:
+            res[lay] = trace(sc, rays, "full")
+        finally:
+            sc.close()
+    a = res[H.LAYOUT_PLOC_FP16]
+    for lay in (H.LAYOUT_PLOC, H.LAYOUT_BVH4_FP16, H.LAYOUT_BVH4):
+        b = res[lay]
+        for k in ("t", "u", "v"):
+            np.testing.assert_array_equal(a[k].view(np.uint32), b[k].view(np.uint32), err_msg=f"{k} layout {lay}")
+        np.testing.assert_array_equal(a["prim"], b["prim"], err_msg=f"layout {lay}")
+    del res
+    say(f"precision: FP16 = FP32 on both trees, all rays; oracle BVH4 on {host_threads()} threads")
     t0 = time.perf_counter()
     ob = O.OracleBVH(v, f)
     t, p, u, vv = ob.trace4(rays, O.MODE4_FP32, threads=host_threads())

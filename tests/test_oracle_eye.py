@@ -69,3 +69,22 @@ def test_superblock_order_inverts_the_oracle_packing():
         assert sorted(order.tolist()) == list(range(w * h))
         tid = np.array([O.superblock_offset(int(i % w), int(i // w), w) for i in order])
         np.testing.assert_array_equal(tid, np.arange(w * h))
+
+
+@pytest.mark.parametrize("w,h", [(512, 512), (640, 480), (64, 32), (100, 96), (512, 40)])
+def test_xcd_stripe_order_gives_each_xcd_its_stripes(w, h):
+    """workloads.xcd_stripe_order (the order hrt_eye_render traces its rays in): a permutation of
+    the pixels; where the frame splits into whole workgroups per XCD, workgroup b (256 rays, run
+    on XCD b % 8) holds only rows r with (r // 4) % 8 == b % 8, and each XCD takes its rows in
+    order; otherwise plain row order."""
+    order = W.xcd_stripe_order(w, h)
+    assert np.array_equal(np.sort(order), np.arange(w * h))
+    if h % 32 or w % 64:
+        assert np.array_equal(order, np.arange(w * h))
+        return
+    rows = (order // w).reshape(-1, 256)
+    xcd = np.arange(rows.shape[0]) % 8
+    assert np.all((rows // 4) % 8 == xcd[:, None])
+    for x in range(8):
+        share = order.reshape(-1, 256)[xcd == x].reshape(-1)
+        assert np.all(np.diff(share) > 0)  # row order within the XCD's share
