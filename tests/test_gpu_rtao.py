@@ -10,8 +10,9 @@ from heavyrt_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
 
-LAYOUTS = [H.LAYOUT_BVH2, H.LAYOUT_BVH4, H.LAYOUT_BVH4_FP16, H.LAYOUT_AUTO, H.LAYOUT_AUTO_FP16]
-IDS = ["bvh2", "bvh4", "bvh4_fp16", "auto", "auto_fp16"]
+LAYOUTS = [H.LAYOUT_BVH2, H.LAYOUT_BVH4, H.LAYOUT_BVH4_FP16, H.LAYOUT_AUTO, H.LAYOUT_AUTO_FP16, H.LAYOUT_PLOC,
+           H.LAYOUT_PLOC_FP16]
+IDS = ["bvh2", "bvh4", "bvh4_fp16", "auto", "auto_fp16", "ploc", "ploc_fp16"]
 
 
 def two_step_and_fused(scene, prim, spp, tmax, seed):

@@ -22,8 +22,8 @@ def same_traces(a, b, rays):
     np.testing.assert_array_equal(trace(a, anyr, "any")["hit"], trace(b, anyr, "any")["hit"])
 
 
-@pytest.mark.parametrize("layout", [H.LAYOUT_BVH2, H.LAYOUT_BVH4_FP16, H.LAYOUT_AUTO_FP16],
-                         ids=["bvh2", "bvh4_fp16", "auto_fp16"])
+@pytest.mark.parametrize("layout", [H.LAYOUT_BVH2, H.LAYOUT_BVH4_FP16, H.LAYOUT_AUTO_FP16, H.LAYOUT_PLOC],
+                         ids=["bvh2", "bvh4_fp16", "auto_fp16", "ploc"])
 def test_snapshot_single_geometry(gpu_ctx, layout):
     v, f = W.rtao_scene(256, 128, 300)
     s = H.Scene(gpu_ctx)
@@ -37,8 +37,8 @@ def test_snapshot_single_geometry(gpu_ctx, layout):
         np.testing.assert_array_equal(r.dump(0, what).view(np.uint32), s.dump(0, what).view(np.uint32))
     if layout == H.LAYOUT_BVH4_FP16:
         np.testing.assert_array_equal(r.dump(0, H.DUMP_QUADS16), s.dump(0, H.DUMP_QUADS16))
-    if layout == H.LAYOUT_AUTO_FP16:
-        for what in (H.DUMP_SIZES, H.DUMP_PAIRS, H.DUMP_QUADS16):
+    if layout in (H.LAYOUT_AUTO_FP16, H.LAYOUT_PLOC):
+        for what in (H.DUMP_SIZES, H.DUMP_PAIRS, H.DUMP_QUADS16, H.DUMP_QUADS32):
             np.testing.assert_array_equal(r.dump(0, what | H.DUMP_ANY_TREE), s.dump(0, what | H.DUMP_ANY_TREE))
     rays = rays_for(v, 20_000, 5300)
     same_traces(s, r, rays)

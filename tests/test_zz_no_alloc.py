@@ -25,7 +25,8 @@ def any_rays(v, n, stream):
     return rays
 
 
-@pytest.mark.parametrize("layout", [H.LAYOUT_BVH2, H.LAYOUT_AUTO_FP16], ids=["bvh2", "auto_fp16"])
+@pytest.mark.parametrize("layout", [H.LAYOUT_BVH2, H.LAYOUT_AUTO_FP16, H.LAYOUT_PLOC],
+                         ids=["bvh2", "auto_fp16", "ploc"])
 def test_noalloc_any_hit_equals_trace_any(gpu_ctx, layout):
     import torch
 
