@@ -61,6 +61,32 @@ def test_eye_render_full_config(gpu_ctx):
     s.close()
 
 
+@pytest.mark.parametrize("w,h", [(640, 480), (72, 40), (192, 64)])
+def test_eye_render_any_frame_size(gpu_ctx, w, h):
+    """hrt_eye_render traces in XCD stripes where the frame splits into whole workgroups per XCD
+    (640x480, 192x64) and in row order otherwise (72x40); every pixel gets its own ray's colour."""
+    import torch
+
+    v, f = W.eye_scene()
+    cam = W.eye_camera(w, h)
+    s = H.Scene(gpu_ctx)
+    s.add_instance(s.add_triangles(v, f))
+    s.commit()
+    say(f"eye render {w}x{h}")
+    out = torch.full((w * h,), 7, dtype=torch.int32, device="cuda")
+    s.eye_render(cam[0], cam[1], w, h, W.EYE_PALETTE, out, stream=torch.cuda.current_stream().cuda_stream)
+    torch.cuda.synchronize()
+    img = out.cpu().numpy().view(np.uint32)
+    rays, xy = O.eye_rays(cam[0], cam[1], w, h)
+    _, prim, _, _ = O.OracleBVH(v, f).trace(rays)
+    want = np.zeros(w * h, dtype=np.uint32)
+    pix = (xy >> 16).astype(np.int64) * w + (xy & 0xFFFF)
+    hit = prim != 0xFFFFFFFF
+    want[pix[hit]] = W.EYE_PALETTE[prim[hit] % W.EYE_PALETTE.size]
+    np.testing.assert_array_equal(img, want)
+    s.close()
+
+
 def test_eye_render_instanced(gpu_ctx):
     """A two-level scene (two spheres as instances of one geometry) renders like its merged mesh."""
     import torch
