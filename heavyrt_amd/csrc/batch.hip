@@ -13,8 +13,10 @@
 //                  each scene's run of the sorted array is exactly its solo
 //                  (code, id) order
 //   runs           a run also starts at every scene boundary
-//   Karras/refit   per scene over its own leaves; node pairs, leaf starts and
-//                  triangle ids are scene-local, as in a solo build
+//   tree           one bottom-up pass per scene builds the hierarchy and the
+//                  boxes together (the same tree as the Karras search); node
+//                  pairs, leaf starts and triangle ids are scene-local, as in
+//                  a solo build
 // The trace is a persistent wave64 nearest-hit kernel over all rays of all
 // scenes (ray i finds its scene by binary search); each scene is walked in
 // BVH2CommonRT's order (BVH2CommonRT.cpp:49-141), so hits equal solo traces.
@@ -214,84 +216,98 @@ __global__ __launch_bounds__(kB) void k_b_run_compact(const uint32_t* __restrict
   if (i == T - 1) leaf_off[off.S] = pos[i] + flags[i];
 }
 
-__device__ inline uint32_t b_leaf_ref(const uint32_t* __restrict__ run_start, uint32_t L, uint32_t n, uint32_t m) {
-  const uint32_t st = run_start[m];
-  const uint32_t e = (m + 1 < L) ? run_start[m + 1] : n;
-  return pack_leaf(st, e - st);
+// ---------------------------------------------------------------------------
+// Hierarchy and boxes in one bottom-up pass per scene: the agglomerative LBVH
+// construction of Apetrei (2014, "Fast and Simple Agglomerative LBVH
+// Construction").  Over a scene's sorted unique codes c[lb..le), a node
+// covering leaves [l, r] has as parent the node that extends it across the
+// boundary where the adjacent codes share the longer prefix, i.e. whose xor is
+// smaller: across r (the parent splits at r and the node is its left child)
+// when l == lb or c[r] ^ c[r+1] < c[l-1] ^ c[l], otherwise across l - 1 (the
+// node is the right child).  The two xors never tie for unique sorted codes,
+// so this is the binary radix tree that Karras's top-down search builds
+// (lbvh.cpp:405-441), and its node numbers follow from the side: a left
+// child is node r, a right child node l, the root node 0 (scene-local).
+//
+// Each split g (between leaves g and g+1) has two slots, one per child side,
+// holding half of the parent's pair -- {lo.xyz, child ref, hi.xyz, far end of
+// the child's leaf range} -- so the parent's pair is its two slots, the
+// range ends zeroed.  A child writes its slot and then arrives at g; the
+// second to arrive reads its sibling's slot, writes the parent's pair at the
+// parent's own number once it knows the parent's side, and climbs on.
+//
+// Synchronisation follows refit.h: k_b_climb_chunks gives one workgroup to
+// each chunk of kRefitChunk global leaves, and a split whose two leaves both
+// lie in the chunk is arrived at only by that workgroup (LDS exchange of the
+// child's far end, workgroup-scoped).  An arrival at a split on the chunk's
+// edge, and a split left with one arrival when the workgroup ends, become
+// entries of the scene's top list; k_b_climb_tops (one workgroup per scene)
+// finishes from those with per-split counters in global memory.
+// ---------------------------------------------------------------------------
+
+// True when node [l, r] of a scene over global leaves [lb, le) is its
+// parent's left child.  Requires [l, r] to be a proper sub-range.
+__device__ inline bool b_is_left(const uint32_t* __restrict__ ucodes, uint32_t lb, uint32_t le, uint32_t l,
+                                 uint32_t r) {
+  if (l == lb) return true;
+  if (r + 1 == le) return false;
+  return (ucodes[r] ^ ucodes[r + 1]) < (ucodes[l - 1] ^ ucodes[l]);
 }
 
-// Karras 2012 per scene (lbvh.cpp:405-441).  Thread j = global leaf index.
-// range[] holds each internal node's leaf range in GLOBAL leaf numbers (the
-// refit compares it with workgroup chunks of global leaves).
-__global__ __launch_bounds__(kB) void k_b_karras(const uint32_t* __restrict__ ucodes_all,
-                                                 const uint32_t* __restrict__ run_start_all,
-                                                 const uint32_t* __restrict__ leaf_off, SceneOffsets off,
-                                                 uint32_t T, NodePair* __restrict__ pairs_all,
-                                                 uint32_t* __restrict__ parent_int_all,
-                                                 uint32_t* __restrict__ parent_leaf_all,
-                                                 uint2* __restrict__ range_all) {
-  const uint32_t jg = blockIdx.x * kB + threadIdx.x;
-  if (jg >= T || jg >= leaf_off[off.S]) return;
-  const uint32_t s = seg_of_wave(leaf_off, off.S, jg);
-  const uint32_t lb = leaf_off[s];
-  const int L = int(leaf_off[s + 1] - lb);
-  const int i = int(jg - lb);
-  if (i >= L - 1) return;
-  const uint32_t* ucodes = ucodes_all + lb;
-  const uint32_t* run_start = run_start_all + lb;
-  const uint32_t n = off.tri[s + 1] - off.tri[s];
-  const uint32_t pb = off.pair[s];
-  const int dr = kdelta(ucodes, L, i, i + 1) - kdelta(ucodes, L, i, i - 1);
-  const int d = (dr > 0) - (dr < 0);
-  const int dmin = kdelta(ucodes, L, i, i - d);
-  int lmax = 2;
-  while (kdelta(ucodes, L, i, i + lmax * d) > dmin) lmax <<= 1;
-  int l = 0;
-  for (int t = lmax >> 1; t > 0; t >>= 1)
-    if (kdelta(ucodes, L, i, i + (l + t) * d) > dmin) l += t;
-  const int j = i + l * d;
-  const int dnode = kdelta(ucodes, L, i, j);
-  int sp = 0;
-  for (int t = (l + 1) >> 1; t > 0; t = (t == 1 ? 0 : (t + 1) >> 1))
-    if (kdelta(ucodes, L, i, i + (sp + t) * d) > dnode) sp += t;
-  const int gamma = i + sp * d + min(d, 0);
-  const bool left_leaf = (min(i, j) == gamma);
-  const bool right_leaf = (max(i, j) == gamma + 1);
-  uint32_t* pw = reinterpret_cast<uint32_t*>(pairs_all + pb + i);
-  uint32_t lref, rref;
-  if (left_leaf) {
-    lref = b_leaf_ref(run_start, uint32_t(L), n, uint32_t(gamma));
-    parent_leaf_all[lb + gamma] = uint32_t(i) << 1;
-  } else {
-    lref = uint32_t(gamma);
-    parent_int_all[pb + gamma] = uint32_t(i) << 1;
-  }
-  if (right_leaf) {
-    rref = b_leaf_ref(run_start, uint32_t(L), n, uint32_t(gamma + 1));
-    parent_leaf_all[lb + gamma + 1] = (uint32_t(i) << 1) | 1u;
-  } else {
-    rref = uint32_t(gamma + 1);
-    parent_int_all[pb + gamma + 1] = (uint32_t(i) << 1) | 1u;
-  }
-  pw[3] = lref;
-  pw[7] = 0u;
-  pw[11] = rref;
-  pw[15] = 0u;
-  range_all[pb + i] = make_uint2(lb + uint32_t(min(i, j)), lb + uint32_t(max(i, j)));
+// Half of a node pair: {lo.xyz, ref}, {hi.xyz, far}.
+struct Half {
+  uint4 a, b;
+};
+
+__device__ inline Half load_half(const uint4* __restrict__ slots, uint32_t k) {
+  return Half{slots[2 * size_t(k)], slots[2 * size_t(k) + 1]};
 }
 
-// Leaf boxes and the bottom-up refit, chunked over global leaves as in
-// lbvh.hip: a node whose global leaf range lies inside the workgroup's chunk
-// synchronises at workgroup scope, the rest at agent scope.
-// One thread per global leaf m: its box goes into its parent's pair (where
-// the refit starts), its triangles into leaf order with scene-local ids, and
-// its run length into the scene's longest run.
+__device__ inline void store_half(uint4* __restrict__ slots, uint32_t k, const Half& h) {
+  slots[2 * size_t(k)] = h.a;
+  slots[2 * size_t(k) + 1] = h.b;
+}
+
+// The node's pair from its children's halves (range ends dropped).
+__device__ inline void put_pair(NodePair* pairs, uint32_t k, const Half& h0, const Half& h1) {
+  uint4* w = reinterpret_cast<uint4*>(pairs + k);
+  w[0] = h0.a;
+  w[1] = make_uint4(h0.b.x, h0.b.y, h0.b.z, 0u);
+  w[2] = h1.a;
+  w[3] = make_uint4(h1.b.x, h1.b.y, h1.b.z, 0u);
+}
+
+// The node's own half: the union of its children's boxes, ref k, far end f.
+__device__ inline Half union_half(const Half& h0, const Half& h1, uint32_t k, uint32_t f) {
+  Half u;
+  u.a.x = __float_as_uint(fminf(__uint_as_float(h0.a.x), __uint_as_float(h1.a.x)));
+  u.a.y = __float_as_uint(fminf(__uint_as_float(h0.a.y), __uint_as_float(h1.a.y)));
+  u.a.z = __float_as_uint(fminf(__uint_as_float(h0.a.z), __uint_as_float(h1.a.z)));
+  u.a.w = k;
+  u.b.x = __float_as_uint(fmaxf(__uint_as_float(h0.b.x), __uint_as_float(h1.b.x)));
+  u.b.y = __float_as_uint(fmaxf(__uint_as_float(h0.b.y), __uint_as_float(h1.b.y)));
+  u.b.z = __float_as_uint(fmaxf(__uint_as_float(h0.b.z), __uint_as_float(h1.b.z)));
+  u.b.w = f;
+  return u;
+}
+
+// Appends top-list entry e (scene-local split << 1 | side) for scene s.
+__device__ inline void push_top(const ManyTrees& trees, uint32_t s, uint32_t e, uint32_t* __restrict__ tops,
+                                uint32_t* __restrict__ ntops, uint32_t* __restrict__ err) {
+  const uint32_t k = __hip_atomic_fetch_add(&ntops[s], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (k < trees.tops_cap(s)) tops[trees.tops_base(s) + k] = e;
+  else __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One thread per global leaf m: its half -- box, leaf ref, far end m -- goes
+// into the slot of the split it arrives at first, its triangles into leaf
+// order with scene-local ids, and its run length into the scene's longest run.
 __global__ __launch_bounds__(kB) void k_b_leaves(const float4* __restrict__ v, const uint32_t* __restrict__ idx,
                                                  const uint32_t* __restrict__ ids_sorted,
                                                  const uint32_t* __restrict__ run_start_all,
                                                  const uint32_t* __restrict__ leaf_off, SceneOffsets off,
-                                                 NodePair* __restrict__ pairs_all,
-                                                 const uint32_t* __restrict__ parent_leaf_all,
+                                                 const uint32_t* __restrict__ ucodes,
+                                                 NodePair* __restrict__ pairs_all, uint4* __restrict__ slots,
                                                  TriPre* __restrict__ tris, uint32_t* __restrict__ perm,
                                                  uint32_t* __restrict__ max_run) {
   const uint32_t total = leaf_off[off.S];
@@ -336,8 +352,10 @@ __global__ __launch_bounds__(kB) void k_b_leaves(const float4* __restrict__ v, c
       pw[11] = 0x80000000u;
       pw[15] = 0u;
     } else {
-      const uint32_t pl = parent_leaf_all[m];
-      store_box(pairs, pl >> 1, pl & 1u, lo, hi);
+      const bool left = b_is_left(ucodes, lb, lb + L, m, m);
+      const Half h{make_uint4(__float_as_uint(lo.x), __float_as_uint(lo.y), __float_as_uint(lo.z), pack_leaf(st, e - st)),
+                   make_uint4(__float_as_uint(hi.x), __float_as_uint(hi.y), __float_as_uint(hi.z), m)};
+      store_half(slots, 2 * (left ? m : m - 1) + (left ? 0u : 1u), h);
     }
   }
   // Longest run per scene.  A wave's leaves span one or a few scenes: the
@@ -352,6 +370,139 @@ __global__ __launch_bounds__(kB) void k_b_leaves(const float4* __restrict__ v, c
     for (int o = 32; o > 0; o >>= 1) mv = max(mv, __shfl_xor(mv, o));
     if (int(threadIdx.x & 63) == leader) atomicMax(&max_run[ls], mv);
     todo &= ~__ballot(mine);
+  }
+}
+
+constexpr uint32_t kClimbDone = 0xFFFFFFFFu;  // LDS exchange word after the second arrival
+
+// Chunk pass.  other[k]: 0 before any arrival at split chunk_lo + k, then the
+// first child's (far end + 1) | side << 31, then kClimbDone.
+__global__ __launch_bounds__(kRefitThreads) void k_b_climb_chunks(ManyTrees trees, const uint32_t* __restrict__ ucodes,
+                                                                 uint4* __restrict__ slots, NodePair* pairs_all,
+                                                                 uint32_t* __restrict__ tops,
+                                                                 uint32_t* __restrict__ ntops,
+                                                                 uint32_t* __restrict__ err) {
+  __shared__ uint32_t other[kRefitChunk];
+  for (uint32_t k = threadIdx.x; k < kRefitChunk; k += kRefitThreads) other[k] = 0u;
+  __syncthreads();
+  const uint32_t total = trees.leaves();
+  const uint32_t chunk_lo = blockIdx.x * kRefitChunk;
+  if (chunk_lo >= total) return;  // the whole workgroup
+  const uint32_t chunk_hi = min(chunk_lo + kRefitChunk, total) - 1u;
+  uint32_t m = chunk_lo + threadIdx.x * kRefitPer;
+  const uint32_t m_end = min(m + kRefitPer, chunk_hi + 1u);
+  TreeLoc t{};
+  if (m < m_end) t = trees.find(m);
+  // the current node: leaves [l, r]; a leaf, or an internal node whose
+  // children's halves are h0 (left) and h1 (right)
+  uint32_t l = 0, r = 0;
+  bool leaf = false, live = false;
+  Half h0{}, h1{};
+  auto next_leaf = [&]() {
+    live = false;
+    for (; m < m_end; m++) {
+      t = trees.advance(t, m);
+      if (t.le - t.lb > 1u) {
+        l = r = m++;
+        leaf = live = true;
+        return;
+      }
+    }
+  };
+  next_leaf();
+  while (live) {
+    NodePair* pairs = pairs_all + t.pb;
+    if (!leaf && l == t.lb && r + 1 == t.le) {  // the root
+      put_pair(pairs, 0, h0, h1);
+      next_leaf();
+      continue;
+    }
+    const bool left = b_is_left(ucodes, t.lb, t.le, l, r);
+    const uint32_t g = left ? r : l - 1;  // the parent's split
+    const uint32_t side = left ? 0u : 1u;
+    const uint32_t far = left ? l : r;
+    // a leaf's slot was written by k_b_leaves; it is read back only by the
+    // second arrival, together with the sibling's
+    Half own{};
+    if (!leaf) {
+      const uint32_t k = (left ? r : l) - t.lb;
+      put_pair(pairs, k, h0, h1);
+      own = union_half(h0, h1, k, far);
+      store_half(slots, 2 * g + side, own);
+    }
+    if (g < chunk_lo || g >= chunk_hi) {  // the sibling starts outside the chunk
+      push_top(trees, t.s, ((g - t.lb) << 1) | side, tops, ntops, err);
+      next_leaf();
+      continue;
+    }
+    // releases this thread's slot, acquires the sibling's
+    const uint32_t old = __hip_atomic_exchange(&other[g - chunk_lo], (far + 1u) | (side << 31), __ATOMIC_ACQ_REL,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+    if (old == 0u) {
+      next_leaf();
+      continue;
+    }
+    other[g - chunk_lo] = kClimbDone;
+    if (leaf) own = load_half(slots, 2 * g + side);
+    const Half sib = load_half(slots, 2 * g + (side ^ 1u));
+    if (left) {
+      r = (old & 0x7FFFFFFFu) - 1u;
+      h0 = own;
+      h1 = sib;
+    } else {
+      l = (old & 0x7FFFFFFFu) - 1u;
+      h0 = sib;
+      h1 = own;
+    }
+    leaf = false;
+  }
+  // splits left with one arrival: the other child is finished by the top pass
+  __syncthreads();
+  for (uint32_t k = threadIdx.x; k < chunk_hi - chunk_lo; k += kRefitThreads) {
+    const uint32_t v = other[k];
+    if (v != 0u && v != kClimbDone) {
+      const uint32_t g = chunk_lo + k;
+      const TreeLoc tg = trees.find(g);
+      push_top(trees, tg.s, ((g - tg.lb) << 1) | (v >> 31), tops, ntops, err);
+    }
+  }
+}
+
+// Top pass: one workgroup per scene.  arrive_all: per-split counters
+// (indexed pb + local split), zero on entry.
+__global__ __launch_bounds__(kRefitTopsThreads) void k_b_climb_tops(ManyTrees trees, const uint32_t* __restrict__ ucodes,
+                                                                   uint4* slots, NodePair* pairs_all,
+                                                                   const uint32_t* __restrict__ tops,
+                                                                   const uint32_t* __restrict__ ntops,
+                                                                   uint32_t* arrive_all) {
+  const TreeLoc t = trees.tree(blockIdx.x);
+  const uint32_t n = min(ntops[t.s], trees.tops_cap(t.s));
+  const uint32_t* list = tops + trees.tops_base(t.s);
+  NodePair* pairs = pairs_all + t.pb;
+  uint32_t* arrive = arrive_all + t.pb;
+  for (uint32_t i = threadIdx.x; i < n; i += kRefitTopsThreads) {
+    uint32_t g = t.lb + (list[i] >> 1), side = list[i] & 1u;
+    Half own = load_half(slots, 2 * g + side);
+    for (;;) {
+      // releases this thread's slot, acquires the sibling's
+      const uint32_t old =
+          __hip_atomic_fetch_add(&arrive[g - t.lb], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (old == 0u) break;
+      const Half sib = load_half(slots, 2 * g + (side ^ 1u));
+      const Half h0 = side ? sib : own, h1 = side ? own : sib;
+      const uint32_t l = h0.b.w, r = h1.b.w;
+      if (l == t.lb && r + 1 == t.le) {
+        put_pair(pairs, 0, h0, h1);
+        break;
+      }
+      const bool left = b_is_left(ucodes, t.lb, t.le, l, r);
+      const uint32_t k = (left ? r : l) - t.lb;
+      put_pair(pairs, k, h0, h1);
+      g = left ? r : l - 1;
+      side = left ? 0u : 1u;
+      own = union_half(h0, h1, k, left ? l : r);
+      store_half(slots, 2 * g + side, own);
+    }
   }
 }
 
@@ -569,9 +720,9 @@ struct hrt_batch {
   uint32_t* perm = nullptr;
   // scratch: one allocation, carved on the host
   void* scratch = nullptr;
-  uint32_t *codes_in, *ka, *va, *kb, *vb, *sort_tmp, *flags, *pos, *scan_tmp, *run_start, *ucodes, *parent_int,
-      *parent_leaf, *arrive, *leaf_off, *max_run, *ntops, *refit_err, *tops, *queue, *spill;
-  uint2* range;
+  uint32_t *codes_in, *ka, *va, *kb, *vb, *sort_tmp, *flags, *pos, *scan_tmp, *run_start, *ucodes, *arrive,
+      *leaf_off, *max_run, *ntops, *refit_err, *tops, *queue, *spill;
+  uint4* slots;  // two halves per split (see k_b_climb_chunks)
   float4* part;
   float* box12;
   size_t spill_words = 0;
@@ -641,15 +792,13 @@ hipError_t enqueue_build(hrt_batch* b, hipStream_t s) {
   k_b_run_flags<<<blocks(T), kB, 0, s>>>(kin, off, T, b->flags);
   if ((e = scan_exclusive(b->flags, b->pos, T, b->scan_tmp, s)) != hipSuccess) return e;
   k_b_run_compact<<<blocks(T), kB, 0, s>>>(kin, b->flags, b->pos, off, T, b->run_start, b->ucodes, b->leaf_off);
-  k_b_karras<<<blocks(T), kB, 0, s>>>(b->ucodes, b->run_start, b->leaf_off, off, T, b->pairs, b->parent_int,
-                                      b->parent_leaf, b->range);
-  k_b_leaves<<<blocks(T), kB, 0, s>>>(b->verts, b->idx, vin, b->run_start, b->leaf_off, off, b->pairs,
-                                      b->parent_leaf, b->tris, b->perm, b->max_run);
+  k_b_leaves<<<blocks(T), kB, 0, s>>>(b->verts, b->idx, vin, b->run_start, b->leaf_off, off, b->ucodes, b->pairs,
+                                      b->slots, b->tris, b->perm, b->max_run);
   const ManyTrees trees{b->leaf_off, off.pair, off.tops, b->S};
-  k_refit_chunks<ManyTrees><<<uint32_t((T + kRefitChunk - 1) / kRefitChunk), kRefitThreads, 0, s>>>(
-      trees, b->pairs, b->parent_int, b->parent_leaf, b->range, b->tops, b->ntops, b->refit_err);
-  k_refit_tops<ManyTrees><<<b->S, kRefitTopsThreads, 0, s>>>(trees, b->pairs, b->parent_int, b->tops, b->ntops,
-                                                            b->arrive);
+  k_b_climb_chunks<<<uint32_t((T + kRefitChunk - 1) / kRefitChunk), kRefitThreads, 0, s>>>(
+      trees, b->ucodes, b->slots, b->pairs, b->tops, b->ntops, b->refit_err);
+  k_b_climb_tops<<<b->S, kRefitTopsThreads, 0, s>>>(trees, b->ucodes, b->slots, b->pairs, b->tops, b->ntops,
+                                                    b->arrive);
   return hipGetLastError();
 }
 
@@ -806,9 +955,9 @@ hrt_status hrt_batch_create(hrt_ctx* ctx, uint32_t nScenes, const uint32_t* nVer
   const size_t o_codes = take(T), o_ka = take(T), o_va = take(T), o_kb = take(T), o_vb = take(T);
   const size_t o_sort = take(radix_sort_scratch_words(T, 4)), o_flags = take(T), o_pos = take(T);
   const size_t o_scan = take(scan_scratch_words(T)), o_rs = take(T), o_uc = take(T);
-  const size_t o_pint = take(P), o_pleaf = take(T), o_arr = take(P), o_loff = take(S + 1), o_mrun = take(2 * size_t(S) + 1);
+  const size_t o_slots = take(16 * T), o_arr = take(P), o_loff = take(S + 1), o_mrun = take(2 * size_t(S) + 1);
   const size_t o_tops = take(TOPS);
-  const size_t o_q = take(64), o_range = take(2 * P), o_part = take(8 * size_t(b->nchunks)), o_box = take(12 * size_t(S));
+  const size_t o_q = take(64), o_part = take(8 * size_t(b->nchunks)), o_box = take(12 * size_t(S));
   const size_t o_spill = take(b->spill_words);
   al(&b->scratch, 4 * words);
   if (e == hipSuccess) e = hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking);
@@ -820,10 +969,9 @@ hrt_status hrt_batch_create(hrt_ctx* ctx, uint32_t nScenes, const uint32_t* nVer
   uint32_t* w = static_cast<uint32_t*>(b->scratch);
   b->codes_in = w + o_codes; b->ka = w + o_ka; b->va = w + o_va; b->kb = w + o_kb; b->vb = w + o_vb;
   b->sort_tmp = w + o_sort; b->flags = w + o_flags; b->pos = w + o_pos; b->scan_tmp = w + o_scan;
-  b->run_start = w + o_rs; b->ucodes = w + o_uc; b->parent_int = w + o_pint; b->parent_leaf = w + o_pleaf;
+  b->run_start = w + o_rs; b->ucodes = w + o_uc; b->slots = reinterpret_cast<uint4*>(w + o_slots);
   b->arrive = w + o_arr; b->leaf_off = w + o_loff; b->max_run = w + o_mrun; b->queue = w + o_q;
   b->ntops = b->max_run + S; b->refit_err = b->ntops + S; b->tops = w + o_tops;
-  b->range = reinterpret_cast<uint2*>(w + o_range);
   b->part = reinterpret_cast<float4*>(w + o_part);
   b->box12 = reinterpret_cast<float*>(w + o_box);
   b->spill = w + o_spill;
