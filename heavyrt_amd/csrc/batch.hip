@@ -314,10 +314,15 @@ __global__ __launch_bounds__(kB) void k_b_leaves(const float4* __restrict__ v, c
   const uint32_t m = blockIdx.x * kB + threadIdx.x;
   const bool valid = m < total;
   uint32_t s = 0, run = 0;
+  // for the two-leaf merge below: this leaf's half and side, its scene's
+  // leaves [lb, lb + L), and the code of the leaf before it
+  Half own{};
+  bool left = false, two = false;
+  uint32_t lb = 0, L = 0, cprev = 0;
   if (valid) {
     s = seg_of_wave(leaf_off, off.S, m);
-    const uint32_t lb = leaf_off[s];
-    const uint32_t L = leaf_off[s + 1] - lb;
+    lb = leaf_off[s];
+    L = leaf_off[s + 1] - lb;
     const uint32_t tb = off.tri[s];
     const uint32_t n = off.tri[s + 1] - tb;
     const float4* vs = v + off.vert[s];
@@ -352,10 +357,43 @@ __global__ __launch_bounds__(kB) void k_b_leaves(const float4* __restrict__ v, c
       pw[11] = 0x80000000u;
       pw[15] = 0u;
     } else {
-      const bool left = b_is_left(ucodes, lb, lb + L, m, m);
-      const Half h{make_uint4(__float_as_uint(lo.x), __float_as_uint(lo.y), __float_as_uint(lo.z), pack_leaf(st, e - st)),
-                   make_uint4(__float_as_uint(hi.x), __float_as_uint(hi.y), __float_as_uint(hi.z), m)};
-      store_half(slots, 2 * (left ? m : m - 1) + (left ? 0u : 1u), h);
+      two = true;
+      cprev = m > lb ? ucodes[m - 1] : 0u;
+      left = b_is_left(ucodes, lb, lb + L, m, m);
+      own = Half{make_uint4(__float_as_uint(lo.x), __float_as_uint(lo.y), __float_as_uint(lo.z), pack_leaf(st, e - st)),
+                 make_uint4(__float_as_uint(hi.x), __float_as_uint(hi.y), __float_as_uint(hi.z), m)};
+    }
+  }
+  // Two sibling leaves in neighbouring lanes (leaf m - 1 the left child and
+  // leaf m the right child of split m - 1) form their node here: lane m takes
+  // its neighbour's half across the wave, writes the node's pair and the
+  // node's own slot, and k_b_climb_chunks starts from the node.  A pair split
+  // across two waves (m a multiple of 64) is left to the climb.
+  const uint32_t lane = threadIdx.x & 63u;
+  Half nb;
+  nb.a = make_uint4(__shfl_up(own.a.x, 1), __shfl_up(own.a.y, 1), __shfl_up(own.a.z, 1), __shfl_up(own.a.w, 1));
+  nb.b = make_uint4(__shfl_up(own.b.x, 1), __shfl_up(own.b.y, 1), __shfl_up(own.b.z, 1), __shfl_up(own.b.w, 1));
+  const bool nb_left = __shfl_up(left ? 1 : 0, 1) != 0;
+  const uint32_t nb_cprev = __shfl_up(cprev, 1);  // code of leaf m - 2
+  const bool right_left = __shfl_down(left ? 1 : 0, 1) != 0;
+  if (two) {
+    const uint32_t le = lb + L;
+    if (left) {
+      // the right neighbour forms the node when it is this leaf's sibling
+      if (!(m + 1 < le && lane != 63u && !right_left)) store_half(slots, 2 * m, own);
+    } else if (lane != 0u && nb_left) {
+      NodePair* pairs = pairs_all + off.pair[s];
+      if (m - 1 == lb && m + 1 == le) {
+        put_pair(pairs, 0, nb, own);  // the two leaves are the whole tree
+      } else {
+        // side of node [m - 1, m], as b_is_left
+        const bool nl = (m - 1 == lb) || (m + 1 != le && (ucodes[m] ^ ucodes[m + 1]) < (nb_cprev ^ cprev));
+        const uint32_t k = (nl ? m : m - 1) - lb;
+        put_pair(pairs, k, nb, own);
+        store_half(slots, 2 * (nl ? m : m - 2) + (nl ? 0u : 1u), union_half(nb, own, k, nl ? m - 1 : m));
+      }
+    } else {
+      store_half(slots, 2 * (m - 1) + 1, own);
     }
   }
   // Longest run per scene.  A wave's leaves span one or a few scenes: the
@@ -393,26 +431,36 @@ __global__ __launch_bounds__(kRefitThreads) void k_b_climb_chunks(ManyTrees tree
   const uint32_t m_end = min(m + kRefitPer, chunk_hi + 1u);
   TreeLoc t{};
   if (m < m_end) t = trees.find(m);
-  // the current node: leaves [l, r]; a leaf, or an internal node whose
-  // children's halves are h0 (left) and h1 (right)
+  // the current node: leaves [l, r]; either one whose own slot k_b_leaves
+  // wrote (a leaf, or a two-leaf node formed there: pre), or an internal
+  // node whose children's halves are h0 (left) and h1 (right)
   uint32_t l = 0, r = 0;
-  bool leaf = false, live = false;
+  bool pre = false, live = false;
   Half h0{}, h1{};
   auto next_leaf = [&]() {
     live = false;
     for (; m < m_end; m++) {
       t = trees.advance(t, m);
-      if (t.le - t.lb > 1u) {
-        l = r = m++;
-        leaf = live = true;
-        return;
+      if (t.le - t.lb < 2u) continue;
+      if (b_is_left(ucodes, t.lb, t.le, m, m)) {
+        // taken into the node formed by leaf m + 1 in k_b_leaves
+        if (m + 1 < t.le && ((m + 1) & 63u) != 0u && !b_is_left(ucodes, t.lb, t.le, m + 1, m + 1)) continue;
+        l = m;
+      } else if ((m & 63u) != 0u && b_is_left(ucodes, t.lb, t.le, m - 1, m - 1)) {
+        if (m - 1 == t.lb && m + 1 == t.le) continue;  // that node is the root, already written
+        l = m - 1;
+      } else {
+        l = m;
       }
+      r = m++;
+      pre = live = true;
+      return;
     }
   };
   next_leaf();
   while (live) {
     NodePair* pairs = pairs_all + t.pb;
-    if (!leaf && l == t.lb && r + 1 == t.le) {  // the root
+    if (!pre && l == t.lb && r + 1 == t.le) {  // the root
       put_pair(pairs, 0, h0, h1);
       next_leaf();
       continue;
@@ -421,10 +469,10 @@ __global__ __launch_bounds__(kRefitThreads) void k_b_climb_chunks(ManyTrees tree
     const uint32_t g = left ? r : l - 1;  // the parent's split
     const uint32_t side = left ? 0u : 1u;
     const uint32_t far = left ? l : r;
-    // a leaf's slot was written by k_b_leaves; it is read back only by the
-    // second arrival, together with the sibling's
+    // a pre node's slot was written by k_b_leaves; it is read back only by
+    // the second arrival, together with the sibling's
     Half own{};
-    if (!leaf) {
+    if (!pre) {
       const uint32_t k = (left ? r : l) - t.lb;
       put_pair(pairs, k, h0, h1);
       own = union_half(h0, h1, k, far);
@@ -443,7 +491,7 @@ __global__ __launch_bounds__(kRefitThreads) void k_b_climb_chunks(ManyTrees tree
       continue;
     }
     other[g - chunk_lo] = kClimbDone;
-    if (leaf) own = load_half(slots, 2 * g + side);
+    if (pre) own = load_half(slots, 2 * g + side);
     const Half sib = load_half(slots, 2 * g + (side ^ 1u));
     if (left) {
       r = (old & 0x7FFFFFFFu) - 1u;
@@ -454,7 +502,7 @@ __global__ __launch_bounds__(kRefitThreads) void k_b_climb_chunks(ManyTrees tree
       h0 = sib;
       h1 = own;
     }
-    leaf = false;
+    pre = false;
   }
   // splits left with one arrival: the other child is finished by the top pass
   __syncthreads();
