@@ -236,14 +236,27 @@ __global__ __launch_bounds__(kB) void k_b_run_compact(const uint32_t* __restrict
 // second to arrive reads its sibling's slot, writes the parent's pair at the
 // parent's own number once it knows the parent's side, and climbs on.
 //
-// Synchronisation follows refit.h: k_b_climb_chunks gives one workgroup to
-// each chunk of kRefitChunk global leaves, and a split whose two leaves both
-// lie in the chunk is arrived at only by that workgroup (LDS exchange of the
-// child's far end, workgroup-scoped).  An arrival at a split on the chunk's
-// edge, and a split left with one arrival when the workgroup ends, become
-// entries of the scene's top list; k_b_climb_tops (one workgroup per scene)
-// finishes from those with per-split counters in global memory.
+// Synchronisation follows refit.h, on three levels, none of which needs
+// device scope inside a kernel: a split whose two leaves lie in one
+// workgroup's range is arrived at only by that workgroup.
+//   k_b_leaves        kB leaves per workgroup: the leaf halves, then every
+//                     node inside the block, halves and codes in LDS
+//   k_b_climb_chunks  kRefitChunk leaves per workgroup, from the events of
+//                     its blocks (arrivals that left a block), LDS exchange
+//                     of the child's far end
+//   k_b_climb_tops    one workgroup per scene, from the arrivals that left a
+//                     chunk, with per-split counters in global memory
+// At each level an arrival at a split on the range's edge, and a split left
+// with one arrival when the workgroup ends, pass to the next level.
 // ---------------------------------------------------------------------------
+
+constexpr uint32_t kClimbDone = 0xFFFFFFFFu;  // exchange word after the second arrival
+
+// Events per leaf-pass block (of kB leaves): arrivals of nodes formed in the
+// block at nodes that are not.  Those cross one of the block's two edges; at
+// most 30 nodes cross an edge (one per common-prefix length of 30-bit codes),
+// each with at most two children from the block: at most 120.
+constexpr uint32_t kEventCap = 128;
 
 // True when node [l, r] of a scene over global leaves [lb, le) is its
 // parent's left child.  Requires [l, r] to be a proper sub-range.
@@ -299,26 +312,39 @@ __device__ inline void push_top(const ManyTrees& trees, uint32_t s, uint32_t e, 
   else __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// One thread per global leaf m: its half -- box, leaf ref, far end m -- goes
-// into the slot of the split it arrives at first, its triangles into leaf
-// order with scene-local ids, and its run length into the scene's longest run.
+// One thread per global leaf m: its half (box, leaf ref, far end m) starts the
+// block's climb, its triangles go into leaf order with scene-local ids, and
+// its run length into the scene's longest run.
 __global__ __launch_bounds__(kB) void k_b_leaves(const float4* __restrict__ v, const uint32_t* __restrict__ idx,
                                                  const uint32_t* __restrict__ ids_sorted,
                                                  const uint32_t* __restrict__ run_start_all,
                                                  const uint32_t* __restrict__ leaf_off, SceneOffsets off,
                                                  const uint32_t* __restrict__ ucodes,
                                                  NodePair* __restrict__ pairs_all, uint4* __restrict__ slots,
+                                                 uint4* __restrict__ ev, uint32_t* __restrict__ evn,
                                                  TriPre* __restrict__ tris, uint32_t* __restrict__ perm,
-                                                 uint32_t* __restrict__ max_run) {
+                                                 uint32_t* __restrict__ max_run, uint32_t* __restrict__ err) {
+  __shared__ Half bh[kB][2];         // halves of the block's splits, one per side
+  __shared__ uint32_t bother[kB];    // exchange words, as in k_b_climb_chunks
+  __shared__ uint32_t bcode[kB + 2];  // codes of leaves blo - 1 .. bhi + 1
+  __shared__ uint32_t nev;
   const uint32_t total = leaf_off[off.S];
+  const uint32_t blo = blockIdx.x * kB;
+  const uint32_t bhi = min(blo + kB, max(total, blo + 1u)) - 1u;  // last leaf of the block (when it has any)
+  bother[threadIdx.x] = 0u;
+  for (uint32_t k = threadIdx.x; k < kB + 2; k += kB) {
+    const uint32_t q = blo + k - 1u;  // wraps for blo = 0, k = 0: not read
+    bcode[k] = (k > 0 || blo > 0) && q < total ? ucodes[q] : 0u;
+  }
+  if (threadIdx.x == 0) nev = 0u;
+  __syncthreads();
   const uint32_t m = blockIdx.x * kB + threadIdx.x;
   const bool valid = m < total;
   uint32_t s = 0, run = 0;
-  // for the two-leaf merge below: this leaf's half and side, its scene's
-  // leaves [lb, lb + L), and the code of the leaf before it
+  // this leaf's half, and its scene's leaves [lb, lb + L) (L > 1: it climbs)
   Half own{};
-  bool left = false, two = false;
-  uint32_t lb = 0, L = 0, cprev = 0;
+  bool two = false;
+  uint32_t lb = 0, L = 0;
   if (valid) {
     s = seg_of_wave(leaf_off, off.S, m);
     lb = leaf_off[s];
@@ -358,43 +384,76 @@ __global__ __launch_bounds__(kB) void k_b_leaves(const float4* __restrict__ v, c
       pw[15] = 0u;
     } else {
       two = true;
-      cprev = m > lb ? ucodes[m - 1] : 0u;
-      left = b_is_left(ucodes, lb, lb + L, m, m);
       own = Half{make_uint4(__float_as_uint(lo.x), __float_as_uint(lo.y), __float_as_uint(lo.z), pack_leaf(st, e - st)),
                  make_uint4(__float_as_uint(hi.x), __float_as_uint(hi.y), __float_as_uint(hi.z), m)};
     }
   }
-  // Two sibling leaves in neighbouring lanes (leaf m - 1 the left child and
-  // leaf m the right child of split m - 1) form their node here: lane m takes
-  // its neighbour's half across the wave, writes the node's pair and the
-  // node's own slot, and k_b_climb_chunks starts from the node.  A pair split
-  // across two waves (m a multiple of 64) is left to the climb.
-  const uint32_t lane = threadIdx.x & 63u;
-  Half nb;
-  nb.a = make_uint4(__shfl_up(own.a.x, 1), __shfl_up(own.a.y, 1), __shfl_up(own.a.z, 1), __shfl_up(own.a.w, 1));
-  nb.b = make_uint4(__shfl_up(own.b.x, 1), __shfl_up(own.b.y, 1), __shfl_up(own.b.z, 1), __shfl_up(own.b.w, 1));
-  const bool nb_left = __shfl_up(left ? 1 : 0, 1) != 0;
-  const uint32_t nb_cprev = __shfl_up(cprev, 1);  // code of leaf m - 2
-  const bool right_left = __shfl_down(left ? 1 : 0, 1) != 0;
+  // Climb inside the block: nodes whose leaves all lie in [blo, bhi] are
+  // formed here, with the halves in LDS.  An arrival at a split on the
+  // block's edge, and a split left with one arrival, store the half in its
+  // slot and become an event for k_b_climb_chunks.
   if (two) {
     const uint32_t le = lb + L;
-    if (left) {
-      // the right neighbour forms the node when it is this leaf's sibling
-      if (!(m + 1 < le && lane != 63u && !right_left)) store_half(slots, 2 * m, own);
-    } else if (lane != 0u && nb_left) {
-      NodePair* pairs = pairs_all + off.pair[s];
-      if (m - 1 == lb && m + 1 == le) {
-        put_pair(pairs, 0, nb, own);  // the two leaves are the whole tree
-      } else {
-        // side of node [m - 1, m], as b_is_left
-        const bool nl = (m - 1 == lb) || (m + 1 != le && (ucodes[m] ^ ucodes[m + 1]) < (nb_cprev ^ cprev));
-        const uint32_t k = (nl ? m : m - 1) - lb;
-        put_pair(pairs, k, nb, own);
-        store_half(slots, 2 * (nl ? m : m - 2) + (nl ? 0u : 1u), union_half(nb, own, k, nl ? m - 1 : m));
+    NodePair* pairs = pairs_all + off.pair[s];
+    auto side_left = [&](uint32_t l, uint32_t r) {  // b_is_left on the block's codes
+      if (l == lb) return true;
+      if (r + 1 == le) return false;
+      return (bcode[r + 1 - blo] ^ bcode[r + 2 - blo]) < (bcode[l - blo] ^ bcode[l + 1 - blo]);
+    };
+    uint32_t l = m, r = m;
+    bool leaf = true;
+    Half h0{}, h1{};
+    for (;;) {
+      if (!leaf && l == lb && r + 1 == le) {
+        put_pair(pairs, 0, h0, h1);
+        break;
       }
-    } else {
-      store_half(slots, 2 * (m - 1) + 1, own);
+      const bool left = side_left(l, r);
+      const uint32_t g = left ? r : l - 1, side = left ? 0u : 1u, far = left ? l : r;
+      if (!leaf) {
+        const uint32_t k = (left ? r : l) - lb;
+        put_pair(pairs, k, h0, h1);
+        own = union_half(h0, h1, k, far);
+      }
+      if (g < blo || g >= bhi) {
+        store_half(slots, 2 * g + side, own);
+        const uint32_t i = atomicAdd(&nev, 1u);
+        if (i < kEventCap) ev[size_t(blockIdx.x) * kEventCap + i] = make_uint4(g, (far << 1) | side, s, 0u);
+        break;
+      }
+      bh[g - blo][side] = own;
+      const uint32_t old = __hip_atomic_exchange(&bother[g - blo], (far + 1u) | (side << 31), __ATOMIC_ACQ_REL,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (old == 0u) break;
+      bother[g - blo] = kClimbDone;
+      const Half sib = bh[g - blo][side ^ 1u];
+      if (left) {
+        r = (old & 0x7FFFFFFFu) - 1u;
+        h0 = own;
+        h1 = sib;
+      } else {
+        l = (old & 0x7FFFFFFFu) - 1u;
+        h0 = sib;
+        h1 = own;
+      }
+      leaf = false;
     }
+  }
+  __syncthreads();
+  // splits left with one arrival (the other child crosses the block's edge)
+  if (blo < total && threadIdx.x < bhi - blo) {
+    const uint32_t v = bother[threadIdx.x];
+    if (v != 0u && v != kClimbDone) {
+      const uint32_t g = blo + threadIdx.x, side = v >> 31, far = (v & 0x7FFFFFFFu) - 1u;
+      store_half(slots, 2 * g + side, bh[threadIdx.x][side]);
+      const uint32_t i = atomicAdd(&nev, 1u);
+      if (i < kEventCap) ev[size_t(blockIdx.x) * kEventCap + i] = make_uint4(g, (far << 1) | side, s, 0u);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    evn[blockIdx.x] = min(nev, kEventCap);
+    if (nev > kEventCap) __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   // Longest run per scene.  A wave's leaves span one or a few scenes: the
   // lanes of each scene reduce first, then one atomic per scene and wave
@@ -411,98 +470,86 @@ __global__ __launch_bounds__(kB) void k_b_leaves(const float4* __restrict__ v, c
   }
 }
 
-constexpr uint32_t kClimbDone = 0xFFFFFFFFu;  // LDS exchange word after the second arrival
+// Chunk pass: one workgroup per kRefitChunk leaves, from the events of the
+// chunk's leaf-pass blocks.  other[k]: 0 before any arrival at split
+// chunk_lo + k, then the first child's (far end + 1) | side << 31, then
+// kClimbDone.
+constexpr uint32_t kBlocksPerChunk = kRefitChunk / kB;
 
-// Chunk pass.  other[k]: 0 before any arrival at split chunk_lo + k, then the
-// first child's (far end + 1) | side << 31, then kClimbDone.
 __global__ __launch_bounds__(kRefitThreads) void k_b_climb_chunks(ManyTrees trees, const uint32_t* __restrict__ ucodes,
                                                                  uint4* __restrict__ slots, NodePair* pairs_all,
+                                                                 const uint4* __restrict__ ev,
+                                                                 const uint32_t* __restrict__ evn,
                                                                  uint32_t* __restrict__ tops,
                                                                  uint32_t* __restrict__ ntops,
                                                                  uint32_t* __restrict__ err) {
   __shared__ uint32_t other[kRefitChunk];
+  __shared__ uint32_t base[kBlocksPerChunk + 1];
   for (uint32_t k = threadIdx.x; k < kRefitChunk; k += kRefitThreads) other[k] = 0u;
-  __syncthreads();
   const uint32_t total = trees.leaves();
   const uint32_t chunk_lo = blockIdx.x * kRefitChunk;
   if (chunk_lo >= total) return;  // the whole workgroup
   const uint32_t chunk_hi = min(chunk_lo + kRefitChunk, total) - 1u;
-  uint32_t m = chunk_lo + threadIdx.x * kRefitPer;
-  const uint32_t m_end = min(m + kRefitPer, chunk_hi + 1u);
-  TreeLoc t{};
-  if (m < m_end) t = trees.find(m);
-  // the current node: leaves [l, r]; either one whose own slot k_b_leaves
-  // wrote (a leaf, or a two-leaf node formed there: pre), or an internal
-  // node whose children's halves are h0 (left) and h1 (right)
-  uint32_t l = 0, r = 0;
-  bool pre = false, live = false;
-  Half h0{}, h1{};
-  auto next_leaf = [&]() {
-    live = false;
-    for (; m < m_end; m++) {
-      t = trees.advance(t, m);
-      if (t.le - t.lb < 2u) continue;
-      if (b_is_left(ucodes, t.lb, t.le, m, m)) {
-        // taken into the node formed by leaf m + 1 in k_b_leaves
-        if (m + 1 < t.le && ((m + 1) & 63u) != 0u && !b_is_left(ucodes, t.lb, t.le, m + 1, m + 1)) continue;
-        l = m;
-      } else if ((m & 63u) != 0u && b_is_left(ucodes, t.lb, t.le, m - 1, m - 1)) {
-        if (m - 1 == t.lb && m + 1 == t.le) continue;  // that node is the root, already written
-        l = m - 1;
-      } else {
-        l = m;
-      }
-      r = m++;
-      pre = live = true;
-      return;
+  const uint32_t b0 = chunk_lo / kB, nb = chunk_hi / kB - b0 + 1u;
+  if (threadIdx.x == 0) {
+    uint32_t a = 0;
+    for (uint32_t k = 0; k < nb; k++) {
+      base[k] = a;
+      a += evn[b0 + k];
     }
-  };
-  next_leaf();
-  while (live) {
+    base[nb] = a;
+  }
+  __syncthreads();
+  for (uint32_t i = threadIdx.x; i < base[nb]; i += kRefitThreads) {
+    uint32_t k = 0;
+    while (base[k + 1] <= i) k++;
+    const uint4 e = ev[size_t(b0 + k) * kEventCap + (i - base[k])];
+    const TreeLoc t = trees.tree(e.z);
     NodePair* pairs = pairs_all + t.pb;
-    if (!pre && l == t.lb && r + 1 == t.le) {  // the root
-      put_pair(pairs, 0, h0, h1);
-      next_leaf();
-      continue;
+    // the arriving child: its half is in slot (g, side); [l, r] and the
+    // children's halves h0, h1 describe the node formed on a second arrival
+    uint32_t g = e.x, side = e.y & 1u, far = e.y >> 1, l = 0, r = 0;
+    bool pre = true;
+    Half own{}, h0{}, h1{};
+    for (;;) {
+      if (!pre) {
+        if (l == t.lb && r + 1 == t.le) {  // the root
+          put_pair(pairs, 0, h0, h1);
+          break;
+        }
+        const bool left = b_is_left(ucodes, t.lb, t.le, l, r);
+        g = left ? r : l - 1;
+        side = left ? 0u : 1u;
+        far = left ? l : r;
+        const uint32_t kk = (left ? r : l) - t.lb;
+        put_pair(pairs, kk, h0, h1);
+        own = union_half(h0, h1, kk, far);
+        store_half(slots, 2 * g + side, own);
+      }
+      if (g < chunk_lo || g >= chunk_hi) {  // the sibling starts outside the chunk
+        push_top(trees, t.s, ((g - t.lb) << 1) | side, tops, ntops, err);
+        break;
+      }
+      // releases this thread's slot, acquires the sibling's
+      const uint32_t old = __hip_atomic_exchange(&other[g - chunk_lo], (far + 1u) | (side << 31), __ATOMIC_ACQ_REL,
+                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
+      if (old == 0u) break;
+      other[g - chunk_lo] = kClimbDone;
+      if (pre) own = load_half(slots, 2 * g + side);
+      const Half sib = load_half(slots, 2 * g + (side ^ 1u));
+      if (side == 0u) {
+        l = far;
+        r = (old & 0x7FFFFFFFu) - 1u;
+        h0 = own;
+        h1 = sib;
+      } else {
+        l = (old & 0x7FFFFFFFu) - 1u;
+        r = far;
+        h0 = sib;
+        h1 = own;
+      }
+      pre = false;
     }
-    const bool left = b_is_left(ucodes, t.lb, t.le, l, r);
-    const uint32_t g = left ? r : l - 1;  // the parent's split
-    const uint32_t side = left ? 0u : 1u;
-    const uint32_t far = left ? l : r;
-    // a pre node's slot was written by k_b_leaves; it is read back only by
-    // the second arrival, together with the sibling's
-    Half own{};
-    if (!pre) {
-      const uint32_t k = (left ? r : l) - t.lb;
-      put_pair(pairs, k, h0, h1);
-      own = union_half(h0, h1, k, far);
-      store_half(slots, 2 * g + side, own);
-    }
-    if (g < chunk_lo || g >= chunk_hi) {  // the sibling starts outside the chunk
-      push_top(trees, t.s, ((g - t.lb) << 1) | side, tops, ntops, err);
-      next_leaf();
-      continue;
-    }
-    // releases this thread's slot, acquires the sibling's
-    const uint32_t old = __hip_atomic_exchange(&other[g - chunk_lo], (far + 1u) | (side << 31), __ATOMIC_ACQ_REL,
-                                               __HIP_MEMORY_SCOPE_WORKGROUP);
-    if (old == 0u) {
-      next_leaf();
-      continue;
-    }
-    other[g - chunk_lo] = kClimbDone;
-    if (pre) own = load_half(slots, 2 * g + side);
-    const Half sib = load_half(slots, 2 * g + (side ^ 1u));
-    if (left) {
-      r = (old & 0x7FFFFFFFu) - 1u;
-      h0 = own;
-      h1 = sib;
-    } else {
-      l = (old & 0x7FFFFFFFu) - 1u;
-      h0 = sib;
-      h1 = own;
-    }
-    pre = false;
   }
   // splits left with one arrival: the other child is finished by the top pass
   __syncthreads();
@@ -771,6 +818,8 @@ struct hrt_batch {
   uint32_t *codes_in, *ka, *va, *kb, *vb, *sort_tmp, *flags, *pos, *scan_tmp, *run_start, *ucodes, *arrive,
       *leaf_off, *max_run, *ntops, *refit_err, *tops, *queue, *spill;
   uint4* slots;  // two halves per split (see k_b_climb_chunks)
+  uint4* ev;     // kEventCap events per leaf-pass block, and their counts
+  uint32_t* evn;
   float4* part;
   float* box12;
   size_t spill_words = 0;
@@ -841,10 +890,10 @@ hipError_t enqueue_build(hrt_batch* b, hipStream_t s) {
   if ((e = scan_exclusive(b->flags, b->pos, T, b->scan_tmp, s)) != hipSuccess) return e;
   k_b_run_compact<<<blocks(T), kB, 0, s>>>(kin, b->flags, b->pos, off, T, b->run_start, b->ucodes, b->leaf_off);
   k_b_leaves<<<blocks(T), kB, 0, s>>>(b->verts, b->idx, vin, b->run_start, b->leaf_off, off, b->ucodes, b->pairs,
-                                      b->slots, b->tris, b->perm, b->max_run);
+                                      b->slots, b->ev, b->evn, b->tris, b->perm, b->max_run, b->refit_err);
   const ManyTrees trees{b->leaf_off, off.pair, off.tops, b->S};
   k_b_climb_chunks<<<uint32_t((T + kRefitChunk - 1) / kRefitChunk), kRefitThreads, 0, s>>>(
-      trees, b->ucodes, b->slots, b->pairs, b->tops, b->ntops, b->refit_err);
+      trees, b->ucodes, b->slots, b->pairs, b->ev, b->evn, b->tops, b->ntops, b->refit_err);
   k_b_climb_tops<<<b->S, kRefitTopsThreads, 0, s>>>(trees, b->ucodes, b->slots, b->pairs, b->tops, b->ntops,
                                                     b->arrive);
   return hipGetLastError();
@@ -1003,7 +1052,8 @@ hrt_status hrt_batch_create(hrt_ctx* ctx, uint32_t nScenes, const uint32_t* nVer
   const size_t o_codes = take(T), o_ka = take(T), o_va = take(T), o_kb = take(T), o_vb = take(T);
   const size_t o_sort = take(radix_sort_scratch_words(T, 4)), o_flags = take(T), o_pos = take(T);
   const size_t o_scan = take(scan_scratch_words(T)), o_rs = take(T), o_uc = take(T);
-  const size_t o_slots = take(16 * T), o_arr = take(P), o_loff = take(S + 1), o_mrun = take(2 * size_t(S) + 1);
+  const size_t nlb = blocks(T);
+  const size_t o_slots = take(16 * T), o_ev = take(4 * kEventCap * nlb), o_evn = take(nlb), o_arr = take(P), o_loff = take(S + 1), o_mrun = take(2 * size_t(S) + 1);
   const size_t o_tops = take(TOPS);
   const size_t o_q = take(64), o_part = take(8 * size_t(b->nchunks)), o_box = take(12 * size_t(S));
   const size_t o_spill = take(b->spill_words);
@@ -1018,6 +1068,7 @@ hrt_status hrt_batch_create(hrt_ctx* ctx, uint32_t nScenes, const uint32_t* nVer
   b->codes_in = w + o_codes; b->ka = w + o_ka; b->va = w + o_va; b->kb = w + o_kb; b->vb = w + o_vb;
   b->sort_tmp = w + o_sort; b->flags = w + o_flags; b->pos = w + o_pos; b->scan_tmp = w + o_scan;
   b->run_start = w + o_rs; b->ucodes = w + o_uc; b->slots = reinterpret_cast<uint4*>(w + o_slots);
+  b->ev = reinterpret_cast<uint4*>(w + o_ev); b->evn = w + o_evn;
   b->arrive = w + o_arr; b->leaf_off = w + o_loff; b->max_run = w + o_mrun; b->queue = w + o_q;
   b->ntops = b->max_run + S; b->refit_err = b->ntops + S; b->tops = w + o_tops;
   b->part = reinterpret_cast<float4*>(w + o_part);
