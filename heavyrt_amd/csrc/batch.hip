@@ -773,7 +773,8 @@ const Candidate kCandidates[] = {
     HRT_P(1, 16, 16, 64),  HRT_P(1, 16, 16, 128), HRT_P(1, 16, 16, 256), HRT_P(1, 16, 32, 64),
     HRT_P(1, 16, 32, 128), HRT_P(1, 16, 8, 128),  HRT_P(1, 8, 16, 128),  HRT_P(1, 24, 16, 128),
     HRT_P(1, 32, 16, 128), HRT_P(2, 16, 16, 128), HRT_P(4, 16, 16, 128), HRT_S(1),
-    HRT_S(4),
+    HRT_S(4),              HRT_P(1, 24, 16, 64),  HRT_P(1, 24, 32, 64),  HRT_P(1, 24, 32, 128),
+    HRT_P(1, 24, 32, 32),
 };
 #undef HRT_P
 #undef HRT_S
